@@ -1,0 +1,169 @@
+/*
+ * libspx -- MI355X-native implementation of SPECTER's per-RK-substep hot path.
+ *
+ * C ABI only: plain pointers and sizes.  Field arguments are DEVICE pointers
+ * (HBM) in the reference's Fortran layouts:
+ *   spectral  (nz, ny, nx/2+1) complex, z fastest      ["S"]
+ *   real      (nx, ny, nz)     real,    x fastest      ["R"]
+ * The element type of fields is set by the plan's precision:
+ *   SPX_FP64        complex = 2 x double, real = double
+ * All transforms are unnormalised, as in the reference (callers scale).
+ * Every call returns 0 on success or an SPX_ERR_* code; spx_last_error()
+ * returns the message.  A plan is bound to one HIP stream; calls on one plan
+ * must come from one host thread at a time, different plans may run
+ * concurrently.  Work is enqueued on the plan's stream and is asynchronous
+ * unless stated otherwise.
+ *
+ * Each entry point names the reference routine it replaces (paths relative to
+ * the SPECTER source tree, src/).
+ */
+#ifndef SPX_H
+#define SPX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  SPX_OK = 0,
+  SPX_ERR_ARG = 1,          /* invalid argument (null pointer, bad size, bad enum)   */
+  SPX_ERR_UNSUPPORTED = 2,  /* valid but not supported (grid too large for LDS, ...) */
+  SPX_ERR_HIP = 3,          /* HIP runtime error                                     */
+  SPX_ERR_ALLOC = 4,        /* device allocation failed                              */
+  SPX_ERR_STATE = 5         /* call out of order / missing field                     */
+};
+
+enum { SPX_FP64 = 0, SPX_FP32_MIXED = 1 };
+enum { SPX_SOLVER_HD = 0, SPX_SOLVER_BOUSS = 1 };
+
+typedef struct spx_plan spx_plan;
+typedef struct spx_state spx_state;
+
+/* Grid description; mirrors grid/kes module globals (pseudospec_mod.fpp:13-91)
+ * and the FCPLAN constructor arguments (fftp/fcgram_mod.f90:49-144). */
+typedef struct {
+  int nx, ny, nz;   /* points per direction; nz INCLUDES the Cz continuation points */
+  int Cz;           /* continuation points in z (0 = periodic z, no tables)          */
+  int oz;           /* FC-Gram matching order d in z                                 */
+  double Lx, Ly;    /* periodic box lengths in units of 2*pi (specter.fpp:683-711)   */
+  double Lz;        /* wall-to-wall length of the physical z domain (:743-749)       */
+  int precision;    /* SPX_FP64                                                      */
+} spx_grid;
+
+/* Sizes of a plan (derived values). */
+typedef struct {
+  int nkx;                 /* nx/2+1                                   */
+  int np;                  /* nz - Cz physical z points                */
+  size_t n_spectral;       /* nz*ny*nkx complex elements of an S array */
+  size_t n_real;           /* nx*ny*nz real elements of an R array     */
+  size_t bytes_spectral;
+  size_t bytes_real;
+  int z_fft_len;           /* length of the z FFT run in LDS (Bluestein size for awkward nz) */
+  int z_bluestein;         /* 1 if the z DFT uses Bluestein's algorithm                      */
+  size_t device_bytes;     /* device memory held by the plan (tables + scratch)              */
+} spx_plan_info;
+
+/* Replaces fcgram_create_plan (fftp/fcgram_mod.f90:49-144) including
+ * load_dirichlet_tables (:180-257): A is the raw A{C}-{d}.dat table (C x d,
+ * Fortran order), Q the raw Q{d}.dat table (d x d, Fortran order); the plan
+ * forms dir = A*Q^T in float64.  A/Q may be NULL when Cz == 0.  stream is a
+ * hipStream_t (NULL = default stream). */
+int spx_plan_create(const spx_grid* g, const double* A, const double* Q, void* stream,
+                    spx_plan** out);
+/* Replaces fcgram_destroy_plan (fftp/fcgram_mod.f90:147-177). */
+int spx_plan_destroy(spx_plan* p);
+int spx_plan_get_info(const spx_plan* p, spx_plan_info* info);
+/* Block the host until the plan's stream is idle. */
+int spx_plan_sync(spx_plan* p);
+
+/* ---- transforms (fftp/fftp.fpp) ------------------------------------------ */
+/* fftp3d_real_to_complex (fftp.fpp:388-425): R -> S. Continuation planes of the
+ * real input (k > nz-Cz) are ignored. */
+int spx_fftp3d_real_to_complex(spx_plan* p, const void* in_real, void* out_spec);
+/* fftp3d_complex_to_real (fftp.fpp:789-821): S -> R, all nz planes.  Like the
+ * reference, the input may be overwritten (here it is preserved). */
+int spx_fftp3d_complex_to_real(spx_plan* p, void* in_spec, void* out_real);
+/* fftp2d_real_to_complex_xy (fftp.fpp:428-524): per-plane 2D r2c + transpose. */
+int spx_fftp2d_real_to_complex_xy(spx_plan* p, const void* in_real, void* out_spec);
+/* fftp2d_complex_to_real_xy (fftp.fpp:824-919): transpose + per-plane 2D c2r. */
+int spx_fftp2d_complex_to_real_xy(spx_plan* p, void* in_spec, void* out_real);
+/* fftp1d_real_to_complex_z (fftp.fpp:720-786): FC continuation + z FFT, in place. */
+int spx_fftp1d_real_to_complex_z(spx_plan* p, void* inout_spec);
+/* fftp1d_complex_to_real_z (fftp.fpp:1060-1094): z IFFT, in place. */
+int spx_fftp1d_complex_to_real_z(spx_plan* p, void* inout_spec);
+/* The continuation loop alone (fftp.fpp:755-773), in place. */
+int spx_fc_continue(spx_plan* p, void* inout_spec);
+
+/* ---- spectral operators (pseudo/pseudospec_hd.f90, pseudospec_phd.f90) ---- */
+int spx_derivk(spx_plan* p, const void* a, void* b, int dir);      /* :28-94 */
+int spx_laplak(spx_plan* p, const void* a, void* b);               /* :97-127 */
+int spx_fc_filter(spx_plan* p, void* a);                           /* :1082-1115 */
+/* gradre (:209-319): (A.grad)A in advective form; d,e,f = x,y,z components. */
+int spx_gradre(spx_plan* p, const void* a, const void* b, const void* c, void* d, void* e,
+               void* f);
+/* advect (pseudospec_phd.f90:23-113): (A.grad)B for a scalar B = d; result e. */
+int spx_advect(spx_plan* p, const void* a, const void* b, const void* c, const void* d,
+               void* e);
+
+/* ---- boundary conditions and projection (boundary/) ---------------------- */
+/* sol_project (boundary_mod.fpp:197-402) with bctarget = 1 and homogeneous
+ * Dirichlet bczsta = bczend = 0 (the configuration used by the velocity BCs). */
+int spx_sol_project(spx_plan* p, void* a, void* b, void* c, void* d, int bctarget, int bczsta,
+                    int bczend);
+/* v_imposebc_and_project (vboundary.f90:67-151) for no-slip walls at z=0,Lz:
+ * rki is the RK stage o, ord the RK order (pressure extrapolation factor of
+ * noslip_z, :195-196); v_zsta/v_zend are the wall tangential velocities (x,y),
+ * may be NULL for zero. */
+int spx_v_imposebc_and_project(spx_plan* p, void* vx, void* vy, void* vz, void* pr, int rki,
+                               int ord, const double* v_zsta, const double* v_zend);
+/* s_imposebc (sboundary.f90:67-120): scalar = 0 at both walls. */
+int spx_s_imposebc(spx_plan* p, void* th);
+
+/* ---- global diagnostics (host-side results, synchronous) ------------------ */
+int spx_energy(spx_plan* p, const void* a, const void* b, const void* c, int kin,
+               double* out);                                   /* pseudospec_hd.f90:405-635 */
+int spx_cross(spx_plan* p, const void* a, const void* b, const void* c, const void* d,
+              const void* e, const void* f, double* out);      /* :778-940, kin = 1 */
+int spx_divergence(spx_plan* p, const void* a, const void* b, const void* c,
+                   double* out);                               /* :1118-1235 */
+int spx_bouncheck_z(spx_plan* p, const void* a, const void* b, double* bot,
+                    double* top);                              /* boundary_mod.fpp:681-801 */
+
+/* ---- device-resident solver state (include/hd, include/bouss, specter.fpp RK loop) ---- */
+typedef struct {
+  int solver;        /* SPX_SOLVER_HD or SPX_SOLVER_BOUSS                      */
+  int ord;           /* RK order ORD (specter.fpp:1157)                        */
+  double dt, nu;
+  double kappa;      /* scalar diffusivity (Boussinesq)                        */
+  double xmom, xtemp;/* gama*xmom, gama*xtemp (specter.fpp:480-491)            */
+  double v_zsta[2];  /* wall velocities (vxzsta, vyzsta)                        */
+  double v_zend[2];  /* (vxzend, vyzend)                                        */
+} spx_params;
+
+int spx_state_create(spx_plan* p, const spx_params* prm, spx_state** out);
+int spx_state_destroy(spx_state* s);
+/* Device pointer of a state field: "vx","vy","vz","pr","fx","fy","fz","th","fs".
+ * Fields are S arrays; the caller fills them (e.g. via spx_fftp3d_real_to_complex). */
+int spx_state_field(spx_state* s, const char* name, void** dev_ptr);
+/* hd_rkstep1 copy (include/hd/hd_rkstep1.f90:4-6). */
+int spx_rkstep1(spx_plan* p, spx_state* s);
+/* one RK stage o (include/hd/hd_rkstep2.f90:3-36 / include/bouss/bouss_rkstep2.f90:3-59) */
+int spx_substep(spx_plan* p, spx_state* s, int o);
+/* nsteps full steps: rkstep1 + o = ord..1 (specter.fpp:1142-1161). */
+int spx_step(spx_plan* p, spx_state* s, int nsteps);
+
+/* ---- memory helpers for callers without a HIP binding (e.g. Fortran) ------ */
+int spx_dev_alloc(size_t bytes, void** ptr);
+int spx_dev_free(void* ptr);
+int spx_memcpy_h2d(spx_plan* p, void* dst, const void* src, size_t bytes);  /* synchronous */
+int spx_memcpy_d2h(spx_plan* p, void* dst, const void* src, size_t bytes);  /* synchronous */
+
+/* Last error message of the calling thread; returns its length. */
+int spx_last_error(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPX_H */

@@ -1,0 +1,621 @@
+"""Float64 CPU restatement of SPECTER's per-RK-substep hot path (oracle).
+
+TEST INFRASTRUCTURE ONLY -- imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+
+Layout convention (byte-identical to the reference's Fortran arrays):
+  spectral  Fortran (nz, ny, nkx) -> numpy (nkx, ny, nz) complex128, index [i, j, k]
+  real      Fortran (nx, ny, nz)  -> numpy (nz, ny, nx)  float64,    index [k, j, i]
+  mixed (z,ky,kx) domain = spectral layout after a z-only inverse transform.
+
+All transforms are unnormalised, as in the reference (callers scale).
+
+Pinning of each function (tests/test_oracle_*.py):
+  * Grid, wavenumbers          closed form (specter.fpp:683-800) + DFT frequency order
+  * dir_matrix                 Q orthogonal, A*Q^T reproduced from the raw tables; the
+                               FC extension of a smooth function is spectrally accurate
+  * fc_continue                FC-Gram derivative KAT of sin(4x)cos(8y)sin(6z)
+                               (tests/fc_dirichlet.f90:14-89) -- a wrong index or a
+                               missing mirror destroys the convergence
+  * fftp* transforms           brute-force DFT on tiny shapes (sign, axes, layout,
+                               Hermitian handling of c2r), round trip = N * identity
+  * derivk/laplak/curlk        closed form on trig fields
+  * gradre / advect            analytic (v.grad)v of trig fields (periodic z limit via
+                               the continuation), products only on physical planes
+  * poisson/laplace_z/sol_project  invariants: div ~ 0 after projection
+                               (tests/poisson.f90:70-76), v_n ~ 0 at walls (:78-83),
+                               laplace_z satisfies d2/dz2 - khom^2 = 0 and the
+                               Neumann data in closed form
+  * energy/cross/divergence    Parseval vs real-space sums (tests/energy.f90:20-32)
+  * hd_substep / bouss_substep composition of the above, order as in
+                               include/hd/hd_rkstep2.f90 and include/bouss/bouss_rkstep2.f90
+                               (parity unpinned beyond its parts: the reference run that
+                               would pin the trajectory cannot be built here)
+
+Parity notes (deliberate, documented):
+  * kz ordering for odd nz: the reference loop specter.fpp:783-786 leaves kz(nz)
+    uninitialised and shifts the negative half by one when nz is odd.  This
+    restatement uses the standard DFT order kz[m] = m if m < (nz+1)//2 else m-nz,
+    which is identical to the reference for every even nz.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.fft as sfft
+
+PI = 3.14159265358979323846  # pseudospec_mod.fpp:71 (module var)
+
+
+def _workers() -> int:
+    return int(os.environ.get("SPX_ORACLE_WORKERS", str(os.cpu_count() or 1)))
+
+
+class _NumpyFFT:
+    """numpy.fft backend with the scipy.fft call signature (used to measure the
+    spread between two independent float64 DFT implementations)."""
+
+    @staticmethod
+    def fft(x, axis=-1, workers=None):
+        return np.fft.fft(x, axis=axis)
+
+    @staticmethod
+    def ifft(x, axis=-1, norm=None, workers=None):
+        return np.fft.ifft(x, axis=axis, norm=norm)
+
+    @staticmethod
+    def rfftn(x, axes=None, workers=None):
+        return np.fft.rfftn(x, axes=axes)
+
+    @staticmethod
+    def irfftn(x, s=None, axes=None, norm=None, workers=None):
+        return np.fft.irfftn(x, s=s, axes=axes, norm=norm)
+
+
+if os.environ.get("SPX_ORACLE_FFT", "scipy") == "numpy":
+    sfft = _NumpyFFT  # noqa: F811
+
+
+# ---------------------------------------------------------------------------
+# FC-Gram tables  (fcgram_mod.f90:180-257)
+# ---------------------------------------------------------------------------
+def load_tables(tdir: str, C: int, d: int):
+    """Read A{C}-{d}.dat (C x d) and Q{d}.dat (d x d): raw float64 stream in
+    Fortran order (fcgram_mod.f90:221-231, 238-244)."""
+    A = np.fromfile(os.path.join(tdir, f"A{C}-{d}.dat"), dtype="<f8")
+    Q = np.fromfile(os.path.join(tdir, f"Q{d}.dat"), dtype="<f8")
+    if A.size != C * d or Q.size != d * d:
+        raise ValueError("FC-Gram table size mismatch")
+    return A.reshape(d, C).T.copy(), Q.reshape(d, d).T.copy()
+
+
+def dir_matrix(A: np.ndarray, Q: np.ndarray) -> np.ndarray:
+    """dir = A * Q^T  (fcgram_mod.f90:250-252), always float64 (:26), as the
+    textbook sum over k in increasing order."""
+    C, d = A.shape
+    D = np.zeros((C, d))
+    for k in range(d):
+        D = D + A[:, k][:, None] * Q[:, k][None, :]
+    return D
+
+
+# ---------------------------------------------------------------------------
+# Grid and wavenumbers  (specter.fpp:683-800; pseudospec_mod.fpp:13-91)
+# ---------------------------------------------------------------------------
+@dataclass
+class Grid:
+    nx: int
+    ny: int
+    nz: int          # total z points including the Cz continuation points
+    Cz: int
+    d: int
+    Lx: float = 1.0  # periodic lengths in units of 2*pi
+    Ly: float = 1.0
+    Lz: float = 1.0  # physical wall-to-wall length
+    dirm: np.ndarray = field(default=None, repr=False)
+
+    def __post_init__(self):
+        nx, ny, nz, Cz = self.nx, self.ny, self.nz, self.Cz
+        self.nkx = nx // 2 + 1
+        self.np_ = nz - Cz
+        # specter.fpp:683-689 (x periodic), 705-711 (y periodic), 743-749 (z FC)
+        rmp = self.Lx * 2.0 * PI / nx
+        self.Dkx = 1.0 / self.Lx
+        self.x = rmp * np.arange(nx, dtype=np.float64)
+        rmp = self.Ly * 2.0 * PI / ny
+        self.Dky = 1.0 / self.Ly
+        self.y = rmp * np.arange(ny, dtype=np.float64)
+        rmp = self.Lz / (nz - Cz - 1)
+        self.dz = rmp
+        self.Dkz = 2.0 * PI / (rmp * nz)
+        self.z = rmp * np.arange(nz, dtype=np.float64)
+        # specter.fpp:772-789: FFT order, Nyquist of even n is negative.
+        kx = np.arange(nx, dtype=np.float64)
+        kx[nx // 2:] -= nx
+        ky = np.arange(ny, dtype=np.float64)
+        ky[ny // 2:] -= ny
+        kz = np.arange(nz, dtype=np.float64)
+        kz[(nz + 1) // 2:] -= nz  # parity note: standard order for odd nz
+        self.kx = (kx * self.Dkx)[: self.nkx]
+        self.ky = ky * self.Dky
+        self.kz = kz * self.Dkz
+        # specter.fpp:791-800: kk2 = kx^2+ky^2+kz^2 ; khom = sqrt(kk2(1,j,i))
+        kxy = self.kx[:, None] ** 2 + self.ky[None, :] ** 2
+        self.kk2 = kxy[:, :, None] + (self.kz ** 2)[None, None, :]
+        self.khom = np.sqrt(self.kk2[:, :, 0])
+
+    @property
+    def spec_shape(self):
+        return (self.nkx, self.ny, self.nz)
+
+    @property
+    def real_shape(self):
+        return (self.nz, self.ny, self.nx)
+
+
+def make_grid(nx, ny, nz, Cz, d, tdir, Lx=1.0, Ly=1.0, Lz=1.0) -> Grid:
+    """fcgram_create_plan (fcgram_mod.f90:49-144) + grid setup."""
+    g = Grid(nx, ny, nz, Cz, d, Lx, Ly, Lz)
+    A, Q = load_tables(tdir, Cz, d)
+    g.dirm = dir_matrix(A, Q)
+    return g
+
+
+# ---------------------------------------------------------------------------
+# Transforms  (fftp.fpp)
+# ---------------------------------------------------------------------------
+def fc_continue(g: Grid, S: np.ndarray) -> None:
+    """FC-Gram Dirichlet continuation of every z line, in place
+    (fftp.fpp:755-773).  Overwrites the C continuation slots from the first d
+    and the last d physical points, accumulating in the reference's order."""
+    n, C, d = g.nz, g.Cz, g.d
+    D = g.dirm
+    top = n - C - d   # 0-based start of the top matching block
+    for ii in range(C):
+        acc = D[ii, 0] * S[:, :, top] + D[C - 1 - ii, 0] * S[:, :, d - 1]
+        for jj in range(1, d):
+            acc = acc + D[ii, jj] * S[:, :, top + jj]
+            acc = acc + D[C - 1 - ii, jj] * S[:, :, d - 1 - jj]
+        S[:, :, n - C + ii] = acc
+
+
+def fftp1d_real_to_complex_z(g: Grid, S: np.ndarray) -> None:
+    """Continuation then forward z DFT in place (fftp.fpp:720-786)."""
+    fc_continue(g, S)
+    S[...] = sfft.fft(S, axis=2, workers=_workers())
+
+
+def fftp1d_complex_to_real_z(g: Grid, S: np.ndarray) -> None:
+    """Unnormalised backward z DFT in place (fftp.fpp:1060-1094)."""
+    S[...] = sfft.ifft(S, axis=2, norm="forward", workers=_workers())
+
+
+def fftp2d_real_to_complex_xy(g: Grid, R: np.ndarray) -> np.ndarray:
+    """Per-plane 2D r2c over (x,y) then transpose to (z,ky,kx)
+    (fftp.fpp:428-524)."""
+    X = sfft.rfftn(R, axes=(1, 2), workers=_workers())      # (nz, ny, nkx)
+    return np.ascontiguousarray(X.transpose(2, 1, 0))        # (nkx, ny, nz)
+
+
+def fftp2d_complex_to_real_xy(g: Grid, S: np.ndarray) -> np.ndarray:
+    """Transpose (z,ky,kx)->(kx,ky,z) then per-plane 2D c2r, unnormalised
+    (fftp.fpp:824-919).  Like FFTW's c2r, the imaginary parts of the kx=0 and
+    kx=nx/2 columns (after the y transform) are ignored."""
+    X = np.ascontiguousarray(S.transpose(2, 1, 0))           # (nz, ny, nkx)
+    return sfft.irfftn(X, s=(g.ny, g.nx), axes=(1, 2), norm="forward",
+                       workers=_workers())
+
+
+def fftp3d_real_to_complex(g: Grid, R: np.ndarray) -> np.ndarray:
+    """fftp.fpp:388-425 (Cy=0 branch :419-421): 2D xy r2c, continuation, z FFT.
+    Continuation planes of the real input are ignored."""
+    S = fftp2d_real_to_complex_xy(g, R)
+    fftp1d_real_to_complex_z(g, S)
+    return S
+
+
+def fftp3d_complex_to_real(g: Grid, S: np.ndarray) -> np.ndarray:
+    """fftp.fpp:789-821: z IFFT (in place on the input, destroying it) then
+    2D xy c2r."""
+    fftp1d_complex_to_real_z(g, S)
+    return fftp2d_complex_to_real_xy(g, S)
+
+
+# ---------------------------------------------------------------------------
+# Pointwise spectral operators  (pseudospec_hd.f90)
+# ---------------------------------------------------------------------------
+def _ik(kvec: np.ndarray, a: np.ndarray) -> np.ndarray:
+    """(0,1)*k*a evaluated as the reference's complex product: (-k*Im a, k*Re a)."""
+    out = np.empty_like(a)
+    out.real = -(kvec * a.imag)
+    out.imag = kvec * a.real
+    return out
+
+
+def derivk(g: Grid, a: np.ndarray, dirn: int) -> np.ndarray:
+    """b = i*k_dir*a  (pseudospec_hd.f90:28-94)."""
+    if dirn == 1:
+        k = g.kx[:, None, None]
+    elif dirn == 2:
+        k = g.ky[None, :, None]
+    else:
+        k = g.kz[None, None, :]
+    return _ik(k, a)
+
+
+def laplak(g: Grid, a: np.ndarray) -> np.ndarray:
+    """b = -kk2*a  (pseudospec_hd.f90:97-127)."""
+    return -g.kk2 * a
+
+
+def curlk(g: Grid, a, b, dirn):
+    """curl component from two derivk calls (pseudospec_hd.f90:130-206)."""
+    if dirn == 1:
+        return derivk(g, b, 2) - derivk(g, a, 3)
+    if dirn == 2:
+        return derivk(g, a, 3) - derivk(g, b, 1)
+    return derivk(g, b, 1) - derivk(g, a, 2)
+
+
+FILTER_ALPHA = 16 * np.log(10.0)   # pseudospec_hd.f90:1099
+FILTER_P = 50.0
+
+
+def filter_factors(g: Grid):
+    """Per-axis exponential filter factors of fc_filter (pseudospec_hd.f90:1103-1105):
+    exp(-alpha*(2*k_i/n_i/Dk_i)**(2p))."""
+    fx = np.exp(-FILTER_ALPHA * (2 * g.kx / g.nx / g.Dkx) ** (2 * FILTER_P))
+    fy = np.exp(-FILTER_ALPHA * (2 * g.ky / g.ny / g.Dky) ** (2 * FILTER_P))
+    fz = np.exp(-FILTER_ALPHA * (2 * g.kz / g.nz / g.Dkz) ** (2 * FILTER_P))
+    return fx, fy, fz
+
+
+def fc_filter(g: Grid, a: np.ndarray) -> None:
+    """a <- ((a*fx)*fy)*fz in place (pseudospec_hd.f90:1082-1115)."""
+    fx, fy, fz = filter_factors(g)
+    a *= fx[:, None, None]
+    a *= fy[None, :, None]
+    a *= fz[None, None, :]
+
+
+def _phys(g: Grid, R: np.ndarray) -> np.ndarray:
+    return R[: g.np_]
+
+
+def gradre(g: Grid, a, b, c):
+    """(A.grad)A in advective form (pseudospec_hd.f90:209-319): three stages of
+    4 c2r each, products on physical planes only, scale 1/(nx*ny*nz)^2, 3 r2c."""
+    npz = g.np_
+    tmp = 1.0 / (float(g.nx) * float(g.ny) * float(g.nz)) ** 2
+    rx = np.zeros(g.real_shape)
+    ry = np.zeros(g.real_shape)
+    rz = np.zeros(g.real_shape)
+    for stage, vcomp in enumerate((a, b, c), start=1):
+        r1 = fftp3d_complex_to_real(g, vcomp.copy())
+        r2 = fftp3d_complex_to_real(g, derivk(g, a, stage))
+        r3 = fftp3d_complex_to_real(g, derivk(g, b, stage))
+        r4 = fftp3d_complex_to_real(g, derivk(g, c, stage))
+        p1, p2, p3, p4 = r1[:npz], r2[:npz], r3[:npz], r4[:npz]
+        if stage == 1:
+            rx[:npz] = p1 * p2
+            ry[:npz] = p1 * p3
+            rz[:npz] = p1 * p4
+        elif stage == 2:
+            rx[:npz] = rx[:npz] + p1 * p2
+            ry[:npz] = ry[:npz] + p1 * p3
+            rz[:npz] = rz[:npz] + p1 * p4
+        else:
+            rx[:npz] = (rx[:npz] + p1 * p2) * tmp
+            ry[:npz] = (ry[:npz] + p1 * p3) * tmp
+            rz[:npz] = (rz[:npz] + p1 * p4) * tmp
+    return (fftp3d_real_to_complex(g, rx), fftp3d_real_to_complex(g, ry),
+            fftp3d_real_to_complex(g, rz))
+
+
+def advect(g: Grid, a, b, c, d):
+    """(A.grad)B for a scalar B (pseudospec_phd.f90:23-113)."""
+    npz = g.np_
+    tmp = 1.0 / (float(g.nx) * float(g.ny) * float(g.nz)) ** 2
+    r3 = np.zeros(g.real_shape)
+    for stage, vcomp in enumerate((a, b, c), start=1):
+        r1 = fftp3d_complex_to_real(g, vcomp.copy())
+        r2 = fftp3d_complex_to_real(g, derivk(g, d, stage))
+        if stage == 1:
+            r3[:npz] = r1[:npz] * r2[:npz]
+        elif stage == 2:
+            r3[:npz] = r3[:npz] + r1[:npz] * r2[:npz]
+        else:
+            r3[:npz] = (r3[:npz] + r1[:npz] * r2[:npz]) * tmp
+    return fftp3d_real_to_complex(g, r3)
+
+
+# ---------------------------------------------------------------------------
+# Boundary conditions and projection  (boundary_mod.fpp, vboundary.f90, sboundary.f90)
+# ---------------------------------------------------------------------------
+def goto_domain_w_boundaries(g: Grid, *arrs) -> None:
+    """z IFFT then x 1/nz on physical points (boundary_mod.fpp:72-150)."""
+    tmp = 1.0 / float(g.nz)
+    for a in arrs:
+        fftp1d_complex_to_real_z(g, a)
+        a[:, :, : g.np_] *= tmp
+
+
+def goto_3d_fourier(g: Grid, *arrs) -> None:
+    """Continuation + z FFT (boundary_mod.fpp:153-194)."""
+    for a in arrs:
+        fftp1d_real_to_complex_z(g, a)
+
+
+def noslip_z(g: Grid, o: int, ordr: int, vx, vy, pr, vbound, pos: int) -> None:
+    """Wall values vx = i*kx*pr*tau, vy = i*ky*pr*tau, tau = 1/o or (o+1)/o
+    (vboundary.f90:154-211); mode (0,0) <- nx*ny*vbound."""
+    ind = 0 if pos == 0 else g.np_ - 1
+    tmp = 1.0 / float(o)
+    if o != ordr:
+        tmp = float(o + 1) * tmp
+    p = pr[:, :, ind]
+    vx[:, :, ind] = _ik(g.kx[:, None], p) * tmp
+    vy[:, :, ind] = _ik(g.ky[None, :], p) * tmp
+    vx[0, 0, ind] = float(g.nx * g.ny) * vbound[0]
+    vy[0, 0, ind] = float(g.nx * g.ny) * vbound[1]
+
+
+def poisson_inhomogeneous(g: Grid, a, b, c) -> np.ndarray:
+    """d = -i*(kx a + ky b + kz c)/kk2, d(0,0,0)=0 (boundary_mod.fpp:405-448)."""
+    s = g.kx[:, None, None] * a + g.ky[None, :, None] * b
+    s = s + g.kz[None, None, :] * c
+    d = np.empty_like(s)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        d.real = s.imag / g.kk2
+        d.imag = -s.real / g.kk2
+    d[0, 0, 0] = 0.0
+    return d
+
+
+def laplace_z(g: Grid, bc: np.ndarray, bczsta: int, bczend: int):
+    """Harmonic solution a (and its z derivative b) in the mixed domain
+    (boundary_mod.fpp:451-678).  bc has shape (nkx, ny, 2).  Only the
+    pure-Neumann case (1,1) is on the hot path; Dirichlet (0,0) is also
+    restated for the conductor case."""
+    khom, Lz = g.khom, g.Lz
+    coef1 = np.empty(khom.shape, np.complex128)
+    coef2 = np.empty(khom.shape, np.complex128)
+    b1, b2 = bc[:, :, 0], bc[:, :, 1]
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        e1 = np.exp(-khom * Lz)
+        if bczsta == 1 and bczend == 1:
+            tmp = 1.0 / (khom * (1 - np.exp(-2 * khom * Lz)))
+            coef1[...] = (b2 - b1 * e1) * tmp
+            coef2[...] = (-b1 + b2 * e1) * tmp
+            coef1[0, 0] = b1[0, 0]
+            coef2[0, 0] = 0.0
+        elif bczsta == 0 and bczend == 0:
+            tmp = 1.0 / (1 - np.exp(-2 * khom * Lz))
+            coef1[...] = (b2 - b1 * e1) * tmp
+            coef2[...] = (b1 - b2 * e1) * tmp
+            coef1[0, 0] = (b2[0, 0] - b1[0, 0]) / Lz
+            coef2[0, 0] = b1[0, 0]
+        else:
+            raise ValueError("unsupported BC combination in laplace_z")
+    z = g.z[None, None, :]
+    kh = khom[:, :, None]
+    with np.errstate(over="ignore", invalid="ignore"):
+        ep = np.exp(kh * (z - Lz))
+        em = np.exp(-kh * z)
+    c1 = coef1[:, :, None]
+    c2 = coef2[:, :, None]
+    a = c1 * ep + c2 * em
+    b = kh * (c1 * ep - c2 * em)
+    a[0, 0, :] = coef1[0, 0].real * g.z + coef2[0, 0].real
+    b[0, 0, :] = coef1[0, 0].real
+    return a, b
+
+
+def sol_project(g: Grid, a, b, c, d_out, bctarget: int, bczsta: int = 0, bczend: int = 0):
+    """Solenoidal projection with a harmonic correction
+    (boundary_mod.fpp:197-402).  a,b,c updated in place, d_out (pressure
+    potential, mixed domain) overwritten."""
+    d = poisson_inhomogeneous(g, a, b, c)
+    a -= _ik(g.kx[:, None, None], d)
+    b -= _ik(g.ky[None, :, None], d)
+    c -= _ik(g.kz[None, None, :], d)
+    tmp = 1.0 / g.nz
+    if bctarget == 0:
+        C1 = d * tmp
+    else:
+        C1 = c * tmp
+    if bczsta == 2 or bczend == 2:
+        raise ValueError("Robin BCs are not on the hot path")
+    fftp1d_complex_to_real_z(g, C1)
+    bc = np.empty((g.nkx, g.ny, 2), np.complex128)
+    sgn = -1.0 if bctarget == 0 else 1.0
+    bc[:, :, 0] = C1[:, :, 0] if bctarget == 1 else -C1[:, :, 0]
+    bc[:, :, 1] = C1[:, :, g.np_ - 1] if bctarget == 1 else -C1[:, :, g.np_ - 1]
+    del sgn
+    C2, C3 = laplace_z(g, bc, bczsta + bctarget, bczend + bctarget)
+    if bctarget == 0:
+        d_out[...] = C1 + C2
+    else:
+        fftp1d_complex_to_real_z(g, d)
+        d_out[...] = d * tmp + C2
+    fftp1d_real_to_complex_z(g, C2)
+    fftp1d_real_to_complex_z(g, C3)
+    a -= _ik(g.kx[:, None, None], C2)
+    b -= _ik(g.ky[None, :, None], C2)
+    c -= C3
+
+
+def v_imposebc_and_project(g: Grid, vx, vy, vz, pr, rki: int, ordr: int,
+                           v_zsta=(0.0, 0.0), v_zend=(0.0, 0.0)) -> None:
+    """vboundary.f90:67-151 for no-slip walls at z=0 and z=Lz."""
+    goto_domain_w_boundaries(g, vx, vy)
+    noslip_z(g, rki, ordr, vx, vy, pr, v_zsta, 0)
+    noslip_z(g, rki, ordr, vx, vy, pr, v_zend, 1)
+    goto_3d_fourier(g, vx, vy)
+    sol_project(g, vx, vy, vz, pr, 1, 0, 0)
+
+
+def s_imposebc(g: Grid, th) -> None:
+    """Constant (zero) scalar at both walls (sboundary.f90:67-165)."""
+    goto_domain_w_boundaries(g, th)
+    th[:, :, 0] = 0.0
+    th[:, :, g.np_ - 1] = 0.0
+    goto_3d_fourier(g, th)
+
+
+# ---------------------------------------------------------------------------
+# RK stage bodies  (include/hd/hd_rkstep{1,2}.f90, include/bouss/bouss_rkstep{1,2}.f90)
+# ---------------------------------------------------------------------------
+@dataclass
+class HDState:
+    vx: np.ndarray
+    vy: np.ndarray
+    vz: np.ndarray
+    pr: np.ndarray
+    fx: np.ndarray
+    fy: np.ndarray
+    fz: np.ndarray
+    th: np.ndarray | None = None
+    fs: np.ndarray | None = None
+
+    def copy(self):
+        return HDState(*(None if v is None else v.copy() for v in
+                         (self.vx, self.vy, self.vz, self.pr, self.fx, self.fy,
+                          self.fz, self.th, self.fs)))
+
+
+def hd_substep(g: Grid, st: HDState, C1, C2, C3, o: int, ordr: int, dt: float, nu: float,
+               v_zsta=(0.0, 0.0), v_zend=(0.0, 0.0)) -> None:
+    """One RK stage o of the HD solver (include/hd/hd_rkstep2.f90:3-36)."""
+    rmp = 1.0 / float(o)
+    C4, C5, C6 = gradre(g, st.vx, st.vy, st.vz)
+    fc_filter(g, C4)
+    fc_filter(g, C5)
+    fc_filter(g, C6)
+    lx, ly, lz = laplak(g, st.vx), laplak(g, st.vy), laplak(g, st.vz)
+    st.vx[...] = C1 + dt * (nu * lx - C4 + st.fx) * rmp
+    st.vy[...] = C2 + dt * (nu * ly - C5 + st.fy) * rmp
+    st.vz[...] = C3 + dt * (nu * lz - C6 + st.fz) * rmp
+    v_imposebc_and_project(g, st.vx, st.vy, st.vz, st.pr, o, ordr, v_zsta, v_zend)
+
+
+def hd_step(g: Grid, st: HDState, ordr: int, dt: float, nu: float) -> None:
+    """hd_rkstep1 copy then o = ord..1 (specter.fpp:1142-1161)."""
+    C1, C2, C3 = st.vx.copy(), st.vy.copy(), st.vz.copy()
+    for o in range(ordr, 0, -1):
+        hd_substep(g, st, C1, C2, C3, o, ordr, dt, nu)
+
+
+def bouss_substep(g: Grid, st: HDState, C1, C2, C3, C7, o: int, ordr: int, dt: float,
+                  nu: float, kappa: float, xmom: float, xtemp: float) -> None:
+    """One RK stage of the Boussinesq solver (include/bouss/bouss_rkstep2.f90:3-59)."""
+    rmp = 1.0 / float(o)
+    C4, C5, C6 = gradre(g, st.vx, st.vy, st.vz)
+    C8 = advect(g, st.vx, st.vy, st.vz, st.th)
+    C6 = C6 - xmom * st.th
+    C8 = C8 - xtemp * st.vz
+    for arr in (C4, C5, C6, C8):
+        fc_filter(g, arr)
+    lx, ly, lz, lt = (laplak(g, st.vx), laplak(g, st.vy), laplak(g, st.vz),
+                      laplak(g, st.th))
+    st.vx[...] = C1 + dt * (nu * lx - C4 + st.fx) * rmp
+    st.vy[...] = C2 + dt * (nu * ly - C5 + st.fy) * rmp
+    st.vz[...] = C3 + dt * (nu * lz - C6 + st.fz) * rmp
+    st.th[...] = C7 + dt * (kappa * lt - C8 + st.fs) * rmp
+    v_imposebc_and_project(g, st.vx, st.vy, st.vz, st.pr, o, ordr)
+    s_imposebc(g, st.th)
+    fc_filter(g, st.th)
+    R1 = fftp3d_complex_to_real(g, st.th.copy())
+    R1 = R1 / g.nx / g.ny / g.nz
+    st.th[...] = fftp3d_real_to_complex(g, R1)
+
+
+def bouss_step(g: Grid, st: HDState, ordr, dt, nu, kappa, xmom, xtemp) -> None:
+    C1, C2, C3, C7 = st.vx.copy(), st.vy.copy(), st.vz.copy(), st.th.copy()
+    for o in range(ordr, 0, -1):
+        bouss_substep(g, st, C1, C2, C3, C7, o, ordr, dt, nu, kappa, xmom, xtemp)
+
+
+# ---------------------------------------------------------------------------
+# Global diagnostics  (pseudospec_hd.f90:405-1235, boundary_mod.fpp:681-801)
+# ---------------------------------------------------------------------------
+def _mean_phys(g: Grid, R1: np.ndarray) -> float:
+    """Weighted mean over physical z in the mixed domain: weight 1 for kx=0,
+    2 for every other kx including Nyquist (pseudospec_hd.f90:602-628)."""
+    tmp = 1.0 / (float(g.nx) * float(g.ny) * float(g.nz)) ** 2 / float(g.nz - g.Cz)
+    part = R1[:, :, : g.np_] * tmp
+    return float(part[0].sum() + 2.0 * part[1:].sum())
+
+
+def _zinv(g: Grid, a):
+    c = a.copy()
+    fftp1d_complex_to_real_z(g, c)
+    return c
+
+
+def energy(g: Grid, a, b, c, kin: int) -> float:
+    """<|v|^2> (kin=1) or the reference's enstrophy (kin=0, which sums
+    wx^2 + wy^2 + wy^2, pseudospec_hd.f90:527,540) (pseudospec_hd.f90:405-635)."""
+    if kin == 1:
+        comps = (a, b, c)
+    elif kin == 0:
+        comps = (curlk(g, b, c, 1), curlk(g, a, c, 2), curlk(g, a, c, 2))
+    else:
+        raise ValueError("kin must be 0 or 1")
+    R1 = None
+    for comp in comps:
+        z = _zinv(g, comp)
+        sq = z.real ** 2 + z.imag ** 2
+        R1 = sq if R1 is None else R1 + sq
+    return _mean_phys(g, R1)
+
+
+def cross(g: Grid, a, b, c, d, e, f) -> float:
+    """<v.f> for kin=1 (pseudospec_hd.f90:778-940)."""
+    R1 = None
+    for p, q in ((a, d), (b, e), (c, f)):
+        zp, zq = _zinv(g, p), _zinv(g, q)
+        r = (zp * np.conj(zq)).real
+        R1 = r if R1 is None else R1 + r
+    return _mean_phys(g, R1)
+
+
+def divergence(g: Grid, a, b, c) -> float:
+    """<|div v|^2> over the physical domain (pseudospec_hd.f90:1118-1235)."""
+    C2 = derivk(g, a, 1)
+    C2 = C2 + derivk(g, b, 2)
+    C2 = C2 + derivk(g, c, 3)
+    z = _zinv(g, C2)
+    return _mean_phys(g, z.real ** 2 + z.imag ** 2)
+
+
+def bouncheck_z(g: Grid, a, b=None):
+    """Mean |a|^2 (+|b|^2) on the z=0 and z=Lz planes (boundary_mod.fpp:681-801)."""
+    tmp = (1.0 / (g.nx * g.ny * g.nz)) ** 2
+    za = _zinv(g, a)
+    R1 = za.real[:, :, 0] ** 2 + za.imag[:, :, 0] ** 2
+    R2 = za.real[:, :, g.np_ - 1] ** 2 + za.imag[:, :, g.np_ - 1] ** 2
+    if b is not None:
+        zb = _zinv(g, b)
+        R1 = R1 + zb.real[:, :, 0] ** 2 + zb.imag[:, :, 0] ** 2
+        R2 = R2 + zb.real[:, :, g.np_ - 1] ** 2 + zb.imag[:, :, g.np_ - 1] ** 2
+    bot = float((R1[0] * tmp).sum() + (2 * R1[1:] * tmp).sum())
+    top = float((R2[0] * tmp).sum() + (2 * R2[1:] * tmp).sum())
+    return bot, top
+
+
+def hdcheck(g: Grid, st: HDState):
+    """energy, enstrophy and injection rate of balance.txt (pseudospec_hd.f90:943-1005)."""
+    return (energy(g, st.vx, st.vy, st.vz, 1), energy(g, st.vx, st.vy, st.vz, 0),
+            cross(g, st.vx, st.vy, st.vz, st.fx, st.fy, st.fz))
+
+
+def vdiagnostic(g: Grid, st: HDState):
+    """<div^2>, wall <v_t^2> (bottom, top), wall <v_n^2> (bottom, top)
+    (vboundary.f90:214-269)."""
+    dv = divergence(g, st.vx, st.vy, st.vz)
+    tb, tt = bouncheck_z(g, st.vx, st.vy)
+    nb, nt = bouncheck_z(g, st.vz)
+    return dv, tb, tt, nb, nt

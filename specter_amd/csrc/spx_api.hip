@@ -1,0 +1,619 @@
+// C ABI of libspx: plan (FCPLAN), transforms (fftp.fpp), operators
+// (pseudospec_*.f90), boundary conditions (boundary/), diagnostics and the
+// device-resident RK solver state (include/hd, include/bouss, specter.fpp).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "spx_common.h"
+
+namespace spx {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+static int alloc_dev(spx_plan* p, void** ptr, size_t bytes) {
+  const hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return fail(SPX_ERR_ALLOC,
+                "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+  }
+  if (p) p->owned.push_back(*ptr);
+  return SPX_OK;
+}
+
+template <typename V>
+static int upload(spx_plan* p, double** dst, const V& host) {
+  int rc = alloc_dev(p, reinterpret_cast<void**>(dst), host.size() * sizeof(double));
+  if (rc) return rc;
+  SPX_HIP(hipMemcpy(*dst, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
+  return SPX_OK;
+}
+
+static int ensure_work(spx_plan* p) {
+  if (p->R[0]) return SPX_OK;
+  for (int i = 0; i < 7; ++i) {
+    int rc = alloc_dev(p, &p->R[i], p->nr * sizeof(double));
+    if (rc) return rc;
+  }
+  return SPX_OK;
+}
+
+// 3D transforms on internal buffers -------------------------------------------
+static int r2c3(spx_plan* p, const void* in, void* out) {
+  int rc;
+  if ((rc = xr2c(p, in, p->X, p->np))) return rc;
+  if ((rc = yfft(p, p->X, 1, p->np))) return rc;
+  return zfft_x2s(p, p->X, out, 0);
+}
+
+static int c2r3(spx_plan* p, const void* in, void* out, int dirn, int nplanes) {
+  int rc;
+  if ((rc = zfft_s2x(p, in, p->X, dirn, nplanes, 0))) return rc;
+  if ((rc = yfft(p, p->X, 0, nplanes))) return rc;
+  return xc2r(p, p->X, out, nplanes);
+}
+
+static double diag_tmp(const spx_plan* p) {
+  const double t = double(p->g.nx) * double(p->g.ny) * double(p->g.nz);
+  return 1.0 / (t * t) / double(p->g.nz - p->g.Cz);
+}
+
+static int fetch_parts(spx_plan* p, size_t count) {
+  if (p->h_part.size() < count) p->h_part.resize(count);
+  SPX_HIP(hipMemcpyAsync(p->h_part.data(), p->part, count * sizeof(double),
+                         hipMemcpyDeviceToHost, p->stream));
+  SPX_HIP(hipStreamSynchronize(p->stream));
+  return SPX_OK;
+}
+
+// weighted sum over (kx,ky) lines: weight 1 for kx = 0, 2 otherwise
+static double weighted_sum(const spx_plan* p, const std::vector<double>& acc) {
+  const long ny = p->g.ny, nl = long(p->g.ny) * p->nkx;
+  double s0 = 0.0, s1 = 0.0;
+  for (long l = 0; l < ny; ++l) s0 += acc[l];
+  for (long l = ny; l < nl; ++l) s1 += acc[l];
+  return s0 + 2.0 * s1;
+}
+
+}  // namespace spx
+
+using namespace spx;
+
+struct spx_state {
+  spx_plan* p = nullptr;
+  spx_params prm;
+  void *vx = nullptr, *vy = nullptr, *vz = nullptr, *pr = nullptr;
+  void *fx = nullptr, *fy = nullptr, *fz = nullptr, *th = nullptr, *fs = nullptr;
+  void *C1 = nullptr, *C2 = nullptr, *C3 = nullptr, *C7 = nullptr;
+  void* N[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<void*> owned;
+};
+
+#define SPX_REQ(cond, msg) \
+  do {                     \
+    if (!(cond)) return fail(SPX_ERR_ARG, msg); \
+  } while (0)
+
+extern "C" {
+
+int spx_last_error(char* buf, size_t len) {
+  if (buf && len) {
+    std::snprintf(buf, len, "%s", g_err.c_str());
+  }
+  return int(g_err.size());
+}
+
+int spx_plan_create(const spx_grid* g, const double* A, const double* Q, void* stream,
+                    spx_plan** out) {
+  SPX_REQ(g && out, "spx_plan_create: null argument");
+  *out = nullptr;
+  SPX_REQ(g->nx >= 2 && g->nx % 2 == 0, "nx must be even and >= 2");
+  SPX_REQ(g->ny >= 2 && g->ny % 2 == 0, "ny must be even and >= 2");
+  SPX_REQ(g->nz >= 2, "nz must be >= 2");
+  SPX_REQ(g->Cz >= 0 && g->Cz < g->nz, "Cz out of range");
+  SPX_REQ(g->Lx > 0 && g->Ly > 0 && g->Lz > 0, "box lengths must be positive");
+  if (g->precision != SPX_FP64)
+    return fail(SPX_ERR_UNSUPPORTED, "only SPX_FP64 plans are available");
+  if (g->Cz > 0) {
+    SPX_REQ(A && Q, "FC-Gram tables A and Q are required when Cz > 0");
+    SPX_REQ(g->oz >= 1, "matching order oz must be >= 1 when Cz > 0");
+    SPX_REQ(g->nz - g->Cz >= g->oz + 1, "too few physical z points for the matching order");
+  } else {
+    SPX_REQ(g->oz == 0, "Cz == 0 requires oz == 0 (periodic z)");
+  }
+  spx_plan* p = new spx_plan();
+  p->g = *g;
+  p->stream = static_cast<hipStream_t>(stream);
+  const int nx = g->nx, ny = g->ny, nz = g->nz, C = g->Cz, d = g->oz;
+  p->nkx = nx / 2 + 1;
+  p->np = nz - C;
+  p->nc = size_t(nz) * ny * p->nkx;
+  p->nr = size_t(nx) * ny * nz;
+  p->csize = 16;
+  int rc = SPX_OK;
+  auto bail = [&](int code) {
+    spx_plan_destroy(p);
+    return code;
+  };
+  // dir = A*Q^T (fcgram_mod.f90:250-252), textbook sum in k order
+  if (C > 0) {
+    p->h_dir.assign(size_t(C) * d, 0.0);
+    for (int ii = 0; ii < C; ++ii)
+      for (int jj = 0; jj < d; ++jj) {
+        double s = 0.0;
+        for (int k = 0; k < d; ++k) s = s + A[ii + size_t(C) * k] * Q[jj + size_t(d) * k];
+        p->h_dir[size_t(ii) * d + jj] = s;
+      }
+    if ((rc = upload(p, &p->d_dir, p->h_dir))) return bail(rc);
+  }
+  // grids and wavenumbers (specter.fpp:683-800)
+  const double PI = 3.14159265358979323846;
+  p->Dkx = 1.0 / g->Lx;
+  p->Dky = 1.0 / g->Ly;
+  if (C > 0) {
+    p->dz = g->Lz / double(nz - C - 1);
+    p->Dkz = 2.0 * PI / (p->dz * nz);
+  } else {
+    p->dz = g->Lz * 2.0 * PI / nz;
+    p->Dkz = 1.0 / g->Lz;
+  }
+  p->h_kx.resize(p->nkx);
+  p->h_ky.resize(ny);
+  p->h_kz.resize(nz);
+  p->h_z.resize(nz);
+  for (int i = 0; i < p->nkx; ++i) p->h_kx[i] = double(i < nx / 2 ? i : i - nx) * p->Dkx;
+  for (int j = 0; j < ny; ++j) p->h_ky[j] = double(j < ny / 2 ? j : j - ny) * p->Dky;
+  for (int k = 0; k < nz; ++k) p->h_kz[k] = double(k < (nz + 1) / 2 ? k : k - nz) * p->Dkz;
+  for (int k = 0; k < nz; ++k) p->h_z[k] = p->dz * double(k);
+  p->h_khom.resize(size_t(p->nkx) * ny);
+  for (int i = 0; i < p->nkx; ++i)
+    for (int j = 0; j < ny; ++j)
+      p->h_khom[size_t(j) + size_t(ny) * i] =
+          std::sqrt(p->h_kx[i] * p->h_kx[i] + p->h_ky[j] * p->h_ky[j]);
+  // fc_filter factors (pseudospec_hd.f90:1099-1105)
+  const double alpha = 16 * std::log(10.0), pw = 2 * 50.0;
+  p->h_fkx.resize(p->nkx);
+  p->h_fky.resize(ny);
+  p->h_fkz.resize(nz);
+  for (int i = 0; i < p->nkx; ++i)
+    p->h_fkx[i] = std::exp(-alpha * std::pow(2 * p->h_kx[i] / nx / p->Dkx, pw));
+  for (int j = 0; j < ny; ++j)
+    p->h_fky[j] = std::exp(-alpha * std::pow(2 * p->h_ky[j] / ny / p->Dky, pw));
+  for (int k = 0; k < nz; ++k)
+    p->h_fkz[k] = std::exp(-alpha * std::pow(2 * p->h_kz[k] / nz / p->Dkz, pw));
+  if ((rc = upload(p, &p->d_kx, p->h_kx)) || (rc = upload(p, &p->d_ky, p->h_ky)) ||
+      (rc = upload(p, &p->d_kz, p->h_kz)) || (rc = upload(p, &p->d_z, p->h_z)) ||
+      (rc = upload(p, &p->d_khom, p->h_khom)) || (rc = upload(p, &p->d_fkx, p->h_fkx)) ||
+      (rc = upload(p, &p->d_fky, p->h_fky)) || (rc = upload(p, &p->d_fkz, p->h_fkz)))
+    return bail(rc);
+  // line FFT plans
+  if ((rc = build_line_fft(p->fx, nx, p->csize, 0))) return bail(rc);
+  if ((rc = build_line_fft(p->fy, ny, p->csize, 16))) return bail(rc);
+  if ((rc = build_line_fft(p->fz, nz, p->csize, 16))) return bail(rc);
+  p->fz2 = p->fz;
+  p->fz2.nl = p->fz.nl > 1 ? p->fz.nl / 2 : 1;
+  // scratch
+  const size_t nlines = size_t(ny) * p->nkx;
+  if ((rc = alloc_dev(p, &p->X, p->nc * 16))) return bail(rc);
+  if ((rc = alloc_dev(p, &p->bc, nlines * 2 * 16))) return bail(rc);
+  if ((rc = alloc_dev(p, reinterpret_cast<void**>(&p->part), nlines * 2 * sizeof(double))))
+    return bail(rc);
+  *out = p;
+  return SPX_OK;
+}
+
+int spx_plan_destroy(spx_plan* p) {
+  if (!p) return SPX_OK;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  else (void)hipDeviceSynchronize();
+  free_line_fft(p->fx);
+  free_line_fft(p->fy);
+  free_line_fft(p->fz);
+  for (void* q : p->owned) (void)hipFree(q);
+  delete p;
+  return SPX_OK;
+}
+
+int spx_plan_get_info(const spx_plan* p, spx_plan_info* info) {
+  SPX_REQ(p && info, "spx_plan_get_info: null argument");
+  info->nkx = p->nkx;
+  info->np = p->np;
+  info->n_spectral = p->nc;
+  info->n_real = p->nr;
+  info->bytes_spectral = p->nc * 16;
+  info->bytes_real = p->nr * 8;
+  info->z_fft_len = p->fz.M;
+  info->z_bluestein = p->fz.blue;
+  size_t b = p->nc * 16 + size_t(p->g.ny) * p->nkx * 48;
+  if (p->R[0]) b += 7 * p->nr * 8;
+  info->device_bytes = b;
+  return SPX_OK;
+}
+
+int spx_plan_sync(spx_plan* p) {
+  SPX_REQ(p, "spx_plan_sync: null plan");
+  SPX_HIP(hipStreamSynchronize(p->stream));
+  return SPX_OK;
+}
+
+// ---- transforms -------------------------------------------------------------
+int spx_fftp3d_real_to_complex(spx_plan* p, const void* in, void* out) {
+  SPX_REQ(p && in && out, "spx_fftp3d_real_to_complex: null argument");
+  return r2c3(p, in, out);
+}
+
+int spx_fftp3d_complex_to_real(spx_plan* p, void* in, void* out) {
+  SPX_REQ(p && in && out, "spx_fftp3d_complex_to_real: null argument");
+  return c2r3(p, in, out, 0, p->g.nz);
+}
+
+int spx_fftp2d_real_to_complex_xy(spx_plan* p, const void* in, void* out) {
+  SPX_REQ(p && in && out, "spx_fftp2d_real_to_complex_xy: null argument");
+  int rc;
+  if ((rc = xr2c(p, in, p->X, p->g.nz))) return rc;
+  if ((rc = yfft(p, p->X, 1, p->g.nz))) return rc;
+  return zfft_x2s(p, p->X, out, 1);
+}
+
+int spx_fftp2d_complex_to_real_xy(spx_plan* p, void* in, void* out) {
+  SPX_REQ(p && in && out, "spx_fftp2d_complex_to_real_xy: null argument");
+  int rc;
+  if ((rc = zfft_s2x(p, in, p->X, 0, p->g.nz, 1))) return rc;
+  if ((rc = yfft(p, p->X, 0, p->g.nz))) return rc;
+  return xc2r(p, p->X, out, p->g.nz);
+}
+
+int spx_fftp1d_real_to_complex_z(spx_plan* p, void* s) {
+  SPX_REQ(p && s, "spx_fftp1d_real_to_complex_z: null argument");
+  return zfft_inplace(p, s, 1, 1, 1.0);
+}
+
+int spx_fftp1d_complex_to_real_z(spx_plan* p, void* s) {
+  SPX_REQ(p && s, "spx_fftp1d_complex_to_real_z: null argument");
+  return zfft_inplace(p, s, 0, 0, 1.0);
+}
+
+int spx_fc_continue(spx_plan* p, void* s) {
+  SPX_REQ(p && s, "spx_fc_continue: null argument");
+  return fc_continue_only(p, s);
+}
+
+// ---- operators ----------------------------------------------------------------
+int spx_derivk(spx_plan* p, const void* a, void* b, int dir) {
+  SPX_REQ(p && a && b, "spx_derivk: null argument");
+  SPX_REQ(dir >= 1 && dir <= 3, "spx_derivk: dir must be 1, 2 or 3");
+  return op_derivk(p, a, b, dir);
+}
+
+int spx_laplak(spx_plan* p, const void* a, void* b) {
+  SPX_REQ(p && a && b, "spx_laplak: null argument");
+  return op_laplak(p, a, b);
+}
+
+int spx_fc_filter(spx_plan* p, void* a) {
+  SPX_REQ(p && a, "spx_fc_filter: null argument");
+  return op_filter(p, a);
+}
+
+int spx_gradre(spx_plan* p, const void* a, const void* b, const void* c, void* d, void* e,
+               void* f) {
+  SPX_REQ(p && a && b && c && d && e && f, "spx_gradre: null argument");
+  int rc;
+  if ((rc = ensure_work(p))) return rc;
+  const void* v[3] = {a, b, c};
+  for (int s = 1; s <= 3; ++s) {
+    if ((rc = c2r3(p, v[s - 1], p->R[0], 0, p->np))) return rc;
+    if ((rc = c2r3(p, a, p->R[1], s, p->np))) return rc;
+    if ((rc = c2r3(p, b, p->R[2], s, p->np))) return rc;
+    if ((rc = c2r3(p, c, p->R[3], s, p->np))) return rc;
+    if ((rc = op_prod_stage(p, s, p->R[0], p->R[1], p->R[2], p->R[3], p->R[4], p->R[5],
+                            p->R[6], 3)))
+      return rc;
+  }
+  if ((rc = r2c3(p, p->R[4], d))) return rc;
+  if ((rc = r2c3(p, p->R[5], e))) return rc;
+  return r2c3(p, p->R[6], f);
+}
+
+int spx_advect(spx_plan* p, const void* a, const void* b, const void* c, const void* dd,
+               void* e) {
+  SPX_REQ(p && a && b && c && dd && e, "spx_advect: null argument");
+  int rc;
+  if ((rc = ensure_work(p))) return rc;
+  const void* v[3] = {a, b, c};
+  for (int s = 1; s <= 3; ++s) {
+    if ((rc = c2r3(p, v[s - 1], p->R[0], 0, p->np))) return rc;
+    if ((rc = c2r3(p, dd, p->R[1], s, p->np))) return rc;
+    if ((rc = op_prod_stage(p, s, p->R[0], p->R[1], nullptr, nullptr, p->R[4], nullptr, nullptr,
+                            1)))
+      return rc;
+  }
+  return r2c3(p, p->R[4], e);
+}
+
+// ---- boundary conditions --------------------------------------------------------
+int spx_sol_project(spx_plan* p, void* a, void* b, void* c, void* d, int bctarget, int bczsta,
+                    int bczend) {
+  SPX_REQ(p && a && b && c && d, "spx_sol_project: null argument");
+  if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "sol_project needs walls in z (Cz > 0)");
+  if (bctarget != 1 || bczsta != 0 || bczend != 0)
+    return fail(SPX_ERR_UNSUPPORTED,
+                "sol_project: only bctarget=1 with homogeneous Dirichlet walls is implemented");
+  int rc;
+  if ((rc = op_poisson_project(p, a, b, c, d))) return rc;
+  if ((rc = proj_bc(p, c))) return rc;
+  if ((rc = proj_pressure(p, d))) return rc;
+  return proj_harmonic(p, a, b, c);
+}
+
+int spx_v_imposebc_and_project(spx_plan* p, void* vx, void* vy, void* vz, void* pr, int rki,
+                               int ord, const double* v_zsta, const double* v_zend) {
+  SPX_REQ(p && vx && vy && vz && pr, "spx_v_imposebc_and_project: null argument");
+  SPX_REQ(rki >= 1 && ord >= 1 && rki <= ord, "spx_v_imposebc_and_project: bad RK stage");
+  if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "no-slip walls need Cz > 0");
+  double tau = 1.0 / double(rki);
+  if (rki != ord) tau = double(rki + 1) * tau;
+  int rc;
+  if ((rc = noslip_fused(p, vx, vy, pr, tau, v_zsta, v_zend))) return rc;
+  return spx_sol_project(p, vx, vy, vz, pr, 1, 0, 0);
+}
+
+int spx_s_imposebc(spx_plan* p, void* th) {
+  SPX_REQ(p && th, "spx_s_imposebc: null argument");
+  if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "scalar walls need Cz > 0");
+  return sconst_fused(p, th);
+}
+
+// ---- diagnostics ------------------------------------------------------------------
+static int sumsq_lines(spx_plan* p, const LinComb& lc, std::vector<double>& acc, double w) {
+  const size_t nl = size_t(p->g.ny) * p->nkx;
+  int rc;
+  if ((rc = zline_sumsq(p, lc, p->part))) return rc;
+  if ((rc = fetch_parts(p, nl))) return rc;
+  for (size_t l = 0; l < nl; ++l) acc[l] += w * p->h_part[l];
+  return SPX_OK;
+}
+
+static LinComb one(const void* a, int dir) {
+  LinComb lc;
+  lc.in[0] = a;
+  lc.in[1] = lc.in[2] = nullptr;
+  lc.dir[0] = dir;
+  lc.dir[1] = lc.dir[2] = 0;
+  lc.sgn[0] = 1;
+  lc.sgn[1] = lc.sgn[2] = 0;
+  return lc;
+}
+
+static LinComb two(const void* a, int da, const void* b, int db, int sg) {
+  LinComb lc = one(a, da);
+  lc.in[1] = b;
+  lc.dir[1] = db;
+  lc.sgn[1] = sg;
+  return lc;
+}
+
+int spx_energy(spx_plan* p, const void* a, const void* b, const void* c, int kin, double* out) {
+  SPX_REQ(p && a && b && c && out, "spx_energy: null argument");
+  SPX_REQ(kin == 0 || kin == 1, "spx_energy: kin must be 0 or 1");
+  std::vector<double> acc(size_t(p->g.ny) * p->nkx, 0.0);
+  int rc;
+  if (kin == 1) {
+    for (const void* q : {a, b, c})
+      if ((rc = sumsq_lines(p, one(q, 0), acc, 1.0))) return rc;
+  } else {
+    // curlk(b,c,1) = d_y c - d_z b ; curlk(a,c,2) = d_z a - d_x c, counted twice
+    // (pseudospec_hd.f90:527,540)
+    if ((rc = sumsq_lines(p, two(c, 2, b, 3, -1), acc, 1.0))) return rc;
+    if ((rc = sumsq_lines(p, two(a, 3, c, 1, -1), acc, 2.0))) return rc;
+  }
+  *out = weighted_sum(p, acc) * diag_tmp(p);
+  return SPX_OK;
+}
+
+int spx_cross(spx_plan* p, const void* a, const void* b, const void* c, const void* d,
+              const void* e, const void* f, double* out) {
+  SPX_REQ(p && a && b && c && d && e && f && out, "spx_cross: null argument");
+  const size_t nl = size_t(p->g.ny) * p->nkx;
+  std::vector<double> acc(nl, 0.0);
+  const void* L[3] = {a, b, c};
+  const void* Rr[3] = {d, e, f};
+  int rc;
+  for (int t = 0; t < 3; ++t) {
+    if ((rc = zline_dot(p, L[t], Rr[t], p->part))) return rc;
+    if ((rc = fetch_parts(p, nl))) return rc;
+    for (size_t l = 0; l < nl; ++l) acc[l] += p->h_part[l];
+  }
+  *out = weighted_sum(p, acc) * diag_tmp(p);
+  return SPX_OK;
+}
+
+int spx_divergence(spx_plan* p, const void* a, const void* b, const void* c, double* out) {
+  SPX_REQ(p && a && b && c && out, "spx_divergence: null argument");
+  LinComb lc = two(a, 1, b, 2, 1);
+  lc.in[2] = c;
+  lc.dir[2] = 3;
+  lc.sgn[2] = 1;
+  std::vector<double> acc(size_t(p->g.ny) * p->nkx, 0.0);
+  int rc = sumsq_lines(p, lc, acc, 1.0);
+  if (rc) return rc;
+  *out = weighted_sum(p, acc) * diag_tmp(p);
+  return SPX_OK;
+}
+
+int spx_bouncheck_z(spx_plan* p, const void* a, const void* b, double* bot, double* top) {
+  SPX_REQ(p && a && bot && top, "spx_bouncheck_z: null argument");
+  const size_t nl = size_t(p->g.ny) * p->nkx;
+  std::vector<double> ab(nl, 0.0), at(nl, 0.0);
+  int rc;
+  for (const void* q : {a, b}) {
+    if (!q) continue;
+    if ((rc = zline_walls(p, q, p->part))) return rc;
+    if ((rc = fetch_parts(p, 2 * nl))) return rc;
+    for (size_t l = 0; l < nl; ++l) {
+      ab[l] += p->h_part[2 * l];
+      at[l] += p->h_part[2 * l + 1];
+    }
+  }
+  const double t = 1.0 / double(long(p->g.nx) * p->g.ny * p->g.nz);
+  *bot = weighted_sum(p, ab) * (t * t);
+  *top = weighted_sum(p, at) * (t * t);
+  return SPX_OK;
+}
+
+// ---- solver state ------------------------------------------------------------------
+int spx_state_create(spx_plan* p, const spx_params* prm, spx_state** out) {
+  SPX_REQ(p && prm && out, "spx_state_create: null argument");
+  SPX_REQ(prm->solver == SPX_SOLVER_HD || prm->solver == SPX_SOLVER_BOUSS,
+          "spx_state_create: unknown solver");
+  SPX_REQ(prm->ord >= 1, "spx_state_create: ord must be >= 1");
+  if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "the solvers need walls in z (Cz > 0)");
+  *out = nullptr;
+  int rc = ensure_work(p);
+  if (rc) return rc;
+  spx_state* s = new spx_state();
+  s->p = p;
+  s->prm = *prm;
+  const bool bouss = prm->solver == SPX_SOLVER_BOUSS;
+  void** fields[] = {&s->vx, &s->vy, &s->vz, &s->pr, &s->fx, &s->fy, &s->fz,
+                     &s->C1, &s->C2, &s->C3, &s->N[0], &s->N[1], &s->N[2]};
+  void** bfields[] = {&s->th, &s->fs, &s->C7, &s->N[3]};
+  auto mk = [&](void** dst) -> int {
+    if (hipMalloc(dst, p->nc * 16) != hipSuccess) return fail(SPX_ERR_ALLOC, "state allocation");
+    s->owned.push_back(*dst);
+    if (hipMemsetAsync(*dst, 0, p->nc * 16, p->stream) != hipSuccess)
+      return fail(SPX_ERR_HIP, "state clear");
+    return SPX_OK;
+  };
+  for (void** f : fields)
+    if ((rc = mk(f))) {
+      spx_state_destroy(s);
+      return rc;
+    }
+  if (bouss)
+    for (void** f : bfields)
+      if ((rc = mk(f))) {
+        spx_state_destroy(s);
+        return rc;
+      }
+  if (hipStreamSynchronize(p->stream) != hipSuccess) {
+    spx_state_destroy(s);
+    return fail(SPX_ERR_HIP, "state clear sync");
+  }
+  *out = s;
+  return SPX_OK;
+}
+
+int spx_state_destroy(spx_state* s) {
+  if (!s) return SPX_OK;
+  if (s->p) (void)hipStreamSynchronize(s->p->stream);
+  for (void* q : s->owned) (void)hipFree(q);
+  delete s;
+  return SPX_OK;
+}
+
+int spx_state_field(spx_state* s, const char* name, void** ptr) {
+  SPX_REQ(s && name && ptr, "spx_state_field: null argument");
+  struct {
+    const char* n;
+    void* v;
+  } tab[] = {{"vx", s->vx}, {"vy", s->vy}, {"vz", s->vz}, {"pr", s->pr}, {"fx", s->fx},
+             {"fy", s->fy}, {"fz", s->fz}, {"th", s->th}, {"fs", s->fs}};
+  for (auto& t : tab)
+    if (std::strcmp(t.n, name) == 0) {
+      if (!t.v) return fail(SPX_ERR_STATE, std::string("field ") + name + " not in this solver");
+      *ptr = t.v;
+      return SPX_OK;
+    }
+  return fail(SPX_ERR_ARG, std::string("unknown field ") + name);
+}
+
+int spx_rkstep1(spx_plan* p, spx_state* s) {
+  SPX_REQ(p && s && s->p == p, "spx_rkstep1: bad plan/state");
+  const size_t b = p->nc * 16;
+  SPX_HIP(hipMemcpyAsync(s->C1, s->vx, b, hipMemcpyDeviceToDevice, p->stream));
+  SPX_HIP(hipMemcpyAsync(s->C2, s->vy, b, hipMemcpyDeviceToDevice, p->stream));
+  SPX_HIP(hipMemcpyAsync(s->C3, s->vz, b, hipMemcpyDeviceToDevice, p->stream));
+  if (s->th) SPX_HIP(hipMemcpyAsync(s->C7, s->th, b, hipMemcpyDeviceToDevice, p->stream));
+  return SPX_OK;
+}
+
+int spx_substep(spx_plan* p, spx_state* s, int o) {
+  SPX_REQ(p && s && s->p == p, "spx_substep: bad plan/state");
+  const spx_params& q = s->prm;
+  SPX_REQ(o >= 1 && o <= q.ord, "spx_substep: stage out of range");
+  const bool bouss = q.solver == SPX_SOLVER_BOUSS;
+  const double rmp = 1.0 / double(o);
+  int rc;
+  if ((rc = spx_gradre(p, s->vx, s->vy, s->vz, s->N[0], s->N[1], s->N[2]))) return rc;
+  if (bouss && (rc = spx_advect(p, s->vx, s->vy, s->vz, s->th, s->N[3]))) return rc;
+  void* v[4] = {s->vx, s->vy, s->vz, s->th};
+  const void* c0[4] = {s->C1, s->C2, s->C3, s->C7};
+  void* nl[4] = {s->N[0], s->N[1], s->N[2], s->N[3]};
+  const void* f[4] = {s->fx, s->fy, s->fz, s->fs};
+  const void* src[4] = {nullptr, nullptr, s->th, s->vz};
+  const double coef[4] = {0.0, 0.0, q.xmom, q.xtemp};
+  const double nu[4] = {q.nu, q.nu, q.nu, q.kappa};
+  if ((rc = op_rk_update(p, bouss ? 4 : 3, v, c0, nl, f, bouss ? src : nullptr, coef, nu, q.dt,
+                         rmp)))
+    return rc;
+  if ((rc = spx_v_imposebc_and_project(p, s->vx, s->vy, s->vz, s->pr, o, q.ord, q.v_zsta,
+                                       q.v_zend)))
+    return rc;
+  if (bouss) {
+    if ((rc = spx_s_imposebc(p, s->th))) return rc;
+    if ((rc = op_filter(p, s->th))) return rc;
+    // round trip of th through real space (include/bouss/bouss_rkstep2.f90:56-59)
+    if ((rc = c2r3(p, s->th, p->R[0], 0, p->np))) return rc;
+    if ((rc = op_real_div3(p, p->R[0], double(p->g.nx), double(p->g.ny), double(p->g.nz))))
+      return rc;
+    if ((rc = r2c3(p, p->R[0], s->th))) return rc;
+  }
+  return SPX_OK;
+}
+
+int spx_step(spx_plan* p, spx_state* s, int nsteps) {
+  SPX_REQ(p && s && s->p == p && nsteps >= 0, "spx_step: bad arguments");
+  int rc;
+  for (int t = 0; t < nsteps; ++t) {
+    if ((rc = spx_rkstep1(p, s))) return rc;
+    for (int o = s->prm.ord; o >= 1; --o)
+      if ((rc = spx_substep(p, s, o))) return rc;
+  }
+  return SPX_OK;
+}
+
+// ---- memory helpers ----------------------------------------------------------------
+int spx_dev_alloc(size_t bytes, void** ptr) {
+  SPX_REQ(ptr, "spx_dev_alloc: null argument");
+  return alloc_dev(nullptr, ptr, bytes);
+}
+
+int spx_dev_free(void* ptr) {
+  SPX_HIP(hipFree(ptr));
+  return SPX_OK;
+}
+
+int spx_memcpy_h2d(spx_plan* p, void* dst, const void* src, size_t bytes) {
+  SPX_REQ(p && dst && src, "spx_memcpy_h2d: null argument");
+  SPX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, p->stream));
+  SPX_HIP(hipStreamSynchronize(p->stream));
+  return SPX_OK;
+}
+
+int spx_memcpy_d2h(spx_plan* p, void* dst, const void* src, size_t bytes) {
+  SPX_REQ(p && dst && src, "spx_memcpy_d2h: null argument");
+  SPX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, p->stream));
+  SPX_HIP(hipStreamSynchronize(p->stream));
+  return SPX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+// Shared definitions of the SPECTER MI355X hot-path library (libspx).
+//
+// Layouts are the reference's Fortran layouts, held in HBM unchanged:
+//   spectral S : (nz, ny, nkx) complex, z fastest   -- fftp.fpp:388-425 output
+//   real     R : (nx, ny, nz)  real,    x fastest
+//   xy-stage X : (nkx, ny, nz) complex, kx fastest  -- the reference's c1 buffer
+//                (fftp.fpp:446) before the (kx,y,z)->(z,y,kx) transpose
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/spx.h"
+
+namespace spx {
+
+template <typename T> struct cvec;
+template <> struct cvec<double> { using type = double2; };
+template <> struct cvec<float> { using type = float2; };
+
+// Device-side description of one batched line DFT of length n, computed either
+// directly (n smooth) or by Bluestein's chirp-z with an M-point smooth FFT.
+constexpr int kMaxStages = 16;
+template <typename C>
+struct LineFFT {
+  int n;      // logical DFT length
+  int M;      // length of the in-LDS FFT actually run (n, or the Bluestein size)
+  int blue;   // 1 = Bluestein
+  int nstage;
+  int radix[kMaxStages];
+  int tpl;    // threads per line
+  int nl;     // lines per workgroup
+  int ldl;    // LDS stride between lines (complex elements)
+  const C* tw;     // W_M[m] = exp(-2 pi i m / M), m < M
+  const C* chirp;  // w[m] = exp(-pi i m^2 / n), m < n        (Bluestein)
+  const C* bfil;   // FFT_M(conj chirp, wrapped) / M, M entries (Bluestein)
+};
+
+// Host-side owner of the tables behind a LineFFT.
+struct LineFFTHost {
+  int n = 0, M = 0, blue = 0, nstage = 0, tpl = 0, nl = 0, ldl = 0;
+  int radix[kMaxStages] = {0};
+  void* tw = nullptr;
+  void* chirp = nullptr;
+  void* bfil = nullptr;
+  size_t lds_bytes(size_t csize) const { return size_t(nl) * size_t(ldl) * csize; }
+  template <typename C> LineFFT<C> dev() const {
+    LineFFT<C> f;
+    f.n = n; f.M = M; f.blue = blue; f.nstage = nstage; f.tpl = tpl; f.nl = nl; f.ldl = ldl;
+    for (int i = 0; i < kMaxStages; ++i) f.radix[i] = radix[i];
+    f.tw = static_cast<const C*>(tw);
+    f.chirp = static_cast<const C*>(chirp);
+    f.bfil = static_cast<const C*>(bfil);
+    return f;
+  }
+};
+
+}  // namespace spx
+
+// Opaque plan (FCPLAN + fftp plans + module globals of kes/grid, fcgram_mod.f90:34-38,
+// pseudospec_mod.fpp:13-91).
+struct spx_plan {
+  spx_grid g;
+  int nkx = 0, np = 0;
+  size_t nc = 0, nr = 0;  // complex / real element counts of S and R arrays
+  hipStream_t stream = nullptr;
+  int csize = 16;         // bytes per complex element of the plan's field type
+  // FC-Gram continuation matrix dir = A*Q^T, row-major [C][d], float64 always.
+  double* d_dir = nullptr;
+  std::vector<double> h_dir;
+  // wavenumbers (float64), filter factors, z grid and khom(kx,ky)
+  double *d_kx = nullptr, *d_ky = nullptr, *d_kz = nullptr;
+  double *d_fkx = nullptr, *d_fky = nullptr, *d_fkz = nullptr;
+  double *d_z = nullptr, *d_khom = nullptr;
+  std::vector<double> h_kx, h_ky, h_kz, h_z, h_khom, h_fkx, h_fky, h_fkz;
+  double Dkx = 0, Dky = 0, Dkz = 0, dz = 0;
+  spx::LineFFTHost fx, fy, fz;
+  spx::LineFFTHost fz2;     // fz with half the lines per workgroup (two-buffer kernels)
+  // scratch
+  void* X = nullptr;        // xy-stage buffer, nc complex
+  void* R[7] = {nullptr};   // real work arrays for gradre/advect (nr reals each)
+  void* bc = nullptr;       // wall data of sol_project, 2 complex per (kx,ky) line
+  double* part = nullptr;   // per-line partial sums of the diagnostics
+  std::vector<double> h_part;
+  std::vector<void*> owned;
+};
+
+namespace spx {
+
+// error handling ------------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+#define SPX_HIP(call)                                                       \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return ::spx::fail(SPX_ERR_HIP, std::string(#call) + ": " +          \
+                                          hipGetErrorString(e_));           \
+  } while (0)
+#define SPX_CHECK_LAUNCH() SPX_HIP(hipGetLastError())
+
+// FFT launchers (spx_fft.hip) ------------------------------------------------
+int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines);
+void free_line_fft(LineFFTHost& f);
+
+// z transforms in place on S (contiguous lines).
+//   cont:   1 = apply the FC continuation before a forward transform
+//   scale:  multiply k < np by scale after a backward transform (1.0 = none)
+int zfft_inplace(spx_plan* p, void* S, int forward, int cont, double scale);
+// the continuation loop alone (fftp.fpp:755-773), in place on S
+int fc_continue_only(spx_plan* p, void* S);
+// forward 3D last pass: X (planes k < np) -> continuation -> z FFT -> S.
+// nofft = 1: transpose only (all nz planes), for fftp2d_real_to_complex_xy.
+int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft);
+// backward 3D first pass: S * (i k_dir) (dir 0 = none) -> z IFFT -> X, planes < nzout.
+int zfft_s2x(spx_plan* p, const void* S, void* X, int dir, int nzout, int nofft);
+int yfft(spx_plan* p, void* X, int forward, int nplanes);
+int xr2c(spx_plan* p, const void* R, void* X, int nplanes);
+int xc2r(spx_plan* p, const void* X, void* R, int nplanes);
+// fused v_imposebc: goto_domain_w_boundaries + noslip_z + goto_3d_fourier on vx,vy
+int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
+                 const double vzsta[2], const double vzend[2]);
+// s_imposebc: z IFFT, 1/nz, th = 0 at both walls, continuation + z FFT
+int sconst_fused(spx_plan* p, void* th);
+// sol_project stages (bctarget = 1, Neumann potential)
+int proj_bc(spx_plan* p, const void* c);             // bc <- IFFT_z(c/nz) at the walls
+int proj_pressure(spx_plan* p, void* d);             // d <- IFFT_z(d)/nz + harmonic a
+int proj_harmonic(spx_plan* p, void* a, void* b, void* c);  // v -= grad(harmonic)
+// z IFFT of a linear combination of derivatives, reduced per line over physical z
+// mode 0: sum |u|^2 ; the combination is acc = m0(in0) (+/-) m1(in1) (+/-) m2(in2)
+struct LinComb {
+  const void* in[3];
+  int dir[3];    // 0 none, 1 x, 2 y, 3 z
+  int sgn[3];    // +1 / -1 for terms 1,2 (term 0 always +1); 0 = term absent
+};
+int zline_sumsq(spx_plan* p, const LinComb& lc, double* line_out);
+int zline_dot(spx_plan* p, const void* a, const void* b, double* line_out);
+int zline_walls(spx_plan* p, const void* a, double* line_out /*2 per line*/);
+
+// pointwise launchers (spx_ops.hip) -------------------------------------------
+int op_derivk(spx_plan* p, const void* a, void* b, int dir);
+int op_laplak(spx_plan* p, const void* a, void* b);
+int op_filter(spx_plan* p, void* a);
+int op_scale(spx_plan* p, void* a, size_t n, double s);
+int op_prod_stage(spx_plan* p, int stage, const void* r1, const void* r2, const void* r3,
+                  const void* r4, void* rx, void* ry, void* rz, int nout);
+int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, void* const* nl,
+                 const void* const* f, const void* const* src, const double* src_coef,
+                 const double* nu, double dt, double rmp);
+int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d);
+int op_axpy_field(spx_plan* p, void* y, const void* x, double alpha);
+int op_real_div3(spx_plan* p, void* r, double nx, double ny, double nz);
+
+}  // namespace spx

@@ -1,0 +1,230 @@
+"""Host-side mirror of the reference's FFTP/FC-Gram, operator and boundary
+interface, over the libspx C ABI.
+
+Method names, argument meaning and in/out conventions follow the reference
+(fftp/fftp.fpp, fftp/fcgram_mod.f90, pseudo/pseudospec_hd.f90,
+boundary/boundary_mod.fpp, boundary/vboundary.f90).  Device arrays are torch
+tensors on a ROCm device using C-order shapes that reproduce the Fortran
+memory layout:
+
+* spectral  Fortran (nz, ny, nkx)  ->  torch (nkx, ny, nz) complex128
+* real      Fortran (nx, ny, nz)   ->  torch (nz, ny, nx)  float64
+
+Every call goes through libspx; there is no host fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_double, c_void_p
+
+import numpy as np
+
+from . import lib as _l
+from .tables import load_dirichlet_tables
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class FCPlan:
+    """FCPLAN + fftp plans + the kes/grid module state (fcgram_mod.f90:34-144,
+    specter.fpp:683-800) for one grid on one HIP stream."""
+
+    def __init__(self, nx: int, ny: int, nz: int, Cz: int, oz: int, Lx: float = 1.0,
+                 Ly: float = 1.0, Lz: float = 1.0, tdir: str | None = None, stream=None,
+                 precision: int = _l.SPX_FP64):
+        lib = _l.load_library()
+        self._lib = lib
+        g = _l.SpxGrid(nx, ny, nz, Cz, oz, Lx, Ly, Lz, precision)
+        A = Q = None
+        if Cz > 0:
+            A, Q = load_dirichlet_tables(Cz, oz, tdir)
+        self.A, self.Q = A, Q
+        torch = _torch()
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        self.stream = stream
+        ptr = c_void_p()
+        _l.check(lib.spx_plan_create(
+            byref(g),
+            A.ctypes.data_as(ctypes.POINTER(c_double)) if A is not None else None,
+            Q.ctypes.data_as(ctypes.POINTER(c_double)) if Q is not None else None,
+            c_void_p(stream), byref(ptr)))
+        self._p = ptr
+        info = _l.SpxPlanInfo()
+        _l.check(lib.spx_plan_get_info(ptr, byref(info)))
+        self.nx, self.ny, self.nz, self.Cz, self.oz = nx, ny, nz, Cz, oz
+        self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
+        self.nkx, self.np = info.nkx, info.np
+        self.z_fft_len, self.z_bluestein = info.z_fft_len, info.z_bluestein
+
+    # -- lifetime ---------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_p", None):
+            self._lib.spx_plan_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._p
+
+    def info(self) -> _l.SpxPlanInfo:
+        info = _l.SpxPlanInfo()
+        _l.check(self._lib.spx_plan_get_info(self._p, byref(info)))
+        return info
+
+    def sync(self):
+        _l.check(self._lib.spx_plan_sync(self._p))
+
+    # -- array helpers ------------------------------------------------------------
+    @property
+    def spec_shape(self):
+        return (self.nkx, self.ny, self.nz)
+
+    @property
+    def real_shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    def empty_spectral(self):
+        torch = _torch()
+        return torch.empty(self.spec_shape, dtype=torch.complex128, device="cuda")
+
+    def empty_real(self):
+        torch = _torch()
+        return torch.empty(self.real_shape, dtype=torch.float64, device="cuda")
+
+    def _spec(self, t, name):
+        torch = _torch()
+        if not (t.is_cuda and t.dtype == torch.complex128 and t.is_contiguous()
+                and tuple(t.shape) == self.spec_shape):
+            raise ValueError(f"{name}: expected contiguous cuda complex128 {self.spec_shape}")
+        return c_void_p(t.data_ptr())
+
+    def _real(self, t, name):
+        torch = _torch()
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+                and tuple(t.shape) == self.real_shape):
+            raise ValueError(f"{name}: expected contiguous cuda float64 {self.real_shape}")
+        return c_void_p(t.data_ptr())
+
+    # -- transforms (fftp.fpp) ----------------------------------------------------------
+    def fftp3d_real_to_complex(self, inp, out):
+        """fftp.fpp:388-425."""
+        _l.check(self._lib.spx_fftp3d_real_to_complex(self._p, self._real(inp, "in"),
+                                                      self._spec(out, "out")))
+        return out
+
+    def fftp3d_complex_to_real(self, inp, out):
+        """fftp.fpp:789-821 (the input may be overwritten, as in the reference)."""
+        _l.check(self._lib.spx_fftp3d_complex_to_real(self._p, self._spec(inp, "in"),
+                                                      self._real(out, "out")))
+        return out
+
+    def fftp2d_real_to_complex_xy(self, inp, out):
+        """fftp.fpp:428-524."""
+        _l.check(self._lib.spx_fftp2d_real_to_complex_xy(self._p, self._real(inp, "in"),
+                                                         self._spec(out, "out")))
+        return out
+
+    def fftp2d_complex_to_real_xy(self, inp, out):
+        """fftp.fpp:824-919."""
+        _l.check(self._lib.spx_fftp2d_complex_to_real_xy(self._p, self._spec(inp, "in"),
+                                                         self._real(out, "out")))
+        return out
+
+    def fftp1d_real_to_complex_z(self, inout):
+        """fftp.fpp:720-786 (continuation + z FFT, in place)."""
+        _l.check(self._lib.spx_fftp1d_real_to_complex_z(self._p, self._spec(inout, "inout")))
+        return inout
+
+    def fftp1d_complex_to_real_z(self, inout):
+        """fftp.fpp:1060-1094 (z IFFT, in place)."""
+        _l.check(self._lib.spx_fftp1d_complex_to_real_z(self._p, self._spec(inout, "inout")))
+        return inout
+
+    def fc_continue(self, inout):
+        """fftp.fpp:755-773."""
+        _l.check(self._lib.spx_fc_continue(self._p, self._spec(inout, "inout")))
+        return inout
+
+    # -- operators (pseudospec_hd.f90 / pseudospec_phd.f90) ------------------------------
+    def derivk(self, a, b, dir: int):
+        _l.check(self._lib.spx_derivk(self._p, self._spec(a, "a"), self._spec(b, "b"), dir))
+        return b
+
+    def laplak(self, a, b):
+        _l.check(self._lib.spx_laplak(self._p, self._spec(a, "a"), self._spec(b, "b")))
+        return b
+
+    def fc_filter(self, a):
+        _l.check(self._lib.spx_fc_filter(self._p, self._spec(a, "a")))
+        return a
+
+    def gradre(self, a, b, c, d, e, f):
+        _l.check(self._lib.spx_gradre(self._p, *(self._spec(t, n) for t, n in
+                                                 zip((a, b, c, d, e, f), "abcdef"))))
+        return d, e, f
+
+    def advect(self, a, b, c, d, e):
+        _l.check(self._lib.spx_advect(self._p, *(self._spec(t, n) for t, n in
+                                                 zip((a, b, c, d, e), "abcde"))))
+        return e
+
+    # -- boundary conditions (boundary/) ----------------------------------------------
+    def sol_project(self, a, b, c, d, bctarget=1, bczsta=0, bczend=0):
+        _l.check(self._lib.spx_sol_project(self._p, self._spec(a, "a"), self._spec(b, "b"),
+                                           self._spec(c, "c"), self._spec(d, "d"), bctarget,
+                                           bczsta, bczend))
+
+    def v_imposebc_and_project(self, vx, vy, vz, pr, rki: int, ord: int, v_zsta=(0.0, 0.0),
+                               v_zend=(0.0, 0.0)):
+        s = (c_double * 2)(*v_zsta)
+        e = (c_double * 2)(*v_zend)
+        _l.check(self._lib.spx_v_imposebc_and_project(
+            self._p, self._spec(vx, "vx"), self._spec(vy, "vy"), self._spec(vz, "vz"),
+            self._spec(pr, "pr"), rki, ord, s, e))
+
+    def s_imposebc(self, th):
+        _l.check(self._lib.spx_s_imposebc(self._p, self._spec(th, "th")))
+
+    # -- diagnostics -----------------------------------------------------------------------
+    def energy(self, a, b, c, kin: int) -> float:
+        out = c_double()
+        _l.check(self._lib.spx_energy(self._p, self._spec(a, "a"), self._spec(b, "b"),
+                                      self._spec(c, "c"), kin, byref(out)))
+        return out.value
+
+    def cross(self, a, b, c, d, e, f) -> float:
+        out = c_double()
+        _l.check(self._lib.spx_cross(self._p, *(self._spec(t, n) for t, n in
+                                                zip((a, b, c, d, e, f), "abcdef")), byref(out)))
+        return out.value
+
+    def divergence(self, a, b, c) -> float:
+        out = c_double()
+        _l.check(self._lib.spx_divergence(self._p, self._spec(a, "a"), self._spec(b, "b"),
+                                          self._spec(c, "c"), byref(out)))
+        return out.value
+
+    def bouncheck_z(self, a, b=None):
+        bot, top = c_double(), c_double()
+        _l.check(self._lib.spx_bouncheck_z(self._p, self._spec(a, "a"),
+                                           self._spec(b, "b") if b is not None else None,
+                                           byref(bot), byref(top)))
+        return bot.value, top.value
+
+    # -- grid values (host copies of what the plan uses) --------------------------------
+    def grid_z(self) -> np.ndarray:
+        if self.Cz > 0:
+            dz = self.Lz / (self.nz - self.Cz - 1)
+        else:
+            dz = self.Lz * 2.0 * 3.14159265358979323846 / self.nz
+        return dz * np.arange(self.nz, dtype=np.float64)

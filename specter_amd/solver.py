@@ -1,0 +1,129 @@
+"""Device-resident HD / Boussinesq solver over libspx (the RK loop of
+specter.fpp:1002-1165 with include/hd and include/bouss stage bodies).
+
+All state lives in HBM inside libspx; host code only sets it up, steps it and
+reads diagnostics.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import byref, c_void_p
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import lib as _l
+from .fftp import FCPlan
+
+
+class _DevView:
+    """__cuda_array_interface__ view of a libspx device array (zero copy)."""
+
+    def __init__(self, ptr: int, shape, typestr: str):
+        self.__cuda_array_interface__ = {
+            "shape": tuple(shape), "typestr": typestr, "data": (ptr, False),
+            "version": 3, "strides": None}
+
+
+@dataclass
+class SolverParams:
+    solver: str = "HD"        # "HD" or "BOUSS"
+    ord: int = 2              # RK order ORD
+    dt: float = 1e-3
+    nu: float = 2e-3
+    kappa: float = 0.0
+    xmom: float = 0.0         # gama*xmom (specter.fpp:480-491)
+    xtemp: float = 0.0        # gama*xtemp
+    v_zsta: tuple = (0.0, 0.0)
+    v_zend: tuple = (0.0, 0.0)
+
+    def to_c(self) -> _l.SpxParams:
+        p = _l.SpxParams()
+        p.solver = _l.SPX_SOLVER_BOUSS if self.solver.upper() == "BOUSS" else _l.SPX_SOLVER_HD
+        p.ord, p.dt, p.nu, p.kappa = self.ord, self.dt, self.nu, self.kappa
+        p.xmom, p.xtemp = self.xmom, self.xtemp
+        p.v_zsta[0], p.v_zsta[1] = self.v_zsta
+        p.v_zend[0], p.v_zend[1] = self.v_zend
+        return p
+
+
+class Solver:
+    """One run: an FCPlan plus a libspx solver state."""
+
+    FIELDS = ("vx", "vy", "vz", "pr", "fx", "fy", "fz", "th", "fs")
+
+    def __init__(self, plan: FCPlan, params: SolverParams):
+        self.plan = plan
+        self.params = params
+        self._lib = plan._lib
+        st = c_void_p()
+        _l.check(self._lib.spx_state_create(plan.handle, byref(params.to_c()), byref(st)))
+        self._s = st
+        self.bouss = params.solver.upper() == "BOUSS"
+
+    def close(self):
+        if getattr(self, "_s", None):
+            self._lib.spx_state_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def field(self, name: str):
+        """Zero-copy torch view (complex128, (nkx, ny, nz)) of a state field."""
+        import torch
+        ptr = c_void_p()
+        _l.check(self._lib.spx_state_field(self._s, name.encode(), byref(ptr)))
+        return torch.as_tensor(_DevView(ptr.value, self.plan.spec_shape, "<c16"), device="cuda")
+
+    # -- set-up -------------------------------------------------------------------
+    def set_real_field(self, name: str, real: np.ndarray):
+        """field <- fftp3d_real_to_complex(real) (restart path, specter.fpp:894-899)."""
+        import torch
+        r = torch.as_tensor(np.ascontiguousarray(real, dtype=np.float64)).cuda()
+        self.plan.fftp3d_real_to_complex(r, self.field(name))
+        self.plan.sync()
+
+    # -- stepping --------------------------------------------------------------------
+    def rkstep1(self):
+        _l.check(self._lib.spx_rkstep1(self.plan.handle, self._s))
+
+    def substep(self, o: int):
+        _l.check(self._lib.spx_substep(self.plan.handle, self._s, o))
+
+    def step(self, nsteps: int = 1):
+        _l.check(self._lib.spx_step(self.plan.handle, self._s, nsteps))
+
+    # -- diagnostics (hd_global.f90 / vdiagnostic) -----------------------------------------
+    def hdcheck(self):
+        p = self.plan
+        vx, vy, vz = self.field("vx"), self.field("vy"), self.field("vz")
+        fx, fy, fz = self.field("fx"), self.field("fy"), self.field("fz")
+        return (p.energy(vx, vy, vz, 1), p.energy(vx, vy, vz, 0),
+                p.cross(vx, vy, vz, fx, fy, fz))
+
+    def vdiagnostic(self):
+        p = self.plan
+        vx, vy, vz = self.field("vx"), self.field("vy"), self.field("vz")
+        dv = p.divergence(vx, vy, vz)
+        tb, tt = p.bouncheck_z(vx, vy)
+        nb, nt = p.bouncheck_z(vz)
+        return dv, tb, tt, nb, nt
+
+    def real_field(self, name: str) -> np.ndarray:
+        """Physical-space field (c2r / (nx ny nz)), physical planes only."""
+        import torch
+        p = self.plan
+        tmp = self.field(name).clone()
+        out = p.empty_real()
+        p.fftp3d_complex_to_real(tmp, out)
+        p.sync()
+        return (out[: p.np] / (p.nx * p.ny * p.nz)).cpu().numpy()
+
+    def spectral(self, name: str) -> np.ndarray:
+        t = self.field(name)
+        self.plan.sync()
+        return t.cpu().numpy()

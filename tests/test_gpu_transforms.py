@@ -1,0 +1,198 @@
+"""GPU parity of the libspx transforms and pointwise operators against the
+float64 oracle, through the C ABI (specter_amd.FCPlan)."""
+import numpy as np
+import pytest
+
+from conftest import TABLES, rel_err
+
+pytestmark = pytest.mark.gpu
+
+# (nx, ny, nz, Cz, d): nz prime (Bluestein), even smooth, 3*19, 7*13 (direct radix 7/13)
+SHAPES = [
+    (16, 16, 41, 25, 5),
+    (16, 16, 64, 25, 5),
+    (32, 24, 57, 25, 5),
+    (12, 20, 91, 27, 8),
+    (64, 64, 89, 25, 5),
+]
+TOL = 1e-13
+
+
+def _proj_tol(shape):
+    """The d=8 continuation amplifies roundoff of white-noise input: a 1-ulp
+    input perturbation moves the oracle's own sol_project output by 1.2e-11
+    (d=5: 5e-15)."""
+    return 1e-12 if shape[4] <= 5 else 1e-10
+
+
+def _setup(shape):
+    import torch
+    from oracle import specter_ref as O
+    from specter_amd import FCPlan
+    nx, ny, nz, C, d = shape
+    g = O.make_grid(nx, ny, nz, C, d, TABLES)
+    p = FCPlan(nx, ny, nz, C, d)
+    return torch, O, g, p
+
+
+def _rand_spec(g, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(g.spec_shape) + 1j * rng.standard_normal(g.spec_shape)
+
+
+def _rand_real(g, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(g.real_shape)
+
+
+def _cu(torch, a):
+    return torch.as_tensor(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_r2c_3d(shape):
+    torch, O, g, p = _setup(shape)
+    R = _rand_real(g, 1)
+    ref = O.fftp3d_real_to_complex(g, R.copy())
+    out = p.empty_spectral()
+    p.fftp3d_real_to_complex(_cu(torch, R), out)
+    assert rel_err(out.cpu().numpy(), ref) < TOL
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_c2r_3d(shape):
+    torch, O, g, p = _setup(shape)
+    S = _rand_spec(g, 2)
+    ref = O.fftp3d_complex_to_real(g, S.copy())
+    out = p.empty_real()
+    p.fftp3d_complex_to_real(_cu(torch, S), out)
+    assert rel_err(out.cpu().numpy(), ref) < TOL
+
+
+@pytest.mark.parametrize("shape", SHAPES[:3], ids=lambda s: "x".join(map(str, s)))
+def test_xy_2d(shape):
+    torch, O, g, p = _setup(shape)
+    R = _rand_real(g, 3)
+    ref = O.fftp2d_real_to_complex_xy(g, R.copy())
+    out = p.empty_spectral()
+    p.fftp2d_real_to_complex_xy(_cu(torch, R), out)
+    assert rel_err(out.cpu().numpy(), ref) < TOL
+    S = _rand_spec(g, 4)
+    ref2 = O.fftp2d_complex_to_real_xy(g, S.copy())
+    out2 = p.empty_real()
+    p.fftp2d_complex_to_real_xy(_cu(torch, S), out2)
+    assert rel_err(out2.cpu().numpy(), ref2) < TOL
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_z_transforms_and_continuation(shape):
+    torch, O, g, p = _setup(shape)
+    S = _rand_spec(g, 5)
+    ref = S.copy()
+    O.fc_continue(g, ref)
+    t = _cu(torch, S)
+    p.fc_continue(t)
+    # continuation alone is the same float64 operation sequence: bit-exact
+    assert np.array_equal(t.cpu().numpy(), ref)
+    ref = S.copy()
+    O.fftp1d_real_to_complex_z(g, ref)
+    t = _cu(torch, S)
+    p.fftp1d_real_to_complex_z(t)
+    assert rel_err(t.cpu().numpy(), ref) < TOL
+    ref = S.copy()
+    O.fftp1d_complex_to_real_z(g, ref)
+    t = _cu(torch, S)
+    p.fftp1d_complex_to_real_z(t)
+    assert rel_err(t.cpu().numpy(), ref) < TOL
+
+
+@pytest.mark.parametrize("shape", SHAPES[:2], ids=lambda s: "x".join(map(str, s)))
+def test_pointwise_operators(shape):
+    torch, O, g, p = _setup(shape)
+    S = _rand_spec(g, 6)
+    for dirn in (1, 2, 3):
+        out = p.empty_spectral()
+        p.derivk(_cu(torch, S), out, dirn)
+        assert np.array_equal(out.cpu().numpy(), O.derivk(g, S, dirn))
+    out = p.empty_spectral()
+    p.laplak(_cu(torch, S), out)
+    assert np.array_equal(out.cpu().numpy(), O.laplak(g, S))
+    t = _cu(torch, S)
+    p.fc_filter(t)
+    ref = S.copy()
+    O.fc_filter(g, ref)
+    assert rel_err(t.cpu().numpy(), ref) < 1e-15
+
+
+@pytest.mark.parametrize("shape", SHAPES[:4], ids=lambda s: "x".join(map(str, s)))
+def test_gradre(shape):
+    torch, O, g, p = _setup(shape)
+    a, b, c = (_rand_spec(g, s) for s in (7, 8, 9))
+    ref = O.gradre(g, a, b, c)
+    outs = [p.empty_spectral() for _ in range(3)]
+    p.gradre(_cu(torch, a), _cu(torch, b), _cu(torch, c), *outs)
+    for o, r in zip(outs, ref):
+        assert rel_err(o.cpu().numpy(), r) < 1e-12
+
+
+@pytest.mark.parametrize("shape", SHAPES[:4], ids=lambda s: "x".join(map(str, s)))
+def test_advect(shape):
+    torch, O, g, p = _setup(shape)
+    a, b, c, d = (_rand_spec(g, s) for s in (10, 11, 12, 13))
+    ref = O.advect(g, a, b, c, d)
+    out = p.empty_spectral()
+    p.advect(_cu(torch, a), _cu(torch, b), _cu(torch, c), _cu(torch, d), out)
+    assert rel_err(out.cpu().numpy(), ref) < 1e-12
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_sol_project(shape):
+    torch, O, g, p = _setup(shape)
+    a, b, c, d = (_rand_spec(g, s) for s in (14, 15, 16, 17))
+    ra, rb, rc, rd = a.copy(), b.copy(), c.copy(), d.copy()
+    O.sol_project(g, ra, rb, rc, rd, 1, 0, 0)
+    ta, tb, tc, td = (_cu(torch, x) for x in (a, b, c, d))
+    p.sol_project(ta, tb, tc, td, 1, 0, 0)
+    for t, r in ((ta, ra), (tb, rb), (tc, rc), (td, rd)):
+        assert rel_err(t.cpu().numpy(), r) < _proj_tol(shape)
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("o,ordr", [(2, 2), (1, 2), (3, 4)])
+def test_v_imposebc_and_project(shape, o, ordr):
+    torch, O, g, p = _setup(shape)
+    vx, vy, vz, pr = (_rand_spec(g, s) for s in (18, 19, 20, 21))
+    r = [x.copy() for x in (vx, vy, vz, pr)]
+    O.v_imposebc_and_project(g, *r, o, ordr, (0.0, 0.0), (0.0, 0.0))
+    ts = [_cu(torch, x) for x in (vx, vy, vz, pr)]
+    p.v_imposebc_and_project(*ts, o, ordr)
+    for t, rr in zip(ts, r):
+        assert rel_err(t.cpu().numpy(), rr) < _proj_tol(shape)
+
+
+@pytest.mark.parametrize("shape", SHAPES[:3], ids=lambda s: "x".join(map(str, s)))
+def test_s_imposebc(shape):
+    torch, O, g, p = _setup(shape)
+    th = _rand_spec(g, 22)
+    r = th.copy()
+    O.s_imposebc(g, r)
+    t = _cu(torch, th)
+    p.s_imposebc(t)
+    assert rel_err(t.cpu().numpy(), r) < 1e-12
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_diagnostics(shape):
+    torch, O, g, p = _setup(shape)
+    a, b, c, d, e, f = (_rand_spec(g, s) for s in range(30, 36))
+    ta, tb, tc, td, te, tf = (_cu(torch, x) for x in (a, b, c, d, e, f))
+    for kin in (0, 1):
+        assert rel_err(p.energy(ta, tb, tc, kin), O.energy(g, a, b, c, kin)) < 1e-12
+    assert rel_err(p.cross(ta, tb, tc, td, te, tf), O.cross(g, a, b, c, d, e, f)) < 1e-12
+    assert rel_err(p.divergence(ta, tb, tc), O.divergence(g, a, b, c)) < 1e-12
+    bt = p.bouncheck_z(ta, tb)
+    rb = O.bouncheck_z(g, a, b)
+    assert rel_err(np.array(bt), np.array(rb)) < 1e-12
+    bt = p.bouncheck_z(tc)
+    rb = O.bouncheck_z(g, c)
+    assert rel_err(np.array(bt), np.array(rb)) < 1e-12
