@@ -1,0 +1,105 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry point
+declared in include/spx.h, and rejects invalid arguments before touching the
+device.  Also host-side logic (tables, initial conditions)."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from specter_amd import lib as L
+    if not os.path.exists(L.LIB_PATH):
+        from specter_amd.build import build
+        build()
+    return L.load_library()
+
+
+def test_every_header_symbol_is_exported(lib):
+    from specter_amd.lib import header_symbols, _SIGS
+    syms = header_symbols()
+    assert len(syms) >= 30
+    out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(ROOT, "specter_amd",
+                                                                      "libspx.so")],
+                         capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    for s in syms:
+        assert s in exported, s
+        assert hasattr(lib, s)
+        assert s in _SIGS, f"{s} has no ctypes signature"
+
+
+def _err(lib):
+    buf = ctypes.create_string_buffer(512)
+    lib.spx_last_error(buf, 512)
+    return buf.value.decode()
+
+
+@pytest.mark.parametrize("grid,code,msg", [
+    ((15, 16, 41, 25, 5, 1.0, 1.0, 1.0, 0), 1, "nx must be even"),
+    ((16, 16, 41, 25, 5, 1.0, -1.0, 1.0, 0), 1, "box lengths"),
+    ((16, 16, 41, 25, 0, 1.0, 1.0, 1.0, 0), 1, "matching order"),
+    ((16, 16, 30, 25, 5, 1.0, 1.0, 1.0, 0), 1, "too few physical"),
+    ((16, 16, 41, 25, 5, 1.0, 1.0, 1.0, 1), 2, "only SPX_FP64"),
+])
+def test_plan_create_validates_before_device(lib, grid, code, msg):
+    from specter_amd.lib import SpxGrid
+    from specter_amd.tables import load_dirichlet_tables
+    A, Q = load_dirichlet_tables(25, 5)
+    g = SpxGrid(*grid)
+    p = ctypes.c_void_p()
+    rc = lib.spx_plan_create(ctypes.byref(g), A.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                             Q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), None,
+                             ctypes.byref(p))
+    assert rc == code
+    assert msg in _err(lib)
+    assert not p.value
+
+
+def test_missing_tables_rejected(lib):
+    from specter_amd.lib import SpxGrid
+    g = SpxGrid(16, 16, 41, 25, 5, 1.0, 1.0, 1.0, 0)
+    p = ctypes.c_void_p()
+    assert lib.spx_plan_create(ctypes.byref(g), None, None, None, ctypes.byref(p)) == 1
+    assert "tables" in _err(lib)
+
+
+def test_null_plan_rejected(lib):
+    assert lib.spx_fftp3d_real_to_complex(None, None, None) == 1
+    assert lib.spx_substep(None, None, 1) == 1
+    assert lib.spx_derivk(None, None, None, 1) == 1
+
+
+def test_table_loader():
+    from specter_amd.tables import load_dirichlet_tables
+    A, Q = load_dirichlet_tables(27, 8)
+    assert A.shape == (27, 8) and Q.shape == (8, 8)
+    with pytest.raises(FileNotFoundError):
+        load_dirichlet_tables(26, 5)
+
+
+def test_initial_condition_properties():
+    from specter_amd import ic
+    nx, ny, nz, C = 16, 16, 41, 25
+    u = ic.spectrum_peaked_velocity(nx, ny, nz, C, seed=7)
+    npz = nz - C
+    e = sum(np.mean(c[:npz] ** 2) for c in u)
+    assert abs(e - 1.0) < 1e-12
+    for c in u:
+        assert np.all(c[0] == 0) and np.all(c[npz - 1] == 0) and np.all(c[npz:] == 0)
+    u2 = ic.spectrum_peaked_velocity(nx, ny, nz, C, seed=7)
+    assert all(np.array_equal(a, b) for a, b in zip(u, u2))       # counter-based RNG
+    u3 = ic.spectrum_peaked_velocity(nx, ny, nz, C, seed=8)
+    assert not np.array_equal(u[0], u3[0])
+    th = ic.thermal_perturbation(nx, ny, nz, C, seed=1)
+    assert np.all(th[0] == 0) and np.all(th[npz - 1] == 0) and abs(th[1:npz - 1].std() - 1e-3) < 2e-4
+    fx, fy, fz = ic.large_scale_forcing(nx, ny, nz, C)
+    x, y, _ = ic.physical_grid(nx, ny, nz, C)
+    env = ic.wall_envelope(nz, C)
+    assert np.allclose(fx[:npz], 0.5 * np.sin(2 * y)[None, :, None] * env[:, None, None])
+    assert np.all(fz == 0)
