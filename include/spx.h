@@ -154,6 +154,16 @@ int spx_substep(spx_plan* p, spx_state* s, int o);
 /* nsteps full steps: rkstep1 + o = ord..1 (specter.fpp:1142-1161). */
 int spx_step(spx_plan* p, spx_state* s, int nsteps);
 
+/* ---- per-kernel timing (no counterpart in the reference; the reference's
+ * gtimer accumulators ffttime/tratime/conttime, fftp_mod.fpp:32-37, play this
+ * role on the CPU).  When enabled, every kernel launch of the plan is bracketed
+ * by HIP events on the plan's stream; spx_prof_resolve synchronises and sums
+ * elapsed time and algorithmic bytes (compulsory HBM reads + writes) per kernel. */
+int spx_prof_enable(spx_plan* p, int on);   /* on = 1 clears previous records */
+int spx_prof_resolve(spx_plan* p, int* nkernels);
+int spx_prof_get(spx_plan* p, int idx, char* name, size_t len, long* launches,
+                 double* total_ms, double* total_bytes);
+
 /* ---- memory helpers for callers without a HIP binding (e.g. Fortran) ------ */
 int spx_dev_alloc(size_t bytes, void** ptr);
 int spx_dev_free(void* ptr);

@@ -18,6 +18,35 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+ProfScope::ProfScope(spx_plan* plan, const char* name, double bytes) : p(plan) {
+  if (!p || !p->prof.on) return;
+  Profiler& pr = p->prof;
+  int kind = -1;
+  for (size_t i = 0; i < pr.names.size(); ++i)
+    if (pr.names[i] == name) kind = int(i);
+  if (kind < 0) {
+    pr.names.push_back(name);
+    kind = int(pr.names.size()) - 1;
+  }
+  while (pr.pool.size() < pr.used + 2) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    pr.pool.push_back(e);
+  }
+  ProfRec r;
+  r.kind = kind;
+  r.a = pr.pool[pr.used++];
+  r.b = pr.pool[pr.used++];
+  r.bytes = bytes;
+  if (hipEventRecord(r.a, p->stream) != hipSuccess) return;
+  pr.recs.push_back(r);
+  rec = int(pr.recs.size()) - 1;
+}
+
+ProfScope::~ProfScope() {
+  if (rec >= 0) (void)hipEventRecord(p->prof.recs[rec].b, p->stream);
+}
+
 static int alloc_dev(spx_plan* p, void** ptr, size_t bytes) {
   const hipError_t e = hipMalloc(ptr, bytes);
   if (e != hipSuccess) {
@@ -217,6 +246,7 @@ int spx_plan_destroy(spx_plan* p) {
   free_line_fft(p->fx);
   free_line_fft(p->fy);
   free_line_fft(p->fz);
+  for (hipEvent_t e : p->prof.pool) (void)hipEventDestroy(e);
   for (void* q : p->owned) (void)hipFree(q);
   delete p;
   return SPX_OK;
@@ -588,6 +618,50 @@ int spx_step(spx_plan* p, spx_state* s, int nsteps) {
     for (int o = s->prm.ord; o >= 1; --o)
       if ((rc = spx_substep(p, s, o))) return rc;
   }
+  return SPX_OK;
+}
+
+// ---- per-kernel timing --------------------------------------------------------------
+int spx_prof_enable(spx_plan* p, int on) {
+  SPX_REQ(p, "spx_prof_enable: null plan");
+  SPX_HIP(hipStreamSynchronize(p->stream));
+  Profiler& pr = p->prof;
+  pr.on = on != 0;
+  if (on) {
+    pr.recs.clear();
+    pr.used = 0;
+    pr.names.clear();
+  }
+  return SPX_OK;
+}
+
+int spx_prof_resolve(spx_plan* p, int* nk) {
+  SPX_REQ(p && nk, "spx_prof_resolve: null argument");
+  SPX_HIP(hipStreamSynchronize(p->stream));
+  Profiler& pr = p->prof;
+  pr.launches.assign(pr.names.size(), 0);
+  pr.total_ms.assign(pr.names.size(), 0.0);
+  pr.total_bytes.assign(pr.names.size(), 0.0);
+  for (const ProfRec& r : pr.recs) {
+    float ms = 0.f;
+    SPX_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    pr.launches[r.kind] += 1;
+    pr.total_ms[r.kind] += double(ms);
+    pr.total_bytes[r.kind] += r.bytes;
+  }
+  *nk = int(pr.names.size());
+  return SPX_OK;
+}
+
+int spx_prof_get(spx_plan* p, int idx, char* name, size_t len, long* launches, double* total_ms,
+                 double* total_bytes) {
+  SPX_REQ(p && name && launches && total_ms && total_bytes, "spx_prof_get: null argument");
+  const Profiler& pr = p->prof;
+  SPX_REQ(idx >= 0 && size_t(idx) < pr.launches.size(), "spx_prof_get: index out of range");
+  std::snprintf(name, len, "%s", pr.names[idx].c_str());
+  *launches = pr.launches[idx];
+  *total_ms = pr.total_ms[idx];
+  *total_bytes = pr.total_bytes[idx];
   return SPX_OK;
 }
 

@@ -58,6 +58,23 @@ struct LineFFTHost {
   }
 };
 
+// Per-launch HIP-event timing of the plan's kernels (off unless enabled).
+struct ProfRec {
+  int kind;
+  hipEvent_t a, b;
+  double bytes;
+};
+struct Profiler {
+  bool on = false;
+  std::vector<std::string> names;
+  std::vector<ProfRec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  // resolved totals per kernel name
+  std::vector<long> launches;
+  std::vector<double> total_ms, total_bytes;
+};
+
 }  // namespace spx
 
 // Opaque plan (FCPLAN + fftp plans + module globals of kes/grid, fcgram_mod.f90:34-38,
@@ -86,6 +103,7 @@ struct spx_plan {
   double* part = nullptr;   // per-line partial sums of the diagnostics
   std::vector<double> h_part;
   std::vector<void*> owned;
+  spx::Profiler prof;
 };
 
 namespace spx {
@@ -101,6 +119,14 @@ int fail(int code, const std::string& msg);
                                           hipGetErrorString(e_));           \
   } while (0)
 #define SPX_CHECK_LAUNCH() SPX_HIP(hipGetLastError())
+
+// RAII timing scope around one launch; records nothing when profiling is off.
+struct ProfScope {
+  spx_plan* p;
+  int rec = -1;
+  ProfScope(spx_plan* plan, const char* name, double bytes);
+  ~ProfScope();
+};
 
 // FFT launchers (spx_fft.hip) ------------------------------------------------
 int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines);

@@ -748,6 +748,7 @@ int zfft_inplace(spx_plan* p, void* S, int forward, int cont, double scale) {
   const long nlines = long(p->g.ny) * p->nkx;
   const unsigned nb = unsigned((nlines + f.nl - 1) / f.nl);
   const int C = p->g.Cz, d = p->g.oz;
+  ProfScope ps_(p, "z_inplace", double(p->nc) * (16.0 + 16.0));
   hipLaunchKernelGGL(k_z_inplace, dim3(nb), blk(f), f.lds_bytes(p->csize), p->stream,
                      static_cast<Cp*>(S), nlines, f.dev<Cp>(), p->np, forward,
                      (cont && C > 0) ? 1 : 0, scale, p->d_dir, C, d);
@@ -760,6 +761,7 @@ int fc_continue_only(spx_plan* p, void* S) {
   const int C = p->g.Cz;
   if (C <= 0) return SPX_OK;
   const long tot = nlines * C;
+  ProfScope ps_(p, "fc_continue", double(nlines) * 16.0 * (2 * p->g.oz + C));
   hipLaunchKernelGGL(k_fc_continue, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, p->stream,
                      static_cast<Cp*>(S), nlines, p->g.nz, p->np, C, p->g.oz, p->d_dir);
   SPX_CHECK_LAUNCH();
@@ -769,6 +771,7 @@ int fc_continue_only(spx_plan* p, void* S) {
 int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft) {
   const LineFFTHost& f = p->fz;
   dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(p->g.ny));
+  ProfScope ps_(p, "z_x2s", double(p->nkx) * p->g.ny * 16.0 * ((nofft ? p->g.nz : p->np) + p->g.nz));
   hipLaunchKernelGGL(k_z_x2s, grid, blk(f), f.lds_bytes(p->csize), p->stream,
                      static_cast<const Cp*>(X), static_cast<Cp*>(S), f.dev<Cp>(), p->nkx,
                      p->g.ny, p->np, nofft, p->d_dir, p->g.Cz, p->g.oz);
@@ -779,6 +782,7 @@ int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft) {
 int zfft_s2x(spx_plan* p, const void* S, void* X, int dirn, int nzout, int nofft) {
   const LineFFTHost& f = p->fz;
   dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(p->g.ny));
+  ProfScope ps_(p, "z_s2x", double(p->nkx) * p->g.ny * 16.0 * (p->g.nz + nzout));
   hipLaunchKernelGGL(k_z_s2x, grid, blk(f), f.lds_bytes(p->csize), p->stream,
                      static_cast<const Cp*>(S), static_cast<Cp*>(X), f.dev<Cp>(), p->nkx,
                      p->g.ny, nzout, dirn, nofft, p->d_kx, p->d_ky, p->d_kz);
@@ -789,6 +793,7 @@ int zfft_s2x(spx_plan* p, const void* S, void* X, int dirn, int nzout, int nofft
 int yfft(spx_plan* p, void* X, int forward, int nplanes) {
   const LineFFTHost& f = p->fy;
   dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(nplanes));
+  ProfScope ps_(p, "y", double(p->nkx) * p->g.ny * nplanes * 32.0);
   hipLaunchKernelGGL(k_y, grid, blk(f), f.lds_bytes(p->csize), p->stream, static_cast<Cp*>(X),
                      f.dev<Cp>(), p->nkx, p->g.ny, forward);
   SPX_CHECK_LAUNCH();
@@ -799,6 +804,7 @@ int xr2c(spx_plan* p, const void* R, void* X, int nplanes) {
   const LineFFTHost& f = p->fx;
   const long npairs = long(p->g.ny) * nplanes / 2;
   dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
+  ProfScope ps_(p, "xr2c", double(p->g.ny) * nplanes * (8.0 * p->g.nx + 16.0 * p->nkx));
   hipLaunchKernelGGL(k_xr2c, grid, blk(f), f.lds_bytes(p->csize), p->stream,
                      static_cast<const Tp*>(R), static_cast<Cp*>(X), f.dev<Cp>(), p->nkx, npairs);
   SPX_CHECK_LAUNCH();
@@ -809,6 +815,7 @@ int xc2r(spx_plan* p, const void* X, void* R, int nplanes) {
   const LineFFTHost& f = p->fx;
   const long npairs = long(p->g.ny) * nplanes / 2;
   dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
+  ProfScope ps_(p, "xc2r", double(p->g.ny) * nplanes * (8.0 * p->g.nx + 16.0 * p->nkx));
   hipLaunchKernelGGL(k_xc2r, grid, blk(f), f.lds_bytes(p->csize), p->stream,
                      static_cast<const Cp*>(X), static_cast<Tp*>(R), f.dev<Cp>(), p->nkx, npairs);
   SPX_CHECK_LAUNCH();
@@ -828,6 +835,7 @@ int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
   wp.nxny = double(p->g.nx * p->g.ny);
   wp.scalar = 0;
   dim3 grid(unsigned((nlines + f.nl - 1) / f.nl), 2);
+  ProfScope ps_(p, "noslip", 4.0 * double(p->nc) * 16.0);
   hipLaunchKernelGGL(k_noslip, grid, blk(f), f.lds_bytes(p->csize), p->stream,
                      static_cast<Cp*>(vx), static_cast<Cp*>(vy), static_cast<const Cp*>(pr),
                      f.dev<Cp>(), nlines, p->g.ny, p->np, 1.0 / double(p->g.nz), wp, p->d_kx,
@@ -843,6 +851,7 @@ int sconst_fused(spx_plan* p, void* th) {
   std::memset(&wp, 0, sizeof(wp));
   wp.scalar = 1;
   dim3 grid(unsigned((nlines + f.nl - 1) / f.nl), 1);
+  ProfScope ps_(p, "sconst", 2.0 * double(p->nc) * 16.0);
   hipLaunchKernelGGL(k_noslip, grid, blk(f), f.lds_bytes(p->csize), p->stream,
                      static_cast<Cp*>(th), static_cast<Cp*>(th), static_cast<const Cp*>(nullptr),
                      f.dev<Cp>(), nlines, p->g.ny, p->np, 1.0 / double(p->g.nz), wp, p->d_kx,
@@ -854,6 +863,7 @@ int sconst_fused(spx_plan* p, void* th) {
 int proj_bc(spx_plan* p, const void* c) {
   const LineFFTHost& f = p->fz;
   const long nlines = long(p->g.ny) * p->nkx;
+  ProfScope ps_(p, "proj_bc", double(p->nc) * 16.0);
   hipLaunchKernelGGL(k_proj_bc, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
                      f.lds_bytes(p->csize), p->stream, static_cast<const Cp*>(c),
                      static_cast<Cp*>(p->bc), f.dev<Cp>(), nlines, p->np,
@@ -865,6 +875,7 @@ int proj_bc(spx_plan* p, const void* c) {
 int proj_pressure(spx_plan* p, void* d) {
   const LineFFTHost& f = p->fz;
   const long nlines = long(p->g.ny) * p->nkx;
+  ProfScope ps_(p, "proj_pressure", 2.0 * double(p->nc) * 16.0);
   hipLaunchKernelGGL(k_proj_pressure, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
                      f.lds_bytes(p->csize), p->stream, static_cast<Cp*>(d),
                      static_cast<const Cp*>(p->bc), f.dev<Cp>(), nlines, p->g.ny,
@@ -877,6 +888,7 @@ int proj_harmonic(spx_plan* p, void* a, void* b, void* c) {
   const LineFFTHost& f = p->fz2;
   const long nlines = long(p->g.ny) * p->nkx;
   if (int rc = lds_limit(k_proj_harmonic, 2 * f.lds_bytes(p->csize))) return rc;
+  ProfScope ps_(p, "proj_harmonic", 6.0 * double(p->nc) * 16.0);
   hipLaunchKernelGGL(k_proj_harmonic, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
                      2 * f.lds_bytes(p->csize), p->stream, static_cast<Cp*>(a),
                      static_cast<Cp*>(b), static_cast<Cp*>(c), static_cast<const Cp*>(p->bc),
@@ -891,6 +903,7 @@ static int zline_launch(spx_plan* p, const LinCombDev& lc, const void* b, int mo
   const long nlines = long(p->g.ny) * p->nkx;
   size_t lds = f.lds_bytes(p->csize) * (mode == 1 ? 2 : 1) + size_t(f.nl) * f.tpl * sizeof(double);
   if (int rc = lds_limit(k_zline_reduce, lds)) return rc;
+  ProfScope ps_(p, "zline_reduce", double(p->nc) * 16.0 * (mode == 1 ? 2 : 1));
   hipLaunchKernelGGL(k_zline_reduce, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f), lds,
                      p->stream, lc, static_cast<const Cp*>(b), mode, out, f.dev<Cp>(), nlines,
                      p->g.ny, p->np, p->d_kx, p->d_ky, p->d_kz);

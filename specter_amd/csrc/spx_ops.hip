@@ -193,6 +193,7 @@ static inline unsigned nblk(long n) { return unsigned((n + 255) / 256); }
 static inline SIdx sidx(spx_plan* p) { return SIdx{p->g.nz, p->g.ny}; }
 
 int op_derivk(spx_plan* p, const void* a, void* b, int dirn) {
+  ProfScope ps_(p, "derivk", 2.0 * double(p->nc) * 16.0);
   hipLaunchKernelGGL(k_derivk, dim3(nblk(p->nc)), dim3(256), 0, p->stream,
                      static_cast<const Cp*>(a), static_cast<Cp*>(b), long(p->nc), sidx(p), dirn,
                      p->d_kx, p->d_ky, p->d_kz);
@@ -201,6 +202,7 @@ int op_derivk(spx_plan* p, const void* a, void* b, int dirn) {
 }
 
 int op_laplak(spx_plan* p, const void* a, void* b) {
+  ProfScope ps_(p, "laplak", 2.0 * double(p->nc) * 16.0);
   hipLaunchKernelGGL(k_laplak, dim3(nblk(p->nc)), dim3(256), 0, p->stream,
                      static_cast<const Cp*>(a), static_cast<Cp*>(b), long(p->nc), sidx(p),
                      p->d_kx, p->d_ky, p->d_kz);
@@ -209,6 +211,7 @@ int op_laplak(spx_plan* p, const void* a, void* b) {
 }
 
 int op_filter(spx_plan* p, void* a) {
+  ProfScope ps_(p, "filter", 2.0 * double(p->nc) * 16.0);
   hipLaunchKernelGGL(k_filter, dim3(nblk(p->nc)), dim3(256), 0, p->stream, static_cast<Cp*>(a),
                      long(p->nc), sidx(p), p->d_fkx, p->d_fky, p->d_fkz);
   SPX_CHECK_LAUNCH();
@@ -216,6 +219,7 @@ int op_filter(spx_plan* p, void* a) {
 }
 
 int op_scale(spx_plan* p, void* a, size_t n, double s) {
+  ProfScope ps_(p, "scale", 16.0 * double(n));
   hipLaunchKernelGGL(k_scale, dim3(nblk(long(n))), dim3(256), 0, p->stream,
                      static_cast<double*>(a), long(n), s);
   SPX_CHECK_LAUNCH();
@@ -224,6 +228,7 @@ int op_scale(spx_plan* p, void* a, size_t n, double s) {
 
 int op_real_div3(spx_plan* p, void* r, double a, double b, double c) {
   const long n = long(p->g.nx) * p->g.ny * p->np;
+  ProfScope ps_(p, "real_div3", 16.0 * double(n));
   hipLaunchKernelGGL(k_div3, dim3(nblk(n)), dim3(256), 0, p->stream, static_cast<double*>(r), n,
                      a, b, c);
   SPX_CHECK_LAUNCH();
@@ -235,6 +240,7 @@ int op_prod_stage(spx_plan* p, int stage, const void* r1, const void* r2, const 
   const long n = long(p->g.nx) * p->g.ny * p->np;
   const double t = double(p->g.nx) * double(p->g.ny) * double(p->g.nz);
   const double tmp = 1.0 / (t * t);
+  ProfScope ps_(p, "prod_stage", 8.0 * double(n) * (nout == 3 ? (stage == 1 ? 7 : 10) : (stage == 1 ? 3 : 4)));
   hipLaunchKernelGGL(k_prod_stage, dim3(nblk(n)), dim3(256), 0, p->stream, stage,
                      static_cast<const double*>(r1), static_cast<const double*>(r2),
                      static_cast<const double*>(r3), static_cast<const double*>(r4),
@@ -262,6 +268,7 @@ int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, 
       r = RkComp{nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, 0.0};
     }
   }
+  ProfScope ps_(p, "rk_update", double(p->nc) * 16.0 * 5 * ncomp + (src ? double(p->nc) * 16.0 * ncomp : 0.0));
   hipLaunchKernelGGL(k_rk_update, dim3(nblk(p->nc)), dim3(256), 0, p->stream, args, ncomp,
                      long(p->nc), sidx(p), p->d_kx, p->d_ky, p->d_kz, p->d_fkx, p->d_fky,
                      p->d_fkz, dt, rmp);
@@ -270,6 +277,7 @@ int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, 
 }
 
 int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d) {
+  ProfScope ps_(p, "poisson_project", 7.0 * double(p->nc) * 16.0);
   hipLaunchKernelGGL(k_poisson_project, dim3(nblk(p->nc)), dim3(256), 0, p->stream,
                      static_cast<Cp*>(a), static_cast<Cp*>(b), static_cast<Cp*>(c),
                      static_cast<Cp*>(d), long(p->nc), sidx(p), p->d_kx, p->d_ky, p->d_kz);
@@ -278,6 +286,7 @@ int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d) {
 }
 
 int op_axpy_field(spx_plan* p, void* y, const void* x, double alpha) {
+  ProfScope ps_(p, "axpy", 3.0 * double(p->nc) * 16.0);
   hipLaunchKernelGGL(k_axpy, dim3(nblk(p->nc)), dim3(256), 0, p->stream, static_cast<Cp*>(y),
                      static_cast<const Cp*>(x), long(p->nc), alpha);
   SPX_CHECK_LAUNCH();
