@@ -147,6 +147,8 @@ def main():
         "field_rel_l2_vs_fp64": 0.0,
         "data": "synthetic (seeded spectrum-peaked solenoidal IC x sin(pi z/Lz))",
         "config": {"workload": w["label"], "grid": [w["nx"], w["ny"], w["nzp"] + w["C"]],
+                   "z_dft": {0: "direct", 1: "bluestein", 2: "rader"}[plan.z_algorithm]
+                            + f" (in-LDS FFT length {plan.z_fft_len})",
                    "physical_points": npts, "total_points": ntot,
                    "total_Mpts_per_s": round(ntot / (ms * 1e-3) / 1e6, 3),
                    "parallelism": "single GPU"},

@@ -61,7 +61,7 @@ typedef struct {
   size_t bytes_spectral;
   size_t bytes_real;
   int z_fft_len;           /* length of the z FFT run in LDS (Bluestein size for awkward nz) */
-  int z_bluestein;         /* 1 if the z DFT uses Bluestein's algorithm                      */
+  int z_algorithm;         /* z DFT algorithm: 0 direct, 1 Bluestein, 2 Rader               */
   size_t device_bytes;     /* device memory held by the plan (tables + scratch)              */
 } spx_plan_info;
 

@@ -66,11 +66,22 @@ static int upload(spx_plan* p, double** dst, const V& host) {
   return SPX_OK;
 }
 
-static int ensure_work(spx_plan* p) {
-  if (p->R[0]) return SPX_OK;
-  for (int i = 0; i < 7; ++i) {
-    int rc = alloc_dev(p, &p->R[i], p->nr * sizeof(double));
-    if (rc) return rc;
+static int ensure_work(spx_plan* p, bool bouss) {
+  int rc;
+  if (!p->XV[0]) {
+    for (int c = 0; c < 3; ++c) {
+      if ((rc = alloc_dev(p, &p->XV[c], p->nc * 16)) || (rc = alloc_dev(p, &p->XZ[c], p->nc * 16)) ||
+          (rc = alloc_dev(p, &p->XY[c], p->nc * 16)) || (rc = alloc_dev(p, &p->RV[c], p->nr * 8)) ||
+          (rc = alloc_dev(p, &p->RO[c], p->nr * 8)))
+        return rc;
+    }
+    if ((rc = alloc_dev(p, &p->R0, p->nr * 8))) return rc;
+  }
+  if (bouss && !p->work_bouss) {
+    if ((rc = alloc_dev(p, &p->XV[3], p->nc * 16)) || (rc = alloc_dev(p, &p->XZ[3], p->nc * 16)) ||
+        (rc = alloc_dev(p, &p->XY[3], p->nc * 16)) || (rc = alloc_dev(p, &p->RO[3], p->nr * 8)))
+      return rc;
+    p->work_bouss = true;
   }
   return SPX_OK;
 }
@@ -261,9 +272,10 @@ int spx_plan_get_info(const spx_plan* p, spx_plan_info* info) {
   info->bytes_spectral = p->nc * 16;
   info->bytes_real = p->nr * 8;
   info->z_fft_len = p->fz.M;
-  info->z_bluestein = p->fz.blue;
+  info->z_algorithm = p->fz.mode;
   size_t b = p->nc * 16 + size_t(p->g.ny) * p->nkx * 48;
-  if (p->R[0]) b += 7 * p->nr * 8;
+  if (p->XV[0]) b += 9 * p->nc * 16 + 7 * p->nr * 8;
+  if (p->work_bouss) b += 3 * p->nc * 16 + p->nr * 8;
   info->device_bytes = b;
   return SPX_OK;
 }
@@ -333,40 +345,78 @@ int spx_fc_filter(spx_plan* p, void* a) {
   return op_filter(p, a);
 }
 
+}  // extern "C"
+
+namespace spx {
+
+// (v.grad) v_c for c = x,y,z in advective form (pseudospec_hd.f90:209-319).
+// The x and y derivatives commute with the z transform (kx, ky are constant
+// along a z line), so each component needs two z IFFTs (v and d_z v) instead
+// of four; the y pass yields v and d_y v from one load; the x pass forms the
+// three derivatives and accumulates the products in the reference's order.
+static int gradre_impl(spx_plan* p, const void* const v[3], void* const out[3]) {
+  int rc;
+  const int np = p->np;
+  for (int c = 0; c < 3; ++c)
+    if ((rc = zfft_s2x(p, v[c], p->XV[c], 0, np, 0, p->XZ[c]))) return rc;
+  for (int c = 0; c < 3; ++c) {
+    if ((rc = yfft(p, p->XV[c], 0, np, nullptr, p->XY[c]))) return rc;
+    if ((rc = yfft(p, p->XZ[c], 0, np))) return rc;
+  }
+  for (int c = 0; c < 3; ++c)
+    if ((rc = xc2r(p, p->XV[c], p->RV[c], np, 0))) return rc;
+  for (int c = 0; c < 3; ++c)
+    if ((rc = xgrad_prod(p, p->XV[c], p->XY[c], p->XZ[c], p->RV[0], p->RV[1], p->RV[2],
+                         p->RO[c])))
+      return rc;
+  for (int c = 0; c < 3; ++c)
+    if ((rc = r2c3(p, p->RO[c], out[c]))) return rc;
+  return SPX_OK;
+}
+
+// (v.grad) th (pseudospec_phd.f90:23-113); have_rv: RV already holds v in
+// real space (computed by gradre_impl on the same v).
+static int advect_impl(spx_plan* p, const void* const v[3], const void* th, void* out,
+                       bool have_rv) {
+  int rc;
+  const int np = p->np;
+  if (!have_rv) {
+    for (int c = 0; c < 3; ++c) {
+      if ((rc = zfft_s2x(p, v[c], p->XV[c], 0, np, 0))) return rc;
+      if ((rc = yfft(p, p->XV[c], 0, np))) return rc;
+      if ((rc = xc2r(p, p->XV[c], p->RV[c], np, 0))) return rc;
+    }
+  }
+  if ((rc = zfft_s2x(p, th, p->XV[3], 0, np, 0, p->XZ[3]))) return rc;
+  if ((rc = yfft(p, p->XV[3], 0, np, nullptr, p->XY[3]))) return rc;
+  if ((rc = yfft(p, p->XZ[3], 0, np))) return rc;
+  if ((rc = xgrad_prod(p, p->XV[3], p->XY[3], p->XZ[3], p->RV[0], p->RV[1], p->RV[2],
+                       p->RO[3])))
+    return rc;
+  return r2c3(p, p->RO[3], out);
+}
+
+}  // namespace spx
+
+extern "C" {
+
 int spx_gradre(spx_plan* p, const void* a, const void* b, const void* c, void* d, void* e,
                void* f) {
   SPX_REQ(p && a && b && c && d && e && f, "spx_gradre: null argument");
   int rc;
-  if ((rc = ensure_work(p))) return rc;
+  if ((rc = ensure_work(p, false))) return rc;
   const void* v[3] = {a, b, c};
-  for (int s = 1; s <= 3; ++s) {
-    if ((rc = c2r3(p, v[s - 1], p->R[0], 0, p->np))) return rc;
-    if ((rc = c2r3(p, a, p->R[1], s, p->np))) return rc;
-    if ((rc = c2r3(p, b, p->R[2], s, p->np))) return rc;
-    if ((rc = c2r3(p, c, p->R[3], s, p->np))) return rc;
-    if ((rc = op_prod_stage(p, s, p->R[0], p->R[1], p->R[2], p->R[3], p->R[4], p->R[5],
-                            p->R[6], 3)))
-      return rc;
-  }
-  if ((rc = r2c3(p, p->R[4], d))) return rc;
-  if ((rc = r2c3(p, p->R[5], e))) return rc;
-  return r2c3(p, p->R[6], f);
+  void* o[3] = {d, e, f};
+  return gradre_impl(p, v, o);
 }
 
 int spx_advect(spx_plan* p, const void* a, const void* b, const void* c, const void* dd,
                void* e) {
   SPX_REQ(p && a && b && c && dd && e, "spx_advect: null argument");
   int rc;
-  if ((rc = ensure_work(p))) return rc;
+  if ((rc = ensure_work(p, true))) return rc;
   const void* v[3] = {a, b, c};
-  for (int s = 1; s <= 3; ++s) {
-    if ((rc = c2r3(p, v[s - 1], p->R[0], 0, p->np))) return rc;
-    if ((rc = c2r3(p, dd, p->R[1], s, p->np))) return rc;
-    if ((rc = op_prod_stage(p, s, p->R[0], p->R[1], nullptr, nullptr, p->R[4], nullptr, nullptr,
-                            1)))
-      return rc;
-  }
-  return r2c3(p, p->R[4], e);
+  return advect_impl(p, v, dd, e, false);
 }
 
 // ---- boundary conditions --------------------------------------------------------
@@ -507,7 +557,7 @@ int spx_state_create(spx_plan* p, const spx_params* prm, spx_state** out) {
   SPX_REQ(prm->ord >= 1, "spx_state_create: ord must be >= 1");
   if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "the solvers need walls in z (Cz > 0)");
   *out = nullptr;
-  int rc = ensure_work(p);
+  int rc = ensure_work(p, prm->solver == SPX_SOLVER_BOUSS);
   if (rc) return rc;
   spx_state* s = new spx_state();
   s->p = p;
@@ -583,8 +633,10 @@ int spx_substep(spx_plan* p, spx_state* s, int o) {
   const bool bouss = q.solver == SPX_SOLVER_BOUSS;
   const double rmp = 1.0 / double(o);
   int rc;
-  if ((rc = spx_gradre(p, s->vx, s->vy, s->vz, s->N[0], s->N[1], s->N[2]))) return rc;
-  if (bouss && (rc = spx_advect(p, s->vx, s->vy, s->vz, s->th, s->N[3]))) return rc;
+  const void* vel[3] = {s->vx, s->vy, s->vz};
+  void* nlo[3] = {s->N[0], s->N[1], s->N[2]};
+  if ((rc = gradre_impl(p, vel, nlo))) return rc;
+  if (bouss && (rc = advect_impl(p, vel, s->th, s->N[3], true))) return rc;
   void* v[4] = {s->vx, s->vy, s->vz, s->th};
   const void* c0[4] = {s->C1, s->C2, s->C3, s->C7};
   void* nl[4] = {s->N[0], s->N[1], s->N[2], s->N[3]};
@@ -602,10 +654,10 @@ int spx_substep(spx_plan* p, spx_state* s, int o) {
     if ((rc = spx_s_imposebc(p, s->th))) return rc;
     if ((rc = op_filter(p, s->th))) return rc;
     // round trip of th through real space (include/bouss/bouss_rkstep2.f90:56-59)
-    if ((rc = c2r3(p, s->th, p->R[0], 0, p->np))) return rc;
-    if ((rc = op_real_div3(p, p->R[0], double(p->g.nx), double(p->g.ny), double(p->g.nz))))
+    if ((rc = c2r3(p, s->th, p->R0, 0, p->np))) return rc;
+    if ((rc = op_real_div3(p, p->R0, double(p->g.nx), double(p->g.ny), double(p->g.nz))))
       return rc;
-    if ((rc = r2c3(p, p->R[0], s->th))) return rc;
+    if ((rc = r2c3(p, p->R0, s->th))) return rc;
   }
   return SPX_OK;
 }

@@ -21,39 +21,59 @@ template <typename T> struct cvec;
 template <> struct cvec<double> { using type = double2; };
 template <> struct cvec<float> { using type = float2; };
 
-// Device-side description of one batched line DFT of length n, computed either
-// directly (n smooth) or by Bluestein's chirp-z with an M-point smooth FFT.
+// Device-side description of one batched line DFT of length n, computed
+// directly (n smooth), by Rader's algorithm (n prime, n-1 smooth) or by
+// Bluestein's chirp-z (otherwise) with an M-point smooth FFT in LDS.
 constexpr int kMaxStages = 16;
 template <typename C>
 struct LineFFT {
   int n;      // logical DFT length
-  int M;      // length of the in-LDS FFT actually run (n, or the Bluestein size)
-  int blue;   // 1 = Bluestein
+  int M;      // length of the in-LDS FFT actually run
+  int mode;   // 0 direct, 1 Bluestein, 2 Rader
   int nstage;
   int radix[kMaxStages];
-  int tpl;    // threads per line
-  int nl;     // lines per workgroup
-  int ldl;    // LDS stride between lines (complex elements)
+  unsigned magic[kMaxStages];  // ceil(2^32 / Ns) of each stage (Ns = product of earlier radices)
+  int tpl, lgtpl;  // threads per line (power of two) and its log2
+  int nl, lgnl;    // lines per workgroup (power of two) and its log2
+  int ldl;         // LDS stride between lines (complex elements)
   const C* tw;     // W_M[m] = exp(-2 pi i m / M), m < M
-  const C* chirp;  // w[m] = exp(-pi i m^2 / n), m < n        (Bluestein)
-  const C* bfil;   // FFT_M(conj chirp, wrapped) / M, M entries (Bluestein)
+  const C* chirp;  // Bluestein: w[m] = exp(-pi i m^2 / n), m < n
+  const C* bfil;   // Bluestein: FFT_M(wrapped conj chirp)/M ; Rader: FFT_{n-1}(b)/(n-1)
+  const int* perm; // Rader: g^q mod n (n-1 entries) then g^-p mod n (n-1 entries)
 };
 
 // Host-side owner of the tables behind a LineFFT.
 struct LineFFTHost {
-  int n = 0, M = 0, blue = 0, nstage = 0, tpl = 0, nl = 0, ldl = 0;
+  int n = 0, M = 0, mode = 0, nstage = 0, tpl = 0, lgtpl = 0, nl = 0, lgnl = 0, ldl = 0;
   int radix[kMaxStages] = {0};
+  unsigned magic[kMaxStages] = {0};
   void* tw = nullptr;
   void* chirp = nullptr;
   void* bfil = nullptr;
-  size_t lds_bytes(size_t csize) const { return size_t(nl) * size_t(ldl) * csize; }
+  void* perm = nullptr;
+  // LDS bytes of a launch: twiddles + nbuf line buffers
+  size_t lds_bytes(size_t csize, int nbuf = 1) const {
+    return (size_t(M) + size_t(nbuf) * size_t(nl) * size_t(ldl)) * csize;
+  }
+  LineFFTHost with_lines(int lines) const {
+    LineFFTHost h = *this;
+    h.nl = lines < 1 ? 1 : lines;
+    h.lgnl = 0;
+    while ((1 << h.lgnl) < h.nl) ++h.lgnl;
+    return h;
+  }
   template <typename C> LineFFT<C> dev() const {
     LineFFT<C> f;
-    f.n = n; f.M = M; f.blue = blue; f.nstage = nstage; f.tpl = tpl; f.nl = nl; f.ldl = ldl;
-    for (int i = 0; i < kMaxStages; ++i) f.radix[i] = radix[i];
+    f.n = n; f.M = M; f.mode = mode; f.nstage = nstage;
+    f.tpl = tpl; f.lgtpl = lgtpl; f.nl = nl; f.lgnl = lgnl; f.ldl = ldl;
+    for (int i = 0; i < kMaxStages; ++i) {
+      f.radix[i] = radix[i];
+      f.magic[i] = magic[i];
+    }
     f.tw = static_cast<const C*>(tw);
     f.chirp = static_cast<const C*>(chirp);
     f.bfil = static_cast<const C*>(bfil);
+    f.perm = static_cast<const int*>(perm);
     return f;
   }
 };
@@ -96,9 +116,18 @@ struct spx_plan {
   double Dkx = 0, Dky = 0, Dkz = 0, dz = 0;
   spx::LineFFTHost fx, fy, fz;
   spx::LineFFTHost fz2;     // fz with half the lines per workgroup (two-buffer kernels)
+  bool work_bouss = false;
   // scratch
   void* X = nullptr;        // xy-stage buffer, nc complex
-  void* R[7] = {nullptr};   // real work arrays for gradre/advect (nr reals each)
+  // gradre / advect work arrays (allocated on first use):
+  //   XV[c]: z+y inverse transform of v_c, XZ[c]: of d_z v_c, XY[c]: y-derivative stage
+  //   RV[c]: v_c in real space, RO[0..3]: nonlinear products (x,y,z, scalar)
+  void* XV[4] = {nullptr};
+  void* XZ[4] = {nullptr};
+  void* XY[4] = {nullptr};
+  void* RV[3] = {nullptr};
+  void* RO[4] = {nullptr};
+  void* R0 = nullptr;       // general real scratch (Boussinesq round trip)
   void* bc = nullptr;       // wall data of sol_project, 2 complex per (kx,ky) line
   double* part = nullptr;   // per-line partial sums of the diagnostics
   std::vector<double> h_part;
@@ -142,10 +171,20 @@ int fc_continue_only(spx_plan* p, void* S);
 // nofft = 1: transpose only (all nz planes), for fftp2d_real_to_complex_xy.
 int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft);
 // backward 3D first pass: S * (i k_dir) (dir 0 = none) -> z IFFT -> X, planes < nzout.
-int zfft_s2x(spx_plan* p, const void* S, void* X, int dir, int nzout, int nofft);
-int yfft(spx_plan* p, void* X, int forward, int nplanes);
+// X2 != nullptr: also X2 <- z IFFT of (i kz) S from the same load (dual).
+int zfft_s2x(spx_plan* p, const void* S, void* X, int dir, int nzout, int nofft,
+             void* X2 = nullptr);
+// y transform of X planes < nplanes: Xout <- y FFT(Xin) (in place when equal);
+// Y2 != nullptr: also Y2 <- y FFT of (i ky) Xin (dual, backward only).
+int yfft(spx_plan* p, void* X, int forward, int nplanes, const void* Xin = nullptr,
+         void* Y2 = nullptr);
 int xr2c(spx_plan* p, const void* R, void* X, int nplanes);
-int xc2r(spx_plan* p, const void* X, void* R, int nplanes);
+// x half complex -> real of planes < nplanes; dirx = 1 multiplies by i kx first.
+int xc2r(spx_plan* p, const void* X, void* R, int nplanes, int dirx = 0);
+// gradient products of one component: out = ((u*Dx + v*Dy) + w*Dz)*tmp with
+// Dx = c2r(i kx YV), Dy = c2r(YY), Dz = c2r(XZ) (pseudospec_hd.f90:254-312 order)
+int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, const void* u,
+               const void* v, const void* w, void* out);
 // fused v_imposebc: goto_domain_w_boundaries + noslip_z + goto_3d_fourier on vx,vy
 int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
                  const double vzsta[2], const double vzend[2]);
@@ -171,8 +210,6 @@ int op_derivk(spx_plan* p, const void* a, void* b, int dir);
 int op_laplak(spx_plan* p, const void* a, void* b);
 int op_filter(spx_plan* p, void* a);
 int op_scale(spx_plan* p, void* a, size_t n, double s);
-int op_prod_stage(spx_plan* p, int stage, const void* r1, const void* r2, const void* r3,
-                  const void* r4, void* rx, void* ry, void* rz, int nout);
 int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, void* const* nl,
                  const void* const* f, const void* const* src, const double* src_coef,
                  const double* nu, double dt, double rmp);

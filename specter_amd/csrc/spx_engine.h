@@ -1,12 +1,15 @@
 // In-LDS batched line DFT engine for gfx950 (CDNA4, wave64).
 //
-// A workgroup owns nl lines of one length; tpl threads work on each line.  A
-// mixed-radix Stockham autosort FFT (radices 2,3,4,5,7,8,11,13) runs in LDS
-// with register staging: every pass reads R strided points per butterfly,
-// applies the W_M twiddles of a single M-entry table, does the radix-R DFT in
-// registers, synchronises, and writes the autosorted result back.  Lengths
-// with a prime factor above 13 use Bluestein's chirp-z algorithm with a smooth
-// M >= 2n-1, still entirely in LDS (one HBM read and one write per line).
+// A workgroup owns nl lines of one length n; tpl threads work on each line.
+// A mixed-radix Stockham autosort FFT (radices 2,3,4,5,7,8,9,11,13) runs in
+// LDS with register staging: every pass reads R strided points per butterfly,
+// applies twiddles from one M-entry table staged in LDS, does the radix-R DFT
+// in registers, synchronises, and writes the autosorted result back.
+//
+// Lengths with a prime factor above 13 are reduced to a smooth FFT inside
+// LDS (one HBM read and one write per line either way):
+//   * Rader (n prime, n-1 smooth): cyclic convolution of length n-1;
+//   * Bluestein (otherwise): chirp-z convolution with a smooth M >= 2n-1.
 // Backward transforms use conj(DFT(conj(x))).
 #pragma once
 
@@ -42,10 +45,10 @@ __device__ __forceinline__ cpx<T> cmul_mi(cpx<T> a) { return mkc<T>(a.y, -a.x); 
 template <typename T>
 __device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return mkc<T>(a.x * s, a.y * s); }
 
-// Radix-R DFT in registers, forward sign.  tw is the W_M table, M/R its stride.
+// Radix-R DFT in registers, forward sign.  tw is the W_M table, s = M/R.
 template <typename T, int R>
 struct Dft {
-  static __device__ __forceinline__ void run(cpx<T>* v, const cpx<T>* __restrict__ tw, int s) {
+  static __device__ __forceinline__ void run(cpx<T>* v, const cpx<T>* tw, int s) {
     // odd R: pair r with R-r
     constexpr int H = (R - 1) / 2;
     cpx<T> a[H + 1], b[H + 1];
@@ -87,6 +90,20 @@ struct Dft<T, 2> {
 };
 
 template <typename T>
+struct Dft<T, 3> {
+  static __device__ __forceinline__ void run(cpx<T>* v, const cpx<T>*, int) {
+    const T c = T(-0.5), s = T(0.86602540378443864676372317075293618347L);
+    const cpx<T> t1 = cadd(v[1], v[2]);
+    const cpx<T> t2 = mkc<T>(v[0].x + c * t1.x, v[0].y + c * t1.y);
+    const cpx<T> d = csub(v[1], v[2]);
+    const cpx<T> t3 = mkc<T>(s * d.y, -s * d.x);  // -i s d
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(t2, t3);
+    v[2] = csub(t2, t3);
+  }
+};
+
+template <typename T>
 struct Dft<T, 4> {
   static __device__ __forceinline__ void run(cpx<T>* v, const cpx<T>*, int) {
     cpx<T> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
@@ -121,15 +138,51 @@ struct Dft<T, 8> {
   }
 };
 
-// Butterflies a thread may stage in registers per pass (<= 16 points).
-template <int R>
-struct MaxB {
-  static constexpr int value = (16 / R) > 0 ? (16 / R) : 1;
+// radix 9 as 3 x 3 with the inner twiddles W_9^{ab}
+template <typename T>
+struct Dft<T, 9> {
+  static __device__ __forceinline__ void run(cpx<T>* v, const cpx<T>* tw, int s) {
+    cpx<T> a[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      cpx<T> t[3] = {v[r], v[r + 3], v[r + 6]};
+      Dft<T, 3>::run(t, nullptr, 0);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) a[q][r] = t[q];
+    }
+    // twiddle a[q][r] by W_9^{q r}
+    a[1][1] = cmul(a[1][1], tw[1 * s]);
+    a[1][2] = cmul(a[1][2], tw[2 * s]);
+    a[2][1] = cmul(a[2][1], tw[2 * s]);
+    a[2][2] = cmul(a[2][2], tw[4 * s]);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      cpx<T> t[3] = {a[q][0], a[q][1], a[q][2]};
+      Dft<T, 3>::run(t, nullptr, 0);
+      v[q] = t[0];
+      v[q + 3] = t[1];
+      v[q + 6] = t[2];
+    }
+  }
 };
 
+// Points a thread stages in registers per pass; the plan sizes threads per
+// line so that no thread holds more (radix 9/11/13 passes hold one butterfly).
+constexpr int kStage = 8;
+template <int R>
+struct MaxB {
+  static constexpr int value = (kStage / R) > 0 ? (kStage / R) : 1;
+};
+
+// j mod Ns with the per-stage magic multiplier (j, Ns < 2^16)
+__device__ __forceinline__ int fastmod(int j, int Ns, unsigned magic) {
+  const int q = int(__umulhi(unsigned(j), magic));
+  return j - q * Ns;
+}
+
 template <typename T, int R>
-__device__ __forceinline__ void stockham_pass(cpx<T>* x, int M, int Ns, int lane, int tpl,
-                                              const cpx<T>* __restrict__ tw) {
+__device__ __forceinline__ void stockham_pass(cpx<T>* x, int M, int Ns, unsigned magic, int lane,
+                                              int tpl, const cpx<T>* tw) {
   constexpr int MB = MaxB<R>::value;
   const int nb = M / R;
   const int tstep = M / (Ns * R);
@@ -142,8 +195,7 @@ __device__ __forceinline__ void stockham_pass(cpx<T>* x, int M, int Ns, int lane
 #pragma unroll
       for (int r = 0; r < R; ++r) v[b][r] = x[j + r * nb];
       if (Ns > 1) {
-        const int jm = j % Ns;
-        const int st = jm * tstep;
+        const int st = fastmod(j, Ns, magic) * tstep;
 #pragma unroll
         for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * st]);
       }
@@ -155,7 +207,7 @@ __device__ __forceinline__ void stockham_pass(cpx<T>* x, int M, int Ns, int lane
   for (int b = 0; b < MB; ++b) {
     const int j = lane + b * tpl;
     if (j < nb) {
-      const int jm = j % Ns;
+      const int jm = Ns > 1 ? fastmod(j, Ns, magic) : 0;
       const int base = (j - jm) * R + jm;
 #pragma unroll
       for (int r = 0; r < R; ++r) x[base + r * Ns] = v[b][r];
@@ -164,45 +216,78 @@ __device__ __forceinline__ void stockham_pass(cpx<T>* x, int M, int Ns, int lane
   __syncthreads();
 }
 
-// Forward FFT of length f.M on the line x (LDS).  Every thread of the
-// workgroup must call it (it synchronises); x may differ per thread group.
-template <typename T>
-__device__ void fft_lds(cpx<T>* x, const LineFFT<cpx<T>>& f, int lane) {
+// Radix sets a kernel instance is compiled for (register pressure follows the
+// largest radix pass compiled in): powers of two; {2,3,4,5,7,8,9}; all.
+enum { RS_POW2 = 0, RS_SMOOTH = 1, RS_ALL = 2 };
+
+// Forward FFT of length f.M on the line x (LDS) with twiddles tw (LDS).
+// Every thread of the workgroup must call it (it synchronises).
+template <typename T, int RS>
+__device__ void fft_lds(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw, int lane) {
   int Ns = 1;
   for (int st = 0; st < f.nstage; ++st) {
     const int R = f.radix[st];
-    switch (R) {
-      case 2: stockham_pass<T, 2>(x, f.M, Ns, lane, f.tpl, f.tw); break;
-      case 3: stockham_pass<T, 3>(x, f.M, Ns, lane, f.tpl, f.tw); break;
-      case 4: stockham_pass<T, 4>(x, f.M, Ns, lane, f.tpl, f.tw); break;
-      case 5: stockham_pass<T, 5>(x, f.M, Ns, lane, f.tpl, f.tw); break;
-      case 7: stockham_pass<T, 7>(x, f.M, Ns, lane, f.tpl, f.tw); break;
-      case 8: stockham_pass<T, 8>(x, f.M, Ns, lane, f.tpl, f.tw); break;
-      case 11: stockham_pass<T, 11>(x, f.M, Ns, lane, f.tpl, f.tw); break;
-      case 13: stockham_pass<T, 13>(x, f.M, Ns, lane, f.tpl, f.tw); break;
-      default: break;
+    const unsigned mg = f.magic[st];
+    if (R == 8) stockham_pass<T, 8>(x, f.M, Ns, mg, lane, f.tpl, tw);
+    else if (R == 4) stockham_pass<T, 4>(x, f.M, Ns, mg, lane, f.tpl, tw);
+    else if (R == 2) stockham_pass<T, 2>(x, f.M, Ns, mg, lane, f.tpl, tw);
+    else if constexpr (RS >= RS_SMOOTH) {
+      if (R == 3) stockham_pass<T, 3>(x, f.M, Ns, mg, lane, f.tpl, tw);
+      else if (R == 5) stockham_pass<T, 5>(x, f.M, Ns, mg, lane, f.tpl, tw);
+      else if (R == 7) stockham_pass<T, 7>(x, f.M, Ns, mg, lane, f.tpl, tw);
+      else if (R == 9) stockham_pass<T, 9>(x, f.M, Ns, mg, lane, f.tpl, tw);
+      else if constexpr (RS >= RS_ALL) {
+        if (R == 11) stockham_pass<T, 11>(x, f.M, Ns, mg, lane, f.tpl, tw);
+        else if (R == 13) stockham_pass<T, 13>(x, f.M, Ns, mg, lane, f.tpl, tw);
+      }
     }
     Ns *= R;
   }
 }
 
-// Forward DFT of length f.n of the n values at x[0..n), result in x[0..n).
-// The caller has synchronised after filling x.
+// Copy the twiddle table into LDS (called by all threads; the caller's next
+// __syncthreads publishes it).
 template <typename T>
-__device__ void line_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, int lane) {
-  if (!f.blue) {
-    fft_lds(x, f, lane);
+__device__ __forceinline__ void stage_twiddles(cpx<T>* tws, const LineFFT<cpx<T>>& f) {
+  for (int m = threadIdx.x; m < f.M; m += blockDim.x) tws[m] = f.tw[m];
+}
+
+// Forward DFT of length f.n of the n values at x[0..n), result in x[0..n).
+// The caller has synchronised after filling x; x spans f.ldl elements.
+template <typename T, int RS>
+__device__ void line_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw, int lane) {
+  const int n = f.n, M = f.M, tpl = f.tpl;
+  if (f.mode == 0) {
+    fft_lds<T, RS>(x, f, tw, lane);
     return;
   }
-  const int n = f.n, M = f.M, tpl = f.tpl;
-  for (int m = lane; m < M; m += tpl)
-    x[m] = (m < n) ? cmul(x[m], f.chirp[m]) : mkc<T>(0, 0);
+  if (f.mode == 1) {  // Bluestein
+    for (int m = lane; m < M; m += tpl)
+      x[m] = (m < n) ? cmul(x[m], f.chirp[m]) : mkc<T>(0, 0);
+    __syncthreads();
+    fft_lds<T, RS>(x, f, tw, lane);
+    for (int m = lane; m < M; m += tpl) x[m] = cconj(cmul(x[m], f.bfil[m]));
+    __syncthreads();
+    fft_lds<T, RS>(x, f, tw, lane);
+    for (int m = lane; m < n; m += tpl) x[m] = cmul(cconj(x[m]), f.chirp[m]);
+    __syncthreads();
+    return;
+  }
+  // Rader: M = n - 1, second buffer b = x + n
+  cpx<T>* b = x + n;
+  const int* pin = f.perm;
+  const int* pout = f.perm + M;
+  const cpx<T> x0 = x[0];
+  for (int q = lane; q < M; q += tpl) b[q] = x[pin[q]];
   __syncthreads();
-  fft_lds(x, f, lane);
-  for (int m = lane; m < M; m += tpl) x[m] = cconj(cmul(x[m], f.bfil[m]));
+  fft_lds<T, RS>(b, f, tw, lane);
+  cpx<T> X0 = mkc<T>(0, 0);
+  if (lane == 0) X0 = cadd(x0, b[0]);
+  for (int m = lane; m < M; m += tpl) b[m] = cconj(cmul(b[m], f.bfil[m]));
   __syncthreads();
-  fft_lds(x, f, lane);
-  for (int m = lane; m < n; m += tpl) x[m] = cmul(cconj(x[m]), f.chirp[m]);
+  fft_lds<T, RS>(b, f, tw, lane);
+  for (int p = lane; p < M; p += tpl) x[pout[p]] = cadd(x0, cconj(b[p]));
+  if (lane == 0) x[0] = X0;
   __syncthreads();
 }
 

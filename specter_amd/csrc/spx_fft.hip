@@ -1,6 +1,7 @@
 // Line-FFT kernels of libspx: x (real<->half-complex), y, z (with the FC
-// continuation and the (kx,y,z)<->(z,y,kx) transposes fused in), and the
-// z-line kernels of the boundary/projection stage.
+// continuation and the (kx,y,z)<->(z,y,kx) transposes fused in), the fused
+// gradient-product x pass of gradre/advect, and the z-line kernels of the
+// boundary / projection / diagnostics stages.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -16,38 +17,47 @@ using LF = LineFFT<Cp>;
 // (0,1)*k*v evaluated as the reference's complex product (-k Im v, k Re v)
 __device__ __forceinline__ Cp ikmul(double k, Cp v) { return mkc<Tp>(-(k * v.y), k * v.x); }
 
+// Twiddles at the head of dynamic LDS, then the line buffers.  Each thread
+// belongs to line `line` (lane `lane` of tpl); xl is its line buffer.
+#define SPX_PROLOGUE(f)                                          \
+  extern __shared__ unsigned char smem[];                        \
+  Cp* const tws = reinterpret_cast<Cp*>(smem);                   \
+  Cp* const lds = tws + (f).M;                                   \
+  stage_twiddles(tws, (f));                                      \
+  const int tid = threadIdx.x;                                   \
+  const int line = tid >> (f).lgtpl, lane = tid & ((f).tpl - 1); \
+  Cp* const xl = lds + line * (f).ldl;                           \
+  (void)xl;
+
+constexpr int KEEP = kStage;  // line elements a thread can hold in registers (n <= KEEP*tpl)
+
 // ---------------------------------------------------------------------------
 // z transforms in place on S (contiguous lines)
 // ---------------------------------------------------------------------------
+template <int RS>
 __global__ void __launch_bounds__(256) k_z_inplace(Cp* __restrict__ S, long nlines, LF f, int np,
                                                    int forward, int cont, double scale,
                                                    const double* __restrict__ dir, int C,
                                                    int d) {
-  extern __shared__ unsigned char smem[];
-  Cp* lds = reinterpret_cast<Cp*>(smem);
-  const int tid = threadIdx.x, NT = blockDim.x, n = f.n, nl = f.nl, ldl = f.ldl;
-  const long line0 = long(blockIdx.x) * nl;
+  SPX_PROLOGUE(f);
+  const int n = f.n;
+  const long ln = long(blockIdx.x) * f.nl + line;
+  const bool ok = ln < nlines;
   const int nload = (forward && cont) ? np : n;
-  for (int e = tid; e < nl * nload; e += NT) {
-    const int l = e / nload, k = e - l * nload;
-    const long ln = line0 + l;
-    Cp v = mkc<Tp>(0, 0);
-    if (ln < nlines) v = S[ln * n + k];
+  for (int k = lane; k < nload; k += f.tpl) {
+    Cp v = ok ? S[ln * n + k] : mkc<Tp>(0, 0);
     if (!forward) v.y = -v.y;
-    lds[l * ldl + k] = v;
+    xl[k] = v;
   }
   __syncthreads();
   if (forward && cont) {
-    fc_continue_lds(lds, nl, ldl, np, C, d, dir);
+    fc_continue_lds(lds, f.nl, f.ldl, np, C, d, dir);
     __syncthreads();
   }
-  const int line = tid / f.tpl, lane = tid - line * f.tpl;
-  line_dft(lds + line * ldl, f, lane);
-  for (int e = tid; e < nl * n; e += NT) {
-    const int l = e / n, k = e - l * n;
-    const long ln = line0 + l;
-    if (ln < nlines) {
-      Cp v = lds[l * ldl + k];
+  line_dft<Tp, RS>(xl, f, tws, lane);
+  if (ok) {
+    for (int k = lane; k < n; k += f.tpl) {
+      Cp v = xl[k];
       if (!forward) {
         v.y = -v.y;
         if (k < np && scale != 1.0) v = cscale(v, scale);
@@ -83,162 +93,183 @@ __global__ void k_fc_continue(Cp* __restrict__ S, long nlines, int n, int np, in
 // ---------------------------------------------------------------------------
 // forward 3D last pass: X(kx,y,z) planes < np -> continuation -> z FFT -> S(z,y,kx)
 // ---------------------------------------------------------------------------
+template <int RS>
 __global__ void __launch_bounds__(256) k_z_x2s(const Cp* __restrict__ X, Cp* __restrict__ S, LF f,
                                                int nkx, int ny, int np, int nofft,
                                                const double* __restrict__ dir, int C, int d) {
-  extern __shared__ unsigned char smem[];
-  Cp* lds = reinterpret_cast<Cp*>(smem);
-  const int tid = threadIdx.x, NT = blockDim.x, n = f.n, nl = f.nl, ldl = f.ldl;
-  const int i0 = blockIdx.x * nl, j = blockIdx.y;
+  SPX_PROLOGUE(f);
+  const int n = f.n;
+  const int i0 = blockIdx.x * f.nl, j = blockIdx.y;
   const int nload = nofft ? n : np;
   const long pl = long(nkx) * ny;
-  for (int e = tid; e < nl * nload; e += NT) {
-    const int k = e / nl, l = e - k * nl, i = i0 + l;
-    Cp v = mkc<Tp>(0, 0);
-    if (i < nkx) v = X[i + long(nkx) * j + pl * k];
-    lds[l * ldl + k] = v;
+  {
+    const int l = tid & (f.nl - 1), i = i0 + l;
+    for (int k = tid >> f.lgnl; k < nload; k += f.tpl) {
+      Cp v = mkc<Tp>(0, 0);
+      if (i < nkx) v = X[i + long(nkx) * j + pl * k];
+      lds[l * f.ldl + k] = v;
+    }
   }
   __syncthreads();
   if (!nofft) {
     if (C > 0) {
-      fc_continue_lds(lds, nl, ldl, np, C, d, dir);
+      fc_continue_lds(lds, f.nl, f.ldl, np, C, d, dir);
       __syncthreads();
     }
-    const int line = tid / f.tpl, lane = tid - line * f.tpl;
-    line_dft(lds + line * ldl, f, lane);
+    line_dft<Tp, RS>(xl, f, tws, lane);
   }
-  for (int e = tid; e < nl * n; e += NT) {
-    const int l = e / n, k = e - l * n, i = i0 + l;
-    if (i < nkx) S[k + long(n) * (j + long(ny) * i)] = lds[l * ldl + k];
-  }
+  const int i = i0 + line;
+  if (i < nkx)
+    for (int k = lane; k < n; k += f.tpl) S[k + long(n) * (j + long(ny) * i)] = xl[k];
 }
 
 // ---------------------------------------------------------------------------
-// backward 3D first pass: S * (i k_dir) -> z IFFT -> X planes < nzout
+// backward 3D first pass: S * (i k_dir) -> z IFFT -> X planes < nzout.
+// DUAL: the same load also gives X2 <- z IFFT of (i kz) S.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_z_s2x(const Cp* __restrict__ S, Cp* __restrict__ X, LF f,
-                                               int nkx, int ny, int nzout, int dirn, int nofft,
+template <int RS, bool DUAL>
+__global__ void __launch_bounds__(256) k_z_s2x(const Cp* __restrict__ S, Cp* __restrict__ X,
+                                               Cp* __restrict__ X2, LF f, int nkx, int ny,
+                                               int nzout, int dirn, int nofft,
                                                const double* __restrict__ kx,
                                                const double* __restrict__ ky,
                                                const double* __restrict__ kz) {
-  extern __shared__ unsigned char smem[];
-  Cp* lds = reinterpret_cast<Cp*>(smem);
-  const int tid = threadIdx.x, NT = blockDim.x, n = f.n, nl = f.nl, ldl = f.ldl;
-  const int i0 = blockIdx.x * nl, j = blockIdx.y;
-  for (int e = tid; e < nl * n; e += NT) {
-    const int l = e / n, k = e - l * n, i = i0 + l;
-    Cp v = mkc<Tp>(0, 0);
-    if (i < nkx) {
-      v = S[k + long(n) * (j + long(ny) * i)];
-      if (dirn == 1) v = ikmul(kx[i], v);
-      else if (dirn == 2) v = ikmul(ky[j], v);
-      else if (dirn == 3) v = ikmul(kz[k], v);
+  SPX_PROLOGUE(f);
+  const int n = f.n;
+  const int i0 = blockIdx.x * f.nl, j = blockIdx.y;
+  const int i = i0 + line;
+  const bool ok = i < nkx;
+  const long base = long(n) * (j + long(ny) * (ok ? i : 0));
+  Cp keep[DUAL ? KEEP : 1];
+#pragma unroll
+  for (int q = 0; q < KEEP; ++q) {
+    const int k = lane + q * f.tpl;
+    if (k < n) {
+      Cp v = mkc<Tp>(0, 0);
+      if (ok) {
+        v = S[base + k];
+        if (dirn == 1) v = ikmul(kx[i], v);
+        else if (dirn == 2) v = ikmul(ky[j], v);
+        else if (dirn == 3) v = ikmul(kz[k], v);
+      }
+      if constexpr (DUAL) keep[q] = v;
+      xl[k] = cconj(v);
     }
-    v.y = -v.y;
-    lds[l * ldl + k] = v;
   }
   __syncthreads();
-  if (!nofft) {
-    const int line = tid / f.tpl, lane = tid - line * f.tpl;
-    line_dft(lds + line * ldl, f, lane);
-  }
+  if (!nofft) line_dft<Tp, RS>(xl, f, tws, lane);
   const long pl = long(nkx) * ny;
-  for (int e = tid; e < nl * nzout; e += NT) {
-    const int k = e / nl, l = e - k * nl, i = i0 + l;
-    if (i < nkx) {
-      Cp v = lds[l * ldl + k];
-      v.y = -v.y;
-      X[i + long(nkx) * j + pl * k] = v;
+  const int l = tid & (f.nl - 1), ii = i0 + l;
+  if (ii < nkx)
+    for (int k = tid >> f.lgnl; k < nzout; k += f.tpl)
+      X[ii + long(nkx) * j + pl * k] = cconj(lds[l * f.ldl + k]);
+  if constexpr (DUAL) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < KEEP; ++q) {
+      const int k = lane + q * f.tpl;
+      if (k < n) xl[k] = cconj(ikmul(kz[k], keep[q]));
     }
+    __syncthreads();
+    line_dft<Tp, RS>(xl, f, tws, lane);
+    if (ii < nkx)
+      for (int k = tid >> f.lgnl; k < nzout; k += f.tpl)
+        X2[ii + long(nkx) * j + pl * k] = cconj(lds[l * f.ldl + k]);
   }
 }
 
 // ---------------------------------------------------------------------------
-// y transform in place on X (lines along y, stride nkx), planes < nplanes
+// y transform of X (lines along y, stride nkx), one plane per blockIdx.y.
+// DUAL (backward): Y2 <- y IFFT of (i ky) Xin from the same load.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_y(Cp* __restrict__ X, LF f, int nkx, int ny, int forward) {
-  extern __shared__ unsigned char smem[];
-  Cp* lds = reinterpret_cast<Cp*>(smem);
-  const int tid = threadIdx.x, NT = blockDim.x, nl = f.nl, ldl = f.ldl;
-  const int i0 = blockIdx.x * nl;
+template <int RS, bool DUAL>
+__global__ void __launch_bounds__(256) k_y(const Cp* Xin, Cp* Xout, Cp* __restrict__ Y2, LF f,
+                                           int nkx, int ny, int forward,
+                                           const double* __restrict__ ky) {
+  SPX_PROLOGUE(f);
+  const int i0 = blockIdx.x * f.nl;
   const long base = long(blockIdx.y) * nkx * ny;
-  for (int e = tid; e < nl * ny; e += NT) {
-    const int jy = e / nl, l = e - jy * nl, i = i0 + l;
+  const int l = tid & (f.nl - 1), ii = i0 + l;
+  for (int jy = tid >> f.lgnl; jy < ny; jy += f.tpl) {
     Cp v = mkc<Tp>(0, 0);
-    if (i < nkx) v = X[base + i + long(nkx) * jy];
+    if (ii < nkx) v = Xin[base + ii + long(nkx) * jy];
     if (!forward) v.y = -v.y;
-    lds[l * ldl + jy] = v;
+    lds[l * f.ldl + jy] = v;
   }
   __syncthreads();
-  const int line = tid / f.tpl, lane = tid - line * f.tpl;
-  line_dft(lds + line * ldl, f, lane);
-  for (int e = tid; e < nl * ny; e += NT) {
-    const int jy = e / nl, l = e - jy * nl, i = i0 + l;
-    if (i < nkx) {
-      Cp v = lds[l * ldl + jy];
-      if (!forward) v.y = -v.y;
-      X[base + i + long(nkx) * jy] = v;
+  Cp keep[DUAL ? KEEP : 1];
+  if constexpr (DUAL) {
+#pragma unroll
+    for (int q = 0; q < KEEP; ++q) {
+      const int k = lane + q * f.tpl;
+      if (k < ny) keep[q] = xl[k];
     }
+  }
+  line_dft<Tp, RS>(xl, f, tws, lane);
+  if (ii < nkx)
+    for (int jy = tid >> f.lgnl; jy < ny; jy += f.tpl) {
+      Cp v = lds[l * f.ldl + jy];
+      if (!forward) v.y = -v.y;
+      Xout[base + ii + long(nkx) * jy] = v;
+    }
+  if constexpr (DUAL) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < KEEP; ++q) {
+      const int k = lane + q * f.tpl;
+      if (k < ny) xl[k] = cconj(ikmul(ky[k], cconj(keep[q])));
+    }
+    __syncthreads();
+    line_dft<Tp, RS>(xl, f, tws, lane);
+    if (ii < nkx)
+      for (int jy = tid >> f.lgnl; jy < ny; jy += f.tpl)
+        Y2[base + ii + long(nkx) * jy] = cconj(lds[l * f.ldl + jy]);
   }
 }
 
 // ---------------------------------------------------------------------------
 // x real -> half complex: rows 2p, 2p+1 packed as one complex line
 // ---------------------------------------------------------------------------
+template <int RS>
 __global__ void __launch_bounds__(256) k_xr2c(const Tp* __restrict__ R, Cp* __restrict__ X, LF f,
                                               int nkx, long npairs) {
-  extern __shared__ unsigned char smem[];
-  Cp* lds = reinterpret_cast<Cp*>(smem);
-  const int tid = threadIdx.x, NT = blockDim.x, nx = f.n, nl = f.nl, ldl = f.ldl;
-  const long p0 = long(blockIdx.x) * nl;
-  for (int e = tid; e < nl * nx; e += NT) {
-    const int l = e / nx, x = e - l * nx;
-    const long p = p0 + l;
-    Cp v = mkc<Tp>(0, 0);
-    if (p < npairs) v = mkc<Tp>(R[x + long(nx) * (2 * p)], R[x + long(nx) * (2 * p + 1)]);
-    lds[l * ldl + x] = v;
-  }
+  SPX_PROLOGUE(f);
+  const int nx = f.n;
+  const long p = long(blockIdx.x) * f.nl + line;
+  const bool ok = p < npairs;
+  const Tp* r0 = R + long(nx) * (2 * p);
+  for (int x = lane; x < nx; x += f.tpl) xl[x] = ok ? mkc<Tp>(r0[x], r0[x + nx]) : mkc<Tp>(0, 0);
   __syncthreads();
-  const int line = tid / f.tpl, lane = tid - line * f.tpl;
-  line_dft(lds + line * ldl, f, lane);
-  const int two = 2 * nkx;
-  for (int e = tid; e < nl * two; e += NT) {
-    const int l = e / two, q = e - l * two;
-    const long p = p0 + l;
-    if (p >= npairs) continue;
-    const int w = q >= nkx ? 1 : 0, k = q - w * nkx;
-    const Cp Z = lds[l * ldl + k];
-    const Cp Zc = lds[l * ldl + (k == 0 ? 0 : nx - k)];
-    Cp o;
-    if (w == 0) {
-      o = mkc<Tp>((Z.x + Zc.x) * 0.5, (Z.y - Zc.y) * 0.5);
-    } else {
-      const Tp dr = Z.x - Zc.x, di = Z.y + Zc.y;
-      o = mkc<Tp>(di * 0.5, -dr * 0.5);
+  line_dft<Tp, RS>(xl, f, tws, lane);
+  if (ok) {
+    Cp* o0 = X + long(nkx) * (2 * p);
+    for (int k = lane; k < nkx; k += f.tpl) {
+      const Cp Z = xl[k];
+      const Cp Zc = xl[k == 0 ? 0 : nx - k];
+      o0[k] = mkc<Tp>((Z.x + Zc.x) * 0.5, (Z.y - Zc.y) * 0.5);
+      o0[k + nkx] = mkc<Tp>((Z.y + Zc.y) * 0.5, -(Z.x - Zc.x) * 0.5);
     }
-    X[k + long(nkx) * (2 * p + w)] = o;
   }
 }
 
-// ---------------------------------------------------------------------------
-// x half complex -> real (FFTW c2r semantics: Im of kx=0 and kx=nx/2 ignored)
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_xc2r(const Cp* __restrict__ X, Tp* __restrict__ R, LF f,
-                                              int nkx, long npairs) {
-  extern __shared__ unsigned char smem[];
-  Cp* lds = reinterpret_cast<Cp*>(smem);
-  const int tid = threadIdx.x, NT = blockDim.x, nx = f.n, nl = f.nl, ldl = f.ldl;
+// Load of an x c2r row pair into LDS: Z = A + iB from the two half spectra
+// (FFTW c2r semantics: Im of kx = 0 and kx = nx/2 ignored), conjugated for the
+// backward transform; dirx multiplies both by i kx first.
+__device__ __forceinline__ void load_c2r_pair(Cp* xl, const Cp* __restrict__ a0, int nkx, int nx,
+                                              int lane, int tpl, bool ok, int dirx,
+                                              const double* __restrict__ kx) {
   const int h = nx / 2;
-  const long p0 = long(blockIdx.x) * nl;
-  for (int e = tid; e < nl * nx; e += NT) {
-    const int l = e / nx, m = e - l * nx;
-    const long p = p0 + l;
+  for (int m = lane; m < nx; m += tpl) {
     Cp v = mkc<Tp>(0, 0);
-    if (p < npairs) {
+    if (ok) {
       const bool up = m > h;
       const int k = up ? nx - m : m;
-      Cp A = X[k + long(nkx) * (2 * p)];
-      Cp B = X[k + long(nkx) * (2 * p + 1)];
+      Cp A = a0[k];
+      Cp B = a0[k + nkx];
+      if (dirx) {
+        A = ikmul(kx[k], A);
+        B = ikmul(kx[k], B);
+      }
       if (k == 0 || k == h) {
         A.y = 0;
         B.y = 0;
@@ -246,18 +277,87 @@ __global__ void __launch_bounds__(256) k_xc2r(const Cp* __restrict__ X, Tp* __re
       v = up ? mkc<Tp>(A.x + B.y, -A.y + B.x) : mkc<Tp>(A.x - B.y, A.y + B.x);
       v.y = -v.y;
     }
-    lds[l * ldl + m] = v;
+    xl[m] = v;
   }
+}
+
+template <int RS>
+__global__ void __launch_bounds__(256) k_xc2r(const Cp* __restrict__ X, Tp* __restrict__ R, LF f,
+                                              int nkx, long npairs, int dirx,
+                                              const double* __restrict__ kx) {
+  SPX_PROLOGUE(f);
+  const int nx = f.n;
+  const long p = long(blockIdx.x) * f.nl + line;
+  const bool ok = p < npairs;
+  load_c2r_pair(xl, X + long(nkx) * (2 * (ok ? p : 0)), nkx, nx, lane, f.tpl, ok, dirx, kx);
   __syncthreads();
-  const int line = tid / f.tpl, lane = tid - line * f.tpl;
-  line_dft(lds + line * ldl, f, lane);
-  for (int e = tid; e < nl * nx; e += NT) {
-    const int l = e / nx, x = e - l * nx;
-    const long p = p0 + l;
-    if (p < npairs) {
-      const Cp z = lds[l * ldl + x];
-      R[x + long(nx) * (2 * p)] = z.x;
-      R[x + long(nx) * (2 * p + 1)] = -z.y;
+  line_dft<Tp, RS>(xl, f, tws, lane);
+  if (ok) {
+    Tp* r0 = R + long(nx) * (2 * p);
+    for (int x = lane; x < nx; x += f.tpl) {
+      const Cp z = xl[x];
+      r0[x] = z.x;
+      r0[x + nx] = -z.y;
+    }
+  }
+}
+
+// ((u*Dx + v*Dy) + w*Dz)*tmp on a row pair, the three derivatives c2r'd in
+// LDS one after the other and accumulated in registers
+// (pseudospec_hd.f90:254-312, pseudospec_phd.f90:63-108).
+template <int RS>
+__global__ void __launch_bounds__(256) k_xgrad_prod(const Cp* __restrict__ YV,
+                                                    const Cp* __restrict__ YY,
+                                                    const Cp* __restrict__ XZ,
+                                                    const Tp* __restrict__ u,
+                                                    const Tp* __restrict__ v,
+                                                    const Tp* __restrict__ w,
+                                                    Tp* __restrict__ out, LF f, int nkx,
+                                                    long npairs, double tmp,
+                                                    const double* __restrict__ kx) {
+#pragma clang fp contract(off)
+  SPX_PROLOGUE(f);
+  const int nx = f.n;
+  const long p = long(blockIdx.x) * f.nl + line;
+  const bool ok = p < npairs;
+  const long ro = long(nx) * (2 * (ok ? p : 0));
+  const long co = long(nkx) * (2 * (ok ? p : 0));
+  Tp accA[KEEP], accB[KEEP];
+  const Cp* srcs[3] = {YV, YY, XZ};
+  const Tp* mult[3] = {u, v, w};
+  for (int r = 0; r < 3; ++r) {
+    load_c2r_pair(xl, srcs[r] + co, nkx, nx, lane, f.tpl, ok, r == 0 ? 1 : 0, kx);
+    __syncthreads();
+    line_dft<Tp, RS>(xl, f, tws, lane);
+    const Tp* m = mult[r] + ro;
+#pragma unroll
+    for (int q = 0; q < KEEP; ++q) {
+      const int x = lane + q * f.tpl;
+      if (x < nx && ok) {
+        const Cp z = xl[x];
+        const Tp a = z.x, b = -z.y;
+        if (r == 0) {
+          accA[q] = m[x] * a;
+          accB[q] = m[x + nx] * b;
+        } else if (r == 1) {
+          accA[q] = accA[q] + m[x] * a;
+          accB[q] = accB[q] + m[x + nx] * b;
+        } else {
+          accA[q] = (accA[q] + m[x] * a) * tmp;
+          accB[q] = (accB[q] + m[x + nx] * b) * tmp;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (ok) {
+#pragma unroll
+    for (int q = 0; q < KEEP; ++q) {
+      const int x = lane + q * f.tpl;
+      if (x < nx) {
+        out[ro + x] = accA[q];
+        out[ro + x + nx] = accB[q];
+      }
     }
   }
 }
@@ -274,6 +374,7 @@ struct WallParams {
   int scalar;          // 1: s_constant_z (zero at walls)
 };
 
+template <int RS>
 __global__ void __launch_bounds__(256) k_noslip(Cp* __restrict__ vx, Cp* __restrict__ vy,
                                                 const Cp* __restrict__ pr, LF f, long nlines,
                                                 int ny, int np, double invn, WallParams wp,
@@ -281,84 +382,56 @@ __global__ void __launch_bounds__(256) k_noslip(Cp* __restrict__ vx, Cp* __restr
                                                 const double* __restrict__ ky,
                                                 const double* __restrict__ dir, int C, int d) {
 #pragma clang fp contract(off)
-  extern __shared__ unsigned char smem[];
-  Cp* lds = reinterpret_cast<Cp*>(smem);
-  const int tid = threadIdx.x, NT = blockDim.x, n = f.n, nl = f.nl, ldl = f.ldl;
+  SPX_PROLOGUE(f);
+  const int n = f.n;
   const int comp = blockIdx.y;
   Cp* V = comp ? vy : vx;
-  const long line0 = long(blockIdx.x) * nl;
-  for (int e = tid; e < nl * n; e += NT) {
-    const int l = e / n, k = e - l * n;
-    const long ln = line0 + l;
-    Cp v = mkc<Tp>(0, 0);
-    if (ln < nlines) v = V[ln * n + k];
-    v.y = -v.y;
-    lds[l * ldl + k] = v;
-  }
+  const long ln = long(blockIdx.x) * f.nl + line;
+  const bool ok = ln < nlines;
+  for (int k = lane; k < n; k += f.tpl) xl[k] = ok ? cconj(V[ln * n + k]) : mkc<Tp>(0, 0);
   __syncthreads();
-  const int line = tid / f.tpl, lane = tid - line * f.tpl;
-  line_dft(lds + line * ldl, f, lane);
-  for (int e = tid; e < nl * np; e += NT) {
-    const int l = e / np, k = e - l * np;
-    Cp v = lds[l * ldl + k];
-    v.y = -v.y;
-    lds[l * ldl + k] = cscale(v, invn);
-  }
+  line_dft<Tp, RS>(xl, f, tws, lane);
+  for (int k = lane; k < np; k += f.tpl) xl[k] = cscale(cconj(xl[k]), invn);
   __syncthreads();
-  for (int e = tid; e < nl * 2; e += NT) {
-    const int l = e >> 1, w = e & 1;
-    const long ln = line0 + l;
-    if (ln >= nlines) continue;
+  if (lane < 2 && ok) {
+    const int w = lane;
     const int ind = w ? np - 1 : 0;
-    Cp v;
+    Cp val;
     if (wp.scalar) {
-      v = mkc<Tp>(0, 0);
+      val = mkc<Tp>(0, 0);
     } else {
       const long i = ln / ny;
       const int j = int(ln - i * ny);
-      const Cp p = pr[ln * n + ind];
-      v = cscale(ikmul(comp ? ky[j] : kx[i], p), wp.tau);
-      if (i == 0 && j == 0) v = mkc<Tp>(wp.nxny * wp.vb[comp][w], 0.0);
+      const Cp pv = pr[ln * n + ind];
+      val = cscale(ikmul(comp ? ky[j] : kx[i], pv), wp.tau);
+      if (i == 0 && j == 0) val = mkc<Tp>(wp.nxny * wp.vb[comp][w], 0.0);
     }
-    lds[l * ldl + ind] = v;
+    xl[ind] = val;
   }
   __syncthreads();
-  fc_continue_lds(lds, nl, ldl, np, C, d, dir);
+  fc_continue_lds(lds, f.nl, f.ldl, np, C, d, dir);
   __syncthreads();
-  line_dft(lds + line * ldl, f, lane);
-  for (int e = tid; e < nl * n; e += NT) {
-    const int l = e / n, k = e - l * n;
-    const long ln = line0 + l;
-    if (ln < nlines) V[ln * n + k] = lds[l * ldl + k];
-  }
+  line_dft<Tp, RS>(xl, f, tws, lane);
+  if (ok)
+    for (int k = lane; k < n; k += f.tpl) V[ln * n + k] = xl[k];
 }
 
 // ---------------------------------------------------------------------------
 // sol_project (boundary_mod.fpp:197-402), bctarget = 1, bczsta = bczend = 0
 // ---------------------------------------------------------------------------
 // bc(1:2) <- IFFT_z(c * (1/nz)) at z = 0 and z = Lz   (:262-352)
+template <int RS>
 __global__ void __launch_bounds__(256) k_proj_bc(const Cp* __restrict__ c, Cp* __restrict__ bc,
                                                  LF f, long nlines, int np, double tmp) {
-  extern __shared__ unsigned char smem[];
-  Cp* lds = reinterpret_cast<Cp*>(smem);
-  const int tid = threadIdx.x, NT = blockDim.x, n = f.n, nl = f.nl, ldl = f.ldl;
-  const long line0 = long(blockIdx.x) * nl;
-  for (int e = tid; e < nl * n; e += NT) {
-    const int l = e / n, k = e - l * n;
-    const long ln = line0 + l;
-    Cp v = mkc<Tp>(0, 0);
-    if (ln < nlines) v = cscale(c[ln * n + k], tmp);
-    v.y = -v.y;
-    lds[l * ldl + k] = v;
-  }
+  SPX_PROLOGUE(f);
+  const int n = f.n;
+  const long ln = long(blockIdx.x) * f.nl + line;
+  const bool ok = ln < nlines;
+  for (int k = lane; k < n; k += f.tpl)
+    xl[k] = ok ? cconj(cscale(c[ln * n + k], tmp)) : mkc<Tp>(0, 0);
   __syncthreads();
-  const int line = tid / f.tpl, lane = tid - line * f.tpl;
-  line_dft(lds + line * ldl, f, lane);
-  for (int e = tid; e < nl * 2; e += NT) {
-    const int l = e >> 1, w = e & 1;
-    const long ln = line0 + l;
-    if (ln < nlines) bc[ln * 2 + w] = cconj(lds[l * ldl + (w ? np - 1 : 0)]);
-  }
+  line_dft<Tp, RS>(xl, f, tws, lane);
+  if (ok && lane < 2) bc[ln * 2 + lane] = cconj(xl[lane ? np - 1 : 0]);
 }
 
 // laplace_z pure-Neumann coefficients (boundary_mod.fpp:531-560)
@@ -381,36 +454,27 @@ __device__ __forceinline__ Coef neumann_coef(Cp b1, Cp b2, double kh, double Lz,
 }
 
 // d <- IFFT_z(d)*tmp + a_harm(z)  (:354-381 with laplace_z :631-675)
+template <int RS>
 __global__ void __launch_bounds__(256) k_proj_pressure(Cp* __restrict__ dd,
                                                        const Cp* __restrict__ bc, LF f,
                                                        long nlines, int ny, double tmp, double Lz,
                                                        const double* __restrict__ khom,
                                                        const double* __restrict__ z) {
 #pragma clang fp contract(off)
-  extern __shared__ unsigned char smem[];
-  Cp* lds = reinterpret_cast<Cp*>(smem);
-  const int tid = threadIdx.x, NT = blockDim.x, n = f.n, nl = f.nl, ldl = f.ldl;
-  const long line0 = long(blockIdx.x) * nl;
-  for (int e = tid; e < nl * n; e += NT) {
-    const int l = e / n, k = e - l * n;
-    const long ln = line0 + l;
-    Cp v = mkc<Tp>(0, 0);
-    if (ln < nlines) v = dd[ln * n + k];
-    v.y = -v.y;
-    lds[l * ldl + k] = v;
-  }
+  SPX_PROLOGUE(f);
+  const int n = f.n;
+  const long ln = long(blockIdx.x) * f.nl + line;
+  const bool ok = ln < nlines;
+  for (int k = lane; k < n; k += f.tpl) xl[k] = ok ? cconj(dd[ln * n + k]) : mkc<Tp>(0, 0);
   __syncthreads();
-  const int line = tid / f.tpl, lane = tid - line * f.tpl;
-  line_dft(lds + line * ldl, f, lane);
-  for (int e = tid; e < nl * n; e += NT) {
-    const int l = e / n, k = e - l * n;
-    const long ln = line0 + l;
-    if (ln >= nlines) continue;
-    const long i = ln / ny;
-    const int j = int(ln - i * ny);
-    const bool zm = (i == 0 && j == 0);
-    const double kh = khom[ln];
-    const Coef cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
+  line_dft<Tp, RS>(xl, f, tws, lane);
+  if (!ok) return;
+  const long i = ln / ny;
+  const int j = int(ln - i * ny);
+  const bool zm = (i == 0 && j == 0);
+  const double kh = khom[ln];
+  const Coef cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
+  for (int k = lane; k < n; k += f.tpl) {
     Cp a;
     if (zm) {
       a = mkc<Tp>(cf.c1.x * z[k] + cf.c2.x, 0.0);
@@ -419,14 +483,13 @@ __global__ void __launch_bounds__(256) k_proj_pressure(Cp* __restrict__ dd,
       const double em = exp(-kh * z[k]);
       a = cadd(cscale(cf.c1, ep), cscale(cf.c2, em));
     }
-    Cp v = lds[l * ldl + k];
-    v.y = -v.y;
-    dd[ln * n + k] = cadd(cscale(v, tmp), a);
+    dd[ln * n + k] = cadd(cscale(cconj(xl[k]), tmp), a);
   }
 }
 
 // harmonic correction: A = FFT_z(cont(a_harm)), B = FFT_z(cont(d a_harm/dz));
-// vx -= i kx A, vy -= i ky A, vz -= B   (:383-399)
+// vx -= i kx A, vy -= i ky A, vz -= B   (:383-399).  Two buffers per line.
+template <int RS>
 __global__ void __launch_bounds__(256) k_proj_harmonic(Cp* __restrict__ va, Cp* __restrict__ vb,
                                                        Cp* __restrict__ vc,
                                                        const Cp* __restrict__ bc, LF f,
@@ -438,21 +501,22 @@ __global__ void __launch_bounds__(256) k_proj_harmonic(Cp* __restrict__ va, Cp* 
                                                        const double* __restrict__ dir, int C,
                                                        int d) {
 #pragma clang fp contract(off)
-  extern __shared__ unsigned char smem[];
-  Cp* ldsA = reinterpret_cast<Cp*>(smem);
-  const int tid = threadIdx.x, NT = blockDim.x, n = f.n, nl = f.nl, ldl = f.ldl;
-  Cp* ldsB = ldsA + nl * ldl;
-  const long line0 = long(blockIdx.x) * nl;
-  for (int e = tid; e < nl * np; e += NT) {
-    const int l = e / np, k = e - l * np;
-    const long ln = line0 + l;
+  SPX_PROLOGUE(f);
+  const int n = f.n;
+  Cp* const ldsB = lds + f.nl * f.ldl;
+  Cp* const xb = ldsB + line * f.ldl;
+  const long ln = long(blockIdx.x) * f.nl + line;
+  const bool ok = ln < nlines;
+  const long i = ok ? ln / ny : 0;
+  const int j = ok ? int(ln - i * ny) : 0;
+  const bool zm = (i == 0 && j == 0);
+  const double kh = ok ? khom[ln] : 0.0;
+  Coef cf;
+  cf.c1 = cf.c2 = mkc<Tp>(0, 0);
+  if (ok) cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
+  for (int k = lane; k < np; k += f.tpl) {
     Cp a = mkc<Tp>(0, 0), b = mkc<Tp>(0, 0);
-    if (ln < nlines) {
-      const long i = ln / ny;
-      const int j = int(ln - i * ny);
-      const bool zm = (i == 0 && j == 0);
-      const double kh = khom[ln];
-      const Coef cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
+    if (ok) {
       if (zm) {
         a = mkc<Tp>(cf.c1.x * z[k] + cf.c2.x, 0.0);
         b = mkc<Tp>(cf.c1.x, 0.0);
@@ -464,26 +528,22 @@ __global__ void __launch_bounds__(256) k_proj_harmonic(Cp* __restrict__ va, Cp* 
         b = cscale(csub(t1, t2), kh);
       }
     }
-    ldsA[l * ldl + k] = a;
-    ldsB[l * ldl + k] = b;
+    xl[k] = a;
+    xb[k] = b;
   }
   __syncthreads();
-  fc_continue_lds(ldsA, nl, ldl, np, C, d, dir);
-  fc_continue_lds(ldsB, nl, ldl, np, C, d, dir);
+  fc_continue_lds(lds, f.nl, f.ldl, np, C, d, dir);
+  fc_continue_lds(ldsB, f.nl, f.ldl, np, C, d, dir);
   __syncthreads();
-  const int line = tid / f.tpl, lane = tid - line * f.tpl;
-  line_dft(ldsA + line * ldl, f, lane);
-  line_dft(ldsB + line * ldl, f, lane);
-  for (int e = tid; e < nl * n; e += NT) {
-    const int l = e / n, k = e - l * n;
-    const long ln = line0 + l;
-    if (ln >= nlines) continue;
-    const long i = ln / ny;
-    const int j = int(ln - i * ny);
-    const Cp A = ldsA[l * ldl + k], B = ldsB[l * ldl + k];
+  line_dft<Tp, RS>(xl, f, tws, lane);
+  line_dft<Tp, RS>(xb, f, tws, lane);
+  if (!ok) return;
+  const double kxi = kx[i], kyj = ky[j];
+  for (int k = lane; k < n; k += f.tpl) {
+    const Cp A = xl[k], B = xb[k];
     const long o = ln * n + k;
-    va[o] = csub(va[o], ikmul(kx[i], A));
-    vb[o] = csub(vb[o], ikmul(ky[j], A));
+    va[o] = csub(va[o], ikmul(kxi, A));
+    vb[o] = csub(vb[o], ikmul(kyj, A));
     vc[o] = csub(vc[o], B);
   }
 }
@@ -515,6 +575,7 @@ __device__ __forceinline__ Cp deriv_apply(Cp v, int dirn, double kxi, double kyj
 }
 
 // mode 0: sum_k<np |IFFT(lincomb)|^2 ; mode 1: Re(IFFT(a) conj IFFT(b)) ; mode 2: walls |.|^2
+template <int RS>
 __global__ void __launch_bounds__(256) k_zline_reduce(LinCombDev lc, const Cp* __restrict__ bsrc,
                                                       int mode, double* __restrict__ out, LF f,
                                                       long nlines, int ny, int np,
@@ -522,19 +583,18 @@ __global__ void __launch_bounds__(256) k_zline_reduce(LinCombDev lc, const Cp* _
                                                       const double* __restrict__ ky,
                                                       const double* __restrict__ kz) {
 #pragma clang fp contract(off)
-  extern __shared__ unsigned char smem[];
-  Cp* ldsA = reinterpret_cast<Cp*>(smem);
-  const int tid = threadIdx.x, NT = blockDim.x, n = f.n, nl = f.nl, ldl = f.ldl;
-  Cp* ldsB = ldsA + nl * ldl;
-  double* red = reinterpret_cast<double*>(ldsB + (mode == 1 ? nl * ldl : 0));
-  const long line0 = long(blockIdx.x) * nl;
-  for (int e = tid; e < nl * n; e += NT) {
-    const int l = e / n, k = e - l * n;
-    const long ln = line0 + l;
+  SPX_PROLOGUE(f);
+  const int n = f.n;
+  Cp* const ldsB = lds + f.nl * f.ldl;
+  Cp* const xb = ldsB + line * f.ldl;
+  double* red = reinterpret_cast<double*>(ldsB + (mode == 1 ? f.nl * f.ldl : 0));
+  const long ln = long(blockIdx.x) * f.nl + line;
+  const bool ok = ln < nlines;
+  const long i = ok ? ln / ny : 0;
+  const int j = ok ? int(ln - i * ny) : 0;
+  for (int k = lane; k < n; k += f.tpl) {
     Cp v = mkc<Tp>(0, 0), w = mkc<Tp>(0, 0);
-    if (ln < nlines) {
-      const long i = ln / ny;
-      const int j = int(ln - i * ny);
+    if (ok) {
       const double kxi = kx[i], kyj = ky[j], kzk = kz[k];
       v = deriv_apply(lc.in[0][ln * n + k], lc.dir[0], kxi, kyj, kzk);
       for (int t = 1; t < 3; ++t) {
@@ -544,48 +604,37 @@ __global__ void __launch_bounds__(256) k_zline_reduce(LinCombDev lc, const Cp* _
       }
       if (mode == 1) w = bsrc[ln * n + k];
     }
-    v.y = -v.y;
-    w.y = -w.y;
-    ldsA[l * ldl + k] = v;
-    if (mode == 1) ldsB[l * ldl + k] = w;
+    xl[k] = cconj(v);
+    if (mode == 1) xb[k] = cconj(w);
   }
   __syncthreads();
-  const int line = tid / f.tpl, lane = tid - line * f.tpl;
-  line_dft(ldsA + line * ldl, f, lane);
-  if (mode == 1) line_dft(ldsB + line * ldl, f, lane);
-  double acc0 = 0.0, acc1 = 0.0;
+  line_dft<Tp, RS>(xl, f, tws, lane);
+  if (mode == 1) line_dft<Tp, RS>(xb, f, tws, lane);
   if (mode == 2) {
-    if (lane == 0) {
-      const Cp b = ldsA[line * ldl], t = ldsA[line * ldl + np - 1];
-      acc0 = b.x * b.x + b.y * b.y;
-      acc1 = t.x * t.x + t.y * t.y;
+    if (lane == 0 && ok) {
+      const Cp b = xl[0], t = xl[np - 1];
+      out[2 * ln] = b.x * b.x + b.y * b.y;
+      out[2 * ln + 1] = t.x * t.x + t.y * t.y;
     }
-  } else {
-    for (int k = lane; k < np; k += f.tpl) {
-      const Cp v = ldsA[line * ldl + k];
-      if (mode == 0) {
-        acc0 += v.x * v.x + v.y * v.y;
-      } else {
-        const Cp w = ldsB[line * ldl + k];
-        // values hold conj(IFFT); Re(p conj(q)) is invariant under conj of both
-        acc0 += v.x * w.x + v.y * w.y;
-      }
+    return;
+  }
+  double acc = 0.0;
+  for (int k = lane; k < np; k += f.tpl) {
+    const Cp v = xl[k];
+    if (mode == 0) {
+      acc += v.x * v.x + v.y * v.y;
+    } else {
+      const Cp w = xb[k];
+      // values hold conj(IFFT); Re(p conj(q)) is invariant under conj of both
+      acc += v.x * w.x + v.y * w.y;
     }
   }
-  const long ln = line0 + line;
-  if (mode == 2) {
-    if (lane == 0 && ln < nlines) {
-      out[2 * ln] = acc0;
-      out[2 * ln + 1] = acc1;
-    }
-  } else {
-    const double s = line_reduce(acc0, red, line, lane, f.tpl);
-    if (lane == 0 && ln < nlines) out[ln] = s;
-  }
+  const double s = line_reduce(acc, red, line, lane, f.tpl);
+  if (lane == 0 && ok) out[ln] = s;
 }
 
 // ---------------------------------------------------------------------------
-// plan construction: factorisation, Bluestein size, long-double tables
+// plan construction: algorithm choice, factorisation, long-double tables
 // ---------------------------------------------------------------------------
 static bool factor_small(int n, std::vector<int>& fac) {
   fac.clear();
@@ -594,7 +643,7 @@ static bool factor_small(int n, std::vector<int>& fac) {
     n /= 2;
     ++a;
   }
-  // powers of two: prefer radix 8, then 4, avoid lone 2 when possible
+  // powers of two: prefer radix 8, then 4, avoid a lone 2 when possible
   int q = a / 3, r = a % 3;
   if (r == 1 && q >= 1) {
     --q;
@@ -606,7 +655,14 @@ static bool factor_small(int n, std::vector<int>& fac) {
     fac.push_back(4);
   }
   for (int i = 0; i < q; ++i) fac.push_back(8);
-  for (int p : {3, 5, 7, 11, 13}) {
+  int b = 0;
+  while (n % 3 == 0) {
+    n /= 3;
+    ++b;
+  }
+  for (int i = 0; i < b / 2; ++i) fac.push_back(9);
+  if (b % 2) fac.push_back(3);
+  for (int p : {5, 7, 11, 13}) {
     while (n % p == 0) {
       n /= p;
       fac.push_back(p);
@@ -621,11 +677,50 @@ static bool smooth7(int m) {
   return m == 1;
 }
 
-static int tpl_for(int M, const std::vector<int>& fac) {
-  int need = 1;
+static bool is_prime(int n) {
+  if (n < 2) return false;
+  for (int d = 2; d * d <= n; ++d)
+    if (n % d == 0) return false;
+  return true;
+}
+
+static long long powmod(long long b, long long e, long long m) {
+  long long r = 1 % m;
+  b %= m;
+  while (e > 0) {
+    if (e & 1) r = r * b % m;
+    b = b * b % m;
+    e >>= 1;
+  }
+  return r;
+}
+
+static int primitive_root(int n) {
+  std::vector<int> pf;
+  int m = n - 1;
+  for (int d = 2; d * d <= m; ++d)
+    if (m % d == 0) {
+      pf.push_back(d);
+      while (m % d == 0) m /= d;
+    }
+  if (m > 1) pf.push_back(m);
+  for (int g = 2; g < n; ++g) {
+    bool ok = true;
+    for (int p : pf)
+      if (powmod(g, (n - 1) / p, n) == 1) {
+        ok = false;
+        break;
+      }
+    if (ok) return g;
+  }
+  return -1;
+}
+
+static int tpl_for(int M, int n, const std::vector<int>& fac) {
+  int need = (n + KEEP - 1) / KEEP;
   for (int R : fac) {
     const int nb = M / R;
-    const int mb = std::max(1, 16 / R);
+    const int mb = std::max(1, kStage / R);
     need = std::max(need, (nb + mb - 1) / mb);
   }
   int t = 1;
@@ -640,14 +735,36 @@ static int dev_upload(void** dst, const void* src, size_t bytes) {
   return SPX_OK;
 }
 
+// naive long-double DFT of length M (used once per plan for the filters)
+static void dft_ld(const std::vector<long double>& xr, const std::vector<long double>& xi,
+                   const std::vector<long double>& wr, const std::vector<long double>& wi,
+                   std::vector<double>& out, long double scale) {
+  const int M = int(xr.size());
+  out.assign(2 * size_t(M), 0.0);
+  for (int k = 0; k < M; ++k) {
+    long double sr = 0, si = 0;
+    for (int m = 0; m < M; ++m) {
+      if (xr[m] == 0.0L && xi[m] == 0.0L) continue;
+      const int idx = int((long long)m * k % M);
+      sr += xr[m] * wr[idx] - xi[m] * wi[idx];
+      si += xr[m] * wi[idx] + xi[m] * wr[idx];
+    }
+    out[2 * k] = double(sr * scale);
+    out[2 * k + 1] = double(si * scale);
+  }
+}
+
 int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
   std::vector<int> fac;
   f.n = n;
   if (factor_small(n, fac)) {
-    f.blue = 0;
+    f.mode = 0;
     f.M = n;
+  } else if (is_prime(n) && factor_small(n - 1, fac)) {
+    f.mode = 2;
+    f.M = n - 1;
   } else {
-    f.blue = 1;
+    f.mode = 1;
     int M = 2 * n - 1;
     while (!smooth7(M)) ++M;
     f.M = M;
@@ -655,18 +772,28 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
   }
   if (int(fac.size()) > kMaxStages) return fail(SPX_ERR_UNSUPPORTED, "FFT length has too many factors");
   f.nstage = int(fac.size());
-  for (int i = 0; i < f.nstage; ++i) f.radix[i] = fac[i];
-  f.tpl = tpl_for(f.M, fac);
+  long long Ns = 1;
+  for (int i = 0; i < f.nstage; ++i) {
+    f.radix[i] = fac[i];
+    f.magic[i] = Ns > 1 ? unsigned(((1ULL << 32) + Ns - 1) / Ns) : 0u;
+    Ns *= fac[i];
+  }
+  f.tpl = tpl_for(f.M, n, fac);
   if (f.tpl > 256)
     return fail(SPX_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " exceeds the LDS engine");
-  f.ldl = f.M + ((f.M % 16 == 0) ? 1 : 0);
+  f.lgtpl = 0;
+  while ((1 << f.lgtpl) < f.tpl) ++f.lgtpl;
+  const int span = f.mode == 0 ? n : (f.mode == 1 ? f.M : 2 * n);
+  f.ldl = span + ((span % 16 == 0) ? 1 : 0);
   const size_t lds_budget = 64 * 1024;
   int nl = std::max(1, 256 / f.tpl);
   if (want_lines > 0) nl = std::min(nl, want_lines);
-  while (nl > 1 && size_t(nl) * f.ldl * csize > lds_budget) nl >>= 1;
-  if (size_t(nl) * f.ldl * csize > 160 * 1024)
+  while (nl > 1 && (size_t(f.M) + size_t(nl) * f.ldl) * csize > lds_budget) nl >>= 1;
+  if ((size_t(f.M) + size_t(nl) * f.ldl) * csize > 160 * 1024)
     return fail(SPX_ERR_UNSUPPORTED, "FFT line does not fit in LDS");
   f.nl = nl;
+  f.lgnl = 0;
+  while ((1 << f.lgnl) < f.nl) ++f.lgnl;
   // tables in long double
   const long double PI2 = 6.283185307179586476925286766559005768L;
   const int M = f.M;
@@ -682,7 +809,7 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
   }
   int rc = dev_upload(&f.tw, tw.data(), tw.size() * sizeof(double));
   if (rc) return rc;
-  if (f.blue) {
+  if (f.mode == 1) {
     // chirp w[m] = exp(-pi i (m^2 mod 2n)/n)
     const long double PI = PI2 / 2;
     std::vector<long double> cr(n), ci(n);
@@ -694,7 +821,7 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
       ch[2 * m] = double(cr[m]);
       ch[2 * m + 1] = double(ci[m]);
     }
-    // b[m] = conj(w[|m|]) wrapped; B = DFT_M(b) / M
+    // b[m] = conj(w[|m|]) wrapped; bfil = DFT_M(b) / M
     std::vector<long double> br(M, 0.0L), bi(M, 0.0L);
     br[0] = cr[0];
     bi[0] = -ci[0];
@@ -704,20 +831,30 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
       br[M - m] = cr[m];
       bi[M - m] = -ci[m];
     }
-    std::vector<double> bf(2 * size_t(M));
-    for (int k = 0; k < M; ++k) {
-      long double sr = 0, si = 0;
-      for (int m = 0; m < M; ++m) {
-        if (br[m] == 0.0L && bi[m] == 0.0L) continue;
-        const int idx = int((long long)m * k % M);
-        sr += br[m] * wr[idx] - bi[m] * wi[idx];
-        si += br[m] * wi[idx] + bi[m] * wr[idx];
-      }
-      bf[2 * k] = double(sr / M);
-      bf[2 * k + 1] = double(si / M);
-    }
+    std::vector<double> bf;
+    dft_ld(br, bi, wr, wi, bf, 1.0L / M);
     if ((rc = dev_upload(&f.chirp, ch.data(), ch.size() * sizeof(double)))) return rc;
     if ((rc = dev_upload(&f.bfil, bf.data(), bf.size() * sizeof(double)))) return rc;
+  } else if (f.mode == 2) {
+    // Rader: a_q = x[g^q], b_m = W_n^(g^-m), X[g^-p] = x0 + (a (*) b)_p
+    const int g = primitive_root(n);
+    if (g < 0) return fail(SPX_ERR_UNSUPPORTED, "no primitive root for Rader length");
+    std::vector<int> perm(2 * size_t(M));
+    const long long ginv = powmod(g, n - 2, n);
+    for (int q = 0; q < M; ++q) {
+      perm[q] = int(powmod(g, q, n));
+      perm[M + q] = int(powmod(ginv, q, n));
+    }
+    std::vector<long double> br(M), bi(M);
+    for (int m = 0; m < M; ++m) {
+      const long double ang = PI2 * (long double)perm[M + m] / (long double)n;
+      br[m] = cosl(ang);
+      bi[m] = -sinl(ang);
+    }
+    std::vector<double> bf;
+    dft_ld(br, bi, wr, wi, bf, 1.0L / M);
+    if ((rc = dev_upload(&f.bfil, bf.data(), bf.size() * sizeof(double)))) return rc;
+    if ((rc = dev_upload(&f.perm, perm.data(), perm.size() * sizeof(int)))) return rc;
   }
   return SPX_OK;
 }
@@ -726,7 +863,8 @@ void free_line_fft(LineFFTHost& f) {
   if (f.tw) (void)hipFree(f.tw);
   if (f.chirp) (void)hipFree(f.chirp);
   if (f.bfil) (void)hipFree(f.bfil);
-  f.tw = f.chirp = f.bfil = nullptr;
+  if (f.perm) (void)hipFree(f.perm);
+  f.tw = f.chirp = f.bfil = f.perm = nullptr;
 }
 
 // ---------------------------------------------------------------------------
@@ -743,13 +881,41 @@ static int lds_limit(K kernel, size_t bytes) {
   return SPX_OK;
 }
 
+// radix set of a plan: the smallest compiled kernel family that covers it
+static int rset_of(const LineFFTHost& f) {
+  int rs = RS_POW2;
+  for (int i = 0; i < f.nstage; ++i) {
+    const int R = f.radix[i];
+    if (R == 11 || R == 13) rs = RS_ALL;
+    else if (R != 2 && R != 4 && R != 8 && rs < RS_SMOOTH) rs = RS_SMOOTH;
+  }
+  return rs;
+}
+
+#define SPX_COMMA ,
+// launch the instance of KER compiled for the plan's radix set
+#define SPX_LAUNCH_RS(F, KER, TARGS, GRID, BLOCK, LDS, STREAM, ...)                   \
+  do {                                                                               \
+    const int rs_ = rset_of(F);                                                      \
+    if (rs_ == RS_POW2) {                                                            \
+      if (int rc_ = lds_limit(KER<RS_POW2 TARGS>, (LDS))) return rc_;                \
+      hipLaunchKernelGGL((KER<RS_POW2 TARGS>), GRID, BLOCK, LDS, STREAM, __VA_ARGS__); \
+    } else if (rs_ == RS_SMOOTH) {                                                   \
+      if (int rc_ = lds_limit(KER<RS_SMOOTH TARGS>, (LDS))) return rc_;              \
+      hipLaunchKernelGGL((KER<RS_SMOOTH TARGS>), GRID, BLOCK, LDS, STREAM, __VA_ARGS__); \
+    } else {                                                                         \
+      if (int rc_ = lds_limit(KER<RS_ALL TARGS>, (LDS))) return rc_;                 \
+      hipLaunchKernelGGL((KER<RS_ALL TARGS>), GRID, BLOCK, LDS, STREAM, __VA_ARGS__); \
+    }                                                                                \
+  } while (0)
+
 int zfft_inplace(spx_plan* p, void* S, int forward, int cont, double scale) {
   const LineFFTHost& f = p->fz;
   const long nlines = long(p->g.ny) * p->nkx;
   const unsigned nb = unsigned((nlines + f.nl - 1) / f.nl);
   const int C = p->g.Cz, d = p->g.oz;
-  ProfScope ps_(p, "z_inplace", double(p->nc) * (16.0 + 16.0));
-  hipLaunchKernelGGL(k_z_inplace, dim3(nb), blk(f), f.lds_bytes(p->csize), p->stream,
+  ProfScope ps_(p, "z_inplace", double(p->nc) * 32.0);
+  SPX_LAUNCH_RS(f, k_z_inplace,, dim3(nb), blk(f), f.lds_bytes(p->csize), p->stream,
                      static_cast<Cp*>(S), nlines, f.dev<Cp>(), p->np, forward,
                      (cont && C > 0) ? 1 : 0, scale, p->d_dir, C, d);
   SPX_CHECK_LAUNCH();
@@ -771,31 +937,53 @@ int fc_continue_only(spx_plan* p, void* S) {
 int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft) {
   const LineFFTHost& f = p->fz;
   dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(p->g.ny));
-  ProfScope ps_(p, "z_x2s", double(p->nkx) * p->g.ny * 16.0 * ((nofft ? p->g.nz : p->np) + p->g.nz));
-  hipLaunchKernelGGL(k_z_x2s, grid, blk(f), f.lds_bytes(p->csize), p->stream,
+  ProfScope ps_(p, "z_x2s",
+                double(p->nkx) * p->g.ny * 16.0 * ((nofft ? p->g.nz : p->np) + p->g.nz));
+  SPX_LAUNCH_RS(f, k_z_x2s,, grid, blk(f), f.lds_bytes(p->csize), p->stream,
                      static_cast<const Cp*>(X), static_cast<Cp*>(S), f.dev<Cp>(), p->nkx,
                      p->g.ny, p->np, nofft, p->d_dir, p->g.Cz, p->g.oz);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
-int zfft_s2x(spx_plan* p, const void* S, void* X, int dirn, int nzout, int nofft) {
+int zfft_s2x(spx_plan* p, const void* S, void* X, int dirn, int nzout, int nofft, void* X2) {
   const LineFFTHost& f = p->fz;
   dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(p->g.ny));
-  ProfScope ps_(p, "z_s2x", double(p->nkx) * p->g.ny * 16.0 * (p->g.nz + nzout));
-  hipLaunchKernelGGL(k_z_s2x, grid, blk(f), f.lds_bytes(p->csize), p->stream,
-                     static_cast<const Cp*>(S), static_cast<Cp*>(X), f.dev<Cp>(), p->nkx,
-                     p->g.ny, nzout, dirn, nofft, p->d_kx, p->d_ky, p->d_kz);
+  const double io = double(p->nkx) * p->g.ny * 16.0;
+  if (X2) {
+    ProfScope ps_(p, "z_s2x_dual", io * (p->g.nz + 2.0 * nzout));
+    SPX_LAUNCH_RS(f, k_z_s2x, SPX_COMMA true, grid, blk(f), f.lds_bytes(p->csize), p->stream,
+                       static_cast<const Cp*>(S), static_cast<Cp*>(X), static_cast<Cp*>(X2),
+                       f.dev<Cp>(), p->nkx, p->g.ny, nzout, dirn, nofft, p->d_kx, p->d_ky,
+                       p->d_kz);
+  } else {
+    ProfScope ps_(p, "z_s2x", io * (p->g.nz + nzout));
+    SPX_LAUNCH_RS(f, k_z_s2x, SPX_COMMA false, grid, blk(f), f.lds_bytes(p->csize), p->stream,
+                       static_cast<const Cp*>(S), static_cast<Cp*>(X), static_cast<Cp*>(nullptr),
+                       f.dev<Cp>(), p->nkx, p->g.ny, nzout, dirn, nofft, p->d_kx, p->d_ky,
+                       p->d_kz);
+  }
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
-int yfft(spx_plan* p, void* X, int forward, int nplanes) {
+int yfft(spx_plan* p, void* X, int forward, int nplanes, const void* Xin, void* Y2) {
   const LineFFTHost& f = p->fy;
   dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(nplanes));
-  ProfScope ps_(p, "y", double(p->nkx) * p->g.ny * nplanes * 32.0);
-  hipLaunchKernelGGL(k_y, grid, blk(f), f.lds_bytes(p->csize), p->stream, static_cast<Cp*>(X),
-                     f.dev<Cp>(), p->nkx, p->g.ny, forward);
+  const Cp* in = static_cast<const Cp*>(Xin ? Xin : X);
+  const double io = double(p->nkx) * p->g.ny * nplanes * 16.0;
+  if (Y2) {
+    if (forward) return fail(SPX_ERR_ARG, "dual y transform is backward only");
+    ProfScope ps_(p, "y_dual", io * 3.0);
+    SPX_LAUNCH_RS(f, k_y, SPX_COMMA true, grid, blk(f), f.lds_bytes(p->csize), p->stream, in,
+                       static_cast<Cp*>(X), static_cast<Cp*>(Y2), f.dev<Cp>(), p->nkx, p->g.ny,
+                       forward, p->d_ky);
+  } else {
+    ProfScope ps_(p, "y", io * 2.0);
+    SPX_LAUNCH_RS(f, k_y, SPX_COMMA false, grid, blk(f), f.lds_bytes(p->csize), p->stream, in,
+                       static_cast<Cp*>(X), static_cast<Cp*>(nullptr), f.dev<Cp>(), p->nkx,
+                       p->g.ny, forward, p->d_ky);
+  }
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
@@ -805,19 +993,37 @@ int xr2c(spx_plan* p, const void* R, void* X, int nplanes) {
   const long npairs = long(p->g.ny) * nplanes / 2;
   dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
   ProfScope ps_(p, "xr2c", double(p->g.ny) * nplanes * (8.0 * p->g.nx + 16.0 * p->nkx));
-  hipLaunchKernelGGL(k_xr2c, grid, blk(f), f.lds_bytes(p->csize), p->stream,
+  SPX_LAUNCH_RS(f, k_xr2c,, grid, blk(f), f.lds_bytes(p->csize), p->stream,
                      static_cast<const Tp*>(R), static_cast<Cp*>(X), f.dev<Cp>(), p->nkx, npairs);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
-int xc2r(spx_plan* p, const void* X, void* R, int nplanes) {
+int xc2r(spx_plan* p, const void* X, void* R, int nplanes, int dirx) {
   const LineFFTHost& f = p->fx;
   const long npairs = long(p->g.ny) * nplanes / 2;
   dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
   ProfScope ps_(p, "xc2r", double(p->g.ny) * nplanes * (8.0 * p->g.nx + 16.0 * p->nkx));
-  hipLaunchKernelGGL(k_xc2r, grid, blk(f), f.lds_bytes(p->csize), p->stream,
-                     static_cast<const Cp*>(X), static_cast<Tp*>(R), f.dev<Cp>(), p->nkx, npairs);
+  SPX_LAUNCH_RS(f, k_xc2r,, grid, blk(f), f.lds_bytes(p->csize), p->stream,
+                     static_cast<const Cp*>(X), static_cast<Tp*>(R), f.dev<Cp>(), p->nkx, npairs,
+                     dirx, p->d_kx);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+
+int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, const void* u,
+               const void* v, const void* w, void* out) {
+  const LineFFTHost& f = p->fx;
+  const long npairs = long(p->g.ny) * p->np / 2;
+  dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
+  const double t = double(p->g.nx) * double(p->g.ny) * double(p->g.nz);
+  const double rows = double(p->g.ny) * p->np;
+  ProfScope ps_(p, "xgrad_prod", rows * (3.0 * 16.0 * p->nkx + 4.0 * 8.0 * p->g.nx));
+  SPX_LAUNCH_RS(f, k_xgrad_prod,, grid, blk(f), f.lds_bytes(p->csize), p->stream,
+                     static_cast<const Cp*>(YV), static_cast<const Cp*>(YY),
+                     static_cast<const Cp*>(XZ), static_cast<const Tp*>(u),
+                     static_cast<const Tp*>(v), static_cast<const Tp*>(w), static_cast<Tp*>(out),
+                     f.dev<Cp>(), p->nkx, npairs, 1.0 / (t * t), p->d_kx);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
@@ -836,7 +1042,7 @@ int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
   wp.scalar = 0;
   dim3 grid(unsigned((nlines + f.nl - 1) / f.nl), 2);
   ProfScope ps_(p, "noslip", 4.0 * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_noslip, grid, blk(f), f.lds_bytes(p->csize), p->stream,
+  SPX_LAUNCH_RS(f, k_noslip,, grid, blk(f), f.lds_bytes(p->csize), p->stream,
                      static_cast<Cp*>(vx), static_cast<Cp*>(vy), static_cast<const Cp*>(pr),
                      f.dev<Cp>(), nlines, p->g.ny, p->np, 1.0 / double(p->g.nz), wp, p->d_kx,
                      p->d_ky, p->d_dir, p->g.Cz, p->g.oz);
@@ -852,7 +1058,7 @@ int sconst_fused(spx_plan* p, void* th) {
   wp.scalar = 1;
   dim3 grid(unsigned((nlines + f.nl - 1) / f.nl), 1);
   ProfScope ps_(p, "sconst", 2.0 * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_noslip, grid, blk(f), f.lds_bytes(p->csize), p->stream,
+  SPX_LAUNCH_RS(f, k_noslip,, grid, blk(f), f.lds_bytes(p->csize), p->stream,
                      static_cast<Cp*>(th), static_cast<Cp*>(th), static_cast<const Cp*>(nullptr),
                      f.dev<Cp>(), nlines, p->g.ny, p->np, 1.0 / double(p->g.nz), wp, p->d_kx,
                      p->d_ky, p->d_dir, p->g.Cz, p->g.oz);
@@ -864,7 +1070,7 @@ int proj_bc(spx_plan* p, const void* c) {
   const LineFFTHost& f = p->fz;
   const long nlines = long(p->g.ny) * p->nkx;
   ProfScope ps_(p, "proj_bc", double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_proj_bc, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
+  SPX_LAUNCH_RS(f, k_proj_bc,, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
                      f.lds_bytes(p->csize), p->stream, static_cast<const Cp*>(c),
                      static_cast<Cp*>(p->bc), f.dev<Cp>(), nlines, p->np,
                      1.0 / double(p->g.nz));
@@ -876,7 +1082,7 @@ int proj_pressure(spx_plan* p, void* d) {
   const LineFFTHost& f = p->fz;
   const long nlines = long(p->g.ny) * p->nkx;
   ProfScope ps_(p, "proj_pressure", 2.0 * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_proj_pressure, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
+  SPX_LAUNCH_RS(f, k_proj_pressure,, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
                      f.lds_bytes(p->csize), p->stream, static_cast<Cp*>(d),
                      static_cast<const Cp*>(p->bc), f.dev<Cp>(), nlines, p->g.ny,
                      1.0 / double(p->g.nz), p->g.Lz, p->d_khom, p->d_z);
@@ -887,13 +1093,12 @@ int proj_pressure(spx_plan* p, void* d) {
 int proj_harmonic(spx_plan* p, void* a, void* b, void* c) {
   const LineFFTHost& f = p->fz2;
   const long nlines = long(p->g.ny) * p->nkx;
-  if (int rc = lds_limit(k_proj_harmonic, 2 * f.lds_bytes(p->csize))) return rc;
+  const size_t lds = f.lds_bytes(p->csize, 2);
   ProfScope ps_(p, "proj_harmonic", 6.0 * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_proj_harmonic, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
-                     2 * f.lds_bytes(p->csize), p->stream, static_cast<Cp*>(a),
-                     static_cast<Cp*>(b), static_cast<Cp*>(c), static_cast<const Cp*>(p->bc),
-                     f.dev<Cp>(), nlines, p->g.ny, p->np, p->g.Lz, p->d_khom, p->d_z, p->d_kx,
-                     p->d_ky, p->d_dir, p->g.Cz, p->g.oz);
+  SPX_LAUNCH_RS(f, k_proj_harmonic,, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f), lds,
+                     p->stream, static_cast<Cp*>(a), static_cast<Cp*>(b), static_cast<Cp*>(c),
+                     static_cast<const Cp*>(p->bc), f.dev<Cp>(), nlines, p->g.ny, p->np, p->g.Lz,
+                     p->d_khom, p->d_z, p->d_kx, p->d_ky, p->d_dir, p->g.Cz, p->g.oz);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
@@ -901,10 +1106,9 @@ int proj_harmonic(spx_plan* p, void* a, void* b, void* c) {
 static int zline_launch(spx_plan* p, const LinCombDev& lc, const void* b, int mode, double* out) {
   const LineFFTHost& f = mode == 1 ? p->fz2 : p->fz;
   const long nlines = long(p->g.ny) * p->nkx;
-  size_t lds = f.lds_bytes(p->csize) * (mode == 1 ? 2 : 1) + size_t(f.nl) * f.tpl * sizeof(double);
-  if (int rc = lds_limit(k_zline_reduce, lds)) return rc;
+  const size_t lds = f.lds_bytes(p->csize, mode == 1 ? 2 : 1) + size_t(f.nl) * f.tpl * sizeof(double);
   ProfScope ps_(p, "zline_reduce", double(p->nc) * 16.0 * (mode == 1 ? 2 : 1));
-  hipLaunchKernelGGL(k_zline_reduce, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f), lds,
+  SPX_LAUNCH_RS(f, k_zline_reduce,, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f), lds,
                      p->stream, lc, static_cast<const Cp*>(b), mode, out, f.dev<Cp>(), nlines,
                      p->g.ny, p->np, p->d_kx, p->d_ky, p->d_kz);
   SPX_CHECK_LAUNCH();

@@ -72,37 +72,6 @@ __global__ void k_div3(double* __restrict__ r, long n, double a, double b, doubl
   if (e < n) r[e] = ((r[e] / a) / b) / c;
 }
 
-// gradre / advect product accumulation on physical planes (pseudospec_hd.f90:254-312)
-__global__ void k_prod_stage(int stage, const double* __restrict__ r1,
-                             const double* __restrict__ r2, const double* __restrict__ r3,
-                             const double* __restrict__ r4, double* __restrict__ rx,
-                             double* __restrict__ ry, double* __restrict__ rz, long n, double tmp,
-                             int nout) {
-#pragma clang fp contract(off)
-  const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  const double a = r1[e];
-  if (stage == 1) {
-    rx[e] = a * r2[e];
-    if (nout == 3) {
-      ry[e] = a * r3[e];
-      rz[e] = a * r4[e];
-    }
-  } else if (stage == 2) {
-    rx[e] = rx[e] + a * r2[e];
-    if (nout == 3) {
-      ry[e] = ry[e] + a * r3[e];
-      rz[e] = rz[e] + a * r4[e];
-    }
-  } else {
-    rx[e] = (rx[e] + a * r2[e]) * tmp;
-    if (nout == 3) {
-      ry[e] = (ry[e] + a * r3[e]) * tmp;
-      rz[e] = (rz[e] + a * r4[e]) * tmp;
-    }
-  }
-}
-
 // Fused fc_filter(N) + laplak(v) + RK stage update (include/hd/hd_rkstep2.f90:11-32):
 //   v = C0 + dt*(nu*(-kk2 v) - ((N*fx)*fy)*fz + F)*rmp
 // with the Boussinesq source N <- N - coef*src applied first
@@ -231,21 +200,6 @@ int op_real_div3(spx_plan* p, void* r, double a, double b, double c) {
   ProfScope ps_(p, "real_div3", 16.0 * double(n));
   hipLaunchKernelGGL(k_div3, dim3(nblk(n)), dim3(256), 0, p->stream, static_cast<double*>(r), n,
                      a, b, c);
-  SPX_CHECK_LAUNCH();
-  return SPX_OK;
-}
-
-int op_prod_stage(spx_plan* p, int stage, const void* r1, const void* r2, const void* r3,
-                  const void* r4, void* rx, void* ry, void* rz, int nout) {
-  const long n = long(p->g.nx) * p->g.ny * p->np;
-  const double t = double(p->g.nx) * double(p->g.ny) * double(p->g.nz);
-  const double tmp = 1.0 / (t * t);
-  ProfScope ps_(p, "prod_stage", 8.0 * double(n) * (nout == 3 ? (stage == 1 ? 7 : 10) : (stage == 1 ? 3 : 4)));
-  hipLaunchKernelGGL(k_prod_stage, dim3(nblk(n)), dim3(256), 0, p->stream, stage,
-                     static_cast<const double*>(r1), static_cast<const double*>(r2),
-                     static_cast<const double*>(r3), static_cast<const double*>(r4),
-                     static_cast<double*>(rx), static_cast<double*>(ry), static_cast<double*>(rz),
-                     n, tmp, nout);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }

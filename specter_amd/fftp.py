@@ -58,7 +58,7 @@ class FCPlan:
         self.nx, self.ny, self.nz, self.Cz, self.oz = nx, ny, nz, Cz, oz
         self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
         self.nkx, self.np = info.nkx, info.np
-        self.z_fft_len, self.z_bluestein = info.z_fft_len, info.z_bluestein
+        self.z_fft_len, self.z_algorithm = info.z_fft_len, info.z_algorithm
 
     # -- lifetime ---------------------------------------------------------------
     def close(self):

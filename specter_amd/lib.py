@@ -30,7 +30,7 @@ class SpxGrid(ctypes.Structure):
 class SpxPlanInfo(ctypes.Structure):
     _fields_ = [("nkx", c_int), ("np", c_int), ("n_spectral", c_size_t), ("n_real", c_size_t),
                 ("bytes_spectral", c_size_t), ("bytes_real", c_size_t), ("z_fft_len", c_int),
-                ("z_bluestein", c_int), ("device_bytes", c_size_t)]
+                ("z_algorithm", c_int), ("device_bytes", c_size_t)]
 
 
 class SpxParams(ctypes.Structure):
