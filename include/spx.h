@@ -169,6 +169,7 @@ int spx_dev_alloc(size_t bytes, void** ptr);
 int spx_dev_free(void* ptr);
 int spx_memcpy_h2d(spx_plan* p, void* dst, const void* src, size_t bytes);  /* synchronous */
 int spx_memcpy_d2h(spx_plan* p, void* dst, const void* src, size_t bytes);  /* synchronous */
+int spx_memcpy_d2d(spx_plan* p, void* dst, const void* src, size_t bytes);  /* on the plan's stream */
 
 /* Last error message of the calling thread; returns its length. */
 int spx_last_error(char* buf, size_t len);

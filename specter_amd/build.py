@@ -33,8 +33,41 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: libspx cannot be built")
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_BIN = os.path.join(FORTRAN_DIR, "spx_hd")
+
+
+def amdflang() -> str | None:
+    for cand in (os.environ.get("FC"), "/opt/rocm/bin/amdflang", shutil.which("amdflang")):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """Compile the Fortran binding module and the SPX_HD driver against
+    libspx.so (skipped when no Fortran compiler is installed)."""
+    fc = amdflang()
+    if fc is None:
+        return None
+    bdir = os.path.join(FORTRAN_DIR, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    cmd = [fc, "-O2", "-J", bdir, os.path.join(FORTRAN_DIR, "spx_bindings.f90"),
+           os.path.join(FORTRAN_DIR, "spx_hd.f90"), "-L", HERE, "-lspx",
+           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib", "-o", FORTRAN_BIN + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True, cwd=bdir)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("amdflang failed building the SPX_HD driver")
+    os.replace(FORTRAN_BIN + ".tmp", FORTRAN_BIN)
+    return FORTRAN_BIN
+
+
 def build(verbose: bool = False) -> str:
-    """Compile every HIP source into specter_amd/libspx.so (always rebuilds)."""
+    """Compile every HIP source into specter_amd/libspx.so (always rebuilds),
+    then the Fortran driver that links it."""
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
     cmd = [hipcc(), *HIPCC_FLAGS, "-I", CSRC, *srcs, "-o", tmp]
@@ -45,6 +78,7 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libspx.so")
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
 
 

@@ -735,6 +735,12 @@ int spx_memcpy_h2d(spx_plan* p, void* dst, const void* src, size_t bytes) {
   return SPX_OK;
 }
 
+int spx_memcpy_d2d(spx_plan* p, void* dst, const void* src, size_t bytes) {
+  SPX_REQ(p && dst && src, "spx_memcpy_d2d: null argument");
+  SPX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, p->stream));
+  return SPX_OK;
+}
+
 int spx_memcpy_d2h(spx_plan* p, void* dst, const void* src, size_t bytes) {
   SPX_REQ(p && dst && src, "spx_memcpy_d2h: null argument");
   SPX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, p->stream));

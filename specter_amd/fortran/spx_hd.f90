@@ -1,0 +1,212 @@
+!=======================================================================
+! SPX_HD: SPECTER-style HD driver over libspx (device-resident state).
+!
+! Reads SPECTER's parameter.inp namelists (&status, &boxparams, &parameter,
+! &velocity, &velbound; specter.fpp:201-239) plus a &grid group that stands
+! in for the compile-time NX/NY/NZ/CZ/OZ/ORD of Makefile.in.  Starts like a
+! reference restart with stat >= 1 (specter.fpp:885-934): the physical-space
+! fields vx, vy, vz (and optionally fx, fy, fz, pr) are read from
+! idir/<name>.NNNN.out as raw float64 arrays (nx, ny, nz-Cz) in Fortran order
+! (mpiio/binary_io.f90:127-151).  The RK loop (specter.fpp:1002-1165) runs on
+! the GPU through libspx; every cstep steps the global quantities of
+! hd_global.f90 are appended to balance.txt (format of pseudospec_hd.f90:993)
+! and the no-slip diagnostics to noslip_diagnostic.txt at full precision.
+! At the end the velocity is written to odir/<name>.MMMM.out (MMMM = stat+1).
+!=======================================================================
+PROGRAM spx_hd
+   USE, INTRINSIC :: iso_c_binding
+   USE spx
+   IMPLICIT NONE
+
+   INTEGER :: nx = 64, ny = 64, nz = 89, Cz = 25, oz = 5, ord = 2
+   CHARACTER(len=100) :: idir = 'in/', odir = 'out/', tdir = 'tables/'
+   INTEGER :: stat = 1, mult = 1, bench = 0, outs = 0
+   REAL(8) :: Lx = 1.d0, Ly = 1.d0, Lz = 1.d0
+   REAL(8) :: dt = 1.d-3
+   INTEGER :: step = 10, tstep = 100, cstep = 5, seed = 1000
+   REAL(8) :: f0 = 0.d0, u0 = 1.d0, kdn = 2.d0, kup = 4.d0, nu = 1.d-3
+   CHARACTER(len=15) :: vbczsta = 'noslip', vbczend = 'noslip'
+   REAL(8) :: vxzsta = 0.d0, vxzend = 0.d0, vyzsta = 0.d0, vyzend = 0.d0
+   NAMELIST /grid/ nx, ny, nz, Cz, oz, ord
+   NAMELIST /status/ idir, odir, tdir, stat, mult, bench, outs
+   NAMELIST /boxparams/ Lx, Ly, Lz
+   NAMELIST /parameter/ dt, step, tstep, cstep, seed
+   NAMELIST /velocity/ f0, u0, kdn, kup, nu
+   NAMELIST /velbound/ vbczsta, vbczend, vxzsta, vxzend, vyzsta, vyzend
+
+   TYPE(spx_grid) :: g
+   TYPE(spx_params) :: prm
+   TYPE(C_PTR) :: plan, state, dreal, dtmp
+   TYPE(C_PTR) :: vx, vy, vz, pr, fx, fy, fz
+   REAL(8), ALLOCATABLE, TARGET :: R(:, :, :), A(:, :), Q(:, :)
+   REAL(8) :: eng, ens, pot, dv, tb, tt, nb, nt
+   INTEGER :: t, timec, np, ios
+   INTEGER(C_SIZE_T) :: rbytes, cbytes
+   CHARACTER(len=4) :: ext
+   LOGICAL :: have
+
+   OPEN(1, file='parameter.inp', status='old', form='formatted')
+   READ(1, NML=grid)
+   REWIND(1)
+   READ(1, NML=status)
+   REWIND(1)
+   READ(1, NML=boxparams)
+   REWIND(1)
+   READ(1, NML=parameter)
+   REWIND(1)
+   READ(1, NML=velocity)
+   REWIND(1)
+   READ(1, NML=velbound, IOSTAT=ios)
+   CLOSE(1)
+   IF (TRIM(vbczsta) .ne. 'noslip' .OR. TRIM(vbczend) .ne. 'noslip') THEN
+      PRINT *, '[ERROR] only no-slip walls in z are supported. Aborting...'
+      STOP 1
+   ENDIF
+   IF (stat .lt. 1) THEN
+      PRINT *, '[ERROR] SPX_HD starts from binary fields (stat >= 1). Aborting...'
+      STOP 1
+   ENDIF
+   np = nz - Cz
+
+   ! FC-Gram tables (fcgram_mod.f90:221-244)
+   ALLOCATE(A(Cz, oz), Q(oz, oz))
+   CALL read_table(TRIM(tdir)//'/A'//itoa(Cz)//'-'//itoa(oz)//'.dat', A, Cz*oz)
+   CALL read_table(TRIM(tdir)//'/Q'//itoa(oz)//'.dat', Q, oz*oz)
+
+   g = spx_grid(nx, ny, nz, Cz, oz, Lx, Ly, Lz, SPX_FP64)
+   CALL spx_check(spx_plan_create(g, A, Q, C_NULL_PTR, plan), 'spx_plan_create')
+   prm = spx_params(SPX_SOLVER_HD, ord, dt, nu, 0.d0, 0.d0, 0.d0, &
+                    (/vxzsta, vyzsta/), (/vxzend, vyzend/))
+   CALL spx_check(spx_state_create(plan, prm, state), 'spx_state_create')
+   CALL spx_check(spx_state_field(state, 'vx'//C_NULL_CHAR, vx), 'field vx')
+   CALL spx_check(spx_state_field(state, 'vy'//C_NULL_CHAR, vy), 'field vy')
+   CALL spx_check(spx_state_field(state, 'vz'//C_NULL_CHAR, vz), 'field vz')
+   CALL spx_check(spx_state_field(state, 'pr'//C_NULL_CHAR, pr), 'field pr')
+   CALL spx_check(spx_state_field(state, 'fx'//C_NULL_CHAR, fx), 'field fx')
+   CALL spx_check(spx_state_field(state, 'fy'//C_NULL_CHAR, fy), 'field fy')
+   CALL spx_check(spx_state_field(state, 'fz'//C_NULL_CHAR, fz), 'field fz')
+
+   ALLOCATE(R(nx, ny, nz))
+   rbytes = INT(nx, C_SIZE_T) * ny * nz * 8
+   cbytes = INT(nz, C_SIZE_T) * ny * (nx / 2 + 1) * 16
+   CALL spx_check(spx_dev_alloc(rbytes, dreal), 'spx_dev_alloc')
+   CALL spx_check(spx_dev_alloc(cbytes, dtmp), 'spx_dev_alloc')
+
+   ! restart: physical fields -> fftp3d_real_to_complex (specter.fpp:894-899)
+   WRITE(ext, '(i4.4)') stat
+   CALL load_field('vx', .TRUE., vx, have)
+   CALL load_field('vy', .TRUE., vy, have)
+   CALL load_field('vz', .TRUE., vz, have)
+   CALL load_field('fx', .FALSE., fx, have)
+   CALL load_field('fy', .FALSE., fy, have)
+   CALL load_field('fz', .FALSE., fz, have)
+   ! pressure p' = dt*p in the mixed domain (specter.fpp:901-910)
+   CALL read_raw(TRIM(idir)//'/pr.'//ext//'.out', have)
+   IF (have) THEN
+      R(:, :, np+1:nz) = 0.d0
+      R = R * dt
+      CALL spx_check(spx_memcpy_h2d(plan, dreal, C_LOC(R), rbytes), 'h2d pr')
+      CALL spx_check(spx_fftp2d_real_to_complex_xy(plan, dreal, pr), 'pr r2c')
+   ENDIF
+
+   ! RK loop with restart counters (specter.fpp:885-891, 1002-1165)
+   timec = 0
+   OPEN(2, file='balance.txt', position='append')
+   OPEN(3, file='noslip_diagnostic.txt', position='append')
+   DO t = 1, step
+      IF (timec .eq. cstep) THEN
+         timec = 0
+         CALL spx_check(spx_energy(plan, vx, vy, vz, 1, eng), 'energy')
+         CALL spx_check(spx_energy(plan, vx, vy, vz, 0, ens), 'enstrophy')
+         CALL spx_check(spx_cross(plan, vx, vy, vz, fx, fy, fz, pot), 'cross')
+         WRITE(2, '(1P E13.6,2E23.16,E24.16)') (t-1)*dt, eng, ens, pot
+         CALL spx_check(spx_divergence(plan, vx, vy, vz, dv), 'divergence')
+         CALL spx_check(spx_bouncheck_z(plan, vx, vy, tb, tt), 'bouncheck')
+         CALL spx_check(spx_bouncheck_z(plan, vz, C_NULL_PTR, nb, nt), 'bouncheck')
+         WRITE(3, '(1P 6E24.16)') (t-1)*dt, dv, tb, tt, nb, nt
+      ENDIF
+      CALL spx_check(spx_step(plan, state, 1), 'spx_step')
+      timec = timec + 1
+   ENDDO
+   CLOSE(2)
+   CLOSE(3)
+
+   ! output (specter.fpp:1005-1040): v/(nx ny nz) in physical space
+   WRITE(ext, '(i4.4)') stat + 1
+   CALL store_field('vx', vx)
+   CALL store_field('vy', vy)
+   CALL store_field('vz', vz)
+
+   CALL spx_check(spx_plan_sync(plan), 'sync')
+   CALL spx_check(spx_dev_free(dreal), 'free')
+   CALL spx_check(spx_dev_free(dtmp), 'free')
+   CALL spx_check(spx_state_destroy(state), 'state destroy')
+   CALL spx_check(spx_plan_destroy(plan), 'plan destroy')
+
+CONTAINS
+
+   FUNCTION itoa(i) RESULT(s)
+      INTEGER, INTENT(IN) :: i
+      CHARACTER(len=:), ALLOCATABLE :: s
+      CHARACTER(len=12) :: buf
+      WRITE(buf, '(I0)') i
+      s = TRIM(buf)
+   END FUNCTION itoa
+
+   SUBROUTINE read_table(path, T, n)
+      CHARACTER(len=*), INTENT(IN) :: path
+      INTEGER, INTENT(IN) :: n
+      REAL(8), INTENT(OUT) :: T(n)
+      LOGICAL :: ex
+      INQUIRE(FILE=path, EXIST=ex)
+      IF (.NOT. ex) THEN
+         PRINT *, 'Could not find table ', path
+         STOP 1
+      ENDIF
+      OPEN(10, FILE=path, FORM='unformatted', ACCESS='stream', ACTION='read')
+      READ(10) T
+      CLOSE(10)
+   END SUBROUTINE read_table
+
+   SUBROUTINE read_raw(path, found)
+      CHARACTER(len=*), INTENT(IN) :: path
+      LOGICAL, INTENT(OUT) :: found
+      INQUIRE(FILE=path, EXIST=found)
+      IF (.NOT. found) RETURN
+      R = 0.d0
+      OPEN(11, FILE=path, FORM='unformatted', ACCESS='stream', ACTION='read')
+      READ(11) R(:, :, 1:np)
+      CLOSE(11)
+   END SUBROUTINE read_raw
+
+   SUBROUTINE load_field(name, required, dst, found)
+      CHARACTER(len=*), INTENT(IN) :: name
+      LOGICAL, INTENT(IN) :: required
+      TYPE(C_PTR), INTENT(IN) :: dst
+      LOGICAL, INTENT(OUT) :: found
+      CALL read_raw(TRIM(idir)//'/'//name//'.'//ext//'.out', found)
+      IF (.NOT. found) THEN
+         IF (required) THEN
+            PRINT *, '[ERROR] missing input field ', name
+            STOP 1
+         ENDIF
+         RETURN
+      ENDIF
+      CALL spx_check(spx_memcpy_h2d(plan, dreal, C_LOC(R), rbytes), 'h2d '//name)
+      CALL spx_check(spx_fftp3d_real_to_complex(plan, dreal, dst), 'r2c '//name)
+   END SUBROUTINE load_field
+
+   SUBROUTINE store_field(name, src)
+      CHARACTER(len=*), INTENT(IN) :: name
+      TYPE(C_PTR), INTENT(IN) :: src
+      CALL spx_check(spx_memcpy_d2d(plan, dtmp, src, cbytes), 'd2d '//name)
+      CALL spx_check(spx_fftp3d_complex_to_real(plan, dtmp, dreal), 'c2r '//name)
+      CALL spx_check(spx_memcpy_d2h(plan, C_LOC(R), dreal, rbytes), 'd2h '//name)
+      R = R / (REAL(nx, 8) * REAL(ny, 8) * REAL(nz, 8))
+      OPEN(12, FILE=TRIM(odir)//'/'//name//'.'//ext//'.out', FORM='unformatted', &
+           ACCESS='stream', STATUS='replace')
+      WRITE(12) R(:, :, 1:np)
+      CLOSE(12)
+   END SUBROUTINE store_field
+
+END PROGRAM spx_hd

@@ -79,6 +79,7 @@ _SIGS = {
     "spx_dev_free": [_P],
     "spx_memcpy_h2d": [_P, _P, _P, c_size_t],
     "spx_memcpy_d2h": [_P, _P, _P, c_size_t],
+    "spx_memcpy_d2d": [_P, _P, _P, c_size_t],
     "spx_last_error": [ctypes.c_char_p, c_size_t],
 }
 

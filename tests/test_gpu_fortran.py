@@ -1,0 +1,93 @@
+"""The Fortran driver (specter_amd/fortran/spx_hd.f90) calls libspx through
+the iso_c_binding module with SPECTER's parameter.inp surface; its balance.txt
+series and output fields must match the same run driven from Python and the
+float64 oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, TABLES, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(ROOT, "specter_amd", "fortran", "spx_hd")
+
+PARAM = """&grid
+nx = {nx}, ny = {ny}, nz = {nz}, Cz = {C}, oz = {d}, ord = {ord}
+/
+&status
+idir = "in", odir = "out", tdir = "{tdir}", stat = 1, mult = 1, bench = 0, outs = 0
+/
+&boxparams
+Lx = 1.0, Ly = 1.0, Lz = 1.0
+/
+&parameter
+dt = {dt}, step = {step}, tstep = 1000, cstep = {cstep}, seed = 1000
+/
+&velocity
+f0 = 0.0, u0 = 1.0, kdn = 2.0, kup = 4.0, nu = {nu}
+/
+&velbound
+vbczsta = "noslip", vbczend = "noslip", vxzsta = 0.0, vxzend = 0.0, vyzsta = 0.0, vyzend = 0.0
+/
+"""
+
+
+def test_fortran_driver_matches_python_and_oracle(tmp_path):
+    from oracle import specter_ref as O
+    from specter_amd import FCPlan, Solver, SolverParams, ic
+    if not os.path.exists(BIN):
+        pytest.fail("Fortran driver not built (specter_amd.build.build_fortran)")
+    nx, ny, nz, C, d, ordr, dt, nu, step, cstep = 32, 32, 57, 25, 5, 2, 1e-3, 2e-3, 12, 4
+    npz = nz - C
+    (tmp_path / "in").mkdir()
+    (tmp_path / "out").mkdir()
+    (tmp_path / "parameter.inp").write_text(PARAM.format(
+        nx=nx, ny=ny, nz=nz, C=C, d=d, ord=ordr, tdir=TABLES, dt=dt, step=step,
+        cstep=cstep, nu=nu))
+    u = ic.spectrum_peaked_velocity(nx, ny, nz, C, seed=1000)
+    fr = ic.large_scale_forcing(nx, ny, nz, C)
+    for name, arr in zip(("vx", "vy", "vz", "fx", "fy", "fz"), (*u, *fr)):
+        # raw float64, Fortran order (nx, ny, nz-Cz) == C order (nz-Cz, ny, nx)
+        np.ascontiguousarray(arr[:npz]).tofile(tmp_path / "in" / f"{name}.0001.out")
+    res = subprocess.run([BIN], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    bal = np.loadtxt(tmp_path / "balance.txt", ndmin=2)
+    nsd = np.loadtxt(tmp_path / "noslip_diagnostic.txt", ndmin=2)
+    assert bal.shape[0] == (step - 1) // cstep
+
+    # the same run from Python through the same library
+    plan = FCPlan(nx, ny, nz, C, d)
+    sol = Solver(plan, SolverParams(solver="HD", ord=ordr, dt=dt, nu=nu))
+    for name, arr in zip(("vx", "vy", "vz", "fx", "fy", "fz"), (*u, *fr)):
+        sol.set_real_field(name, arr)
+    # and the oracle
+    g = O.make_grid(nx, ny, nz, C, d, TABLES)
+    st = O.HDState(*(O.fftp3d_real_to_complex(g, a.copy()) for a in u),
+                   np.zeros(g.spec_shape, complex),
+                   *(O.fftp3d_real_to_complex(g, a.copy()) for a in fr))
+    rows_py, rows_or = [], []
+    timec = 0
+    for t in range(1, step + 1):
+        if timec == cstep:
+            timec = 0
+            rows_py.append(((t - 1) * dt, *sol.hdcheck()))
+            rows_or.append(((t - 1) * dt, *O.hdcheck(g, st), *O.vdiagnostic(g, st)))
+        sol.step(1)
+        O.hd_step(g, st, ordr, dt, nu)
+        timec += 1
+    rows_py = np.array(rows_py)
+    rows_or = np.array(rows_or)
+    # balance.txt is written with 16 decimal digits (E23.16)
+    assert np.allclose(bal[:, 1:3], rows_py[:, 1:3], rtol=1e-14, atol=0)
+    assert np.allclose(bal[:, 1:3], rows_or[:, 1:3], rtol=1e-11, atol=0)
+    # <div^2>, wall <v_t^2>, <v_n^2> (vboundary.f90:214-269) on the energy scale
+    assert np.max(np.abs(nsd[:, 1:] - rows_or[:, 4:])) < 1e-11 * np.max(rows_or[:, 1])
+    # output fields (physical space, v/(nx ny nz))
+    for name in ("vx", "vy", "vz"):
+        out = np.fromfile(tmp_path / "out" / f"{name}.0002.out").reshape(npz, ny, nx)
+        assert np.array_equal(out, sol.real_field(name)), name
+        ref = O.fftp3d_complex_to_real(g, getattr(st, name).copy())[:npz] / (nx * ny * nz)
+        assert rel_l2(out, ref) < 1e-11, name
