@@ -1,12 +1,19 @@
 #!/usr/bin/env python3
-"""RK-substep throughput of the SPECTER hot path on one MI355X.
+"""RK-substep throughput of the SPECTER hot path on one MI355X, mixed
+precision (float32 nonlinear term, float64 state / continuation / pressure /
+walls) against the float64 path.
 
-A "step" is one Runge-Kutta stage (one pass of the hot path: gradre with its
-12 c2r + 3 r2c, filter + Laplacian + RK update, no-slip walls, pressure
-projection) over the whole grid.  Inputs are resident in HBM before the timed
-region.  Prints one JSON line.
+A "step" is one Runge-Kutta stage (one pass of the hot path: the nonlinear
+term (v.grad)v with its z/y/x transforms and products, filter + Laplacian +
+RK update, no-slip walls, pressure projection) over the whole grid.  Inputs
+are resident in HBM before the timed region.  Both precisions are timed on
+the same workload; ``value`` is the mixed-precision rate, and the line also
+carries the float64 rate, the speed-up and the relative L2 field error of the
+mixed run against the float64 run after ``--err-steps`` full RK steps from
+the same initial field.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hd256|hd64]
+                    [--precision both|mixed|fp64] [--err-steps N]
 """
 from __future__ import annotations
 
@@ -23,28 +30,37 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 WORKLOADS = {
-    # configs[1]: HD 256^3 (z physical) d=5 C=25 RK4 nu=5e-4, forced
+    # HD 256^3 (z physical) d=5 C=25 RK4 nu=5e-4, forced
     "hd256": dict(nx=256, ny=256, nzp=256, C=25, d=5, ord=4, nu=5e-4, dt=5e-4, forcing=True,
                   label="HD 256x256x(256+25) FC-Gram d=5 C=25, RK4, nu=5e-4, dt=5e-4, "
                         "forcing 0.5(sin2y,sin2x,0)sin(pi z)"),
-    # configs[0]: HD 64^3 d=5 C=25 RK2 nu=2e-3 dt=1e-3, unforced
+    # HD 64^3 d=5 C=25 RK2 nu=2e-3 dt=1e-3, unforced
     "hd64": dict(nx=64, ny=64, nzp=64, C=25, d=5, ord=2, nu=2e-3, dt=1e-3, forcing=False,
                  label="HD 64x64x(64+25) FC-Gram d=5 C=25, RK2, nu=2e-3, dt=1e-3"),
 }
 
 
-def _setup(w, seed=1000):
+def _fields(w, seed=1000):
     import numpy as np
-    from specter_amd import FCPlan, Solver, SolverParams, ic
-    nx, ny, nz, C, d = w["nx"], w["ny"], w["nzp"] + w["C"], w["C"], w["d"]
-    plan = FCPlan(nx, ny, nz, C, d)
-    sol = Solver(plan, SolverParams(solver="HD", ord=w["ord"], dt=w["dt"], nu=w["nu"]))
+    from specter_amd import ic
+    nx, ny, nz, C = w["nx"], w["ny"], w["nzp"] + w["C"], w["C"]
     u = ic.spectrum_peaked_velocity(nx, ny, nz, C, seed)
     zero = np.zeros((nz, ny, nx))
     f = ic.large_scale_forcing(nx, ny, nz, C) if w["forcing"] else (zero, zero, zero)
+    return u, f
+
+
+def _plan(w, precision):
+    from specter_amd import FCPlan
+    return FCPlan(w["nx"], w["ny"], w["nzp"] + w["C"], w["C"], w["d"], precision=precision)
+
+
+def _solver(plan, w, u, f):
+    from specter_amd import Solver, SolverParams
+    sol = Solver(plan, SolverParams(solver="HD", ord=w["ord"], dt=w["dt"], nu=w["nu"]))
     for name, arr in zip(("vx", "vy", "vz", "fx", "fy", "fz"), (*u, *f)):
         sol.set_real_field(name, arr)
-    return plan, sol, u, f
+    return sol
 
 
 def _run_substeps(sol, ordr, nsub, start=0):
@@ -97,12 +113,66 @@ def _cpu_baseline(w, u, f, nsub):
     return dt, workers
 
 
+def _time(sol, w, steps, warmup):
+    """ms per RK stage over `steps` stages after `warmup` untimed ones (HIP
+    events on the solver's stream, which is torch's current stream)."""
+    import torch
+    pos = _run_substeps(sol, w["ord"], warmup)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    pos = _run_substeps(sol, w["ord"], steps, pos)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps, time.perf_counter() - t0, pos
+
+
+def _kernels(plan, sol, w, steps, pos):
+    rows = _profile(plan, sol, w["ord"], steps, pos)
+    rows.sort(key=lambda r: -r[2])
+    table = {r[0]: {"launches_per_step": round(r[1] / steps, 2),
+                    "ms_per_step": round(r[2] / steps, 4),
+                    "GBps": round(r[3] / (r[2] * 1e-3) / 1e9, 1)} for r in rows}
+    return rows, table
+
+
+def _roofline(rows):
+    name, n, tot_ms, tot_b = rows[0]
+    avg_ms = tot_ms / n
+    ach = (tot_b / n) / (avg_ms * 1e-3) / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": tot_b / n}
+
+
+def _field_error(w, u, f, nsteps, plans):
+    """Relative L2 error of each velocity component of the mixed run against
+    the float64 run after nsteps full RK steps from the same fields."""
+    import numpy as np
+    sols = {k: _solver(p, w, u, f) for k, p in plans.items()}
+    for s in sols.values():
+        s.step(nsteps)
+    out = {}
+    for name in ("vx", "vy", "vz"):
+        a = sols["mixed"].spectral(name)
+        b = sols["fp64"].spectral(name)
+        out[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    for s in sols.values():
+        s.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8, help="timed RK stages")
     ap.add_argument("--warmup", type=int, default=4, help="untimed RK stages")
     ap.add_argument("--workload", default="hd256", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="both", choices=("both", "mixed", "fp64"))
+    ap.add_argument("--err-steps", type=int, default=100,
+                    help="full RK steps of the mixed-vs-float64 field comparison (0 = skip)")
     ap.add_argument("--cpu-substeps", type=int, default=1,
                     help="RK stages of the CPU baseline sample (0 = skip)")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel pass")
@@ -111,62 +181,63 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the "
               "multi-GPU slab decomposition is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
-    import torch
+    from specter_amd.lib import SPX_FP32_MIXED, SPX_FP64
     w = WORKLOADS[args.workload]
-    plan, sol, u, f = _setup(w)
+    u, f = _fields(w)
     npts = w["nx"] * w["ny"] * w["nzp"]
     ntot = w["nx"] * w["ny"] * (w["nzp"] + w["C"])
-    pos = _run_substeps(sol, w["ord"], args.warmup)
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e0.record()
-    pos = _run_substeps(sol, w["ord"], args.steps, pos)
-    e1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1) / args.steps
-    value = npts / (ms * 1e-3) / 1e6
-
+    modes = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
+    plans, res = {}, {}
+    for mode in modes:
+        plan = _plan(w, SPX_FP32_MIXED if mode == "mixed" else SPX_FP64)
+        plans[mode] = plan
+        sol = _solver(plan, w, u, f)
+        ms, wall, pos = _time(sol, w, args.steps, args.warmup)
+        r = {"ms_per_step": ms, "wall_s": wall, "value": npts / (ms * 1e-3) / 1e6}
+        if not args.no_profile:
+            r["rows"], r["kernels"] = _kernels(plan, sol, w, args.steps, pos)
+        sol.close()
+        res[mode] = r
+    head = modes[0]
+    hr = res[head]
     line = {
-        "metric": "RK substep Mpts/s (physical points per RK stage per second)",
-        "value": round(value, 3),
+        "metric": "RK substep Mpts/s (physical points per RK stage per second), "
+                  + ("FP32-mixed" if head == "mixed" else "FP64"),
+        "value": round(hr["value"], 3),
         "unit": "Mpts/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
-        "wall_s": round(wall, 3),
+        "ms_per_step": round(hr["ms_per_step"], 4),
+        "wall_s": round(hr["wall_s"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "precision_mode": "fp64",
-        "field_rel_l2_vs_fp64": 0.0,
+        "dtype": "f32 nonlinear term / f64 state, continuation, pressure, walls"
+                 if head == "mixed" else "f64",
+        "precision_mode": head,
         "data": "synthetic (seeded spectrum-peaked solenoidal IC x sin(pi z/Lz))",
         "config": {"workload": w["label"], "grid": [w["nx"], w["ny"], w["nzp"] + w["C"]],
-                   "z_dft": {0: "direct", 1: "bluestein", 2: "rader"}[plan.z_algorithm]
-                            + f" (in-LDS FFT length {plan.z_fft_len})",
+                   "z_dft": {0: "direct", 1: "bluestein", 2: "rader"}[
+                       plans[head].z_algorithm] + f" (in-LDS FFT length {plans[head].z_fft_len})",
                    "physical_points": npts, "total_points": ntot,
-                   "total_Mpts_per_s": round(ntot / (ms * 1e-3) / 1e6, 3),
+                   "total_Mpts_per_s": round(ntot / (hr["ms_per_step"] * 1e-3) / 1e6, 3),
                    "parallelism": "single GPU"},
     }
+    if "fp64" in res and head == "mixed":
+        line["fp64"] = {"value": round(res["fp64"]["value"], 3),
+                        "ms_per_step": round(res["fp64"]["ms_per_step"], 4)}
+        line["speedup_vs_fp64"] = round(res["fp64"]["ms_per_step"] / hr["ms_per_step"], 3)
+        if args.err_steps > 0:
+            err = _field_error(w, u, f, args.err_steps, plans)
+            line["field_rel_l2_vs_fp64"] = {"steps": args.err_steps,
+                                            **{k: float(f"{v:.3e}") for k, v in err.items()}}
     if not args.no_profile:
-        rows = _profile(plan, sol, w["ord"], args.steps, pos)
-        rows.sort(key=lambda r: -r[2])
-        name, n, tot_ms, tot_b = rows[0]
-        avg_ms = tot_ms / n
-        ach = (tot_b / n) / (avg_ms * 1e-3) / 1e9
-        line["roofline"] = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1),
-                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                            "avg_launch_ms": round(avg_ms, 5),
-                            "algorithmic_bytes_per_launch": tot_b / n}
-        line["kernels"] = {r[0]: {"launches_per_step": round(r[1] / args.steps, 2),
-                                  "ms_per_step": round(r[2] / args.steps, 4),
-                                  "GBps": round(r[3] / (r[2] * 1e-3) / 1e9, 1)} for r in rows}
+        line["roofline"] = _roofline(hr["rows"])
+        line["kernels"] = hr["kernels"]
+        if "fp64" in res and head == "mixed":
+            line["fp64"]["roofline"] = _roofline(res["fp64"]["rows"])
+            line["fp64"]["kernels"] = res["fp64"]["kernels"]
     if args.cpu_substeps > 0:
         t, cores = _cpu_baseline(w, u, f, args.cpu_substeps)
         line["cpu_baseline"] = {"value": round(npts / t / 1e6, 4), "unit": "Mpts/s",

@@ -5,8 +5,14 @@
  * (HBM) in the reference's Fortran layouts:
  *   spectral  (nz, ny, nx/2+1) complex, z fastest      ["S"]
  *   real      (nx, ny, nz)     real,    x fastest      ["R"]
- * The element type of fields is set by the plan's precision:
- *   SPX_FP64        complex = 2 x double, real = double
+ * Fields are float64 (complex = 2 x double) in every plan.  The plan's
+ * precision selects the arithmetic of the nonlinear term:
+ *   SPX_FP64        everything in float64
+ *   SPX_FP32_MIXED  gradre / advect (their z, y, x transforms and products)
+ *                   run in float32 and write complex float (2 x float)
+ *                   results; the FC continuation sums, the Poisson and
+ *                   harmonic pressure solve, the wall conditions, the RK
+ *                   update and every public transform stay float64.
  * All transforms are unnormalised, as in the reference (callers scale).
  * Every call returns 0 on success or an SPX_ERR_* code; spx_last_error()
  * returns the message.  A plan is bound to one HIP stream; calls on one plan
@@ -49,7 +55,7 @@ typedef struct {
   int oz;           /* FC-Gram matching order d in z                                 */
   double Lx, Ly;    /* periodic box lengths in units of 2*pi (specter.fpp:683-711)   */
   double Lz;        /* wall-to-wall length of the physical z domain (:743-749)       */
-  int precision;    /* SPX_FP64                                                      */
+  int precision;    /* SPX_FP64 or SPX_FP32_MIXED                                    */
 } spx_grid;
 
 /* Sizes of a plan (derived values). */
@@ -100,10 +106,12 @@ int spx_fc_continue(spx_plan* p, void* inout_spec);
 int spx_derivk(spx_plan* p, const void* a, void* b, int dir);      /* :28-94 */
 int spx_laplak(spx_plan* p, const void* a, void* b);               /* :97-127 */
 int spx_fc_filter(spx_plan* p, void* a);                           /* :1082-1115 */
-/* gradre (:209-319): (A.grad)A in advective form; d,e,f = x,y,z components. */
+/* gradre (:209-319): (A.grad)A in advective form; d,e,f = x,y,z components
+ * (complex float in SPX_FP32_MIXED plans, complex double otherwise). */
 int spx_gradre(spx_plan* p, const void* a, const void* b, const void* c, void* d, void* e,
                void* f);
-/* advect (pseudospec_phd.f90:23-113): (A.grad)B for a scalar B = d; result e. */
+/* advect (pseudospec_phd.f90:23-113): (A.grad)B for a scalar B = d; result e
+ * (complex float in SPX_FP32_MIXED plans). */
 int spx_advect(spx_plan* p, const void* a, const void* b, const void* c, const void* d,
                void* e);
 

@@ -66,17 +66,36 @@ def build_fortran(verbose: bool = False) -> str | None:
 
 
 def build(verbose: bool = False) -> str:
-    """Compile every HIP source into specter_amd/libspx.so (always rebuilds),
-    then the Fortran driver that links it."""
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    """Compile every HIP source into specter_amd/libspx.so (always rebuilds;
+    one hipcc process per source, in parallel), then the Fortran driver that
+    links it."""
+    odir = os.path.join(CSRC, "_build")
+    os.makedirs(odir, exist_ok=True)
+    objs, procs = [], []
+    for src in SOURCES:
+        obj = os.path.join(odir, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        cmd = [hipcc(), *HIPCC_FLAGS, "-c", "-I", CSRC, os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                      text=True))
+    failed = False
+    for src, pr in zip(SOURCES, procs):
+        out, _ = pr.communicate()
+        if pr.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed building libspx.so")
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", CSRC, *srcs, "-o", tmp]
+    cmd = [hipcc(), *[f for f in HIPCC_FLAGS if f != "-O3"], *objs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libspx.so")
+        raise RuntimeError("hipcc failed linking libspx.so")
     os.replace(tmp, LIB)
     build_fortran(verbose)
     return LIB

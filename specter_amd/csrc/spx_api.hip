@@ -66,20 +66,22 @@ static int upload(spx_plan* p, double** dst, const V& host) {
   return SPX_OK;
 }
 
+// gradre / advect work arrays in the nonlinear pipeline's precision
 static int ensure_work(spx_plan* p, bool bouss) {
   int rc;
+  const size_t cb = p->nc * p->csize, rb = p->nr * p->rsize;
   if (!p->XV[0]) {
     for (int c = 0; c < 3; ++c) {
-      if ((rc = alloc_dev(p, &p->XV[c], p->nc * 16)) || (rc = alloc_dev(p, &p->XZ[c], p->nc * 16)) ||
-          (rc = alloc_dev(p, &p->XY[c], p->nc * 16)) || (rc = alloc_dev(p, &p->RV[c], p->nr * 8)) ||
-          (rc = alloc_dev(p, &p->RO[c], p->nr * 8)))
+      if ((rc = alloc_dev(p, &p->XV[c], cb)) || (rc = alloc_dev(p, &p->XZ[c], cb)) ||
+          (rc = alloc_dev(p, &p->XY[c], cb)) || (rc = alloc_dev(p, &p->RV[c], rb)) ||
+          (rc = alloc_dev(p, &p->RO[c], rb)))
         return rc;
     }
     if ((rc = alloc_dev(p, &p->R0, p->nr * 8))) return rc;
   }
   if (bouss && !p->work_bouss) {
-    if ((rc = alloc_dev(p, &p->XV[3], p->nc * 16)) || (rc = alloc_dev(p, &p->XZ[3], p->nc * 16)) ||
-        (rc = alloc_dev(p, &p->XY[3], p->nc * 16)) || (rc = alloc_dev(p, &p->RO[3], p->nr * 8)))
+    if ((rc = alloc_dev(p, &p->XV[3], cb)) || (rc = alloc_dev(p, &p->XZ[3], cb)) ||
+        (rc = alloc_dev(p, &p->XY[3], cb)) || (rc = alloc_dev(p, &p->RO[3], rb)))
       return rc;
     p->work_bouss = true;
   }
@@ -87,11 +89,12 @@ static int ensure_work(spx_plan* p, bool bouss) {
 }
 
 // 3D transforms on internal buffers -------------------------------------------
-static int r2c3(spx_plan* p, const void* in, void* out) {
+// lo: float32 buffers and kernels of a mixed plan (X holds float32 data then)
+static int r2c3(spx_plan* p, const void* in, void* out, bool lo = false) {
   int rc;
-  if ((rc = xr2c(p, in, p->X, p->np))) return rc;
-  if ((rc = yfft(p, p->X, 1, p->np))) return rc;
-  return zfft_x2s(p, p->X, out, 0);
+  if ((rc = xr2c(p, in, p->X, p->np, lo))) return rc;
+  if ((rc = yfft(p, p->X, 1, p->np, nullptr, nullptr, lo))) return rc;
+  return zfft_x2s(p, p->X, out, 0, lo);
 }
 
 static int c2r3(spx_plan* p, const void* in, void* out, int dirn, int nplanes) {
@@ -160,8 +163,8 @@ int spx_plan_create(const spx_grid* g, const double* A, const double* Q, void* s
   SPX_REQ(g->nz >= 2, "nz must be >= 2");
   SPX_REQ(g->Cz >= 0 && g->Cz < g->nz, "Cz out of range");
   SPX_REQ(g->Lx > 0 && g->Ly > 0 && g->Lz > 0, "box lengths must be positive");
-  if (g->precision != SPX_FP64)
-    return fail(SPX_ERR_UNSUPPORTED, "only SPX_FP64 plans are available");
+  SPX_REQ(g->precision == SPX_FP64 || g->precision == SPX_FP32_MIXED,
+          "precision must be SPX_FP64 or SPX_FP32_MIXED");
   if (g->Cz > 0) {
     SPX_REQ(A && Q, "FC-Gram tables A and Q are required when Cz > 0");
     SPX_REQ(g->oz >= 1, "matching order oz must be >= 1 when Cz > 0");
@@ -177,7 +180,9 @@ int spx_plan_create(const spx_grid* g, const double* A, const double* Q, void* s
   p->np = nz - C;
   p->nc = size_t(nz) * ny * p->nkx;
   p->nr = size_t(nx) * ny * nz;
-  p->csize = 16;
+  p->fp32 = g->precision == SPX_FP32_MIXED;
+  p->csize = p->fp32 ? 8 : 16;
+  p->rsize = p->fp32 ? 4 : 8;
   int rc = SPX_OK;
   auto bail = [&](int code) {
     spx_plan_destroy(p);
@@ -235,11 +240,15 @@ int spx_plan_create(const spx_grid* g, const double* A, const double* Q, void* s
       (rc = upload(p, &p->d_fky, p->h_fky)) || (rc = upload(p, &p->d_fkz, p->h_fkz)))
     return bail(rc);
   // line FFT plans
-  if ((rc = build_line_fft(p->fx, nx, p->csize, 0))) return bail(rc);
-  if ((rc = build_line_fft(p->fy, ny, p->csize, 16))) return bail(rc);
-  if ((rc = build_line_fft(p->fz, nz, p->csize, 16))) return bail(rc);
-  p->fz2 = p->fz;
-  p->fz2.nl = p->fz.nl > 1 ? p->fz.nl / 2 : 1;
+  if ((rc = build_line_fft(p->fx, nx, 16, 0))) return bail(rc);
+  if ((rc = build_line_fft(p->fy, ny, 16, 16))) return bail(rc);
+  if ((rc = build_line_fft(p->fz, nz, 16, 16))) return bail(rc);
+  p->fz2 = p->fz.with_lines(p->fz.nl / 2);
+  if (p->fp32) {
+    if ((rc = build_line_fft(p->fx32, nx, 8, 0))) return bail(rc);
+    if ((rc = build_line_fft(p->fy32, ny, 8, 16))) return bail(rc);
+    if ((rc = build_line_fft(p->fz32, nz, 8, 16))) return bail(rc);
+  }
   // scratch
   const size_t nlines = size_t(ny) * p->nkx;
   if ((rc = alloc_dev(p, &p->X, p->nc * 16))) return bail(rc);
@@ -257,6 +266,9 @@ int spx_plan_destroy(spx_plan* p) {
   free_line_fft(p->fx);
   free_line_fft(p->fy);
   free_line_fft(p->fz);
+  free_line_fft(p->fx32);
+  free_line_fft(p->fy32);
+  free_line_fft(p->fz32);
   for (hipEvent_t e : p->prof.pool) (void)hipEventDestroy(e);
   for (void* q : p->owned) (void)hipFree(q);
   delete p;
@@ -274,8 +286,8 @@ int spx_plan_get_info(const spx_plan* p, spx_plan_info* info) {
   info->z_fft_len = p->fz.M;
   info->z_algorithm = p->fz.mode;
   size_t b = p->nc * 16 + size_t(p->g.ny) * p->nkx * 48;
-  if (p->XV[0]) b += 9 * p->nc * 16 + 7 * p->nr * 8;
-  if (p->work_bouss) b += 3 * p->nc * 16 + p->nr * 8;
+  if (p->XV[0]) b += 9 * p->nc * p->csize + 6 * p->nr * p->rsize + p->nr * 8;
+  if (p->work_bouss) b += 3 * p->nc * p->csize + p->nr * p->rsize;
   info->device_bytes = b;
   return SPX_OK;
 }
@@ -354,23 +366,26 @@ namespace spx {
 // along a z line), so each component needs two z IFFTs (v and d_z v) instead
 // of four; the y pass yields v and d_y v from one load; the x pass forms the
 // three derivatives and accumulates the products in the reference's order.
+// In mixed plans the whole pipeline runs in float32: the first z pass reads
+// the float64 state and the outputs are float32.
 static int gradre_impl(spx_plan* p, const void* const v[3], void* const out[3]) {
   int rc;
   const int np = p->np;
+  const bool lo = p->fp32;
   for (int c = 0; c < 3; ++c)
-    if ((rc = zfft_s2x(p, v[c], p->XV[c], 0, np, 0, p->XZ[c]))) return rc;
+    if ((rc = zfft_s2x(p, v[c], p->XV[c], 0, np, 0, p->XZ[c], lo))) return rc;
   for (int c = 0; c < 3; ++c) {
-    if ((rc = yfft(p, p->XV[c], 0, np, nullptr, p->XY[c]))) return rc;
-    if ((rc = yfft(p, p->XZ[c], 0, np))) return rc;
+    if ((rc = yfft(p, p->XV[c], 0, np, nullptr, p->XY[c], lo))) return rc;
+    if ((rc = yfft(p, p->XZ[c], 0, np, nullptr, nullptr, lo))) return rc;
   }
   for (int c = 0; c < 3; ++c)
-    if ((rc = xc2r(p, p->XV[c], p->RV[c], np, 0))) return rc;
+    if ((rc = xc2r(p, p->XV[c], p->RV[c], np, 0, lo))) return rc;
   for (int c = 0; c < 3; ++c)
     if ((rc = xgrad_prod(p, p->XV[c], p->XY[c], p->XZ[c], p->RV[0], p->RV[1], p->RV[2],
-                         p->RO[c])))
+                         p->RO[c], lo)))
       return rc;
   for (int c = 0; c < 3; ++c)
-    if ((rc = r2c3(p, p->RO[c], out[c]))) return rc;
+    if ((rc = r2c3(p, p->RO[c], out[c], lo))) return rc;
   return SPX_OK;
 }
 
@@ -380,20 +395,21 @@ static int advect_impl(spx_plan* p, const void* const v[3], const void* th, void
                        bool have_rv) {
   int rc;
   const int np = p->np;
+  const bool lo = p->fp32;
   if (!have_rv) {
     for (int c = 0; c < 3; ++c) {
-      if ((rc = zfft_s2x(p, v[c], p->XV[c], 0, np, 0))) return rc;
-      if ((rc = yfft(p, p->XV[c], 0, np))) return rc;
-      if ((rc = xc2r(p, p->XV[c], p->RV[c], np, 0))) return rc;
+      if ((rc = zfft_s2x(p, v[c], p->XV[c], 0, np, 0, nullptr, lo))) return rc;
+      if ((rc = yfft(p, p->XV[c], 0, np, nullptr, nullptr, lo))) return rc;
+      if ((rc = xc2r(p, p->XV[c], p->RV[c], np, 0, lo))) return rc;
     }
   }
-  if ((rc = zfft_s2x(p, th, p->XV[3], 0, np, 0, p->XZ[3]))) return rc;
-  if ((rc = yfft(p, p->XV[3], 0, np, nullptr, p->XY[3]))) return rc;
-  if ((rc = yfft(p, p->XZ[3], 0, np))) return rc;
+  if ((rc = zfft_s2x(p, th, p->XV[3], 0, np, 0, p->XZ[3], lo))) return rc;
+  if ((rc = yfft(p, p->XV[3], 0, np, nullptr, p->XY[3], lo))) return rc;
+  if ((rc = yfft(p, p->XZ[3], 0, np, nullptr, nullptr, lo))) return rc;
   if ((rc = xgrad_prod(p, p->XV[3], p->XY[3], p->XZ[3], p->RV[0], p->RV[1], p->RV[2],
-                       p->RO[3])))
+                       p->RO[3], lo)))
     return rc;
-  return r2c3(p, p->RO[3], out);
+  return r2c3(p, p->RO[3], out, lo);
 }
 
 }  // namespace spx
@@ -566,10 +582,13 @@ int spx_state_create(spx_plan* p, const spx_params* prm, spx_state** out) {
   void** fields[] = {&s->vx, &s->vy, &s->vz, &s->pr, &s->fx, &s->fy, &s->fz,
                      &s->C1, &s->C2, &s->C3, &s->N[0], &s->N[1], &s->N[2]};
   void** bfields[] = {&s->th, &s->fs, &s->C7, &s->N[3]};
+  // the nonlinear terms N are in the nonlinear pipeline's precision
   auto mk = [&](void** dst) -> int {
-    if (hipMalloc(dst, p->nc * 16) != hipSuccess) return fail(SPX_ERR_ALLOC, "state allocation");
+    const bool nl = dst == &s->N[0] || dst == &s->N[1] || dst == &s->N[2] || dst == &s->N[3];
+    const size_t bytes = p->nc * (nl ? p->csize : 16);
+    if (hipMalloc(dst, bytes) != hipSuccess) return fail(SPX_ERR_ALLOC, "state allocation");
     s->owned.push_back(*dst);
-    if (hipMemsetAsync(*dst, 0, p->nc * 16, p->stream) != hipSuccess)
+    if (hipMemsetAsync(*dst, 0, bytes, p->stream) != hipSuccess)
       return fail(SPX_ERR_HIP, "state clear");
     return SPX_OK;
   };
@@ -645,7 +664,7 @@ int spx_substep(spx_plan* p, spx_state* s, int o) {
   const double coef[4] = {0.0, 0.0, q.xmom, q.xtemp};
   const double nu[4] = {q.nu, q.nu, q.nu, q.kappa};
   if ((rc = op_rk_update(p, bouss ? 4 : 3, v, c0, nl, f, bouss ? src : nullptr, coef, nu, q.dt,
-                         rmp)))
+                         rmp, p->fp32)))
     return rc;
   if ((rc = spx_v_imposebc_and_project(p, s->vx, s->vy, s->vz, s->pr, o, q.ord, q.v_zsta,
                                        q.v_zend)))

@@ -104,7 +104,14 @@ struct spx_plan {
   int nkx = 0, np = 0;
   size_t nc = 0, nr = 0;  // complex / real element counts of S and R arrays
   hipStream_t stream = nullptr;
-  int csize = 16;         // bytes per complex element of the plan's field type
+  // The state, the public transforms, the boundary / projection / diagnostics
+  // kernels and the RK update are float64 in every plan.  A mixed-precision
+  // plan (SPX_FP32_MIXED) evaluates the nonlinear term (gradre / advect:
+  // z, y, x transforms and products) in float32; csize / rsize are the
+  // element sizes of that pipeline's buffers and of the nonlinear terms.
+  int csize = 16;         // bytes per complex element of the nonlinear pipeline
+  int rsize = 8;          // bytes per real element of the nonlinear pipeline
+  bool fp32 = false;      // SPX_FP32_MIXED
   // FC-Gram continuation matrix dir = A*Q^T, row-major [C][d], float64 always.
   double* d_dir = nullptr;
   std::vector<double> h_dir;
@@ -114,12 +121,15 @@ struct spx_plan {
   double *d_z = nullptr, *d_khom = nullptr;
   std::vector<double> h_kx, h_ky, h_kz, h_z, h_khom, h_fkx, h_fky, h_fkz;
   double Dkx = 0, Dky = 0, Dkz = 0, dz = 0;
-  spx::LineFFTHost fx, fy, fz;
-  spx::LineFFTHost fz2;     // fz with half the lines per workgroup (two-buffer kernels)
+  // float64 line FFTs (x, y, z; fz2: z with half the lines per workgroup for
+  // the two-buffer kernels) and, in mixed plans, the float32 line FFTs of the
+  // nonlinear pipeline.
+  spx::LineFFTHost fx, fy, fz, fz2;
+  spx::LineFFTHost fx32, fy32, fz32;
   bool work_bouss = false;
   // scratch
-  void* X = nullptr;        // xy-stage buffer, nc complex
-  // gradre / advect work arrays (allocated on first use):
+  void* X = nullptr;        // xy-stage buffer, nc float64 complex
+  // gradre / advect work arrays (allocated on first use, csize / rsize elements):
   //   XV[c]: z+y inverse transform of v_c, XZ[c]: of d_z v_c, XY[c]: y-derivative stage
   //   RV[c]: v_c in real space, RO[0..3]: nonlinear products (x,y,z, scalar)
   void* XV[4] = {nullptr};
@@ -158,6 +168,10 @@ struct ProfScope {
 };
 
 // FFT launchers (spx_fft.hip) ------------------------------------------------
+// Every launcher works on float64 buffers; those with an `lo` argument run the
+// float32 kernels of a mixed plan instead when lo is true (zfft_s2x then reads
+// a float64 S and writes float32 X; the others read and write float32).
+// csize: bytes per complex element (16: float64 tables, 8: float32 tables)
 int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines);
 void free_line_fft(LineFFTHost& f);
 
@@ -169,22 +183,22 @@ int zfft_inplace(spx_plan* p, void* S, int forward, int cont, double scale);
 int fc_continue_only(spx_plan* p, void* S);
 // forward 3D last pass: X (planes k < np) -> continuation -> z FFT -> S.
 // nofft = 1: transpose only (all nz planes), for fftp2d_real_to_complex_xy.
-int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft);
+int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft, bool lo = false);
 // backward 3D first pass: S * (i k_dir) (dir 0 = none) -> z IFFT -> X, planes < nzout.
 // X2 != nullptr: also X2 <- z IFFT of (i kz) S from the same load (dual).
 int zfft_s2x(spx_plan* p, const void* S, void* X, int dir, int nzout, int nofft,
-             void* X2 = nullptr);
+             void* X2 = nullptr, bool lo = false);
 // y transform of X planes < nplanes: Xout <- y FFT(Xin) (in place when equal);
 // Y2 != nullptr: also Y2 <- y FFT of (i ky) Xin (dual, backward only).
 int yfft(spx_plan* p, void* X, int forward, int nplanes, const void* Xin = nullptr,
-         void* Y2 = nullptr);
-int xr2c(spx_plan* p, const void* R, void* X, int nplanes);
+         void* Y2 = nullptr, bool lo = false);
+int xr2c(spx_plan* p, const void* R, void* X, int nplanes, bool lo = false);
 // x half complex -> real of planes < nplanes; dirx = 1 multiplies by i kx first.
-int xc2r(spx_plan* p, const void* X, void* R, int nplanes, int dirx = 0);
+int xc2r(spx_plan* p, const void* X, void* R, int nplanes, int dirx = 0, bool lo = false);
 // gradient products of one component: out = ((u*Dx + v*Dy) + w*Dz)*tmp with
 // Dx = c2r(i kx YV), Dy = c2r(YY), Dz = c2r(XZ) (pseudospec_hd.f90:254-312 order)
 int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, const void* u,
-               const void* v, const void* w, void* out);
+               const void* v, const void* w, void* out, bool lo = false);
 // fused v_imposebc: goto_domain_w_boundaries + noslip_z + goto_3d_fourier on vx,vy
 int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
                  const double vzsta[2], const double vzend[2]);
@@ -210,9 +224,10 @@ int op_derivk(spx_plan* p, const void* a, void* b, int dir);
 int op_laplak(spx_plan* p, const void* a, void* b);
 int op_filter(spx_plan* p, void* a);
 int op_scale(spx_plan* p, void* a, size_t n, double s);
+// nl: float64, or float32 with nl_lo (the nonlinear terms of a mixed plan)
 int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, void* const* nl,
                  const void* const* f, const void* const* src, const double* src_coef,
-                 const double* nu, double dt, double rmp);
+                 const double* nu, double dt, double rmp, bool nl_lo = false);
 int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d);
 int op_axpy_field(spx_plan* p, void* y, const void* x, double alpha);
 int op_real_div3(spx_plan* p, void* r, double nx, double ny, double nz);

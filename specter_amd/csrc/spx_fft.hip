@@ -2,6 +2,12 @@
 // continuation and the (kx,y,z)<->(z,y,kx) transposes fused in), the fused
 // gradient-product x pass of gradre/advect, and the z-line kernels of the
 // boundary / projection / diagnostics stages.
+//
+// Precision: the transform kernels are templated on their arithmetic / buffer
+// type T (double; float for the nonlinear pipeline of mixed-precision plans,
+// where the first z pass reads the float64 state, TI = double).  The
+// boundary, projection and diagnostics kernels compute in float64; they are
+// templated on the storage type TS of the fields and instantiated for double.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -10,23 +16,23 @@
 
 namespace spx {
 
-using Tp = double;  // element precision of the FP64 path
-using Cp = cpx<Tp>;
-using LF = LineFFT<Cp>;
-
 // (0,1)*k*v evaluated as the reference's complex product (-k Im v, k Re v)
-__device__ __forceinline__ Cp ikmul(double k, Cp v) { return mkc<Tp>(-(k * v.y), k * v.x); }
+template <typename T>
+__device__ __forceinline__ cpx<T> ikmul(T k, cpx<T> v) { return mkc<T>(-(k * v.y), k * v.x); }
+
+template <typename TO, typename TI>
+__device__ __forceinline__ cpx<TO> ccast(cpx<TI> v) { return mkc<TO>(TO(v.x), TO(v.y)); }
 
 // Twiddles at the head of dynamic LDS, then the line buffers.  Each thread
 // belongs to line `line` (lane `lane` of tpl); xl is its line buffer.
-#define SPX_PROLOGUE(f)                                          \
+#define SPX_PROLOGUE(CT, f)                                      \
   extern __shared__ unsigned char smem[];                        \
-  Cp* const tws = reinterpret_cast<Cp*>(smem);                   \
-  Cp* const lds = tws + (f).M;                                   \
+  CT* const tws = reinterpret_cast<CT*>(smem);                   \
+  CT* const lds = tws + (f).M;                                   \
   stage_twiddles(tws, (f));                                      \
   const int tid = threadIdx.x;                                   \
   const int line = tid >> (f).lgtpl, lane = tid & ((f).tpl - 1); \
-  Cp* const xl = lds + line * (f).ldl;                           \
+  CT* const xl = lds + line * (f).ldl;                           \
   (void)xl;
 
 constexpr int KEEP = kStage;  // line elements a thread can hold in registers (n <= KEEP*tpl)
@@ -34,18 +40,20 @@ constexpr int KEEP = kStage;  // line elements a thread can hold in registers (n
 // ---------------------------------------------------------------------------
 // z transforms in place on S (contiguous lines)
 // ---------------------------------------------------------------------------
-template <int RS>
-__global__ void __launch_bounds__(256) k_z_inplace(Cp* __restrict__ S, long nlines, LF f, int np,
-                                                   int forward, int cont, double scale,
+template <typename T, int RS>
+__global__ void __launch_bounds__(256) k_z_inplace(cpx<T>* __restrict__ S, long nlines,
+                                                   LineFFT<cpx<T>> f, int np, int forward,
+                                                   int cont, double scale,
                                                    const double* __restrict__ dir, int C,
                                                    int d) {
-  SPX_PROLOGUE(f);
+  using Cp = cpx<T>;
+  SPX_PROLOGUE(Cp, f);
   const int n = f.n;
   const long ln = long(blockIdx.x) * f.nl + line;
   const bool ok = ln < nlines;
   const int nload = (forward && cont) ? np : n;
   for (int k = lane; k < nload; k += f.tpl) {
-    Cp v = ok ? S[ln * n + k] : mkc<Tp>(0, 0);
+    Cp v = ok ? S[ln * n + k] : mkc<T>(0, 0);
     if (!forward) v.y = -v.y;
     xl[k] = v;
   }
@@ -54,50 +62,53 @@ __global__ void __launch_bounds__(256) k_z_inplace(Cp* __restrict__ S, long nlin
     fc_continue_lds(lds, f.nl, f.ldl, np, C, d, dir);
     __syncthreads();
   }
-  line_dft<Tp, RS>(xl, f, tws, lane);
+  line_dft<T, RS>(xl, f, tws, lane);
   if (ok) {
     for (int k = lane; k < n; k += f.tpl) {
       Cp v = xl[k];
       if (!forward) {
         v.y = -v.y;
-        if (k < np && scale != 1.0) v = cscale(v, scale);
+        if (k < np && scale != 1.0) v = cscale(v, T(scale));
       }
       S[ln * n + k] = v;
     }
   }
 }
 
-// Continuation only (fftp.fpp:755-773), in place on S.
-__global__ void k_fc_continue(Cp* __restrict__ S, long nlines, int n, int np, int C, int d,
+// Continuation only (fftp.fpp:755-773), in place on S, float64 arithmetic.
+template <typename T>
+__global__ void k_fc_continue(cpx<T>* __restrict__ S, long nlines, int n, int np, int C, int d,
                               const double* __restrict__ dir) {
 #pragma clang fp contract(off)
   const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= nlines * C) return;
   const long ln = e / C;
   const int ii = int(e - ln * C);
-  const Cp* x = S + ln * n;
+  const cpx<T>* x = S + ln * n;
   const int top = np - d;
   const double* D0 = dir + ii * d;
   const double* D1 = dir + (C - 1 - ii) * d;
-  double ar = D0[0] * x[top].x + D1[0] * x[d - 1].x;
-  double ai = D0[0] * x[top].y + D1[0] * x[d - 1].y;
+  double ar = D0[0] * double(x[top].x) + D1[0] * double(x[d - 1].x);
+  double ai = D0[0] * double(x[top].y) + D1[0] * double(x[d - 1].y);
   for (int jj = 1; jj < d; ++jj) {
-    ar = ar + D0[jj] * x[top + jj].x;
-    ar = ar + D1[jj] * x[d - 1 - jj].x;
-    ai = ai + D0[jj] * x[top + jj].y;
-    ai = ai + D1[jj] * x[d - 1 - jj].y;
+    ar = ar + D0[jj] * double(x[top + jj].x);
+    ar = ar + D1[jj] * double(x[d - 1 - jj].x);
+    ai = ai + D0[jj] * double(x[top + jj].y);
+    ai = ai + D1[jj] * double(x[d - 1 - jj].y);
   }
-  S[ln * n + np + ii] = mkc<Tp>(ar, ai);
+  S[ln * n + np + ii] = mkc<T>(T(ar), T(ai));
 }
 
 // ---------------------------------------------------------------------------
 // forward 3D last pass: X(kx,y,z) planes < np -> continuation -> z FFT -> S(z,y,kx)
 // ---------------------------------------------------------------------------
-template <int RS>
-__global__ void __launch_bounds__(256) k_z_x2s(const Cp* __restrict__ X, Cp* __restrict__ S, LF f,
-                                               int nkx, int ny, int np, int nofft,
-                                               const double* __restrict__ dir, int C, int d) {
-  SPX_PROLOGUE(f);
+template <typename T, int RS>
+__global__ void __launch_bounds__(256) k_z_x2s(const cpx<T>* __restrict__ X, cpx<T>* __restrict__ S,
+                                               LineFFT<cpx<T>> f, int nkx, int ny, int np,
+                                               int nofft, const double* __restrict__ dir, int C,
+                                               int d) {
+  using Cp = cpx<T>;
+  SPX_PROLOGUE(Cp, f);
   const int n = f.n;
   const int i0 = blockIdx.x * f.nl, j = blockIdx.y;
   const int nload = nofft ? n : np;
@@ -105,7 +116,7 @@ __global__ void __launch_bounds__(256) k_z_x2s(const Cp* __restrict__ X, Cp* __r
   {
     const int l = tid & (f.nl - 1), i = i0 + l;
     for (int k = tid >> f.lgnl; k < nload; k += f.tpl) {
-      Cp v = mkc<Tp>(0, 0);
+      Cp v = mkc<T>(0, 0);
       if (i < nkx) v = X[i + long(nkx) * j + pl * k];
       lds[l * f.ldl + k] = v;
     }
@@ -116,7 +127,7 @@ __global__ void __launch_bounds__(256) k_z_x2s(const Cp* __restrict__ X, Cp* __r
       fc_continue_lds(lds, f.nl, f.ldl, np, C, d, dir);
       __syncthreads();
     }
-    line_dft<Tp, RS>(xl, f, tws, lane);
+    line_dft<T, RS>(xl, f, tws, lane);
   }
   const int i = i0 + line;
   if (i < nkx)
@@ -127,14 +138,15 @@ __global__ void __launch_bounds__(256) k_z_x2s(const Cp* __restrict__ X, Cp* __r
 // backward 3D first pass: S * (i k_dir) -> z IFFT -> X planes < nzout.
 // DUAL: the same load also gives X2 <- z IFFT of (i kz) S.
 // ---------------------------------------------------------------------------
-template <int RS, bool DUAL>
-__global__ void __launch_bounds__(256) k_z_s2x(const Cp* __restrict__ S, Cp* __restrict__ X,
-                                               Cp* __restrict__ X2, LF f, int nkx, int ny,
-                                               int nzout, int dirn, int nofft,
+template <typename T, int RS, bool DUAL, typename TI>
+__global__ void __launch_bounds__(256) k_z_s2x(const cpx<TI>* __restrict__ S, cpx<T>* __restrict__ X,
+                                               cpx<T>* __restrict__ X2, LineFFT<cpx<T>> f,
+                                               int nkx, int ny, int nzout, int dirn, int nofft,
                                                const double* __restrict__ kx,
                                                const double* __restrict__ ky,
                                                const double* __restrict__ kz) {
-  SPX_PROLOGUE(f);
+  using Cp = cpx<T>;
+  SPX_PROLOGUE(Cp, f);
   const int n = f.n;
   const int i0 = blockIdx.x * f.nl, j = blockIdx.y;
   const int i = i0 + line;
@@ -145,19 +157,19 @@ __global__ void __launch_bounds__(256) k_z_s2x(const Cp* __restrict__ S, Cp* __r
   for (int q = 0; q < KEEP; ++q) {
     const int k = lane + q * f.tpl;
     if (k < n) {
-      Cp v = mkc<Tp>(0, 0);
+      Cp v = mkc<T>(0, 0);
       if (ok) {
-        v = S[base + k];
-        if (dirn == 1) v = ikmul(kx[i], v);
-        else if (dirn == 2) v = ikmul(ky[j], v);
-        else if (dirn == 3) v = ikmul(kz[k], v);
+        v = ccast<T>(S[base + k]);
+        if (dirn == 1) v = ikmul(T(kx[i]), v);
+        else if (dirn == 2) v = ikmul(T(ky[j]), v);
+        else if (dirn == 3) v = ikmul(T(kz[k]), v);
       }
       if constexpr (DUAL) keep[q] = v;
       xl[k] = cconj(v);
     }
   }
   __syncthreads();
-  if (!nofft) line_dft<Tp, RS>(xl, f, tws, lane);
+  if (!nofft) line_dft<T, RS>(xl, f, tws, lane);
   const long pl = long(nkx) * ny;
   const int l = tid & (f.nl - 1), ii = i0 + l;
   if (ii < nkx)
@@ -168,10 +180,10 @@ __global__ void __launch_bounds__(256) k_z_s2x(const Cp* __restrict__ S, Cp* __r
 #pragma unroll
     for (int q = 0; q < KEEP; ++q) {
       const int k = lane + q * f.tpl;
-      if (k < n) xl[k] = cconj(ikmul(kz[k], keep[q]));
+      if (k < n) xl[k] = cconj(ikmul(T(kz[k]), keep[q]));
     }
     __syncthreads();
-    line_dft<Tp, RS>(xl, f, tws, lane);
+    line_dft<T, RS>(xl, f, tws, lane);
     if (ii < nkx)
       for (int k = tid >> f.lgnl; k < nzout; k += f.tpl)
         X2[ii + long(nkx) * j + pl * k] = cconj(lds[l * f.ldl + k]);
@@ -182,16 +194,17 @@ __global__ void __launch_bounds__(256) k_z_s2x(const Cp* __restrict__ S, Cp* __r
 // y transform of X (lines along y, stride nkx), one plane per blockIdx.y.
 // DUAL (backward): Y2 <- y IFFT of (i ky) Xin from the same load.
 // ---------------------------------------------------------------------------
-template <int RS, bool DUAL>
-__global__ void __launch_bounds__(256) k_y(const Cp* Xin, Cp* Xout, Cp* __restrict__ Y2, LF f,
-                                           int nkx, int ny, int forward,
-                                           const double* __restrict__ ky) {
-  SPX_PROLOGUE(f);
+template <typename T, int RS, bool DUAL>
+__global__ void __launch_bounds__(256) k_y(const cpx<T>* Xin, cpx<T>* Xout,
+                                           cpx<T>* __restrict__ Y2, LineFFT<cpx<T>> f, int nkx,
+                                           int ny, int forward, const double* __restrict__ ky) {
+  using Cp = cpx<T>;
+  SPX_PROLOGUE(Cp, f);
   const int i0 = blockIdx.x * f.nl;
   const long base = long(blockIdx.y) * nkx * ny;
   const int l = tid & (f.nl - 1), ii = i0 + l;
   for (int jy = tid >> f.lgnl; jy < ny; jy += f.tpl) {
-    Cp v = mkc<Tp>(0, 0);
+    Cp v = mkc<T>(0, 0);
     if (ii < nkx) v = Xin[base + ii + long(nkx) * jy];
     if (!forward) v.y = -v.y;
     lds[l * f.ldl + jy] = v;
@@ -205,7 +218,7 @@ __global__ void __launch_bounds__(256) k_y(const Cp* Xin, Cp* Xout, Cp* __restri
       if (k < ny) keep[q] = xl[k];
     }
   }
-  line_dft<Tp, RS>(xl, f, tws, lane);
+  line_dft<T, RS>(xl, f, tws, lane);
   if (ii < nkx)
     for (int jy = tid >> f.lgnl; jy < ny; jy += f.tpl) {
       Cp v = lds[l * f.ldl + jy];
@@ -217,10 +230,10 @@ __global__ void __launch_bounds__(256) k_y(const Cp* Xin, Cp* Xout, Cp* __restri
 #pragma unroll
     for (int q = 0; q < KEEP; ++q) {
       const int k = lane + q * f.tpl;
-      if (k < ny) xl[k] = cconj(ikmul(ky[k], cconj(keep[q])));
+      if (k < ny) xl[k] = cconj(ikmul(T(ky[k]), cconj(keep[q])));
     }
     __syncthreads();
-    line_dft<Tp, RS>(xl, f, tws, lane);
+    line_dft<T, RS>(xl, f, tws, lane);
     if (ii < nkx)
       for (int jy = tid >> f.lgnl; jy < ny; jy += f.tpl)
         Y2[base + ii + long(nkx) * jy] = cconj(lds[l * f.ldl + jy]);
@@ -230,24 +243,26 @@ __global__ void __launch_bounds__(256) k_y(const Cp* Xin, Cp* Xout, Cp* __restri
 // ---------------------------------------------------------------------------
 // x real -> half complex: rows 2p, 2p+1 packed as one complex line
 // ---------------------------------------------------------------------------
-template <int RS>
-__global__ void __launch_bounds__(256) k_xr2c(const Tp* __restrict__ R, Cp* __restrict__ X, LF f,
-                                              int nkx, long npairs) {
-  SPX_PROLOGUE(f);
+template <typename T, int RS>
+__global__ void __launch_bounds__(256) k_xr2c(const T* __restrict__ R, cpx<T>* __restrict__ X,
+                                              LineFFT<cpx<T>> f, int nkx, long npairs) {
+  using Cp = cpx<T>;
+  SPX_PROLOGUE(Cp, f);
   const int nx = f.n;
   const long p = long(blockIdx.x) * f.nl + line;
   const bool ok = p < npairs;
-  const Tp* r0 = R + long(nx) * (2 * p);
-  for (int x = lane; x < nx; x += f.tpl) xl[x] = ok ? mkc<Tp>(r0[x], r0[x + nx]) : mkc<Tp>(0, 0);
+  const T* r0 = R + long(nx) * (2 * (ok ? p : 0));
+  for (int x = lane; x < nx; x += f.tpl) xl[x] = ok ? mkc<T>(r0[x], r0[x + nx]) : mkc<T>(0, 0);
   __syncthreads();
-  line_dft<Tp, RS>(xl, f, tws, lane);
+  line_dft<T, RS>(xl, f, tws, lane);
   if (ok) {
     Cp* o0 = X + long(nkx) * (2 * p);
+    const T h = T(0.5);
     for (int k = lane; k < nkx; k += f.tpl) {
       const Cp Z = xl[k];
       const Cp Zc = xl[k == 0 ? 0 : nx - k];
-      o0[k] = mkc<Tp>((Z.x + Zc.x) * 0.5, (Z.y - Zc.y) * 0.5);
-      o0[k + nkx] = mkc<Tp>((Z.y + Zc.y) * 0.5, -(Z.x - Zc.x) * 0.5);
+      o0[k] = mkc<T>((Z.x + Zc.x) * h, (Z.y - Zc.y) * h);
+      o0[k + nkx] = mkc<T>((Z.y + Zc.y) * h, -(Z.x - Zc.x) * h);
     }
   }
 }
@@ -255,45 +270,47 @@ __global__ void __launch_bounds__(256) k_xr2c(const Tp* __restrict__ R, Cp* __re
 // Load of an x c2r row pair into LDS: Z = A + iB from the two half spectra
 // (FFTW c2r semantics: Im of kx = 0 and kx = nx/2 ignored), conjugated for the
 // backward transform; dirx multiplies both by i kx first.
-__device__ __forceinline__ void load_c2r_pair(Cp* xl, const Cp* __restrict__ a0, int nkx, int nx,
-                                              int lane, int tpl, bool ok, int dirx,
+template <typename T>
+__device__ __forceinline__ void load_c2r_pair(cpx<T>* xl, const cpx<T>* __restrict__ a0, int nkx,
+                                              int nx, int lane, int tpl, bool ok, int dirx,
                                               const double* __restrict__ kx) {
   const int h = nx / 2;
   for (int m = lane; m < nx; m += tpl) {
-    Cp v = mkc<Tp>(0, 0);
+    cpx<T> v = mkc<T>(0, 0);
     if (ok) {
       const bool up = m > h;
       const int k = up ? nx - m : m;
-      Cp A = a0[k];
-      Cp B = a0[k + nkx];
+      cpx<T> A = a0[k];
+      cpx<T> B = a0[k + nkx];
       if (dirx) {
-        A = ikmul(kx[k], A);
-        B = ikmul(kx[k], B);
+        A = ikmul(T(kx[k]), A);
+        B = ikmul(T(kx[k]), B);
       }
       if (k == 0 || k == h) {
         A.y = 0;
         B.y = 0;
       }
-      v = up ? mkc<Tp>(A.x + B.y, -A.y + B.x) : mkc<Tp>(A.x - B.y, A.y + B.x);
+      v = up ? mkc<T>(A.x + B.y, -A.y + B.x) : mkc<T>(A.x - B.y, A.y + B.x);
       v.y = -v.y;
     }
     xl[m] = v;
   }
 }
 
-template <int RS>
-__global__ void __launch_bounds__(256) k_xc2r(const Cp* __restrict__ X, Tp* __restrict__ R, LF f,
-                                              int nkx, long npairs, int dirx,
+template <typename T, int RS>
+__global__ void __launch_bounds__(256) k_xc2r(const cpx<T>* __restrict__ X, T* __restrict__ R,
+                                              LineFFT<cpx<T>> f, int nkx, long npairs, int dirx,
                                               const double* __restrict__ kx) {
-  SPX_PROLOGUE(f);
+  using Cp = cpx<T>;
+  SPX_PROLOGUE(Cp, f);
   const int nx = f.n;
   const long p = long(blockIdx.x) * f.nl + line;
   const bool ok = p < npairs;
   load_c2r_pair(xl, X + long(nkx) * (2 * (ok ? p : 0)), nkx, nx, lane, f.tpl, ok, dirx, kx);
   __syncthreads();
-  line_dft<Tp, RS>(xl, f, tws, lane);
+  line_dft<T, RS>(xl, f, tws, lane);
   if (ok) {
-    Tp* r0 = R + long(nx) * (2 * p);
+    T* r0 = R + long(nx) * (2 * p);
     for (int x = lane; x < nx; x += f.tpl) {
       const Cp z = xl[x];
       r0[x] = z.x;
@@ -305,37 +322,38 @@ __global__ void __launch_bounds__(256) k_xc2r(const Cp* __restrict__ X, Tp* __re
 // ((u*Dx + v*Dy) + w*Dz)*tmp on a row pair, the three derivatives c2r'd in
 // LDS one after the other and accumulated in registers
 // (pseudospec_hd.f90:254-312, pseudospec_phd.f90:63-108).
-template <int RS>
-__global__ void __launch_bounds__(256) k_xgrad_prod(const Cp* __restrict__ YV,
-                                                    const Cp* __restrict__ YY,
-                                                    const Cp* __restrict__ XZ,
-                                                    const Tp* __restrict__ u,
-                                                    const Tp* __restrict__ v,
-                                                    const Tp* __restrict__ w,
-                                                    Tp* __restrict__ out, LF f, int nkx,
-                                                    long npairs, double tmp,
-                                                    const double* __restrict__ kx) {
+template <typename T, int RS>
+__global__ void __launch_bounds__(256) k_xgrad_prod(const cpx<T>* __restrict__ YV,
+                                                    const cpx<T>* __restrict__ YY,
+                                                    const cpx<T>* __restrict__ XZ,
+                                                    const T* __restrict__ u,
+                                                    const T* __restrict__ v,
+                                                    const T* __restrict__ w, T* __restrict__ out,
+                                                    LineFFT<cpx<T>> f, int nkx, long npairs,
+                                                    double tmpd, const double* __restrict__ kx) {
 #pragma clang fp contract(off)
-  SPX_PROLOGUE(f);
+  using Cp = cpx<T>;
+  SPX_PROLOGUE(Cp, f);
   const int nx = f.n;
+  const T tmp = T(tmpd);
   const long p = long(blockIdx.x) * f.nl + line;
   const bool ok = p < npairs;
   const long ro = long(nx) * (2 * (ok ? p : 0));
   const long co = long(nkx) * (2 * (ok ? p : 0));
-  Tp accA[KEEP], accB[KEEP];
+  T accA[KEEP], accB[KEEP];
   const Cp* srcs[3] = {YV, YY, XZ};
-  const Tp* mult[3] = {u, v, w};
+  const T* mult[3] = {u, v, w};
   for (int r = 0; r < 3; ++r) {
     load_c2r_pair(xl, srcs[r] + co, nkx, nx, lane, f.tpl, ok, r == 0 ? 1 : 0, kx);
     __syncthreads();
-    line_dft<Tp, RS>(xl, f, tws, lane);
-    const Tp* m = mult[r] + ro;
+    line_dft<T, RS>(xl, f, tws, lane);
+    const T* m = mult[r] + ro;
 #pragma unroll
     for (int q = 0; q < KEEP; ++q) {
       const int x = lane + q * f.tpl;
       if (x < nx && ok) {
         const Cp z = xl[x];
-        const Tp a = z.x, b = -z.y;
+        const T a = z.x, b = -z.y;
         if (r == 0) {
           accA[q] = m[x] * a;
           accB[q] = m[x + nx] * b;
@@ -363,10 +381,14 @@ __global__ void __launch_bounds__(256) k_xgrad_prod(const Cp* __restrict__ YV,
 }
 
 // ---------------------------------------------------------------------------
+// float64 boundary kernels; TS is the storage type of the fields
+// ---------------------------------------------------------------------------
+using Cd = cpx<double>;
+using LFd = LineFFT<Cd>;
+
 // goto_domain_w_boundaries + noslip_z + goto_3d_fourier fused per line
 // (boundary_mod.fpp:72-194, vboundary.f90:154-211, sboundary.f90:122-165).
 // comp 0: vx (kx), comp 1: vy (ky); scalar mode: wall values are zero.
-// ---------------------------------------------------------------------------
 struct WallParams {
   double tau;          // pressure time-extrapolation factor
   double vb[2][2];     // [comp][pos] wall velocity
@@ -374,100 +396,99 @@ struct WallParams {
   int scalar;          // 1: s_constant_z (zero at walls)
 };
 
-template <int RS>
-__global__ void __launch_bounds__(256) k_noslip(Cp* __restrict__ vx, Cp* __restrict__ vy,
-                                                const Cp* __restrict__ pr, LF f, long nlines,
+template <typename TS, int RS>
+__global__ void __launch_bounds__(256) k_noslip(cpx<TS>* __restrict__ vx, cpx<TS>* __restrict__ vy,
+                                                const Cd* __restrict__ pr, LFd f, long nlines,
                                                 int ny, int np, double invn, WallParams wp,
                                                 const double* __restrict__ kx,
                                                 const double* __restrict__ ky,
                                                 const double* __restrict__ dir, int C, int d) {
 #pragma clang fp contract(off)
-  SPX_PROLOGUE(f);
+  SPX_PROLOGUE(Cd, f);
   const int n = f.n;
   const int comp = blockIdx.y;
-  Cp* V = comp ? vy : vx;
+  cpx<TS>* V = comp ? vy : vx;
   const long ln = long(blockIdx.x) * f.nl + line;
   const bool ok = ln < nlines;
-  for (int k = lane; k < n; k += f.tpl) xl[k] = ok ? cconj(V[ln * n + k]) : mkc<Tp>(0, 0);
+  for (int k = lane; k < n; k += f.tpl)
+    xl[k] = ok ? cconj(ccast<double>(V[ln * n + k])) : mkc<double>(0, 0);
   __syncthreads();
-  line_dft<Tp, RS>(xl, f, tws, lane);
+  line_dft<double, RS>(xl, f, tws, lane);
   for (int k = lane; k < np; k += f.tpl) xl[k] = cscale(cconj(xl[k]), invn);
   __syncthreads();
   if (lane < 2 && ok) {
     const int w = lane;
     const int ind = w ? np - 1 : 0;
-    Cp val;
+    Cd val;
     if (wp.scalar) {
-      val = mkc<Tp>(0, 0);
+      val = mkc<double>(0, 0);
     } else {
       const long i = ln / ny;
       const int j = int(ln - i * ny);
-      const Cp pv = pr[ln * n + ind];
+      const Cd pv = pr[ln * n + ind];
       val = cscale(ikmul(comp ? ky[j] : kx[i], pv), wp.tau);
-      if (i == 0 && j == 0) val = mkc<Tp>(wp.nxny * wp.vb[comp][w], 0.0);
+      if (i == 0 && j == 0) val = mkc<double>(wp.nxny * wp.vb[comp][w], 0.0);
     }
     xl[ind] = val;
   }
   __syncthreads();
   fc_continue_lds(lds, f.nl, f.ldl, np, C, d, dir);
   __syncthreads();
-  line_dft<Tp, RS>(xl, f, tws, lane);
+  line_dft<double, RS>(xl, f, tws, lane);
   if (ok)
-    for (int k = lane; k < n; k += f.tpl) V[ln * n + k] = xl[k];
+    for (int k = lane; k < n; k += f.tpl) V[ln * n + k] = ccast<TS>(xl[k]);
 }
 
-// ---------------------------------------------------------------------------
 // sol_project (boundary_mod.fpp:197-402), bctarget = 1, bczsta = bczend = 0
-// ---------------------------------------------------------------------------
 // bc(1:2) <- IFFT_z(c * (1/nz)) at z = 0 and z = Lz   (:262-352)
-template <int RS>
-__global__ void __launch_bounds__(256) k_proj_bc(const Cp* __restrict__ c, Cp* __restrict__ bc,
-                                                 LF f, long nlines, int np, double tmp) {
-  SPX_PROLOGUE(f);
+template <typename TS, int RS>
+__global__ void __launch_bounds__(256) k_proj_bc(const cpx<TS>* __restrict__ c, Cd* __restrict__ bc,
+                                                 LFd f, long nlines, int np, double tmp) {
+  SPX_PROLOGUE(Cd, f);
   const int n = f.n;
   const long ln = long(blockIdx.x) * f.nl + line;
   const bool ok = ln < nlines;
   for (int k = lane; k < n; k += f.tpl)
-    xl[k] = ok ? cconj(cscale(c[ln * n + k], tmp)) : mkc<Tp>(0, 0);
+    xl[k] = ok ? cconj(cscale(ccast<double>(c[ln * n + k]), tmp)) : mkc<double>(0, 0);
   __syncthreads();
-  line_dft<Tp, RS>(xl, f, tws, lane);
+  line_dft<double, RS>(xl, f, tws, lane);
   if (ok && lane < 2) bc[ln * 2 + lane] = cconj(xl[lane ? np - 1 : 0]);
 }
 
 // laplace_z pure-Neumann coefficients (boundary_mod.fpp:531-560)
 struct Coef {
-  Cp c1, c2;
+  Cd c1, c2;
 };
-__device__ __forceinline__ Coef neumann_coef(Cp b1, Cp b2, double kh, double Lz, bool zero_mode) {
+__device__ __forceinline__ Coef neumann_coef(Cd b1, Cd b2, double kh, double Lz, bool zero_mode) {
 #pragma clang fp contract(off)
   Coef r;
   if (zero_mode) {
     r.c1 = b1;
-    r.c2 = mkc<Tp>(0, 0);
+    r.c2 = mkc<double>(0, 0);
     return r;
   }
   const double tmp = 1.0 / (kh * (1 - exp(-2 * kh * Lz)));
   const double e1 = exp(-kh * Lz);
   r.c1 = cscale(csub(b2, cscale(b1, e1)), tmp);
-  r.c2 = cscale(cadd(mkc<Tp>(-b1.x, -b1.y), cscale(b2, e1)), tmp);
+  r.c2 = cscale(cadd(mkc<double>(-b1.x, -b1.y), cscale(b2, e1)), tmp);
   return r;
 }
 
-// d <- IFFT_z(d)*tmp + a_harm(z)  (:354-381 with laplace_z :631-675)
+// d <- IFFT_z(d)*tmp + a_harm(z)  (:354-381 with laplace_z :631-675); d is float64
 template <int RS>
-__global__ void __launch_bounds__(256) k_proj_pressure(Cp* __restrict__ dd,
-                                                       const Cp* __restrict__ bc, LF f,
+__global__ void __launch_bounds__(256) k_proj_pressure(Cd* __restrict__ dd,
+                                                       const Cd* __restrict__ bc, LFd f,
                                                        long nlines, int ny, double tmp, double Lz,
                                                        const double* __restrict__ khom,
                                                        const double* __restrict__ z) {
 #pragma clang fp contract(off)
-  SPX_PROLOGUE(f);
+  SPX_PROLOGUE(Cd, f);
   const int n = f.n;
   const long ln = long(blockIdx.x) * f.nl + line;
   const bool ok = ln < nlines;
-  for (int k = lane; k < n; k += f.tpl) xl[k] = ok ? cconj(dd[ln * n + k]) : mkc<Tp>(0, 0);
+  for (int k = lane; k < n; k += f.tpl) xl[k] = ok ? cconj(dd[ln * n + k]) : mkc<double>(0, 0);
   __syncthreads();
-  line_dft<Tp, RS>(xl, f, tws, lane);
+  line_dft<double, RS>(xl, f, tws, lane);
   if (!ok) return;
   const long i = ln / ny;
   const int j = int(ln - i * ny);
@@ -475,9 +496,9 @@ __global__ void __launch_bounds__(256) k_proj_pressure(Cp* __restrict__ dd,
   const double kh = khom[ln];
   const Coef cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
   for (int k = lane; k < n; k += f.tpl) {
-    Cp a;
+    Cd a;
     if (zm) {
-      a = mkc<Tp>(cf.c1.x * z[k] + cf.c2.x, 0.0);
+      a = mkc<double>(cf.c1.x * z[k] + cf.c2.x, 0.0);
     } else {
       const double ep = exp(kh * (z[k] - Lz));
       const double em = exp(-kh * z[k]);
@@ -489,10 +510,11 @@ __global__ void __launch_bounds__(256) k_proj_pressure(Cp* __restrict__ dd,
 
 // harmonic correction: A = FFT_z(cont(a_harm)), B = FFT_z(cont(d a_harm/dz));
 // vx -= i kx A, vy -= i ky A, vz -= B   (:383-399).  Two buffers per line.
-template <int RS>
-__global__ void __launch_bounds__(256) k_proj_harmonic(Cp* __restrict__ va, Cp* __restrict__ vb,
-                                                       Cp* __restrict__ vc,
-                                                       const Cp* __restrict__ bc, LF f,
+template <typename TS, int RS>
+__global__ void __launch_bounds__(256) k_proj_harmonic(cpx<TS>* __restrict__ va,
+                                                       cpx<TS>* __restrict__ vb,
+                                                       cpx<TS>* __restrict__ vc,
+                                                       const Cd* __restrict__ bc, LFd f,
                                                        long nlines, int ny, int np, double Lz,
                                                        const double* __restrict__ khom,
                                                        const double* __restrict__ z,
@@ -501,10 +523,10 @@ __global__ void __launch_bounds__(256) k_proj_harmonic(Cp* __restrict__ va, Cp* 
                                                        const double* __restrict__ dir, int C,
                                                        int d) {
 #pragma clang fp contract(off)
-  SPX_PROLOGUE(f);
+  SPX_PROLOGUE(Cd, f);
   const int n = f.n;
-  Cp* const ldsB = lds + f.nl * f.ldl;
-  Cp* const xb = ldsB + line * f.ldl;
+  Cd* const ldsB = lds + f.nl * f.ldl;
+  Cd* const xb = ldsB + line * f.ldl;
   const long ln = long(blockIdx.x) * f.nl + line;
   const bool ok = ln < nlines;
   const long i = ok ? ln / ny : 0;
@@ -512,18 +534,18 @@ __global__ void __launch_bounds__(256) k_proj_harmonic(Cp* __restrict__ va, Cp* 
   const bool zm = (i == 0 && j == 0);
   const double kh = ok ? khom[ln] : 0.0;
   Coef cf;
-  cf.c1 = cf.c2 = mkc<Tp>(0, 0);
+  cf.c1 = cf.c2 = mkc<double>(0, 0);
   if (ok) cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
   for (int k = lane; k < np; k += f.tpl) {
-    Cp a = mkc<Tp>(0, 0), b = mkc<Tp>(0, 0);
+    Cd a = mkc<double>(0, 0), b = mkc<double>(0, 0);
     if (ok) {
       if (zm) {
-        a = mkc<Tp>(cf.c1.x * z[k] + cf.c2.x, 0.0);
-        b = mkc<Tp>(cf.c1.x, 0.0);
+        a = mkc<double>(cf.c1.x * z[k] + cf.c2.x, 0.0);
+        b = mkc<double>(cf.c1.x, 0.0);
       } else {
         const double ep = exp(kh * (z[k] - Lz));
         const double em = exp(-kh * z[k]);
-        const Cp t1 = cscale(cf.c1, ep), t2 = cscale(cf.c2, em);
+        const Cd t1 = cscale(cf.c1, ep), t2 = cscale(cf.c2, em);
         a = cadd(t1, t2);
         b = cscale(csub(t1, t2), kh);
       }
@@ -535,21 +557,21 @@ __global__ void __launch_bounds__(256) k_proj_harmonic(Cp* __restrict__ va, Cp* 
   fc_continue_lds(lds, f.nl, f.ldl, np, C, d, dir);
   fc_continue_lds(ldsB, f.nl, f.ldl, np, C, d, dir);
   __syncthreads();
-  line_dft<Tp, RS>(xl, f, tws, lane);
-  line_dft<Tp, RS>(xb, f, tws, lane);
+  line_dft<double, RS>(xl, f, tws, lane);
+  line_dft<double, RS>(xb, f, tws, lane);
   if (!ok) return;
   const double kxi = kx[i], kyj = ky[j];
   for (int k = lane; k < n; k += f.tpl) {
-    const Cp A = xl[k], B = xb[k];
+    const Cd A = xl[k], B = xb[k];
     const long o = ln * n + k;
-    va[o] = csub(va[o], ikmul(kxi, A));
-    vb[o] = csub(vb[o], ikmul(kyj, A));
-    vc[o] = csub(vc[o], B);
+    va[o] = ccast<TS>(csub(ccast<double>(va[o]), ikmul(kxi, A)));
+    vb[o] = ccast<TS>(csub(ccast<double>(vb[o]), ikmul(kyj, A)));
+    vc[o] = ccast<TS>(csub(ccast<double>(vc[o]), B));
   }
 }
 
 // ---------------------------------------------------------------------------
-// diagnostics: per-line reductions over physical z in the mixed domain
+// diagnostics: per-line float64 reductions over physical z in the mixed domain
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double line_reduce(double v, double* red, int line, int lane, int tpl) {
   red[line * tpl + lane] = v;
@@ -561,13 +583,14 @@ __device__ __forceinline__ double line_reduce(double v, double* red, int line, i
   return red[line * tpl];
 }
 
+template <typename TS>
 struct LinCombDev {
-  const Cp* in[3];
+  const cpx<TS>* in[3];
   int dir[3];
   int sgn[3];
 };
 
-__device__ __forceinline__ Cp deriv_apply(Cp v, int dirn, double kxi, double kyj, double kzk) {
+__device__ __forceinline__ Cd deriv_apply(Cd v, int dirn, double kxi, double kyj, double kzk) {
   if (dirn == 1) return ikmul(kxi, v);
   if (dirn == 2) return ikmul(kyj, v);
   if (dirn == 3) return ikmul(kzk, v);
@@ -575,44 +598,45 @@ __device__ __forceinline__ Cp deriv_apply(Cp v, int dirn, double kxi, double kyj
 }
 
 // mode 0: sum_k<np |IFFT(lincomb)|^2 ; mode 1: Re(IFFT(a) conj IFFT(b)) ; mode 2: walls |.|^2
-template <int RS>
-__global__ void __launch_bounds__(256) k_zline_reduce(LinCombDev lc, const Cp* __restrict__ bsrc,
-                                                      int mode, double* __restrict__ out, LF f,
+template <typename TS, int RS>
+__global__ void __launch_bounds__(256) k_zline_reduce(LinCombDev<TS> lc,
+                                                      const cpx<TS>* __restrict__ bsrc, int mode,
+                                                      double* __restrict__ out, LFd f,
                                                       long nlines, int ny, int np,
                                                       const double* __restrict__ kx,
                                                       const double* __restrict__ ky,
                                                       const double* __restrict__ kz) {
 #pragma clang fp contract(off)
-  SPX_PROLOGUE(f);
+  SPX_PROLOGUE(Cd, f);
   const int n = f.n;
-  Cp* const ldsB = lds + f.nl * f.ldl;
-  Cp* const xb = ldsB + line * f.ldl;
+  Cd* const ldsB = lds + f.nl * f.ldl;
+  Cd* const xb = ldsB + line * f.ldl;
   double* red = reinterpret_cast<double*>(ldsB + (mode == 1 ? f.nl * f.ldl : 0));
   const long ln = long(blockIdx.x) * f.nl + line;
   const bool ok = ln < nlines;
   const long i = ok ? ln / ny : 0;
   const int j = ok ? int(ln - i * ny) : 0;
   for (int k = lane; k < n; k += f.tpl) {
-    Cp v = mkc<Tp>(0, 0), w = mkc<Tp>(0, 0);
+    Cd v = mkc<double>(0, 0), w = mkc<double>(0, 0);
     if (ok) {
       const double kxi = kx[i], kyj = ky[j], kzk = kz[k];
-      v = deriv_apply(lc.in[0][ln * n + k], lc.dir[0], kxi, kyj, kzk);
+      v = deriv_apply(ccast<double>(lc.in[0][ln * n + k]), lc.dir[0], kxi, kyj, kzk);
       for (int t = 1; t < 3; ++t) {
         if (lc.sgn[t] == 0) continue;
-        const Cp u = deriv_apply(lc.in[t][ln * n + k], lc.dir[t], kxi, kyj, kzk);
+        const Cd u = deriv_apply(ccast<double>(lc.in[t][ln * n + k]), lc.dir[t], kxi, kyj, kzk);
         v = lc.sgn[t] > 0 ? cadd(v, u) : csub(v, u);
       }
-      if (mode == 1) w = bsrc[ln * n + k];
+      if (mode == 1) w = ccast<double>(bsrc[ln * n + k]);
     }
     xl[k] = cconj(v);
     if (mode == 1) xb[k] = cconj(w);
   }
   __syncthreads();
-  line_dft<Tp, RS>(xl, f, tws, lane);
-  if (mode == 1) line_dft<Tp, RS>(xb, f, tws, lane);
+  line_dft<double, RS>(xl, f, tws, lane);
+  if (mode == 1) line_dft<double, RS>(xb, f, tws, lane);
   if (mode == 2) {
     if (lane == 0 && ok) {
-      const Cp b = xl[0], t = xl[np - 1];
+      const Cd b = xl[0], t = xl[np - 1];
       out[2 * ln] = b.x * b.x + b.y * b.y;
       out[2 * ln + 1] = t.x * t.x + t.y * t.y;
     }
@@ -620,11 +644,11 @@ __global__ void __launch_bounds__(256) k_zline_reduce(LinCombDev lc, const Cp* _
   }
   double acc = 0.0;
   for (int k = lane; k < np; k += f.tpl) {
-    const Cp v = xl[k];
+    const Cd v = xl[k];
     if (mode == 0) {
       acc += v.x * v.x + v.y * v.y;
     } else {
-      const Cp w = xb[k];
+      const Cd w = xb[k];
       // values hold conj(IFFT); Re(p conj(q)) is invariant under conj of both
       acc += v.x * w.x + v.y * w.y;
     }
@@ -735,6 +759,14 @@ static int dev_upload(void** dst, const void* src, size_t bytes) {
   return SPX_OK;
 }
 
+// upload a complex table in the plan's precision (csize 16: float64, 8: float32)
+static int upload_table(void** dst, const std::vector<double>& v, int csize) {
+  if (csize == 16) return dev_upload(dst, v.data(), v.size() * sizeof(double));
+  std::vector<float> f(v.size());
+  for (size_t i = 0; i < v.size(); ++i) f[i] = float(v[i]);
+  return dev_upload(dst, f.data(), f.size() * sizeof(float));
+}
+
 // naive long-double DFT of length M (used once per plan for the filters)
 static void dft_ld(const std::vector<long double>& xr, const std::vector<long double>& xi,
                    const std::vector<long double>& wr, const std::vector<long double>& wi,
@@ -807,7 +839,7 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
     tw[2 * m] = double(wr[m]);
     tw[2 * m + 1] = double(wi[m]);
   }
-  int rc = dev_upload(&f.tw, tw.data(), tw.size() * sizeof(double));
+  int rc = upload_table(&f.tw, tw, csize);
   if (rc) return rc;
   if (f.mode == 1) {
     // chirp w[m] = exp(-pi i (m^2 mod 2n)/n)
@@ -833,8 +865,8 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
     }
     std::vector<double> bf;
     dft_ld(br, bi, wr, wi, bf, 1.0L / M);
-    if ((rc = dev_upload(&f.chirp, ch.data(), ch.size() * sizeof(double)))) return rc;
-    if ((rc = dev_upload(&f.bfil, bf.data(), bf.size() * sizeof(double)))) return rc;
+    if ((rc = upload_table(&f.chirp, ch, csize))) return rc;
+    if ((rc = upload_table(&f.bfil, bf, csize))) return rc;
   } else if (f.mode == 2) {
     // Rader: a_q = x[g^q], b_m = W_n^(g^-m), X[g^-p] = x0 + (a (*) b)_p
     const int g = primitive_root(n);
@@ -853,7 +885,7 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
     }
     std::vector<double> bf;
     dft_ld(br, bi, wr, wi, bf, 1.0L / M);
-    if ((rc = dev_upload(&f.bfil, bf.data(), bf.size() * sizeof(double)))) return rc;
+    if ((rc = upload_table(&f.bfil, bf, csize))) return rc;
     if ((rc = dev_upload(&f.perm, perm.data(), perm.size() * sizeof(int)))) return rc;
   }
   return SPX_OK;
@@ -893,31 +925,50 @@ static int rset_of(const LineFFTHost& f) {
 }
 
 #define SPX_COMMA ,
-// launch the instance of KER compiled for the plan's radix set
-#define SPX_LAUNCH_RS(F, KER, TARGS, GRID, BLOCK, LDS, STREAM, ...)                   \
-  do {                                                                               \
-    const int rs_ = rset_of(F);                                                      \
-    if (rs_ == RS_POW2) {                                                            \
-      if (int rc_ = lds_limit(KER<RS_POW2 TARGS>, (LDS))) return rc_;                \
-      hipLaunchKernelGGL((KER<RS_POW2 TARGS>), GRID, BLOCK, LDS, STREAM, __VA_ARGS__); \
-    } else if (rs_ == RS_SMOOTH) {                                                   \
-      if (int rc_ = lds_limit(KER<RS_SMOOTH TARGS>, (LDS))) return rc_;              \
-      hipLaunchKernelGGL((KER<RS_SMOOTH TARGS>), GRID, BLOCK, LDS, STREAM, __VA_ARGS__); \
-    } else {                                                                         \
-      if (int rc_ = lds_limit(KER<RS_ALL TARGS>, (LDS))) return rc_;                 \
-      hipLaunchKernelGGL((KER<RS_ALL TARGS>), GRID, BLOCK, LDS, STREAM, __VA_ARGS__); \
-    }                                                                                \
+// launch the instance KER<TY, RS TARGS> compiled for the plan's radix set
+#define SPX_LAUNCH_RS(F, KER, TY, TARGS, GRID, BLOCK, LDS, STREAM, ...)                      \
+  do {                                                                                       \
+    const int rs_ = rset_of(F);                                                              \
+    if (rs_ == RS_POW2) {                                                                    \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_POW2 TARGS>, (LDS))) return rc_;           \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_POW2 TARGS>), GRID, BLOCK, LDS, STREAM,        \
+                         __VA_ARGS__);                                                       \
+    } else if (rs_ == RS_SMOOTH) {                                                           \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_SMOOTH TARGS>, (LDS))) return rc_;         \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_SMOOTH TARGS>), GRID, BLOCK, LDS, STREAM,      \
+                         __VA_ARGS__);                                                       \
+    } else {                                                                                 \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_ALL TARGS>, (LDS))) return rc_;            \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_ALL TARGS>), GRID, BLOCK, LDS, STREAM,         \
+                         __VA_ARGS__);                                                       \
+    }                                                                                        \
   } while (0)
+
+template <typename T>
+static inline cpx<T>* C_(void* p) { return static_cast<cpx<T>*>(p); }
+template <typename T>
+static inline const cpx<T>* C_(const void* p) { return static_cast<const cpx<T>*>(p); }
+
+// ---- transforms ----
+// lo selects the float32 kernels and tables of a mixed plan.
+static inline const LineFFTHost& fxp(const spx_plan* p, bool lo) { return lo ? p->fx32 : p->fx; }
+static inline const LineFFTHost& fyp(const spx_plan* p, bool lo) { return lo ? p->fy32 : p->fy; }
+static inline const LineFFTHost& fzp(const spx_plan* p, bool lo) { return lo ? p->fz32 : p->fz; }
+
+static int check_lo(const spx_plan* p, bool lo) {
+  if (lo && !p->fp32) return fail(SPX_ERR_STATE, "float32 kernels need a mixed-precision plan");
+  return SPX_OK;
+}
 
 int zfft_inplace(spx_plan* p, void* S, int forward, int cont, double scale) {
   const LineFFTHost& f = p->fz;
   const long nlines = long(p->g.ny) * p->nkx;
   const unsigned nb = unsigned((nlines + f.nl - 1) / f.nl);
   const int C = p->g.Cz, d = p->g.oz;
-  ProfScope ps_(p, "z_inplace", double(p->nc) * 32.0);
-  SPX_LAUNCH_RS(f, k_z_inplace,, dim3(nb), blk(f), f.lds_bytes(p->csize), p->stream,
-                     static_cast<Cp*>(S), nlines, f.dev<Cp>(), p->np, forward,
-                     (cont && C > 0) ? 1 : 0, scale, p->d_dir, C, d);
+  ProfScope ps_(p, "z_inplace", double(p->nc) * 2.0 * 16.0);
+  SPX_LAUNCH_RS(f, k_z_inplace, double, , dim3(nb), blk(f), f.lds_bytes(16), p->stream,
+                C_<double>(S), nlines, f.dev<Cd>(), p->np, forward, (cont && C > 0) ? 1 : 0,
+                scale, p->d_dir, C, d);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
@@ -928,110 +979,160 @@ int fc_continue_only(spx_plan* p, void* S) {
   if (C <= 0) return SPX_OK;
   const long tot = nlines * C;
   ProfScope ps_(p, "fc_continue", double(nlines) * 16.0 * (2 * p->g.oz + C));
-  hipLaunchKernelGGL(k_fc_continue, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, p->stream,
-                     static_cast<Cp*>(S), nlines, p->g.nz, p->np, C, p->g.oz, p->d_dir);
+  hipLaunchKernelGGL(k_fc_continue<double>, dim3(unsigned((tot + 255) / 256)), dim3(256), 0,
+                     p->stream, C_<double>(S), nlines, p->g.nz, p->np, C, p->g.oz, p->d_dir);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
-int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft) {
-  const LineFFTHost& f = p->fz;
+template <typename T>
+static int zfft_x2s_t(spx_plan* p, const void* X, void* S, int nofft, const LineFFTHost& f) {
   dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(p->g.ny));
-  ProfScope ps_(p, "z_x2s",
-                double(p->nkx) * p->g.ny * 16.0 * ((nofft ? p->g.nz : p->np) + p->g.nz));
-  SPX_LAUNCH_RS(f, k_z_x2s,, grid, blk(f), f.lds_bytes(p->csize), p->stream,
-                     static_cast<const Cp*>(X), static_cast<Cp*>(S), f.dev<Cp>(), p->nkx,
-                     p->g.ny, p->np, nofft, p->d_dir, p->g.Cz, p->g.oz);
+  ProfScope ps_(p, "z_x2s", double(p->nkx) * p->g.ny * sizeof(cpx<T>) *
+                                ((nofft ? p->g.nz : p->np) + p->g.nz));
+  SPX_LAUNCH_RS(f, k_z_x2s, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
+                C_<T>(X), C_<T>(S), f.dev<cpx<T>>(), p->nkx, p->g.ny, p->np, nofft, p->d_dir,
+                p->g.Cz, p->g.oz);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
+int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft, bool lo) {
+  if (int rc = check_lo(p, lo)) return rc;
+  return lo ? zfft_x2s_t<float>(p, X, S, nofft, p->fz32)
+            : zfft_x2s_t<double>(p, X, S, nofft, p->fz);
+}
 
-int zfft_s2x(spx_plan* p, const void* S, void* X, int dirn, int nzout, int nofft, void* X2) {
-  const LineFFTHost& f = p->fz;
+// S is float64 in both precisions; X (and X2) are T
+template <typename T>
+static int zfft_s2x_t(spx_plan* p, const void* S, void* X, int dirn, int nzout, int nofft,
+                      void* X2, const LineFFTHost& f) {
   dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(p->g.ny));
-  const double io = double(p->nkx) * p->g.ny * 16.0;
+  const double line = double(p->nkx) * p->g.ny;
+  const Cd* in = C_<double>(S);
   if (X2) {
-    ProfScope ps_(p, "z_s2x_dual", io * (p->g.nz + 2.0 * nzout));
-    SPX_LAUNCH_RS(f, k_z_s2x, SPX_COMMA true, grid, blk(f), f.lds_bytes(p->csize), p->stream,
-                       static_cast<const Cp*>(S), static_cast<Cp*>(X), static_cast<Cp*>(X2),
-                       f.dev<Cp>(), p->nkx, p->g.ny, nzout, dirn, nofft, p->d_kx, p->d_ky,
-                       p->d_kz);
+    ProfScope ps_(p, "z_s2x_dual", line * (16.0 * p->g.nz + 2.0 * sizeof(cpx<T>) * nzout));
+    SPX_LAUNCH_RS(f, k_z_s2x, T, SPX_COMMA true SPX_COMMA double, grid, blk(f),
+                  f.lds_bytes(sizeof(cpx<T>)), p->stream, in, C_<T>(X), C_<T>(X2),
+                  f.dev<cpx<T>>(), p->nkx, p->g.ny, nzout, dirn, nofft, p->d_kx, p->d_ky,
+                  p->d_kz);
   } else {
-    ProfScope ps_(p, "z_s2x", io * (p->g.nz + nzout));
-    SPX_LAUNCH_RS(f, k_z_s2x, SPX_COMMA false, grid, blk(f), f.lds_bytes(p->csize), p->stream,
-                       static_cast<const Cp*>(S), static_cast<Cp*>(X), static_cast<Cp*>(nullptr),
-                       f.dev<Cp>(), p->nkx, p->g.ny, nzout, dirn, nofft, p->d_kx, p->d_ky,
-                       p->d_kz);
+    ProfScope ps_(p, "z_s2x", line * (16.0 * p->g.nz + sizeof(cpx<T>) * nzout));
+    SPX_LAUNCH_RS(f, k_z_s2x, T, SPX_COMMA false SPX_COMMA double, grid, blk(f),
+                  f.lds_bytes(sizeof(cpx<T>)), p->stream, in, C_<T>(X),
+                  static_cast<cpx<T>*>(nullptr), f.dev<cpx<T>>(), p->nkx, p->g.ny, nzout, dirn,
+                  nofft, p->d_kx, p->d_ky, p->d_kz);
   }
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
+int zfft_s2x(spx_plan* p, const void* S, void* X, int dirn, int nzout, int nofft, void* X2,
+             bool lo) {
+  if (int rc = check_lo(p, lo)) return rc;
+  return lo ? zfft_s2x_t<float>(p, S, X, dirn, nzout, nofft, X2, p->fz32)
+            : zfft_s2x_t<double>(p, S, X, dirn, nzout, nofft, X2, p->fz);
+}
 
-int yfft(spx_plan* p, void* X, int forward, int nplanes, const void* Xin, void* Y2) {
-  const LineFFTHost& f = p->fy;
+template <typename T>
+static int yfft_t(spx_plan* p, void* X, int forward, int nplanes, const void* Xin, void* Y2,
+                  const LineFFTHost& f) {
   dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(nplanes));
-  const Cp* in = static_cast<const Cp*>(Xin ? Xin : X);
-  const double io = double(p->nkx) * p->g.ny * nplanes * 16.0;
+  const cpx<T>* in = C_<T>(Xin ? Xin : static_cast<const void*>(X));
+  const double io = double(p->nkx) * p->g.ny * nplanes * sizeof(cpx<T>);
   if (Y2) {
     if (forward) return fail(SPX_ERR_ARG, "dual y transform is backward only");
     ProfScope ps_(p, "y_dual", io * 3.0);
-    SPX_LAUNCH_RS(f, k_y, SPX_COMMA true, grid, blk(f), f.lds_bytes(p->csize), p->stream, in,
-                       static_cast<Cp*>(X), static_cast<Cp*>(Y2), f.dev<Cp>(), p->nkx, p->g.ny,
-                       forward, p->d_ky);
+    SPX_LAUNCH_RS(f, k_y, T, SPX_COMMA true, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
+                  p->stream, in, C_<T>(X), C_<T>(Y2), f.dev<cpx<T>>(), p->nkx, p->g.ny, forward,
+                  p->d_ky);
   } else {
     ProfScope ps_(p, "y", io * 2.0);
-    SPX_LAUNCH_RS(f, k_y, SPX_COMMA false, grid, blk(f), f.lds_bytes(p->csize), p->stream, in,
-                       static_cast<Cp*>(X), static_cast<Cp*>(nullptr), f.dev<Cp>(), p->nkx,
-                       p->g.ny, forward, p->d_ky);
+    SPX_LAUNCH_RS(f, k_y, T, SPX_COMMA false, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
+                  p->stream, in, C_<T>(X), static_cast<cpx<T>*>(nullptr), f.dev<cpx<T>>(),
+                  p->nkx, p->g.ny, forward, p->d_ky);
   }
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
+int yfft(spx_plan* p, void* X, int forward, int nplanes, const void* Xin, void* Y2, bool lo) {
+  if (int rc = check_lo(p, lo)) return rc;
+  return lo ? yfft_t<float>(p, X, forward, nplanes, Xin, Y2, p->fy32)
+            : yfft_t<double>(p, X, forward, nplanes, Xin, Y2, p->fy);
+}
 
-int xr2c(spx_plan* p, const void* R, void* X, int nplanes) {
-  const LineFFTHost& f = p->fx;
+template <typename T>
+static int xr2c_t(spx_plan* p, const void* R, void* X, int nplanes, const LineFFTHost& f) {
   const long npairs = long(p->g.ny) * nplanes / 2;
   dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
-  ProfScope ps_(p, "xr2c", double(p->g.ny) * nplanes * (8.0 * p->g.nx + 16.0 * p->nkx));
-  SPX_LAUNCH_RS(f, k_xr2c,, grid, blk(f), f.lds_bytes(p->csize), p->stream,
-                     static_cast<const Tp*>(R), static_cast<Cp*>(X), f.dev<Cp>(), p->nkx, npairs);
+  ProfScope ps_(p, "xr2c", double(p->g.ny) * nplanes *
+                               (double(sizeof(T)) * p->g.nx + double(sizeof(cpx<T>)) * p->nkx));
+  SPX_LAUNCH_RS(f, k_xr2c, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
+                static_cast<const T*>(R), C_<T>(X), f.dev<cpx<T>>(), p->nkx, npairs);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
+int xr2c(spx_plan* p, const void* R, void* X, int nplanes, bool lo) {
+  if (int rc = check_lo(p, lo)) return rc;
+  return lo ? xr2c_t<float>(p, R, X, nplanes, p->fx32) : xr2c_t<double>(p, R, X, nplanes, p->fx);
+}
 
-int xc2r(spx_plan* p, const void* X, void* R, int nplanes, int dirx) {
-  const LineFFTHost& f = p->fx;
+template <typename T>
+static int xc2r_t(spx_plan* p, const void* X, void* R, int nplanes, int dirx,
+                  const LineFFTHost& f) {
   const long npairs = long(p->g.ny) * nplanes / 2;
   dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
-  ProfScope ps_(p, "xc2r", double(p->g.ny) * nplanes * (8.0 * p->g.nx + 16.0 * p->nkx));
-  SPX_LAUNCH_RS(f, k_xc2r,, grid, blk(f), f.lds_bytes(p->csize), p->stream,
-                     static_cast<const Cp*>(X), static_cast<Tp*>(R), f.dev<Cp>(), p->nkx, npairs,
-                     dirx, p->d_kx);
+  ProfScope ps_(p, "xc2r", double(p->g.ny) * nplanes *
+                               (double(sizeof(T)) * p->g.nx + double(sizeof(cpx<T>)) * p->nkx));
+  SPX_LAUNCH_RS(f, k_xc2r, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
+                C_<T>(X), static_cast<T*>(R), f.dev<cpx<T>>(), p->nkx, npairs, dirx, p->d_kx);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
+int xc2r(spx_plan* p, const void* X, void* R, int nplanes, int dirx, bool lo) {
+  if (int rc = check_lo(p, lo)) return rc;
+  return lo ? xc2r_t<float>(p, X, R, nplanes, dirx, p->fx32)
+            : xc2r_t<double>(p, X, R, nplanes, dirx, p->fx);
+}
 
-int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, const void* u,
-               const void* v, const void* w, void* out) {
-  const LineFFTHost& f = p->fx;
+template <typename T>
+static int xgrad_prod_t(spx_plan* p, const void* YV, const void* YY, const void* XZ,
+                        const void* u, const void* v, const void* w, void* out,
+                        const LineFFTHost& f) {
   const long npairs = long(p->g.ny) * p->np / 2;
   dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
   const double t = double(p->g.nx) * double(p->g.ny) * double(p->g.nz);
   const double rows = double(p->g.ny) * p->np;
-  ProfScope ps_(p, "xgrad_prod", rows * (3.0 * 16.0 * p->nkx + 4.0 * 8.0 * p->g.nx));
-  SPX_LAUNCH_RS(f, k_xgrad_prod,, grid, blk(f), f.lds_bytes(p->csize), p->stream,
-                     static_cast<const Cp*>(YV), static_cast<const Cp*>(YY),
-                     static_cast<const Cp*>(XZ), static_cast<const Tp*>(u),
-                     static_cast<const Tp*>(v), static_cast<const Tp*>(w), static_cast<Tp*>(out),
-                     f.dev<Cp>(), p->nkx, npairs, 1.0 / (t * t), p->d_kx);
+  ProfScope ps_(p, "xgrad_prod", rows * (3.0 * sizeof(cpx<T>) * p->nkx +
+                                         4.0 * sizeof(T) * p->g.nx));
+  SPX_LAUNCH_RS(f, k_xgrad_prod, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
+                C_<T>(YV), C_<T>(YY), C_<T>(XZ), static_cast<const T*>(u),
+                static_cast<const T*>(v), static_cast<const T*>(w), static_cast<T*>(out),
+                f.dev<cpx<T>>(), p->nkx, npairs, 1.0 / (t * t), p->d_kx);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, const void* u,
+               const void* v, const void* w, void* out, bool lo) {
+  if (int rc = check_lo(p, lo)) return rc;
+  return lo ? xgrad_prod_t<float>(p, YV, YY, XZ, u, v, w, out, p->fx32)
+            : xgrad_prod_t<double>(p, YV, YY, XZ, u, v, w, out, p->fx);
+}
+
+// ---- float64 boundary / projection / diagnostics kernels ----
+static int noslip_launch(spx_plan* p, void* vx, void* vy, const void* pr, const WallParams& wp,
+                         int ncomp, const char* name) {
+  const LineFFTHost& f = p->fz;
+  const long nlines = long(p->g.ny) * p->nkx;
+  dim3 grid(unsigned((nlines + f.nl - 1) / f.nl), unsigned(ncomp));
+  ProfScope ps_(p, name, 2.0 * ncomp * double(p->nc) * 16.0);
+  SPX_LAUNCH_RS(f, k_noslip, double, , grid, blk(f), f.lds_bytes(16), p->stream, C_<double>(vx),
+                C_<double>(vy), C_<double>(pr), f.dev<Cd>(), nlines, p->g.ny, p->np,
+                1.0 / double(p->g.nz), wp, p->d_kx, p->d_ky, p->d_dir, p->g.Cz, p->g.oz);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
 int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
                  const double vzsta[2], const double vzend[2]) {
-  const LineFFTHost& f = p->fz;
-  const long nlines = long(p->g.ny) * p->nkx;
   WallParams wp;
   wp.tau = tau;
   wp.vb[0][0] = vzsta ? vzsta[0] : 0.0;
@@ -1040,40 +1141,23 @@ int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
   wp.vb[1][1] = vzend ? vzend[1] : 0.0;
   wp.nxny = double(p->g.nx * p->g.ny);
   wp.scalar = 0;
-  dim3 grid(unsigned((nlines + f.nl - 1) / f.nl), 2);
-  ProfScope ps_(p, "noslip", 4.0 * double(p->nc) * 16.0);
-  SPX_LAUNCH_RS(f, k_noslip,, grid, blk(f), f.lds_bytes(p->csize), p->stream,
-                     static_cast<Cp*>(vx), static_cast<Cp*>(vy), static_cast<const Cp*>(pr),
-                     f.dev<Cp>(), nlines, p->g.ny, p->np, 1.0 / double(p->g.nz), wp, p->d_kx,
-                     p->d_ky, p->d_dir, p->g.Cz, p->g.oz);
-  SPX_CHECK_LAUNCH();
-  return SPX_OK;
+  return noslip_launch(p, vx, vy, pr, wp, 2, "noslip");
 }
 
 int sconst_fused(spx_plan* p, void* th) {
-  const LineFFTHost& f = p->fz;
-  const long nlines = long(p->g.ny) * p->nkx;
   WallParams wp;
   std::memset(&wp, 0, sizeof(wp));
   wp.scalar = 1;
-  dim3 grid(unsigned((nlines + f.nl - 1) / f.nl), 1);
-  ProfScope ps_(p, "sconst", 2.0 * double(p->nc) * 16.0);
-  SPX_LAUNCH_RS(f, k_noslip,, grid, blk(f), f.lds_bytes(p->csize), p->stream,
-                     static_cast<Cp*>(th), static_cast<Cp*>(th), static_cast<const Cp*>(nullptr),
-                     f.dev<Cp>(), nlines, p->g.ny, p->np, 1.0 / double(p->g.nz), wp, p->d_kx,
-                     p->d_ky, p->d_dir, p->g.Cz, p->g.oz);
-  SPX_CHECK_LAUNCH();
-  return SPX_OK;
+  return noslip_launch(p, th, th, nullptr, wp, 1, "sconst");
 }
 
 int proj_bc(spx_plan* p, const void* c) {
   const LineFFTHost& f = p->fz;
   const long nlines = long(p->g.ny) * p->nkx;
   ProfScope ps_(p, "proj_bc", double(p->nc) * 16.0);
-  SPX_LAUNCH_RS(f, k_proj_bc,, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
-                     f.lds_bytes(p->csize), p->stream, static_cast<const Cp*>(c),
-                     static_cast<Cp*>(p->bc), f.dev<Cp>(), nlines, p->np,
-                     1.0 / double(p->g.nz));
+  SPX_LAUNCH_RS(f, k_proj_bc, double, , dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
+                f.lds_bytes(16), p->stream, C_<double>(c), C_<double>(p->bc), f.dev<Cd>(), nlines,
+                p->np, 1.0 / double(p->g.nz));
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
@@ -1082,10 +1166,21 @@ int proj_pressure(spx_plan* p, void* d) {
   const LineFFTHost& f = p->fz;
   const long nlines = long(p->g.ny) * p->nkx;
   ProfScope ps_(p, "proj_pressure", 2.0 * double(p->nc) * 16.0);
-  SPX_LAUNCH_RS(f, k_proj_pressure,, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
-                     f.lds_bytes(p->csize), p->stream, static_cast<Cp*>(d),
-                     static_cast<const Cp*>(p->bc), f.dev<Cp>(), nlines, p->g.ny,
-                     1.0 / double(p->g.nz), p->g.Lz, p->d_khom, p->d_z);
+  const int rs = rset_of(f);
+  const dim3 grid(unsigned((nlines + f.nl - 1) / f.nl));
+  const size_t lds = f.lds_bytes(16);
+  if (rs == RS_POW2)
+    hipLaunchKernelGGL(k_proj_pressure<RS_POW2>, grid, blk(f), lds, p->stream, C_<double>(d),
+                       C_<double>(p->bc), f.dev<Cd>(), nlines, p->g.ny, 1.0 / double(p->g.nz),
+                       p->g.Lz, p->d_khom, p->d_z);
+  else if (rs == RS_SMOOTH)
+    hipLaunchKernelGGL(k_proj_pressure<RS_SMOOTH>, grid, blk(f), lds, p->stream, C_<double>(d),
+                       C_<double>(p->bc), f.dev<Cd>(), nlines, p->g.ny, 1.0 / double(p->g.nz),
+                       p->g.Lz, p->d_khom, p->d_z);
+  else
+    hipLaunchKernelGGL(k_proj_pressure<RS_ALL>, grid, blk(f), lds, p->stream, C_<double>(d),
+                       C_<double>(p->bc), f.dev<Cd>(), nlines, p->g.ny, 1.0 / double(p->g.nz),
+                       p->g.Lz, p->d_khom, p->d_z);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
@@ -1093,56 +1188,56 @@ int proj_pressure(spx_plan* p, void* d) {
 int proj_harmonic(spx_plan* p, void* a, void* b, void* c) {
   const LineFFTHost& f = p->fz2;
   const long nlines = long(p->g.ny) * p->nkx;
-  const size_t lds = f.lds_bytes(p->csize, 2);
+  const size_t lds = f.lds_bytes(16, 2);
   ProfScope ps_(p, "proj_harmonic", 6.0 * double(p->nc) * 16.0);
-  SPX_LAUNCH_RS(f, k_proj_harmonic,, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f), lds,
-                     p->stream, static_cast<Cp*>(a), static_cast<Cp*>(b), static_cast<Cp*>(c),
-                     static_cast<const Cp*>(p->bc), f.dev<Cp>(), nlines, p->g.ny, p->np, p->g.Lz,
-                     p->d_khom, p->d_z, p->d_kx, p->d_ky, p->d_dir, p->g.Cz, p->g.oz);
+  SPX_LAUNCH_RS(f, k_proj_harmonic, double, , dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
+                lds, p->stream, C_<double>(a), C_<double>(b), C_<double>(c), C_<double>(p->bc),
+                f.dev<Cd>(), nlines, p->g.ny, p->np, p->g.Lz, p->d_khom, p->d_z, p->d_kx,
+                p->d_ky, p->d_dir, p->g.Cz, p->g.oz);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
-static int zline_launch(spx_plan* p, const LinCombDev& lc, const void* b, int mode, double* out) {
+static int zline_launch(spx_plan* p, const LinComb& lc, const void* b, int mode, double* out) {
   const LineFFTHost& f = mode == 1 ? p->fz2 : p->fz;
   const long nlines = long(p->g.ny) * p->nkx;
-  const size_t lds = f.lds_bytes(p->csize, mode == 1 ? 2 : 1) + size_t(f.nl) * f.tpl * sizeof(double);
+  LinCombDev<double> d;
+  for (int t = 0; t < 3; ++t) {
+    d.in[t] = static_cast<const Cd*>(lc.in[t]);
+    d.dir[t] = lc.dir[t];
+    d.sgn[t] = lc.sgn[t];
+  }
+  const size_t lds = f.lds_bytes(16, mode == 1 ? 2 : 1) + size_t(f.nl) * f.tpl * sizeof(double);
   ProfScope ps_(p, "zline_reduce", double(p->nc) * 16.0 * (mode == 1 ? 2 : 1));
-  SPX_LAUNCH_RS(f, k_zline_reduce,, dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f), lds,
-                     p->stream, lc, static_cast<const Cp*>(b), mode, out, f.dev<Cp>(), nlines,
-                     p->g.ny, p->np, p->d_kx, p->d_ky, p->d_kz);
+  SPX_LAUNCH_RS(f, k_zline_reduce, double, , dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
+                lds, p->stream, d, C_<double>(b), mode, out, f.dev<Cd>(), nlines, p->g.ny, p->np,
+                p->d_kx, p->d_ky, p->d_kz);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
 int zline_sumsq(spx_plan* p, const LinComb& lc, double* line_out) {
-  LinCombDev d;
-  for (int t = 0; t < 3; ++t) {
-    d.in[t] = static_cast<const Cp*>(lc.in[t]);
-    d.dir[t] = lc.dir[t];
-    d.sgn[t] = lc.sgn[t];
-  }
-  return zline_launch(p, d, nullptr, 0, line_out);
+  return zline_launch(p, lc, nullptr, 0, line_out);
 }
 
 int zline_dot(spx_plan* p, const void* a, const void* b, double* line_out) {
-  LinCombDev d;
-  d.in[0] = static_cast<const Cp*>(a);
-  d.in[1] = d.in[2] = nullptr;
-  d.dir[0] = d.dir[1] = d.dir[2] = 0;
-  d.sgn[0] = 1;
-  d.sgn[1] = d.sgn[2] = 0;
-  return zline_launch(p, d, b, 1, line_out);
+  LinComb lc;
+  lc.in[0] = a;
+  lc.in[1] = lc.in[2] = nullptr;
+  lc.dir[0] = lc.dir[1] = lc.dir[2] = 0;
+  lc.sgn[0] = 1;
+  lc.sgn[1] = lc.sgn[2] = 0;
+  return zline_launch(p, lc, b, 1, line_out);
 }
 
 int zline_walls(spx_plan* p, const void* a, double* line_out) {
-  LinCombDev d;
-  d.in[0] = static_cast<const Cp*>(a);
-  d.in[1] = d.in[2] = nullptr;
-  d.dir[0] = d.dir[1] = d.dir[2] = 0;
-  d.sgn[0] = 1;
-  d.sgn[1] = d.sgn[2] = 0;
-  return zline_launch(p, d, nullptr, 2, line_out);
+  LinComb lc;
+  lc.in[0] = a;
+  lc.in[1] = lc.in[2] = nullptr;
+  lc.dir[0] = lc.dir[1] = lc.dir[2] = 0;
+  lc.sgn[0] = 1;
+  lc.sgn[1] = lc.sgn[2] = 0;
+  return zline_launch(p, lc, nullptr, 2, line_out);
 }
 
 }  // namespace spx

@@ -1,14 +1,28 @@
 // Pointwise spectral / real-space kernels of libspx.  Floating-point
 // contraction is disabled so each kernel performs exactly the reference's
 // operation sequence (pseudospec_hd.f90, boundary_mod.fpp, include/*/).
+//
+// The kernels compute in float64 and are templated on the storage type of
+// their fields; the state is float64 in every plan, and only the nonlinear
+// terms read by the RK update are float32 in mixed-precision plans.  The
+// kernels are bound by HBM traffic, so float64 arithmetic costs nothing.
 #include "spx_engine.h"
 
 namespace spx {
 
-using Tp = double;
-using Cp = cpx<Tp>;
+using Cp = cpx<double>;
 
-__device__ __forceinline__ Cp ik(double k, Cp v) { return mkc<Tp>(-(k * v.y), k * v.x); }
+__device__ __forceinline__ Cp ik(double k, Cp v) { return mkc<double>(-(k * v.y), k * v.x); }
+
+template <typename TS>
+__device__ __forceinline__ Cp ld(const cpx<TS>* a, long e) {
+  const cpx<TS> v = a[e];
+  return mkc<double>(double(v.x), double(v.y));
+}
+template <typename TS>
+__device__ __forceinline__ void st(cpx<TS>* a, long e, Cp v) {
+  a[e] = mkc<TS>(TS(v.x), TS(v.y));
+}
 
 // index helpers for S (nz, ny, nkx) with z fastest
 struct SIdx {
@@ -22,7 +36,9 @@ struct SIdx {
 };
 
 // derivk (pseudospec_hd.f90:28-94)
-__global__ void k_derivk(const Cp* __restrict__ a, Cp* __restrict__ b, long nc, SIdx ix, int dirn,
+template <typename TS>
+__global__ void k_derivk(const cpx<TS>* __restrict__ a, cpx<TS>* __restrict__ b, long nc, SIdx ix,
+                         int dirn,
                          const double* __restrict__ kx, const double* __restrict__ ky,
                          const double* __restrict__ kz) {
   const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -31,11 +47,12 @@ __global__ void k_derivk(const Cp* __restrict__ a, Cp* __restrict__ b, long nc, 
   long i;
   ix.split(e, k, j, i);
   const double kk = dirn == 1 ? kx[i] : (dirn == 2 ? ky[j] : kz[k]);
-  b[e] = ik(kk, a[e]);
+  st(b, e, ik(kk, ld(a, e)));
 }
 
 // laplak (pseudospec_hd.f90:97-127): b = -kk2*a, kk2 = (kx^2+ky^2)+kz^2
-__global__ void k_laplak(const Cp* __restrict__ a, Cp* __restrict__ b, long nc, SIdx ix,
+template <typename TS>
+__global__ void k_laplak(const cpx<TS>* __restrict__ a, cpx<TS>* __restrict__ b, long nc, SIdx ix,
                          const double* __restrict__ kx, const double* __restrict__ ky,
                          const double* __restrict__ kz) {
 #pragma clang fp contract(off)
@@ -45,11 +62,12 @@ __global__ void k_laplak(const Cp* __restrict__ a, Cp* __restrict__ b, long nc, 
   long i;
   ix.split(e, k, j, i);
   const double kk2 = (kx[i] * kx[i] + ky[j] * ky[j]) + kz[k] * kz[k];
-  b[e] = cscale(a[e], -kk2);
+  st(b, e, cscale(ld(a, e), -kk2));
 }
 
 // fc_filter (pseudospec_hd.f90:1082-1115): ((a*fx)*fy)*fz
-__global__ void k_filter(Cp* __restrict__ a, long nc, SIdx ix, const double* __restrict__ fx,
+template <typename TS>
+__global__ void k_filter(cpx<TS>* __restrict__ a, long nc, SIdx ix, const double* __restrict__ fx,
                          const double* __restrict__ fy, const double* __restrict__ fz) {
 #pragma clang fp contract(off)
   const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -57,39 +75,44 @@ __global__ void k_filter(Cp* __restrict__ a, long nc, SIdx ix, const double* __r
   int k, j;
   long i;
   ix.split(e, k, j, i);
-  a[e] = cscale(cscale(cscale(a[e], fx[i]), fy[j]), fz[k]);
+  st(a, e, cscale(cscale(cscale(ld(a, e), fx[i]), fy[j]), fz[k]));
 }
 
-__global__ void k_scale(double* __restrict__ a, long n, double s) {
+template <typename TR>
+__global__ void k_scale(TR* __restrict__ a, long n, double s) {
   const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e < n) a[e] *= s;
+  if (e < n) a[e] = TR(double(a[e]) * s);
 }
 
 // real r <- ((r/nx)/ny)/nz  (include/bouss/bouss_rkstep2.f90:57)
-__global__ void k_div3(double* __restrict__ r, long n, double a, double b, double c) {
+template <typename TR>
+__global__ void k_div3(TR* __restrict__ r, long n, double a, double b, double c) {
 #pragma clang fp contract(off)
   const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e < n) r[e] = ((r[e] / a) / b) / c;
+  if (e < n) r[e] = TR(((double(r[e]) / a) / b) / c);
 }
 
 // Fused fc_filter(N) + laplak(v) + RK stage update (include/hd/hd_rkstep2.f90:11-32):
 //   v = C0 + dt*(nu*(-kk2 v) - ((N*fx)*fy)*fz + F)*rmp
 // with the Boussinesq source N <- N - coef*src applied first
 // (include/bouss/bouss_rkstep2.f90:19-27).
+template <typename TN>
 struct RkComp {
   Cp* v;
   const Cp* c0;
-  const Cp* nl;
+  const cpx<TN>* nl;    // nonlinear term, float64 or float32
   const Cp* f;
-  const Cp* src;   // may be null
+  const Cp* src;        // may be null
   double src_coef;
   double nu;
 };
+template <typename TN>
 struct RkArgs {
-  RkComp c[4];
+  RkComp<TN> c[4];
 };
 
-__global__ void k_rk_update(RkArgs args, int ncomp, long nc, SIdx ix,
+template <typename TN>
+__global__ void k_rk_update(RkArgs<TN> args, int ncomp, long nc, SIdx ix,
                             const double* __restrict__ kx, const double* __restrict__ ky,
                             const double* __restrict__ kz, const double* __restrict__ fx,
                             const double* __restrict__ fy, const double* __restrict__ fz,
@@ -107,25 +130,27 @@ __global__ void k_rk_update(RkArgs args, int ncomp, long nc, SIdx ix,
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     if (q >= ncomp) break;
-    const RkComp& c = args.c[q];
-    Cp nl = c.nl[e];
-    if (c.src) nl = csub(nl, cscale(c.src[e], c.src_coef));
+    const RkComp<TN>& c = args.c[q];
+    Cp nl = ld(c.nl, e);
+    if (c.src) nl = csub(nl, cscale(ld(c.src, e), c.src_coef));
     nl = cscale(cscale(cscale(nl, fx[i]), fy[j]), fz[k]);
-    const Cp lap = cscale(c.v[e], -kk2);
-    const Cp t = cadd(csub(cscale(lap, c.nu), nl), c.f[e]);
-    out[q] = cadd(c.c0[e], cscale(cscale(t, dt), rmp));
+    const Cp lap = cscale(ld(c.v, e), -kk2);
+    const Cp t = cadd(csub(cscale(lap, c.nu), nl), ld(c.f, e));
+    out[q] = cadd(ld(c.c0, e), cscale(cscale(t, dt), rmp));
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     if (q >= ncomp) break;
-    args.c[q].v[e] = out[q];
+    st(args.c[q].v, e, out[q]);
   }
 }
 
 // poisson_inhomogeneous + first projection (boundary_mod.fpp:227-240, 405-448):
 //   d = -i(kx a + ky b + kz c)/kk2, d(0,0,0) = 0 ; a -= i kx d ; b -= i ky d ; c -= i kz d
-__global__ void k_poisson_project(Cp* __restrict__ a, Cp* __restrict__ b, Cp* __restrict__ c,
-                                  Cp* __restrict__ d, long nc, SIdx ix,
+// d is float64 for every plan (the pressure potential is kept in float64).
+template <typename TS>
+__global__ void k_poisson_project(cpx<TS>* __restrict__ a, cpx<TS>* __restrict__ b,
+                                  cpx<TS>* __restrict__ c, Cp* __restrict__ d, long nc, SIdx ix,
                                   const double* __restrict__ kx, const double* __restrict__ ky,
                                   const double* __restrict__ kz) {
 #pragma clang fp contract(off)
@@ -136,52 +161,57 @@ __global__ void k_poisson_project(Cp* __restrict__ a, Cp* __restrict__ b, Cp* __
   ix.split(e, k, j, i);
   const double kxi = kx[i], kyj = ky[j], kzk = kz[k];
   const double kk2 = (kxi * kxi + kyj * kyj) + kzk * kzk;
-  Cp A = a[e], B = b[e], Cc = c[e];
+  const Cp A = ld(a, e), B = ld(b, e), Cc = ld(c, e);
   Cp s = cadd(cscale(A, kxi), cscale(B, kyj));
   s = cadd(s, cscale(Cc, kzk));
   Cp dd;
   if (e == 0) {
-    dd = mkc<Tp>(0, 0);
+    dd = mkc<double>(0, 0);
   } else {
-    dd = mkc<Tp>(s.y / kk2, -s.x / kk2);
+    dd = mkc<double>(s.y / kk2, -s.x / kk2);
   }
   d[e] = dd;
-  a[e] = csub(A, ik(kxi, dd));
-  b[e] = csub(B, ik(kyj, dd));
-  c[e] = csub(Cc, ik(kzk, dd));
+  st(a, e, csub(A, ik(kxi, dd)));
+  st(b, e, csub(B, ik(kyj, dd)));
+  st(c, e, csub(Cc, ik(kzk, dd)));
 }
 
-__global__ void k_axpy(Cp* __restrict__ y, const Cp* __restrict__ x, long nc, double alpha) {
+template <typename TS>
+__global__ void k_axpy(cpx<TS>* __restrict__ y, const cpx<TS>* __restrict__ x, long nc,
+                       double alpha) {
 #pragma clang fp contract(off)
   const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e < nc) y[e] = csub(y[e], cscale(x[e], alpha));
+  if (e < nc) st(y, e, csub(ld(y, e), cscale(ld(x, e), alpha)));
 }
 
 // ---------------------------------------------------------------------------
 static inline unsigned nblk(long n) { return unsigned((n + 255) / 256); }
 static inline SIdx sidx(spx_plan* p) { return SIdx{p->g.nz, p->g.ny}; }
 
+template <typename TS>
+static inline cpx<TS>* CS(void* a) { return static_cast<cpx<TS>*>(a); }
+template <typename TS>
+static inline const cpx<TS>* CS(const void* a) { return static_cast<const cpx<TS>*>(a); }
+
 int op_derivk(spx_plan* p, const void* a, void* b, int dirn) {
   ProfScope ps_(p, "derivk", 2.0 * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_derivk, dim3(nblk(p->nc)), dim3(256), 0, p->stream,
-                     static_cast<const Cp*>(a), static_cast<Cp*>(b), long(p->nc), sidx(p), dirn,
-                     p->d_kx, p->d_ky, p->d_kz);
+  hipLaunchKernelGGL(k_derivk<double>, dim3(nblk(p->nc)), dim3(256), 0, p->stream, CS<double>(a),
+                     CS<double>(b), long(p->nc), sidx(p), dirn, p->d_kx, p->d_ky, p->d_kz);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
 int op_laplak(spx_plan* p, const void* a, void* b) {
   ProfScope ps_(p, "laplak", 2.0 * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_laplak, dim3(nblk(p->nc)), dim3(256), 0, p->stream,
-                     static_cast<const Cp*>(a), static_cast<Cp*>(b), long(p->nc), sidx(p),
-                     p->d_kx, p->d_ky, p->d_kz);
+  hipLaunchKernelGGL(k_laplak<double>, dim3(nblk(p->nc)), dim3(256), 0, p->stream, CS<double>(a),
+                     CS<double>(b), long(p->nc), sidx(p), p->d_kx, p->d_ky, p->d_kz);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
 int op_filter(spx_plan* p, void* a) {
   ProfScope ps_(p, "filter", 2.0 * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_filter, dim3(nblk(p->nc)), dim3(256), 0, p->stream, static_cast<Cp*>(a),
+  hipLaunchKernelGGL(k_filter<double>, dim3(nblk(p->nc)), dim3(256), 0, p->stream, CS<double>(a),
                      long(p->nc), sidx(p), p->d_fkx, p->d_fky, p->d_fkz);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
@@ -189,7 +219,7 @@ int op_filter(spx_plan* p, void* a) {
 
 int op_scale(spx_plan* p, void* a, size_t n, double s) {
   ProfScope ps_(p, "scale", 16.0 * double(n));
-  hipLaunchKernelGGL(k_scale, dim3(nblk(long(n))), dim3(256), 0, p->stream,
+  hipLaunchKernelGGL(k_scale<double>, dim3(nblk(long(n))), dim3(256), 0, p->stream,
                      static_cast<double*>(a), long(n), s);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
@@ -198,51 +228,60 @@ int op_scale(spx_plan* p, void* a, size_t n, double s) {
 int op_real_div3(spx_plan* p, void* r, double a, double b, double c) {
   const long n = long(p->g.nx) * p->g.ny * p->np;
   ProfScope ps_(p, "real_div3", 16.0 * double(n));
-  hipLaunchKernelGGL(k_div3, dim3(nblk(n)), dim3(256), 0, p->stream, static_cast<double*>(r), n,
-                     a, b, c);
+  hipLaunchKernelGGL(k_div3<double>, dim3(nblk(n)), dim3(256), 0, p->stream,
+                     static_cast<double*>(r), n, a, b, c);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
-int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, void* const* nl,
-                 const void* const* f, const void* const* src, const double* src_coef,
-                 const double* nu, double dt, double rmp) {
-  RkArgs args;
+template <typename TN>
+static int rk_launch(spx_plan* p, int ncomp, void* const* v, const void* const* c0,
+                     void* const* nl, const void* const* f, const void* const* src,
+                     const double* src_coef, const double* nu, double dt, double rmp) {
+  RkArgs<TN> args;
   for (int c = 0; c < 4; ++c) {
-    RkComp& r = args.c[c];
+    RkComp<TN>& r = args.c[c];
     if (c < ncomp) {
-      r.v = static_cast<Cp*>(v[c]);
-      r.c0 = static_cast<const Cp*>(c0[c]);
-      r.nl = static_cast<const Cp*>(nl[c]);
-      r.f = static_cast<const Cp*>(f[c]);
-      r.src = src ? static_cast<const Cp*>(src[c]) : nullptr;
+      r.v = CS<double>(v[c]);
+      r.c0 = CS<double>(c0[c]);
+      r.nl = CS<TN>(static_cast<const void*>(nl[c]));
+      r.f = CS<double>(f[c]);
+      r.src = src ? CS<double>(src[c]) : nullptr;
       r.src_coef = src_coef ? src_coef[c] : 0.0;
       r.nu = nu[c];
     } else {
-      r = RkComp{nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, 0.0};
+      r = RkComp<TN>{nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, 0.0};
     }
   }
-  ProfScope ps_(p, "rk_update", double(p->nc) * 16.0 * 5 * ncomp + (src ? double(p->nc) * 16.0 * ncomp : 0.0));
-  hipLaunchKernelGGL(k_rk_update, dim3(nblk(p->nc)), dim3(256), 0, p->stream, args, ncomp,
+  ProfScope ps_(p, "rk_update",
+                double(p->nc) * ncomp * (4.0 * 16.0 + sizeof(cpx<TN>) + (src ? 16.0 : 0.0)));
+  hipLaunchKernelGGL(k_rk_update<TN>, dim3(nblk(p->nc)), dim3(256), 0, p->stream, args, ncomp,
                      long(p->nc), sidx(p), p->d_kx, p->d_ky, p->d_kz, p->d_fkx, p->d_fky,
                      p->d_fkz, dt, rmp);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
+int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, void* const* nl,
+                 const void* const* f, const void* const* src, const double* src_coef,
+                 const double* nu, double dt, double rmp, bool nl_lo) {
+  return nl_lo ? rk_launch<float>(p, ncomp, v, c0, nl, f, src, src_coef, nu, dt, rmp)
+               : rk_launch<double>(p, ncomp, v, c0, nl, f, src, src_coef, nu, dt, rmp);
+}
+
 int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d) {
   ProfScope ps_(p, "poisson_project", 7.0 * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_poisson_project, dim3(nblk(p->nc)), dim3(256), 0, p->stream,
-                     static_cast<Cp*>(a), static_cast<Cp*>(b), static_cast<Cp*>(c),
-                     static_cast<Cp*>(d), long(p->nc), sidx(p), p->d_kx, p->d_ky, p->d_kz);
+  hipLaunchKernelGGL(k_poisson_project<double>, dim3(nblk(p->nc)), dim3(256), 0, p->stream,
+                     CS<double>(a), CS<double>(b), CS<double>(c), CS<double>(d), long(p->nc),
+                     sidx(p), p->d_kx, p->d_ky, p->d_kz);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
 int op_axpy_field(spx_plan* p, void* y, const void* x, double alpha) {
   ProfScope ps_(p, "axpy", 3.0 * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_axpy, dim3(nblk(p->nc)), dim3(256), 0, p->stream, static_cast<Cp*>(y),
-                     static_cast<const Cp*>(x), long(p->nc), alpha);
+  hipLaunchKernelGGL(k_axpy<double>, dim3(nblk(p->nc)), dim3(256), 0, p->stream, CS<double>(y),
+                     CS<double>(x), long(p->nc), alpha);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }

@@ -10,6 +10,9 @@ memory layout:
 * spectral  Fortran (nz, ny, nkx)  ->  torch (nkx, ny, nz) complex128
 * real      Fortran (nx, ny, nz)   ->  torch (nz, ny, nx)  float64
 
+In mixed-precision plans (``precision=SPX_FP32_MIXED``) the outputs of
+``gradre`` and ``advect`` are complex64; every other array is float64.
+
 Every call goes through libspx; there is no host fallback.
 """
 from __future__ import annotations
@@ -59,6 +62,7 @@ class FCPlan:
         self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
         self.nkx, self.np = info.nkx, info.np
         self.z_fft_len, self.z_algorithm = info.z_fft_len, info.z_algorithm
+        self.precision = precision
 
     # -- lifetime ---------------------------------------------------------------
     def close(self):
@@ -101,11 +105,26 @@ class FCPlan:
         torch = _torch()
         return torch.empty(self.real_shape, dtype=torch.float64, device="cuda")
 
-    def _spec(self, t, name):
+    @property
+    def mixed(self) -> bool:
+        return self.precision == _l.SPX_FP32_MIXED
+
+    @property
+    def nonlinear_dtype(self):
+        """dtype of the gradre / advect results."""
         torch = _torch()
-        if not (t.is_cuda and t.dtype == torch.complex128 and t.is_contiguous()
+        return torch.complex64 if self.mixed else torch.complex128
+
+    def empty_nonlinear(self):
+        torch = _torch()
+        return torch.empty(self.spec_shape, dtype=self.nonlinear_dtype, device="cuda")
+
+    def _spec(self, t, name, dtype=None):
+        torch = _torch()
+        dtype = dtype or torch.complex128
+        if not (t.is_cuda and t.dtype == dtype and t.is_contiguous()
                 and tuple(t.shape) == self.spec_shape):
-            raise ValueError(f"{name}: expected contiguous cuda complex128 {self.spec_shape}")
+            raise ValueError(f"{name}: expected contiguous cuda {dtype} {self.spec_shape}")
         return c_void_p(t.data_ptr())
 
     def _real(self, t, name):
@@ -169,13 +188,18 @@ class FCPlan:
         return a
 
     def gradre(self, a, b, c, d, e, f):
-        _l.check(self._lib.spx_gradre(self._p, *(self._spec(t, n) for t, n in
-                                                 zip((a, b, c, d, e, f), "abcdef"))))
+        """d, e, f: ``nonlinear_dtype`` (complex64 in mixed plans)."""
+        nd = self.nonlinear_dtype
+        _l.check(self._lib.spx_gradre(self._p, self._spec(a, "a"), self._spec(b, "b"),
+                                      self._spec(c, "c"), self._spec(d, "d", nd),
+                                      self._spec(e, "e", nd), self._spec(f, "f", nd)))
         return d, e, f
 
     def advect(self, a, b, c, d, e):
+        """e: ``nonlinear_dtype`` (complex64 in mixed plans)."""
         _l.check(self._lib.spx_advect(self._p, *(self._spec(t, n) for t, n in
-                                                 zip((a, b, c, d, e), "abcde"))))
+                                                 zip((a, b, c, d), "abcd")),
+                                      self._spec(e, "e", self.nonlinear_dtype)))
         return e
 
     # -- boundary conditions (boundary/) ----------------------------------------------

@@ -36,3 +36,50 @@ def rel_l2(a, b):
     b = np.asarray(b)
     den = np.linalg.norm(b.ravel())
     return float(np.linalg.norm((a - b).ravel()) / (den if den else 1.0))
+
+
+# HD 64x64x(64+25), d=5, C=25, RK2, dt=1e-3, nu=2e-3, unforced, 100 steps
+CFG1 = dict(shape=(64, 64, 89, 25, 5), prm=dict(solver="HD", ord=2, dt=1e-3, nu=2e-3),
+            nsteps=100, diag_every=10, seed=1000)
+
+
+def oracle_trajectory(shape, prm, nsteps, seed=1000, diag_every=0):
+    """Float64 oracle run from the seeded spectrum-peaked field: final state
+    and the (energy, enstrophy, injection, <div^2>, wall values) series."""
+    from oracle import specter_ref as O
+    from specter_amd import ic
+    nx, ny, nz, C, d = shape
+    g = O.make_grid(nx, ny, nz, C, d, TABLES)
+    u = ic.spectrum_peaked_velocity(nx, ny, nz, C, seed)
+    zero = np.zeros(g.spec_shape, complex)
+    st = O.HDState(*(O.fftp3d_real_to_complex(g, a.copy()) for a in u), zero.copy(),
+                   zero.copy(), zero.copy(), zero.copy())
+    series = []
+    for t in range(nsteps):
+        if diag_every and t % diag_every == 0:
+            series.append(O.hdcheck(g, st) + O.vdiagnostic(g, st))
+        O.hd_step(g, st, prm["ord"], prm["dt"], prm["nu"])
+    return g, st, np.array(series)
+
+
+def gpu_trajectory(shape, prm, nsteps, seed=1000, diag_every=0, precision=0):
+    """The same run on the GPU through libspx (precision: SPX_FP64 = 0,
+    SPX_FP32_MIXED = 1)."""
+    from specter_amd import FCPlan, Solver, SolverParams, ic
+    nx, ny, nz, C, d = shape
+    plan = FCPlan(nx, ny, nz, C, d, precision=precision)
+    sol = Solver(plan, SolverParams(**prm))
+    for name, arr in zip(("vx", "vy", "vz"), ic.spectrum_peaked_velocity(nx, ny, nz, C, seed)):
+        sol.set_real_field(name, arr)
+    series = []
+    for t in range(nsteps):
+        if diag_every and t % diag_every == 0:
+            series.append(sol.hdcheck() + sol.vdiagnostic())
+        sol.step(1)
+    return sol, np.array(series)
+
+
+@pytest.fixture(scope="session")
+def cfg1_oracle():
+    return oracle_trajectory(CFG1["shape"], CFG1["prm"], CFG1["nsteps"], CFG1["seed"],
+                             CFG1["diag_every"])

@@ -45,7 +45,7 @@ def _err(lib):
     ((16, 16, 41, 25, 5, 1.0, -1.0, 1.0, 0), 1, "box lengths"),
     ((16, 16, 41, 25, 0, 1.0, 1.0, 1.0, 0), 1, "matching order"),
     ((16, 16, 30, 25, 5, 1.0, 1.0, 1.0, 0), 1, "too few physical"),
-    ((16, 16, 41, 25, 5, 1.0, 1.0, 1.0, 1), 2, "only SPX_FP64"),
+    ((16, 16, 41, 25, 5, 1.0, 1.0, 1.0, 7), 1, "precision must be"),
 ])
 def test_plan_create_validates_before_device(lib, grid, code, msg):
     from specter_amd.lib import SpxGrid

@@ -1,0 +1,107 @@
+"""Mixed-precision plans (SPX_FP32_MIXED) against the float64 oracle.
+
+The nonlinear term (gradre / advect: every z, y, x transform and product) runs
+in float32; the state, the FC continuation sums, the pressure solve, the wall
+conditions and the RK update stay float64.  Tolerances (relative L2 against
+the float64 oracle):
+
+* one gradre / advect evaluation: 1e-5 (float32 transforms of length <= 89
+  give a few 1e-7; the continuation of float32 data adds its amplification)
+* fields after 100 RK2 steps of the 64^3 workload: 1e-5, with the energy and
+  enstrophy series to 1e-5 and the wall values to 1e-7 of the energy scale.
+"""
+import numpy as np
+import pytest
+
+from conftest import CFG1, TABLES, gpu_trajectory, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SPX_FP32_MIXED = 1
+
+
+def _random_fields(g, rng, n):
+    from oracle import specter_ref as O
+    out = []
+    for _ in range(n):
+        r = rng.standard_normal(g.real_shape)
+        r[g.np_:] = 0.0
+        out.append(O.fftp3d_real_to_complex(g, r))
+    return out
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 41, 25, 5), (32, 32, 57, 25, 5),
+                                   (24, 16, 89, 25, 5), (16, 32, 43, 27, 8)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_mixed_gradre_advect(shape):
+    import torch
+    from oracle import specter_ref as O
+    from specter_amd import FCPlan
+    nx, ny, nz, C, d = shape
+    g = O.make_grid(nx, ny, nz, C, d, TABLES)
+    rng = np.random.default_rng(7)
+    a, b, c, th = _random_fields(g, rng, 4)
+    plan = FCPlan(nx, ny, nz, C, d, precision=SPX_FP32_MIXED)
+    assert plan.nonlinear_dtype == torch.complex64
+    dev = [torch.as_tensor(x).cuda() for x in (a, b, c, th)]
+    outs = [plan.empty_nonlinear() for _ in range(4)]
+    plan.gradre(dev[0], dev[1], dev[2], *outs[:3])
+    plan.advect(dev[0], dev[1], dev[2], dev[3], outs[3])
+    plan.sync()
+    ref = list(O.gradre(g, a, b, c)) + [O.advect(g, a, b, c, th)]
+    for name, o, r in zip(("gx", "gy", "gz", "adv"), outs, ref):
+        err = rel_l2(o.cpu().numpy().astype(np.complex128), r)
+        print(f"{shape} mixed {name}: rel L2 {err:.2e}")
+        assert err < 1e-5, name
+    # the float64 inputs are left untouched
+    for x, t in zip((a, b, c, th), dev):
+        assert np.array_equal(t.cpu().numpy(), x)
+
+
+def test_mixed_cfg1_100_steps(cfg1_oracle):
+    """64x64x(64+25), d=5, RK2, dt=1e-3, nu=2e-3, 100 steps in a mixed plan."""
+    _, st, sr = cfg1_oracle
+    sol, sg = gpu_trajectory(CFG1["shape"], CFG1["prm"], CFG1["nsteps"], CFG1["seed"],
+                             CFG1["diag_every"], precision=SPX_FP32_MIXED)
+    for name in ("vx", "vy", "vz", "pr"):
+        err = rel_l2(sol.spectral(name), getattr(st, name))
+        print(f"mixed cfg1 {name} rel L2 after 100 steps: {err:.3e}")
+        assert err < 1e-5, name
+    for col, label in ((0, "energy"), (1, "enstrophy"), (2, "injection")):
+        if not np.any(sr[:, col]):
+            continue
+        e = np.max(np.abs(sg[:, col] - sr[:, col]) / np.abs(sr[:, col]))
+        print(f"mixed cfg1 {label} series max rel err: {e:.3e}")
+        assert e < 1e-5, label
+    scale = np.max(np.abs(sr[:, 0]))
+    e = np.max(np.abs(sg[:, 3:] - sr[:, 3:])) / scale
+    print(f"mixed cfg1 div/wall series max err / energy: {e:.3e}")
+    assert e < 1e-7
+
+
+def test_mixed_bouss_steps():
+    """Boussinesq RK4 on a d=8 grid: mixed plan vs the float64 oracle."""
+    from oracle import specter_ref as O
+    from specter_amd import FCPlan, Solver, SolverParams, ic
+    nx, ny, nz, C, d = 16, 16, 43, 27, 8
+    prm = dict(solver="BOUSS", ord=4, dt=1e-3, nu=2e-3, kappa=2e-3, xmom=1.0, xtemp=1.0)
+    g = O.make_grid(nx, ny, nz, C, d, TABLES)
+    plan = FCPlan(nx, ny, nz, C, d, precision=SPX_FP32_MIXED)
+    sp = SolverParams(**prm)
+    sol = Solver(plan, sp)
+    u = ic.spectrum_peaked_velocity(nx, ny, nz, C, 1000)
+    th = ic.thermal_perturbation(nx, ny, nz, C, 1001)
+    for name, arr in zip(("vx", "vy", "vz", "th"), (*u, th)):
+        sol.set_real_field(name, arr)
+    zero = np.zeros(g.spec_shape, complex)
+    st = O.HDState(*(O.fftp3d_real_to_complex(g, a.copy()) for a in u), zero.copy(),
+                   zero.copy(), zero.copy(), zero.copy())
+    st.th = O.fftp3d_real_to_complex(g, th.copy())
+    st.fs = zero.copy()
+    for _ in range(5):
+        sol.step(1)
+        O.bouss_step(g, st, sp.ord, sp.dt, sp.nu, sp.kappa, sp.xmom, sp.xtemp)
+    for name in ("vx", "vy", "vz", "th"):
+        err = rel_l2(sol.spectral(name), getattr(st, name))
+        print(f"mixed bouss {name} rel L2 after 5 steps: {err:.3e}")
+        assert err < 1e-5, name

@@ -11,7 +11,7 @@ within 10x that spread (or 1e-12, whichever is larger)."""
 import numpy as np
 import pytest
 
-from conftest import TABLES, rel_l2
+from conftest import CFG1, TABLES, gpu_trajectory, rel_l2
 
 pytestmark = pytest.mark.gpu
 ULP = 2.0 ** -53
@@ -99,13 +99,13 @@ def test_bouss_steps():
     _check_floor(sol, st, st2, ("vx", "vy", "vz", "th"))
 
 
-def test_cfg1_100_steps_parity():
+def test_cfg1_100_steps_parity(cfg1_oracle):
     """64x64x(64+25), d=5, RK2, dt=1e-3, nu=2e-3, 100 steps.  Measured:
     GPU-vs-oracle field rel L2 4.1e-11, oracle spread under per-transform
     roundoff 5-7e-12; the bound below keeps a 2.5x margin."""
-    sol, st, sg, sr = _run_pair((64, 64, 89, 25, 5),
-                                dict(solver="HD", ord=2, dt=1e-3, nu=2e-3), 100,
-                                diag_every=10)
+    _, st, sr = cfg1_oracle
+    sol, sg = gpu_trajectory(CFG1["shape"], CFG1["prm"], CFG1["nsteps"], CFG1["seed"],
+                             CFG1["diag_every"])
     for name in ("vx", "vy", "vz"):
         err = rel_l2(sol.spectral(name), getattr(st, name))
         print(f"cfg1 {name} rel L2 after 100 steps: {err:.3e}")
