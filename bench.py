@@ -64,7 +64,11 @@ def _solver(plan, w, u, f):
 
 
 def _run_substeps(sol, ordr, nsub, start=0):
-    """nsub RK stages cycling o = ord..1, with the rkstep1 copy before o = ord."""
+    """nsub RK stages cycling o = ord..1, with the rkstep1 copy before o = ord.
+    Whole steps go through spx_step (one captured HIP graph per step)."""
+    if start % ordr == 0 and nsub % ordr == 0:
+        sol.step(nsub // ordr)
+        return start + nsub
     for s in range(start, start + nsub):
         o = ordr - (s % ordr)
         if o == ordr:

@@ -161,6 +161,9 @@ int spx_rkstep1(spx_plan* p, spx_state* s);
 int spx_substep(spx_plan* p, spx_state* s, int o);
 /* nsteps full steps: rkstep1 + o = ord..1 (specter.fpp:1142-1161). */
 int spx_step(spx_plan* p, spx_state* s, int nsteps);
+/* Whether spx_step replays one captured HIP graph per RK step (default 1)
+ * instead of launching every kernel; no counterpart in the reference. */
+int spx_state_set_graph(spx_state* s, int on);
 
 /* ---- per-kernel timing (no counterpart in the reference; the reference's
  * gtimer accumulators ffttime/tratime/conttime, fftp_mod.fpp:32-37, play this

@@ -138,6 +138,11 @@ struct spx_state {
   void *C1 = nullptr, *C2 = nullptr, *C3 = nullptr, *C7 = nullptr;
   void* N[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<void*> owned;
+  // one full RK step (rkstep1 + ord stages) captured as a HIP graph
+  bool use_graph = true;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
 };
 
 #define SPX_REQ(cond, msg) \
@@ -611,9 +616,18 @@ int spx_state_create(spx_plan* p, const spx_params* prm, spx_state** out) {
   return SPX_OK;
 }
 
+static void drop_graph(spx_state* s) {
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->gexec = nullptr;
+  s->graph = nullptr;
+}
+
 int spx_state_destroy(spx_state* s) {
   if (!s) return SPX_OK;
   if (s->p) (void)hipStreamSynchronize(s->p->stream);
+  drop_graph(s);
+  if (s->cap) (void)hipStreamDestroy(s->cap);
   for (void* q : s->owned) (void)hipFree(q);
   delete s;
   return SPX_OK;
@@ -637,12 +651,9 @@ int spx_state_field(spx_state* s, const char* name, void** ptr) {
 
 int spx_rkstep1(spx_plan* p, spx_state* s) {
   SPX_REQ(p && s && s->p == p, "spx_rkstep1: bad plan/state");
-  const size_t b = p->nc * 16;
-  SPX_HIP(hipMemcpyAsync(s->C1, s->vx, b, hipMemcpyDeviceToDevice, p->stream));
-  SPX_HIP(hipMemcpyAsync(s->C2, s->vy, b, hipMemcpyDeviceToDevice, p->stream));
-  SPX_HIP(hipMemcpyAsync(s->C3, s->vz, b, hipMemcpyDeviceToDevice, p->stream));
-  if (s->th) SPX_HIP(hipMemcpyAsync(s->C7, s->th, b, hipMemcpyDeviceToDevice, p->stream));
-  return SPX_OK;
+  const void* src[4] = {s->vx, s->vy, s->vz, s->th};
+  void* dst[4] = {s->C1, s->C2, s->C3, s->C7};
+  return op_copy_fields(p, s->th ? 4 : 3, src, dst);
 }
 
 int spx_substep(spx_plan* p, spx_state* s, int o) {
@@ -681,13 +692,65 @@ int spx_substep(spx_plan* p, spx_state* s, int o) {
   return SPX_OK;
 }
 
+static int step_once(spx_plan* p, spx_state* s) {
+  int rc;
+  if ((rc = spx_rkstep1(p, s))) return rc;
+  for (int o = s->prm.ord; o >= 1; --o)
+    if ((rc = spx_substep(p, s, o))) return rc;
+  return SPX_OK;
+}
+
+// Capture step_once on a private stream (a capture needs a stream other than
+// the null stream); the graph is then launched on the plan's stream.
+static int capture_step(spx_plan* p, spx_state* s) {
+  if (!s->cap) SPX_HIP(hipStreamCreateWithFlags(&s->cap, hipStreamNonBlocking));
+  hipStream_t orig = p->stream;
+  p->stream = s->cap;
+  hipError_t e = hipStreamBeginCapture(s->cap, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) {
+    p->stream = orig;
+    return fail(SPX_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+  }
+  const int rc = step_once(p, s);
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(s->cap, &g);
+  p->stream = orig;
+  if (rc != SPX_OK || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc != SPX_OK ? rc
+                        : fail(SPX_ERR_HIP, std::string("hipStreamEndCapture: ") +
+                                                hipGetErrorString(e));
+  }
+  s->graph = g;
+  e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    drop_graph(s);
+    return fail(SPX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  return SPX_OK;
+}
+
 int spx_step(spx_plan* p, spx_state* s, int nsteps) {
   SPX_REQ(p && s && s->p == p && nsteps >= 0, "spx_step: bad arguments");
   int rc;
-  for (int t = 0; t < nsteps; ++t) {
-    if ((rc = spx_rkstep1(p, s))) return rc;
-    for (int o = s->prm.ord; o >= 1; --o)
-      if ((rc = spx_substep(p, s, o))) return rc;
+  if (nsteps == 0) return SPX_OK;
+  // per-kernel timing needs the individual launches
+  if (!s->use_graph || p->prof.on) {
+    for (int t = 0; t < nsteps; ++t)
+      if ((rc = step_once(p, s))) return rc;
+    return SPX_OK;
+  }
+  if (!s->gexec && (rc = capture_step(p, s))) return rc;
+  for (int t = 0; t < nsteps; ++t) SPX_HIP(hipGraphLaunch(s->gexec, p->stream));
+  return SPX_OK;
+}
+
+int spx_state_set_graph(spx_state* s, int on) {
+  SPX_REQ(s, "spx_state_set_graph: null state");
+  s->use_graph = on != 0;
+  if (!s->use_graph) {
+    if (s->p) SPX_HIP(hipStreamSynchronize(s->p->stream));
+    drop_graph(s);
   }
   return SPX_OK;
 }

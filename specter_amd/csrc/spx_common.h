@@ -230,6 +230,7 @@ int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, 
                  const double* nu, double dt, double rmp, bool nl_lo = false);
 int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d);
 int op_axpy_field(spx_plan* p, void* y, const void* x, double alpha);
+int op_copy_fields(spx_plan* p, int n, const void* const* src, void* const* dst);
 int op_real_div3(spx_plan* p, void* r, double nx, double ny, double nz);
 
 }  // namespace spx

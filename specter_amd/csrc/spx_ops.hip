@@ -184,6 +184,20 @@ __global__ void k_axpy(cpx<TS>* __restrict__ y, const cpx<TS>* __restrict__ x, l
   if (e < nc) st(y, e, csub(ld(y, e), cscale(ld(x, e), alpha)));
 }
 
+// dst[q] <- src[q] for q < n (the RK start copies; a kernel rather than
+// memcpy nodes so that a captured step holds kernels only)
+struct CopyArgs {
+  const Cp* src[4];
+  Cp* dst[4];
+};
+__global__ void k_copy_fields(CopyArgs args, int n, long nc) {
+  const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= nc) return;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (q < n) args.dst[q][e] = args.src[q][e];
+}
+
 // ---------------------------------------------------------------------------
 static inline unsigned nblk(long n) { return unsigned((n + 255) / 256); }
 static inline SIdx sidx(spx_plan* p) { return SIdx{p->g.nz, p->g.ny}; }
@@ -274,6 +288,19 @@ int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d) {
   hipLaunchKernelGGL(k_poisson_project<double>, dim3(nblk(p->nc)), dim3(256), 0, p->stream,
                      CS<double>(a), CS<double>(b), CS<double>(c), CS<double>(d), long(p->nc),
                      sidx(p), p->d_kx, p->d_ky, p->d_kz);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+
+int op_copy_fields(spx_plan* p, int n, const void* const* src, void* const* dst) {
+  CopyArgs args;
+  for (int q = 0; q < 4; ++q) {
+    args.src[q] = q < n ? CS<double>(src[q]) : nullptr;
+    args.dst[q] = q < n ? CS<double>(dst[q]) : nullptr;
+  }
+  ProfScope ps_(p, "copy", 2.0 * n * double(p->nc) * 16.0);
+  hipLaunchKernelGGL(k_copy_fields, dim3(nblk(p->nc)), dim3(256), 0, p->stream, args, n,
+                     long(p->nc));
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }

@@ -71,6 +71,7 @@ _SIGS = {
     "spx_rkstep1": [_P, _P],
     "spx_substep": [_P, _P, c_int],
     "spx_step": [_P, _P, c_int],
+    "spx_state_set_graph": [_P, c_int],
     "spx_prof_enable": [_P, c_int],
     "spx_prof_resolve": [_P, POINTER(c_int)],
     "spx_prof_get": [_P, c_int, c_char_p, c_size_t, POINTER(ctypes.c_long), POINTER(c_double),

@@ -95,7 +95,12 @@ class Solver:
         _l.check(self._lib.spx_substep(self.plan.handle, self._s, o))
 
     def step(self, nsteps: int = 1):
+        """nsteps full RK steps (one captured HIP graph per step unless
+        use_graph(False) was called)."""
         _l.check(self._lib.spx_step(self.plan.handle, self._s, nsteps))
+
+    def use_graph(self, on: bool = True):
+        _l.check(self._lib.spx_state_set_graph(self._s, int(on)))
 
     # -- diagnostics (hd_global.f90 / vdiagnostic) -----------------------------------------
     def hdcheck(self):
