@@ -84,6 +84,19 @@ int spx_plan_get_info(const spx_plan* p, spx_plan_info* info);
 /* Block the host until the plan's stream is idle. */
 int spx_plan_sync(spx_plan* p);
 
+/* ---- batched plans (no counterpart in the reference; for ensembles of
+ * independent runs on one grid).  A batched plan holds nmembers runs of the
+ * grid g; every field argument then holds the members one after the other
+ * (member b of a spectral array at element offset b*n_spectral, of a real
+ * array at b*n_real), and every call processes the nactive leading members
+ * (default: all).  Diagnostics write one value per active member to their
+ * output arrays.  spx_plan_create is spx_plan_create_batch with nmembers = 1.
+ * Results of a member do not depend on the other members. */
+int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, int nmembers,
+                          void* stream, spx_plan** out);
+int spx_plan_get_members(const spx_plan* p, int* nmembers, int* nactive);
+int spx_plan_set_active(spx_plan* p, int nactive);
+
 /* ---- transforms (fftp/fftp.fpp) ------------------------------------------ */
 /* fftp3d_real_to_complex (fftp.fpp:388-425): R -> S. Continuation planes of the
  * real input (k > nz-Cz) are ignored. */
@@ -129,7 +142,8 @@ int spx_v_imposebc_and_project(spx_plan* p, void* vx, void* vy, void* vz, void* 
 /* s_imposebc (sboundary.f90:67-120): scalar = 0 at both walls. */
 int spx_s_imposebc(spx_plan* p, void* th);
 
-/* ---- global diagnostics (host-side results, synchronous) ------------------ */
+/* ---- global diagnostics (host-side results, synchronous; batched plans write
+ * one value per active member to each output) ------------------------------ */
 int spx_energy(spx_plan* p, const void* a, const void* b, const void* c, int kin,
                double* out);                                   /* pseudospec_hd.f90:405-635 */
 int spx_cross(spx_plan* p, const void* a, const void* b, const void* c, const void* d,
@@ -164,6 +178,12 @@ int spx_step(spx_plan* p, spx_state* s, int nsteps);
 /* Whether spx_step replays one captured HIP graph per RK step (default 1)
  * instead of launching every kernel; no counterpart in the reference. */
 int spx_state_set_graph(spx_state* s, int on);
+/* Diffusivities of one member (nu for the velocity, kappa for th); members
+ * start with the values of spx_state_create's params. */
+int spx_state_set_member(spx_state* s, int member, double nu, double kappa);
+/* Copy member src's fields (v, pr, forcing, th, fs) and diffusivities onto
+ * member dst, e.g. to keep the active members leading when a run ends. */
+int spx_state_copy_member(spx_state* s, int dst, int src);
 
 /* ---- per-kernel timing (no counterpart in the reference; the reference's
  * gtimer accumulators ffttime/tratime/conttime, fftp_mod.fpp:32-37, play this

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 
 #include "spx_common.h"
@@ -69,7 +70,7 @@ static int upload(spx_plan* p, double** dst, const V& host) {
 // gradre / advect work arrays in the nonlinear pipeline's precision
 static int ensure_work(spx_plan* p, bool bouss) {
   int rc;
-  const size_t cb = p->nc * p->csize, rb = p->nr * p->rsize;
+  const size_t cb = p->nc * p->nb * p->csize, rb = p->nr * p->nb * p->rsize;
   if (!p->XV[0]) {
     for (int c = 0; c < 3; ++c) {
       if ((rc = alloc_dev(p, &p->XV[c], cb)) || (rc = alloc_dev(p, &p->XZ[c], cb)) ||
@@ -77,7 +78,7 @@ static int ensure_work(spx_plan* p, bool bouss) {
           (rc = alloc_dev(p, &p->RO[c], rb)))
         return rc;
     }
-    if ((rc = alloc_dev(p, &p->R0, p->nr * 8))) return rc;
+    if ((rc = alloc_dev(p, &p->R0, p->nr * p->nb * 8))) return rc;
   }
   if (bouss && !p->work_bouss) {
     if ((rc = alloc_dev(p, &p->XV[3], cb)) || (rc = alloc_dev(p, &p->XZ[3], cb)) ||
@@ -117,13 +118,26 @@ static int fetch_parts(spx_plan* p, size_t count) {
   return SPX_OK;
 }
 
-// weighted sum over (kx,ky) lines: weight 1 for kx = 0, 2 otherwise
-static double weighted_sum(const spx_plan* p, const std::vector<double>& acc) {
-  const long ny = p->g.ny, nl = long(p->g.ny) * p->nkx;
+// weighted sum over the (kx,ky) lines of member b: weight 1 for kx = 0, 2 otherwise
+static double weighted_sum(const spx_plan* p, const std::vector<double>& acc, int b = 0) {
+  const long ny = p->g.ny, nl = long(p->g.ny) * p->nkx, o = long(b) * nl;
   double s0 = 0.0, s1 = 0.0;
-  for (long l = 0; l < ny; ++l) s0 += acc[l];
-  for (long l = ny; l < nl; ++l) s1 += acc[l];
+  for (long l = 0; l < ny; ++l) s0 += acc[o + l];
+  for (long l = ny; l < nl; ++l) s1 += acc[o + l];
   return s0 + 2.0 * s1;
+}
+
+static inline void* coff(void* a, size_t elems, int b) {
+  return static_cast<char*>(a) + size_t(b) * elems * 16;
+}
+static inline const void* coff(const void* a, size_t elems, int b) {
+  return static_cast<const char*>(a) + size_t(b) * elems * 16;
+}
+static inline void* roff(void* a, size_t elems, int b) {
+  return static_cast<char*>(a) + size_t(b) * elems * 8;
+}
+static inline const void* roff(const void* a, size_t elems, int b) {
+  return static_cast<const char*>(a) + size_t(b) * elems * 8;
 }
 
 }  // namespace spx
@@ -138,11 +152,15 @@ struct spx_state {
   void *C1 = nullptr, *C2 = nullptr, *C3 = nullptr, *C7 = nullptr;
   void* N[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<void*> owned;
-  // one full RK step (rkstep1 + ord stages) captured as a HIP graph
+  // diffusivities per (member, kx): nu for the velocity, kappa for th
+  double* d_nu = nullptr;
+  double* d_kappa = nullptr;
+  std::vector<double> h_nu, h_kappa;
+  // one full RK step (rkstep1 + ord stages) captured as a HIP graph, per
+  // active-member count
   bool use_graph = true;
   hipStream_t cap = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 };
 
 #define SPX_REQ(cond, msg) \
@@ -161,8 +179,14 @@ int spx_last_error(char* buf, size_t len) {
 
 int spx_plan_create(const spx_grid* g, const double* A, const double* Q, void* stream,
                     spx_plan** out) {
+  return spx_plan_create_batch(g, A, Q, 1, stream, out);
+}
+
+int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, int nmembers,
+                          void* stream, spx_plan** out) {
   SPX_REQ(g && out, "spx_plan_create: null argument");
   *out = nullptr;
+  SPX_REQ(nmembers >= 1, "nmembers must be >= 1");
   SPX_REQ(g->nx >= 2 && g->nx % 2 == 0, "nx must be even and >= 2");
   SPX_REQ(g->ny >= 2 && g->ny % 2 == 0, "ny must be even and >= 2");
   SPX_REQ(g->nz >= 2, "nz must be >= 2");
@@ -179,6 +203,8 @@ int spx_plan_create(const spx_grid* g, const double* A, const double* Q, void* s
   }
   spx_plan* p = new spx_plan();
   p->g = *g;
+  p->nb = p->nact = nmembers;
+  p->mem = 1;
   p->stream = static_cast<hipStream_t>(stream);
   const int nx = g->nx, ny = g->ny, nz = g->nz, C = g->Cz, d = g->oz;
   p->nkx = nx / 2 + 1;
@@ -239,10 +265,18 @@ int spx_plan_create(const spx_grid* g, const double* A, const double* Q, void* s
     p->h_fky[j] = std::exp(-alpha * std::pow(2 * p->h_ky[j] / ny / p->Dky, pw));
   for (int k = 0; k < nz; ++k)
     p->h_fkz[k] = std::exp(-alpha * std::pow(2 * p->h_kz[k] / nz / p->Dkz, pw));
-  if ((rc = upload(p, &p->d_kx, p->h_kx)) || (rc = upload(p, &p->d_ky, p->h_ky)) ||
+  // tables indexed by the (member, kx) line group repeat once per member
+  auto repeat = [&](const std::vector<double>& v) {
+    std::vector<double> r;
+    r.reserve(v.size() * size_t(nmembers));
+    for (int b = 0; b < nmembers; ++b) r.insert(r.end(), v.begin(), v.end());
+    return r;
+  };
+  if ((rc = upload(p, &p->d_kx, repeat(p->h_kx))) || (rc = upload(p, &p->d_ky, p->h_ky)) ||
       (rc = upload(p, &p->d_kz, p->h_kz)) || (rc = upload(p, &p->d_z, p->h_z)) ||
-      (rc = upload(p, &p->d_khom, p->h_khom)) || (rc = upload(p, &p->d_fkx, p->h_fkx)) ||
-      (rc = upload(p, &p->d_fky, p->h_fky)) || (rc = upload(p, &p->d_fkz, p->h_fkz)))
+      (rc = upload(p, &p->d_khom, repeat(p->h_khom))) ||
+      (rc = upload(p, &p->d_fkx, repeat(p->h_fkx))) || (rc = upload(p, &p->d_fky, p->h_fky)) ||
+      (rc = upload(p, &p->d_fkz, p->h_fkz)))
     return bail(rc);
   // line FFT plans
   if ((rc = build_line_fft(p->fx, nx, 16, 0))) return bail(rc);
@@ -255,8 +289,8 @@ int spx_plan_create(const spx_grid* g, const double* A, const double* Q, void* s
     if ((rc = build_line_fft(p->fz32, nz, 8, 16))) return bail(rc);
   }
   // scratch
-  const size_t nlines = size_t(ny) * p->nkx;
-  if ((rc = alloc_dev(p, &p->X, p->nc * 16))) return bail(rc);
+  const size_t nlines = size_t(ny) * p->nkx * nmembers;
+  if ((rc = alloc_dev(p, &p->X, p->nc * nmembers * 16))) return bail(rc);
   if ((rc = alloc_dev(p, &p->bc, nlines * 2 * 16))) return bail(rc);
   if ((rc = alloc_dev(p, reinterpret_cast<void**>(&p->part), nlines * 2 * sizeof(double))))
     return bail(rc);
@@ -293,7 +327,21 @@ int spx_plan_get_info(const spx_plan* p, spx_plan_info* info) {
   size_t b = p->nc * 16 + size_t(p->g.ny) * p->nkx * 48;
   if (p->XV[0]) b += 9 * p->nc * p->csize + 6 * p->nr * p->rsize + p->nr * 8;
   if (p->work_bouss) b += 3 * p->nc * p->csize + p->nr * p->rsize;
-  info->device_bytes = b;
+  info->device_bytes = b * size_t(p->nb);
+  return SPX_OK;
+}
+
+int spx_plan_get_members(const spx_plan* p, int* nmembers, int* nactive) {
+  SPX_REQ(p && nmembers && nactive, "spx_plan_get_members: null argument");
+  *nmembers = p->nb;
+  *nactive = p->nact;
+  return SPX_OK;
+}
+
+int spx_plan_set_active(spx_plan* p, int nactive) {
+  SPX_REQ(p, "spx_plan_set_active: null plan");
+  SPX_REQ(nactive >= 1 && nactive <= p->nb, "spx_plan_set_active: nactive out of range");
+  p->nact = nactive;
   return SPX_OK;
 }
 
@@ -304,44 +352,64 @@ int spx_plan_sync(spx_plan* p) {
 }
 
 // ---- transforms -------------------------------------------------------------
+// The 3D and 2D transforms take member-contiguous real arrays, so batched
+// plans run them member by member; everything line-based runs on all active
+// members at once.
 int spx_fftp3d_real_to_complex(spx_plan* p, const void* in, void* out) {
   SPX_REQ(p && in && out, "spx_fftp3d_real_to_complex: null argument");
-  return r2c3(p, in, out);
+  MemberView mv(p, 1);
+  for (int b = 0; b < p->nact; ++b)
+    if (int rc = r2c3(p, roff(in, p->nr, b), coff(out, p->nc, b))) return rc;
+  return SPX_OK;
 }
 
 int spx_fftp3d_complex_to_real(spx_plan* p, void* in, void* out) {
   SPX_REQ(p && in && out, "spx_fftp3d_complex_to_real: null argument");
-  return c2r3(p, in, out, 0, p->g.nz);
+  MemberView mv(p, 1);
+  for (int b = 0; b < p->nact; ++b)
+    if (int rc = c2r3(p, coff(in, p->nc, b), roff(out, p->nr, b), 0, p->g.nz)) return rc;
+  return SPX_OK;
 }
 
 int spx_fftp2d_real_to_complex_xy(spx_plan* p, const void* in, void* out) {
   SPX_REQ(p && in && out, "spx_fftp2d_real_to_complex_xy: null argument");
+  MemberView mv(p, 1);
   int rc;
-  if ((rc = xr2c(p, in, p->X, p->g.nz))) return rc;
-  if ((rc = yfft(p, p->X, 1, p->g.nz))) return rc;
-  return zfft_x2s(p, p->X, out, 1);
+  for (int b = 0; b < p->nact; ++b) {
+    if ((rc = xr2c(p, roff(in, p->nr, b), p->X, p->g.nz))) return rc;
+    if ((rc = yfft(p, p->X, 1, p->g.nz))) return rc;
+    if ((rc = zfft_x2s(p, p->X, coff(out, p->nc, b), 1))) return rc;
+  }
+  return SPX_OK;
 }
 
 int spx_fftp2d_complex_to_real_xy(spx_plan* p, void* in, void* out) {
   SPX_REQ(p && in && out, "spx_fftp2d_complex_to_real_xy: null argument");
+  MemberView mv(p, 1);
   int rc;
-  if ((rc = zfft_s2x(p, in, p->X, 0, p->g.nz, 1))) return rc;
-  if ((rc = yfft(p, p->X, 0, p->g.nz))) return rc;
-  return xc2r(p, p->X, out, p->g.nz);
+  for (int b = 0; b < p->nact; ++b) {
+    if ((rc = zfft_s2x(p, coff(in, p->nc, b), p->X, 0, p->g.nz, 1))) return rc;
+    if ((rc = yfft(p, p->X, 0, p->g.nz))) return rc;
+    if ((rc = xc2r(p, p->X, roff(out, p->nr, b), p->g.nz))) return rc;
+  }
+  return SPX_OK;
 }
 
 int spx_fftp1d_real_to_complex_z(spx_plan* p, void* s) {
   SPX_REQ(p && s, "spx_fftp1d_real_to_complex_z: null argument");
+  MemberView mv(p, p->nact);
   return zfft_inplace(p, s, 1, 1, 1.0);
 }
 
 int spx_fftp1d_complex_to_real_z(spx_plan* p, void* s) {
   SPX_REQ(p && s, "spx_fftp1d_complex_to_real_z: null argument");
+  MemberView mv(p, p->nact);
   return zfft_inplace(p, s, 0, 0, 1.0);
 }
 
 int spx_fc_continue(spx_plan* p, void* s) {
   SPX_REQ(p && s, "spx_fc_continue: null argument");
+  MemberView mv(p, p->nact);
   return fc_continue_only(p, s);
 }
 
@@ -349,16 +417,19 @@ int spx_fc_continue(spx_plan* p, void* s) {
 int spx_derivk(spx_plan* p, const void* a, void* b, int dir) {
   SPX_REQ(p && a && b, "spx_derivk: null argument");
   SPX_REQ(dir >= 1 && dir <= 3, "spx_derivk: dir must be 1, 2 or 3");
+  MemberView mv(p, p->nact);
   return op_derivk(p, a, b, dir);
 }
 
 int spx_laplak(spx_plan* p, const void* a, void* b) {
   SPX_REQ(p && a && b, "spx_laplak: null argument");
+  MemberView mv(p, p->nact);
   return op_laplak(p, a, b);
 }
 
 int spx_fc_filter(spx_plan* p, void* a) {
   SPX_REQ(p && a, "spx_fc_filter: null argument");
+  MemberView mv(p, p->nact);
   return op_filter(p, a);
 }
 
@@ -426,6 +497,7 @@ int spx_gradre(spx_plan* p, const void* a, const void* b, const void* c, void* d
   SPX_REQ(p && a && b && c && d && e && f, "spx_gradre: null argument");
   int rc;
   if ((rc = ensure_work(p, false))) return rc;
+  MemberView mv(p, p->nact);
   const void* v[3] = {a, b, c};
   void* o[3] = {d, e, f};
   return gradre_impl(p, v, o);
@@ -436,6 +508,7 @@ int spx_advect(spx_plan* p, const void* a, const void* b, const void* c, const v
   SPX_REQ(p && a && b && c && dd && e, "spx_advect: null argument");
   int rc;
   if ((rc = ensure_work(p, true))) return rc;
+  MemberView mv(p, p->nact);
   const void* v[3] = {a, b, c};
   return advect_impl(p, v, dd, e, false);
 }
@@ -448,6 +521,7 @@ int spx_sol_project(spx_plan* p, void* a, void* b, void* c, void* d, int bctarge
   if (bctarget != 1 || bczsta != 0 || bczend != 0)
     return fail(SPX_ERR_UNSUPPORTED,
                 "sol_project: only bctarget=1 with homogeneous Dirichlet walls is implemented");
+  MemberView mv(p, p->nact);
   int rc;
   if ((rc = op_poisson_project(p, a, b, c, d))) return rc;
   if ((rc = proj_bc(p, c))) return rc;
@@ -462,6 +536,7 @@ int spx_v_imposebc_and_project(spx_plan* p, void* vx, void* vy, void* vz, void* 
   if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "no-slip walls need Cz > 0");
   double tau = 1.0 / double(rki);
   if (rki != ord) tau = double(rki + 1) * tau;
+  MemberView mv(p, p->nact);
   int rc;
   if ((rc = noslip_fused(p, vx, vy, pr, tau, v_zsta, v_zend))) return rc;
   return spx_sol_project(p, vx, vy, vz, pr, 1, 0, 0);
@@ -470,12 +545,15 @@ int spx_v_imposebc_and_project(spx_plan* p, void* vx, void* vy, void* vz, void* 
 int spx_s_imposebc(spx_plan* p, void* th) {
   SPX_REQ(p && th, "spx_s_imposebc: null argument");
   if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "scalar walls need Cz > 0");
+  MemberView mv(p, p->nact);
   return sconst_fused(p, th);
 }
 
 // ---- diagnostics ------------------------------------------------------------------
+// (the diagnostics run in a view of all active members; acc holds one value
+// per line of every member)
 static int sumsq_lines(spx_plan* p, const LinComb& lc, std::vector<double>& acc, double w) {
-  const size_t nl = size_t(p->g.ny) * p->nkx;
+  const size_t nl = size_t(p->wlines());
   int rc;
   if ((rc = zline_sumsq(p, lc, p->part))) return rc;
   if ((rc = fetch_parts(p, nl))) return rc;
@@ -505,7 +583,8 @@ static LinComb two(const void* a, int da, const void* b, int db, int sg) {
 int spx_energy(spx_plan* p, const void* a, const void* b, const void* c, int kin, double* out) {
   SPX_REQ(p && a && b && c && out, "spx_energy: null argument");
   SPX_REQ(kin == 0 || kin == 1, "spx_energy: kin must be 0 or 1");
-  std::vector<double> acc(size_t(p->g.ny) * p->nkx, 0.0);
+  MemberView mv(p, p->nact);
+  std::vector<double> acc(size_t(p->wlines()), 0.0);
   int rc;
   if (kin == 1) {
     for (const void* q : {a, b, c})
@@ -516,14 +595,15 @@ int spx_energy(spx_plan* p, const void* a, const void* b, const void* c, int kin
     if ((rc = sumsq_lines(p, two(c, 2, b, 3, -1), acc, 1.0))) return rc;
     if ((rc = sumsq_lines(p, two(a, 3, c, 1, -1), acc, 2.0))) return rc;
   }
-  *out = weighted_sum(p, acc) * diag_tmp(p);
+  for (int b = 0; b < p->nact; ++b) out[b] = weighted_sum(p, acc, b) * diag_tmp(p);
   return SPX_OK;
 }
 
 int spx_cross(spx_plan* p, const void* a, const void* b, const void* c, const void* d,
               const void* e, const void* f, double* out) {
   SPX_REQ(p && a && b && c && d && e && f && out, "spx_cross: null argument");
-  const size_t nl = size_t(p->g.ny) * p->nkx;
+  MemberView mv(p, p->nact);
+  const size_t nl = size_t(p->wlines());
   std::vector<double> acc(nl, 0.0);
   const void* L[3] = {a, b, c};
   const void* Rr[3] = {d, e, f};
@@ -533,7 +613,7 @@ int spx_cross(spx_plan* p, const void* a, const void* b, const void* c, const vo
     if ((rc = fetch_parts(p, nl))) return rc;
     for (size_t l = 0; l < nl; ++l) acc[l] += p->h_part[l];
   }
-  *out = weighted_sum(p, acc) * diag_tmp(p);
+  for (int b = 0; b < p->nact; ++b) out[b] = weighted_sum(p, acc, b) * diag_tmp(p);
   return SPX_OK;
 }
 
@@ -543,16 +623,18 @@ int spx_divergence(spx_plan* p, const void* a, const void* b, const void* c, dou
   lc.in[2] = c;
   lc.dir[2] = 3;
   lc.sgn[2] = 1;
-  std::vector<double> acc(size_t(p->g.ny) * p->nkx, 0.0);
+  MemberView mv(p, p->nact);
+  std::vector<double> acc(size_t(p->wlines()), 0.0);
   int rc = sumsq_lines(p, lc, acc, 1.0);
   if (rc) return rc;
-  *out = weighted_sum(p, acc) * diag_tmp(p);
+  for (int m = 0; m < p->nact; ++m) out[m] = weighted_sum(p, acc, m) * diag_tmp(p);
   return SPX_OK;
 }
 
 int spx_bouncheck_z(spx_plan* p, const void* a, const void* b, double* bot, double* top) {
   SPX_REQ(p && a && bot && top, "spx_bouncheck_z: null argument");
-  const size_t nl = size_t(p->g.ny) * p->nkx;
+  MemberView mv(p, p->nact);
+  const size_t nl = size_t(p->wlines());
   std::vector<double> ab(nl, 0.0), at(nl, 0.0);
   int rc;
   for (const void* q : {a, b}) {
@@ -565,8 +647,10 @@ int spx_bouncheck_z(spx_plan* p, const void* a, const void* b, double* bot, doub
     }
   }
   const double t = 1.0 / double(long(p->g.nx) * p->g.ny * p->g.nz);
-  *bot = weighted_sum(p, ab) * (t * t);
-  *top = weighted_sum(p, at) * (t * t);
+  for (int m = 0; m < p->nact; ++m) {
+    bot[m] = weighted_sum(p, ab, m) * (t * t);
+    top[m] = weighted_sum(p, at, m) * (t * t);
+  }
   return SPX_OK;
 }
 
@@ -590,7 +674,7 @@ int spx_state_create(spx_plan* p, const spx_params* prm, spx_state** out) {
   // the nonlinear terms N are in the nonlinear pipeline's precision
   auto mk = [&](void** dst) -> int {
     const bool nl = dst == &s->N[0] || dst == &s->N[1] || dst == &s->N[2] || dst == &s->N[3];
-    const size_t bytes = p->nc * (nl ? p->csize : 16);
+    const size_t bytes = p->nc * size_t(p->nb) * (nl ? p->csize : 16);
     if (hipMalloc(dst, bytes) != hipSuccess) return fail(SPX_ERR_ALLOC, "state allocation");
     s->owned.push_back(*dst);
     if (hipMemsetAsync(*dst, 0, bytes, p->stream) != hipSuccess)
@@ -608,6 +692,23 @@ int spx_state_create(spx_plan* p, const spx_params* prm, spx_state** out) {
         spx_state_destroy(s);
         return rc;
       }
+  // per-(member, kx) diffusivity tables, every member starting from prm
+  s->h_nu.assign(size_t(p->nb) * p->nkx, prm->nu);
+  s->h_kappa.assign(size_t(p->nb) * p->nkx, prm->kappa);
+  const size_t tb = s->h_nu.size() * sizeof(double);
+  for (double** t : {&s->d_nu, &s->d_kappa}) {
+    if (hipMalloc(reinterpret_cast<void**>(t), tb) != hipSuccess) {
+      *t = nullptr;
+      spx_state_destroy(s);
+      return fail(SPX_ERR_ALLOC, "state allocation");
+    }
+    s->owned.push_back(*t);
+  }
+  if (hipMemcpy(s->d_nu, s->h_nu.data(), tb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->d_kappa, s->h_kappa.data(), tb, hipMemcpyHostToDevice) != hipSuccess) {
+    spx_state_destroy(s);
+    return fail(SPX_ERR_HIP, "state tables upload");
+  }
   if (hipStreamSynchronize(p->stream) != hipSuccess) {
     spx_state_destroy(s);
     return fail(SPX_ERR_HIP, "state clear sync");
@@ -617,10 +718,11 @@ int spx_state_create(spx_plan* p, const spx_params* prm, spx_state** out) {
 }
 
 static void drop_graph(spx_state* s) {
-  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
-  if (s->graph) (void)hipGraphDestroy(s->graph);
-  s->gexec = nullptr;
-  s->graph = nullptr;
+  for (auto& kv : s->graphs) {
+    if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+    if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
+  }
+  s->graphs.clear();
 }
 
 int spx_state_destroy(spx_state* s) {
@@ -651,6 +753,7 @@ int spx_state_field(spx_state* s, const char* name, void** ptr) {
 
 int spx_rkstep1(spx_plan* p, spx_state* s) {
   SPX_REQ(p && s && s->p == p, "spx_rkstep1: bad plan/state");
+  MemberView mv(p, p->nact);
   const void* src[4] = {s->vx, s->vy, s->vz, s->th};
   void* dst[4] = {s->C1, s->C2, s->C3, s->C7};
   return op_copy_fields(p, s->th ? 4 : 3, src, dst);
@@ -663,6 +766,7 @@ int spx_substep(spx_plan* p, spx_state* s, int o) {
   const bool bouss = q.solver == SPX_SOLVER_BOUSS;
   const double rmp = 1.0 / double(o);
   int rc;
+  MemberView mv(p, p->nact);
   const void* vel[3] = {s->vx, s->vy, s->vz};
   void* nlo[3] = {s->N[0], s->N[1], s->N[2]};
   if ((rc = gradre_impl(p, vel, nlo))) return rc;
@@ -673,7 +777,7 @@ int spx_substep(spx_plan* p, spx_state* s, int o) {
   const void* f[4] = {s->fx, s->fy, s->fz, s->fs};
   const void* src[4] = {nullptr, nullptr, s->th, s->vz};
   const double coef[4] = {0.0, 0.0, q.xmom, q.xtemp};
-  const double nu[4] = {q.nu, q.nu, q.nu, q.kappa};
+  const double* nu[4] = {s->d_nu, s->d_nu, s->d_nu, s->d_kappa};
   if ((rc = op_rk_update(p, bouss ? 4 : 3, v, c0, nl, f, bouss ? src : nullptr, coef, nu, q.dt,
                          rmp, p->fp32)))
     return rc;
@@ -721,12 +825,13 @@ static int capture_step(spx_plan* p, spx_state* s) {
                         : fail(SPX_ERR_HIP, std::string("hipStreamEndCapture: ") +
                                                 hipGetErrorString(e));
   }
-  s->graph = g;
-  e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
   if (e != hipSuccess) {
-    drop_graph(s);
+    (void)hipGraphDestroy(g);
     return fail(SPX_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
   }
+  s->graphs[p->nact] = std::make_pair(g, x);
   return SPX_OK;
 }
 
@@ -740,9 +845,43 @@ int spx_step(spx_plan* p, spx_state* s, int nsteps) {
       if ((rc = step_once(p, s))) return rc;
     return SPX_OK;
   }
-  if (!s->gexec && (rc = capture_step(p, s))) return rc;
-  for (int t = 0; t < nsteps; ++t) SPX_HIP(hipGraphLaunch(s->gexec, p->stream));
+  if (!s->graphs.count(p->nact) && (rc = capture_step(p, s))) return rc;
+  hipGraphExec_t x = s->graphs[p->nact].second;
+  for (int t = 0; t < nsteps; ++t) SPX_HIP(hipGraphLaunch(x, p->stream));
   return SPX_OK;
+}
+
+int spx_state_set_member(spx_state* s, int member, double nu, double kappa) {
+  SPX_REQ(s && s->p, "spx_state_set_member: null state");
+  spx_plan* p = s->p;
+  SPX_REQ(member >= 0 && member < p->nb, "spx_state_set_member: member out of range");
+  const size_t o = size_t(member) * p->nkx;
+  for (int i = 0; i < p->nkx; ++i) {
+    s->h_nu[o + i] = nu;
+    s->h_kappa[o + i] = kappa;
+  }
+  const size_t tb = size_t(p->nkx) * sizeof(double);
+  // the host tables stay alive with the state, so the copies may be asynchronous
+  SPX_HIP(hipMemcpyAsync(s->d_nu + o, s->h_nu.data() + o, tb, hipMemcpyHostToDevice, p->stream));
+  SPX_HIP(hipMemcpyAsync(s->d_kappa + o, s->h_kappa.data() + o, tb, hipMemcpyHostToDevice,
+                         p->stream));
+  return SPX_OK;
+}
+
+int spx_state_copy_member(spx_state* s, int dst, int src) {
+  SPX_REQ(s && s->p, "spx_state_copy_member: null state");
+  spx_plan* p = s->p;
+  SPX_REQ(dst >= 0 && dst < p->nb && src >= 0 && src < p->nb,
+          "spx_state_copy_member: member out of range");
+  if (dst == src) return SPX_OK;
+  const size_t b = p->nc * 16;
+  for (void* f : {s->vx, s->vy, s->vz, s->pr, s->fx, s->fy, s->fz, s->th, s->fs}) {
+    if (!f) continue;
+    SPX_HIP(hipMemcpyAsync(coff(f, p->nc, dst), coff(static_cast<const void*>(f), p->nc, src), b,
+                           hipMemcpyDeviceToDevice, p->stream));
+  }
+  const size_t o = size_t(p->nkx);
+  return spx_state_set_member(s, dst, s->h_nu[src * o], s->h_kappa[src * o]);
 }
 
 int spx_state_set_graph(spx_state* s, int on) {

@@ -102,8 +102,21 @@ struct Profiler {
 struct spx_plan {
   spx_grid g;
   int nkx = 0, np = 0;
-  size_t nc = 0, nr = 0;  // complex / real element counts of S and R arrays
+  size_t nc = 0, nr = 0;  // complex / real element counts of one member's S and R arrays
   hipStream_t stream = nullptr;
+  // Batched plans hold nb independent runs ("members") of the same grid.
+  // Every field argument then holds the members one after the other (member b
+  // at offset b*nc or b*nr).  Calls process the nact leading members; mem is
+  // the member count of the current launch view (see MemberView).  Spectral
+  // and z-line kernels treat the members as extra (kx, ky) lines; the
+  // xy-stage and real buffers of a multi-plane pass hold, per member, only
+  // the planes the pass works on, members one after the other, so the y and
+  // x passes see mem times as many planes.
+  int nb = 1, nact = 1, mem = 1;
+  int wkx() const { return nkx * mem; }
+  long wlines() const { return long(g.ny) * nkx * mem; }
+  size_t wnc() const { return nc * size_t(mem); }
+  size_t wnr() const { return nr * size_t(mem); }
   // The state, the public transforms, the boundary / projection / diagnostics
   // kernels and the RK update are float64 in every plan.  A mixed-precision
   // plan (SPX_FP32_MIXED) evaluates the nonlinear term (gradre / advect:
@@ -115,7 +128,8 @@ struct spx_plan {
   // FC-Gram continuation matrix dir = A*Q^T, row-major [C][d], float64 always.
   double* d_dir = nullptr;
   std::vector<double> h_dir;
-  // wavenumbers (float64), filter factors, z grid and khom(kx,ky)
+  // wavenumbers (float64), filter factors, z grid and khom(kx,ky); the kx,
+  // fkx and khom tables are repeated once per member
   double *d_kx = nullptr, *d_ky = nullptr, *d_kz = nullptr;
   double *d_fkx = nullptr, *d_fky = nullptr, *d_fkz = nullptr;
   double *d_z = nullptr, *d_khom = nullptr;
@@ -158,6 +172,14 @@ int fail(int code, const std::string& msg);
                                           hipGetErrorString(e_));           \
   } while (0)
 #define SPX_CHECK_LAUNCH() SPX_HIP(hipGetLastError())
+
+// Sets the member count of the launch view for a scope.
+struct MemberView {
+  spx_plan* p;
+  int old;
+  MemberView(spx_plan* plan, int m) : p(plan), old(plan->mem) { plan->mem = m; }
+  ~MemberView() { p->mem = old; }
+};
 
 // RAII timing scope around one launch; records nothing when profiling is off.
 struct ProfScope {
@@ -225,9 +247,10 @@ int op_laplak(spx_plan* p, const void* a, void* b);
 int op_filter(spx_plan* p, void* a);
 int op_scale(spx_plan* p, void* a, size_t n, double s);
 // nl: float64, or float32 with nl_lo (the nonlinear terms of a mixed plan)
+// nu[c]: device table of the component's diffusivity per (member, kx), nb*nkx values
 int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, void* const* nl,
                  const void* const* f, const void* const* src, const double* src_coef,
-                 const double* nu, double dt, double rmp, bool nl_lo = false);
+                 const double* const* nu, double dt, double rmp, bool nl_lo = false);
 int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d);
 int op_axpy_field(spx_plan* p, void* y, const void* x, double alpha);
 int op_copy_fields(spx_plan* p, int n, const void* const* src, void* const* dst);

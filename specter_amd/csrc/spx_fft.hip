@@ -101,15 +101,18 @@ __global__ void k_fc_continue(cpx<T>* __restrict__ S, long nlines, int n, int np
 
 // ---------------------------------------------------------------------------
 // forward 3D last pass: X(kx,y,z) planes < np -> continuation -> z FFT -> S(z,y,kx)
+// blockIdx.z = member: S advances by n*ny*nkx, X by xstride elements per member.
 // ---------------------------------------------------------------------------
 template <typename T, int RS>
 __global__ void __launch_bounds__(256) k_z_x2s(const cpx<T>* __restrict__ X, cpx<T>* __restrict__ S,
                                                LineFFT<cpx<T>> f, int nkx, int ny, int np,
                                                int nofft, const double* __restrict__ dir, int C,
-                                               int d) {
+                                               int d, long xstride) {
   using Cp = cpx<T>;
   SPX_PROLOGUE(Cp, f);
   const int n = f.n;
+  X += blockIdx.z * xstride;
+  S += blockIdx.z * (long(n) * ny * nkx);
   const int i0 = blockIdx.x * f.nl, j = blockIdx.y;
   const int nload = nofft ? n : np;
   const long pl = long(nkx) * ny;
@@ -137,6 +140,7 @@ __global__ void __launch_bounds__(256) k_z_x2s(const cpx<T>* __restrict__ X, cpx
 // ---------------------------------------------------------------------------
 // backward 3D first pass: S * (i k_dir) -> z IFFT -> X planes < nzout.
 // DUAL: the same load also gives X2 <- z IFFT of (i kz) S.
+// blockIdx.z = member: S advances by n*ny*nkx, X and X2 by nkx*ny*nzout.
 // ---------------------------------------------------------------------------
 template <typename T, int RS, bool DUAL, typename TI>
 __global__ void __launch_bounds__(256) k_z_s2x(const cpx<TI>* __restrict__ S, cpx<T>* __restrict__ X,
@@ -148,6 +152,12 @@ __global__ void __launch_bounds__(256) k_z_s2x(const cpx<TI>* __restrict__ S, cp
   using Cp = cpx<T>;
   SPX_PROLOGUE(Cp, f);
   const int n = f.n;
+  {
+    const long xs = blockIdx.z * (long(nkx) * ny * nzout);
+    S += blockIdx.z * (long(n) * ny * nkx);
+    X += xs;
+    if constexpr (DUAL) X2 += xs;
+  }
   const int i0 = blockIdx.x * f.nl, j = blockIdx.y;
   const int i = i0 + line;
   const bool ok = i < nkx;
@@ -427,7 +437,7 @@ __global__ void __launch_bounds__(256) k_noslip(cpx<TS>* __restrict__ vx, cpx<TS
       const int j = int(ln - i * ny);
       const Cd pv = pr[ln * n + ind];
       val = cscale(ikmul(comp ? ky[j] : kx[i], pv), wp.tau);
-      if (i == 0 && j == 0) val = mkc<double>(wp.nxny * wp.vb[comp][w], 0.0);
+      if (kx[i] == 0.0 && ky[j] == 0.0) val = mkc<double>(wp.nxny * wp.vb[comp][w], 0.0);
     }
     xl[ind] = val;
   }
@@ -490,10 +500,9 @@ __global__ void __launch_bounds__(256) k_proj_pressure(Cd* __restrict__ dd,
   __syncthreads();
   line_dft<double, RS>(xl, f, tws, lane);
   if (!ok) return;
-  const long i = ln / ny;
-  const int j = int(ln - i * ny);
-  const bool zm = (i == 0 && j == 0);
+  // (kx, ky) = (0, 0) is the only line with khom = 0
   const double kh = khom[ln];
+  const bool zm = kh == 0.0;
   const Coef cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
   for (int k = lane; k < n; k += f.tpl) {
     Cd a;
@@ -531,8 +540,8 @@ __global__ void __launch_bounds__(256) k_proj_harmonic(cpx<TS>* __restrict__ va,
   const bool ok = ln < nlines;
   const long i = ok ? ln / ny : 0;
   const int j = ok ? int(ln - i * ny) : 0;
-  const bool zm = (i == 0 && j == 0);
   const double kh = ok ? khom[ln] : 0.0;
+  const bool zm = kh == 0.0;
   Coef cf;
   cf.c1 = cf.c2 = mkc<double>(0, 0);
   if (ok) cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
@@ -962,10 +971,10 @@ static int check_lo(const spx_plan* p, bool lo) {
 
 int zfft_inplace(spx_plan* p, void* S, int forward, int cont, double scale) {
   const LineFFTHost& f = p->fz;
-  const long nlines = long(p->g.ny) * p->nkx;
+  const long nlines = p->wlines();
   const unsigned nb = unsigned((nlines + f.nl - 1) / f.nl);
   const int C = p->g.Cz, d = p->g.oz;
-  ProfScope ps_(p, "z_inplace", double(p->nc) * 2.0 * 16.0);
+  ProfScope ps_(p, "z_inplace", double(p->wnc()) * 2.0 * 16.0);
   SPX_LAUNCH_RS(f, k_z_inplace, double, , dim3(nb), blk(f), f.lds_bytes(16), p->stream,
                 C_<double>(S), nlines, f.dev<Cd>(), p->np, forward, (cont && C > 0) ? 1 : 0,
                 scale, p->d_dir, C, d);
@@ -974,7 +983,7 @@ int zfft_inplace(spx_plan* p, void* S, int forward, int cont, double scale) {
 }
 
 int fc_continue_only(spx_plan* p, void* S) {
-  const long nlines = long(p->g.ny) * p->nkx;
+  const long nlines = p->wlines();
   const int C = p->g.Cz;
   if (C <= 0) return SPX_OK;
   const long tot = nlines * C;
@@ -987,12 +996,14 @@ int fc_continue_only(spx_plan* p, void* S) {
 
 template <typename T>
 static int zfft_x2s_t(spx_plan* p, const void* X, void* S, int nofft, const LineFFTHost& f) {
-  dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(p->g.ny));
-  ProfScope ps_(p, "z_x2s", double(p->nkx) * p->g.ny * sizeof(cpx<T>) *
+  // members: grid.z; X holds np planes per member (all nz with nofft)
+  dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(p->g.ny), unsigned(p->mem));
+  const long xstride = long(p->nkx) * p->g.ny * (nofft ? p->g.nz : p->np);
+  ProfScope ps_(p, "z_x2s", double(p->wkx()) * p->g.ny * sizeof(cpx<T>) *
                                 ((nofft ? p->g.nz : p->np) + p->g.nz));
   SPX_LAUNCH_RS(f, k_z_x2s, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
                 C_<T>(X), C_<T>(S), f.dev<cpx<T>>(), p->nkx, p->g.ny, p->np, nofft, p->d_dir,
-                p->g.Cz, p->g.oz);
+                p->g.Cz, p->g.oz, xstride);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
@@ -1006,8 +1017,9 @@ int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft, bool lo) {
 template <typename T>
 static int zfft_s2x_t(spx_plan* p, const void* S, void* X, int dirn, int nzout, int nofft,
                       void* X2, const LineFFTHost& f) {
-  dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(p->g.ny));
-  const double line = double(p->nkx) * p->g.ny;
+  // members: grid.z; X holds nzout planes per member
+  dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(p->g.ny), unsigned(p->mem));
+  const double line = double(p->wkx()) * p->g.ny;
   const Cd* in = C_<double>(S);
   if (X2) {
     ProfScope ps_(p, "z_s2x_dual", line * (16.0 * p->g.nz + 2.0 * sizeof(cpx<T>) * nzout));
@@ -1035,9 +1047,11 @@ int zfft_s2x(spx_plan* p, const void* S, void* X, int dirn, int nzout, int nofft
 template <typename T>
 static int yfft_t(spx_plan* p, void* X, int forward, int nplanes, const void* Xin, void* Y2,
                   const LineFFTHost& f) {
-  dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(nplanes));
+  // members are consecutive blocks of nplanes planes
+  const int npl = nplanes * p->mem;
+  dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(npl));
   const cpx<T>* in = C_<T>(Xin ? Xin : static_cast<const void*>(X));
-  const double io = double(p->nkx) * p->g.ny * nplanes * sizeof(cpx<T>);
+  const double io = double(p->nkx) * p->g.ny * npl * sizeof(cpx<T>);
   if (Y2) {
     if (forward) return fail(SPX_ERR_ARG, "dual y transform is backward only");
     ProfScope ps_(p, "y_dual", io * 3.0);
@@ -1061,9 +1075,9 @@ int yfft(spx_plan* p, void* X, int forward, int nplanes, const void* Xin, void* 
 
 template <typename T>
 static int xr2c_t(spx_plan* p, const void* R, void* X, int nplanes, const LineFFTHost& f) {
-  const long npairs = long(p->g.ny) * nplanes / 2;
+  const long npairs = long(p->g.ny) * nplanes * p->mem / 2;
   dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
-  ProfScope ps_(p, "xr2c", double(p->g.ny) * nplanes *
+  ProfScope ps_(p, "xr2c", double(p->g.ny) * nplanes * p->mem *
                                (double(sizeof(T)) * p->g.nx + double(sizeof(cpx<T>)) * p->nkx));
   SPX_LAUNCH_RS(f, k_xr2c, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
                 static_cast<const T*>(R), C_<T>(X), f.dev<cpx<T>>(), p->nkx, npairs);
@@ -1078,9 +1092,9 @@ int xr2c(spx_plan* p, const void* R, void* X, int nplanes, bool lo) {
 template <typename T>
 static int xc2r_t(spx_plan* p, const void* X, void* R, int nplanes, int dirx,
                   const LineFFTHost& f) {
-  const long npairs = long(p->g.ny) * nplanes / 2;
+  const long npairs = long(p->g.ny) * nplanes * p->mem / 2;
   dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
-  ProfScope ps_(p, "xc2r", double(p->g.ny) * nplanes *
+  ProfScope ps_(p, "xc2r", double(p->g.ny) * nplanes * p->mem *
                                (double(sizeof(T)) * p->g.nx + double(sizeof(cpx<T>)) * p->nkx));
   SPX_LAUNCH_RS(f, k_xc2r, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
                 C_<T>(X), static_cast<T*>(R), f.dev<cpx<T>>(), p->nkx, npairs, dirx, p->d_kx);
@@ -1097,10 +1111,10 @@ template <typename T>
 static int xgrad_prod_t(spx_plan* p, const void* YV, const void* YY, const void* XZ,
                         const void* u, const void* v, const void* w, void* out,
                         const LineFFTHost& f) {
-  const long npairs = long(p->g.ny) * p->np / 2;
+  const long npairs = long(p->g.ny) * p->np * p->mem / 2;
   dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
   const double t = double(p->g.nx) * double(p->g.ny) * double(p->g.nz);
-  const double rows = double(p->g.ny) * p->np;
+  const double rows = double(p->g.ny) * p->np * p->mem;
   ProfScope ps_(p, "xgrad_prod", rows * (3.0 * sizeof(cpx<T>) * p->nkx +
                                          4.0 * sizeof(T) * p->g.nx));
   SPX_LAUNCH_RS(f, k_xgrad_prod, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
@@ -1121,9 +1135,9 @@ int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, cons
 static int noslip_launch(spx_plan* p, void* vx, void* vy, const void* pr, const WallParams& wp,
                          int ncomp, const char* name) {
   const LineFFTHost& f = p->fz;
-  const long nlines = long(p->g.ny) * p->nkx;
+  const long nlines = p->wlines();
   dim3 grid(unsigned((nlines + f.nl - 1) / f.nl), unsigned(ncomp));
-  ProfScope ps_(p, name, 2.0 * ncomp * double(p->nc) * 16.0);
+  ProfScope ps_(p, name, 2.0 * ncomp * double(p->wnc()) * 16.0);
   SPX_LAUNCH_RS(f, k_noslip, double, , grid, blk(f), f.lds_bytes(16), p->stream, C_<double>(vx),
                 C_<double>(vy), C_<double>(pr), f.dev<Cd>(), nlines, p->g.ny, p->np,
                 1.0 / double(p->g.nz), wp, p->d_kx, p->d_ky, p->d_dir, p->g.Cz, p->g.oz);
@@ -1153,8 +1167,8 @@ int sconst_fused(spx_plan* p, void* th) {
 
 int proj_bc(spx_plan* p, const void* c) {
   const LineFFTHost& f = p->fz;
-  const long nlines = long(p->g.ny) * p->nkx;
-  ProfScope ps_(p, "proj_bc", double(p->nc) * 16.0);
+  const long nlines = p->wlines();
+  ProfScope ps_(p, "proj_bc", double(p->wnc()) * 16.0);
   SPX_LAUNCH_RS(f, k_proj_bc, double, , dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
                 f.lds_bytes(16), p->stream, C_<double>(c), C_<double>(p->bc), f.dev<Cd>(), nlines,
                 p->np, 1.0 / double(p->g.nz));
@@ -1164,8 +1178,8 @@ int proj_bc(spx_plan* p, const void* c) {
 
 int proj_pressure(spx_plan* p, void* d) {
   const LineFFTHost& f = p->fz;
-  const long nlines = long(p->g.ny) * p->nkx;
-  ProfScope ps_(p, "proj_pressure", 2.0 * double(p->nc) * 16.0);
+  const long nlines = p->wlines();
+  ProfScope ps_(p, "proj_pressure", 2.0 * double(p->wnc()) * 16.0);
   const int rs = rset_of(f);
   const dim3 grid(unsigned((nlines + f.nl - 1) / f.nl));
   const size_t lds = f.lds_bytes(16);
@@ -1187,9 +1201,9 @@ int proj_pressure(spx_plan* p, void* d) {
 
 int proj_harmonic(spx_plan* p, void* a, void* b, void* c) {
   const LineFFTHost& f = p->fz2;
-  const long nlines = long(p->g.ny) * p->nkx;
+  const long nlines = p->wlines();
   const size_t lds = f.lds_bytes(16, 2);
-  ProfScope ps_(p, "proj_harmonic", 6.0 * double(p->nc) * 16.0);
+  ProfScope ps_(p, "proj_harmonic", 6.0 * double(p->wnc()) * 16.0);
   SPX_LAUNCH_RS(f, k_proj_harmonic, double, , dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
                 lds, p->stream, C_<double>(a), C_<double>(b), C_<double>(c), C_<double>(p->bc),
                 f.dev<Cd>(), nlines, p->g.ny, p->np, p->g.Lz, p->d_khom, p->d_z, p->d_kx,
@@ -1200,7 +1214,7 @@ int proj_harmonic(spx_plan* p, void* a, void* b, void* c) {
 
 static int zline_launch(spx_plan* p, const LinComb& lc, const void* b, int mode, double* out) {
   const LineFFTHost& f = mode == 1 ? p->fz2 : p->fz;
-  const long nlines = long(p->g.ny) * p->nkx;
+  const long nlines = p->wlines();
   LinCombDev<double> d;
   for (int t = 0; t < 3; ++t) {
     d.in[t] = static_cast<const Cd*>(lc.in[t]);
@@ -1208,7 +1222,7 @@ static int zline_launch(spx_plan* p, const LinComb& lc, const void* b, int mode,
     d.sgn[t] = lc.sgn[t];
   }
   const size_t lds = f.lds_bytes(16, mode == 1 ? 2 : 1) + size_t(f.nl) * f.tpl * sizeof(double);
-  ProfScope ps_(p, "zline_reduce", double(p->nc) * 16.0 * (mode == 1 ? 2 : 1));
+  ProfScope ps_(p, "zline_reduce", double(p->wnc()) * 16.0 * (mode == 1 ? 2 : 1));
   SPX_LAUNCH_RS(f, k_zline_reduce, double, , dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
                 lds, p->stream, d, C_<double>(b), mode, out, f.dev<Cd>(), nlines, p->g.ny, p->np,
                 p->d_kx, p->d_ky, p->d_kz);

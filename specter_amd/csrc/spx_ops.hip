@@ -104,7 +104,7 @@ struct RkComp {
   const Cp* f;
   const Cp* src;        // may be null
   double src_coef;
-  double nu;
+  const double* nu;     // diffusivity per (member, kx) line group
 };
 template <typename TN>
 struct RkArgs {
@@ -135,7 +135,7 @@ __global__ void k_rk_update(RkArgs<TN> args, int ncomp, long nc, SIdx ix,
     if (c.src) nl = csub(nl, cscale(ld(c.src, e), c.src_coef));
     nl = cscale(cscale(cscale(nl, fx[i]), fy[j]), fz[k]);
     const Cp lap = cscale(ld(c.v, e), -kk2);
-    const Cp t = cadd(csub(cscale(lap, c.nu), nl), ld(c.f, e));
+    const Cp t = cadd(csub(cscale(lap, c.nu[i]), nl), ld(c.f, e));
     out[q] = cadd(ld(c.c0, e), cscale(cscale(t, dt), rmp));
   }
 #pragma unroll
@@ -165,7 +165,7 @@ __global__ void k_poisson_project(cpx<TS>* __restrict__ a, cpx<TS>* __restrict__
   Cp s = cadd(cscale(A, kxi), cscale(B, kyj));
   s = cadd(s, cscale(Cc, kzk));
   Cp dd;
-  if (e == 0) {
+  if (kk2 == 0.0) {  // the (0,0,0) mode
     dd = mkc<double>(0, 0);
   } else {
     dd = mkc<double>(s.y / kk2, -s.x / kk2);
@@ -208,25 +208,25 @@ template <typename TS>
 static inline const cpx<TS>* CS(const void* a) { return static_cast<const cpx<TS>*>(a); }
 
 int op_derivk(spx_plan* p, const void* a, void* b, int dirn) {
-  ProfScope ps_(p, "derivk", 2.0 * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_derivk<double>, dim3(nblk(p->nc)), dim3(256), 0, p->stream, CS<double>(a),
-                     CS<double>(b), long(p->nc), sidx(p), dirn, p->d_kx, p->d_ky, p->d_kz);
+  ProfScope ps_(p, "derivk", 2.0 * double(p->wnc()) * 16.0);
+  hipLaunchKernelGGL(k_derivk<double>, dim3(nblk(long(p->wnc()))), dim3(256), 0, p->stream, CS<double>(a),
+                     CS<double>(b), long(p->wnc()), sidx(p), dirn, p->d_kx, p->d_ky, p->d_kz);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
 int op_laplak(spx_plan* p, const void* a, void* b) {
-  ProfScope ps_(p, "laplak", 2.0 * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_laplak<double>, dim3(nblk(p->nc)), dim3(256), 0, p->stream, CS<double>(a),
-                     CS<double>(b), long(p->nc), sidx(p), p->d_kx, p->d_ky, p->d_kz);
+  ProfScope ps_(p, "laplak", 2.0 * double(p->wnc()) * 16.0);
+  hipLaunchKernelGGL(k_laplak<double>, dim3(nblk(long(p->wnc()))), dim3(256), 0, p->stream, CS<double>(a),
+                     CS<double>(b), long(p->wnc()), sidx(p), p->d_kx, p->d_ky, p->d_kz);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
 int op_filter(spx_plan* p, void* a) {
-  ProfScope ps_(p, "filter", 2.0 * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_filter<double>, dim3(nblk(p->nc)), dim3(256), 0, p->stream, CS<double>(a),
-                     long(p->nc), sidx(p), p->d_fkx, p->d_fky, p->d_fkz);
+  ProfScope ps_(p, "filter", 2.0 * double(p->wnc()) * 16.0);
+  hipLaunchKernelGGL(k_filter<double>, dim3(nblk(long(p->wnc()))), dim3(256), 0, p->stream, CS<double>(a),
+                     long(p->wnc()), sidx(p), p->d_fkx, p->d_fky, p->d_fkz);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
@@ -240,7 +240,7 @@ int op_scale(spx_plan* p, void* a, size_t n, double s) {
 }
 
 int op_real_div3(spx_plan* p, void* r, double a, double b, double c) {
-  const long n = long(p->g.nx) * p->g.ny * p->np;
+  const long n = long(p->g.nx) * p->g.ny * p->np * p->mem;
   ProfScope ps_(p, "real_div3", 16.0 * double(n));
   hipLaunchKernelGGL(k_div3<double>, dim3(nblk(n)), dim3(256), 0, p->stream,
                      static_cast<double*>(r), n, a, b, c);
@@ -251,7 +251,7 @@ int op_real_div3(spx_plan* p, void* r, double a, double b, double c) {
 template <typename TN>
 static int rk_launch(spx_plan* p, int ncomp, void* const* v, const void* const* c0,
                      void* const* nl, const void* const* f, const void* const* src,
-                     const double* src_coef, const double* nu, double dt, double rmp) {
+                     const double* src_coef, const double* const* nu, double dt, double rmp) {
   RkArgs<TN> args;
   for (int c = 0; c < 4; ++c) {
     RkComp<TN>& r = args.c[c];
@@ -264,13 +264,13 @@ static int rk_launch(spx_plan* p, int ncomp, void* const* v, const void* const* 
       r.src_coef = src_coef ? src_coef[c] : 0.0;
       r.nu = nu[c];
     } else {
-      r = RkComp<TN>{nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, 0.0};
+      r = RkComp<TN>{nullptr, nullptr, nullptr, nullptr, nullptr, 0.0, nullptr};
     }
   }
   ProfScope ps_(p, "rk_update",
-                double(p->nc) * ncomp * (4.0 * 16.0 + sizeof(cpx<TN>) + (src ? 16.0 : 0.0)));
-  hipLaunchKernelGGL(k_rk_update<TN>, dim3(nblk(p->nc)), dim3(256), 0, p->stream, args, ncomp,
-                     long(p->nc), sidx(p), p->d_kx, p->d_ky, p->d_kz, p->d_fkx, p->d_fky,
+                double(p->wnc()) * ncomp * (4.0 * 16.0 + sizeof(cpx<TN>) + (src ? 16.0 : 0.0)));
+  hipLaunchKernelGGL(k_rk_update<TN>, dim3(nblk(long(p->wnc()))), dim3(256), 0, p->stream, args, ncomp,
+                     long(p->wnc()), sidx(p), p->d_kx, p->d_ky, p->d_kz, p->d_fkx, p->d_fky,
                      p->d_fkz, dt, rmp);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
@@ -278,15 +278,15 @@ static int rk_launch(spx_plan* p, int ncomp, void* const* v, const void* const* 
 
 int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, void* const* nl,
                  const void* const* f, const void* const* src, const double* src_coef,
-                 const double* nu, double dt, double rmp, bool nl_lo) {
+                 const double* const* nu, double dt, double rmp, bool nl_lo) {
   return nl_lo ? rk_launch<float>(p, ncomp, v, c0, nl, f, src, src_coef, nu, dt, rmp)
                : rk_launch<double>(p, ncomp, v, c0, nl, f, src, src_coef, nu, dt, rmp);
 }
 
 int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d) {
-  ProfScope ps_(p, "poisson_project", 7.0 * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_poisson_project<double>, dim3(nblk(p->nc)), dim3(256), 0, p->stream,
-                     CS<double>(a), CS<double>(b), CS<double>(c), CS<double>(d), long(p->nc),
+  ProfScope ps_(p, "poisson_project", 7.0 * double(p->wnc()) * 16.0);
+  hipLaunchKernelGGL(k_poisson_project<double>, dim3(nblk(long(p->wnc()))), dim3(256), 0, p->stream,
+                     CS<double>(a), CS<double>(b), CS<double>(c), CS<double>(d), long(p->wnc()),
                      sidx(p), p->d_kx, p->d_ky, p->d_kz);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
@@ -298,17 +298,17 @@ int op_copy_fields(spx_plan* p, int n, const void* const* src, void* const* dst)
     args.src[q] = q < n ? CS<double>(src[q]) : nullptr;
     args.dst[q] = q < n ? CS<double>(dst[q]) : nullptr;
   }
-  ProfScope ps_(p, "copy", 2.0 * n * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_copy_fields, dim3(nblk(p->nc)), dim3(256), 0, p->stream, args, n,
-                     long(p->nc));
+  ProfScope ps_(p, "copy", 2.0 * n * double(p->wnc()) * 16.0);
+  hipLaunchKernelGGL(k_copy_fields, dim3(nblk(long(p->wnc()))), dim3(256), 0, p->stream, args, n,
+                     long(p->wnc()));
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
 int op_axpy_field(spx_plan* p, void* y, const void* x, double alpha) {
-  ProfScope ps_(p, "axpy", 3.0 * double(p->nc) * 16.0);
-  hipLaunchKernelGGL(k_axpy<double>, dim3(nblk(p->nc)), dim3(256), 0, p->stream, CS<double>(y),
-                     CS<double>(x), long(p->nc), alpha);
+  ProfScope ps_(p, "axpy", 3.0 * double(p->wnc()) * 16.0);
+  hipLaunchKernelGGL(k_axpy<double>, dim3(nblk(long(p->wnc()))), dim3(256), 0, p->stream, CS<double>(y),
+                     CS<double>(x), long(p->wnc()), alpha);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }

@@ -13,6 +13,10 @@ memory layout:
 In mixed-precision plans (``precision=SPX_FP32_MIXED``) the outputs of
 ``gradre`` and ``advect`` are complex64; every other array is float64.
 
+A batched plan (``members > 1``) holds independent runs of one grid: arrays
+get a leading member axis, calls process the ``active`` leading members, and
+the diagnostics return one value per active member.
+
 Every call goes through libspx; there is no host fallback.
 """
 from __future__ import annotations
@@ -37,7 +41,7 @@ class FCPlan:
 
     def __init__(self, nx: int, ny: int, nz: int, Cz: int, oz: int, Lx: float = 1.0,
                  Ly: float = 1.0, Lz: float = 1.0, tdir: str | None = None, stream=None,
-                 precision: int = _l.SPX_FP64):
+                 precision: int = _l.SPX_FP64, members: int = 1):
         lib = _l.load_library()
         self._lib = lib
         g = _l.SpxGrid(nx, ny, nz, Cz, oz, Lx, Ly, Lz, precision)
@@ -50,11 +54,13 @@ class FCPlan:
             stream = torch.cuda.current_stream().cuda_stream
         self.stream = stream
         ptr = c_void_p()
-        _l.check(lib.spx_plan_create(
+        _l.check(lib.spx_plan_create_batch(
             byref(g),
             A.ctypes.data_as(ctypes.POINTER(c_double)) if A is not None else None,
             Q.ctypes.data_as(ctypes.POINTER(c_double)) if Q is not None else None,
-            c_void_p(stream), byref(ptr)))
+            int(members), c_void_p(stream), byref(ptr)))
+        self.members = int(members)
+        self.active = int(members)
         self._p = ptr
         info = _l.SpxPlanInfo()
         _l.check(lib.spx_plan_get_info(ptr, byref(info)))
@@ -89,13 +95,32 @@ class FCPlan:
         _l.check(self._lib.spx_plan_sync(self._p))
 
     # -- array helpers ------------------------------------------------------------
+    def set_active(self, n: int):
+        """Process the n leading members in every later call."""
+        _l.check(self._lib.spx_plan_set_active(self._p, int(n)))
+        self.active = int(n)
+
     @property
-    def spec_shape(self):
+    def member_spec_shape(self):
         return (self.nkx, self.ny, self.nz)
 
     @property
-    def real_shape(self):
+    def member_real_shape(self):
         return (self.nz, self.ny, self.nx)
+
+    @property
+    def spec_shape(self):
+        s = self.member_spec_shape
+        return s if self.members == 1 else (self.members, *s)
+
+    @property
+    def real_shape(self):
+        s = self.member_real_shape
+        return s if self.members == 1 else (self.members, *s)
+
+    def _out(self, vals):
+        """scalar for single plans, one value per active member otherwise"""
+        return vals[0] if self.members == 1 else np.array(vals[: self.active])
 
     def empty_spectral(self):
         torch = _torch()
@@ -116,6 +141,7 @@ class FCPlan:
         return torch.complex64 if self.mixed else torch.complex128
 
     def empty_nonlinear(self):
+        """gradre / advect result array (all members)."""
         torch = _torch()
         return torch.empty(self.spec_shape, dtype=self.nonlinear_dtype, device="cuda")
 
@@ -220,30 +246,33 @@ class FCPlan:
         _l.check(self._lib.spx_s_imposebc(self._p, self._spec(th, "th")))
 
     # -- diagnostics -----------------------------------------------------------------------
-    def energy(self, a, b, c, kin: int) -> float:
-        out = c_double()
+    def _vals(self):
+        return (c_double * self.members)()
+
+    def energy(self, a, b, c, kin: int):
+        out = self._vals()
         _l.check(self._lib.spx_energy(self._p, self._spec(a, "a"), self._spec(b, "b"),
-                                      self._spec(c, "c"), kin, byref(out)))
-        return out.value
+                                      self._spec(c, "c"), kin, out))
+        return self._out(list(out))
 
-    def cross(self, a, b, c, d, e, f) -> float:
-        out = c_double()
+    def cross(self, a, b, c, d, e, f):
+        out = self._vals()
         _l.check(self._lib.spx_cross(self._p, *(self._spec(t, n) for t, n in
-                                                zip((a, b, c, d, e, f), "abcdef")), byref(out)))
-        return out.value
+                                                zip((a, b, c, d, e, f), "abcdef")), out))
+        return self._out(list(out))
 
-    def divergence(self, a, b, c) -> float:
-        out = c_double()
+    def divergence(self, a, b, c):
+        out = self._vals()
         _l.check(self._lib.spx_divergence(self._p, self._spec(a, "a"), self._spec(b, "b"),
-                                          self._spec(c, "c"), byref(out)))
-        return out.value
+                                          self._spec(c, "c"), out))
+        return self._out(list(out))
 
     def bouncheck_z(self, a, b=None):
-        bot, top = c_double(), c_double()
+        bot, top = self._vals(), self._vals()
         _l.check(self._lib.spx_bouncheck_z(self._p, self._spec(a, "a"),
                                            self._spec(b, "b") if b is not None else None,
-                                           byref(bot), byref(top)))
-        return bot.value, top.value
+                                           bot, top))
+        return self._out(list(bot)), self._out(list(top))
 
     # -- grid values (host copies of what the plan uses) --------------------------------
     def grid_z(self) -> np.ndarray:

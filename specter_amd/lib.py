@@ -72,6 +72,12 @@ _SIGS = {
     "spx_substep": [_P, _P, c_int],
     "spx_step": [_P, _P, c_int],
     "spx_state_set_graph": [_P, c_int],
+    "spx_plan_create_batch": [POINTER(SpxGrid), POINTER(c_double), POINTER(c_double), c_int, _P,
+                              POINTER(_P)],
+    "spx_plan_get_members": [_P, POINTER(c_int), POINTER(c_int)],
+    "spx_plan_set_active": [_P, c_int],
+    "spx_state_set_member": [_P, c_int, c_double, c_double],
+    "spx_state_copy_member": [_P, c_int, c_int],
     "spx_prof_enable": [_P, c_int],
     "spx_prof_resolve": [_P, POINTER(c_int)],
     "spx_prof_get": [_P, c_int, c_char_p, c_size_t, POINTER(ctypes.c_long), POINTER(c_double),

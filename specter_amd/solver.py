@@ -65,6 +65,9 @@ class Solver:
         if getattr(self, "_s", None):
             self._lib.spx_state_destroy(self._s)
             self._s = None
+        if getattr(self, "_single", None) is not None:
+            self._single.close()
+            self._single = None
 
     def __del__(self):
         try:
@@ -73,19 +76,40 @@ class Solver:
             pass
 
     def field(self, name: str):
-        """Zero-copy torch view (complex128, (nkx, ny, nz)) of a state field."""
+        """Zero-copy torch view (complex128, (nkx, ny, nz), with a leading
+        member axis for batched plans) of a state field."""
         import torch
         ptr = c_void_p()
         _l.check(self._lib.spx_state_field(self._s, name.encode(), byref(ptr)))
         return torch.as_tensor(_DevView(ptr.value, self.plan.spec_shape, "<c16"), device="cuda")
 
+    def _member_plan(self) -> FCPlan:
+        """A single plan of the same grid on the same stream, for the I/O of
+        one member of a batched state."""
+        if getattr(self, "_single", None) is None:
+            p = self.plan
+            self._single = FCPlan(p.nx, p.ny, p.nz, p.Cz, p.oz, p.Lx, p.Ly, p.Lz,
+                                  stream=p.stream, precision=p.precision)
+        return self._single
+
     # -- set-up -------------------------------------------------------------------
-    def set_real_field(self, name: str, real: np.ndarray):
-        """field <- fftp3d_real_to_complex(real) (restart path, specter.fpp:894-899)."""
+    def set_real_field(self, name: str, real: np.ndarray, member: int | None = None):
+        """field <- fftp3d_real_to_complex(real) (restart path, specter.fpp:894-899).
+        For batched plans, member selects the one run to set."""
         import torch
         r = torch.as_tensor(np.ascontiguousarray(real, dtype=np.float64)).cuda()
-        self.plan.fftp3d_real_to_complex(r, self.field(name))
+        if member is None:
+            self.plan.fftp3d_real_to_complex(r, self.field(name))
+        else:
+            self._member_plan().fftp3d_real_to_complex(r, self.field(name)[member])
         self.plan.sync()
+
+    def set_member(self, member: int, nu: float, kappa: float = 0.0):
+        """Diffusivities of one member of a batched state."""
+        _l.check(self._lib.spx_state_set_member(self._s, member, nu, kappa))
+
+    def copy_member(self, dst: int, src: int):
+        _l.check(self._lib.spx_state_copy_member(self._s, dst, src))
 
     # -- stepping --------------------------------------------------------------------
     def rkstep1(self):
@@ -118,17 +142,23 @@ class Solver:
         nb, nt = p.bouncheck_z(vz)
         return dv, tb, tt, nb, nt
 
-    def real_field(self, name: str) -> np.ndarray:
+    def real_field(self, name: str, member: int | None = None) -> np.ndarray:
         """Physical-space field (c2r / (nx ny nz)), physical planes only."""
-        import torch
         p = self.plan
-        tmp = self.field(name).clone()
+        src = self.field(name)
+        if member is not None:
+            src, p = src[member], self._member_plan()
+        elif p.members > 1:
+            raise ValueError("real_field of a batched state needs a member")
+        tmp = src.clone()
         out = p.empty_real()
         p.fftp3d_complex_to_real(tmp, out)
         p.sync()
         return (out[: p.np] / (p.nx * p.ny * p.nz)).cpu().numpy()
 
-    def spectral(self, name: str) -> np.ndarray:
+    def spectral(self, name: str, member: int | None = None) -> np.ndarray:
         t = self.field(name)
+        if member is not None:
+            t = t[member]
         self.plan.sync()
         return t.cpu().numpy()

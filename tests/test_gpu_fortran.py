@@ -88,6 +88,10 @@ def test_fortran_driver_matches_python_and_oracle(tmp_path):
     # output fields (physical space, v/(nx ny nz))
     for name in ("vx", "vy", "vz"):
         out = np.fromfile(tmp_path / "out" / f"{name}.0002.out").reshape(npz, ny, nx)
-        assert np.array_equal(out, sol.real_field(name)), name
+        py = sol.real_field(name)
+        diff = np.abs(out - py)
+        print(f"{name}: driver vs Python max |diff| {diff.max():.3e} at "
+              f"{np.unravel_index(diff.argmax(), diff.shape)}, rel L2 {rel_l2(out, py):.3e}")
+        assert np.array_equal(out, py), name
         ref = O.fftp3d_complex_to_real(g, getattr(st, name).copy())[:npz] / (nx * ny * nz)
         assert rel_l2(out, ref) < 1e-11, name
