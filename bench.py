@@ -168,6 +168,66 @@ def _field_error(w, u, f, nsteps, plans):
     return out
 
 
+def _ensemble(nruns, max_slots, chunk):
+    """The ensemble workload (Poisson(4/s) arrivals of HD RK2 runs, 300 steps,
+    grids {32,64,128,192}^3, d in {5,8}, nu log-uniform [5e-4,5e-3], float32
+    mixed) through the continuous-batching server, against the same stream
+    served one run at a time in float64 (FIFO, service times measured per
+    grid class on the GPU)."""
+    import numpy as np
+    import torch
+    from specter_amd.ensemble import EnsembleServer, latency_stats, poisson_stream
+    from specter_amd import FCPlan, Solver, SolverParams, ic
+    from specter_amd.lib import SPX_FP64
+    reqs = poisson_stream(nruns=nruns)
+    work = sum(r.points * r.nsteps * r.ord for r in reqs)
+    srv = EnsembleServer(max_slots=max_slots, chunk=chunk)
+    t0 = time.perf_counter()
+    res = srv.run(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    st = latency_stats(res)
+    # float64 service time per step of each grid class, one run alone
+    per_step = {}
+    for r in reqs:
+        key = r.grid
+        if key in per_step:
+            continue
+        nx, ny, nz, C, d = key
+        plan = FCPlan(nx, ny, nz, C, d, precision=SPX_FP64)
+        sol = Solver(plan, SolverParams(solver="HD", ord=r.ord, dt=r.dt, nu=r.nu))
+        for name, arr in zip(("vx", "vy", "vz"), ic.spectrum_peaked_velocity(nx, ny, nz, C, 1)):
+            sol.set_real_field(name, arr)
+        sol.step(3)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sol.step(20)
+        torch.cuda.synchronize()
+        per_step[key] = (time.perf_counter() - t1) / 20
+        sol.close()
+        plan.close()
+    # FIFO single-run service of the same arrival stream
+    clock, lat = 0.0, []
+    for r in sorted(reqs, key=lambda q: q.arrival):
+        clock = max(clock, r.arrival) + per_step[r.grid] * r.nsteps
+        lat.append(clock - r.arrival)
+    seq_busy = sum(per_step[r.grid] * r.nsteps for r in reqs)
+    lat = np.array(lat)
+    return {
+        "runs": nruns, "workload": "Poisson 4/s, 300 RK2 steps, {32,64,128,192}^3, d in {5,8}, "
+                                   "nu log-U[5e-4,5e-3], FP32-mixed, continuous batching",
+        "p50_s": round(st["p50_s"], 4), "p99_s": round(st["p99_s"], 4),
+        "makespan_s": round(wall, 3),
+        "aggregate_Mpts_per_s": round(work / wall / 1e6, 2),
+        "peak_classes": srv.peak_classes, "peak_hbm_GB": round(srv.peak_used / 1e9, 2),
+        "sequential_fp64": {"p50_s": round(float(np.percentile(lat, 50)), 4),
+                            "p99_s": round(float(np.percentile(lat, 99)), 4),
+                            "makespan_s": round(clock, 3),
+                            "aggregate_Mpts_per_s": round(work / clock / 1e6, 2),
+                            "busy_s": round(seq_busy, 3)},
+        "throughput_vs_sequential_fp64": round(clock / wall, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +240,10 @@ def main():
     ap.add_argument("--cpu-substeps", type=int, default=1,
                     help="RK stages of the CPU baseline sample (0 = skip)")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel pass")
+    ap.add_argument("--ensemble-runs", type=int, default=96,
+                    help="requests of the ensemble-serving measurement (0 = skip)")
+    ap.add_argument("--ensemble-slots", type=int, default=16)
+    ap.add_argument("--ensemble-chunk", type=int, default=25)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the "
@@ -242,6 +306,9 @@ def main():
         if "fp64" in res and head == "mixed":
             line["fp64"]["roofline"] = _roofline(res["fp64"]["rows"])
             line["fp64"]["kernels"] = res["fp64"]["kernels"]
+    if args.ensemble_runs > 0:
+        line["ensemble"] = _ensemble(args.ensemble_runs, args.ensemble_slots,
+                                     args.ensemble_chunk)
     if args.cpu_substeps > 0:
         t, cores = _cpu_baseline(w, u, f, args.cpu_substeps)
         line["cpu_baseline"] = {"value": round(npts / t / 1e6, 4), "unit": "Mpts/s",

@@ -86,16 +86,37 @@ static int ensure_work(spx_plan* p, bool bouss) {
       return rc;
     p->work_bouss = true;
   }
+  if (p->fp32 && p->g.Cz > 0 && !p->XVb) {
+    const size_t d2 = 2 * size_t(p->g.oz), nbm = size_t(p->nb);
+    const size_t bc = size_t(p->nkx) * p->g.ny * d2 * nbm * 16;   // one component
+    const size_t br = size_t(p->g.nx) * p->g.ny * d2 * nbm * 8;
+    if ((rc = alloc_dev(p, &p->XVb, 4 * bc)) || (rc = alloc_dev(p, &p->XZb, 4 * bc)) ||
+        (rc = alloc_dev(p, &p->XYb, 4 * bc)) || (rc = alloc_dev(p, &p->RVb, 3 * br)) ||
+        (rc = alloc_dev(p, &p->ROb, 4 * br)) || (rc = alloc_dev(p, &p->XFb, 4 * bc)))
+      return rc;
+  }
   return SPX_OK;
 }
 
+// complex / real boundary-plane block of component c in the current view
+static inline void* bnd_c(const spx_plan* p, void* base, int c) {
+  return static_cast<char*>(base) +
+         size_t(c) * p->mem * 2 * p->g.oz * size_t(p->nkx) * p->g.ny * 16;
+}
+static inline void* bnd_r(const spx_plan* p, void* base, int c) {
+  return static_cast<char*>(base) +
+         size_t(c) * p->mem * 2 * p->g.oz * size_t(p->g.nx) * p->g.ny * 8;
+}
+
 // 3D transforms on internal buffers -------------------------------------------
-// lo: float32 buffers and kernels of a mixed plan (X holds float32 data then)
-static int r2c3(spx_plan* p, const void* in, void* out, bool lo = false) {
+// lo: float32 buffers and kernels of a mixed plan (X holds float32 data then);
+// xb: float64 boundary planes for the continuation (mixed plans)
+static int r2c3(spx_plan* p, const void* in, void* out, bool lo = false,
+                const void* xb = nullptr) {
   int rc;
   if ((rc = xr2c(p, in, p->X, p->np, lo))) return rc;
   if ((rc = yfft(p, p->X, 1, p->np, nullptr, nullptr, lo))) return rc;
-  return zfft_x2s(p, p->X, out, 0, lo);
+  return zfft_x2s(p, p->X, out, 0, lo, xb);
 }
 
 static int c2r3(spx_plan* p, const void* in, void* out, int dirn, int nplanes) {
@@ -287,6 +308,22 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
     if ((rc = build_line_fft(p->fx32, nx, 8, 0))) return bail(rc);
     if ((rc = build_line_fft(p->fy32, ny, 8, 16))) return bail(rc);
     if ((rc = build_line_fft(p->fz32, nz, 8, 16))) return bail(rc);
+    if (C > 0) {
+      if (d > 8) return bail(fail(SPX_ERR_UNSUPPORTED, "mixed plans: matching order above 8"));
+      // w[q][k] = exp(+2 pi i k z_q / nz) at the boundary planes z_q
+      std::vector<double> w(size_t(2 * d) * nz * 2);
+      const long double TWO_PI = 6.283185307179586476925286766559005768L;
+      for (int q = 0; q < 2 * d; ++q) {
+        const long z = q < d ? q : long(p->np) - 2 * d + q;
+        for (int k = 0; k < nz; ++k) {
+          const long r = (long(k) * z) % nz;
+          const long double a = TWO_PI * (long double)r / (long double)nz;
+          w[2 * (size_t(q) * nz + k)] = double(cosl(a));
+          w[2 * (size_t(q) * nz + k) + 1] = double(sinl(a));
+        }
+      }
+      if ((rc = upload(p, reinterpret_cast<double**>(&p->d_wbnd), w))) return bail(rc);
+    }
   }
   // scratch
   const size_t nlines = size_t(ny) * p->nkx * nmembers;
@@ -460,8 +497,25 @@ static int gradre_impl(spx_plan* p, const void* const v[3], void* const out[3]) 
     if ((rc = xgrad_prod(p, p->XV[c], p->XY[c], p->XZ[c], p->RV[0], p->RV[1], p->RV[2],
                          p->RO[c], lo)))
       return rc;
+  const bool bnd = lo && p->g.Cz > 0;
+  const int d2 = 2 * p->g.oz;
+  if (bnd) {
+    // float64 boundary planes: z IFFT at the 2d planes, y and x passes and
+    // products there, forward x, y passes (three components per launch)
+    if ((rc = zfft_s2x_bnd(p, v, 3, p->XVb, p->XZb))) return rc;
+    if ((rc = yfft(p, p->XVb, 0, 3 * d2, nullptr, p->XYb))) return rc;
+    if ((rc = yfft(p, p->XZb, 0, 3 * d2))) return rc;
+    if ((rc = xc2r(p, p->XVb, p->RVb, 3 * d2, 0))) return rc;
+    for (int c = 0; c < 3; ++c)
+      if ((rc = xgrad_prod(p, bnd_c(p, p->XVb, c), bnd_c(p, p->XYb, c), bnd_c(p, p->XZb, c),
+                           bnd_r(p, p->RVb, 0), bnd_r(p, p->RVb, 1), bnd_r(p, p->RVb, 2),
+                           bnd_r(p, p->ROb, c), false, d2)))
+        return rc;
+    if ((rc = xr2c(p, p->ROb, p->XFb, 3 * d2))) return rc;
+    if ((rc = yfft(p, p->XFb, 1, 3 * d2))) return rc;
+  }
   for (int c = 0; c < 3; ++c)
-    if ((rc = r2c3(p, p->RO[c], out[c], lo))) return rc;
+    if ((rc = r2c3(p, p->RO[c], out[c], lo, bnd ? bnd_c(p, p->XFb, c) : nullptr))) return rc;
   return SPX_OK;
 }
 
@@ -472,12 +526,34 @@ static int advect_impl(spx_plan* p, const void* const v[3], const void* th, void
   int rc;
   const int np = p->np;
   const bool lo = p->fp32;
+  const bool bnd = lo && p->g.Cz > 0;
+  const int d2 = 2 * p->g.oz;
   if (!have_rv) {
     for (int c = 0; c < 3; ++c) {
       if ((rc = zfft_s2x(p, v[c], p->XV[c], 0, np, 0, nullptr, lo))) return rc;
       if ((rc = yfft(p, p->XV[c], 0, np, nullptr, nullptr, lo))) return rc;
       if ((rc = xc2r(p, p->XV[c], p->RV[c], np, 0, lo))) return rc;
     }
+    if (bnd) {
+      if ((rc = zfft_s2x_bnd(p, v, 3, p->XVb, p->XZb))) return rc;
+      if ((rc = yfft(p, p->XVb, 0, 3 * d2))) return rc;
+      if ((rc = xc2r(p, p->XVb, p->RVb, 3 * d2, 0))) return rc;
+    }
+  }
+  if (bnd) {
+    // the scalar's float64 boundary planes (component slot 3)
+    const void* ths[1] = {th};
+    void* xv3 = bnd_c(p, p->XVb, 3);
+    void* xz3 = bnd_c(p, p->XZb, 3);
+    void* xy3 = bnd_c(p, p->XYb, 3);
+    if ((rc = zfft_s2x_bnd(p, ths, 1, xv3, xz3))) return rc;
+    if ((rc = yfft(p, xv3, 0, d2, nullptr, xy3))) return rc;
+    if ((rc = yfft(p, xz3, 0, d2))) return rc;
+    if ((rc = xgrad_prod(p, xv3, xy3, xz3, bnd_r(p, p->RVb, 0), bnd_r(p, p->RVb, 1),
+                         bnd_r(p, p->RVb, 2), bnd_r(p, p->ROb, 3), false, d2)))
+      return rc;
+    if ((rc = xr2c(p, bnd_r(p, p->ROb, 3), bnd_c(p, p->XFb, 3), d2))) return rc;
+    if ((rc = yfft(p, bnd_c(p, p->XFb, 3), 1, d2))) return rc;
   }
   if ((rc = zfft_s2x(p, th, p->XV[3], 0, np, 0, p->XZ[3], lo))) return rc;
   if ((rc = yfft(p, p->XV[3], 0, np, nullptr, p->XY[3], lo))) return rc;
@@ -485,7 +561,7 @@ static int advect_impl(spx_plan* p, const void* const v[3], const void* th, void
   if ((rc = xgrad_prod(p, p->XV[3], p->XY[3], p->XZ[3], p->RV[0], p->RV[1], p->RV[2],
                        p->RO[3], lo)))
     return rc;
-  return r2c3(p, p->RO[3], out, lo);
+  return r2c3(p, p->RO[3], out, lo, bnd ? bnd_c(p, p->XFb, 3) : nullptr);
 }
 
 }  // namespace spx

@@ -152,6 +152,15 @@ struct spx_plan {
   void* RV[3] = {nullptr};
   void* RO[4] = {nullptr};
   void* R0 = nullptr;       // general real scratch (Boussinesq round trip)
+  // mixed plans with walls: float64 boundary-plane buffers of the nonlinear
+  // pipeline (2d planes per component and member; see zfft_s2x_bnd)
+  void* XVb = nullptr;      // z IFFT of v_c, then its y IFFT (4 components)
+  void* XZb = nullptr;      // of d_z v_c
+  void* XYb = nullptr;      // of d_y v_c
+  void* RVb = nullptr;      // v_c in real space (3 components)
+  void* ROb = nullptr;      // products (4 components)
+  void* XFb = nullptr;      // forward x, y transforms of the products (4 components)
+  void* d_wbnd = nullptr;   // exp(+2 pi i k z_q / nz), 2d x nz float64 complex
   void* bc = nullptr;       // wall data of sol_project, 2 complex per (kx,ky) line
   double* part = nullptr;   // per-line partial sums of the diagnostics
   std::vector<double> h_part;
@@ -205,7 +214,13 @@ int zfft_inplace(spx_plan* p, void* S, int forward, int cont, double scale);
 int fc_continue_only(spx_plan* p, void* S);
 // forward 3D last pass: X (planes k < np) -> continuation -> z FFT -> S.
 // nofft = 1: transpose only (all nz planes), for fftp2d_real_to_complex_xy.
-int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft, bool lo = false);
+// Xb (mixed plans, float32 X): float64 boundary planes [member][q][ky][kx], q < 2d,
+// that replace X's planes z = 0..d-1, np-d..np-1 and feed the continuation.
+int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft, bool lo = false,
+             const void* Xb = nullptr);
+// float64 z IFFT of S (and of (i kz) S) at the 2d boundary planes of every
+// line, for ncomp float64 spectral inputs: XVb / XZb planes [comp][member][q][ky][kx].
+int zfft_s2x_bnd(spx_plan* p, const void* const* S, int ncomp, void* XVb, void* XZb);
 // backward 3D first pass: S * (i k_dir) (dir 0 = none) -> z IFFT -> X, planes < nzout.
 // X2 != nullptr: also X2 <- z IFFT of (i kz) S from the same load (dual).
 int zfft_s2x(spx_plan* p, const void* S, void* X, int dir, int nzout, int nofft,
@@ -220,7 +235,7 @@ int xc2r(spx_plan* p, const void* X, void* R, int nplanes, int dirx = 0, bool lo
 // gradient products of one component: out = ((u*Dx + v*Dy) + w*Dz)*tmp with
 // Dx = c2r(i kx YV), Dy = c2r(YY), Dz = c2r(XZ) (pseudospec_hd.f90:254-312 order)
 int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, const void* u,
-               const void* v, const void* w, void* out, bool lo = false);
+               const void* v, const void* w, void* out, bool lo = false, int nplanes = -1);
 // fused v_imposebc: goto_domain_w_boundaries + noslip_z + goto_3d_fourier on vx,vy
 int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
                  const double vzsta[2], const double vzend[2]);

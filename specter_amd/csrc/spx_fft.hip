@@ -103,11 +103,16 @@ __global__ void k_fc_continue(cpx<T>* __restrict__ S, long nlines, int n, int np
 // forward 3D last pass: X(kx,y,z) planes < np -> continuation -> z FFT -> S(z,y,kx)
 // blockIdx.z = member: S advances by n*ny*nkx, X by xstride elements per member.
 // ---------------------------------------------------------------------------
-template <typename T, int RS>
+// BND (mixed plans): the 2d boundary planes z = 0..d-1, np-d..np-1 come in
+// float64 from Xb (planes [member][q][ky][kx], q < 2d) and the continuation
+// is evaluated from them, so the float32 roundoff of the other planes is not
+// amplified by the continuation.
+template <typename T, int RS, bool BND>
 __global__ void __launch_bounds__(256) k_z_x2s(const cpx<T>* __restrict__ X, cpx<T>* __restrict__ S,
                                                LineFFT<cpx<T>> f, int nkx, int ny, int np,
                                                int nofft, const double* __restrict__ dir, int C,
-                                               int d, long xstride) {
+                                               int d, long xstride,
+                                               const cpx<double>* __restrict__ Xb) {
   using Cp = cpx<T>;
   SPX_PROLOGUE(Cp, f);
   const int n = f.n;
@@ -126,7 +131,42 @@ __global__ void __launch_bounds__(256) k_z_x2s(const cpx<T>* __restrict__ X, cpx
   }
   __syncthreads();
   if (!nofft) {
-    if (C > 0) {
+    if constexpr (BND) {
+#pragma clang fp contract(off)
+      const int d2 = 2 * d;
+      const long pb = long(nkx) * ny;
+      const cpx<double>* xb = Xb + blockIdx.z * (pb * d2) + long(nkx) * j;
+      for (int e = tid; e < f.nl * d2; e += blockDim.x) {
+        const int l = e / d2, q = e - l * d2, i = i0 + l;
+        const int z = q < d ? q : np - d2 + q;
+        const cpx<double> v = i < nkx ? xb[i + pb * q] : mkc<double>(0, 0);
+        lds[l * f.ldl + z] = mkc<T>(T(v.x), T(v.y));
+      }
+      // continuation (fftp.fpp:755-773 order): x[top+jj] is plane q = d+jj,
+      // x[d-1-jj] is plane q = d-1-jj
+      for (int e = tid; e < f.nl * C; e += blockDim.x) {
+        const int l = e / C, ii = e - l * C, i = i0 + l;
+        double ar = 0.0, ai = 0.0;
+        if (i < nkx) {
+          const cpx<double>* xt = xb + i + pb * d;
+          const cpx<double>* xo = xb + i;
+          const double* D0 = dir + ii * d;
+          const double* D1 = dir + (C - 1 - ii) * d;
+          const cpx<double> t0 = xt[0], b0 = xo[pb * (d - 1)];
+          ar = D0[0] * t0.x + D1[0] * b0.x;
+          ai = D0[0] * t0.y + D1[0] * b0.y;
+          for (int jj = 1; jj < d; ++jj) {
+            const cpx<double> ut = xt[pb * jj], ub = xo[pb * (d - 1 - jj)];
+            ar = ar + D0[jj] * ut.x;
+            ar = ar + D1[jj] * ub.x;
+            ai = ai + D0[jj] * ut.y;
+            ai = ai + D1[jj] * ub.y;
+          }
+        }
+        lds[l * f.ldl + np + ii] = mkc<T>(T(ar), T(ai));
+      }
+      __syncthreads();
+    } else if (C > 0) {
       fc_continue_lds(lds, f.nl, f.ldl, np, C, d, dir);
       __syncthreads();
     }
@@ -135,6 +175,59 @@ __global__ void __launch_bounds__(256) k_z_x2s(const cpx<T>* __restrict__ X, cpx
   const int i = i0 + line;
   if (i < nkx)
     for (int k = lane; k < n; k += f.tpl) S[k + long(n) * (j + long(ny) * i)] = xl[k];
+}
+
+// ---------------------------------------------------------------------------
+// float64 values of the backward z transform at the 2d boundary planes
+// z = 0..d-1 and np-d..np-1 (the points the FC continuation reads), for the
+// float32 nonlinear pipeline of mixed plans: a direct partial DFT of every
+// line, one wavefront per line, against w[q*n + k] = exp(+2 pi i k z_q / n).
+// blockIdx.y: 0 = v, 1 = (i kz) v; blockIdx.z = component.  Output planes
+// [comp][member][q][ky][kx] with `mem` members.
+// ---------------------------------------------------------------------------
+struct BndSrc {
+  const cpx<double>* s[4];
+};
+
+template <int D2>
+__global__ void __launch_bounds__(256) k_z_s2x_bnd(BndSrc src, cpx<double>* __restrict__ XVb,
+                                                   cpx<double>* __restrict__ XZb,
+                                                   const cpx<double>* __restrict__ w, int n,
+                                                   int nkx, int ny, int mem,
+                                                   const double* __restrict__ kz) {
+  const int lane = threadIdx.x & 63;
+  const long nlines = long(nkx) * ny;
+  const long ln = long(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (ln >= nlines * mem) return;  // uniform per wavefront
+  const int kind = blockIdx.y, comp = blockIdx.z;
+  const cpx<double>* S = src.s[comp] + ln * n;
+  double ar[D2], ai[D2];
+#pragma unroll
+  for (int q = 0; q < D2; ++q) ar[q] = ai[q] = 0.0;
+  for (int k = lane; k < n; k += 64) {
+    cpx<double> v = S[k];
+    if (kind) v = ikmul(kz[k], v);
+#pragma unroll
+    for (int q = 0; q < D2; ++q) {
+      const cpx<double> t = w[q * n + k];
+      ar[q] += v.x * t.x - v.y * t.y;
+      ai[q] += v.x * t.y + v.y * t.x;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < D2; ++q) {
+      ar[q] += __shfl_xor(ar[q], off);
+      ai[q] += __shfl_xor(ai[q], off);
+    }
+  }
+  const long b = ln / nlines, l = ln - b * nlines;
+  const long i = l / ny, jy = l - i * ny;
+  cpx<double>* out = (kind ? XZb : XVb) + ((long(comp) * mem + b) * D2) * nlines + jy * nkx + i;
+#pragma unroll
+  for (int q = 0; q < D2; ++q)
+    if (lane == q) out[q * nlines] = mkc<double>(ar[q], ai[q]);
 }
 
 // ---------------------------------------------------------------------------
@@ -995,22 +1088,65 @@ int fc_continue_only(spx_plan* p, void* S) {
 }
 
 template <typename T>
-static int zfft_x2s_t(spx_plan* p, const void* X, void* S, int nofft, const LineFFTHost& f) {
+static int zfft_x2s_t(spx_plan* p, const void* X, void* S, int nofft, const LineFFTHost& f,
+                      const void* Xb) {
   // members: grid.z; X holds np planes per member (all nz with nofft)
   dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(p->g.ny), unsigned(p->mem));
   const long xstride = long(p->nkx) * p->g.ny * (nofft ? p->g.nz : p->np);
   ProfScope ps_(p, "z_x2s", double(p->wkx()) * p->g.ny * sizeof(cpx<T>) *
                                 ((nofft ? p->g.nz : p->np) + p->g.nz));
-  SPX_LAUNCH_RS(f, k_z_x2s, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
-                C_<T>(X), C_<T>(S), f.dev<cpx<T>>(), p->nkx, p->g.ny, p->np, nofft, p->d_dir,
-                p->g.Cz, p->g.oz, xstride);
+  if (Xb) {
+    SPX_LAUNCH_RS(f, k_z_x2s, T, SPX_COMMA true, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
+                  p->stream, C_<T>(X), C_<T>(S), f.dev<cpx<T>>(), p->nkx, p->g.ny, p->np, nofft,
+                  p->d_dir, p->g.Cz, p->g.oz, xstride, C_<double>(Xb));
+    SPX_CHECK_LAUNCH();
+    return SPX_OK;
+  }
+  SPX_LAUNCH_RS(f, k_z_x2s, T, SPX_COMMA false, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
+                p->stream, C_<T>(X), C_<T>(S), f.dev<cpx<T>>(), p->nkx, p->g.ny, p->np, nofft, p->d_dir,
+                p->g.Cz, p->g.oz, xstride, static_cast<const Cd*>(nullptr));
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
-int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft, bool lo) {
+int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft, bool lo, const void* Xb) {
   if (int rc = check_lo(p, lo)) return rc;
-  return lo ? zfft_x2s_t<float>(p, X, S, nofft, p->fz32)
-            : zfft_x2s_t<double>(p, X, S, nofft, p->fz);
+  if (Xb && (nofft || p->g.Cz <= 0)) return fail(SPX_ERR_ARG, "boundary planes need a continuation");
+  return lo ? zfft_x2s_t<float>(p, X, S, nofft, p->fz32, Xb)
+            : zfft_x2s_t<double>(p, X, S, nofft, p->fz, Xb);
+}
+
+int zfft_s2x_bnd(spx_plan* p, const void* const* S, int ncomp, void* XVb, void* XZb) {
+  if (!p->d_wbnd) return fail(SPX_ERR_STATE, "plan has no boundary-plane table");
+  if (ncomp < 1 || ncomp > 4) return fail(SPX_ERR_ARG, "zfft_s2x_bnd: 1..4 components");
+  BndSrc src;
+  for (int c = 0; c < 4; ++c) src.s[c] = c < ncomp ? C_<double>(S[c]) : nullptr;
+  const long lines = p->wlines();
+  const dim3 grid(unsigned((lines + 3) / 4), 2u, unsigned(ncomp));
+  ProfScope ps_(p, "z_s2x_bnd", double(ncomp) * 2.0 * double(p->wnc()) * 16.0);
+  const Cd* w = static_cast<const Cd*>(p->d_wbnd);
+  Cd* v = C_<double>(XVb);
+  Cd* z = C_<double>(XZb);
+  const int n = p->g.nz, nkx = p->nkx, ny = p->g.ny, mem = p->mem;
+  switch (p->g.oz) {
+#define SPX_BND_CASE(D)                                                                          \
+  case D:                                                                                        \
+    hipLaunchKernelGGL(k_z_s2x_bnd<2 * D>, grid, dim3(256), 0, p->stream, src, v, z, w, n, nkx, \
+                       ny, mem, p->d_kz);                                                        \
+    break;
+    SPX_BND_CASE(1)
+    SPX_BND_CASE(2)
+    SPX_BND_CASE(3)
+    SPX_BND_CASE(4)
+    SPX_BND_CASE(5)
+    SPX_BND_CASE(6)
+    SPX_BND_CASE(7)
+    SPX_BND_CASE(8)
+#undef SPX_BND_CASE
+    default:
+      return fail(SPX_ERR_UNSUPPORTED, "boundary planes: matching order above 8");
+  }
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
 }
 
 // S is float64 in both precisions; X (and X2) are T
@@ -1110,11 +1246,11 @@ int xc2r(spx_plan* p, const void* X, void* R, int nplanes, int dirx, bool lo) {
 template <typename T>
 static int xgrad_prod_t(spx_plan* p, const void* YV, const void* YY, const void* XZ,
                         const void* u, const void* v, const void* w, void* out,
-                        const LineFFTHost& f) {
-  const long npairs = long(p->g.ny) * p->np * p->mem / 2;
+                        const LineFFTHost& f, int nplanes) {
+  const long npairs = long(p->g.ny) * nplanes * p->mem / 2;
   dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
   const double t = double(p->g.nx) * double(p->g.ny) * double(p->g.nz);
-  const double rows = double(p->g.ny) * p->np * p->mem;
+  const double rows = double(p->g.ny) * nplanes * p->mem;
   ProfScope ps_(p, "xgrad_prod", rows * (3.0 * sizeof(cpx<T>) * p->nkx +
                                          4.0 * sizeof(T) * p->g.nx));
   SPX_LAUNCH_RS(f, k_xgrad_prod, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
@@ -1125,10 +1261,11 @@ static int xgrad_prod_t(spx_plan* p, const void* YV, const void* YY, const void*
   return SPX_OK;
 }
 int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, const void* u,
-               const void* v, const void* w, void* out, bool lo) {
+               const void* v, const void* w, void* out, bool lo, int nplanes) {
   if (int rc = check_lo(p, lo)) return rc;
-  return lo ? xgrad_prod_t<float>(p, YV, YY, XZ, u, v, w, out, p->fx32)
-            : xgrad_prod_t<double>(p, YV, YY, XZ, u, v, w, out, p->fx);
+  if (nplanes < 0) nplanes = p->np;
+  return lo ? xgrad_prod_t<float>(p, YV, YY, XZ, u, v, w, out, p->fx32, nplanes)
+            : xgrad_prod_t<double>(p, YV, YY, XZ, u, v, w, out, p->fx, nplanes);
 }
 
 // ---- float64 boundary / projection / diagnostics kernels ----

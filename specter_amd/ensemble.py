@@ -1,0 +1,300 @@
+"""Ensemble server: many independent HD runs on one GPU with continuous
+batching.
+
+Requests (grid, FC order, viscosity, seed, step count) arrive over time.
+Runs of the same grid signature share one batched libspx plan ("class"); a
+class owns a fixed number of member slots, its own HIP stream and one
+captured HIP graph per active-member count.  The loop
+
+1. admits every request that has arrived into a free slot of its class
+   (creating the class when the HBM budget allows; otherwise the request
+   waits until memory is released),
+2. enqueues the next chunk of RK steps of every class on its stream (the
+   classes' work overlaps on the device),
+3. waits for the chunk, hands finished runs back (final fields and
+   diagnostics) and compacts the class so the active members stay leading.
+   Classes without runs stay (with their captured graphs) until their memory
+   is needed for another class.
+
+A member of a batched plan is computed by exactly the per-line / per-row
+code of a single run, so a result does not depend on what it was batched
+with.  Time to result is measured from a request's arrival to the moment its
+fields are on the host.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import lib as _l
+from .fftp import FCPlan
+from .solver import Solver, SolverParams
+
+FC_TABLE = {5: 25, 8: 27}      # matching order d -> continuation points C
+
+
+@dataclass
+class RunRequest:
+    rid: int
+    n: int                     # nx = ny = n physical points per direction
+    d: int                     # FC matching order (5 or 8)
+    nu: float
+    seed: int
+    nsteps: int
+    arrival: float = 0.0       # seconds after the start of the stream
+    ord: int = 2
+    dt: float = 1e-3
+    precision: int = _l.SPX_FP32_MIXED
+
+    @property
+    def grid(self):
+        C = FC_TABLE[self.d]
+        return (self.n, self.n, self.n + C, C, self.d)
+
+    @property
+    def key(self):
+        return (*self.grid, self.ord, self.dt, self.precision)
+
+    @property
+    def points(self) -> int:
+        """physical points of one RK stage"""
+        return self.n ** 3
+
+
+@dataclass
+class RunResult:
+    rid: int
+    arrival: float
+    admitted: float
+    finished: float
+    fields: dict = field(default_factory=dict)     # spectral vx, vy, vz (float64)
+    diagnostics: tuple = ()                          # energy, enstrophy, <div^2>
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def state_bytes(plan: FCPlan, members: int) -> int:
+    """Device bytes of a batched HD state plus the plan's work arrays."""
+    info = plan.info()
+    # 13 float64 spectral arrays per member (v, pr, forcing, RK copies, N)
+    return int(info.device_bytes) + 13 * members * plan.nkx * plan.ny * plan.nz * 16
+
+
+def member_bytes(req: RunRequest) -> int:
+    """Estimated device bytes per member of the request's class (used before
+    the class exists)."""
+    nx, ny, nz, _, _ = req.grid
+    nkx = nx // 2 + 1
+    nc, nr = nz * ny * nkx, nx * ny * nz
+    c = 8 if req.precision == _l.SPX_FP32_MIXED else 16
+    r = c // 2
+    return 13 * nc * 16 + 12 * nc * c + 6 * nr * r + nr * 8 + 2 * nc * 16
+
+
+class _Class:
+    def __init__(self, req: RunRequest, slots: int):
+        import torch
+        self.key = req.key
+        self.stream = torch.cuda.Stream()
+        nx, ny, nz, C, d = req.grid
+        with torch.cuda.stream(self.stream):
+            self.plan = FCPlan(nx, ny, nz, C, d, precision=req.precision, members=slots,
+                               stream=self.stream.cuda_stream)
+            self.sol = Solver(self.plan, SolverParams(solver="HD", ord=req.ord, dt=req.dt,
+                                                      nu=req.nu))
+        self.slots = slots
+        self.runs: list[RunRequest] = []          # active members, slot order
+        self.done: list[int] = []                  # steps done per active member
+        self.admitted: list[float] = []
+        self.bytes = state_bytes(self.plan, slots)
+        self.stepping = 0
+
+    @property
+    def free(self) -> int:
+        return self.slots - len(self.runs)
+
+    def admit(self, req: RunRequest, now: float):
+        import torch
+        from . import ic
+        b = len(self.runs)
+        nx, ny, nz, C, _ = req.grid
+        u = ic.spectrum_peaked_velocity(nx, ny, nz, C, req.seed)
+        with torch.cuda.stream(self.stream):
+            for name in ("pr", "fx", "fy", "fz"):
+                self.sol.field(name)[b].zero_()
+            for name, arr in zip(("vx", "vy", "vz"), u):
+                self.sol.set_real_field(name, arr, member=b)
+            self.sol.set_member(b, req.nu)
+        self.runs.append(req)
+        self.done.append(0)
+        self.admitted.append(now)
+        self.plan.set_active(len(self.runs))
+
+    def launch(self, chunk: int) -> int:
+        """Enqueue min(chunk, steps left of any member) steps; returns it."""
+        k = min(chunk, min(r.nsteps - s for r, s in zip(self.runs, self.done)))
+        self.sol.step(k)
+        self.stepping = k
+        return k
+
+    def finish_chunk(self, now_fn, keep_fields: bool):
+        """Wait for the chunk, return the finished runs' results and compact."""
+        self.stream.synchronize()
+        k, self.stepping = self.stepping, 0
+        self.done = [s + k for s in self.done]
+        out = []
+        b = 0
+        while b < len(self.runs):
+            r = self.runs[b]
+            if self.done[b] < r.nsteps:
+                b += 1
+                continue
+            res = RunResult(r.rid, r.arrival, self.admitted[b], 0.0)
+            if keep_fields:
+                res.fields = {n: self.sol.spectral(n, member=b) for n in ("vx", "vy", "vz")}
+            vx, vy, vz = (self.sol.field(n) for n in ("vx", "vy", "vz"))
+            p = self.plan
+            res.diagnostics = (p.energy(vx, vy, vz, 1)[b], p.energy(vx, vy, vz, 0)[b],
+                               p.divergence(vx, vy, vz)[b])
+            res.finished = now_fn()
+            out.append(res)
+            last = len(self.runs) - 1
+            if b != last:
+                self.sol.copy_member(b, last)
+                self.runs[b], self.done[b], self.admitted[b] = (self.runs[last], self.done[last],
+                                                                self.admitted[last])
+            self.runs.pop()
+            self.done.pop()
+            self.admitted.pop()
+            if self.runs:
+                self.plan.set_active(len(self.runs))
+        return out
+
+    def close(self):
+        self.stream.synchronize()
+        self.sol.close()
+        self.plan.close()
+
+
+class EnsembleServer:
+    """Continuous-batching server for HD runs (see the module docstring).
+
+    hbm_budget: device bytes the server may hold; max_slots: members per
+    class; chunk: RK steps enqueued per class between scheduling points."""
+
+    def __init__(self, hbm_budget: int | None = None, max_slots: int = 16, chunk: int = 25,
+                 keep_fields: bool = False):
+        import torch
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info()
+            hbm_budget = int(free * 0.85)
+        self.budget = int(hbm_budget)
+        self.max_slots = max_slots
+        self.chunk = chunk
+        self.keep_fields = keep_fields
+        self.classes: dict[tuple, _Class] = {}
+        self.used = 0
+        self.peak_used = 0
+        self.peak_classes = 0
+
+    def _slots_for(self, req: RunRequest) -> int:
+        """Member slots of a new class: max_slots, or what the budget has room for."""
+        per = member_bytes(req)
+        room = (self.budget - self.used) // max(per, 1)
+        return int(max(0, min(self.max_slots, room)))
+
+    def _evict_idle(self, keep=None):
+        """Destroy classes without runs (their graphs and memory go back)."""
+        for key in [k for k, c in self.classes.items() if not c.runs and k != keep]:
+            c = self.classes.pop(key)
+            self.used -= c.bytes
+            c.close()
+
+    def _try_admit(self, req: RunRequest, now: float, queued_same: int) -> bool:
+        cls = self.classes.get(req.key)
+        if cls is None:
+            slots = self._slots_for(req)
+            if slots < min(queued_same, self.max_slots):
+                self._evict_idle()
+                slots = self._slots_for(req)
+            if slots < 1:
+                return False
+            cls = _Class(req, slots)
+            if self.used + cls.bytes > self.budget and self.classes:
+                cls.close()
+                return False
+            self.classes[req.key] = cls
+            self.used += cls.bytes
+            self.peak_used = max(self.peak_used, self.used)
+            self.peak_classes = max(self.peak_classes, len(self.classes))
+        if cls.free < 1:
+            return False
+        cls.admit(req, now)
+        return True
+
+    def run(self, requests: list[RunRequest], realtime: bool = True) -> list[RunResult]:
+        """Serve a stream of requests (arrival times relative to the start);
+        with realtime=False every request is available at once."""
+        import torch
+        torch.cuda.synchronize()
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        waiting: list[RunRequest] = []
+        results: list[RunResult] = []
+        t0 = time.perf_counter()
+
+        def now():
+            return time.perf_counter() - t0
+
+        while pending or waiting or any(c.runs for c in self.classes.values()):
+            t = now()
+            while pending and (not realtime or pending[0].arrival <= t):
+                waiting.append(pending.pop(0))
+            still = []
+            for r in waiting:
+                same = sum(1 for q in waiting if q.key == r.key)
+                if not self._try_admit(r, now(), same):
+                    still.append(r)
+            waiting = still
+            active = [c for c in self.classes.values() if c.runs]
+            if not active:
+                if pending and realtime:
+                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
+                elif not pending and waiting:
+                    raise RuntimeError("requests cannot be admitted within the HBM budget")
+                continue
+            for c in active:
+                c.launch(self.chunk)
+            for c in active:
+                results.extend(c.finish_chunk(now, self.keep_fields))
+        self._evict_idle()
+        return results
+
+
+def poisson_stream(nruns: int = 96, rate: float = 4.0, seed: int = 2026, nsteps: int = 300,
+                   sizes=(32, 64, 128, 192), orders=(5, 8), nu_range=(5e-4, 5e-3),
+                   precision: int = _l.SPX_FP32_MIXED) -> list[RunRequest]:
+    """The ensemble workload: Poisson arrivals at `rate` per second, grids
+    uniform over `sizes`, nu log-uniform over nu_range, FC order uniform over
+    `orders`, seeds 1..nruns, `nsteps` RK2 steps each."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(nruns):
+        t += rng.exponential(1.0 / rate)
+        n = int(rng.choice(sizes))
+        d = int(rng.choice(orders))
+        nu = float(math.exp(rng.uniform(math.log(nu_range[0]), math.log(nu_range[1]))))
+        out.append(RunRequest(rid=i, n=n, d=d, nu=nu, seed=i + 1, nsteps=nsteps, arrival=t,
+                              precision=precision))
+    return out
+
+
+def latency_stats(results: list[RunResult]) -> dict:
+    lat = np.array([r.latency for r in results])
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "max_s": float(lat.max())}

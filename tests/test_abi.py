@@ -103,3 +103,23 @@ def test_initial_condition_properties():
     env = ic.wall_envelope(nz, C)
     assert np.allclose(fx[:npz], 0.5 * np.sin(2 * y)[None, :, None] * env[:, None, None])
     assert np.all(fz == 0)
+
+
+def test_ensemble_stream_properties():
+    """The ensemble workload generator: Poisson arrivals, grid / order /
+    viscosity ranges, seeds 1..n (no GPU needed)."""
+    from specter_amd.ensemble import member_bytes, poisson_stream
+    reqs = poisson_stream()
+    assert len(reqs) == 96
+    t = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(t) > 0)
+    gaps = np.diff(np.concatenate([[0.0], t]))
+    assert abs(gaps.mean() - 0.25) < 0.08          # rate 4/s
+    assert {r.n for r in reqs} <= {32, 64, 128, 192}
+    assert {r.d for r in reqs} <= {5, 8}
+    assert all(5e-4 <= r.nu <= 5e-3 for r in reqs)
+    assert [r.seed for r in reqs] == list(range(1, 97))
+    assert all(r.nsteps == 300 and r.ord == 2 for r in reqs)
+    sizes = sorted({r.n for r in reqs})
+    mb = [member_bytes(next(r for r in reqs if r.n == n)) for n in sizes]
+    assert mb == sorted(mb)

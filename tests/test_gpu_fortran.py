@@ -92,6 +92,8 @@ def test_fortran_driver_matches_python_and_oracle(tmp_path):
         diff = np.abs(out - py)
         print(f"{name}: driver vs Python max |diff| {diff.max():.3e} at "
               f"{np.unravel_index(diff.argmax(), diff.shape)}, rel L2 {rel_l2(out, py):.3e}")
-        assert np.array_equal(out, py), name
+        # same library trajectory; the driver's final v/(nx ny nz) is Fortran
+        # arithmetic, which the compiler may evaluate as a reciprocal product
+        assert diff.max() <= 4 * np.finfo(float).eps * np.abs(py).max(), name
         ref = O.fftp3d_complex_to_real(g, getattr(st, name).copy())[:npz] / (nx * ny * nz)
         assert rel_l2(out, ref) < 1e-11, name
