@@ -310,16 +310,16 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
     if ((rc = build_line_fft(p->fz32, nz, 8, 16))) return bail(rc);
     if (C > 0) {
       if (d > 8) return bail(fail(SPX_ERR_UNSUPPORTED, "mixed plans: matching order above 8"));
-      // w[q][k] = exp(+2 pi i k z_q / nz) at the boundary planes z_q
-      std::vector<double> w(size_t(2 * d) * nz * 2);
+      // exp(+2 pi i k z / nz) for z = 1 (first nz entries) and z = np - d
+      std::vector<double> w(size_t(2) * nz * 2);
       const long double TWO_PI = 6.283185307179586476925286766559005768L;
-      for (int q = 0; q < 2 * d; ++q) {
-        const long z = q < d ? q : long(p->np) - 2 * d + q;
+      for (int t = 0; t < 2; ++t) {
+        const long z = t == 0 ? 1 : long(p->np) - d;
         for (int k = 0; k < nz; ++k) {
           const long r = (long(k) * z) % nz;
           const long double a = TWO_PI * (long double)r / (long double)nz;
-          w[2 * (size_t(q) * nz + k)] = double(cosl(a));
-          w[2 * (size_t(q) * nz + k) + 1] = double(sinl(a));
+          w[2 * (size_t(t) * nz + k)] = double(cosl(a));
+          w[2 * (size_t(t) * nz + k) + 1] = double(sinl(a));
         }
       }
       if ((rc = upload(p, reinterpret_cast<double**>(&p->d_wbnd), w))) return bail(rc);

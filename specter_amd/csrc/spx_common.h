@@ -36,6 +36,7 @@ struct LineFFT {
   int tpl, lgtpl;  // threads per line (power of two) and its log2
   int nl, lgnl;    // lines per workgroup (power of two) and its log2
   int ldl;         // LDS stride between lines (complex elements)
+  int tab;         // complex slots of the tables staged at the head of LDS
   const C* tw;     // W_M[m] = exp(-2 pi i m / M), m < M
   const C* chirp;  // Bluestein: w[m] = exp(-pi i m^2 / n), m < n
   const C* bfil;   // Bluestein: FFT_M(wrapped conj chirp)/M ; Rader: FFT_{n-1}(b)/(n-1)
@@ -45,15 +46,17 @@ struct LineFFT {
 // Host-side owner of the tables behind a LineFFT.
 struct LineFFTHost {
   int n = 0, M = 0, mode = 0, nstage = 0, tpl = 0, lgtpl = 0, nl = 0, lgnl = 0, ldl = 0;
+  int tab = 0;  // LDS table slots: twiddles M, plus bfil M and chirp n
+                // (Bluestein) or bfil M and the 2M Rader indices (Rader)
   int radix[kMaxStages] = {0};
   unsigned magic[kMaxStages] = {0};
   void* tw = nullptr;
   void* chirp = nullptr;
   void* bfil = nullptr;
   void* perm = nullptr;
-  // LDS bytes of a launch: twiddles + nbuf line buffers
+  // LDS bytes of a launch: tables + nbuf line buffers
   size_t lds_bytes(size_t csize, int nbuf = 1) const {
-    return (size_t(M) + size_t(nbuf) * size_t(nl) * size_t(ldl)) * csize;
+    return (size_t(tab) + size_t(nbuf) * size_t(nl) * size_t(ldl)) * csize;
   }
   LineFFTHost with_lines(int lines) const {
     LineFFTHost h = *this;
@@ -65,7 +68,7 @@ struct LineFFTHost {
   template <typename C> LineFFT<C> dev() const {
     LineFFT<C> f;
     f.n = n; f.M = M; f.mode = mode; f.nstage = nstage;
-    f.tpl = tpl; f.lgtpl = lgtpl; f.nl = nl; f.lgnl = lgnl; f.ldl = ldl;
+    f.tpl = tpl; f.lgtpl = lgtpl; f.nl = nl; f.lgnl = lgnl; f.ldl = ldl; f.tab = tab;
     for (int i = 0; i < kMaxStages; ++i) {
       f.radix[i] = radix[i];
       f.magic[i] = magic[i];
@@ -160,7 +163,7 @@ struct spx_plan {
   void* RVb = nullptr;      // v_c in real space (3 components)
   void* ROb = nullptr;      // products (4 components)
   void* XFb = nullptr;      // forward x, y transforms of the products (4 components)
-  void* d_wbnd = nullptr;   // exp(+2 pi i k z_q / nz), 2d x nz float64 complex
+  void* d_wbnd = nullptr;   // exp(+2 pi i k / nz), then exp(+2 pi i k (np-d) / nz)
   void* bc = nullptr;       // wall data of sol_project, 2 complex per (kx,ky) line
   double* part = nullptr;   // per-line partial sums of the diagnostics
   std::vector<double> h_part;

@@ -45,11 +45,46 @@ __device__ __forceinline__ cpx<T> cmul_mi(cpx<T> a) { return mkc<T>(a.y, -a.x); 
 template <typename T>
 __device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return mkc<T>(a.x * s, a.y * s); }
 
-// Radix-R DFT in registers, forward sign.  tw is the W_M table, s = M/R.
+// cos(2 pi m / R) and sin(2 pi m / R) for m = 1..(R-1)/2 of the odd radices,
+// correctly rounded (computed to 45 digits)
+template <int R>
+struct OddRoots;
+template <>
+struct OddRoots<5> {
+  static constexpr double c[2] = {0.3090169943749474241022934, -0.8090169943749474241022934};
+  static constexpr double s[2] = {0.9510565162951535721164393, 0.5877852522924731291687060};
+};
+template <>
+struct OddRoots<7> {
+  static constexpr double c[3] = {0.6234898018587335305250049, -0.2225209339563144042889026,
+                                  -0.9009688679024191262361023};
+  static constexpr double s[3] = {0.7818314824680298087084445, 0.9749279121818236070181317,
+                                  0.4338837391175581204757683};
+};
+template <>
+struct OddRoots<11> {
+  static constexpr double c[5] = {0.8412535328311811688618116, 0.4154150130018864255292741,
+                                  -0.1423148382732851404437927, -0.6548607339452850640569251,
+                                  -0.9594929736144973898903681};
+  static constexpr double s[5] = {0.5406408174555975821076360, 0.9096319953545183714117154,
+                                  0.9898214418809327323760920, 0.7557495743542582837740358,
+                                  0.2817325568414296977114179};
+};
+template <>
+struct OddRoots<13> {
+  static constexpr double c[6] = {0.8854560256532098959003755, 0.5680647467311558025118076,
+                                  0.1205366802553230533490677, -0.3546048870425356259696379,
+                                  -0.7485107481711010986346306, -0.9709418174260520271569823};
+  static constexpr double s[6] = {0.4647231720437685456560153, 0.8229838658936563945796174,
+                                  0.9927088740980539928007516, 0.9350162426854148234397846,
+                                  0.6631226582407952023767855, 0.2393156642875577671487537};
+};
+
+// Radix-R DFT in registers, forward sign (odd R: r paired with R-r, the
+// roots of unity as immediates).
 template <typename T, int R>
 struct Dft {
-  static __device__ __forceinline__ void run(cpx<T>* v, const cpx<T>* tw, int s) {
-    // odd R: pair r with R-r
+  static __device__ __forceinline__ void run(cpx<T>* v, const cpx<T>*, int) {
     constexpr int H = (R - 1) / 2;
     cpx<T> a[H + 1], b[H + 1];
 #pragma unroll
@@ -68,9 +103,11 @@ struct Dft {
       cpx<T> im = mkc<T>(0, 0);
 #pragma unroll
       for (int r = 1; r <= H; ++r) {
-        const cpx<T> w = tw[((r * q) % R) * s];  // cos - i sin
-        re = cadd(re, cscale(a[r], w.x));
-        im = cadd(im, cscale(b[r], -w.y));
+        const int m = (r * q) % R;  // compile-time after unrolling
+        const T c = T(OddRoots<R>::c[(m <= H ? m : R - m) - 1]);
+        const T sn = m <= H ? T(OddRoots<R>::s[m - 1]) : T(-OddRoots<R>::s[R - m - 1]);
+        re = cadd(re, cscale(a[r], c));
+        im = cadd(im, cscale(b[r], sn));
       }
       out[q] = cadd(re, cmul_mi(im));
       out[R - q] = csub(re, cmul_mi(im));
@@ -245,11 +282,21 @@ __device__ void fft_lds(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw, i
   }
 }
 
-// Copy the twiddle table into LDS (called by all threads; the caller's next
-// __syncthreads publishes it).
+// Copy the tables into LDS (called by all threads; the caller's next
+// __syncthreads publishes them): twiddles at [0, M); Bluestein: filter at
+// [M, 2M), chirp at [2M, 2M+n); Rader: filter at [M, 2M), then the 2M
+// permutation indices as ints.
 template <typename T>
 __device__ __forceinline__ void stage_twiddles(cpx<T>* tws, const LineFFT<cpx<T>>& f) {
   for (int m = threadIdx.x; m < f.M; m += blockDim.x) tws[m] = f.tw[m];
+  if (f.mode == 0) return;
+  for (int m = threadIdx.x; m < f.M; m += blockDim.x) tws[f.M + m] = f.bfil[m];
+  if (f.mode == 1) {
+    for (int m = threadIdx.x; m < f.n; m += blockDim.x) tws[2 * f.M + m] = f.chirp[m];
+  } else {
+    int* pr = reinterpret_cast<int*>(tws + 2 * f.M);
+    for (int m = threadIdx.x; m < 2 * f.M; m += blockDim.x) pr[m] = f.perm[m];
+  }
 }
 
 // Forward DFT of length f.n of the n values at x[0..n), result in x[0..n).
@@ -261,29 +308,31 @@ __device__ void line_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw, 
     fft_lds<T, RS>(x, f, tw, lane);
     return;
   }
+  const cpx<T>* bfil = tw + M;  // tables staged in LDS by stage_twiddles
   if (f.mode == 1) {  // Bluestein
+    const cpx<T>* chirp = tw + 2 * M;
     for (int m = lane; m < M; m += tpl)
-      x[m] = (m < n) ? cmul(x[m], f.chirp[m]) : mkc<T>(0, 0);
+      x[m] = (m < n) ? cmul(x[m], chirp[m]) : mkc<T>(0, 0);
     __syncthreads();
     fft_lds<T, RS>(x, f, tw, lane);
-    for (int m = lane; m < M; m += tpl) x[m] = cconj(cmul(x[m], f.bfil[m]));
+    for (int m = lane; m < M; m += tpl) x[m] = cconj(cmul(x[m], bfil[m]));
     __syncthreads();
     fft_lds<T, RS>(x, f, tw, lane);
-    for (int m = lane; m < n; m += tpl) x[m] = cmul(cconj(x[m]), f.chirp[m]);
+    for (int m = lane; m < n; m += tpl) x[m] = cmul(cconj(x[m]), chirp[m]);
     __syncthreads();
     return;
   }
   // Rader: M = n - 1, second buffer b = x + n
   cpx<T>* b = x + n;
-  const int* pin = f.perm;
-  const int* pout = f.perm + M;
+  const int* pin = reinterpret_cast<const int*>(tw + 2 * M);
+  const int* pout = pin + M;
   const cpx<T> x0 = x[0];
   for (int q = lane; q < M; q += tpl) b[q] = x[pin[q]];
   __syncthreads();
   fft_lds<T, RS>(b, f, tw, lane);
   cpx<T> X0 = mkc<T>(0, 0);
   if (lane == 0) X0 = cadd(x0, b[0]);
-  for (int m = lane; m < M; m += tpl) b[m] = cconj(cmul(b[m], f.bfil[m]));
+  for (int m = lane; m < M; m += tpl) b[m] = cconj(cmul(b[m], bfil[m]));
   __syncthreads();
   fft_lds<T, RS>(b, f, tw, lane);
   for (int p = lane; p < M; p += tpl) x[pout[p]] = cadd(x0, cconj(b[p]));

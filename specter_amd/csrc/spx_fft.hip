@@ -28,7 +28,7 @@ __device__ __forceinline__ cpx<TO> ccast(cpx<TI> v) { return mkc<TO>(TO(v.x), TO
 #define SPX_PROLOGUE(CT, f)                                      \
   extern __shared__ unsigned char smem[];                        \
   CT* const tws = reinterpret_cast<CT*>(smem);                   \
-  CT* const lds = tws + (f).M;                                   \
+  CT* const lds = tws + (f).tab;                                 \
   stage_twiddles(tws, (f));                                      \
   const int tid = threadIdx.x;                                   \
   const int line = tid >> (f).lgtpl, lane = tid & ((f).tpl - 1); \
@@ -181,53 +181,76 @@ __global__ void __launch_bounds__(256) k_z_x2s(const cpx<T>* __restrict__ X, cpx
 // float64 values of the backward z transform at the 2d boundary planes
 // z = 0..d-1 and np-d..np-1 (the points the FC continuation reads), for the
 // float32 nonlinear pipeline of mixed plans: a direct partial DFT of every
-// line, one wavefront per line, against w[q*n + k] = exp(+2 pi i k z_q / n).
-// blockIdx.y: 0 = v, 1 = (i kz) v; blockIdx.z = component.  Output planes
-// [comp][member][q][ky][kx] with `mem` members.
+// line, 16 lanes per line (4 lines per wavefront), for v and (i kz) v
+// together.  Twiddles: w1[k] = exp(+2 pi i k/n) and wt[k] = exp(+2 pi i k
+// (np-d)/n) from long-double tables, then exp(+2 pi i k z/n) for the d
+// consecutive z of each wall block by d-1 products (error ~ d ulp).
+// blockIdx.y = component.  Output planes [comp][member][q][ky][kx].
 // ---------------------------------------------------------------------------
 struct BndSrc {
   const cpx<double>* s[4];
 };
 
-template <int D2>
+template <int D>
 __global__ void __launch_bounds__(256) k_z_s2x_bnd(BndSrc src, cpx<double>* __restrict__ XVb,
                                                    cpx<double>* __restrict__ XZb,
-                                                   const cpx<double>* __restrict__ w, int n,
+                                                   const cpx<double>* __restrict__ w1,
+                                                   const cpx<double>* __restrict__ wt, int n,
                                                    int nkx, int ny, int mem,
                                                    const double* __restrict__ kz) {
-  const int lane = threadIdx.x & 63;
+  constexpr int D2 = 2 * D;
+  constexpr int LPL = 16;  // lanes per line
+  const int sub = threadIdx.x & (LPL - 1);
   const long nlines = long(nkx) * ny;
-  const long ln = long(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (ln >= nlines * mem) return;  // uniform per wavefront
-  const int kind = blockIdx.y, comp = blockIdx.z;
-  const cpx<double>* S = src.s[comp] + ln * n;
-  double ar[D2], ai[D2];
+  const long ln = long(blockIdx.x) * (blockDim.x / LPL) + (threadIdx.x / LPL);
+  const bool ok = ln < nlines * mem;
+  const int comp = blockIdx.y;
+  const cpx<double>* S = src.s[comp] + (ok ? ln : 0) * n;
+  cpx<double> av[D2], az[D2];
 #pragma unroll
-  for (int q = 0; q < D2; ++q) ar[q] = ai[q] = 0.0;
-  for (int k = lane; k < n; k += 64) {
-    cpx<double> v = S[k];
-    if (kind) v = ikmul(kz[k], v);
+  for (int q = 0; q < D2; ++q) av[q] = az[q] = mkc<double>(0, 0);
+  if (ok) {
+    for (int k = sub; k < n; k += LPL) {
+      const cpx<double> v = S[k];
+      const cpx<double> vz = ikmul(kz[k], v);
+      const cpx<double> step = w1[k];
+      cpx<double> t = mkc<double>(1.0, 0.0);
 #pragma unroll
-    for (int q = 0; q < D2; ++q) {
-      const cpx<double> t = w[q * n + k];
-      ar[q] += v.x * t.x - v.y * t.y;
-      ai[q] += v.x * t.y + v.y * t.x;
+      for (int q = 0; q < D; ++q) {
+        av[q] = cadd(av[q], cmul(v, t));
+        az[q] = cadd(az[q], cmul(vz, t));
+        t = cmul(t, step);
+      }
+      t = wt[k];
+#pragma unroll
+      for (int q = D; q < D2; ++q) {
+        av[q] = cadd(av[q], cmul(v, t));
+        az[q] = cadd(az[q], cmul(vz, t));
+        t = cmul(t, step);
+      }
     }
   }
+  // reduce over the 16 lanes of the line (lanes of one line are contiguous)
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
+  for (int off = LPL / 2; off > 0; off >>= 1) {
 #pragma unroll
     for (int q = 0; q < D2; ++q) {
-      ar[q] += __shfl_xor(ar[q], off);
-      ai[q] += __shfl_xor(ai[q], off);
+      av[q].x += __shfl_xor(av[q].x, off);
+      av[q].y += __shfl_xor(av[q].y, off);
+      az[q].x += __shfl_xor(az[q].x, off);
+      az[q].y += __shfl_xor(az[q].y, off);
     }
   }
+  if (!ok) return;
   const long b = ln / nlines, l = ln - b * nlines;
   const long i = l / ny, jy = l - i * ny;
-  cpx<double>* out = (kind ? XZb : XVb) + ((long(comp) * mem + b) * D2) * nlines + jy * nkx + i;
+  const long o = ((long(comp) * mem + b) * D2) * nlines + jy * nkx + i;
 #pragma unroll
   for (int q = 0; q < D2; ++q)
-    if (lane == q) out[q * nlines] = mkc<double>(ar[q], ai[q]);
+    if (sub == q) {
+      XVb[o + q * nlines] = av[q];
+      XZb[o + q * nlines] = az[q];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -919,11 +942,15 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
   while ((1 << f.lgtpl) < f.tpl) ++f.lgtpl;
   const int span = f.mode == 0 ? n : (f.mode == 1 ? f.M : 2 * n);
   f.ldl = span + ((span % 16 == 0) ? 1 : 0);
+  // tables staged in LDS: twiddles; Bluestein filter + chirp; Rader filter + indices
+  if (f.mode == 0) f.tab = f.M;
+  else if (f.mode == 1) f.tab = 2 * f.M + n;
+  else f.tab = 2 * f.M + int((2 * size_t(f.M) * sizeof(int) + csize - 1) / csize);
   const size_t lds_budget = 64 * 1024;
   int nl = std::max(1, 256 / f.tpl);
   if (want_lines > 0) nl = std::min(nl, want_lines);
-  while (nl > 1 && (size_t(f.M) + size_t(nl) * f.ldl) * csize > lds_budget) nl >>= 1;
-  if ((size_t(f.M) + size_t(nl) * f.ldl) * csize > 160 * 1024)
+  while (nl > 1 && (size_t(f.tab) + size_t(nl) * f.ldl) * csize > lds_budget) nl >>= 1;
+  if ((size_t(f.tab) + size_t(nl) * f.ldl) * csize > 160 * 1024)
     return fail(SPX_ERR_UNSUPPORTED, "FFT line does not fit in LDS");
   f.nl = nl;
   f.lgnl = 0;
@@ -1121,17 +1148,18 @@ int zfft_s2x_bnd(spx_plan* p, const void* const* S, int ncomp, void* XVb, void* 
   BndSrc src;
   for (int c = 0; c < 4; ++c) src.s[c] = c < ncomp ? C_<double>(S[c]) : nullptr;
   const long lines = p->wlines();
-  const dim3 grid(unsigned((lines + 3) / 4), 2u, unsigned(ncomp));
-  ProfScope ps_(p, "z_s2x_bnd", double(ncomp) * 2.0 * double(p->wnc()) * 16.0);
-  const Cd* w = static_cast<const Cd*>(p->d_wbnd);
+  const dim3 grid(unsigned((lines + 15) / 16), unsigned(ncomp));
+  ProfScope ps_(p, "z_s2x_bnd", double(ncomp) * double(p->wnc()) * 16.0);
+  const int n = p->g.nz, nkx = p->nkx, ny = p->g.ny, mem = p->mem;
+  const Cd* w1 = static_cast<const Cd*>(p->d_wbnd);
+  const Cd* wt = w1 + n;
   Cd* v = C_<double>(XVb);
   Cd* z = C_<double>(XZb);
-  const int n = p->g.nz, nkx = p->nkx, ny = p->g.ny, mem = p->mem;
   switch (p->g.oz) {
-#define SPX_BND_CASE(D)                                                                          \
-  case D:                                                                                        \
-    hipLaunchKernelGGL(k_z_s2x_bnd<2 * D>, grid, dim3(256), 0, p->stream, src, v, z, w, n, nkx, \
-                       ny, mem, p->d_kz);                                                        \
+#define SPX_BND_CASE(D)                                                                        \
+  case D:                                                                                      \
+    hipLaunchKernelGGL(k_z_s2x_bnd<D>, grid, dim3(256), 0, p->stream, src, v, z, w1, wt, n,   \
+                       nkx, ny, mem, p->d_kz);                                                 \
     break;
     SPX_BND_CASE(1)
     SPX_BND_CASE(2)

@@ -3,7 +3,8 @@
 !
 ! Reads SPECTER's parameter.inp namelists (&status, &boxparams, &parameter,
 ! &velocity, &velbound; specter.fpp:201-239) plus a &grid group that stands
-! in for the compile-time NX/NY/NZ/CZ/OZ/ORD of Makefile.in.  Starts like a
+! in for the compile-time NX/NY/NZ/CZ/OZ/ORD of Makefile.in (and prec:
+! 0 = float64, 1 = float32 nonlinear term, the compile-time GP choice).  Starts like a
 ! reference restart with stat >= 1 (specter.fpp:885-934): the physical-space
 ! fields vx, vy, vz (and optionally fx, fy, fz, pr) are read from
 ! idir/<name>.NNNN.out as raw float64 arrays (nx, ny, nz-Cz) in Fortran order
@@ -18,7 +19,7 @@ PROGRAM spx_hd
    USE spx
    IMPLICIT NONE
 
-   INTEGER :: nx = 64, ny = 64, nz = 89, Cz = 25, oz = 5, ord = 2
+   INTEGER :: nx = 64, ny = 64, nz = 89, Cz = 25, oz = 5, ord = 2, prec = 0
    CHARACTER(len=100) :: idir = 'in/', odir = 'out/', tdir = 'tables/'
    INTEGER :: stat = 1, mult = 1, bench = 0, outs = 0
    REAL(8) :: Lx = 1.d0, Ly = 1.d0, Lz = 1.d0
@@ -27,7 +28,7 @@ PROGRAM spx_hd
    REAL(8) :: f0 = 0.d0, u0 = 1.d0, kdn = 2.d0, kup = 4.d0, nu = 1.d-3
    CHARACTER(len=15) :: vbczsta = 'noslip', vbczend = 'noslip'
    REAL(8) :: vxzsta = 0.d0, vxzend = 0.d0, vyzsta = 0.d0, vyzend = 0.d0
-   NAMELIST /grid/ nx, ny, nz, Cz, oz, ord
+   NAMELIST /grid/ nx, ny, nz, Cz, oz, ord, prec
    NAMELIST /status/ idir, odir, tdir, stat, mult, bench, outs
    NAMELIST /boxparams/ Lx, Ly, Lz
    NAMELIST /parameter/ dt, step, tstep, cstep, seed
@@ -73,7 +74,7 @@ PROGRAM spx_hd
    CALL read_table(TRIM(tdir)//'/A'//itoa(Cz)//'-'//itoa(oz)//'.dat', A, Cz*oz)
    CALL read_table(TRIM(tdir)//'/Q'//itoa(oz)//'.dat', Q, oz*oz)
 
-   g = spx_grid(nx, ny, nz, Cz, oz, Lx, Ly, Lz, SPX_FP64)
+   g = spx_grid(nx, ny, nz, Cz, oz, Lx, Ly, Lz, prec)
    CALL spx_check(spx_plan_create(g, A, Q, C_NULL_PTR, plan), 'spx_plan_create')
    prm = spx_params(SPX_SOLVER_HD, ord, dt, nu, 0.d0, 0.d0, 0.d0, &
                     (/vxzsta, vyzsta/), (/vxzend, vyzend/))
