@@ -37,6 +37,23 @@ __device__ __forceinline__ cpx<TO> ccast(cpx<TI> v) { return mkc<TO>(TO(v.x), TO
 
 constexpr int KEEP = kStage;  // line elements a thread can hold in registers (n <= KEEP*tpl)
 
+// XCD-aware tile order of a 2D grid.  Workgroups are handed to the 8 XCDs
+// (each with its own L2) round-robin by linear id; the kx tiles of the
+// transposing kernels are narrower than a cache line, so neighbouring tiles
+// must meet in one L2.  Each XCD gets a contiguous range of logical tiles
+// (a bijection on [0, gx*gy)): returns the logical (x, y) of this workgroup.
+__device__ __forceinline__ void xcd_tile(int& bx, int& by) {
+  constexpr int X = 8;
+  const int gx = gridDim.x;
+  const int n = gx * gridDim.y;
+  const int pid = blockIdx.x + gx * blockIdx.y;
+  const int q = n / X, r = n % X;
+  const int x = pid % X, slot = pid / X;
+  const int l = x * q + (x < r ? x : r) + slot;
+  by = l / gx;
+  bx = l - by * gx;
+}
+
 // ---------------------------------------------------------------------------
 // z transforms in place on S (contiguous lines)
 // ---------------------------------------------------------------------------
@@ -118,7 +135,9 @@ __global__ void __launch_bounds__(256) k_z_x2s(const cpx<T>* __restrict__ X, cpx
   const int n = f.n;
   X += blockIdx.z * xstride;
   S += blockIdx.z * (long(n) * ny * nkx);
-  const int i0 = blockIdx.x * f.nl, j = blockIdx.y;
+  int bx, j;
+  xcd_tile(bx, j);
+  const int i0 = bx * f.nl;
   const int nload = nofft ? n : np;
   const long pl = long(nkx) * ny;
   {
@@ -274,7 +293,9 @@ __global__ void __launch_bounds__(256) k_z_s2x(const cpx<TI>* __restrict__ S, cp
     X += xs;
     if constexpr (DUAL) X2 += xs;
   }
-  const int i0 = blockIdx.x * f.nl, j = blockIdx.y;
+  int bx, j;
+  xcd_tile(bx, j);
+  const int i0 = bx * f.nl;
   const int i = i0 + line;
   const bool ok = i < nkx;
   const long base = long(n) * (j + long(ny) * (ok ? i : 0));
@@ -326,8 +347,10 @@ __global__ void __launch_bounds__(256) k_y(const cpx<T>* Xin, cpx<T>* Xout,
                                            int ny, int forward, const double* __restrict__ ky) {
   using Cp = cpx<T>;
   SPX_PROLOGUE(Cp, f);
-  const int i0 = blockIdx.x * f.nl;
-  const long base = long(blockIdx.y) * nkx * ny;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i0 = bx * f.nl;
+  const long base = long(by) * nkx * ny;
   const int l = tid & (f.nl - 1), ii = i0 + l;
   for (int jy = tid >> f.lgnl; jy < ny; jy += f.tpl) {
     Cp v = mkc<T>(0, 0);
