@@ -304,6 +304,18 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
   if ((rc = build_line_fft(p->fy, ny, 16, 16))) return bail(rc);
   if ((rc = build_line_fft(p->fz, nz, 16, 16))) return bail(rc);
   p->fz2 = p->fz.with_lines(p->fz.nl / 2);
+  if (C > 0) {
+    // exp(+2 pi i k (np-1) / nz): the top-wall value of a z line as one DFT sum
+    std::vector<double> w(size_t(nz) * 2);
+    const long double TWO_PI = 6.283185307179586476925286766559005768L;
+    for (int k = 0; k < nz; ++k) {
+      const long r = (long(k) * (p->np - 1)) % nz;
+      const long double ang = TWO_PI * (long double)r / (long double)nz;
+      w[2 * size_t(k)] = double(cosl(ang));
+      w[2 * size_t(k) + 1] = double(sinl(ang));
+    }
+    if ((rc = upload(p, reinterpret_cast<double**>(&p->d_wtop), w))) return bail(rc);
+  }
   if (p->fp32) {
     if ((rc = build_line_fft(p->fx32, nx, 8, 0))) return bail(rc);
     if ((rc = build_line_fft(p->fy32, ny, 8, 16))) return bail(rc);
@@ -599,8 +611,7 @@ int spx_sol_project(spx_plan* p, void* a, void* b, void* c, void* d, int bctarge
                 "sol_project: only bctarget=1 with homogeneous Dirichlet walls is implemented");
   MemberView mv(p, p->nact);
   int rc;
-  if ((rc = op_poisson_project(p, a, b, c, d))) return rc;
-  if ((rc = proj_bc(p, c))) return rc;
+  if ((rc = op_poisson_project_bc(p, a, b, c, d))) return rc;
   if ((rc = proj_pressure(p, d))) return rc;
   return proj_harmonic(p, a, b, c);
 }

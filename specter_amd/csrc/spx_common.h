@@ -164,6 +164,7 @@ struct spx_plan {
   void* ROb = nullptr;      // products (4 components)
   void* XFb = nullptr;      // forward x, y transforms of the products (4 components)
   void* d_wbnd = nullptr;   // exp(+2 pi i k / nz), then exp(+2 pi i k (np-d) / nz)
+  void* d_wtop = nullptr;   // exp(+2 pi i k (np-1) / nz): the top wall of sol_project
   void* bc = nullptr;       // wall data of sol_project, 2 complex per (kx,ky) line
   double* part = nullptr;   // per-line partial sums of the diagnostics
   std::vector<double> h_part;
@@ -270,6 +271,8 @@ int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, 
                  const void* const* f, const void* const* src, const double* src_coef,
                  const double* const* nu, double dt, double rmp, bool nl_lo = false);
 int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d);
+// poisson_project plus the wall data bc of sol_project (replaces proj_bc)
+int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d);
 int op_axpy_field(spx_plan* p, void* y, const void* x, double alpha);
 int op_copy_fields(spx_plan* p, int n, const void* const* src, void* const* dst);
 int op_real_div3(spx_plan* p, void* r, double nx, double ny, double nz);

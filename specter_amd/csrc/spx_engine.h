@@ -304,7 +304,9 @@ __device__ __forceinline__ void stage_twiddles(cpx<T>* tws, const LineFFT<cpx<T>
 template <typename T, int RS>
 __device__ void line_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw, int lane) {
   const int n = f.n, M = f.M, tpl = f.tpl;
-  if (f.mode == 0) {
+  // the power-of-two family is only used for direct transforms: keep the
+  // Rader / Bluestein code (and its registers) out of those instances
+  if (RS == RS_POW2 || f.mode == 0) {
     fft_lds<T, RS>(x, f, tw, lane);
     return;
   }

@@ -225,51 +225,49 @@ __global__ void __launch_bounds__(256) k_z_s2x_bnd(BndSrc src, cpx<double>* __re
   const bool ok = ln < nlines * mem;
   const int comp = blockIdx.y;
   const cpx<double>* S = src.s[comp] + (ok ? ln : 0) * n;
-  cpx<double> av[D2], az[D2];
-#pragma unroll
-  for (int q = 0; q < D2; ++q) av[q] = az[q] = mkc<double>(0, 0);
-  if (ok) {
-    for (int k = sub; k < n; k += LPL) {
-      const cpx<double> v = S[k];
-      const cpx<double> vz = ikmul(kz[k], v);
-      const cpx<double> step = w1[k];
-      cpx<double> t = mkc<double>(1.0, 0.0);
-#pragma unroll
-      for (int q = 0; q < D; ++q) {
-        av[q] = cadd(av[q], cmul(v, t));
-        az[q] = cadd(az[q], cmul(vz, t));
-        t = cmul(t, step);
-      }
-      t = wt[k];
-#pragma unroll
-      for (int q = D; q < D2; ++q) {
-        av[q] = cadd(av[q], cmul(v, t));
-        az[q] = cadd(az[q], cmul(vz, t));
-        t = cmul(t, step);
-      }
-    }
-  }
-  // reduce over the 16 lanes of the line (lanes of one line are contiguous)
-#pragma unroll
-  for (int off = LPL / 2; off > 0; off >>= 1) {
-#pragma unroll
-    for (int q = 0; q < D2; ++q) {
-      av[q].x += __shfl_xor(av[q].x, off);
-      av[q].y += __shfl_xor(av[q].y, off);
-      az[q].x += __shfl_xor(az[q].x, off);
-      az[q].y += __shfl_xor(az[q].y, off);
-    }
-  }
-  if (!ok) return;
-  const long b = ln / nlines, l = ln - b * nlines;
+  const long b = ok ? ln / nlines : 0, l = ok ? ln - b * nlines : 0;
   const long i = l / ny, jy = l - i * ny;
   const long o = ((long(comp) * mem + b) * D2) * nlines + jy * nkx + i;
+  // v, then (i kz) v: one accumulator set live at a time
+  for (int kind = 0; kind < 2; ++kind) {
+    cpx<double> acc[D2];
 #pragma unroll
-  for (int q = 0; q < D2; ++q)
-    if (sub == q) {
-      XVb[o + q * nlines] = av[q];
-      XZb[o + q * nlines] = az[q];
+    for (int q = 0; q < D2; ++q) acc[q] = mkc<double>(0, 0);
+    if (ok) {
+      for (int k = sub; k < n; k += LPL) {
+        cpx<double> v = S[k];
+        if (kind) v = ikmul(kz[k], v);
+        const cpx<double> step = w1[k];
+        cpx<double> t = mkc<double>(1.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < D; ++q) {
+          acc[q] = cadd(acc[q], cmul(v, t));
+          t = cmul(t, step);
+        }
+        t = wt[k];
+#pragma unroll
+        for (int q = D; q < D2; ++q) {
+          acc[q] = cadd(acc[q], cmul(v, t));
+          t = cmul(t, step);
+        }
+      }
     }
+    // reduce over the 16 lanes of the line (lanes of one line are contiguous)
+#pragma unroll
+    for (int off = LPL / 2; off > 0; off >>= 1) {
+#pragma unroll
+      for (int q = 0; q < D2; ++q) {
+        acc[q].x += __shfl_xor(acc[q].x, off);
+        acc[q].y += __shfl_xor(acc[q].y, off);
+      }
+    }
+    if (ok) {
+      cpx<double>* out = kind ? XZb : XVb;
+#pragma unroll
+      for (int q = 0; q < D2; ++q)
+        if (sub == q) out[o + q * nlines] = acc[q];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1066,8 +1064,9 @@ static int lds_limit(K kernel, size_t bytes) {
 }
 
 // radix set of a plan: the smallest compiled kernel family that covers it
+// (Rader and Bluestein plans never use the power-of-two family)
 static int rset_of(const LineFFTHost& f) {
-  int rs = RS_POW2;
+  int rs = f.mode == 0 ? RS_POW2 : RS_SMOOTH;
   for (int i = 0; i < f.nstage; ++i) {
     const int R = f.radix[i];
     if (R == 11 || R == 13) rs = RS_ALL;

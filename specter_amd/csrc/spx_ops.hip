@@ -176,6 +176,58 @@ __global__ void k_poisson_project(cpx<TS>* __restrict__ a, cpx<TS>* __restrict__
   st(c, e, csub(Cc, ik(kzk, dd)));
 }
 
+// The same per element, one wavefront per (kx, ky) line, plus the wall data
+// of sol_project (boundary_mod.fpp:262-352): bc = IFFT_z(c/nz) at z = 0 and
+// z = Lz, evaluated as the two DFT sums sum_k c_k/nz and
+// sum_k c_k/nz exp(+2 pi i k (np-1)/nz) instead of a full inverse transform.
+template <typename TS>
+__global__ void __launch_bounds__(256) k_poisson_project_bc(
+    cpx<TS>* __restrict__ a, cpx<TS>* __restrict__ b, cpx<TS>* __restrict__ c,
+    Cp* __restrict__ d, Cp* __restrict__ bc, long nlines, int nz, int ny,
+    const double* __restrict__ kx, const double* __restrict__ ky, const double* __restrict__ kz,
+    const Cp* __restrict__ wtop, double tmp) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const long ln = long(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (ln >= nlines) return;  // uniform per wavefront
+  const long i = ln / ny;
+  const int j = int(ln - i * ny);
+  const double kxi = kx[i], kyj = ky[j];
+  Cp s0 = mkc<double>(0, 0), s1 = mkc<double>(0, 0);
+  for (int k = lane; k < nz; k += 64) {
+    const long e = ln * nz + k;
+    const double kzk = kz[k];
+    const double kk2 = (kxi * kxi + kyj * kyj) + kzk * kzk;
+    const Cp A = ld(a, e), B = ld(b, e), Cc = ld(c, e);
+    Cp sm = cadd(cscale(A, kxi), cscale(B, kyj));
+    sm = cadd(sm, cscale(Cc, kzk));
+    Cp dd;
+    if (kk2 == 0.0) {  // the (0,0,0) mode
+      dd = mkc<double>(0, 0);
+    } else {
+      dd = mkc<double>(sm.y / kk2, -sm.x / kk2);
+    }
+    d[e] = dd;
+    st(a, e, csub(A, ik(kxi, dd)));
+    st(b, e, csub(B, ik(kyj, dd)));
+    st(c, e, csub(Cc, ik(kzk, dd)));
+    const Cp c1 = cscale(ld(c, e), tmp);
+    s0 = cadd(s0, c1);
+    s1 = cadd(s1, cmul(c1, wtop[k]));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s0.x += __shfl_xor(s0.x, off);
+    s0.y += __shfl_xor(s0.y, off);
+    s1.x += __shfl_xor(s1.x, off);
+    s1.y += __shfl_xor(s1.y, off);
+  }
+  if (lane == 0) {
+    bc[2 * ln] = s0;
+    bc[2 * ln + 1] = s1;
+  }
+}
+
 template <typename TS>
 __global__ void k_axpy(cpx<TS>* __restrict__ y, const cpx<TS>* __restrict__ x, long nc,
                        double alpha) {
@@ -288,6 +340,18 @@ int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d) {
   hipLaunchKernelGGL(k_poisson_project<double>, dim3(nblk(long(p->wnc()))), dim3(256), 0, p->stream,
                      CS<double>(a), CS<double>(b), CS<double>(c), CS<double>(d), long(p->wnc()),
                      sidx(p), p->d_kx, p->d_ky, p->d_kz);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+
+int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d) {
+  if (!p->d_wtop) return fail(SPX_ERR_STATE, "plan has no wall table");
+  const long nlines = p->wlines();
+  ProfScope ps_(p, "poisson_project_bc", 7.0 * double(p->wnc()) * 16.0);
+  hipLaunchKernelGGL(k_poisson_project_bc<double>, dim3(unsigned((nlines + 3) / 4)), dim3(256), 0,
+                     p->stream, CS<double>(a), CS<double>(b), CS<double>(c), CS<double>(d),
+                     static_cast<Cp*>(p->bc), nlines, p->g.nz, p->g.ny, p->d_kx, p->d_ky,
+                     p->d_kz, static_cast<const Cp*>(p->d_wtop), 1.0 / double(p->g.nz));
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
