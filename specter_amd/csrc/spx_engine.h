@@ -253,6 +253,38 @@ __device__ __forceinline__ void stockham_pass(cpx<T>* x, int M, int Ns, unsigned
   __syncthreads();
 }
 
+// The same pass from src to a different buffer dst: no barrier is needed
+// between the loads and the stores, one after them.
+template <typename T, int R>
+__device__ __forceinline__ void stockham_pass_pp(const cpx<T>* src, cpx<T>* dst, int M, int Ns,
+                                                 unsigned magic, int lane, int tpl,
+                                                 const cpx<T>* tw) {
+  constexpr int MB = MaxB<R>::value;
+  const int nb = M / R;
+  const int tstep = M / (Ns * R);
+  const int s = M / R;
+#pragma unroll
+  for (int b = 0; b < MB; ++b) {
+    const int j = lane + b * tpl;
+    if (j < nb) {
+      cpx<T> v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = src[j + r * nb];
+      const int jm = Ns > 1 ? fastmod(j, Ns, magic) : 0;
+      if (Ns > 1) {
+        const int st = jm * tstep;
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[r * st]);
+      }
+      Dft<T, R>::run(v, tw, s);
+      const int base = (j - jm) * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
+    }
+  }
+  __syncthreads();
+}
+
 // Radix sets a kernel instance is compiled for (register pressure follows the
 // largest radix pass compiled in): powers of two; {2,3,4,5,7,8,9}; all.
 enum { RS_POW2 = 0, RS_SMOOTH = 1, RS_ALL = 2 };
@@ -280,6 +312,36 @@ __device__ void fft_lds(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw, i
     }
     Ns *= R;
   }
+}
+
+// Forward FFT of length f.M ping-ponging between a (input) and b; returns
+// the buffer that holds the result.  Every thread must call it.
+template <typename T, int RS>
+__device__ cpx<T>* fft_lds_pp(cpx<T>* a, cpx<T>* b, const LineFFT<cpx<T>>& f, const cpx<T>* tw,
+                              int lane) {
+  int Ns = 1;
+  for (int st = 0; st < f.nstage; ++st) {
+    const int R = f.radix[st];
+    const unsigned mg = f.magic[st];
+    if (R == 8) stockham_pass_pp<T, 8>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
+    else if (R == 4) stockham_pass_pp<T, 4>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
+    else if (R == 2) stockham_pass_pp<T, 2>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
+    else if constexpr (RS >= RS_SMOOTH) {
+      if (R == 3) stockham_pass_pp<T, 3>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
+      else if (R == 5) stockham_pass_pp<T, 5>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
+      else if (R == 7) stockham_pass_pp<T, 7>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
+      else if (R == 9) stockham_pass_pp<T, 9>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
+      else if constexpr (RS >= RS_ALL) {
+        if (R == 11) stockham_pass_pp<T, 11>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
+        else if (R == 13) stockham_pass_pp<T, 13>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
+      }
+    }
+    Ns *= R;
+    cpx<T>* t = a;
+    a = b;
+    b = t;
+  }
+  return a;
 }
 
 // Copy the tables into LDS (called by all threads; the caller's next
@@ -324,20 +386,22 @@ __device__ void line_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw, 
     __syncthreads();
     return;
   }
-  // Rader: M = n - 1, second buffer b = x + n
+  // Rader: M = n - 1, second buffer b = x + n; the two length-M FFTs
+  // ping-pong between b and x (x is free once b holds the permuted input);
+  // an even number of passes in total leaves the result in b
   cpx<T>* b = x + n;
   const int* pin = reinterpret_cast<const int*>(tw + 2 * M);
   const int* pout = pin + M;
   const cpx<T> x0 = x[0];
   for (int q = lane; q < M; q += tpl) b[q] = x[pin[q]];
   __syncthreads();
-  fft_lds<T, RS>(b, f, tw, lane);
+  cpx<T>* r = fft_lds_pp<T, RS>(b, x, f, tw, lane);
   cpx<T> X0 = mkc<T>(0, 0);
-  if (lane == 0) X0 = cadd(x0, b[0]);
-  for (int m = lane; m < M; m += tpl) b[m] = cconj(cmul(b[m], bfil[m]));
+  if (lane == 0) X0 = cadd(x0, r[0]);
+  for (int m = lane; m < M; m += tpl) r[m] = cconj(cmul(r[m], bfil[m]));
   __syncthreads();
-  fft_lds<T, RS>(b, f, tw, lane);
-  for (int p = lane; p < M; p += tpl) x[pout[p]] = cadd(x0, cconj(b[p]));
+  r = fft_lds_pp<T, RS>(r, r == b ? x : b, f, tw, lane);
+  for (int p = lane; p < M; p += tpl) x[pout[p]] = cadd(x0, cconj(r[p]));
   if (lane == 0) x[0] = X0;
   __syncthreads();
 }
