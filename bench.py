@@ -142,12 +142,32 @@ def _kernels(plan, sol, w, steps, pos):
     return rows, table
 
 
-def _roofline(rows):
+def _measured_traffic(workload, mode):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of the 256^3 workload (read + write, calibrated on a
+    plain copy); {} for other workloads or when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "hd256_hbm_traffic.md")
+    if workload != "hd256" or not os.path.exists(path):
+        return {}
+    out = {}
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[1] == mode:
+            try:
+                out[cells[0]] = (float(cells[2]) + float(cells[3])) * 2 ** 20
+            except ValueError:
+                pass
+    return out
+
+
+def _roofline(rows, traffic):
     name, n, tot_ms, tot_b = rows[0]
     avg_ms = tot_ms / n
     ach = (tot_b / n) / (avg_ms * 1e-3) / 1e9
+    t = traffic.get(name)
     return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": round(t) if t else None,
             "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": tot_b / n}
 
 
@@ -301,10 +321,11 @@ def main():
             line["field_rel_l2_vs_fp64"] = {"steps": args.err_steps,
                                             **{k: float(f"{v:.3e}") for k, v in err.items()}}
     if not args.no_profile:
-        line["roofline"] = _roofline(hr["rows"])
+        line["roofline"] = _roofline(hr["rows"], _measured_traffic(args.workload, head))
         line["kernels"] = hr["kernels"]
         if "fp64" in res and head == "mixed":
-            line["fp64"]["roofline"] = _roofline(res["fp64"]["rows"])
+            line["fp64"]["roofline"] = _roofline(res["fp64"]["rows"],
+                                                 _measured_traffic(args.workload, "fp64"))
             line["fp64"]["kernels"] = res["fp64"]["kernels"]
     if args.ensemble_runs > 0:
         line["ensemble"] = _ensemble(args.ensemble_runs, args.ensemble_slots,

@@ -19,9 +19,10 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-ProfScope::ProfScope(spx_plan* plan, const char* name, double bytes) : p(plan) {
+ProfScope::ProfScope(spx_plan* plan, const char* name0, double bytes) : p(plan) {
   if (!p || !p->prof.on) return;
   Profiler& pr = p->prof;
+  const std::string name = std::string(name0) + pr.suffix;
   int kind = -1;
   for (size_t i = 0; i < pr.names.size(); ++i)
     if (pr.names[i] == name) kind = int(i);
@@ -514,6 +515,7 @@ static int gradre_impl(spx_plan* p, const void* const v[3], void* const out[3]) 
   if (bnd) {
     // float64 boundary planes: z IFFT at the 2d planes, y and x passes and
     // products there, forward x, y passes (three components per launch)
+    ProfTag tag(p->prof, "_bnd");
     if ((rc = zfft_s2x_bnd(p, v, 3, p->XVb, p->XZb))) return rc;
     if ((rc = yfft(p, p->XVb, 0, 3 * d2, nullptr, p->XYb))) return rc;
     if ((rc = yfft(p, p->XZb, 0, 3 * d2))) return rc;
@@ -547,6 +549,7 @@ static int advect_impl(spx_plan* p, const void* const v[3], const void* th, void
       if ((rc = xc2r(p, p->XV[c], p->RV[c], np, 0, lo))) return rc;
     }
     if (bnd) {
+      ProfTag tag(p->prof, "_bnd");
       if ((rc = zfft_s2x_bnd(p, v, 3, p->XVb, p->XZb))) return rc;
       if ((rc = yfft(p, p->XVb, 0, 3 * d2))) return rc;
       if ((rc = xc2r(p, p->XVb, p->RVb, 3 * d2, 0))) return rc;
@@ -554,6 +557,7 @@ static int advect_impl(spx_plan* p, const void* const v[3], const void* th, void
   }
   if (bnd) {
     // the scalar's float64 boundary planes (component slot 3)
+    ProfTag tag(p->prof, "_bnd");
     const void* ths[1] = {th};
     void* xv3 = bnd_c(p, p->XVb, 3);
     void* xz3 = bnd_c(p, p->XZb, 3);

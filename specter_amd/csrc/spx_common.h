@@ -89,6 +89,7 @@ struct ProfRec {
 };
 struct Profiler {
   bool on = false;
+  const char* suffix = "";  // appended to the names of launches (e.g. "_bnd")
   std::vector<std::string> names;
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> pool;
@@ -185,6 +186,14 @@ int fail(int code, const std::string& msg);
                                           hipGetErrorString(e_));           \
   } while (0)
 #define SPX_CHECK_LAUNCH() SPX_HIP(hipGetLastError())
+
+// Tags the timing records of the launches in a scope with a name suffix.
+struct ProfTag {
+  spx::Profiler& pr;
+  const char* old;
+  ProfTag(spx::Profiler& prof, const char* sfx) : pr(prof), old(prof.suffix) { prof.suffix = sfx; }
+  ~ProfTag() { pr.suffix = old; }
+};
 
 // Sets the member count of the launch view for a scope.
 struct MemberView {

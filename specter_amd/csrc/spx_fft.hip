@@ -1171,7 +1171,7 @@ int zfft_s2x_bnd(spx_plan* p, const void* const* S, int ncomp, void* XVb, void* 
   for (int c = 0; c < 4; ++c) src.s[c] = c < ncomp ? C_<double>(S[c]) : nullptr;
   const long lines = p->wlines();
   const dim3 grid(unsigned((lines + 15) / 16), unsigned(ncomp));
-  ProfScope ps_(p, "z_s2x_bnd", double(ncomp) * double(p->wnc()) * 16.0);
+  ProfScope ps_(p, "z_s2x", double(ncomp) * double(p->wnc()) * 16.0);
   const int n = p->g.nz, nkx = p->nkx, ny = p->g.ny, mem = p->mem;
   const Cd* w1 = static_cast<const Cd*>(p->d_wbnd);
   const Cd* wt = w1 + n;
