@@ -869,9 +869,31 @@ int spx_substep(spx_plan* p, spx_state* s, int o) {
   const void* src[4] = {nullptr, nullptr, s->th, s->vz};
   const double coef[4] = {0.0, 0.0, q.xmom, q.xtemp};
   const double* nu[4] = {s->d_nu, s->d_nu, s->d_nu, s->d_kappa};
-  if ((rc = op_rk_update(p, bouss ? 4 : 3, v, c0, nl, f, bouss ? src : nullptr, coef, nu, q.dt,
-                         rmp, p->fp32)))
-    return rc;
+  if (!bouss) {
+    // HD: the RK update is applied as the wall kernel loads vx, vy and as the
+    // projection loads vz (one pass over the state less per component)
+    RkLoad rk;
+    std::memset(&rk, 0, sizeof(rk));
+    rk.nl32 = p->fp32 ? 1 : 0;
+    rk.kx = p->d_kx;
+    rk.ky = p->d_ky;
+    rk.kz = p->d_kz;
+    rk.fkx = p->d_fkx;
+    rk.fky = p->d_fky;
+    rk.fkz = p->d_fkz;
+    rk.dt = q.dt;
+    rk.rmp = rmp;
+    for (int c = 0; c < 2; ++c) rk.c[c] = RkLoadComp{c0[c], nl[c], f[c], s->d_nu};
+    double tau = 1.0 / double(o);
+    if (o != q.ord) tau = double(o + 1) * tau;
+    if ((rc = noslip_fused(p, s->vx, s->vy, s->pr, tau, q.v_zsta, q.v_zend, &rk))) return rc;
+    RkLoad rz = rk;
+    rz.c[0] = RkLoadComp{c0[2], nl[2], f[2], s->d_nu};
+    if ((rc = op_poisson_project_bc(p, s->vx, s->vy, s->vz, s->pr, &rz))) return rc;
+    if ((rc = proj_pressure(p, s->pr))) return rc;
+    return proj_harmonic(p, s->vx, s->vy, s->vz);
+  }
+  if ((rc = op_rk_update(p, 4, v, c0, nl, f, src, coef, nu, q.dt, rmp, p->fp32))) return rc;
   if ((rc = spx_v_imposebc_and_project(p, s->vx, s->vy, s->vz, s->pr, o, q.ord, q.v_zsta,
                                        q.v_zend)))
     return rc;

@@ -187,6 +187,22 @@ int fail(int code, const std::string& msg);
   } while (0)
 #define SPX_CHECK_LAUNCH() SPX_HIP(hipGetLastError())
 
+// The RK stage update of one component applied while a kernel loads it
+// (include/hd/hd_rkstep2.f90:11-32, same operation order as op_rk_update):
+//   v <- c0 + ((nu*(-kk2 v) - ((nl*fx)*fy)*fz + f)*dt)*rmp
+struct RkLoadComp {
+  const void* c0;       // float64 complex
+  const void* nl;       // nonlinear term, float64 or float32 complex (nl32)
+  const void* f;        // float64 complex forcing
+  const double* nu;     // diffusivity per (member, kx)
+};
+struct RkLoad {
+  RkLoadComp c[2];
+  int nl32;
+  const double *kx, *ky, *kz, *fkx, *fky, *fkz;
+  double dt, rmp;
+};
+
 // Tags the timing records of the launches in a scope with a name suffix.
 struct ProfTag {
   spx::Profiler& pr;
@@ -250,8 +266,9 @@ int xc2r(spx_plan* p, const void* X, void* R, int nplanes, int dirx = 0, bool lo
 int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, const void* u,
                const void* v, const void* w, void* out, bool lo = false, int nplanes = -1);
 // fused v_imposebc: goto_domain_w_boundaries + noslip_z + goto_3d_fourier on vx,vy
+// rk: the RK update of vx, vy (c[0], c[1]) is applied on load (null: none)
 int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
-                 const double vzsta[2], const double vzend[2]);
+                 const double vzsta[2], const double vzend[2], const RkLoad* rk = nullptr);
 // s_imposebc: z IFFT, 1/nz, th = 0 at both walls, continuation + z FFT
 int sconst_fused(spx_plan* p, void* th);
 // sol_project stages (bctarget = 1, Neumann potential)
@@ -281,7 +298,8 @@ int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, 
                  const double* const* nu, double dt, double rmp, bool nl_lo = false);
 int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d);
 // poisson_project plus the wall data bc of sol_project (replaces proj_bc)
-int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d);
+int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d,
+                          const RkLoad* rk = nullptr);  // rk: update of c (c[0]) on load
 int op_axpy_field(spx_plan* p, void* y, const void* x, double alpha);
 int op_copy_fields(spx_plan* p, int n, const void* const* src, void* const* dst);
 int op_real_div3(spx_plan* p, void* r, double nx, double ny, double nz);

@@ -406,6 +406,27 @@ __device__ void line_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw, 
   __syncthreads();
 }
 
+// RkLoad applied to component q's value v at element e = line*nz + k,
+// line = i*ny + j.
+__device__ __forceinline__ cpx<double> rk_on_load(const RkLoad& r, int q, cpx<double> v, long e,
+                                                  long i, int j, int k) {
+#pragma clang fp contract(off)
+  const RkLoadComp& c = r.c[q];
+  const double kk2 = (r.kx[i] * r.kx[i] + r.ky[j] * r.ky[j]) + r.kz[k] * r.kz[k];
+  cpx<double> nl;
+  if (r.nl32) {
+    const cpx<float> t = static_cast<const cpx<float>*>(c.nl)[e];
+    nl = mkc<double>(double(t.x), double(t.y));
+  } else {
+    nl = static_cast<const cpx<double>*>(c.nl)[e];
+  }
+  nl = cscale(cscale(cscale(nl, r.fkx[i]), r.fky[j]), r.fkz[k]);
+  const cpx<double> lap = cscale(v, -kk2);
+  const cpx<double> t =
+      cadd(csub(cscale(lap, c.nu[i]), nl), static_cast<const cpx<double>*>(c.f)[e]);
+  return cadd(static_cast<const cpx<double>*>(c.c0)[e], cscale(cscale(t, r.dt), r.rmp));
+}
+
 // FC-Gram Dirichlet continuation of nl lines held in LDS (fftp.fpp:755-773):
 // x[np+ii] = sum_jj dir(ii,jj) x[np-d+jj] + dir(C-1-ii,jj) x[d-1-jj],
 // accumulated in the reference's order, in float64.

@@ -549,7 +549,8 @@ __global__ void __launch_bounds__(256) k_noslip(cpx<TS>* __restrict__ vx, cpx<TS
                                                 int ny, int np, double invn, WallParams wp,
                                                 const double* __restrict__ kx,
                                                 const double* __restrict__ ky,
-                                                const double* __restrict__ dir, int C, int d) {
+                                                const double* __restrict__ dir, int C, int d,
+                                                RkLoad rk, int rk_on) {
 #pragma clang fp contract(off)
   SPX_PROLOGUE(Cd, f);
   const int n = f.n;
@@ -557,8 +558,16 @@ __global__ void __launch_bounds__(256) k_noslip(cpx<TS>* __restrict__ vx, cpx<TS
   cpx<TS>* V = comp ? vy : vx;
   const long ln = long(blockIdx.x) * f.nl + line;
   const bool ok = ln < nlines;
-  for (int k = lane; k < n; k += f.tpl)
-    xl[k] = ok ? cconj(ccast<double>(V[ln * n + k])) : mkc<double>(0, 0);
+  const long li = ok ? ln / ny : 0;
+  const int lj = ok ? int(ln - li * ny) : 0;
+  for (int k = lane; k < n; k += f.tpl) {
+    Cd v = mkc<double>(0, 0);
+    if (ok) {
+      v = ccast<double>(V[ln * n + k]);
+      if (rk_on) v = rk_on_load(rk, comp, v, ln * n + k, li, lj, k);
+    }
+    xl[k] = cconj(v);
+  }
   __syncthreads();
   line_dft<double, RS>(xl, f, tws, lane);
   for (int k = lane; k < np; k += f.tpl) xl[k] = cscale(cconj(xl[k]), invn);
@@ -1320,20 +1329,26 @@ int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, cons
 
 // ---- float64 boundary / projection / diagnostics kernels ----
 static int noslip_launch(spx_plan* p, void* vx, void* vy, const void* pr, const WallParams& wp,
-                         int ncomp, const char* name) {
+                         int ncomp, const char* name, const RkLoad* rk) {
   const LineFFTHost& f = p->fz;
   const long nlines = p->wlines();
   dim3 grid(unsigned((nlines + f.nl - 1) / f.nl), unsigned(ncomp));
-  ProfScope ps_(p, name, 2.0 * ncomp * double(p->wnc()) * 16.0);
+  RkLoad r;
+  std::memset(&r, 0, sizeof(r));
+  if (rk) r = *rk;
+  const double extra = rk ? (rk->nl32 ? 8.0 : 16.0) + 32.0 : 0.0;  // nl, c0, f reads
+  ProfScope ps_(p, rk ? "noslip_rk" : name,
+                ncomp * double(p->wnc()) * (2.0 * 16.0 + extra));
   SPX_LAUNCH_RS(f, k_noslip, double, , grid, blk(f), f.lds_bytes(16), p->stream, C_<double>(vx),
                 C_<double>(vy), C_<double>(pr), f.dev<Cd>(), nlines, p->g.ny, p->np,
-                1.0 / double(p->g.nz), wp, p->d_kx, p->d_ky, p->d_dir, p->g.Cz, p->g.oz);
+                1.0 / double(p->g.nz), wp, p->d_kx, p->d_ky, p->d_dir, p->g.Cz, p->g.oz, r,
+                rk ? 1 : 0);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 
 int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
-                 const double vzsta[2], const double vzend[2]) {
+                 const double vzsta[2], const double vzend[2], const RkLoad* rk) {
   WallParams wp;
   wp.tau = tau;
   wp.vb[0][0] = vzsta ? vzsta[0] : 0.0;
@@ -1342,14 +1357,14 @@ int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
   wp.vb[1][1] = vzend ? vzend[1] : 0.0;
   wp.nxny = double(p->g.nx * p->g.ny);
   wp.scalar = 0;
-  return noslip_launch(p, vx, vy, pr, wp, 2, "noslip");
+  return noslip_launch(p, vx, vy, pr, wp, 2, "noslip", rk);
 }
 
 int sconst_fused(spx_plan* p, void* th) {
   WallParams wp;
   std::memset(&wp, 0, sizeof(wp));
   wp.scalar = 1;
-  return noslip_launch(p, th, th, nullptr, wp, 1, "sconst");
+  return noslip_launch(p, th, th, nullptr, wp, 1, "sconst", nullptr);
 }
 
 int proj_bc(spx_plan* p, const void* c) {

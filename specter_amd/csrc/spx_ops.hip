@@ -6,6 +6,8 @@
 // their fields; the state is float64 in every plan, and only the nonlinear
 // terms read by the RK update are float32 in mixed-precision plans.  The
 // kernels are bound by HBM traffic, so float64 arithmetic costs nothing.
+#include <cstring>
+
 #include "spx_engine.h"
 
 namespace spx {
@@ -185,7 +187,7 @@ __global__ void __launch_bounds__(256) k_poisson_project_bc(
     cpx<TS>* __restrict__ a, cpx<TS>* __restrict__ b, cpx<TS>* __restrict__ c,
     Cp* __restrict__ d, Cp* __restrict__ bc, long nlines, int nz, int ny,
     const double* __restrict__ kx, const double* __restrict__ ky, const double* __restrict__ kz,
-    const Cp* __restrict__ wtop, double tmp) {
+    const Cp* __restrict__ wtop, double tmp, RkLoad rk, int rk_on) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
   const long ln = long(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -198,7 +200,9 @@ __global__ void __launch_bounds__(256) k_poisson_project_bc(
     const long e = ln * nz + k;
     const double kzk = kz[k];
     const double kk2 = (kxi * kxi + kyj * kyj) + kzk * kzk;
-    const Cp A = ld(a, e), B = ld(b, e), Cc = ld(c, e);
+    const Cp A = ld(a, e), B = ld(b, e);
+    Cp Cc = ld(c, e);
+    if (rk_on) Cc = rk_on_load(rk, 0, Cc, e, i, j, k);
     Cp sm = cadd(cscale(A, kxi), cscale(B, kyj));
     sm = cadd(sm, cscale(Cc, kzk));
     Cp dd;
@@ -344,14 +348,20 @@ int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d) {
   return SPX_OK;
 }
 
-int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d) {
+int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d, const RkLoad* rk) {
   if (!p->d_wtop) return fail(SPX_ERR_STATE, "plan has no wall table");
   const long nlines = p->wlines();
-  ProfScope ps_(p, "poisson_project_bc", 7.0 * double(p->wnc()) * 16.0);
+  RkLoad r;
+  std::memset(&r, 0, sizeof(r));
+  if (rk) r = *rk;
+  const double extra = rk ? (rk->nl32 ? 8.0 : 16.0) + 32.0 : 0.0;
+  ProfScope ps_(p, rk ? "poisson_project_bc_rk" : "poisson_project_bc",
+                double(p->wnc()) * (7.0 * 16.0 + extra));
   hipLaunchKernelGGL(k_poisson_project_bc<double>, dim3(unsigned((nlines + 3) / 4)), dim3(256), 0,
                      p->stream, CS<double>(a), CS<double>(b), CS<double>(c), CS<double>(d),
                      static_cast<Cp*>(p->bc), nlines, p->g.nz, p->g.ny, p->d_kx, p->d_ky,
-                     p->d_kz, static_cast<const Cp*>(p->d_wtop), 1.0 / double(p->g.nz));
+                     p->d_kz, static_cast<const Cp*>(p->d_wtop), 1.0 / double(p->g.nz), r,
+                     rk ? 1 : 0);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
