@@ -520,11 +520,9 @@ static int gradre_impl(spx_plan* p, const void* const v[3], void* const out[3]) 
     if ((rc = yfft(p, p->XVb, 0, 3 * d2, nullptr, p->XYb))) return rc;
     if ((rc = yfft(p, p->XZb, 0, 3 * d2))) return rc;
     if ((rc = xc2r(p, p->XVb, p->RVb, 3 * d2, 0))) return rc;
-    for (int c = 0; c < 3; ++c)
-      if ((rc = xgrad_prod(p, bnd_c(p, p->XVb, c), bnd_c(p, p->XYb, c), bnd_c(p, p->XZb, c),
-                           bnd_r(p, p->RVb, 0), bnd_r(p, p->RVb, 1), bnd_r(p, p->RVb, 2),
-                           bnd_r(p, p->ROb, c), false, d2)))
-        return rc;
+    if ((rc = xgrad_prod(p, p->XVb, p->XYb, p->XZb, bnd_r(p, p->RVb, 0), bnd_r(p, p->RVb, 1),
+                         bnd_r(p, p->RVb, 2), p->ROb, false, d2, 3)))
+      return rc;
     if ((rc = xr2c(p, p->ROb, p->XFb, 3 * d2))) return rc;
     if ((rc = yfft(p, p->XFb, 1, 3 * d2))) return rc;
   }

@@ -263,8 +263,11 @@ int xr2c(spx_plan* p, const void* R, void* X, int nplanes, bool lo = false);
 int xc2r(spx_plan* p, const void* X, void* R, int nplanes, int dirx = 0, bool lo = false);
 // gradient products of one component: out = ((u*Dx + v*Dy) + w*Dz)*tmp with
 // Dx = c2r(i kx YV), Dy = c2r(YY), Dz = c2r(XZ) (pseudospec_hd.f90:254-312 order)
+// ncomp > 1: components stored back to back (YV/YY/XZ blocks of nkx*ny*nplanes*mem
+// complex, out blocks of nx*ny*nplanes*mem real), one launch
 int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, const void* u,
-               const void* v, const void* w, void* out, bool lo = false, int nplanes = -1);
+               const void* v, const void* w, void* out, bool lo = false, int nplanes = -1,
+               int ncomp = 1);
 // fused v_imposebc: goto_domain_w_boundaries + noslip_z + goto_3d_fourier on vx,vy
 // rk: the RK update of vx, vy (c[0], c[1]) is applied on load (null: none)
 int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,

@@ -228,7 +228,9 @@ __global__ void __launch_bounds__(256) k_z_s2x_bnd(BndSrc src, cpx<double>* __re
   const long b = ok ? ln / nlines : 0, l = ok ? ln - b * nlines : 0;
   const long i = l / ny, jy = l - i * ny;
   const long o = ((long(comp) * mem + b) * D2) * nlines + jy * nkx + i;
-  // v, then (i kz) v: one accumulator set live at a time
+  // v, then (i kz) v: one accumulator set live at a time.  The top block
+  // uses exp(+2 pi i k (np-d+q)/n) = wt[k] exp(+2 pi i k q/n): both blocks
+  // share the twiddle recurrence, the top one on u_k = v_k wt[k].
   for (int kind = 0; kind < 2; ++kind) {
     cpx<double> acc[D2];
 #pragma unroll
@@ -237,18 +239,14 @@ __global__ void __launch_bounds__(256) k_z_s2x_bnd(BndSrc src, cpx<double>* __re
       for (int k = sub; k < n; k += LPL) {
         cpx<double> v = S[k];
         if (kind) v = ikmul(kz[k], v);
+        const cpx<double> u = cmul(v, wt[k]);
         const cpx<double> step = w1[k];
         cpx<double> t = mkc<double>(1.0, 0.0);
 #pragma unroll
         for (int q = 0; q < D; ++q) {
           acc[q] = cadd(acc[q], cmul(v, t));
-          t = cmul(t, step);
-        }
-        t = wt[k];
-#pragma unroll
-        for (int q = D; q < D2; ++q) {
-          acc[q] = cadd(acc[q], cmul(v, t));
-          t = cmul(t, step);
+          acc[D + q] = cadd(acc[D + q], cmul(u, t));
+          if (q + 1 < D) t = cmul(t, step);
         }
       }
     }
@@ -477,10 +475,16 @@ __global__ void __launch_bounds__(256) k_xgrad_prod(const cpx<T>* __restrict__ Y
                                                     const T* __restrict__ v,
                                                     const T* __restrict__ w, T* __restrict__ out,
                                                     LineFFT<cpx<T>> f, int nkx, long npairs,
-                                                    double tmpd, const double* __restrict__ kx) {
+                                                    double tmpd, const double* __restrict__ kx,
+                                                    long cstride, long rstride) {
 #pragma clang fp contract(off)
   using Cp = cpx<T>;
   SPX_PROLOGUE(Cp, f);
+  // blockIdx.y: component (YV, YY, XZ advance by cstride, out by rstride)
+  YV += blockIdx.y * cstride;
+  YY += blockIdx.y * cstride;
+  XZ += blockIdx.y * cstride;
+  out += blockIdx.y * rstride;
   const int nx = f.n;
   const T tmp = T(tmpd);
   const long p = long(blockIdx.x) * f.nl + line;
@@ -1305,26 +1309,28 @@ int xc2r(spx_plan* p, const void* X, void* R, int nplanes, int dirx, bool lo) {
 template <typename T>
 static int xgrad_prod_t(spx_plan* p, const void* YV, const void* YY, const void* XZ,
                         const void* u, const void* v, const void* w, void* out,
-                        const LineFFTHost& f, int nplanes) {
+                        const LineFFTHost& f, int nplanes, int ncomp) {
   const long npairs = long(p->g.ny) * nplanes * p->mem / 2;
-  dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
+  dim3 grid(unsigned((npairs + f.nl - 1) / f.nl), unsigned(ncomp));
+  const long cstride = long(p->nkx) * p->g.ny * nplanes * p->mem;
+  const long rstride = long(p->g.nx) * p->g.ny * nplanes * p->mem;
   const double t = double(p->g.nx) * double(p->g.ny) * double(p->g.nz);
-  const double rows = double(p->g.ny) * nplanes * p->mem;
+  const double rows = double(p->g.ny) * nplanes * p->mem * ncomp;
   ProfScope ps_(p, "xgrad_prod", rows * (3.0 * sizeof(cpx<T>) * p->nkx +
                                          4.0 * sizeof(T) * p->g.nx));
   SPX_LAUNCH_RS(f, k_xgrad_prod, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
                 C_<T>(YV), C_<T>(YY), C_<T>(XZ), static_cast<const T*>(u),
                 static_cast<const T*>(v), static_cast<const T*>(w), static_cast<T*>(out),
-                f.dev<cpx<T>>(), p->nkx, npairs, 1.0 / (t * t), p->d_kx);
+                f.dev<cpx<T>>(), p->nkx, npairs, 1.0 / (t * t), p->d_kx, cstride, rstride);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, const void* u,
-               const void* v, const void* w, void* out, bool lo, int nplanes) {
+               const void* v, const void* w, void* out, bool lo, int nplanes, int ncomp) {
   if (int rc = check_lo(p, lo)) return rc;
   if (nplanes < 0) nplanes = p->np;
-  return lo ? xgrad_prod_t<float>(p, YV, YY, XZ, u, v, w, out, p->fx32, nplanes)
-            : xgrad_prod_t<double>(p, YV, YY, XZ, u, v, w, out, p->fx, nplanes);
+  return lo ? xgrad_prod_t<float>(p, YV, YY, XZ, u, v, w, out, p->fx32, nplanes, ncomp)
+            : xgrad_prod_t<double>(p, YV, YY, XZ, u, v, w, out, p->fx, nplanes, ncomp);
 }
 
 // ---- float64 boundary / projection / diagnostics kernels ----
