@@ -263,7 +263,7 @@ def main():
     ap.add_argument("--ensemble-runs", type=int, default=96,
                     help="requests of the ensemble-serving measurement (0 = skip)")
     ap.add_argument("--ensemble-slots", type=int, default=16)
-    ap.add_argument("--ensemble-chunk", type=int, default=25)
+    ap.add_argument("--ensemble-chunk", type=int, default=5, help="RK steps of a class's first chunk")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the "

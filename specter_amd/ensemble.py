@@ -8,11 +8,15 @@ captured HIP graph per active-member count.  The loop
 
 1. admits every request that has arrived into a free slot of its class
    (creating the class when the HBM budget allows; otherwise the request
-   waits until memory is released),
-2. enqueues the next chunk of RK steps of every class on its stream (the
-   classes' work overlaps on the device),
-3. waits for the chunk, hands finished runs back (final fields and
-   diagnostics) and compacts the class so the active members stay leading.
+   waits until memory is released); a busy class takes new members at the
+   end of its current chunk,
+2. keeps every class stepping independently on its own stream: as soon as a
+   class's chunk of RK steps has finished (polled with an event, the host
+   never blocks on a busy stream), its finished runs are handed back (final
+   fields and diagnostics), the class is compacted so the active members
+   stay leading, waiting requests join, and the next chunk is enqueued.
+   Chunks are sized to about `chunk_seconds` of GPU time from the class's
+   measured rate, so small grids cycle fast and big ones do not hold them up.
    Classes without runs stay (with their captured graphs) until their memory
    is needed for another class.
 
@@ -112,7 +116,11 @@ class _Class:
         self.done: list[int] = []                  # steps done per active member
         self.admitted: list[float] = []
         self.bytes = state_bytes(self.plan, slots)
-        self.stepping = 0
+        self.stepping = 0          # steps of the chunk in flight (0: idle)
+        self.stepping_members = 0  # members the chunk in flight advances
+        self.event = torch.cuda.Event()
+        self.t_launch = 0.0
+        self.rate = None           # measured RK steps per second of this class
 
     @property
     def free(self) -> int:
@@ -135,18 +143,29 @@ class _Class:
         self.admitted.append(now)
         self.plan.set_active(len(self.runs))
 
-    def launch(self, chunk: int) -> int:
+    def launch(self, chunk: int, now: float) -> int:
         """Enqueue min(chunk, steps left of any member) steps; returns it."""
         k = min(chunk, min(r.nsteps - s for r, s in zip(self.runs, self.done)))
         self.sol.step(k)
+        self.event.record(self.stream)
         self.stepping = k
+        self.stepping_members = len(self.runs)
+        self.t_launch = now
         return k
 
+    def ready(self) -> bool:
+        return self.stepping > 0 and self.event.query()
+
     def finish_chunk(self, now_fn, keep_fields: bool):
-        """Wait for the chunk, return the finished runs' results and compact."""
+        """Return the finished runs' results after a completed chunk and compact."""
         self.stream.synchronize()
         k, self.stepping = self.stepping, 0
-        self.done = [s + k for s in self.done]
+        el = now_fn() - self.t_launch
+        if el > 0:
+            r = k / el
+            self.rate = r if self.rate is None else 0.5 * (self.rate + r)
+        m = self.stepping_members
+        self.done = [s + k if b < m else s for b, s in enumerate(self.done)]
         out = []
         b = 0
         while b < len(self.runs):
@@ -187,15 +206,17 @@ class EnsembleServer:
     hbm_budget: device bytes the server may hold; max_slots: members per
     class; chunk: RK steps enqueued per class between scheduling points."""
 
-    def __init__(self, hbm_budget: int | None = None, max_slots: int = 16, chunk: int = 25,
-                 keep_fields: bool = False):
+    def __init__(self, hbm_budget: int | None = None, max_slots: int = 16, chunk: int = 5,
+                 keep_fields: bool = False, chunk_seconds: float = 0.05, max_chunk: int = 100):
         import torch
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = int(free * 0.85)
         self.budget = int(hbm_budget)
         self.max_slots = max_slots
-        self.chunk = chunk
+        self.chunk = chunk                  # steps of a class's first chunk
+        self.chunk_seconds = chunk_seconds
+        self.max_chunk = max_chunk
         self.keep_fields = keep_fields
         self.classes: dict[tuple, _Class] = {}
         self.used = 0
@@ -237,6 +258,11 @@ class EnsembleServer:
         cls.admit(req, now)
         return True
 
+    def _chunk_for(self, c: _Class) -> int:
+        if c.rate is None:
+            return self.chunk
+        return int(max(1, min(self.max_chunk, round(c.rate * self.chunk_seconds))))
+
     def run(self, requests: list[RunRequest], realtime: bool = True) -> list[RunResult]:
         """Serve a stream of requests (arrival times relative to the start);
         with realtime=False every request is available at once."""
@@ -254,23 +280,38 @@ class EnsembleServer:
             t = now()
             while pending and (not realtime or pending[0].arrival <= t):
                 waiting.append(pending.pop(0))
+            progressed = False
+            # chunks that have finished: results, compaction
+            for c in list(self.classes.values()):
+                if c.ready():
+                    results.extend(c.finish_chunk(now, self.keep_fields))
+                    progressed = True
+            # admissions into idle classes (or new classes)
             still = []
             for r in waiting:
+                c = self.classes.get(r.key)
+                if c is not None and c.stepping:
+                    still.append(r)
+                    continue
                 same = sum(1 for q in waiting if q.key == r.key)
-                if not self._try_admit(r, now(), same):
+                if self._try_admit(r, now(), same):
+                    progressed = True
+                else:
                     still.append(r)
             waiting = still
-            active = [c for c in self.classes.values() if c.runs]
-            if not active:
-                if pending and realtime:
-                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.05)))
-                elif not pending and waiting:
+            # idle classes with runs start their next chunk
+            for c in self.classes.values():
+                if c.runs and not c.stepping:
+                    c.launch(self._chunk_for(c), now())
+                    progressed = True
+            if not progressed:
+                busy = any(c.stepping for c in self.classes.values())
+                if not busy and not pending and waiting:
                     raise RuntimeError("requests cannot be admitted within the HBM budget")
-                continue
-            for c in active:
-                c.launch(self.chunk)
-            for c in active:
-                results.extend(c.finish_chunk(now, self.keep_fields))
+                if busy:
+                    time.sleep(0.0002)
+                elif pending and realtime:
+                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.01)))
         self._evict_idle()
         return results
 
