@@ -153,6 +153,24 @@ int spx_divergence(spx_plan* p, const void* a, const void* b, const void* c,
 int spx_bouncheck_z(spx_plan* p, const void* a, const void* b, double* bot,
                     double* top);                              /* boundary_mod.fpp:681-801 */
 
+/* variance (pseudospec_phd.f90:116-176): <|a|^2> (kin=1) or <|kk2 a|^2>
+ * (kin=0) over physical z; product (:178-235): <Re(a conj b)>.  pscheck
+ * (:237-262) is variance(th,1), variance(th,0), product(th,fs). */
+int spx_variance(spx_plan* p, const void* a, int kin, double* out);
+int spx_product(spx_plan* p, const void* a, const void* b, double* out);
+/* Wall and solution diagnostics the reference does not compute (defined by
+ * this library): mean wall shear nu <d(a,b)/dz> as (tau_x, tau_y) at z = 0
+ * then z = Lz (4 values per member); Nusselt number 1 - Lz <d th/dz> at
+ * z = 0 and z = Lz for theta_total = 1 - z/Lz + th (2 values per member);
+ * max |div v| over the physical points (physical units); the horizontal
+ * shell spectrum of <|v|^2> over physical z, shells s - 1/2 <= |(kx,ky)|/Dkx
+ * < s + 1/2, nbins values per member. */
+int spx_wall_shear(spx_plan* p, const void* a, const void* b, double nu, double* out);
+int spx_nusselt(spx_plan* p, const void* th, double* out);
+int spx_divergence_max(spx_plan* p, const void* a, const void* b, const void* c, double* out);
+int spx_energy_spectrum(spx_plan* p, const void* a, const void* b, const void* c, int nbins,
+                        double* out);
+
 /* ---- device-resident solver state (include/hd, include/bouss, specter.fpp RK loop) ---- */
 typedef struct {
   int solver;        /* SPX_SOLVER_HD or SPX_SOLVER_BOUSS                      */

@@ -619,3 +619,80 @@ def vdiagnostic(g: Grid, st: HDState):
     tb, tt = bouncheck_z(g, st.vx, st.vy)
     nb, nt = bouncheck_z(g, st.vz)
     return dv, tb, tt, nb, nt
+
+
+# ---------------------------------------------------------------------------
+# Scalar diagnostics  (pseudospec_phd.f90:116-321)
+# ---------------------------------------------------------------------------
+def variance(g: Grid, a, kin: int) -> float:
+    """<|a|^2> (kin=1) or <|kk2 a|^2> (kin=0) over physical z
+    (pseudospec_phd.f90:116-176)."""
+    at = a if kin == 1 else g.kk2 * a
+    z = _zinv(g, at)
+    return _mean_phys(g, z.real ** 2 + z.imag ** 2)
+
+
+def product(g: Grid, a, b) -> float:
+    """<Re(a conj b)> over physical z (pseudospec_phd.f90:178-235)."""
+    za, zb = _zinv(g, a), _zinv(g, b)
+    return _mean_phys(g, (za * np.conj(zb)).real)
+
+
+def pscheck(g: Grid, th, fs):
+    """scalar.txt quantities (pseudospec_phd.f90:237-262): <th^2>,
+    <|kk2 th|^2>, <th fs>."""
+    return variance(g, th, 1), variance(g, th, 0), product(g, th, fs)
+
+
+# ---------------------------------------------------------------------------
+# Wall and solution diagnostics the reference does not compute (defined here)
+# ---------------------------------------------------------------------------
+def _plane_mean_dz(g: Grid, a):
+    """Horizontal mean of d a/dz at z = 0 and z = Lz in physical units:
+    the (kx, ky) = (0, 0) line of IFFT_z(i kz a) / (nx ny nz)."""
+    line = _ik(g.kz, a[0, 0, :])
+    z = np.fft.ifft(line) * g.nz          # unnormalised backward DFT
+    s = 1.0 / (g.nx * g.ny * g.nz)
+    return z[0].real * s, z[g.np_ - 1].real * s
+
+
+def wall_shear(g: Grid, a, b, nu: float):
+    """Mean wall shear stress nu <d(u_x, u_y)/dz> at z = 0 and z = Lz:
+    (tau_x bottom, tau_y bottom, tau_x top, tau_y top)."""
+    ab, at = _plane_mean_dz(g, a)
+    bb, bt = _plane_mean_dz(g, b)
+    return nu * ab, nu * bb, nu * at, nu * bt
+
+
+def nusselt(g: Grid, th):
+    """Nusselt number at z = 0 and z = Lz for theta_total = 1 - z/Lz + th
+    (heated bottom wall): Nu = 1 - Lz <d th/dz>_wall."""
+    b, t = _plane_mean_dz(g, th)
+    return 1.0 - g.Lz * b, 1.0 - g.Lz * t
+
+
+def divergence_max(g: Grid, a, b, c) -> float:
+    """max |div v| over the physical points, physical units."""
+    d = derivk(g, a, 1) + derivk(g, b, 2) + derivk(g, c, 3)
+    R = fftp3d_complex_to_real(g, d.copy())[: g.np_]
+    return float(np.abs(R).max() / (g.nx * g.ny * g.nz))
+
+
+def energy_spectrum_h(g: Grid, a, b, c, nbins: int):
+    """Horizontal shell spectrum of <|v|^2> over physical z: bin s collects
+    the (kx, ky) lines with s - 1/2 <= |(kx, ky)|/Dk < s + 1/2 (Dk = 1/Lx);
+    the bins sum to energy(kin=1) when nbins covers every shell."""
+    tmp = 1.0 / (float(g.nx) * float(g.ny) * float(g.nz)) ** 2 / float(g.nz - g.Cz)
+    R1 = None
+    for comp in (a, b, c):
+        z = _zinv(g, comp)
+        sq = z.real ** 2 + z.imag ** 2
+        R1 = sq if R1 is None else R1 + sq
+    lines = R1[:, :, : g.np_].sum(axis=2) * tmp
+    lines[1:] *= 2.0
+    kh = np.sqrt(g.kx[:, None] ** 2 + g.ky[None, :] ** 2) / g.Dkx
+    s = np.floor(kh + 0.5).astype(int)
+    out = np.zeros(nbins)
+    m = s < nbins
+    np.add.at(out, s[m], lines[m])
+    return out

@@ -1,6 +1,7 @@
 // C ABI of libspx: plan (FCPLAN), transforms (fftp.fpp), operators
 // (pseudospec_*.f90), boundary conditions (boundary/), diagnostics and the
 // device-resident RK solver state (include/hd, include/bouss, specter.fpp).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -342,7 +343,9 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
   const size_t nlines = size_t(ny) * p->nkx * nmembers;
   if ((rc = alloc_dev(p, &p->X, p->nc * nmembers * 16))) return bail(rc);
   if ((rc = alloc_dev(p, &p->bc, nlines * 2 * 16))) return bail(rc);
-  if ((rc = alloc_dev(p, reinterpret_cast<void**>(&p->part), nlines * 2 * sizeof(double))))
+  // per-line partials of the diagnostics (up to 4 per line), block maxima (64 per member)
+  const size_t nparts = std::max(nlines * 4, size_t(64) * nmembers);
+  if ((rc = alloc_dev(p, reinterpret_cast<void**>(&p->part), nparts * sizeof(double))))
     return bail(rc);
   *out = p;
   return SPX_OK;
@@ -739,6 +742,123 @@ int spx_bouncheck_z(spx_plan* p, const void* a, const void* b, double* bot, doub
   for (int m = 0; m < p->nact; ++m) {
     bot[m] = weighted_sum(p, ab, m) * (t * t);
     top[m] = weighted_sum(p, at, m) * (t * t);
+  }
+  return SPX_OK;
+}
+
+// ---- scalar and wall diagnostics ---------------------------------------------------
+int spx_variance(spx_plan* p, const void* a, int kin, double* out) {
+  SPX_REQ(p && a && out, "spx_variance: null argument");
+  SPX_REQ(kin == 0 || kin == 1, "spx_variance: kin must be 0 or 1");
+  MemberView mv(p, p->nact);
+  std::vector<double> acc(size_t(p->wlines()), 0.0);
+  int rc = sumsq_lines(p, one(a, kin == 1 ? 0 : 4), acc, 1.0);
+  if (rc) return rc;
+  for (int m = 0; m < p->nact; ++m) out[m] = weighted_sum(p, acc, m) * diag_tmp(p);
+  return SPX_OK;
+}
+
+int spx_product(spx_plan* p, const void* a, const void* b, double* out) {
+  SPX_REQ(p && a && b && out, "spx_product: null argument");
+  MemberView mv(p, p->nact);
+  const size_t nl = size_t(p->wlines());
+  int rc;
+  if ((rc = zline_dot(p, a, b, p->part))) return rc;
+  if ((rc = fetch_parts(p, nl))) return rc;
+  std::vector<double> acc(p->h_part.begin(), p->h_part.begin() + nl);
+  for (int m = 0; m < p->nact; ++m) out[m] = weighted_sum(p, acc, m) * diag_tmp(p);
+  return SPX_OK;
+}
+
+// horizontal means of d a/dz at z = 0 and z = Lz (physical units), per member
+static int plane_mean_dz(spx_plan* p, const void* a, std::vector<double>& bot,
+                         std::vector<double>& top) {
+  const size_t nl = size_t(p->g.ny) * p->nkx;
+  int rc;
+  if ((rc = zline_wall_values(p, one(a, 3), p->part))) return rc;
+  if ((rc = fetch_parts(p, 4 * nl * p->nact))) return rc;
+  const double s = 1.0 / (double(p->g.nx) * double(p->g.ny) * double(p->g.nz));
+  bot.assign(p->nact, 0.0);
+  top.assign(p->nact, 0.0);
+  for (int m = 0; m < p->nact; ++m) {
+    const size_t l0 = size_t(m) * nl;  // the (kx, ky) = (0, 0) line of member m
+    bot[m] = p->h_part[4 * l0] * s;
+    top[m] = p->h_part[4 * l0 + 2] * s;
+  }
+  return SPX_OK;
+}
+
+int spx_wall_shear(spx_plan* p, const void* a, const void* b, double nu, double* out) {
+  SPX_REQ(p && a && b && out, "spx_wall_shear: null argument");
+  SPX_REQ(p->g.Cz > 0, "spx_wall_shear: needs walls in z");
+  MemberView mv(p, p->nact);
+  std::vector<double> ab, at, bb, bt;
+  int rc;
+  if ((rc = plane_mean_dz(p, a, ab, at)) || (rc = plane_mean_dz(p, b, bb, bt))) return rc;
+  for (int m = 0; m < p->nact; ++m) {
+    out[4 * m] = nu * ab[m];
+    out[4 * m + 1] = nu * bb[m];
+    out[4 * m + 2] = nu * at[m];
+    out[4 * m + 3] = nu * bt[m];
+  }
+  return SPX_OK;
+}
+
+int spx_nusselt(spx_plan* p, const void* th, double* out) {
+  SPX_REQ(p && th && out, "spx_nusselt: null argument");
+  SPX_REQ(p->g.Cz > 0, "spx_nusselt: needs walls in z");
+  MemberView mv(p, p->nact);
+  std::vector<double> b, t;
+  int rc = plane_mean_dz(p, th, b, t);
+  if (rc) return rc;
+  for (int m = 0; m < p->nact; ++m) {
+    out[2 * m] = 1.0 - p->g.Lz * b[m];
+    out[2 * m + 1] = 1.0 - p->g.Lz * t[m];
+  }
+  return SPX_OK;
+}
+
+int spx_divergence_max(spx_plan* p, const void* a, const void* b, const void* c, double* out) {
+  SPX_REQ(p && a && b && c && out, "spx_divergence_max: null argument");
+  int rc;
+  if (!p->DS && (rc = alloc_dev(p, &p->DS, p->nc * p->nb * 16))) return rc;
+  if (!p->DR && (rc = alloc_dev(p, &p->DR, p->nr * p->nb * 8))) return rc;
+  MemberView mv(p, p->nact);
+  if ((rc = op_div_spec(p, a, b, c, p->DS))) return rc;
+  // physical planes of each member, members back to back
+  if ((rc = c2r3(p, p->DS, p->DR, 0, p->np))) return rc;
+  const long per = long(p->g.nx) * p->g.ny * p->np;
+  int nblocks = 0;
+  if ((rc = op_absmax(p, p->DR, per, per, p->part, &nblocks))) return rc;
+  if ((rc = fetch_parts(p, size_t(nblocks) * p->nact))) return rc;
+  const double s = 1.0 / (double(p->g.nx) * double(p->g.ny) * double(p->g.nz));
+  for (int m = 0; m < p->nact; ++m) {
+    double mx = 0.0;
+    for (int q = 0; q < nblocks; ++q) mx = std::max(mx, p->h_part[size_t(m) * nblocks + q]);
+    out[m] = mx * s;
+  }
+  return SPX_OK;
+}
+
+int spx_energy_spectrum(spx_plan* p, const void* a, const void* b, const void* c, int nbins,
+                        double* out) {
+  SPX_REQ(p && a && b && c && out && nbins > 0, "spx_energy_spectrum: bad argument");
+  MemberView mv(p, p->nact);
+  std::vector<double> acc(size_t(p->wlines()), 0.0);
+  int rc;
+  for (const void* q : {a, b, c})
+    if ((rc = sumsq_lines(p, one(q, 0), acc, 1.0))) return rc;
+  const long ny = p->g.ny, nl = long(p->g.ny) * p->nkx;
+  const double tmp = diag_tmp(p);
+  for (int m = 0; m < p->nact; ++m) {
+    double* o = out + size_t(m) * nbins;
+    std::fill(o, o + nbins, 0.0);
+    for (long l = 0; l < nl; ++l) {
+      const long i = l / ny, j = l - i * ny;
+      const double kh = std::sqrt(p->h_kx[i] * p->h_kx[i] + p->h_ky[j] * p->h_ky[j]) / p->Dkx;
+      const long sh = long(std::floor(kh + 0.5));
+      if (sh < nbins) o[sh] += (i == 0 ? 1.0 : 2.0) * acc[size_t(m) * nl + l] * tmp;
+    }
   }
   return SPX_OK;
 }

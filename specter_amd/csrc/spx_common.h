@@ -166,6 +166,8 @@ struct spx_plan {
   void* XFb = nullptr;      // forward x, y transforms of the products (4 components)
   void* d_wbnd = nullptr;   // exp(+2 pi i k / nz), then exp(+2 pi i k (np-d) / nz)
   void* d_wtop = nullptr;   // exp(+2 pi i k (np-1) / nz): the top wall of sol_project
+  void* DS = nullptr;       // diagnostics scratch: spectral (nb members), allocated on use
+  void* DR = nullptr;       // diagnostics scratch: real (nb members)
   void* bc = nullptr;       // wall data of sol_project, 2 complex per (kx,ky) line
   double* part = nullptr;   // per-line partial sums of the diagnostics
   std::vector<double> h_part;
@@ -282,12 +284,14 @@ int proj_harmonic(spx_plan* p, void* a, void* b, void* c);  // v -= grad(harmoni
 // mode 0: sum |u|^2 ; the combination is acc = m0(in0) (+/-) m1(in1) (+/-) m2(in2)
 struct LinComb {
   const void* in[3];
-  int dir[3];    // 0 none, 1 x, 2 y, 3 z
+  int dir[3];    // 0 none, 1 x, 2 y, 3 z (i k_dir), 4 kk2
   int sgn[3];    // +1 / -1 for terms 1,2 (term 0 always +1); 0 = term absent
 };
 int zline_sumsq(spx_plan* p, const LinComb& lc, double* line_out);
 int zline_dot(spx_plan* p, const void* a, const void* b, double* line_out);
 int zline_walls(spx_plan* p, const void* a, double* line_out /*2 per line*/);
+// IFFT_z(lincomb) at z = 0 and z = Lz: (re, im) bottom, (re, im) top per line
+int zline_wall_values(spx_plan* p, const LinComb& lc, double* line_out /*4 per line*/);
 
 // pointwise launchers (spx_ops.hip) -------------------------------------------
 int op_derivk(spx_plan* p, const void* a, void* b, int dir);
@@ -305,6 +309,10 @@ int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d,
                           const RkLoad* rk = nullptr);  // rk: update of c (c[0]) on load
 int op_axpy_field(spx_plan* p, void* y, const void* x, double alpha);
 int op_copy_fields(spx_plan* p, int n, const void* const* src, void* const* dst);
+// d <- i kx a + i ky b + i kz c (all active members)
+int op_div_spec(spx_plan* p, const void* a, const void* b, const void* c, void* d);
+// per member: block maxima of |r| over n values at member stride ms -> out[member][block]
+int op_absmax(spx_plan* p, const void* r, long n, long ms, double* out, int* nblocks);
 int op_real_div3(spx_plan* p, void* r, double nx, double ny, double nz);
 
 }  // namespace spx

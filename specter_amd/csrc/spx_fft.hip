@@ -749,14 +749,18 @@ struct LinCombDev {
   int sgn[3];
 };
 
+// dirn 1..3: (i k_dirn) v ; 4: kk2 v with kk2 = (kx^2 + ky^2) + kz^2
 __device__ __forceinline__ Cd deriv_apply(Cd v, int dirn, double kxi, double kyj, double kzk) {
+#pragma clang fp contract(off)
   if (dirn == 1) return ikmul(kxi, v);
   if (dirn == 2) return ikmul(kyj, v);
   if (dirn == 3) return ikmul(kzk, v);
+  if (dirn == 4) return cscale(v, (kxi * kxi + kyj * kyj) + kzk * kzk);
   return v;
 }
 
-// mode 0: sum_k<np |IFFT(lincomb)|^2 ; mode 1: Re(IFFT(a) conj IFFT(b)) ; mode 2: walls |.|^2
+// mode 0: sum_k<np |IFFT(lincomb)|^2 ; mode 1: Re(IFFT(a) conj IFFT(b)) ; mode 2: walls |.|^2 ;
+// mode 3: the wall values IFFT(lincomb) at z = 0 and z = Lz (4 doubles per line)
 template <typename TS, int RS>
 __global__ void __launch_bounds__(256) k_zline_reduce(LinCombDev<TS> lc,
                                                       const cpx<TS>* __restrict__ bsrc, int mode,
@@ -798,6 +802,17 @@ __global__ void __launch_bounds__(256) k_zline_reduce(LinCombDev<TS> lc,
       const Cd b = xl[0], t = xl[np - 1];
       out[2 * ln] = b.x * b.x + b.y * b.y;
       out[2 * ln + 1] = t.x * t.x + t.y * t.y;
+    }
+    return;
+  }
+  if (mode == 3) {
+    // the line holds conj(IFFT)
+    if (lane == 0 && ok) {
+      const Cd b = xl[0], t = xl[np - 1];
+      out[4 * ln] = b.x;
+      out[4 * ln + 1] = -b.y;
+      out[4 * ln + 2] = t.x;
+      out[4 * ln + 3] = -t.y;
     }
     return;
   }
@@ -1450,6 +1465,10 @@ int zline_dot(spx_plan* p, const void* a, const void* b, double* line_out) {
   lc.sgn[0] = 1;
   lc.sgn[1] = lc.sgn[2] = 0;
   return zline_launch(p, lc, b, 1, line_out);
+}
+
+int zline_wall_values(spx_plan* p, const LinComb& lc, double* line_out) {
+  return zline_launch(p, lc, nullptr, 3, line_out);
 }
 
 int zline_walls(spx_plan* p, const void* a, double* line_out) {

@@ -254,6 +254,40 @@ __global__ void k_copy_fields(CopyArgs args, int n, long nc) {
     if (q < n) args.dst[q][e] = args.src[q][e];
 }
 
+// d = ((i kx a) + (i ky b)) + (i kz c)  (divergence, pseudospec_hd.f90:1118-1235 order)
+template <typename TS>
+__global__ void k_div_spec(const cpx<TS>* __restrict__ a, const cpx<TS>* __restrict__ b,
+                           const cpx<TS>* __restrict__ c, Cp* __restrict__ d, long nc, SIdx ix,
+                           const double* __restrict__ kx, const double* __restrict__ ky,
+                           const double* __restrict__ kz) {
+#pragma clang fp contract(off)
+  const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= nc) return;
+  int k, j;
+  long i;
+  ix.split(e, k, j, i);
+  Cp s = cadd(ik(kx[i], ld(a, e)), ik(ky[j], ld(b, e)));
+  d[e] = cadd(s, ik(kz[k], ld(c, e)));
+}
+
+// block maxima of |r| over [0, n) of member blockIdx.y (members ms apart)
+__global__ void __launch_bounds__(256) k_absmax(const double* __restrict__ r, long n, long ms,
+                                                double* __restrict__ out) {
+  __shared__ double red[256];
+  const double* x = r + blockIdx.y * ms;
+  double m = 0.0;
+  for (long e = long(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+       e += long(gridDim.x) * blockDim.x)
+    m = fmax(m, fabs(x[e]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[long(blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
 // ---------------------------------------------------------------------------
 static inline unsigned nblk(long n) { return unsigned((n + 255) / 256); }
 static inline SIdx sidx(spx_plan* p) { return SIdx{p->g.nz, p->g.ny}; }
@@ -376,6 +410,25 @@ int op_copy_fields(spx_plan* p, int n, const void* const* src, void* const* dst)
   hipLaunchKernelGGL(k_copy_fields, dim3(nblk(long(p->wnc()))), dim3(256), 0, p->stream, args, n,
                      long(p->wnc()));
   SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+
+int op_div_spec(spx_plan* p, const void* a, const void* b, const void* c, void* d) {
+  ProfScope ps_(p, "div_spec", 4.0 * double(p->wnc()) * 16.0);
+  hipLaunchKernelGGL(k_div_spec<double>, dim3(nblk(long(p->wnc()))), dim3(256), 0, p->stream,
+                     CS<double>(a), CS<double>(b), CS<double>(c), CS<double>(d),
+                     long(p->wnc()), sidx(p), p->d_kx, p->d_ky, p->d_kz);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+
+int op_absmax(spx_plan* p, const void* r, long n, long ms, double* out, int* nblocks) {
+  const int nb = 64;  // blocks per member: out holds mem * 64 values
+  ProfScope ps_(p, "absmax", 8.0 * double(n) * p->mem);
+  hipLaunchKernelGGL(k_absmax, dim3(nb, unsigned(p->mem)), dim3(256), 0, p->stream,
+                     static_cast<const double*>(r), n, ms, out);
+  SPX_CHECK_LAUNCH();
+  *nblocks = nb;
   return SPX_OK;
 }
 

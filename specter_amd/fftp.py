@@ -274,6 +274,50 @@ class FCPlan:
                                            bot, top))
         return self._out(list(bot)), self._out(list(top))
 
+    def variance(self, a, kin: int):
+        """pseudospec_phd.f90:116-176."""
+        out = self._vals()
+        _l.check(self._lib.spx_variance(self._p, self._spec(a, "a"), kin, out))
+        return self._out(list(out))
+
+    def product(self, a, b):
+        """pseudospec_phd.f90:178-235."""
+        out = self._vals()
+        _l.check(self._lib.spx_product(self._p, self._spec(a, "a"), self._spec(b, "b"), out))
+        return self._out(list(out))
+
+    def _per_member(self, vals, k):
+        arr = np.array(vals).reshape(self.members, k)[: self.active]
+        return tuple(arr[0]) if self.members == 1 else arr
+
+    def wall_shear(self, a, b, nu: float):
+        """(tau_x, tau_y) at z = 0 then z = Lz: nu <d(a, b)/dz> over the wall plane."""
+        out = (c_double * (4 * self.members))()
+        _l.check(self._lib.spx_wall_shear(self._p, self._spec(a, "a"), self._spec(b, "b"),
+                                          c_double(nu), out))
+        return self._per_member(list(out), 4)
+
+    def nusselt(self, th):
+        """Nusselt number at z = 0 and z = Lz for theta_total = 1 - z/Lz + th."""
+        out = (c_double * (2 * self.members))()
+        _l.check(self._lib.spx_nusselt(self._p, self._spec(th, "th"), out))
+        return self._per_member(list(out), 2)
+
+    def divergence_max(self, a, b, c):
+        """max |div v| over the physical points, physical units."""
+        out = self._vals()
+        _l.check(self._lib.spx_divergence_max(self._p, self._spec(a, "a"), self._spec(b, "b"),
+                                              self._spec(c, "c"), out))
+        return self._out(list(out))
+
+    def energy_spectrum(self, a, b, c, nbins: int):
+        """Horizontal shell spectrum of <|v|^2> over physical z (nbins shells)."""
+        out = (c_double * (nbins * self.members))()
+        _l.check(self._lib.spx_energy_spectrum(self._p, self._spec(a, "a"), self._spec(b, "b"),
+                                               self._spec(c, "c"), nbins, out))
+        arr = np.array(list(out)).reshape(self.members, nbins)[: self.active]
+        return arr[0] if self.members == 1 else arr
+
     # -- grid values (host copies of what the plan uses) --------------------------------
     def grid_z(self) -> np.ndarray:
         if self.Cz > 0:

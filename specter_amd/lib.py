@@ -72,6 +72,12 @@ _SIGS = {
     "spx_substep": [_P, _P, c_int],
     "spx_step": [_P, _P, c_int],
     "spx_state_set_graph": [_P, c_int],
+    "spx_variance": [_P, _P, c_int, POINTER(c_double)],
+    "spx_product": [_P, _P, _P, POINTER(c_double)],
+    "spx_wall_shear": [_P, _P, _P, c_double, POINTER(c_double)],
+    "spx_nusselt": [_P, _P, POINTER(c_double)],
+    "spx_divergence_max": [_P, _P, _P, _P, POINTER(c_double)],
+    "spx_energy_spectrum": [_P, _P, _P, _P, c_int, POINTER(c_double)],
     "spx_plan_create_batch": [POINTER(SpxGrid), POINTER(c_double), POINTER(c_double), c_int, _P,
                               POINTER(_P)],
     "spx_plan_get_members": [_P, POINTER(c_int), POINTER(c_int)],

@@ -142,6 +142,23 @@ class Solver:
         nb, nt = p.bouncheck_z(vz)
         return dv, tb, tt, nb, nt
 
+    def pscheck(self):
+        """scalar.txt quantities (pseudospec_phd.f90:237-262)."""
+        p = self.plan
+        th, fs = self.field("th"), self.field("fs")
+        return p.variance(th, 1), p.variance(th, 0), p.product(th, fs)
+
+    def wall_diagnostics(self):
+        """Wall shear (nu <du_t/dz> at both walls), max |div v| and, for the
+        Boussinesq solver, the Nusselt number at both walls."""
+        p = self.plan
+        vx, vy, vz = self.field("vx"), self.field("vy"), self.field("vz")
+        out = {"wall_shear": p.wall_shear(vx, vy, self.params.nu),
+               "divergence_max": p.divergence_max(vx, vy, vz)}
+        if self.bouss:
+            out["nusselt"] = p.nusselt(self.field("th"))
+        return out
+
     def real_field(self, name: str, member: int | None = None) -> np.ndarray:
         """Physical-space field (c2r / (nx ny nz)), physical planes only."""
         p = self.plan

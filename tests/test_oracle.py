@@ -338,3 +338,69 @@ def test_divergence_kat():
                              * np.sin(8 * Y) + 0.4 * np.sin(4 * X) * np.cos(8 * Y)) * np.ones(g.real_shape)
     ref = np.mean(div[: g.np_] ** 2)
     assert abs(O.divergence(g, a, b, c) - ref) / ref < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# scalar (pseudospec_phd.f90) and wall diagnostics
+# ---------------------------------------------------------------------------
+def test_variance_product_closed_forms():
+    g = grid(32, 32, 64 + 25, 25, 5)
+    X, Y, Z = g.x[None, None, :], g.y[None, :, None], g.z[:, None, None]
+    ones = np.ones(g.real_shape)
+    th = np.cos(2 * X) * np.cos(3 * Y) * ones           # constant in z
+    S = O.fftp3d_real_to_complex(g, th.copy())
+    zero = np.zeros_like(S)
+    # kin=1 is the one-component energy; product(a, a) is the same sum
+    assert O.variance(g, S, 1) == O.energy(g, S, zero, zero, 1)
+    assert abs(O.product(g, S, S) - O.variance(g, S, 1)) < 1e-15
+    assert abs(O.variance(g, S, 1) - 0.25) < 1e-12
+    # kin=0: kk2 = 2^2 + 3^2 on a z-constant field (its continuation is flat)
+    assert abs(O.variance(g, S, 0) - 169 * 0.25) / (169 * 0.25) < 1e-6
+    # product of two different modes vanishes, of a mode with itself scaled
+    other = O.fftp3d_real_to_complex(g, (np.sin(X) * ones).copy())
+    assert abs(O.product(g, S, other)) < 1e-14
+    assert abs(O.product(g, S, 3 * S) - 0.75) < 1e-12
+    # pscheck = (variance 1, variance 0, product)
+    assert O.pscheck(g, S, other) == (O.variance(g, S, 1), O.variance(g, S, 0),
+                                      O.product(g, S, other))
+
+
+def test_wall_shear_nusselt_divmax_closed_forms():
+    g = grid(32, 32, 64 + 25, 25, 5)
+    X, Y, Z = g.x[None, None, :], g.y[None, :, None], g.z[:, None, None]
+    ones = np.ones(g.real_shape)
+    Lz = g.Lz
+    # u_x = 2 z (1 + 0.3 cos x): mean shear nu*2 at both walls (cos x averages out)
+    u = 2 * Z * (1 + 0.3 * np.cos(X)) * ones
+    v = -0.5 * Z * Z * ones                             # d/dz = -z: 0 at bottom, -Lz at top
+    Su = O.fftp3d_real_to_complex(g, u.copy())
+    Sv = O.fftp3d_real_to_complex(g, v.copy())
+    tb_x, tb_y, tt_x, tt_y = O.wall_shear(g, Su, Sv, 0.01)
+    assert abs(tb_x - 0.02) < 1e-8 and abs(tt_x - 0.02) < 1e-8
+    assert abs(tb_y) < 1e-8 and abs(tt_y + 0.01 * Lz) < 1e-8
+    # theta' = a z (Lz - z): d/dz = a Lz at z=0, -a Lz at z=Lz
+    a = 0.7
+    th = a * Z * (Lz - Z) * np.cos(0 * X) * ones
+    nb, nt = O.nusselt(g, O.fftp3d_real_to_complex(g, th.copy()))
+    assert abs(nb - (1 - a * Lz * Lz)) < 1e-8 and abs(nt - (1 + a * Lz * Lz)) < 1e-8
+    # div of (sin 2x, 0, 0) is 2 cos 2x: max 2
+    S1 = O.fftp3d_real_to_complex(g, (np.sin(2 * X) * ones).copy())
+    zero = np.zeros_like(S1)
+    assert abs(O.divergence_max(g, S1, zero, zero) - 2.0) < 1e-12
+    # a planted sign error in the z derivative flips the shear
+    assert O.wall_shear(g, -Su, Sv, 0.01)[0] < 0
+
+
+def test_energy_spectrum_h():
+    g = grid(32, 32, 41, 25, 5)
+    X, Y = g.x[None, None, :], g.y[None, :, None]
+    ones = np.ones(g.real_shape)
+    S = O.fftp3d_real_to_complex(g, (np.cos(3 * X) * ones).copy())
+    zero = np.zeros_like(S)
+    sp = O.energy_spectrum_h(g, S, zero, zero, 24)
+    assert abs(sp[3] - 0.5) < 1e-13 and abs(sp.sum() - sp[3]) < 1e-13
+    # every shell: the bins add up to the energy
+    rng = np.random.default_rng(3)
+    comps = [O.fftp3d_real_to_complex(g, rng.standard_normal(g.real_shape)) for _ in range(3)]
+    sp = O.energy_spectrum_h(g, *comps, 40)
+    assert abs(sp.sum() - O.energy(g, *comps, 1)) / O.energy(g, *comps, 1) < 1e-12
