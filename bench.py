@@ -34,6 +34,10 @@ WORKLOADS = {
     "hd256": dict(nx=256, ny=256, nzp=256, C=25, d=5, ord=4, nu=5e-4, dt=5e-4, forcing=True,
                   label="HD 256x256x(256+25) FC-Gram d=5 C=25, RK4, nu=5e-4, dt=5e-4, "
                         "forcing 0.5(sin2y,sin2x,0)sin(pi z)"),
+    # HD 1024^3 (one GPU's share of a production run) d=5 C=25 RK4 nu=1e-4, forced
+    "hd1024": dict(nx=1024, ny=1024, nzp=1024, C=25, d=5, ord=4, nu=1e-4, dt=1e-4, forcing=True,
+                   label="HD 1024x1024x(1024+25) FC-Gram d=5 C=25, RK4, nu=1e-4, dt=1e-4, "
+                         "forcing 0.5(sin2y,sin2x,0)sin(pi z)"),
     # HD 64^3 d=5 C=25 RK2 nu=2e-3 dt=1e-3, unforced
     "hd64": dict(nx=64, ny=64, nzp=64, C=25, d=5, ord=2, nu=2e-3, dt=1e-3, forcing=False,
                  label="HD 64x64x(64+25) FC-Gram d=5 C=25, RK2, nu=2e-3, dt=1e-3"),
@@ -261,6 +265,98 @@ def _ensemble(nruns, max_slots, chunk):
     }
 
 
+def _log(msg):
+    print(f"[bench] {time.strftime('%H:%M:%S')} {msg}", file=sys.stderr, flush=True)
+
+
+def _bench_large(args, w):
+    """1024^3: fields generated on the GPU; the mixed run and the float64
+    run do not fit together, so they run one after the other from the same
+    initial state (kept on the host); each runs --err-steps full steps for
+    the error comparison, then is timed."""
+    import numpy as np
+    import torch
+    from specter_amd import ic
+    from specter_amd.lib import SPX_FP32_MIXED, SPX_FP64, SpxError
+    nx, ny, nz, C = w["nx"], w["ny"], w["nzp"] + w["C"], w["C"]
+    npts = nx * ny * w["nzp"]
+    names = ("vx", "vy", "vz", "fx", "fy", "fz")
+    _log("initial fields on the device")
+    u = ic.spectrum_peaked_velocity_device(nx, ny, nz, C, 1000)
+    f = ic.large_scale_forcing_device(nx, ny, nz, C)
+    res, host_ic, diag = {}, None, {}
+    for mode in ("mixed", "fp64"):
+        prec = SPX_FP32_MIXED if mode == "mixed" else SPX_FP64
+        try:
+            plan = _plan(w, prec)
+            from specter_amd import Solver, SolverParams
+            sol = Solver(plan, SolverParams(solver="HD", ord=w["ord"], dt=w["dt"], nu=w["nu"]))
+        except SpxError as e:
+            res[mode] = {"error": str(e)}
+            _log(f"{mode}: {e}")
+            continue
+        if host_ic is None:
+            for name, arr in zip(names, (*u, *f)):
+                sol.set_real_field(name, arr)
+            del u, f
+            torch.cuda.empty_cache()
+            host_ic = {n: sol.spectral(n) for n in names}
+        else:
+            for n in names:
+                sol.field(n).copy_(torch.from_numpy(host_ic[n]))
+            plan.sync()
+        _log(f"{mode}: {args.err_steps} steps")
+        sol.step(args.err_steps)
+        plan.sync()
+        vx, vy, vz = sol.field("vx"), sol.field("vy"), sol.field("vz")
+        diag[mode] = {"spectrum": np.array(plan.energy_spectrum(vx, vy, vz, nx // 2)),
+                      "dissipation": w["nu"] * plan.energy(vx, vy, vz, 0),
+                      "wall_shear": np.array(plan.wall_shear(vx, vy, w["nu"])),
+                      "fields": {n: sol.spectral(n) for n in ("vx", "vy", "vz")}}
+        _log(f"{mode}: timing")
+        ms, wall, pos = _time(sol, w, args.steps, args.warmup)
+        r = {"ms_per_step": ms, "value": npts / (ms * 1e-3) / 1e6}
+        if not args.no_profile:
+            r["rows"], r["kernels"] = _kernels(plan, sol, w, args.steps, pos)
+        res[mode] = r
+        sol.close()
+        plan.close()
+        del sol, plan
+        torch.cuda.empty_cache()
+    hr = res["mixed"]
+    line = {
+        "metric": "RK substep Mpts/s (physical points per RK stage per second), FP32-mixed",
+        "value": round(hr["value"], 3), "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(hr["ms_per_step"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 nonlinear term / f64 state, continuation, pressure, walls",
+        "precision_mode": "mixed", "data": "synthetic (seeded spectrum-peaked solenoidal IC "
+        "generated on the GPU x sin(pi z/Lz))",
+        "config": {"workload": w["label"], "grid": [nx, ny, nz], "physical_points": npts,
+                   "parallelism": "single GPU"},
+    }
+    if not args.no_profile:
+        line["roofline"] = _roofline(hr["rows"], {})
+        line["kernels"] = hr["kernels"]
+    f64 = res.get("fp64", {})
+    if "ms_per_step" in f64:
+        line["fp64"] = {"value": round(f64["value"], 3), "ms_per_step": round(f64["ms_per_step"], 4)}
+        if not args.no_profile:
+            line["fp64"]["kernels"] = f64["kernels"]
+        line["speedup_vs_fp64"] = round(f64["ms_per_step"] / hr["ms_per_step"], 3)
+        a, b = diag["mixed"], diag["fp64"]
+        err = {"steps": args.err_steps}
+        for n in ("vx", "vy", "vz"):
+            err[n] = float(f"{np.linalg.norm(a['fields'][n] - b['fields'][n]) / np.linalg.norm(b['fields'][n]):.3e}")
+        err["energy_spectrum"] = float(f"{np.linalg.norm(a['spectrum'] - b['spectrum']) / np.linalg.norm(b['spectrum']):.3e}")
+        err["dissipation"] = float(f"{abs(a['dissipation'] - b['dissipation']) / abs(b['dissipation']):.3e}")
+        err["wall_shear"] = float(f"{np.abs(a['wall_shear'] - b['wall_shear']).max() / np.abs(b['wall_shear']).max():.3e}")
+        line["error_vs_fp64"] = err
+    else:
+        line["fp64"] = f64
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,6 +380,9 @@ def main():
         sys.exit(2)
     from specter_amd.lib import SPX_FP32_MIXED, SPX_FP64
     w = WORKLOADS[args.workload]
+    if args.workload == "hd1024":
+        _bench_large(args, w)
+        return
     u, f = _fields(w)
     npts = w["nx"] * w["ny"] * w["nzp"]
     ntot = w["nx"] * w["ny"] * (w["nzp"] + w["C"])

@@ -36,6 +36,9 @@ __device__ __forceinline__ cpx<TO> ccast(cpx<TI> v) { return mkc<TO>(TO(v.x), TO
   (void)xl;
 
 constexpr int KEEP = kStage;  // line elements a thread can hold in registers (n <= KEEP*tpl)
+// Workgroups hold nl lines of tpl threads: 256 threads, or one line of up to
+// 512 threads for the longest lines (e.g. Bluestein length 2100 for nz = 1049).
+constexpr int kMaxBlock = 512;
 
 // XCD-aware tile order of a 2D grid.  Workgroups are handed to the 8 XCDs
 // (each with its own L2) round-robin by linear id; the kx tiles of the
@@ -58,7 +61,7 @@ __device__ __forceinline__ void xcd_tile(int& bx, int& by) {
 // z transforms in place on S (contiguous lines)
 // ---------------------------------------------------------------------------
 template <typename T, int RS>
-__global__ void __launch_bounds__(256) k_z_inplace(cpx<T>* __restrict__ S, long nlines,
+__global__ void __launch_bounds__(kMaxBlock) k_z_inplace(cpx<T>* __restrict__ S, long nlines,
                                                    LineFFT<cpx<T>> f, int np, int forward,
                                                    int cont, double scale,
                                                    const double* __restrict__ dir, int C,
@@ -125,7 +128,7 @@ __global__ void k_fc_continue(cpx<T>* __restrict__ S, long nlines, int n, int np
 // is evaluated from them, so the float32 roundoff of the other planes is not
 // amplified by the continuation.
 template <typename T, int RS, bool BND>
-__global__ void __launch_bounds__(256) k_z_x2s(const cpx<T>* __restrict__ X, cpx<T>* __restrict__ S,
+__global__ void __launch_bounds__(kMaxBlock) k_z_x2s(const cpx<T>* __restrict__ X, cpx<T>* __restrict__ S,
                                                LineFFT<cpx<T>> f, int nkx, int ny, int np,
                                                int nofft, const double* __restrict__ dir, int C,
                                                int d, long xstride,
@@ -274,7 +277,7 @@ __global__ void __launch_bounds__(256) k_z_s2x_bnd(BndSrc src, cpx<double>* __re
 // blockIdx.z = member: S advances by n*ny*nkx, X and X2 by nkx*ny*nzout.
 // ---------------------------------------------------------------------------
 template <typename T, int RS, bool DUAL, typename TI>
-__global__ void __launch_bounds__(256) k_z_s2x(const cpx<TI>* __restrict__ S, cpx<T>* __restrict__ X,
+__global__ void __launch_bounds__(kMaxBlock) k_z_s2x(const cpx<TI>* __restrict__ S, cpx<T>* __restrict__ X,
                                                cpx<T>* __restrict__ X2, LineFFT<cpx<T>> f,
                                                int nkx, int ny, int nzout, int dirn, int nofft,
                                                const double* __restrict__ kx,
@@ -338,7 +341,7 @@ __global__ void __launch_bounds__(256) k_z_s2x(const cpx<TI>* __restrict__ S, cp
 // DUAL (backward): Y2 <- y IFFT of (i ky) Xin from the same load.
 // ---------------------------------------------------------------------------
 template <typename T, int RS, bool DUAL>
-__global__ void __launch_bounds__(256) k_y(const cpx<T>* Xin, cpx<T>* Xout,
+__global__ void __launch_bounds__(kMaxBlock) k_y(const cpx<T>* Xin, cpx<T>* Xout,
                                            cpx<T>* __restrict__ Y2, LineFFT<cpx<T>> f, int nkx,
                                            int ny, int forward, const double* __restrict__ ky) {
   using Cp = cpx<T>;
@@ -389,7 +392,7 @@ __global__ void __launch_bounds__(256) k_y(const cpx<T>* Xin, cpx<T>* Xout,
 // x real -> half complex: rows 2p, 2p+1 packed as one complex line
 // ---------------------------------------------------------------------------
 template <typename T, int RS>
-__global__ void __launch_bounds__(256) k_xr2c(const T* __restrict__ R, cpx<T>* __restrict__ X,
+__global__ void __launch_bounds__(kMaxBlock) k_xr2c(const T* __restrict__ R, cpx<T>* __restrict__ X,
                                               LineFFT<cpx<T>> f, int nkx, long npairs) {
   using Cp = cpx<T>;
   SPX_PROLOGUE(Cp, f);
@@ -443,7 +446,7 @@ __device__ __forceinline__ void load_c2r_pair(cpx<T>* xl, const cpx<T>* __restri
 }
 
 template <typename T, int RS>
-__global__ void __launch_bounds__(256) k_xc2r(const cpx<T>* __restrict__ X, T* __restrict__ R,
+__global__ void __launch_bounds__(kMaxBlock) k_xc2r(const cpx<T>* __restrict__ X, T* __restrict__ R,
                                               LineFFT<cpx<T>> f, int nkx, long npairs, int dirx,
                                               const double* __restrict__ kx) {
   using Cp = cpx<T>;
@@ -468,7 +471,7 @@ __global__ void __launch_bounds__(256) k_xc2r(const cpx<T>* __restrict__ X, T* _
 // LDS one after the other and accumulated in registers
 // (pseudospec_hd.f90:254-312, pseudospec_phd.f90:63-108).
 template <typename T, int RS>
-__global__ void __launch_bounds__(256) k_xgrad_prod(const cpx<T>* __restrict__ YV,
+__global__ void __launch_bounds__(kMaxBlock) k_xgrad_prod(const cpx<T>* __restrict__ YV,
                                                     const cpx<T>* __restrict__ YY,
                                                     const cpx<T>* __restrict__ XZ,
                                                     const T* __restrict__ u,
@@ -548,7 +551,7 @@ struct WallParams {
 };
 
 template <typename TS, int RS>
-__global__ void __launch_bounds__(256) k_noslip(cpx<TS>* __restrict__ vx, cpx<TS>* __restrict__ vy,
+__global__ void __launch_bounds__(kMaxBlock) k_noslip(cpx<TS>* __restrict__ vx, cpx<TS>* __restrict__ vy,
                                                 const Cd* __restrict__ pr, LFd f, long nlines,
                                                 int ny, int np, double invn, WallParams wp,
                                                 const double* __restrict__ kx,
@@ -602,7 +605,7 @@ __global__ void __launch_bounds__(256) k_noslip(cpx<TS>* __restrict__ vx, cpx<TS
 // sol_project (boundary_mod.fpp:197-402), bctarget = 1, bczsta = bczend = 0
 // bc(1:2) <- IFFT_z(c * (1/nz)) at z = 0 and z = Lz   (:262-352)
 template <typename TS, int RS>
-__global__ void __launch_bounds__(256) k_proj_bc(const cpx<TS>* __restrict__ c, Cd* __restrict__ bc,
+__global__ void __launch_bounds__(kMaxBlock) k_proj_bc(const cpx<TS>* __restrict__ c, Cd* __restrict__ bc,
                                                  LFd f, long nlines, int np, double tmp) {
   SPX_PROLOGUE(Cd, f);
   const int n = f.n;
@@ -636,7 +639,7 @@ __device__ __forceinline__ Coef neumann_coef(Cd b1, Cd b2, double kh, double Lz,
 
 // d <- IFFT_z(d)*tmp + a_harm(z)  (:354-381 with laplace_z :631-675); d is float64
 template <int RS>
-__global__ void __launch_bounds__(256) k_proj_pressure(Cd* __restrict__ dd,
+__global__ void __launch_bounds__(kMaxBlock) k_proj_pressure(Cd* __restrict__ dd,
                                                        const Cd* __restrict__ bc, LFd f,
                                                        long nlines, int ny, double tmp, double Lz,
                                                        const double* __restrict__ khom,
@@ -670,7 +673,7 @@ __global__ void __launch_bounds__(256) k_proj_pressure(Cd* __restrict__ dd,
 // harmonic correction: A = FFT_z(cont(a_harm)), B = FFT_z(cont(d a_harm/dz));
 // vx -= i kx A, vy -= i ky A, vz -= B   (:383-399).  Two buffers per line.
 template <typename TS, int RS>
-__global__ void __launch_bounds__(256) k_proj_harmonic(cpx<TS>* __restrict__ va,
+__global__ void __launch_bounds__(kMaxBlock) k_proj_harmonic(cpx<TS>* __restrict__ va,
                                                        cpx<TS>* __restrict__ vb,
                                                        cpx<TS>* __restrict__ vc,
                                                        const Cd* __restrict__ bc, LFd f,
@@ -762,7 +765,7 @@ __device__ __forceinline__ Cd deriv_apply(Cd v, int dirn, double kxi, double kyj
 // mode 0: sum_k<np |IFFT(lincomb)|^2 ; mode 1: Re(IFFT(a) conj IFFT(b)) ; mode 2: walls |.|^2 ;
 // mode 3: the wall values IFFT(lincomb) at z = 0 and z = Lz (4 doubles per line)
 template <typename TS, int RS>
-__global__ void __launch_bounds__(256) k_zline_reduce(LinCombDev<TS> lc,
+__global__ void __launch_bounds__(kMaxBlock) k_zline_reduce(LinCombDev<TS> lc,
                                                       const cpx<TS>* __restrict__ bsrc, int mode,
                                                       double* __restrict__ out, LFd f,
                                                       long nlines, int ny, int np,
@@ -985,7 +988,7 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
     Ns *= fac[i];
   }
   f.tpl = tpl_for(f.M, n, fac);
-  if (f.tpl > 256)
+  if (f.tpl > kMaxBlock)
     return fail(SPX_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " exceeds the LDS engine");
   f.lgtpl = 0;
   while ((1 << f.lgtpl) < f.tpl) ++f.lgtpl;

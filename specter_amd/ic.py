@@ -91,3 +91,72 @@ def thermal_perturbation(nx, ny, nz, Cz, seed, amp=1e-3):
     th = np.zeros((nz, ny, nx))
     th[1:npz - 1] = amp * rng.standard_normal((npz - 2, ny, nx))
     return th
+
+
+def spectrum_peaked_velocity_device(nx, ny, nz, Cz, seed, Lx=1.0, Ly=1.0, Lz=1.0,
+                                    kpeak=3.0):
+    """The same construction as spectrum_peaked_velocity, on the GPU with
+    torch (for grids whose host generation would take minutes and tens of
+    GB): phases from torch's Philox generator on the device, so a seed fixes
+    the field for a given device type and torch version (not bit-identical
+    to the numpy generator).  Returns three float64 (nz, ny, nx) cuda
+    tensors, computed one component at a time to bound the memory."""
+    import torch
+    dev = torch.device("cuda")
+    npz = nz - Cz
+    dz = Lz / (npz - 1)
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    f64 = torch.float64
+    kx = torch.fft.fftfreq(nx, d=1.0 / nx, dtype=f64, device=dev) / Lx
+    ky = torch.fft.fftfreq(ny, d=1.0 / ny, dtype=f64, device=dev) / Ly
+    kz = torch.fft.rfftfreq(npz, d=dz, dtype=f64, device=dev) * 2.0 * PI
+    KX, KY, KZ = kx[None, None, :], ky[None, :, None], kz[:, None, None]
+    K2 = KZ ** 2 + KY ** 2 + KX ** 2                       # (nkz, ny, nx)
+    K = torch.sqrt(K2)
+    amp = torch.where(K > 0, K * torch.exp(-(K / kpeak) ** 2), torch.zeros_like(K))
+    del K
+    comps = []
+    for _ in range(3):
+        phase = torch.rand(K2.shape, generator=g, dtype=f64, device=dev) * (2.0 * PI)
+        comps.append(torch.polar(amp, phase))
+        del phase
+    del amp
+    K2s = torch.where(K2 > 0, K2, torch.ones_like(K2))
+    kdotu = (KX * comps[0] + KY * comps[1] + KZ * comps[2]) / K2s
+    del K2s, K2
+    comps = [comps[0] - KX * kdotu, comps[1] - KY * kdotu, comps[2] - KZ * kdotu]
+    del kdotu
+    env = torch.as_tensor(wall_envelope(nz, Cz), device=dev)
+    out = []
+    for i in range(3):
+        c = comps[i]
+        comps[i] = None
+        r = torch.fft.irfftn(c, s=(npz, ny, nx), dim=(0, 1, 2))
+        del c
+        full = torch.zeros((nz, ny, nx), dtype=f64, device=dev)
+        full[:npz] = r * env[:, None, None]
+        del r
+        out.append(full)
+    e = sum(float(torch.mean(f[:npz] ** 2)) for f in out)
+    s = 1.0 / e ** 0.5
+    for f in out:
+        f.mul_(s)
+    return tuple(out)
+
+
+def large_scale_forcing_device(nx, ny, nz, Cz, amp=0.5, Lx=1.0, Ly=1.0, Lz=1.0):
+    """large_scale_forcing on the GPU (float64 cuda tensors)."""
+    import torch
+    dev = torch.device("cuda")
+    x, y, _ = physical_grid(nx, ny, nz, Cz, Lx, Ly, Lz)
+    env = torch.as_tensor(wall_envelope(nz, Cz), device=dev)
+    npz = nz - Cz
+    xs = torch.as_tensor(np.sin(2 * x), device=dev)
+    ys = torch.as_tensor(np.sin(2 * y), device=dev)
+    fx = torch.zeros((nz, ny, nx), dtype=torch.float64, device=dev)
+    fy = torch.zeros_like(fx)
+    fz = torch.zeros_like(fx)
+    fx[:npz] = amp * ys[None, :, None] * env[:, None, None] * torch.ones_like(xs)[None, None, :]
+    fy[:npz] = amp * xs[None, None, :] * env[:, None, None] * torch.ones_like(ys)[None, :, None]
+    return fx, fy, fz

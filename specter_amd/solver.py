@@ -93,11 +93,15 @@ class Solver:
         return self._single
 
     # -- set-up -------------------------------------------------------------------
-    def set_real_field(self, name: str, real: np.ndarray, member: int | None = None):
+    def set_real_field(self, name: str, real, member: int | None = None):
         """field <- fftp3d_real_to_complex(real) (restart path, specter.fpp:894-899).
-        For batched plans, member selects the one run to set."""
+        real: numpy array or cuda tensor (nz, ny, nx).  For batched plans,
+        member selects the one run to set."""
         import torch
-        r = torch.as_tensor(np.ascontiguousarray(real, dtype=np.float64)).cuda()
+        if isinstance(real, torch.Tensor):
+            r = real.to(device="cuda", dtype=torch.float64).contiguous()
+        else:
+            r = torch.as_tensor(np.ascontiguousarray(real, dtype=np.float64)).cuda()
         if member is None:
             self.plan.fftp3d_real_to_complex(r, self.field(name))
         else:
