@@ -351,7 +351,7 @@ __device__ cpx<T>* fft_lds_pp(cpx<T>* a, cpx<T>* b, const LineFFT<cpx<T>>& f, co
 template <typename T>
 __device__ __forceinline__ void stage_twiddles(cpx<T>* tws, const LineFFT<cpx<T>>& f) {
   for (int m = threadIdx.x; m < f.M; m += blockDim.x) tws[m] = f.tw[m];
-  if (f.mode == 0) return;
+  if (f.mode == 0 || f.tab == f.M) return;  // tab == M: filter etc. read from global memory
   for (int m = threadIdx.x; m < f.M; m += blockDim.x) tws[f.M + m] = f.bfil[m];
   if (f.mode == 1) {
     for (int m = threadIdx.x; m < f.n; m += blockDim.x) tws[2 * f.M + m] = f.chirp[m];
@@ -372,9 +372,11 @@ __device__ void line_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw, 
     fft_lds<T, RS>(x, f, tw, lane);
     return;
   }
-  const cpx<T>* bfil = tw + M;  // tables staged in LDS by stage_twiddles
+  // tables staged in LDS by stage_twiddles, or global when they would not fit
+  const bool staged = f.tab > M;
+  const cpx<T>* bfil = staged ? tw + M : f.bfil;
   if (f.mode == 1) {  // Bluestein
-    const cpx<T>* chirp = tw + 2 * M;
+    const cpx<T>* chirp = staged ? tw + 2 * M : f.chirp;
     for (int m = lane; m < M; m += tpl)
       x[m] = (m < n) ? cmul(x[m], chirp[m]) : mkc<T>(0, 0);
     __syncthreads();
@@ -390,7 +392,7 @@ __device__ void line_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw, 
   // ping-pong between b and x (x is free once b holds the permuted input);
   // an even number of passes in total leaves the result in b
   cpx<T>* b = x + n;
-  const int* pin = reinterpret_cast<const int*>(tw + 2 * M);
+  const int* pin = staged ? reinterpret_cast<const int*>(tw + 2 * M) : f.perm;
   const int* pout = pin + M;
   const cpx<T> x0 = x[0];
   for (int q = lane; q < M; q += tpl) b[q] = x[pin[q]];

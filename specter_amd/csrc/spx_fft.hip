@@ -1002,6 +1002,9 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
   int nl = std::max(1, 256 / f.tpl);
   if (want_lines > 0) nl = std::min(nl, want_lines);
   while (nl > 1 && (size_t(f.tab) + size_t(nl) * f.ldl) * csize > lds_budget) nl >>= 1;
+  // very long lines: leave the filter / chirp / index tables in global memory
+  // rather than halve the workgroups per CU
+  if ((size_t(f.tab) + size_t(nl) * f.ldl) * csize > 80 * 1024) f.tab = f.M;
   if ((size_t(f.tab) + size_t(nl) * f.ldl) * csize > 160 * 1024)
     return fail(SPX_ERR_UNSUPPORTED, "FFT line does not fit in LDS");
   f.nl = nl;
