@@ -12,7 +12,7 @@ carries the float64 rate, the speed-up and the relative L2 field error of the
 mixed run against the float64 run after ``--err-steps`` full RK steps from
 the same initial field.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hd256|hd64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hd256|hd64|hd1024|bouss512]
                     [--precision both|mixed|fp64] [--err-steps N]
 """
 from __future__ import annotations
@@ -38,6 +38,14 @@ WORKLOADS = {
     "hd1024": dict(nx=1024, ny=1024, nzp=1024, C=25, d=5, ord=4, nu=1e-4, dt=1e-4, forcing=True,
                    label="HD 1024x1024x(1024+25) FC-Gram d=5 C=25, RK4, nu=1e-4, dt=1e-4, "
                          "forcing 0.5(sin2y,sin2x,0)sin(pi z)"),
+    # Boussinesq 512^3 d=8 C=27 RK4 nu=kappa=2e-4, heated bottom wall: theta is the
+    # deviation from the conduction profile 1 - z/Lz (0 on both walls), gravity along
+    # z (buoyancy xmom*theta in the vz equation, heat current xtemp*vz)
+    "bouss512": dict(nx=512, ny=512, nzp=512, C=27, d=8, ord=4, nu=2e-4, kappa=2e-4, dt=1e-4,
+                     forcing=False, solver="BOUSS", xmom=1.0, xtemp=1.0,
+                     label="Boussinesq 512x512x(512+27) FC-Gram d=8 C=27, RK4, nu=kappa=2e-4, "
+                           "dt=1e-4, heated bottom wall (theta = 1 - z/Lz + theta'), "
+                           "theta' = 1e-3 white noise, v = 0"),
     # HD 64^3 d=5 C=25 RK2 nu=2e-3 dt=1e-3, unforced
     "hd64": dict(nx=64, ny=64, nzp=64, C=25, d=5, ord=2, nu=2e-3, dt=1e-3, forcing=False,
                  label="HD 64x64x(64+25) FC-Gram d=5 C=25, RK2, nu=2e-3, dt=1e-3"),
@@ -270,35 +278,41 @@ def _log(msg):
 
 
 def _bench_large(args, w):
-    """1024^3: fields generated on the GPU; the mixed run and the float64
-    run do not fit together, so they run one after the other from the same
-    initial state (kept on the host); each runs --err-steps full steps for
-    the error comparison, then is timed."""
+    """The 1024^3 HD and 512^3 Boussinesq workloads: the mixed run and the
+    float64 run go one after the other from the same initial state (kept on
+    the host; at 1024^3 the two do not fit together); each runs --err-steps
+    full steps for the error comparison, then is timed."""
     import numpy as np
     import torch
     from specter_amd import ic
     from specter_amd.lib import SPX_FP32_MIXED, SPX_FP64, SpxError
     nx, ny, nz, C = w["nx"], w["ny"], w["nzp"] + w["C"], w["C"]
     npts = nx * ny * w["nzp"]
-    names = ("vx", "vy", "vz", "fx", "fy", "fz")
-    _log("initial fields on the device")
-    u = ic.spectrum_peaked_velocity_device(nx, ny, nz, C, 1000)
-    f = ic.large_scale_forcing_device(nx, ny, nz, C)
-    torch.cuda.empty_cache()   # the generator's temporaries go back to the device
+    bouss = w.get("solver") == "BOUSS"
+    _log("initial fields")
+    if bouss:
+        names = ("th",)
+        fields = [ic.thermal_perturbation(nx, ny, nz, C, 1001)]
+    else:
+        names = ("vx", "vy", "vz", "fx", "fy", "fz")
+        fields = [*ic.spectrum_peaked_velocity_device(nx, ny, nz, C, 1000),
+                  *ic.large_scale_forcing_device(nx, ny, nz, C)]
+        torch.cuda.empty_cache()   # the generator's temporaries go back to the device
+    out_names = ("vx", "vy", "vz", "th") if bouss else ("vx", "vy", "vz")
     res, host_ic, diag = {}, None, {}
     for mode in ("mixed", "fp64"):
         prec = SPX_FP32_MIXED if mode == "mixed" else SPX_FP64
         try:
             plan = _plan(w, prec)
             from specter_amd import Solver, SolverParams
-            sol = Solver(plan, SolverParams(solver="HD", ord=w["ord"], dt=w["dt"], nu=w["nu"]))
+            prm = dict(solver="BOUSS", kappa=w["kappa"], xmom=w["xmom"],
+                       xtemp=w["xtemp"]) if bouss else dict(solver="HD")
+            sol = Solver(plan, SolverParams(ord=w["ord"], dt=w["dt"], nu=w["nu"], **prm))
         except SpxError as e:
             res[mode] = {"error": str(e)}
             _log(f"{mode}: {e}")
             continue
         if host_ic is None:
-            fields = [*u, *f]
-            del u, f
             for i, name in enumerate(names):
                 sol.set_real_field(name, fields[i])
                 fields[i] = None
@@ -315,7 +329,9 @@ def _bench_large(args, w):
         diag[mode] = {"spectrum": np.array(plan.energy_spectrum(vx, vy, vz, nx // 2)),
                       "dissipation": w["nu"] * plan.energy(vx, vy, vz, 0),
                       "wall_shear": np.array(plan.wall_shear(vx, vy, w["nu"])),
-                      "fields": {n: sol.spectral(n) for n in ("vx", "vy", "vz")}}
+                      "fields": {n: sol.spectral(n) for n in out_names}}
+        if bouss:
+            diag[mode]["nusselt"] = np.array(plan.nusselt(sol.field("th")))
         _log(f"{mode}: timing")
         ms, wall, pos = _time(sol, w, args.steps, args.warmup)
         r = {"ms_per_step": ms, "value": npts / (ms * 1e-3) / 1e6}
@@ -337,8 +353,10 @@ def _bench_large(args, w):
         "warmup": args.warmup, "ms_per_step": round(hr["ms_per_step"], 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 nonlinear term / f64 state, continuation, pressure, walls",
-        "precision_mode": "mixed", "data": "synthetic (seeded spectrum-peaked solenoidal IC "
-        "generated on the GPU x sin(pi z/Lz))",
+        "precision_mode": "mixed",
+        "data": "synthetic (seeded white-noise theta', v = 0)" if bouss else
+                "synthetic (seeded spectrum-peaked solenoidal IC generated on the GPU "
+                "x sin(pi z/Lz))",
         "config": {"workload": w["label"], "grid": [nx, ny, nz], "physical_points": npts,
                    "parallelism": "single GPU"},
     }
@@ -353,11 +371,14 @@ def _bench_large(args, w):
         line["speedup_vs_fp64"] = round(f64["ms_per_step"] / hr["ms_per_step"], 3)
         a, b = diag["mixed"], diag["fp64"]
         err = {"steps": args.err_steps}
-        for n in ("vx", "vy", "vz"):
+        for n in out_names:
             err[n] = float(f"{np.linalg.norm(a['fields'][n] - b['fields'][n]) / np.linalg.norm(b['fields'][n]):.3e}")
         err["energy_spectrum"] = float(f"{np.linalg.norm(a['spectrum'] - b['spectrum']) / np.linalg.norm(b['spectrum']):.3e}")
         err["dissipation"] = float(f"{abs(a['dissipation'] - b['dissipation']) / abs(b['dissipation']):.3e}")
         err["wall_shear"] = float(f"{np.abs(a['wall_shear'] - b['wall_shear']).max() / np.abs(b['wall_shear']).max():.3e}")
+        if bouss:
+            err["nusselt"] = float(f"{np.abs(a['nusselt'] - b['nusselt']).max() / np.abs(b['nusselt']).max():.3e}")
+            line["nusselt_fp64"] = [round(float(x), 10) for x in b["nusselt"]]
         line["error_vs_fp64"] = err
     else:
         line["fp64"] = f64
@@ -387,7 +408,7 @@ def main():
         sys.exit(2)
     from specter_amd.lib import SPX_FP32_MIXED, SPX_FP64
     w = WORKLOADS[args.workload]
-    if args.workload == "hd1024":
+    if args.workload in ("hd1024", "bouss512"):
         _bench_large(args, w)
         return
     u, f = _fields(w)
