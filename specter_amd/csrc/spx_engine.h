@@ -253,12 +253,75 @@ __device__ __forceinline__ void stockham_pass(cpx<T>* x, int M, int Ns, unsigned
   __syncthreads();
 }
 
-// The same pass from src to a different buffer dst: no barrier is needed
-// between the loads and the stores, one after them.
-template <typename T, int R>
+// Load / store variants of a pass: the index permutations, chirps and
+// pointwise filters of the Rader and Bluestein convolutions are applied on
+// the first pass's loads and the last passes' stores instead of in passes
+// of their own (one LDS round trip and one barrier less each).
+enum { LD_PLAIN = 0, LD_GATHER = 1, LD_CHIRP = 2 };
+enum { ST_PLAIN = 0, ST_FILTER = 1, ST_SCATTER = 2, ST_CHIRP = 3 };
+
+template <typename T>
+struct PassIO {
+  int n;                 // LD_CHIRP / ST_CHIRP: input / output length
+  const int* gin;        // LD_GATHER: element q of the pass input is src[gin[q]]
+  const int* gout;       // ST_SCATTER: output m goes to dst[gout[m]] as x0 + conj(v)
+  const cpx<T>* filt;    // ST_FILTER: output m becomes conj(v * filt[m])
+  const cpx<T>* chirp;   // LD_CHIRP: src[q] * chirp[q] (0 for q >= n); ST_CHIRP: conj(v) * chirp[m], m < n
+  cpx<T> x0;             // Rader: the input's element 0
+};
+
+template <typename T, int R, int LD>
+__device__ __forceinline__ void pass_load(cpx<T>* v, const cpx<T>* src, int j, int nb,
+                                          const PassIO<T>& io) {
+  if constexpr (LD == LD_GATHER) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[io.gin[j + r * nb]];
+  } else if constexpr (LD == LD_CHIRP) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int q = j + r * nb;
+      v[r] = q < io.n ? cmul(src[q], io.chirp[q]) : mkc<T>(0, 0);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[j + r * nb];
+  }
+}
+
+// X0 (Rader's output 0 = x0 + the first convolution FFT's element 0) is
+// taken by the thread that stores element 0 under ST_FILTER.
+template <typename T, int R, int ST>
+__device__ __forceinline__ void pass_store(cpx<T>* dst, const cpx<T>* v, int base, int Ns,
+                                           const PassIO<T>& io, cpx<T>& X0) {
+  if constexpr (ST == ST_FILTER) {
+    if (base == 0) X0 = cadd(io.x0, v[0]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int m = base + r * Ns;
+      dst[m] = cconj(cmul(v[r], io.filt[m]));
+    }
+  } else if constexpr (ST == ST_SCATTER) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[io.gout[base + r * Ns]] = cadd(io.x0, cconj(v[r]));
+  } else if constexpr (ST == ST_CHIRP) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int m = base + r * Ns;
+      if (m < io.n) dst[m] = cmul(cconj(v[r]), io.chirp[m]);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
+  }
+}
+
+// A pass from src to a different buffer dst: no barrier is needed between
+// the loads and the stores, one after them.
+template <typename T, int R, int LD, int ST>
 __device__ __forceinline__ void stockham_pass_pp(const cpx<T>* src, cpx<T>* dst, int M, int Ns,
                                                  unsigned magic, int lane, int tpl,
-                                                 const cpx<T>* tw) {
+                                                 const cpx<T>* tw, const PassIO<T>& io,
+                                                 cpx<T>& X0) {
   constexpr int MB = MaxB<R>::value;
   const int nb = M / R;
   const int tstep = M / (Ns * R);
@@ -268,8 +331,7 @@ __device__ __forceinline__ void stockham_pass_pp(const cpx<T>* src, cpx<T>* dst,
     const int j = lane + b * tpl;
     if (j < nb) {
       cpx<T> v[R];
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[r] = src[j + r * nb];
+      pass_load<T, R, LD>(v, src, j, nb, io);
       const int jm = Ns > 1 ? fastmod(j, Ns, magic) : 0;
       if (Ns > 1) {
         const int st = jm * tstep;
@@ -277,22 +339,59 @@ __device__ __forceinline__ void stockham_pass_pp(const cpx<T>* src, cpx<T>* dst,
         for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[r * st]);
       }
       Dft<T, R>::run(v, tw, s);
-      const int base = (j - jm) * R + jm;
+      pass_store<T, R, ST>(dst, v, (j - jm) * R + jm, Ns, io, X0);
+    }
+  }
+  __syncthreads();
+}
+
+// In-place pass with load / store variants (two barriers).
+template <typename T, int R, int LD, int ST>
+__device__ __forceinline__ void stockham_pass_io(cpx<T>* x, int M, int Ns, unsigned magic,
+                                                 int lane, int tpl, const cpx<T>* tw,
+                                                 const PassIO<T>& io, cpx<T>& X0) {
+  constexpr int MB = MaxB<R>::value;
+  const int nb = M / R;
+  const int tstep = M / (Ns * R);
+  const int s = M / R;
+  cpx<T> v[MB][R];
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
+  for (int b = 0; b < MB; ++b) {
+    const int j = lane + b * tpl;
+    if (j < nb) {
+      pass_load<T, R, LD>(v[b], x, j, nb, io);
+      if (Ns > 1) {
+        const int st = fastmod(j, Ns, magic) * tstep;
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[b][r] = cmul(v[b][r], tw[r * st]);
+      }
+      Dft<T, R>::run(v[b], tw, s);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < MB; ++b) {
+    const int j = lane + b * tpl;
+    if (j < nb) {
+      const int jm = Ns > 1 ? fastmod(j, Ns, magic) : 0;
+      pass_store<T, R, ST>(x, v[b], (j - jm) * R + jm, Ns, io, X0);
     }
   }
   __syncthreads();
 }
 
 // Radix sets a kernel instance is compiled for (register pressure follows the
-// largest radix pass compiled in): powers of two; {2,3,4,5,7,8,9}; all.
-enum { RS_POW2 = 0, RS_SMOOTH = 1, RS_ALL = 2 };
+// largest radix pass compiled in): powers of two; {2,3,4,5,7,8,9}; all;
+// {2,...,9} with the convolution tables (filter, chirp, indices) read from
+// global memory instead of LDS (very long lines).  The table location is a
+// compile-time property so that every table access has a known address space.
+enum { RS_POW2 = 0, RS_SMOOTH = 1, RS_ALL = 2, RS_GTAB = 3 };
 
 // Forward FFT of length f.M on the line x (LDS) with twiddles tw (LDS).
 // Every thread of the workgroup must call it (it synchronises).
 template <typename T, int RS>
-__device__ void fft_lds(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw, int lane) {
+__device__ __forceinline__ void fft_lds(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw,
+                                        int lane) {
   int Ns = 1;
   for (int st = 0; st < f.nstage; ++st) {
     const int R = f.radix[st];
@@ -305,7 +404,7 @@ __device__ void fft_lds(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw, i
       else if (R == 5) stockham_pass<T, 5>(x, f.M, Ns, mg, lane, f.tpl, tw);
       else if (R == 7) stockham_pass<T, 7>(x, f.M, Ns, mg, lane, f.tpl, tw);
       else if (R == 9) stockham_pass<T, 9>(x, f.M, Ns, mg, lane, f.tpl, tw);
-      else if constexpr (RS >= RS_ALL) {
+      else if constexpr (RS == RS_ALL) {
         if (R == 11) stockham_pass<T, 11>(x, f.M, Ns, mg, lane, f.tpl, tw);
         else if (R == 13) stockham_pass<T, 13>(x, f.M, Ns, mg, lane, f.tpl, tw);
       }
@@ -314,34 +413,56 @@ __device__ void fft_lds(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw, i
   }
 }
 
-// Forward FFT of length f.M ping-ponging between a (input) and b; returns
-// the buffer that holds the result.  Every thread must call it.
-template <typename T, int RS>
-__device__ cpx<T>* fft_lds_pp(cpx<T>* a, cpx<T>* b, const LineFFT<cpx<T>>& f, const cpx<T>* tw,
-                              int lane) {
+// One pass of stage st of a convolution FFT (radix dispatch); PP: from src
+// to dst, else in place on src.
+template <typename T, int RS, bool PP, int LD, int ST>
+__device__ __forceinline__ void conv_pass(int st, cpx<T>* src, cpx<T>* dst,
+                                          const LineFFT<cpx<T>>& f, const cpx<T>* tw, int lane,
+                                          const PassIO<T>& io, cpx<T>& X0) {
   int Ns = 1;
-  for (int st = 0; st < f.nstage; ++st) {
-    const int R = f.radix[st];
-    const unsigned mg = f.magic[st];
-    if (R == 8) stockham_pass_pp<T, 8>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
-    else if (R == 4) stockham_pass_pp<T, 4>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
-    else if (R == 2) stockham_pass_pp<T, 2>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
-    else if constexpr (RS >= RS_SMOOTH) {
-      if (R == 3) stockham_pass_pp<T, 3>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
-      else if (R == 5) stockham_pass_pp<T, 5>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
-      else if (R == 7) stockham_pass_pp<T, 7>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
-      else if (R == 9) stockham_pass_pp<T, 9>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
-      else if constexpr (RS >= RS_ALL) {
-        if (R == 11) stockham_pass_pp<T, 11>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
-        else if (R == 13) stockham_pass_pp<T, 13>(a, b, f.M, Ns, mg, lane, f.tpl, tw);
-      }
+  for (int u = 0; u < st; ++u) Ns *= f.radix[u];
+  const int R = f.radix[st];
+  const unsigned mg = f.magic[st];
+#define SPX_CONV_PASS(RR)                                                                    \
+  do {                                                                                       \
+    if constexpr (PP)                                                                        \
+      stockham_pass_pp<T, RR, LD, ST>(src, dst, f.M, Ns, mg, lane, f.tpl, tw, io, X0);       \
+    else                                                                                     \
+      stockham_pass_io<T, RR, LD, ST>(src, f.M, Ns, mg, lane, f.tpl, tw, io, X0);            \
+  } while (0)
+  if (R == 8) SPX_CONV_PASS(8);
+  else if (R == 4) SPX_CONV_PASS(4);
+  else if (R == 2) SPX_CONV_PASS(2);
+  else if constexpr (RS >= RS_SMOOTH) {
+    if (R == 3) SPX_CONV_PASS(3);
+    else if (R == 5) SPX_CONV_PASS(5);
+    else if (R == 7) SPX_CONV_PASS(7);
+    else if (R == 9) SPX_CONV_PASS(9);
+    else if constexpr (RS == RS_ALL) {
+      if (R == 11) SPX_CONV_PASS(11);
+      else if (R == 13) SPX_CONV_PASS(13);
     }
-    Ns *= R;
-    cpx<T>* t = a;
-    a = b;
-    b = t;
   }
-  return a;
+#undef SPX_CONV_PASS
+}
+
+// The two length-M FFTs of a Rader (PP) or Bluestein (in place) convolution
+// as 2P passes, P = f.nstage >= 2: pass 0 loads with LD0, pass P-1 stores
+// with STM, pass 2P-1 with STE.  Ping-pong passes alternate a -> b -> a ...,
+// so pass 2P-1 stores into a.
+template <typename T, int RS, bool PP, int LD0, int STM, int STE>
+__device__ __forceinline__ void conv_passes(cpx<T>* a, cpx<T>* b, const LineFFT<cpx<T>>& f,
+                                            const cpx<T>* tw, int lane, const PassIO<T>& io,
+                                            cpx<T>& X0) {
+  const int P = f.nstage;
+  conv_pass<T, RS, PP, LD0, ST_PLAIN>(0, a, b, f, tw, lane, io, X0);
+  for (int t = 1; t < 2 * P - 1; ++t) {
+    cpx<T>* src = (t & 1) ? b : a;
+    cpx<T>* dst = (t & 1) ? a : b;
+    if (t == P - 1) conv_pass<T, RS, PP, LD_PLAIN, STM>(t, src, dst, f, tw, lane, io, X0);
+    else conv_pass<T, RS, PP, LD_PLAIN, ST_PLAIN>(t < P ? t : t - P, src, dst, f, tw, lane, io, X0);
+  }
+  conv_pass<T, RS, PP, LD_PLAIN, STE>(P - 1, b, a, f, tw, lane, io, X0);
 }
 
 // Copy the tables into LDS (called by all threads; the caller's next
@@ -351,7 +472,7 @@ __device__ cpx<T>* fft_lds_pp(cpx<T>* a, cpx<T>* b, const LineFFT<cpx<T>>& f, co
 template <typename T>
 __device__ __forceinline__ void stage_twiddles(cpx<T>* tws, const LineFFT<cpx<T>>& f) {
   for (int m = threadIdx.x; m < f.M; m += blockDim.x) tws[m] = f.tw[m];
-  if (f.mode == 0 || f.tab == f.M) return;  // tab == M: filter etc. read from global memory
+  if (f.mode == 0 || f.tab == f.M) return;  // RS_GTAB plans read them from global memory
   for (int m = threadIdx.x; m < f.M; m += blockDim.x) tws[f.M + m] = f.bfil[m];
   if (f.mode == 1) {
     for (int m = threadIdx.x; m < f.n; m += blockDim.x) tws[2 * f.M + m] = f.chirp[m];
@@ -364,46 +485,36 @@ __device__ __forceinline__ void stage_twiddles(cpx<T>* tws, const LineFFT<cpx<T>
 // Forward DFT of length f.n of the n values at x[0..n), result in x[0..n).
 // The caller has synchronised after filling x; x spans f.ldl elements.
 template <typename T, int RS>
-__device__ void line_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw, int lane) {
-  const int n = f.n, M = f.M, tpl = f.tpl;
+__device__ __forceinline__ void line_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw,
+                                         int lane) {
+  const int n = f.n, M = f.M;
   // the power-of-two family is only used for direct transforms: keep the
   // Rader / Bluestein code (and its registers) out of those instances
   if (RS == RS_POW2 || f.mode == 0) {
     fft_lds<T, RS>(x, f, tw, lane);
     return;
   }
-  // tables staged in LDS by stage_twiddles, or global when they would not fit
-  const bool staged = f.tab > M;
-  const cpx<T>* bfil = staged ? tw + M : f.bfil;
-  if (f.mode == 1) {  // Bluestein
-    const cpx<T>* chirp = staged ? tw + 2 * M : f.chirp;
-    for (int m = lane; m < M; m += tpl)
-      x[m] = (m < n) ? cmul(x[m], chirp[m]) : mkc<T>(0, 0);
-    __syncthreads();
-    fft_lds<T, RS>(x, f, tw, lane);
-    for (int m = lane; m < M; m += tpl) x[m] = cconj(cmul(x[m], bfil[m]));
-    __syncthreads();
-    fft_lds<T, RS>(x, f, tw, lane);
-    for (int m = lane; m < n; m += tpl) x[m] = cmul(cconj(x[m]), chirp[m]);
-    __syncthreads();
+  // tables staged in LDS by stage_twiddles, or global (RS_GTAB)
+  constexpr bool G = RS == RS_GTAB;
+  PassIO<T> io;
+  io.n = n;
+  io.filt = G ? f.bfil : tw + M;
+  cpx<T> X0 = mkc<T>(0, 0);
+  if (f.mode == 1) {  // Bluestein: chirp, FFT, filter, FFT, chirp; in place
+    io.gin = io.gout = nullptr;
+    io.chirp = G ? f.chirp : tw + 2 * M;
+    io.x0 = mkc<T>(0, 0);
+    conv_passes<T, RS, false, LD_CHIRP, ST_FILTER, ST_CHIRP>(x, x, f, tw, lane, io, X0);
     return;
   }
-  // Rader: M = n - 1, second buffer b = x + n; the two length-M FFTs
-  // ping-pong between b and x (x is free once b holds the permuted input);
-  // an even number of passes in total leaves the result in b
-  cpx<T>* b = x + n;
-  const int* pin = staged ? reinterpret_cast<const int*>(tw + 2 * M) : f.perm;
-  const int* pout = pin + M;
-  const cpx<T> x0 = x[0];
-  for (int q = lane; q < M; q += tpl) b[q] = x[pin[q]];
-  __syncthreads();
-  cpx<T>* r = fft_lds_pp<T, RS>(b, x, f, tw, lane);
-  cpx<T> X0 = mkc<T>(0, 0);
-  if (lane == 0) X0 = cadd(x0, r[0]);
-  for (int m = lane; m < M; m += tpl) r[m] = cconj(cmul(r[m], bfil[m]));
-  __syncthreads();
-  r = fft_lds_pp<T, RS>(r, r == b ? x : b, f, tw, lane);
-  for (int p = lane; p < M; p += tpl) x[pout[p]] = cadd(x0, cconj(r[p]));
+  // Rader: M = n - 1, second buffer x + n.  The first FFT gathers its input
+  // x[g^q] on its first pass and applies the filter on its last; the second
+  // scatters x0 + conj(.) to x[g^-p] on its last pass, which stores into x.
+  io.gin = G ? f.perm : reinterpret_cast<const int*>(tw + 2 * M);
+  io.gout = io.gin + M;
+  io.chirp = nullptr;
+  io.x0 = x[0];
+  conv_passes<T, RS, true, LD_GATHER, ST_FILTER, ST_SCATTER>(x, x + n, f, tw, lane, io, X0);
   if (lane == 0) x[0] = X0;
   __syncthreads();
 }

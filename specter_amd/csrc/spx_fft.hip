@@ -1003,8 +1003,11 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
   if (want_lines > 0) nl = std::min(nl, want_lines);
   while (nl > 1 && (size_t(f.tab) + size_t(nl) * f.ldl) * csize > lds_budget) nl >>= 1;
   // very long lines: leave the filter / chirp / index tables in global memory
-  // rather than halve the workgroups per CU
-  if ((size_t(f.tab) + size_t(nl) * f.ldl) * csize > 80 * 1024) f.tab = f.M;
+  // rather than halve the workgroups per CU (kernels built for radices <= 9)
+  bool small_radices = true;
+  for (int R : fac) small_radices = small_radices && R <= 9;
+  if (f.mode != 0 && small_radices && (size_t(f.tab) + size_t(nl) * f.ldl) * csize > 80 * 1024)
+    f.tab = f.M;
   if ((size_t(f.tab) + size_t(nl) * f.ldl) * csize > 160 * 1024)
     return fail(SPX_ERR_UNSUPPORTED, "FFT line does not fit in LDS");
   f.nl = nl;
@@ -1100,6 +1103,7 @@ static int lds_limit(K kernel, size_t bytes) {
 // radix set of a plan: the smallest compiled kernel family that covers it
 // (Rader and Bluestein plans never use the power-of-two family)
 static int rset_of(const LineFFTHost& f) {
+  if (f.mode != 0 && f.tab == f.M) return RS_GTAB;  // radices <= 9 (build_line_fft)
   int rs = f.mode == 0 ? RS_POW2 : RS_SMOOTH;
   for (int i = 0; i < f.nstage; ++i) {
     const int R = f.radix[i];
@@ -1122,9 +1126,13 @@ static int rset_of(const LineFFTHost& f) {
       if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_SMOOTH TARGS>, (LDS))) return rc_;         \
       hipLaunchKernelGGL((KER<TY SPX_COMMA RS_SMOOTH TARGS>), GRID, BLOCK, LDS, STREAM,      \
                          __VA_ARGS__);                                                       \
-    } else {                                                                                 \
+    } else if (rs_ == RS_ALL) {                                                              \
       if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_ALL TARGS>, (LDS))) return rc_;            \
       hipLaunchKernelGGL((KER<TY SPX_COMMA RS_ALL TARGS>), GRID, BLOCK, LDS, STREAM,         \
+                         __VA_ARGS__);                                                       \
+    } else {                                                                                 \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_GTAB TARGS>, (LDS))) return rc_;           \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_GTAB TARGS>), GRID, BLOCK, LDS, STREAM,        \
                          __VA_ARGS__);                                                       \
     }                                                                                        \
   } while (0)
@@ -1420,8 +1428,12 @@ int proj_pressure(spx_plan* p, void* d) {
     hipLaunchKernelGGL(k_proj_pressure<RS_SMOOTH>, grid, blk(f), lds, p->stream, C_<double>(d),
                        C_<double>(p->bc), f.dev<Cd>(), nlines, p->g.ny, 1.0 / double(p->g.nz),
                        p->g.Lz, p->d_khom, p->d_z);
-  else
+  else if (rs == RS_ALL)
     hipLaunchKernelGGL(k_proj_pressure<RS_ALL>, grid, blk(f), lds, p->stream, C_<double>(d),
+                       C_<double>(p->bc), f.dev<Cd>(), nlines, p->g.ny, 1.0 / double(p->g.nz),
+                       p->g.Lz, p->d_khom, p->d_z);
+  else
+    hipLaunchKernelGGL(k_proj_pressure<RS_GTAB>, grid, blk(f), lds, p->stream, C_<double>(d),
                        C_<double>(p->bc), f.dev<Cd>(), nlines, p->g.ny, 1.0 / double(p->g.nz),
                        p->g.Lz, p->d_khom, p->d_z);
   SPX_CHECK_LAUNCH();
