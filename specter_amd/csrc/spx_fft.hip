@@ -499,9 +499,19 @@ __global__ void __launch_bounds__(kMaxBlock) k_xgrad_prod(const cpx<T>* __restri
   const T* mult[3] = {u, v, w};
   for (int r = 0; r < 3; ++r) {
     load_c2r_pair(xl, srcs[r] + co, nkx, nx, lane, f.tpl, ok, r == 0 ? 1 : 0, kx);
+    // the velocity values this thread multiplies with, loaded before the
+    // transform so that their latency overlaps it
+    const T* m = mult[r] + ro;
+    T ma[KEEP], mb[KEEP];
+#pragma unroll
+    for (int q = 0; q < KEEP; ++q) {
+      const int x = lane + q * f.tpl;
+      const bool in = x < nx && ok;
+      ma[q] = in ? m[x] : T(0);
+      mb[q] = in ? m[x + nx] : T(0);
+    }
     __syncthreads();
     line_dft<T, RS>(xl, f, tws, lane);
-    const T* m = mult[r] + ro;
 #pragma unroll
     for (int q = 0; q < KEEP; ++q) {
       const int x = lane + q * f.tpl;
@@ -509,14 +519,14 @@ __global__ void __launch_bounds__(kMaxBlock) k_xgrad_prod(const cpx<T>* __restri
         const Cp z = xl[x];
         const T a = z.x, b = -z.y;
         if (r == 0) {
-          accA[q] = m[x] * a;
-          accB[q] = m[x + nx] * b;
+          accA[q] = ma[q] * a;
+          accB[q] = mb[q] * b;
         } else if (r == 1) {
-          accA[q] = accA[q] + m[x] * a;
-          accB[q] = accB[q] + m[x + nx] * b;
+          accA[q] = accA[q] + ma[q] * a;
+          accB[q] = accB[q] + mb[q] * b;
         } else {
-          accA[q] = (accA[q] + m[x] * a) * tmp;
-          accB[q] = (accB[q] + m[x + nx] * b) * tmp;
+          accA[q] = (accA[q] + ma[q] * a) * tmp;
+          accB[q] = (accB[q] + mb[q] * b) * tmp;
         }
       }
     }
@@ -567,6 +577,15 @@ __global__ void __launch_bounds__(kMaxBlock) k_noslip(cpx<TS>* __restrict__ vx, 
   const bool ok = ln < nlines;
   const long li = ok ? ln / ny : 0;
   const int lj = ok ? int(ln - li * ny) : 0;
+  // wall value of lanes 0 (z = 0) and 1 (z = Lz), read before the transform
+  Cd val = mkc<double>(0, 0);
+  if (lane < 2 && ok && !wp.scalar) {
+    const int ind = lane ? np - 1 : 0;
+    const Cd pv = pr[ln * n + ind];
+    const double kxi = kx[li], kyj = ky[lj];
+    val = cscale(ikmul(comp ? kyj : kxi, pv), wp.tau);
+    if (kxi == 0.0 && kyj == 0.0) val = mkc<double>(wp.nxny * wp.vb[comp][lane], 0.0);
+  }
   for (int k = lane; k < n; k += f.tpl) {
     Cd v = mkc<double>(0, 0);
     if (ok) {
@@ -579,21 +598,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_noslip(cpx<TS>* __restrict__ vx, 
   line_dft<double, RS>(xl, f, tws, lane);
   for (int k = lane; k < np; k += f.tpl) xl[k] = cscale(cconj(xl[k]), invn);
   __syncthreads();
-  if (lane < 2 && ok) {
-    const int w = lane;
-    const int ind = w ? np - 1 : 0;
-    Cd val;
-    if (wp.scalar) {
-      val = mkc<double>(0, 0);
-    } else {
-      const long i = ln / ny;
-      const int j = int(ln - i * ny);
-      const Cd pv = pr[ln * n + ind];
-      val = cscale(ikmul(comp ? ky[j] : kx[i], pv), wp.tau);
-      if (kx[i] == 0.0 && ky[j] == 0.0) val = mkc<double>(wp.nxny * wp.vb[comp][w], 0.0);
-    }
-    xl[ind] = val;
-  }
+  if (lane < 2 && ok) xl[lane ? np - 1 : 0] = val;
   __syncthreads();
   fc_continue_lds(lds, f.nl, f.ldl, np, C, d, dir);
   __syncthreads();
@@ -650,13 +655,16 @@ __global__ void __launch_bounds__(kMaxBlock) k_proj_pressure(Cd* __restrict__ dd
   const long ln = long(blockIdx.x) * f.nl + line;
   const bool ok = ln < nlines;
   for (int k = lane; k < n; k += f.tpl) xl[k] = ok ? cconj(dd[ln * n + k]) : mkc<double>(0, 0);
+  // (kx, ky) = (0, 0) is the only line with khom = 0; coefficients formed
+  // before the transform
+  const double kh = ok ? khom[ln] : 1.0;
+  const bool zm = kh == 0.0;
+  Coef cf;
+  cf.c1 = cf.c2 = mkc<double>(0, 0);
+  if (ok) cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
   __syncthreads();
   line_dft<double, RS>(xl, f, tws, lane);
   if (!ok) return;
-  // (kx, ky) = (0, 0) is the only line with khom = 0
-  const double kh = khom[ln];
-  const bool zm = kh == 0.0;
-  const Coef cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
   for (int k = lane; k < n; k += f.tpl) {
     Cd a;
     if (zm) {
@@ -1002,12 +1010,19 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
   int nl = std::max(1, 256 / f.tpl);
   if (want_lines > 0) nl = std::min(nl, want_lines);
   while (nl > 1 && (size_t(f.tab) + size_t(nl) * f.ldl) * csize > lds_budget) nl >>= 1;
-  // very long lines: leave the filter / chirp / index tables in global memory
-  // rather than halve the workgroups per CU (kernels built for radices <= 9)
+  // Rader / Bluestein: leave the filter / chirp / index tables in global
+  // memory (L1/L2-resident) when staging them in LDS would cost workgroups
+  // per CU, counted up to 4 per CU (beyond that registers bound the
+  // occupancy of these kernels); kernels for this are built for radices <= 9
   bool small_radices = true;
   for (int R : fac) small_radices = small_radices && R <= 9;
-  if (f.mode != 0 && small_radices && (size_t(f.tab) + size_t(nl) * f.ldl) * csize > 80 * 1024)
-    f.tab = f.M;
+  if (f.mode != 0 && small_radices) {
+    const size_t lds_cu = 160 * 1024, lines = size_t(nl) * f.ldl;
+    auto wgs = [&](size_t tab) {
+      return std::min<size_t>(4, lds_cu / ((tab + lines) * size_t(csize)));
+    };
+    if (wgs(size_t(f.M)) > wgs(size_t(f.tab))) f.tab = f.M;
+  }
   if ((size_t(f.tab) + size_t(nl) * f.ldl) * csize > 160 * 1024)
     return fail(SPX_ERR_UNSUPPORTED, "FFT line does not fit in LDS");
   f.nl = nl;
