@@ -987,23 +987,23 @@ int spx_substep(spx_plan* p, spx_state* s, int o) {
   const void* src[4] = {nullptr, nullptr, s->th, s->vz};
   const double coef[4] = {0.0, 0.0, q.xmom, q.xtemp};
   const double* nu[4] = {s->d_nu, s->d_nu, s->d_nu, s->d_kappa};
+  // the RK update of vx, vy is applied as the wall kernel loads them; HD
+  // updates vz as the projection loads it
+  RkLoad rk;
+  std::memset(&rk, 0, sizeof(rk));
+  rk.nl32 = p->fp32 ? 1 : 0;
+  rk.kx = p->d_kx;
+  rk.ky = p->d_ky;
+  rk.kz = p->d_kz;
+  rk.fkx = p->d_fkx;
+  rk.fky = p->d_fky;
+  rk.fkz = p->d_fkz;
+  rk.dt = q.dt;
+  rk.rmp = rmp;
+  for (int c = 0; c < 2; ++c) rk.c[c] = RkLoadComp{c0[c], nl[c], f[c], s->d_nu};
+  double tau = 1.0 / double(o);
+  if (o != q.ord) tau = double(o + 1) * tau;
   if (!bouss) {
-    // HD: the RK update is applied as the wall kernel loads vx, vy and as the
-    // projection loads vz (one pass over the state less per component)
-    RkLoad rk;
-    std::memset(&rk, 0, sizeof(rk));
-    rk.nl32 = p->fp32 ? 1 : 0;
-    rk.kx = p->d_kx;
-    rk.ky = p->d_ky;
-    rk.kz = p->d_kz;
-    rk.fkx = p->d_fkx;
-    rk.fky = p->d_fky;
-    rk.fkz = p->d_fkz;
-    rk.dt = q.dt;
-    rk.rmp = rmp;
-    for (int c = 0; c < 2; ++c) rk.c[c] = RkLoadComp{c0[c], nl[c], f[c], s->d_nu};
-    double tau = 1.0 / double(o);
-    if (o != q.ord) tau = double(o + 1) * tau;
     if ((rc = noslip_fused(p, s->vx, s->vy, s->pr, tau, q.v_zsta, q.v_zend, &rk))) return rc;
     RkLoad rz = rk;
     rz.c[0] = RkLoadComp{c0[2], nl[2], f[2], s->d_nu};
@@ -1011,20 +1011,22 @@ int spx_substep(spx_plan* p, spx_state* s, int o) {
     if ((rc = proj_pressure(p, s->pr))) return rc;
     return proj_harmonic(p, s->vx, s->vy, s->vz);
   }
-  if ((rc = op_rk_update(p, 4, v, c0, nl, f, src, coef, nu, q.dt, rmp, p->fp32))) return rc;
-  if ((rc = spx_v_imposebc_and_project(p, s->vx, s->vy, s->vz, s->pr, o, q.ord, q.v_zsta,
-                                       q.v_zend)))
+  // Boussinesq: vz and th are updated together first (the buoyancy and heat
+  // current terms couple them through their values before this stage)
+  if ((rc = op_rk_update(p, 2, v + 2, c0 + 2, nl + 2, f + 2, src + 2, coef + 2, nu + 2, q.dt, rmp,
+                         p->fp32)))
     return rc;
-  if (bouss) {
-    if ((rc = spx_s_imposebc(p, s->th))) return rc;
-    if ((rc = op_filter(p, s->th))) return rc;
-    // round trip of th through real space (include/bouss/bouss_rkstep2.f90:56-59)
-    if ((rc = c2r3(p, s->th, p->R0, 0, p->np))) return rc;
-    if ((rc = op_real_div3(p, p->R0, double(p->g.nx), double(p->g.ny), double(p->g.nz))))
-      return rc;
-    if ((rc = r2c3(p, p->R0, s->th))) return rc;
-  }
-  return SPX_OK;
+  if ((rc = noslip_fused(p, s->vx, s->vy, s->pr, tau, q.v_zsta, q.v_zend, &rk))) return rc;
+  if ((rc = op_poisson_project_bc(p, s->vx, s->vy, s->vz, s->pr))) return rc;
+  if ((rc = proj_pressure(p, s->pr))) return rc;
+  if ((rc = proj_harmonic(p, s->vx, s->vy, s->vz))) return rc;
+  if ((rc = spx_s_imposebc(p, s->th))) return rc;
+  if ((rc = op_filter(p, s->th))) return rc;
+  // round trip of th through real space (include/bouss/bouss_rkstep2.f90:56-59)
+  if ((rc = c2r3(p, s->th, p->R0, 0, p->np))) return rc;
+  if ((rc = op_real_div3(p, p->R0, double(p->g.nx), double(p->g.ny), double(p->g.nz))))
+    return rc;
+  return r2c3(p, p->R0, s->th);
 }
 
 static int step_once(spx_plan* p, spx_state* s) {
