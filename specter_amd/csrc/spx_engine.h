@@ -211,6 +211,22 @@ struct MaxB {
   static constexpr int value = (kStage / R) > 0 ? (kStage / R) : 1;
 };
 
+// Synchronisation between the passes of one line.  A line held by threads of
+// one wavefront (tpl <= 64) needs only the wavefront's own ordering of its LDS
+// accesses (a wavefront's LDS instructions complete in order) and a compiler
+// fence; a line spread over several wavefronts needs the workgroup barrier.
+// line_dft ends with a workgroup barrier either way, so code between
+// transforms may read other lines' buffers.
+__device__ __forceinline__ void line_sync(int tpl) {
+  if (tpl <= 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // j mod Ns with the per-stage magic multiplier (j, Ns < 2^16)
 __device__ __forceinline__ int fastmod(int j, int Ns, unsigned magic) {
   const int q = int(__umulhi(unsigned(j), magic));
@@ -239,7 +255,7 @@ __device__ __forceinline__ void stockham_pass(cpx<T>* x, int M, int Ns, unsigned
       Dft<T, R>::run(v[b], tw, s);
     }
   }
-  __syncthreads();
+  line_sync(tpl);
 #pragma unroll
   for (int b = 0; b < MB; ++b) {
     const int j = lane + b * tpl;
@@ -250,7 +266,7 @@ __device__ __forceinline__ void stockham_pass(cpx<T>* x, int M, int Ns, unsigned
       for (int r = 0; r < R; ++r) x[base + r * Ns] = v[b][r];
     }
   }
-  __syncthreads();
+  line_sync(tpl);
 }
 
 // Load / store variants of a pass: the index permutations, chirps and
@@ -342,7 +358,7 @@ __device__ __forceinline__ void stockham_pass_pp(const cpx<T>* src, cpx<T>* dst,
       pass_store<T, R, ST>(dst, v, (j - jm) * R + jm, Ns, io, X0);
     }
   }
-  __syncthreads();
+  line_sync(tpl);
 }
 
 // In-place pass with load / store variants (two barriers).
@@ -368,7 +384,7 @@ __device__ __forceinline__ void stockham_pass_io(cpx<T>* x, int M, int Ns, unsig
       Dft<T, R>::run(v[b], tw, s);
     }
   }
-  __syncthreads();
+  line_sync(tpl);
 #pragma unroll
   for (int b = 0; b < MB; ++b) {
     const int j = lane + b * tpl;
@@ -377,7 +393,7 @@ __device__ __forceinline__ void stockham_pass_io(cpx<T>* x, int M, int Ns, unsig
       pass_store<T, R, ST>(x, v[b], (j - jm) * R + jm, Ns, io, X0);
     }
   }
-  __syncthreads();
+  line_sync(tpl);
 }
 
 // Radix sets a kernel instance is compiled for (register pressure follows the
@@ -388,7 +404,7 @@ __device__ __forceinline__ void stockham_pass_io(cpx<T>* x, int M, int Ns, unsig
 enum { RS_POW2 = 0, RS_SMOOTH = 1, RS_ALL = 2, RS_GTAB = 3 };
 
 // Forward FFT of length f.M on the line x (LDS) with twiddles tw (LDS).
-// Every thread of the workgroup must call it (it synchronises).
+// Every thread of the workgroup must call it (it synchronises the line).
 template <typename T, int RS>
 __device__ __forceinline__ void fft_lds(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw,
                                         int lane) {
@@ -492,6 +508,7 @@ __device__ __forceinline__ void line_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, co
   // Rader / Bluestein code (and its registers) out of those instances
   if (RS == RS_POW2 || f.mode == 0) {
     fft_lds<T, RS>(x, f, tw, lane);
+    __syncthreads();
     return;
   }
   // tables staged in LDS by stage_twiddles, or global (RS_GTAB)
@@ -505,6 +522,7 @@ __device__ __forceinline__ void line_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, co
     io.chirp = G ? f.chirp : tw + 2 * M;
     io.x0 = mkc<T>(0, 0);
     conv_passes<T, RS, false, LD_CHIRP, ST_FILTER, ST_CHIRP>(x, x, f, tw, lane, io, X0);
+    __syncthreads();
     return;
   }
   // Rader: M = n - 1, second buffer x + n.  The first FFT gathers its input

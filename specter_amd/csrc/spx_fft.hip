@@ -706,6 +706,19 @@ __global__ void __launch_bounds__(kMaxBlock) k_proj_harmonic(cpx<TS>* __restrict
   Coef cf;
   cf.c1 = cf.c2 = mkc<double>(0, 0);
   if (ok) cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
+  // the field values this thread corrects, read before the transforms (the
+  // two line buffers bound the occupancy, not the registers)
+  Cd pa[KEEP], pb[KEEP], pc[KEEP];
+#pragma unroll
+  for (int q = 0; q < KEEP; ++q) {
+    const int k = lane + q * f.tpl;
+    if (ok && k < n) {
+      const long o = ln * n + k;
+      pa[q] = ccast<double>(va[o]);
+      pb[q] = ccast<double>(vb[o]);
+      pc[q] = ccast<double>(vc[o]);
+    }
+  }
   for (int k = lane; k < np; k += f.tpl) {
     Cd a = mkc<double>(0, 0), b = mkc<double>(0, 0);
     if (ok) {
@@ -731,12 +744,16 @@ __global__ void __launch_bounds__(kMaxBlock) k_proj_harmonic(cpx<TS>* __restrict
   line_dft<double, RS>(xb, f, tws, lane);
   if (!ok) return;
   const double kxi = kx[i], kyj = ky[j];
-  for (int k = lane; k < n; k += f.tpl) {
-    const Cd A = xl[k], B = xb[k];
-    const long o = ln * n + k;
-    va[o] = ccast<TS>(csub(ccast<double>(va[o]), ikmul(kxi, A)));
-    vb[o] = ccast<TS>(csub(ccast<double>(vb[o]), ikmul(kyj, A)));
-    vc[o] = ccast<TS>(csub(ccast<double>(vc[o]), B));
+#pragma unroll
+  for (int q = 0; q < KEEP; ++q) {
+    const int k = lane + q * f.tpl;
+    if (k < n) {
+      const Cd A = xl[k], B = xb[k];
+      const long o = ln * n + k;
+      va[o] = ccast<TS>(csub(pa[q], ikmul(kxi, A)));
+      vb[o] = ccast<TS>(csub(pb[q], ikmul(kyj, A)));
+      vc[o] = ccast<TS>(csub(pc[q], B));
+    }
   }
 }
 
