@@ -231,42 +231,48 @@ __global__ void __launch_bounds__(256) k_z_s2x_bnd(BndSrc src, cpx<double>* __re
   const long b = ok ? ln / nlines : 0, l = ok ? ln - b * nlines : 0;
   const long i = l / ny, jy = l - i * ny;
   const long o = ((long(comp) * mem + b) * D2) * nlines + jy * nkx + i;
-  // v, then (i kz) v: one accumulator set live at a time.  The top block
-  // uses exp(+2 pi i k (np-d+q)/n) = wt[k] exp(+2 pi i k q/n): both blocks
-  // share the twiddle recurrence, the top one on u_k = v_k wt[k].
-  for (int kind = 0; kind < 2; ++kind) {
-    cpx<double> acc[D2];
+  // v and (i kz) v from one load of the line.  The top block uses
+  // exp(+2 pi i k (np-d+q)/n) = wt[k] exp(+2 pi i k q/n): both blocks share
+  // the twiddle recurrence, the top one on u_k = v_k wt[k].
+  cpx<double> acc[2][D2];
 #pragma unroll
-    for (int q = 0; q < D2; ++q) acc[q] = mkc<double>(0, 0);
-    if (ok) {
-      for (int k = sub; k < n; k += LPL) {
-        cpx<double> v = S[k];
-        if (kind) v = ikmul(kz[k], v);
-        const cpx<double> u = cmul(v, wt[k]);
-        const cpx<double> step = w1[k];
-        cpx<double> t = mkc<double>(1.0, 0.0);
+  for (int kind = 0; kind < 2; ++kind)
 #pragma unroll
-        for (int q = 0; q < D; ++q) {
-          acc[q] = cadd(acc[q], cmul(v, t));
-          acc[D + q] = cadd(acc[D + q], cmul(u, t));
-          if (q + 1 < D) t = cmul(t, step);
-        }
+    for (int q = 0; q < D2; ++q) acc[kind][q] = mkc<double>(0, 0);
+  if (ok) {
+    for (int k = sub; k < n; k += LPL) {
+      const cpx<double> v0 = S[k];
+      const cpx<double> v1 = ikmul(kz[k], v0);
+      const cpx<double> wtk = wt[k];
+      const cpx<double> u0 = cmul(v0, wtk), u1 = cmul(v1, wtk);
+      const cpx<double> step = w1[k];
+      cpx<double> t = mkc<double>(1.0, 0.0);
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        acc[0][q] = cadd(acc[0][q], cmul(v0, t));
+        acc[0][D + q] = cadd(acc[0][D + q], cmul(u0, t));
+        acc[1][q] = cadd(acc[1][q], cmul(v1, t));
+        acc[1][D + q] = cadd(acc[1][D + q], cmul(u1, t));
+        if (q + 1 < D) t = cmul(t, step);
       }
     }
-    // reduce over the 16 lanes of the line (lanes of one line are contiguous)
+  }
+  // reduce over the 16 lanes of the line (lanes of one line are contiguous)
+#pragma unroll
+  for (int kind = 0; kind < 2; ++kind) {
 #pragma unroll
     for (int off = LPL / 2; off > 0; off >>= 1) {
 #pragma unroll
       for (int q = 0; q < D2; ++q) {
-        acc[q].x += __shfl_xor(acc[q].x, off);
-        acc[q].y += __shfl_xor(acc[q].y, off);
+        acc[kind][q].x += __shfl_xor(acc[kind][q].x, off);
+        acc[kind][q].y += __shfl_xor(acc[kind][q].y, off);
       }
     }
     if (ok) {
       cpx<double>* out = kind ? XZb : XVb;
 #pragma unroll
       for (int q = 0; q < D2; ++q)
-        if (sub == q) out[o + q * nlines] = acc[q];
+        if (sub == q) out[o + q * nlines] = acc[kind][q];
     }
   }
 }
