@@ -169,6 +169,14 @@ struct spx_plan {
   void* DS = nullptr;       // diagnostics scratch: spectral (nb members), allocated on use
   void* DR = nullptr;       // diagnostics scratch: real (nb members)
   void* bc = nullptr;       // wall data of sol_project, 2 complex per (kx,ky) line
+  // harmonic-correction basis of sol_project: for each distinct khom value u,
+  // rows 2u and 2u+1 (nz complex each) hold FFT_z(cont(exp(kh (z - Lz)))) and
+  // FFT_z(cont(exp(-kh z))) (kh = 0: of z and of 1); d_hidx maps a line of any
+  // member to its u.  Null when it could not be allocated: the harmonic kernel
+  // then transforms the two profiles of every line itself.
+  void* d_htab = nullptr;
+  int* d_hidx = nullptr;
+  int nhtab = 0;
   double* part = nullptr;   // per-line partial sums of the diagnostics
   std::vector<double> h_part;
   std::vector<void*> owned;
@@ -280,6 +288,7 @@ int sconst_fused(spx_plan* p, void* th);
 int proj_bc(spx_plan* p, const void* c);             // bc <- IFFT_z(c/nz) at the walls
 int proj_pressure(spx_plan* p, void* d);             // d <- IFFT_z(d)/nz + harmonic a
 int proj_harmonic(spx_plan* p, void* a, void* b, void* c);  // v -= grad(harmonic)
+int build_harmonic_table(spx_plan* p);  // p->d_htab / d_hidx (kept null on allocation failure)
 // z IFFT of a linear combination of derivatives, reduced per line over physical z
 // mode 0: sum |u|^2 ; the combination is acc = m0(in0) (+/-) m1(in1) (+/-) m2(in2)
 struct LinComb {

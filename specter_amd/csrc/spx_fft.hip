@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <unordered_map>
 
 #include "spx_engine.h"
 
@@ -763,6 +764,71 @@ __global__ void __launch_bounds__(kMaxBlock) k_proj_harmonic(cpx<TS>* __restrict
   }
 }
 
+// The two harmonic profiles of each distinct khom value u on the physical
+// points, rows 2u and 2u+1 of tab (continued and transformed afterwards).
+__global__ void k_harm_profiles(Cd* __restrict__ tab, const double* __restrict__ khv, int U,
+                                int n, int np, const double* __restrict__ z, double Lz) {
+#pragma clang fp contract(off)
+  const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= long(U) * np) return;
+  const int u = int(e / np), k = int(e - long(u) * np);
+  const double kh = khv[u];
+  double a1, a2;
+  if (kh == 0.0) {
+    a1 = z[k];
+    a2 = 1.0;
+  } else {
+    a1 = exp(kh * (z[k] - Lz));
+    a2 = exp(-kh * z[k]);
+  }
+  tab[(2 * long(u)) * n + k] = mkc<double>(a1, 0.0);
+  tab[(2 * long(u) + 1) * n + k] = mkc<double>(a2, 0.0);
+}
+
+// Harmonic correction from the basis table, one wavefront per line:
+// A = c1 P1 + c2 P2, B = kh (c1 P1 - c2 P2) (kh = 0: A = Re c1 P1 + Re c2 P2,
+// B = Re c1 P2); vx -= i kx A, vy -= i ky A, vz -= B   (:383-399).
+template <typename TS>
+__global__ void __launch_bounds__(256) k_proj_harmonic_tab(cpx<TS>* __restrict__ va,
+                                                         cpx<TS>* __restrict__ vb,
+                                                         cpx<TS>* __restrict__ vc,
+                                                         const Cd* __restrict__ bc,
+                                                         const Cd* __restrict__ tab,
+                                                         const int* __restrict__ hidx,
+                                                         long nlines, int ny, int n, double Lz,
+                                                         const double* __restrict__ khom,
+                                                         const double* __restrict__ kx,
+                                                         const double* __restrict__ ky) {
+#pragma clang fp contract(off)
+  const long ln = long(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (ln >= nlines) return;
+  const long i = ln / ny;
+  const int j = int(ln - i * ny);
+  const double kh = khom[ln];
+  const bool zm = kh == 0.0;
+  const Coef cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
+  const Cd* P1 = tab + 2 * long(hidx[ln]) * n;
+  const Cd* P2 = P1 + n;
+  const double kxi = kx[i], kyj = ky[j];
+  for (int k = lane; k < n; k += 64) {
+    const Cd p1 = P1[k], p2 = P2[k];
+    Cd A, B;
+    if (zm) {
+      A = cadd(cscale(p1, cf.c1.x), cscale(p2, cf.c2.x));
+      B = cscale(p2, cf.c1.x);
+    } else {
+      const Cd u1 = cmul(cf.c1, p1), u2 = cmul(cf.c2, p2);
+      A = cadd(u1, u2);
+      B = cscale(csub(u1, u2), kh);
+    }
+    const long o = ln * n + k;
+    va[o] = ccast<TS>(csub(ccast<double>(va[o]), ikmul(kxi, A)));
+    vb[o] = ccast<TS>(csub(ccast<double>(vb[o]), ikmul(kyj, A)));
+    vc[o] = ccast<TS>(csub(ccast<double>(vc[o]), B));
+  }
+}
+
 // ---------------------------------------------------------------------------
 // diagnostics: per-line float64 reductions over physical z in the mixed domain
 // ---------------------------------------------------------------------------
@@ -1478,9 +1544,70 @@ int proj_pressure(spx_plan* p, void* d) {
   return SPX_OK;
 }
 
+int build_harmonic_table(spx_plan* p) {
+  const int n = p->g.nz, np = p->np, ny = p->g.ny;
+  const size_t lines1 = size_t(p->nkx) * ny;
+  std::unordered_map<uint64_t, int> seen;
+  std::vector<double> khv;
+  std::vector<int> idx(lines1 * size_t(p->nb));
+  for (size_t l = 0; l < lines1; ++l) {
+    uint64_t key;
+    std::memcpy(&key, &p->h_khom[l], sizeof(key));
+    auto it = seen.find(key);
+    if (it == seen.end()) {
+      it = seen.emplace(key, int(khv.size())).first;
+      khv.push_back(p->h_khom[l]);
+    }
+    idx[l] = it->second;
+  }
+  for (int b = 1; b < p->nb; ++b)
+    std::copy(idx.begin(), idx.begin() + long(lines1), idx.begin() + long(b * lines1));
+  const int U = int(khv.size());
+  void *tab = nullptr, *hid = nullptr, *kv = nullptr;
+  const size_t tab_bytes = 2 * size_t(U) * n * sizeof(Cd);
+  if (hipMalloc(&tab, tab_bytes) != hipSuccess) {
+    (void)hipGetLastError();  // no table: proj_harmonic transforms per line
+    return SPX_OK;
+  }
+  p->owned.push_back(tab);
+  SPX_HIP(hipMalloc(&hid, idx.size() * sizeof(int)));
+  p->owned.push_back(hid);
+  SPX_HIP(hipMalloc(&kv, khv.size() * sizeof(double)));
+  SPX_HIP(hipMemcpy(hid, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+  SPX_HIP(hipMemcpy(kv, khv.data(), khv.size() * sizeof(double), hipMemcpyHostToDevice));
+  const long tot = long(U) * np;
+  hipLaunchKernelGGL(k_harm_profiles, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, p->stream,
+                     static_cast<Cd*>(tab), static_cast<const double*>(kv), U, n, np, p->d_z,
+                     p->g.Lz);
+  SPX_CHECK_LAUNCH();
+  // continuation + forward z FFT of the 2U profile rows
+  const LineFFTHost& f = p->fz;
+  const long rows = 2L * U;
+  SPX_LAUNCH_RS(f, k_z_inplace, double, , dim3(unsigned((rows + f.nl - 1) / f.nl)), blk(f),
+                f.lds_bytes(16), p->stream, static_cast<Cd*>(tab), rows, f.dev<Cd>(), np, 1, 1,
+                1.0, p->d_dir, p->g.Cz, p->g.oz);
+  SPX_CHECK_LAUNCH();
+  SPX_HIP(hipStreamSynchronize(p->stream));
+  (void)hipFree(kv);
+  p->d_htab = tab;
+  p->d_hidx = static_cast<int*>(hid);
+  p->nhtab = U;
+  return SPX_OK;
+}
+
 int proj_harmonic(spx_plan* p, void* a, void* b, void* c) {
-  const LineFFTHost& f = p->fz2;
   const long nlines = p->wlines();
+  if (p->d_htab) {
+    ProfScope ps_(p, "proj_harmonic",
+                  6.0 * double(p->wnc()) * 16.0 + 2.0 * p->nhtab * double(p->g.nz) * 16.0);
+    hipLaunchKernelGGL(k_proj_harmonic_tab<double>, dim3(unsigned((nlines + 3) / 4)), dim3(256), 0,
+                       p->stream, C_<double>(a), C_<double>(b), C_<double>(c),
+                       C_<double>(p->bc), static_cast<const Cd*>(p->d_htab), p->d_hidx, nlines,
+                       p->g.ny, p->g.nz, p->g.Lz, p->d_khom, p->d_kx, p->d_ky);
+    SPX_CHECK_LAUNCH();
+    return SPX_OK;
+  }
+  const LineFFTHost& f = p->fz2;
   const size_t lds = f.lds_bytes(16, 2);
   ProfScope ps_(p, "proj_harmonic", 6.0 * double(p->wnc()) * 16.0);
   SPX_LAUNCH_RS(f, k_proj_harmonic, double, , dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
