@@ -305,19 +305,27 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_s2x(const cpx<TI>* __restrict__
   const int i = i0 + line;
   const bool ok = i < nkx;
   const long base = long(n) * (j + long(ny) * (ok ? i : 0));
-  Cp keep[DUAL ? KEEP : 1];
+  // the line (times i k_dirn).  The dual transform needs it again: float
+  // kernels keep it in registers, float64 kernels read it again (the second
+  // read hits in L2 and keeps the registers free for occupancy)
+  constexpr bool KEEPV = DUAL && sizeof(T) == 4;
+  auto load = [&](int k) {
+    Cp v = mkc<T>(0, 0);
+    if (ok) {
+      v = ccast<T>(S[base + k]);
+      if (dirn == 1) v = ikmul(T(kx[i]), v);
+      else if (dirn == 2) v = ikmul(T(ky[j]), v);
+      else if (dirn == 3) v = ikmul(T(kz[k]), v);
+    }
+    return v;
+  };
+  Cp keep[KEEPV ? KEEP : 1];
 #pragma unroll
   for (int q = 0; q < KEEP; ++q) {
     const int k = lane + q * f.tpl;
     if (k < n) {
-      Cp v = mkc<T>(0, 0);
-      if (ok) {
-        v = ccast<T>(S[base + k]);
-        if (dirn == 1) v = ikmul(T(kx[i]), v);
-        else if (dirn == 2) v = ikmul(T(ky[j]), v);
-        else if (dirn == 3) v = ikmul(T(kz[k]), v);
-      }
-      if constexpr (DUAL) keep[q] = v;
+      const Cp v = load(k);
+      if constexpr (KEEPV) keep[q] = v;
       xl[k] = cconj(v);
     }
   }
@@ -333,7 +341,10 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_s2x(const cpx<TI>* __restrict__
 #pragma unroll
     for (int q = 0; q < KEEP; ++q) {
       const int k = lane + q * f.tpl;
-      if (k < n) xl[k] = cconj(ikmul(T(kz[k]), keep[q]));
+      if (k < n) {
+        if constexpr (KEEPV) xl[k] = cconj(ikmul(T(kz[k]), keep[q]));
+        else xl[k] = cconj(ikmul(T(kz[k]), load(k)));
+      }
     }
     __syncthreads();
     line_dft<T, RS>(xl, f, tws, lane);
