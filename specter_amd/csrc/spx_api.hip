@@ -382,6 +382,10 @@ int spx_plan_get_info(const spx_plan* p, spx_plan_info* info) {
   if (p->XV[0]) b += 9 * p->nc * p->csize + 6 * p->nr * p->rsize + p->nr * 8;
   if (p->work_bouss) b += 3 * p->nc * p->csize + p->nr * p->rsize;
   info->device_bytes = b * size_t(p->nb);
+  // harmonic basis table (shared by the members) and its line index
+  if (p->d_htab)
+    info->device_bytes += 2 * size_t(p->nhtab) * p->g.nz * 16 +
+                          size_t(p->g.ny) * p->nkx * p->nb * sizeof(int);
   return SPX_OK;
 }
 
