@@ -9,10 +9,15 @@
 ! fields vx, vy, vz (and optionally fx, fy, fz, pr) are read from
 ! idir/<name>.NNNN.out as raw float64 arrays (nx, ny, nz-Cz) in Fortran order
 ! (mpiio/binary_io.f90:127-151).  The RK loop (specter.fpp:1002-1165) runs on
-! the GPU through libspx; every cstep steps the global quantities of
+! the GPU through libspx with the reference's restart counters
+! (specter.fpp:885-891): every cstep steps the global quantities of
 ! hd_global.f90 are appended to balance.txt (format of pseudospec_hd.f90:993)
-! and the no-slip diagnostics to noslip_diagnostic.txt at full precision.
-! At the end the velocity is written to odir/<name>.MMMM.out (MMMM = stat+1).
+! and the no-slip diagnostics to noslip_diagnostic.txt at full precision;
+! every tstep steps (bench = 0) the fields are written as in
+! specter.fpp:1003-1053: vx, vy, vz (and wx, wy, wz when outs >= 1) as
+! v/(nx ny nz) and pr as p'/(nx ny dt) on the physical planes, to
+! odir/<name>.MMMM.out.  x.txt, y.txt, z.txt hold the grid
+! (specter.fpp:752-765).
 !=======================================================================
 PROGRAM spx_hd
    USE, INTRINSIC :: iso_c_binding
@@ -41,7 +46,8 @@ PROGRAM spx_hd
    TYPE(C_PTR) :: vx, vy, vz, pr, fx, fy, fz
    REAL(8), ALLOCATABLE, TARGET :: R(:, :, :), A(:, :), Q(:, :)
    REAL(8) :: eng, ens, pot, dv, tb, tt, nb, nt
-   INTEGER :: t, timec, np, ios
+   INTEGER :: t, timec, timet, tind, ini, np, ios
+   REAL(8), ALLOCATABLE, TARGET :: R2(:, :, :)
    INTEGER(C_SIZE_T) :: rbytes, cbytes
    CHARACTER(len=4) :: ext
    LOGICAL :: have
@@ -110,11 +116,32 @@ PROGRAM spx_hd
       CALL spx_check(spx_fftp2d_real_to_complex_xy(plan, dreal, pr), 'pr r2c')
    ENDIF
 
+   CALL write_grid()
+
    ! RK loop with restart counters (specter.fpp:885-891, 1002-1165)
-   timec = 0
+   ini = (stat - 1) * tstep + 1
+   tind = stat
+   timet = 0
+   timec = MODULO(ini - 1, cstep)
    OPEN(2, file='balance.txt', position='append')
    OPEN(3, file='noslip_diagnostic.txt', position='append')
-   DO t = 1, step
+   DO t = ini, step
+      IF (timet .eq. tstep .AND. bench .eq. 0) THEN
+         timet = 0
+         tind = tind + 1
+         WRITE(ext, '(i4.4)') tind
+         IF (outs .ge. 1) THEN
+            ! curlk (pseudospec_hd.f90:130-206): wx = dy vz - dz vy,
+            ! wy = dz vx - dx vz, wz = dx vy - dy vx
+            CALL store_curl('wx', vz, 2, vy, 3)
+            CALL store_curl('wy', vx, 3, vz, 1)
+            CALL store_curl('wz', vy, 1, vx, 2)
+         ENDIF
+         CALL store_field('vx', vx)
+         CALL store_field('vy', vy)
+         CALL store_field('vz', vz)
+         CALL store_pressure()
+      ENDIF
       IF (timec .eq. cstep) THEN
          timec = 0
          CALL spx_check(spx_energy(plan, vx, vy, vz, 1, eng), 'energy')
@@ -127,16 +154,11 @@ PROGRAM spx_hd
          WRITE(3, '(1P 6E24.16)') (t-1)*dt, dv, tb, tt, nb, nt
       ENDIF
       CALL spx_check(spx_step(plan, state, 1), 'spx_step')
+      timet = timet + 1
       timec = timec + 1
    ENDDO
    CLOSE(2)
    CLOSE(3)
-
-   ! output (specter.fpp:1005-1040): v/(nx ny nz) in physical space
-   WRITE(ext, '(i4.4)') stat + 1
-   CALL store_field('vx', vx)
-   CALL store_field('vy', vy)
-   CALL store_field('vz', vz)
 
    CALL spx_check(spx_plan_sync(plan), 'sync')
    CALL spx_check(spx_dev_free(dreal), 'free')
@@ -204,10 +226,63 @@ CONTAINS
       CALL spx_check(spx_fftp3d_complex_to_real(plan, dtmp, dreal), 'c2r '//name)
       CALL spx_check(spx_memcpy_d2h(plan, C_LOC(R), dreal, rbytes), 'd2h '//name)
       R = R / (REAL(nx, 8) * REAL(ny, 8) * REAL(nz, 8))
+      CALL write_raw(name)
+   END SUBROUTINE store_field
+
+   ! R (physical planes) -> odir/<name>.ext.out, raw float64 (binary_io.f90:165-226)
+   SUBROUTINE write_raw(name)
+      CHARACTER(len=*), INTENT(IN) :: name
       OPEN(12, FILE=TRIM(odir)//'/'//name//'.'//ext//'.out', FORM='unformatted', &
            ACCESS='stream', STATUS='replace')
       WRITE(12) R(:, :, 1:np)
       CLOSE(12)
-   END SUBROUTINE store_field
+   END SUBROUTINE write_raw
+
+   ! one vorticity component: d_dp b - d_dm a, as v/(nx ny nz) in physical space
+   SUBROUTINE store_curl(name, b, dp, a, dm)
+      CHARACTER(len=*), INTENT(IN) :: name
+      TYPE(C_PTR), INTENT(IN) :: a, b
+      INTEGER, INTENT(IN) :: dp, dm
+      IF (.NOT. ALLOCATED(R2)) ALLOCATE(R2(nx, ny, nz))
+      CALL spx_check(spx_derivk(plan, a, dtmp, dm), 'derivk '//name)
+      CALL spx_check(spx_fftp3d_complex_to_real(plan, dtmp, dreal), 'c2r '//name)
+      CALL spx_check(spx_memcpy_d2h(plan, C_LOC(R2), dreal, rbytes), 'd2h '//name)
+      CALL spx_check(spx_derivk(plan, b, dtmp, dp), 'derivk '//name)
+      CALL spx_check(spx_fftp3d_complex_to_real(plan, dtmp, dreal), 'c2r '//name)
+      CALL spx_check(spx_memcpy_d2h(plan, C_LOC(R), dreal, rbytes), 'd2h '//name)
+      R = (R - R2) / (REAL(nx, 8) * REAL(ny, 8) * REAL(nz, 8))
+      CALL write_raw(name)
+   END SUBROUTINE store_curl
+
+   ! pressure p = p'/dt from the mixed (z, ky, kx) domain (specter.fpp:1041-1053)
+   SUBROUTINE store_pressure()
+      CALL spx_check(spx_memcpy_d2d(plan, dtmp, pr, cbytes), 'd2d pr')
+      CALL spx_check(spx_fftp2d_complex_to_real_xy(plan, dtmp, dreal), 'c2r pr')
+      CALL spx_check(spx_memcpy_d2h(plan, C_LOC(R), dreal, rbytes), 'd2h pr')
+      R = R / (REAL(nx, 8) * REAL(ny, 8) * dt)
+      CALL write_raw('pr')
+   END SUBROUTINE store_pressure
+
+   ! x.txt, y.txt, z.txt (specter.fpp:683-765: periodic x, y; z on the
+   ! physical points of the FC-Gram grid)
+   SUBROUTINE write_grid()
+      INTEGER :: i
+      REAL(8), PARAMETER :: pi = 3.14159265358979323846d0
+      OPEN(13, file='x.txt', action='write', status='replace')
+      DO i = 1, nx
+         WRITE(13, FMT='(1P E23.15)') (Lx * 2.d0 * pi / nx) * (i - 1)
+      ENDDO
+      CLOSE(13)
+      OPEN(13, file='y.txt', action='write', status='replace')
+      DO i = 1, ny
+         WRITE(13, FMT='(1P E23.15)') (Ly * 2.d0 * pi / ny) * (i - 1)
+      ENDDO
+      CLOSE(13)
+      OPEN(13, file='z.txt', action='write', status='replace')
+      DO i = 1, np
+         WRITE(13, FMT='(1P E23.15)') (Lz / (np - 1)) * (i - 1)
+      ENDDO
+      CLOSE(13)
+   END SUBROUTINE write_grid
 
 END PROGRAM spx_hd

@@ -18,13 +18,13 @@ PARAM = """&grid
 nx = {nx}, ny = {ny}, nz = {nz}, Cz = {C}, oz = {d}, ord = {ord}
 /
 &status
-idir = "in", odir = "out", tdir = "{tdir}", stat = 1, mult = 1, bench = 0, outs = 0
+idir = "in", odir = "out", tdir = "{tdir}", stat = 1, mult = 1, bench = 0, outs = 1
 /
 &boxparams
 Lx = 1.0, Ly = 1.0, Lz = 1.0
 /
 &parameter
-dt = {dt}, step = {step}, tstep = 1000, cstep = {cstep}, seed = 1000
+dt = {dt}, step = {step}, tstep = {tstep}, cstep = {cstep}, seed = 1000
 /
 &velocity
 f0 = 0.0, u0 = 1.0, kdn = 2.0, kup = 4.0, nu = {nu}
@@ -41,12 +41,13 @@ def test_fortran_driver_matches_python_and_oracle(tmp_path):
     if not os.path.exists(BIN):
         pytest.fail("Fortran driver not built (specter_amd.build.build_fortran)")
     nx, ny, nz, C, d, ordr, dt, nu, step, cstep = 32, 32, 57, 25, 5, 2, 1e-3, 2e-3, 12, 4
+    tstep = 8   # one output (index 0002) at the start of step 9: the fields after 8 steps
     npz = nz - C
     (tmp_path / "in").mkdir()
     (tmp_path / "out").mkdir()
     (tmp_path / "parameter.inp").write_text(PARAM.format(
         nx=nx, ny=ny, nz=nz, C=C, d=d, ord=ordr, tdir=TABLES, dt=dt, step=step,
-        cstep=cstep, nu=nu))
+        tstep=tstep, cstep=cstep, nu=nu))
     u = ic.spectrum_peaked_velocity(nx, ny, nz, C, seed=1000)
     fr = ic.large_scale_forcing(nx, ny, nz, C)
     for name, arr in zip(("vx", "vy", "vz", "fx", "fy", "fz"), (*u, *fr)):
@@ -70,7 +71,13 @@ def test_fortran_driver_matches_python_and_oracle(tmp_path):
                    *(O.fftp3d_real_to_complex(g, a.copy()) for a in fr))
     rows_py, rows_or = [], []
     timec = 0
+    snap = None
     for t in range(1, step + 1):
+        if t == tstep + 1:
+            snap = {n: sol.real_field(n) for n in ("vx", "vy", "vz")}
+            snap["pr"] = plan.fftp2d_complex_to_real_xy(sol.field("pr").clone(),
+                                                        plan.empty_real()).cpu().numpy()
+            snap_or = {n: getattr(st, n).copy() for n in ("vx", "vy", "vz")}
         if timec == cstep:
             timec = 0
             rows_py.append(((t - 1) * dt, *sol.hdcheck()))
@@ -85,15 +92,31 @@ def test_fortran_driver_matches_python_and_oracle(tmp_path):
     assert np.allclose(bal[:, 1:3], rows_or[:, 1:3], rtol=1e-11, atol=0)
     # <div^2>, wall <v_t^2>, <v_n^2> (vboundary.f90:214-269) on the energy scale
     assert np.max(np.abs(nsd[:, 1:] - rows_or[:, 4:])) < 1e-11 * np.max(rows_or[:, 1])
-    # output fields (physical space, v/(nx ny nz))
+    # output fields after tstep steps (physical space, v/(nx ny nz))
     for name in ("vx", "vy", "vz"):
         out = np.fromfile(tmp_path / "out" / f"{name}.0002.out").reshape(npz, ny, nx)
-        py = sol.real_field(name)
+        py = snap[name]
         diff = np.abs(out - py)
         print(f"{name}: driver vs Python max |diff| {diff.max():.3e} at "
               f"{np.unravel_index(diff.argmax(), diff.shape)}, rel L2 {rel_l2(out, py):.3e}")
         # same library trajectory; the driver's final v/(nx ny nz) is Fortran
         # arithmetic, which the compiler may evaluate as a reciprocal product
         assert diff.max() <= 4 * np.finfo(float).eps * np.abs(py).max(), name
-        ref = O.fftp3d_complex_to_real(g, getattr(st, name).copy())[:npz] / (nx * ny * nz)
+        ref = O.fftp3d_complex_to_real(g, snap_or[name].copy())[:npz] / (nx * ny * nz)
         assert rel_l2(out, ref) < 1e-11, name
+    # pressure p'/(nx ny dt) from the mixed domain (specter.fpp:1041-1053)
+    out = np.fromfile(tmp_path / "out" / "pr.0002.out").reshape(npz, ny, nx)
+    py = snap["pr"][:npz] / (nx * ny * dt)
+    assert np.abs(out - py).max() <= 4 * np.finfo(float).eps * np.abs(py).max()
+    # vorticity (outs = 1): wx = dy vz - dz vy etc. against the oracle
+    for name, (b, db, a, da) in {"wx": ("vz", 2, "vy", 3), "wy": ("vx", 3, "vz", 1),
+                                 "wz": ("vy", 1, "vx", 2)}.items():
+        out = np.fromfile(tmp_path / "out" / f"{name}.0002.out").reshape(npz, ny, nx)
+        w = O.derivk(g, snap_or[b], db) - O.derivk(g, snap_or[a], da)
+        ref = O.fftp3d_complex_to_real(g, w)[:npz] / (nx * ny * nz)
+        assert rel_l2(out, ref) < 1e-10, name
+    # no second output within 12 steps; the grid files
+    assert not (tmp_path / "out" / "vx.0003.out").exists()
+    x, y, z = ic.physical_grid(nx, ny, nz, C)
+    assert np.allclose(np.loadtxt(tmp_path / "x.txt"), x, rtol=1e-14, atol=1e-15)
+    assert np.allclose(np.loadtxt(tmp_path / "z.txt"), z[:npz], rtol=1e-14, atol=1e-15)
