@@ -131,8 +131,9 @@ class _Class:
         from . import ic
         b = len(self.runs)
         nx, ny, nz, C, _ = req.grid
-        u = ic.spectrum_peaked_velocity(nx, ny, nz, C, req.seed)
         with torch.cuda.stream(self.stream):
+            # generated on the class's stream: the host only enqueues work
+            u = ic.spectrum_peaked_velocity_device(nx, ny, nz, C, req.seed)
             for name in ("pr", "fx", "fy", "fz"):
                 self.sol.field(name)[b].zero_()
             for name, arr in zip(("vx", "vy", "vz"), u):

@@ -13,7 +13,9 @@ def _alone(req):
     nx, ny, nz, C, d = req.grid
     plan = FCPlan(nx, ny, nz, C, d, precision=req.precision)
     sol = Solver(plan, SolverParams(solver="HD", ord=req.ord, dt=req.dt, nu=req.nu))
-    for name, arr in zip(("vx", "vy", "vz"), ic.spectrum_peaked_velocity(nx, ny, nz, C, req.seed)):
+    # the server generates its initial fields on the GPU
+    u = ic.spectrum_peaked_velocity_device(nx, ny, nz, C, req.seed)
+    for name, arr in zip(("vx", "vy", "vz"), u):
         sol.set_real_field(name, arr)
     sol.step(req.nsteps)
     return {n: sol.spectral(n) for n in ("vx", "vy", "vz")}
