@@ -117,6 +117,23 @@ MODULE spx
          TYPE(C_PTR), VALUE :: plan, a, b
          REAL(C_DOUBLE), INTENT(OUT) :: bot, top
       END FUNCTION
+      ! scalar diagnostics (pseudospec_phd.f90:116-235) and the Nusselt number
+      INTEGER(C_INT) FUNCTION spx_variance(plan, a, kin, out) BIND(C, name='spx_variance')
+         IMPORT :: C_INT, C_PTR, C_DOUBLE
+         TYPE(C_PTR), VALUE :: plan, a
+         INTEGER(C_INT), VALUE :: kin
+         REAL(C_DOUBLE), INTENT(OUT) :: out
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION spx_product(plan, a, b, out) BIND(C, name='spx_product')
+         IMPORT :: C_INT, C_PTR, C_DOUBLE
+         TYPE(C_PTR), VALUE :: plan, a, b
+         REAL(C_DOUBLE), INTENT(OUT) :: out
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION spx_nusselt(plan, th, out) BIND(C, name='spx_nusselt')
+         IMPORT :: C_INT, C_PTR, C_DOUBLE
+         TYPE(C_PTR), VALUE :: plan, th
+         REAL(C_DOUBLE), INTENT(OUT) :: out(2)
+      END FUNCTION
       ! device-resident solver state (specter.fpp:1002-1165 RK loop)
       INTEGER(C_INT) FUNCTION spx_state_create(plan, prm, state) BIND(C, name='spx_state_create')
          IMPORT :: C_INT, C_PTR, spx_params

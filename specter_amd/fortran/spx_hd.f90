@@ -18,6 +18,16 @@
 ! v/(nx ny nz) and pr as p'/(nx ny dt) on the physical planes, to
 ! odir/<name>.MMMM.out.  x.txt, y.txt, z.txt hold the grid
 ! (specter.fpp:752-765).
+!
+! solver = 'BOUSS' in &grid (the reference's compile-time SOLVER choice)
+! runs the Boussinesq solver: &boussinesq (gama, xmom, xtemp; the
+! coefficients used are gama*xmom and gama*xtemp, specter.fpp:476-491) and
+! &scalar (kappa) are read, th (and optionally fs) start from
+! th.NNNN.out / fs.NNNN.out, th is written every tstep steps, and every
+! cstep steps scalar.txt (pseudospec_phd.f90:237-262 format),
+! scalar_constant_diagnostic.txt (<th^2> at the walls, sboundary.f90:167-210,
+! at full precision) and nusselt.txt (Nu at z = 0 and z = Lz for theta = 1 - z/Lz + th; the
+! reference computes no Nusselt number) are appended.
 !=======================================================================
 PROGRAM spx_hd
    USE, INTRINSIC :: iso_c_binding
@@ -25,6 +35,9 @@ PROGRAM spx_hd
    IMPLICIT NONE
 
    INTEGER :: nx = 64, ny = 64, nz = 89, Cz = 25, oz = 5, ord = 2, prec = 0
+   CHARACTER(len=10) :: solver = 'HD'
+   REAL(8) :: gama = 1.d0, xmom = 1.d0, xtemp = 1.d0
+   REAL(8) :: c0 = 0.d0, s0 = 0.d0, skdn = 1.d0, skup = 2.d0, kappa = 1.d-3
    CHARACTER(len=100) :: idir = 'in/', odir = 'out/', tdir = 'tables/'
    INTEGER :: stat = 1, mult = 1, bench = 0, outs = 0
    REAL(8) :: Lx = 1.d0, Ly = 1.d0, Lz = 1.d0
@@ -33,7 +46,9 @@ PROGRAM spx_hd
    REAL(8) :: f0 = 0.d0, u0 = 1.d0, kdn = 2.d0, kup = 4.d0, nu = 1.d-3
    CHARACTER(len=15) :: vbczsta = 'noslip', vbczend = 'noslip'
    REAL(8) :: vxzsta = 0.d0, vxzend = 0.d0, vyzsta = 0.d0, vyzend = 0.d0
-   NAMELIST /grid/ nx, ny, nz, Cz, oz, ord, prec
+   NAMELIST /grid/ nx, ny, nz, Cz, oz, ord, prec, solver
+   NAMELIST /boussinesq/ gama, xmom, xtemp
+   NAMELIST /scalar/ c0, s0, skdn, skup, kappa
    NAMELIST /status/ idir, odir, tdir, stat, mult, bench, outs
    NAMELIST /boxparams/ Lx, Ly, Lz
    NAMELIST /parameter/ dt, step, tstep, cstep, seed
@@ -43,7 +58,9 @@ PROGRAM spx_hd
    TYPE(spx_grid) :: g
    TYPE(spx_params) :: prm
    TYPE(C_PTR) :: plan, state, dreal, dtmp
-   TYPE(C_PTR) :: vx, vy, vz, pr, fx, fy, fz
+   TYPE(C_PTR) :: vx, vy, vz, pr, fx, fy, fz, th, fs
+   LOGICAL :: bouss
+   REAL(8) :: sv, sg, sp, nus(2)
    REAL(8), ALLOCATABLE, TARGET :: R(:, :, :), A(:, :), Q(:, :)
    REAL(8) :: eng, ens, pot, dv, tb, tt, nb, nt
    INTEGER :: t, timec, timet, tind, ini, np, ios
@@ -64,6 +81,18 @@ PROGRAM spx_hd
    READ(1, NML=velocity)
    REWIND(1)
    READ(1, NML=velbound, IOSTAT=ios)
+   bouss = TRIM(solver) .eq. 'BOUSS'
+   IF (bouss) THEN
+      REWIND(1)
+      READ(1, NML=boussinesq)
+      REWIND(1)
+      READ(1, NML=scalar)
+      xmom = xmom * gama
+      xtemp = xtemp * gama
+   ELSE IF (TRIM(solver) .ne. 'HD') THEN
+      PRINT *, '[ERROR] solver must be HD or BOUSS. Aborting...'
+      STOP 1
+   ENDIF
    CLOSE(1)
    IF (TRIM(vbczsta) .ne. 'noslip' .OR. TRIM(vbczend) .ne. 'noslip') THEN
       PRINT *, '[ERROR] only no-slip walls in z are supported. Aborting...'
@@ -82,8 +111,13 @@ PROGRAM spx_hd
 
    g = spx_grid(nx, ny, nz, Cz, oz, Lx, Ly, Lz, prec)
    CALL spx_check(spx_plan_create(g, A, Q, C_NULL_PTR, plan), 'spx_plan_create')
-   prm = spx_params(SPX_SOLVER_HD, ord, dt, nu, 0.d0, 0.d0, 0.d0, &
-                    (/vxzsta, vyzsta/), (/vxzend, vyzend/))
+   IF (bouss) THEN
+      prm = spx_params(SPX_SOLVER_BOUSS, ord, dt, nu, kappa, xmom, xtemp, &
+                       (/vxzsta, vyzsta/), (/vxzend, vyzend/))
+   ELSE
+      prm = spx_params(SPX_SOLVER_HD, ord, dt, nu, 0.d0, 0.d0, 0.d0, &
+                       (/vxzsta, vyzsta/), (/vxzend, vyzend/))
+   ENDIF
    CALL spx_check(spx_state_create(plan, prm, state), 'spx_state_create')
    CALL spx_check(spx_state_field(state, 'vx'//C_NULL_CHAR, vx), 'field vx')
    CALL spx_check(spx_state_field(state, 'vy'//C_NULL_CHAR, vy), 'field vy')
@@ -92,6 +126,10 @@ PROGRAM spx_hd
    CALL spx_check(spx_state_field(state, 'fx'//C_NULL_CHAR, fx), 'field fx')
    CALL spx_check(spx_state_field(state, 'fy'//C_NULL_CHAR, fy), 'field fy')
    CALL spx_check(spx_state_field(state, 'fz'//C_NULL_CHAR, fz), 'field fz')
+   IF (bouss) THEN
+      CALL spx_check(spx_state_field(state, 'th'//C_NULL_CHAR, th), 'field th')
+      CALL spx_check(spx_state_field(state, 'fs'//C_NULL_CHAR, fs), 'field fs')
+   ENDIF
 
    ALLOCATE(R(nx, ny, nz))
    rbytes = INT(nx, C_SIZE_T) * ny * nz * 8
@@ -107,6 +145,10 @@ PROGRAM spx_hd
    CALL load_field('fx', .FALSE., fx, have)
    CALL load_field('fy', .FALSE., fy, have)
    CALL load_field('fz', .FALSE., fz, have)
+   IF (bouss) THEN
+      CALL load_field('th', .TRUE., th, have)
+      CALL load_field('fs', .FALSE., fs, have)
+   ENDIF
    ! pressure p' = dt*p in the mixed domain (specter.fpp:901-910)
    CALL read_raw(TRIM(idir)//'/pr.'//ext//'.out', have)
    IF (have) THEN
@@ -125,6 +167,11 @@ PROGRAM spx_hd
    timec = MODULO(ini - 1, cstep)
    OPEN(2, file='balance.txt', position='append')
    OPEN(3, file='noslip_diagnostic.txt', position='append')
+   IF (bouss) THEN
+      OPEN(4, file='scalar.txt', position='append')
+      OPEN(7, file='scalar_constant_diagnostic.txt', position='append')
+      OPEN(8, file='nusselt.txt', position='append')
+   ENDIF
    DO t = ini, step
       IF (timet .eq. tstep .AND. bench .eq. 0) THEN
          timet = 0
@@ -141,6 +188,7 @@ PROGRAM spx_hd
          CALL store_field('vy', vy)
          CALL store_field('vz', vz)
          CALL store_pressure()
+         IF (bouss) CALL store_field('th', th)
       ENDIF
       IF (timec .eq. cstep) THEN
          timec = 0
@@ -152,6 +200,17 @@ PROGRAM spx_hd
          CALL spx_check(spx_bouncheck_z(plan, vx, vy, tb, tt), 'bouncheck')
          CALL spx_check(spx_bouncheck_z(plan, vz, C_NULL_PTR, nb, nt), 'bouncheck')
          WRITE(3, '(1P 6E24.16)') (t-1)*dt, dv, tb, tt, nb, nt
+         IF (bouss) THEN
+            ! pscheck (pseudospec_phd.f90:237-262): <th^2>, <|k^2 th|^2>, <th fs>
+            CALL spx_check(spx_variance(plan, th, 1, sv), 'variance')
+            CALL spx_check(spx_variance(plan, th, 0, sg), 'variance')
+            CALL spx_check(spx_product(plan, th, fs, sp), 'product')
+            WRITE(4, '(1P E13.6,2E22.14,E23.14)') (t-1)*dt, sv, sg, sp
+            CALL spx_check(spx_bouncheck_z(plan, th, C_NULL_PTR, tb, tt), 'bouncheck th')
+            WRITE(7, '(1P 3E24.16)') (t-1)*dt, tb, tt
+            CALL spx_check(spx_nusselt(plan, th, nus), 'nusselt')
+            WRITE(8, '(1P 3E24.16)') (t-1)*dt, nus(1), nus(2)
+         ENDIF
       ENDIF
       CALL spx_check(spx_step(plan, state, 1), 'spx_step')
       timet = timet + 1
@@ -159,6 +218,11 @@ PROGRAM spx_hd
    ENDDO
    CLOSE(2)
    CLOSE(3)
+   IF (bouss) THEN
+      CLOSE(4)
+      CLOSE(7)
+      CLOSE(8)
+   ENDIF
 
    CALL spx_check(spx_plan_sync(plan), 'sync')
    CALL spx_check(spx_dev_free(dreal), 'free')

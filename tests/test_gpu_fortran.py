@@ -15,7 +15,13 @@ pytestmark = pytest.mark.gpu
 BIN = os.path.join(ROOT, "specter_amd", "fortran", "spx_hd")
 
 PARAM = """&grid
-nx = {nx}, ny = {ny}, nz = {nz}, Cz = {C}, oz = {d}, ord = {ord}
+nx = {nx}, ny = {ny}, nz = {nz}, Cz = {C}, oz = {d}, ord = {ord}, solver = "{solver}"
+/
+&boussinesq
+gama = 0.5, xmom = 2.0, xtemp = 1.0
+/
+&scalar
+c0 = 0.0, s0 = 0.0, skdn = 1.0, skup = 2.0, kappa = {kappa}
 /
 &status
 idir = "in", odir = "out", tdir = "{tdir}", stat = 1, mult = 1, bench = 0, outs = 1
@@ -47,7 +53,7 @@ def test_fortran_driver_matches_python_and_oracle(tmp_path):
     (tmp_path / "out").mkdir()
     (tmp_path / "parameter.inp").write_text(PARAM.format(
         nx=nx, ny=ny, nz=nz, C=C, d=d, ord=ordr, tdir=TABLES, dt=dt, step=step,
-        tstep=tstep, cstep=cstep, nu=nu))
+        tstep=tstep, cstep=cstep, nu=nu, solver="HD", kappa=1e-3))
     u = ic.spectrum_peaked_velocity(nx, ny, nz, C, seed=1000)
     fr = ic.large_scale_forcing(nx, ny, nz, C)
     for name, arr in zip(("vx", "vy", "vz", "fx", "fy", "fz"), (*u, *fr)):
@@ -120,3 +126,71 @@ def test_fortran_driver_matches_python_and_oracle(tmp_path):
     x, y, z = ic.physical_grid(nx, ny, nz, C)
     assert np.allclose(np.loadtxt(tmp_path / "x.txt"), x, rtol=1e-14, atol=1e-15)
     assert np.allclose(np.loadtxt(tmp_path / "z.txt"), z[:npz], rtol=1e-14, atol=1e-15)
+
+
+def test_fortran_driver_boussinesq(tmp_path):
+    """solver = BOUSS: the scalar series (scalar.txt, wall <th^2>, Nusselt)
+    and the th output against the Python-driven run and the oracle."""
+    from oracle import specter_ref as O
+    from specter_amd import FCPlan, Solver, SolverParams, ic
+    if not os.path.exists(BIN):
+        pytest.fail("Fortran driver not built (specter_amd.build.build_fortran)")
+    nx, ny, nz, C, d, ordr, dt, nu, kappa, step, cstep = 16, 16, 41, 25, 5, 4, 1e-3, 2e-3, 3e-3, 7, 2
+    xmom, xtemp = 0.5 * 2.0, 0.5 * 1.0          # gama * xmom, gama * xtemp
+    tstep = 5
+    npz = nz - C
+    (tmp_path / "in").mkdir()
+    (tmp_path / "out").mkdir()
+    (tmp_path / "parameter.inp").write_text(PARAM.format(
+        nx=nx, ny=ny, nz=nz, C=C, d=d, ord=ordr, tdir=TABLES, dt=dt, step=step,
+        tstep=tstep, cstep=cstep, nu=nu, solver="BOUSS", kappa=kappa))
+    u = ic.spectrum_peaked_velocity(nx, ny, nz, C, seed=1000)
+    th = 1e-2 * ic.spectrum_peaked_velocity(nx, ny, nz, C, seed=1001)[0]
+    fs = 0.1 * ic.spectrum_peaked_velocity(nx, ny, nz, C, seed=1002)[1]
+    for name, arr in zip(("vx", "vy", "vz", "th", "fs"), (*u, th, fs)):
+        np.ascontiguousarray(arr[:npz]).tofile(tmp_path / "in" / f"{name}.0001.out")
+    res = subprocess.run([BIN], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    sca = np.loadtxt(tmp_path / "scalar.txt", ndmin=2)
+    scd = np.loadtxt(tmp_path / "scalar_constant_diagnostic.txt", ndmin=2)
+    nus = np.loadtxt(tmp_path / "nusselt.txt", ndmin=2)
+    assert sca.shape[0] == scd.shape[0] == nus.shape[0] == (step - 1) // cstep
+
+    prm = SolverParams(solver="BOUSS", ord=ordr, dt=dt, nu=nu, kappa=kappa, xmom=xmom,
+                       xtemp=xtemp)
+    plan = FCPlan(nx, ny, nz, C, d)
+    sol = Solver(plan, prm)
+    for name, arr in zip(("vx", "vy", "vz", "th", "fs"), (*u, th, fs)):
+        sol.set_real_field(name, arr)
+    g = O.make_grid(nx, ny, nz, C, d, TABLES)
+    zero = np.zeros(g.spec_shape, complex)
+    st = O.HDState(*(O.fftp3d_real_to_complex(g, a.copy()) for a in u), zero.copy(),
+                   zero.copy(), zero.copy(), zero.copy())
+    st.th = O.fftp3d_real_to_complex(g, th.copy())
+    st.fs = O.fftp3d_real_to_complex(g, fs.copy())
+    rows_py, rows_or, snap = [], [], None
+    timec = 0
+    for t in range(1, step + 1):
+        if t == tstep + 1:
+            snap = sol.real_field("th")
+            snap_or = st.th.copy()
+        if timec == cstep:
+            timec = 0
+            rows_py.append(((t - 1) * dt, *sol.pscheck()))
+            rows_or.append(((t - 1) * dt, *O.pscheck(g, st.th, st.fs),
+                            *O.bouncheck_z(g, st.th), *O.nusselt(g, st.th)))
+        sol.step(1)
+        O.bouss_step(g, st, ordr, dt, nu, kappa, xmom, xtemp)
+        timec += 1
+    rows_py = np.array(rows_py)
+    rows_or = np.array(rows_or)
+    # scalar.txt carries 14 decimal digits (E22.14)
+    assert np.allclose(sca[:, 1:4], rows_py[:, 1:4], rtol=1e-12, atol=0)
+    assert np.allclose(sca[:, 1:3], rows_or[:, 1:3], rtol=1e-9, atol=0)
+    scale = np.max(rows_or[:, 1])
+    assert np.max(np.abs(scd[:, 1:] - rows_or[:, 4:6])) < 1e-9 * scale
+    assert np.max(np.abs(nus[:, 1:] - rows_or[:, 6:8])) < 1e-9
+    out = np.fromfile(tmp_path / "out" / "th.0002.out").reshape(npz, ny, nx)
+    assert np.abs(out - snap).max() <= 4 * np.finfo(float).eps * np.abs(snap).max()
+    ref = O.fftp3d_complex_to_real(g, snap_or)[:npz] / (nx * ny * nz)
+    assert rel_l2(out, ref) < 1e-9
