@@ -12,7 +12,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspx.so")
-SOURCES = ["spx_fft.hip", "spx_ops.hip", "spx_api.hip"]
+# (source, defines): spx_fft.hip is compiled in three parts (see its header)
+SOURCES = [("spx_fft.hip", ["-DSPX_PART=1"]), ("spx_fft.hip", ["-DSPX_PART=2"]),
+           ("spx_fft.hip", ["-DSPX_PART=3"]), ("spx_ops.hip", []), ("spx_api.hip", [])]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -72,10 +74,11 @@ def build(verbose: bool = False) -> str:
     odir = os.path.join(CSRC, "_build")
     os.makedirs(odir, exist_ok=True)
     objs, procs = [], []
-    for src in SOURCES:
-        obj = os.path.join(odir, os.path.splitext(src)[0] + ".o")
+    for src, defs in SOURCES:
+        tag = "".join(d.split("=")[-1] for d in defs)
+        obj = os.path.join(odir, os.path.splitext(src)[0] + (f"_{tag}" if tag else "") + ".o")
         objs.append(obj)
-        cmd = [hipcc(), *HIPCC_FLAGS, "-c", "-I", CSRC, os.path.join(CSRC, src), "-o", obj]
+        cmd = [hipcc(), *HIPCC_FLAGS, *defs, "-c", "-I", CSRC, os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,

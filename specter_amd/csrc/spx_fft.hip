@@ -3,6 +3,11 @@
 // gradient-product x pass of gradre/advect, and the z-line kernels of the
 // boundary / projection / diagnostics stages.
 //
+// Build: the host part of this file is compiled in three parts so that the
+// kernel instantiations build in parallel (SPX_PART = 1: plan tables and z
+// transforms, 2: y and x transforms, 3: boundary, projection and diagnostics
+// launchers); without SPX_PART everything is compiled in one unit.
+//
 // Precision: the transform kernels are templated on their arithmetic / buffer
 // type T (double; float for the nonlinear pipeline of mixed-precision plans,
 // where the first z pass reads the float64 state, TI = double).  The
@@ -777,7 +782,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_proj_harmonic(cpx<TS>* __restrict
 
 // The two harmonic profiles of each distinct khom value u on the physical
 // points, rows 2u and 2u+1 of tab (continued and transformed afterwards).
-__global__ void k_harm_profiles(Cd* __restrict__ tab, const double* __restrict__ khv, int U,
+static __global__ void k_harm_profiles(Cd* __restrict__ tab, const double* __restrict__ khv, int U,
                                 int n, int np, const double* __restrict__ z, double Lz) {
 #pragma clang fp contract(off)
   const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -942,6 +947,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_zline_reduce(LinCombDev<TS> lc,
   if (lane == 0 && ok) out[ln] = s;
 }
 
+#if !defined(SPX_PART) || SPX_PART == 1
 // ---------------------------------------------------------------------------
 // plan construction: algorithm choice, factorisation, long-double tables
 // ---------------------------------------------------------------------------
@@ -1200,6 +1206,7 @@ void free_line_fft(LineFFTHost& f) {
   if (f.perm) (void)hipFree(f.perm);
   f.tw = f.chirp = f.bfil = f.perm = nullptr;
 }
+#endif  // SPX_PART 1 (plan tables)
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -1217,7 +1224,7 @@ static int lds_limit(K kernel, size_t bytes) {
 
 // radix set of a plan: the smallest compiled kernel family that covers it
 // (Rader and Bluestein plans never use the power-of-two family)
-static int rset_of(const LineFFTHost& f) {
+[[maybe_unused]] static int rset_of(const LineFFTHost& f) {
   if (f.mode != 0 && f.tab == f.M) return RS_GTAB;  // radices <= 9 (build_line_fft)
   int rs = f.mode == 0 ? RS_POW2 : RS_SMOOTH;
   for (int i = 0; i < f.nstage; ++i) {
@@ -1263,11 +1270,12 @@ static inline const LineFFTHost& fxp(const spx_plan* p, bool lo) { return lo ? p
 static inline const LineFFTHost& fyp(const spx_plan* p, bool lo) { return lo ? p->fy32 : p->fy; }
 static inline const LineFFTHost& fzp(const spx_plan* p, bool lo) { return lo ? p->fz32 : p->fz; }
 
-static int check_lo(const spx_plan* p, bool lo) {
+[[maybe_unused]] static int check_lo(const spx_plan* p, bool lo) {
   if (lo && !p->fp32) return fail(SPX_ERR_STATE, "float32 kernels need a mixed-precision plan");
   return SPX_OK;
 }
 
+#if !defined(SPX_PART) || SPX_PART == 1
 int zfft_inplace(spx_plan* p, void* S, int forward, int cont, double scale) {
   const LineFFTHost& f = p->fz;
   const long nlines = p->wlines();
@@ -1386,7 +1394,9 @@ int zfft_s2x(spx_plan* p, const void* S, void* X, int dirn, int nzout, int nofft
   return lo ? zfft_s2x_t<float>(p, S, X, dirn, nzout, nofft, X2, p->fz32)
             : zfft_s2x_t<double>(p, S, X, dirn, nzout, nofft, X2, p->fz);
 }
+#endif  // SPX_PART 1 (z transforms)
 
+#if !defined(SPX_PART) || SPX_PART == 2
 template <typename T>
 static int yfft_t(spx_plan* p, void* X, int forward, int nplanes, const void* Xin, void* Y2,
                   const LineFFTHost& f) {
@@ -1476,7 +1486,9 @@ int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, cons
   return lo ? xgrad_prod_t<float>(p, YV, YY, XZ, u, v, w, out, p->fx32, nplanes, ncomp)
             : xgrad_prod_t<double>(p, YV, YY, XZ, u, v, w, out, p->fx, nplanes, ncomp);
 }
+#endif  // SPX_PART 2 (y and x transforms)
 
+#if !defined(SPX_PART) || SPX_PART == 3
 // ---- float64 boundary / projection / diagnostics kernels ----
 static int noslip_launch(spx_plan* p, void* vx, void* vy, const void* pr, const WallParams& wp,
                          int ncomp, const char* name, const RkLoad* rk) {
@@ -1674,5 +1686,6 @@ int zline_walls(spx_plan* p, const void* a, double* line_out) {
   lc.sgn[1] = lc.sgn[2] = 0;
   return zline_launch(p, lc, nullptr, 2, line_out);
 }
+#endif  // SPX_PART 3 (boundary, projection, diagnostics)
 
 }  // namespace spx
