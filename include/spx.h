@@ -75,7 +75,11 @@ typedef struct {
  * load_dirichlet_tables (:180-257): A is the raw A{C}-{d}.dat table (C x d,
  * Fortran order), Q the raw Q{d}.dat table (d x d, Fortran order); the plan
  * forms dir = A*Q^T in float64.  A/Q may be NULL when Cz == 0.  stream is a
- * hipStream_t (NULL = default stream). */
+ * hipStream_t (NULL = default stream).  Plans with walls also hold the
+ * harmonic-correction basis of sol_project, 2 * (distinct sqrt(kx^2+ky^2))
+ * * nz complex float64 values (53 MB at 256^3, 2.8 GB at 1024^3); with the
+ * environment variable SPX_HARMONIC_TABLE=0 at creation, or when it cannot be
+ * allocated, the correction transforms its profiles per line instead. */
 int spx_plan_create(const spx_grid* g, const double* A, const double* Q, void* stream,
                     spx_plan** out);
 /* Replaces fcgram_destroy_plan (fftp/fcgram_mod.f90:147-177). */

@@ -15,6 +15,7 @@
 // templated on the storage type TS of the fields and instantiated for double.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <unordered_map>
 
@@ -1568,6 +1569,10 @@ int proj_pressure(spx_plan* p, void* d) {
 }
 
 int build_harmonic_table(spx_plan* p) {
+  // SPX_HARMONIC_TABLE=0: no table (the harmonic kernel transforms per line;
+  // for plans whose memory is tighter than 2 * (distinct kh) * nz complex values)
+  if (const char* e = std::getenv("SPX_HARMONIC_TABLE"))
+    if (e[0] == '0') return SPX_OK;
   const int n = p->g.nz, np = p->np, ny = p->g.ny;
   const size_t lines1 = size_t(p->nkx) * ny;
   std::unordered_map<uint64_t, int> seen;

@@ -171,6 +171,26 @@ def test_v_imposebc_and_project(shape, o, ordr):
 
 
 @pytest.mark.parametrize("shape", SHAPES[:3], ids=lambda s: "x".join(map(str, s)))
+def test_sol_project_without_basis_table(shape, monkeypatch):
+    """SPX_HARMONIC_TABLE=0: the harmonic correction transforms its profiles
+    per line; same results as the table path and the oracle."""
+    torch, O, g, p = _setup(shape)
+    from specter_amd import FCPlan
+    monkeypatch.setenv("SPX_HARMONIC_TABLE", "0")
+    q = FCPlan(*shape)
+    vx, vy, vz, pr = (_rand_spec(g, s) for s in (18, 19, 20, 21))
+    r = [x.copy() for x in (vx, vy, vz, pr)]
+    O.v_imposebc_and_project(g, *r, 2, 2, (0.0, 0.0), (0.0, 0.0))
+    ta = [_cu(torch, x) for x in (vx, vy, vz, pr)]
+    tb = [_cu(torch, x) for x in (vx, vy, vz, pr)]
+    p.v_imposebc_and_project(*ta, 2, 2)
+    q.v_imposebc_and_project(*tb, 2, 2)
+    for a, b, rr in zip(ta, tb, r):
+        assert rel_err(b.cpu().numpy(), rr) < _proj_tol(shape)
+        assert rel_err(b.cpu().numpy(), a.cpu().numpy()) < 1e-12
+
+
+@pytest.mark.parametrize("shape", SHAPES[:3], ids=lambda s: "x".join(map(str, s)))
 def test_s_imposebc(shape):
     torch, O, g, p = _setup(shape)
     th = _rand_spec(g, 22)
