@@ -344,6 +344,7 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
   const size_t nlines = size_t(ny) * p->nkx * nmembers;
   if ((rc = alloc_dev(p, &p->X, p->nc * nmembers * 16))) return bail(rc);
   if ((rc = alloc_dev(p, &p->bc, nlines * 2 * 16))) return bail(rc);
+  if ((rc = alloc_dev(p, &p->pw, nlines * 2 * 16))) return bail(rc);
   // per-line partials of the diagnostics (up to 4 per line), block maxima (64 per member)
   const size_t nparts = std::max(nlines * 4, size_t(64) * nmembers);
   if ((rc = alloc_dev(p, reinterpret_cast<void**>(&p->part), nparts * sizeof(double))))
@@ -973,7 +974,11 @@ int spx_rkstep1(spx_plan* p, spx_state* s) {
   return op_copy_fields(p, s->th ? 4 : 3, src, dst);
 }
 
-int spx_substep(spx_plan* p, spx_state* s, int o) {
+// One RK stage.  full_pressure = false (inside a full step, for every stage
+// but the last): the pressure is formed only at the two walls, which is all
+// the next stage's wall update reads; the last stage of the step forms all of
+// it, so the state after a step is the reference's.
+static int substep_impl(spx_plan* p, spx_state* s, int o, bool full_pressure) {
   SPX_REQ(p && s && s->p == p, "spx_substep: bad plan/state");
   const spx_params& q = s->prm;
   SPX_REQ(o >= 1 && o <= q.ord, "spx_substep: stage out of range");
@@ -1012,8 +1017,9 @@ int spx_substep(spx_plan* p, spx_state* s, int o) {
     if ((rc = noslip_fused(p, s->vx, s->vy, s->pr, tau, q.v_zsta, q.v_zend, &rk))) return rc;
     RkLoad rz = rk;
     rz.c[0] = RkLoadComp{c0[2], nl[2], f[2], s->d_nu};
-    if ((rc = op_poisson_project_bc(p, s->vx, s->vy, s->vz, s->pr, &rz))) return rc;
-    if ((rc = proj_pressure(p, s->pr))) return rc;
+    if ((rc = op_poisson_project_bc(p, s->vx, s->vy, s->vz, s->pr, &rz, !full_pressure)))
+      return rc;
+    if ((rc = full_pressure ? proj_pressure(p, s->pr) : pressure_walls(p, s->pr))) return rc;
     return proj_harmonic(p, s->vx, s->vy, s->vz);
   }
   // Boussinesq: vz and th are updated together first (the buoyancy and heat
@@ -1022,8 +1028,9 @@ int spx_substep(spx_plan* p, spx_state* s, int o) {
                          p->fp32)))
     return rc;
   if ((rc = noslip_fused(p, s->vx, s->vy, s->pr, tau, q.v_zsta, q.v_zend, &rk))) return rc;
-  if ((rc = op_poisson_project_bc(p, s->vx, s->vy, s->vz, s->pr))) return rc;
-  if ((rc = proj_pressure(p, s->pr))) return rc;
+  if ((rc = op_poisson_project_bc(p, s->vx, s->vy, s->vz, s->pr, nullptr, !full_pressure)))
+    return rc;
+  if ((rc = full_pressure ? proj_pressure(p, s->pr) : pressure_walls(p, s->pr))) return rc;
   if ((rc = proj_harmonic(p, s->vx, s->vy, s->vz))) return rc;
   if ((rc = spx_s_imposebc(p, s->th))) return rc;
   if ((rc = op_filter(p, s->th))) return rc;
@@ -1034,11 +1041,13 @@ int spx_substep(spx_plan* p, spx_state* s, int o) {
   return r2c3(p, p->R0, s->th);
 }
 
+int spx_substep(spx_plan* p, spx_state* s, int o) { return substep_impl(p, s, o, true); }
+
 static int step_once(spx_plan* p, spx_state* s) {
   int rc;
   if ((rc = spx_rkstep1(p, s))) return rc;
   for (int o = s->prm.ord; o >= 1; --o)
-    if ((rc = spx_substep(p, s, o))) return rc;
+    if ((rc = substep_impl(p, s, o, o == 1))) return rc;
   return SPX_OK;
 }
 

@@ -169,6 +169,7 @@ struct spx_plan {
   void* DS = nullptr;       // diagnostics scratch: spectral (nb members), allocated on use
   void* DR = nullptr;       // diagnostics scratch: real (nb members)
   void* bc = nullptr;       // wall data of sol_project, 2 complex per (kx,ky) line
+  void* pw = nullptr;       // IFFT_z(d)/nz at the two walls, 2 complex per line
   // harmonic-correction basis of sol_project: for each distinct khom value u,
   // rows 2u and 2u+1 (nz complex each) hold FFT_z(cont(exp(kh (z - Lz)))) and
   // FFT_z(cont(exp(-kh z))) (kh = 0: of z and of 1); d_hidx maps a line of any
@@ -287,6 +288,9 @@ int sconst_fused(spx_plan* p, void* th);
 // sol_project stages (bctarget = 1, Neumann potential)
 int proj_bc(spx_plan* p, const void* c);             // bc <- IFFT_z(c/nz) at the walls
 int proj_pressure(spx_plan* p, void* d);             // d <- IFFT_z(d)/nz + harmonic a
+// proj_pressure at z = 0 and z = np-1 only, from p->pw (the rest of d keeps
+// the potential): all the next stage's wall update reads
+int pressure_walls(spx_plan* p, void* d);
 int proj_harmonic(spx_plan* p, void* a, void* b, void* c);  // v -= grad(harmonic)
 int build_harmonic_table(spx_plan* p);  // p->d_htab / d_hidx (kept null on allocation failure)
 // z IFFT of a linear combination of derivatives, reduced per line over physical z
@@ -314,8 +318,11 @@ int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, 
                  const double* const* nu, double dt, double rmp, bool nl_lo = false);
 int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d);
 // poisson_project plus the wall data bc of sol_project (replaces proj_bc)
+// walls_only: d is left unwritten at z = 0 and z = np-1 and IFFT_z(d)/nz at
+// those two points goes to p->pw (for pressure_walls)
 int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d,
-                          const RkLoad* rk = nullptr);  // rk: update of c (c[0]) on load
+                          const RkLoad* rk = nullptr,  // rk: update of c (c[0]) on load
+                          bool walls_only = false);
 int op_axpy_field(spx_plan* p, void* y, const void* x, double alpha);
 int op_copy_fields(spx_plan* p, int n, const void* const* src, void* const* dst);
 // d <- i kx a + i ky b + i kz c (all active members)

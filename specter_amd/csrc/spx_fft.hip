@@ -702,6 +702,33 @@ __global__ void __launch_bounds__(kMaxBlock) k_proj_pressure(Cd* __restrict__ dd
   }
 }
 
+// k_proj_pressure at z = 0 and z = np-1 only, one thread per line, from the
+// wall values pw = IFFT_z(d)/nz formed by the projection kernel
+static __global__ void k_pressure_walls(Cd* __restrict__ dd, const Cd* __restrict__ bc,
+                                 const Cd* __restrict__ pw, long nlines, int n, int np,
+                                 double Lz, const double* __restrict__ khom,
+                                 const double* __restrict__ z) {
+#pragma clang fp contract(off)
+  const long ln = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (ln >= nlines) return;
+  const double kh = khom[ln];
+  const bool zm = kh == 0.0;
+  const Coef cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
+#pragma unroll
+  for (int w = 0; w < 2; ++w) {
+    const int k = w ? np - 1 : 0;
+    Cd a;
+    if (zm) {
+      a = mkc<double>(cf.c1.x * z[k] + cf.c2.x, 0.0);
+    } else {
+      const double ep = exp(kh * (z[k] - Lz));
+      const double em = exp(-kh * z[k]);
+      a = cadd(cscale(cf.c1, ep), cscale(cf.c2, em));
+    }
+    dd[ln * n + k] = cadd(pw[2 * ln + w], a);
+  }
+}
+
 // harmonic correction: A = FFT_z(cont(a_harm)), B = FFT_z(cont(d a_harm/dz));
 // vx -= i kx A, vy -= i ky A, vz -= B   (:383-399).  Two buffers per line.
 template <typename TS, int RS>
@@ -1564,6 +1591,16 @@ int proj_pressure(spx_plan* p, void* d) {
     hipLaunchKernelGGL(k_proj_pressure<RS_GTAB>, grid, blk(f), lds, p->stream, C_<double>(d),
                        C_<double>(p->bc), f.dev<Cd>(), nlines, p->g.ny, 1.0 / double(p->g.nz),
                        p->g.Lz, p->d_khom, p->d_z);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+
+int pressure_walls(spx_plan* p, void* d) {
+  const long nlines = p->wlines();
+  ProfScope ps_(p, "pressure_walls", double(nlines) * 6.0 * 16.0);
+  hipLaunchKernelGGL(k_pressure_walls, dim3(unsigned((nlines + 255) / 256)), dim3(256), 0,
+                     p->stream, C_<double>(d), C_<double>(p->bc), C_<double>(p->pw), nlines,
+                     p->g.nz, p->np, p->g.Lz, p->d_khom, p->d_z);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }

@@ -187,7 +187,8 @@ __global__ void __launch_bounds__(256) k_poisson_project_bc(
     cpx<TS>* __restrict__ a, cpx<TS>* __restrict__ b, cpx<TS>* __restrict__ c,
     Cp* __restrict__ d, Cp* __restrict__ bc, long nlines, int nz, int ny,
     const double* __restrict__ kx, const double* __restrict__ ky, const double* __restrict__ kz,
-    const Cp* __restrict__ wtop, double tmp, RkLoad rk, int rk_on) {
+    const Cp* __restrict__ wtop, double tmp, RkLoad rk, int rk_on, Cp* __restrict__ pw,
+    int np) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
   const long ln = long(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -196,6 +197,7 @@ __global__ void __launch_bounds__(256) k_poisson_project_bc(
   const int j = int(ln - i * ny);
   const double kxi = kx[i], kyj = ky[j];
   Cp s0 = mkc<double>(0, 0), s1 = mkc<double>(0, 0);
+  Cp q0 = mkc<double>(0, 0), q1 = mkc<double>(0, 0);  // IFFT_z(d) at the walls (pw)
   for (int k = lane; k < nz; k += 64) {
     const long e = ln * nz + k;
     const double kzk = kz[k];
@@ -211,7 +213,13 @@ __global__ void __launch_bounds__(256) k_poisson_project_bc(
     } else {
       dd = mkc<double>(sm.y / kk2, -sm.x / kk2);
     }
-    d[e] = dd;
+    if (pw) {
+      q0 = cadd(q0, dd);
+      q1 = cadd(q1, cmul(dd, wtop[k]));
+      if (k != 0 && k != np - 1) d[e] = dd;
+    } else {
+      d[e] = dd;
+    }
     st(a, e, csub(A, ik(kxi, dd)));
     st(b, e, csub(B, ik(kyj, dd)));
     st(c, e, csub(Cc, ik(kzk, dd)));
@@ -229,6 +237,19 @@ __global__ void __launch_bounds__(256) k_poisson_project_bc(
   if (lane == 0) {
     bc[2 * ln] = s0;
     bc[2 * ln + 1] = s1;
+  }
+  if (pw) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      q0.x += __shfl_xor(q0.x, off);
+      q0.y += __shfl_xor(q0.y, off);
+      q1.x += __shfl_xor(q1.x, off);
+      q1.y += __shfl_xor(q1.y, off);
+    }
+    if (lane == 0) {
+      pw[2 * ln] = cscale(q0, tmp);
+      pw[2 * ln + 1] = cscale(q1, tmp);
+    }
   }
 }
 
@@ -382,7 +403,8 @@ int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d) {
   return SPX_OK;
 }
 
-int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d, const RkLoad* rk) {
+int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d, const RkLoad* rk,
+                          bool walls_only) {
   if (!p->d_wtop) return fail(SPX_ERR_STATE, "plan has no wall table");
   const long nlines = p->wlines();
   RkLoad r;
@@ -395,7 +417,7 @@ int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d, const
                      p->stream, CS<double>(a), CS<double>(b), CS<double>(c), CS<double>(d),
                      static_cast<Cp*>(p->bc), nlines, p->g.nz, p->g.ny, p->d_kx, p->d_ky,
                      p->d_kz, static_cast<const Cp*>(p->d_wtop), 1.0 / double(p->g.nz), r,
-                     rk ? 1 : 0);
+                     rk ? 1 : 0, walls_only ? static_cast<Cp*>(p->pw) : nullptr, p->np);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
