@@ -286,7 +286,6 @@ int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
 // s_imposebc: z IFFT, 1/nz, th = 0 at both walls, continuation + z FFT
 int sconst_fused(spx_plan* p, void* th);
 // sol_project stages (bctarget = 1, Neumann potential)
-int proj_bc(spx_plan* p, const void* c);             // bc <- IFFT_z(c/nz) at the walls
 int proj_pressure(spx_plan* p, void* d);             // d <- IFFT_z(d)/nz + harmonic a
 // proj_pressure at z = 0 and z = np-1 only, from p->pw (the rest of d keeps
 // the potential): all the next stage's wall update reads
@@ -317,7 +316,7 @@ int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, 
                  const void* const* f, const void* const* src, const double* src_coef,
                  const double* const* nu, double dt, double rmp, bool nl_lo = false);
 int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d);
-// poisson_project plus the wall data bc of sol_project (replaces proj_bc)
+// poisson_project plus the wall data bc of sol_project (boundary_mod.fpp:262-352)
 // walls_only: d is left unwritten at z = 0 and z = np-1 and IFFT_z(d)/nz at
 // those two points goes to p->pw (for pressure_walls)
 int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d,

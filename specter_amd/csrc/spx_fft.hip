@@ -631,22 +631,6 @@ __global__ void __launch_bounds__(kMaxBlock) k_noslip(cpx<TS>* __restrict__ vx, 
     for (int k = lane; k < n; k += f.tpl) V[ln * n + k] = ccast<TS>(xl[k]);
 }
 
-// sol_project (boundary_mod.fpp:197-402), bctarget = 1, bczsta = bczend = 0
-// bc(1:2) <- IFFT_z(c * (1/nz)) at z = 0 and z = Lz   (:262-352)
-template <typename TS, int RS>
-__global__ void __launch_bounds__(kMaxBlock) k_proj_bc(const cpx<TS>* __restrict__ c, Cd* __restrict__ bc,
-                                                 LFd f, long nlines, int np, double tmp) {
-  SPX_PROLOGUE(Cd, f);
-  const int n = f.n;
-  const long ln = long(blockIdx.x) * f.nl + line;
-  const bool ok = ln < nlines;
-  for (int k = lane; k < n; k += f.tpl)
-    xl[k] = ok ? cconj(cscale(ccast<double>(c[ln * n + k]), tmp)) : mkc<double>(0, 0);
-  __syncthreads();
-  line_dft<double, RS>(xl, f, tws, lane);
-  if (ok && lane < 2) bc[ln * 2 + lane] = cconj(xl[lane ? np - 1 : 0]);
-}
-
 // laplace_z pure-Neumann coefficients (boundary_mod.fpp:531-560)
 struct Coef {
   Cd c1, c2;
@@ -1555,17 +1539,6 @@ int sconst_fused(spx_plan* p, void* th) {
   std::memset(&wp, 0, sizeof(wp));
   wp.scalar = 1;
   return noslip_launch(p, th, th, nullptr, wp, 1, "sconst", nullptr);
-}
-
-int proj_bc(spx_plan* p, const void* c) {
-  const LineFFTHost& f = p->fz;
-  const long nlines = p->wlines();
-  ProfScope ps_(p, "proj_bc", double(p->wnc()) * 16.0);
-  SPX_LAUNCH_RS(f, k_proj_bc, double, , dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
-                f.lds_bytes(16), p->stream, C_<double>(c), C_<double>(p->bc), f.dev<Cd>(), nlines,
-                p->np, 1.0 / double(p->g.nz));
-  SPX_CHECK_LAUNCH();
-  return SPX_OK;
 }
 
 int proj_pressure(spx_plan* p, void* d) {
