@@ -12,8 +12,14 @@ carries the float64 rate, the speed-up and the relative L2 field error of the
 mixed run against the float64 run after ``--err-steps`` full RK steps from
 the same initial field.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hd256|hd64|hd1024|bouss512]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload hd512|hd256|hd64|hd1024|bouss512]
                     [--precision both|mixed|fp64] [--err-steps N]
+
+Timing runs whole RK steps through spx_step (one captured HIP graph per
+step): the warmup is rounded up to whole steps (the stages actually run are
+reported as ``warmup_stages``), and the timed stages are whole steps when K
+is a multiple of the RK order.
 """
 from __future__ import annotations
 
@@ -29,15 +35,20 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+_FORCING = "forcing 0.5(sin2y,sin2x,0)sin(pi z)"
 WORKLOADS = {
+    # HD 512^3 (z physical) d=5 C=25 RK4 nu=2.5e-4, forced as the 256^3 workload
+    "hd512": dict(nx=512, ny=512, nzp=512, C=25, d=5, ord=4, nu=2.5e-4, dt=2.5e-4, forcing=True,
+                  label="HD 512x512x(512+25) FC-Gram d=5 C=25, RK4, nu=2.5e-4, dt=2.5e-4, "
+                        + _FORCING),
     # HD 256^3 (z physical) d=5 C=25 RK4 nu=5e-4, forced
     "hd256": dict(nx=256, ny=256, nzp=256, C=25, d=5, ord=4, nu=5e-4, dt=5e-4, forcing=True,
                   label="HD 256x256x(256+25) FC-Gram d=5 C=25, RK4, nu=5e-4, dt=5e-4, "
-                        "forcing 0.5(sin2y,sin2x,0)sin(pi z)"),
+                        + _FORCING),
     # HD 1024^3 (one GPU's share of a production run) d=5 C=25 RK4 nu=1e-4, forced
     "hd1024": dict(nx=1024, ny=1024, nzp=1024, C=25, d=5, ord=4, nu=1e-4, dt=1e-4, forcing=True,
                    label="HD 1024x1024x(1024+25) FC-Gram d=5 C=25, RK4, nu=1e-4, dt=1e-4, "
-                         "forcing 0.5(sin2y,sin2x,0)sin(pi z)"),
+                         + _FORCING),
     # Boussinesq 512^3 d=8 C=27 RK4 nu=kappa=2e-4, heated bottom wall: theta is the
     # deviation from the conduction profile 1 - z/Lz (0 on both walls), gravity along
     # z (buoyancy xmom*theta in the vz equation, heat current xtemp*vz)
@@ -50,16 +61,11 @@ WORKLOADS = {
     "hd64": dict(nx=64, ny=64, nzp=64, C=25, d=5, ord=2, nu=2e-3, dt=1e-3, forcing=False,
                  label="HD 64x64x(64+25) FC-Gram d=5 C=25, RK2, nu=2e-3, dt=1e-3"),
 }
+DEFAULT_WORKLOAD = "hd512"
 
 
-def _fields(w, seed=1000):
-    import numpy as np
-    from specter_amd import ic
-    nx, ny, nz, C = w["nx"], w["ny"], w["nzp"] + w["C"], w["C"]
-    u = ic.spectrum_peaked_velocity(nx, ny, nz, C, seed)
-    zero = np.zeros((nz, ny, nx))
-    f = ic.large_scale_forcing(nx, ny, nz, C) if w["forcing"] else (zero, zero, zero)
-    return u, f
+def _log(msg):
+    print(f"[bench] {time.strftime('%H:%M:%S')} {msg}", file=sys.stderr, flush=True)
 
 
 def _plan(w, precision):
@@ -67,15 +73,29 @@ def _plan(w, precision):
     return FCPlan(w["nx"], w["ny"], w["nzp"] + w["C"], w["C"], w["d"], precision=precision)
 
 
-def _solver(plan, w, u, f):
-    from specter_amd import Solver, SolverParams
-    sol = Solver(plan, SolverParams(solver="HD", ord=w["ord"], dt=w["dt"], nu=w["nu"]))
-    for name, arr in zip(("vx", "vy", "vz", "fx", "fy", "fz"), (*u, *f)):
-        sol.set_real_field(name, arr)
-    return sol
+def _params(w):
+    from specter_amd import SolverParams
+    if w.get("solver") == "BOUSS":
+        return SolverParams(solver="BOUSS", ord=w["ord"], dt=w["dt"], nu=w["nu"],
+                            kappa=w["kappa"], xmom=w["xmom"], xtemp=w["xtemp"])
+    return SolverParams(solver="HD", ord=w["ord"], dt=w["dt"], nu=w["nu"])
 
 
-def _run_substeps(sol, ordr, nsub, start=0):
+def _initial_fields(w):
+    """Real-space initial fields of the workload: {name: array or cuda tensor}.
+    HD: the seeded spectrum-peaked velocity (and the forcing) generated on the
+    GPU; Boussinesq: v = 0 and a seeded white-noise theta'."""
+    from specter_amd import ic
+    nx, ny, nz, C = w["nx"], w["ny"], w["nzp"] + w["C"], w["C"]
+    if w.get("solver") == "BOUSS":
+        return {"th": ic.thermal_perturbation(nx, ny, nz, C, 1001)}
+    out = dict(zip(("vx", "vy", "vz"), ic.spectrum_peaked_velocity_device(nx, ny, nz, C, 1000)))
+    if w["forcing"]:
+        out.update(zip(("fx", "fy", "fz"), ic.large_scale_forcing_device(nx, ny, nz, C)))
+    return out
+
+
+def _stages(sol, ordr, nsub, start=0):
     """nsub RK stages cycling o = ord..1, with the rkstep1 copy before o = ord.
     Whole steps go through spx_step (one captured HIP graph per step)."""
     if start % ordr == 0 and nsub % ordr == 0:
@@ -93,7 +113,7 @@ def _profile(plan, sol, ordr, nsub, start):
     lib = plan._lib
     from specter_amd.lib import check
     check(lib.spx_prof_enable(plan.handle, 1))
-    _run_substeps(sol, ordr, nsub, start)
+    _stages(sol, ordr, nsub, start)
     nk = ctypes.c_int()
     check(lib.spx_prof_resolve(plan.handle, ctypes.byref(nk)))
     rows = []
@@ -109,40 +129,23 @@ def _profile(plan, sol, ordr, nsub, start):
     return rows
 
 
-def _cpu_baseline(w, u, f, nsub):
-    """The float64 oracle (numpy/scipy restatement) on the host cores, on a
-    bounded sample of the same workload: nsub RK stages."""
-    import numpy as np
-    workers = int(os.environ.get("OMP_NUM_THREADS", str(os.cpu_count() or 1)))
-    os.environ["SPX_ORACLE_WORKERS"] = str(workers)
-    from oracle import specter_ref as O
-    nx, ny, nz, C, d = w["nx"], w["ny"], w["nzp"] + w["C"], w["C"], w["d"]
-    g = O.make_grid(nx, ny, nz, C, d, os.path.join(ROOT, "specter_amd", "tables"))
-    st = O.HDState(*(O.fftp3d_real_to_complex(g, a.copy()) for a in u),
-                   np.zeros(g.spec_shape, complex),
-                   *(O.fftp3d_real_to_complex(g, a.copy()) for a in f))
-    C1, C2, C3 = st.vx.copy(), st.vy.copy(), st.vz.copy()
-    t0 = time.perf_counter()
-    for s in range(nsub):
-        O.hd_substep(g, st, C1, C2, C3, w["ord"] - (s % w["ord"]), w["ord"], w["dt"], w["nu"])
-    dt = (time.perf_counter() - t0) / nsub
-    return dt, workers
-
-
 def _time(sol, w, steps, warmup):
-    """ms per RK stage over `steps` stages after `warmup` untimed ones (HIP
-    events on the solver's stream, which is torch's current stream)."""
+    """ms per RK stage over `steps` stages after the warmup rounded up to whole
+    RK steps (HIP events on the solver's stream, which is torch's current
+    stream).  Returns (ms, wall s, stage position, warmup stages run)."""
     import torch
-    pos = _run_substeps(sol, w["ord"], warmup)
+    ordr = w["ord"]
+    wst = -(-warmup // ordr) * ordr
+    pos = _stages(sol, ordr, wst)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record()
-    pos = _run_substeps(sol, w["ord"], steps, pos)
+    pos = _stages(sol, ordr, steps, pos)
     e1.record()
     torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / steps, time.perf_counter() - t0, pos
+    return e0.elapsed_time(e1) / steps, time.perf_counter() - t0, pos, wst
 
 
 def _kernels(plan, sol, w, steps, pos):
@@ -156,10 +159,10 @@ def _kernels(plan, sol, w, steps, pos):
 
 def _measured_traffic(workload, mode):
     """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of the 256^3 workload (read + write, calibrated on a
-    plain copy); {} for other workloads or when the summary is absent."""
-    path = os.path.join(ROOT, "profiles", "hd256_hbm_traffic.md")
-    if workload != "hd256" or not os.path.exists(path):
+    WRITE_SIZE summary of the workload (read + write MiB columns, calibrated
+    on a plain copy); {} when no summary is committed for it."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_hbm_traffic.md")
+    if not os.path.exists(path):
         return {}
     out = {}
     for line in open(path):
@@ -183,34 +186,65 @@ def _roofline(rows, traffic):
             "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": tot_b / n}
 
 
-def _field_error(w, u, f, nsteps, plans):
-    """Errors of the mixed run against the float64 run after nsteps full RK
-    steps from the same fields: relative L2 of each velocity component, of
-    the horizontal energy spectrum, of the dissipation nu*<|omega|^2> (the
-    reference's enstrophy) and of the mean wall shear."""
+def _diagnostics(w, plan, sol, bouss, out_names):
     import numpy as np
-    sols = {k: _solver(p, w, u, f) for k, p in plans.items()}
-    for s in sols.values():
-        s.step(nsteps)
-    out = {}
-    for name in ("vx", "vy", "vz"):
-        a = sols["mixed"].spectral(name)
-        b = sols["fp64"].spectral(name)
-        out[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    q = {}
-    for k, s in sols.items():
-        p = s.plan
-        vx, vy, vz = s.field("vx"), s.field("vy"), s.field("vz")
-        q[k] = (np.array(p.energy_spectrum(vx, vy, vz, w["nx"] // 2)),
-                w["nu"] * p.energy(vx, vy, vz, 0),
-                np.array(p.wall_shear(vx, vy, w["nu"])))
-    (sm, dm, tm), (sf, df, tf) = q["mixed"], q["fp64"]
-    out["energy_spectrum"] = float(np.linalg.norm(sm - sf) / np.linalg.norm(sf))
-    out["dissipation"] = float(abs(dm - df) / abs(df))
-    out["wall_shear"] = float(np.abs(tm - tf).max() / np.abs(tf).max())
-    for s in sols.values():
-        s.close()
-    return out
+    vx, vy, vz = sol.field("vx"), sol.field("vy"), sol.field("vz")
+    d = {"spectrum": np.array(plan.energy_spectrum(vx, vy, vz, w["nx"] // 2)),
+         "dissipation": w["nu"] * plan.energy(vx, vy, vz, 0),
+         "wall_shear": np.array(plan.wall_shear(vx, vy, w["nu"])),
+         "fields": {n: sol.spectral(n) for n in out_names}}
+    if bouss:
+        d["nusselt"] = np.array(plan.nusselt(sol.field("th")))
+    return d
+
+
+def _errors(a, b, out_names, bouss, steps):
+    import numpy as np
+
+    def rel(x, y, norm):
+        d = norm(y)
+        return float(f"{norm(x - y) / d:.3e}") if d > 0 else None
+
+    l2 = np.linalg.norm
+    amax = lambda v: float(np.abs(v).max())
+    err = {"steps": steps}
+    for n in out_names:
+        err[n] = rel(a["fields"][n], b["fields"][n], l2)
+    err["energy_spectrum"] = rel(a["spectrum"], b["spectrum"], l2)
+    err["dissipation"] = rel(a["dissipation"], b["dissipation"], abs)
+    err["wall_shear"] = rel(a["wall_shear"], b["wall_shear"], amax)
+    if bouss:
+        err["nusselt"] = rel(a["nusselt"], b["nusselt"], amax)
+    return err
+
+
+def _cpu_baseline(w, spec, nsub):
+    """The float64 oracle (numpy/scipy restatement) on the host cores, on a
+    bounded sample of the same workload: nsub RK stages from the spectral
+    initial state the GPU runs start from."""
+    import numpy as np
+    workers = int(os.environ.get("OMP_NUM_THREADS", str(os.cpu_count() or 1)))
+    os.environ["SPX_ORACLE_WORKERS"] = str(workers)
+    from oracle import specter_ref as O
+    nx, ny, nz, C, d = w["nx"], w["ny"], w["nzp"] + w["C"], w["C"], w["d"]
+    g = O.make_grid(nx, ny, nz, C, d, os.path.join(ROOT, "specter_amd", "tables"))
+    zero = lambda: np.zeros(g.spec_shape, complex)
+    get = lambda n: spec[n].copy() if n in spec else zero()
+    bouss = w.get("solver") == "BOUSS"
+    st = O.HDState(get("vx"), get("vy"), get("vz"), zero(), get("fx"), get("fy"), get("fz"),
+                   get("th") if bouss else None, zero() if bouss else None)
+    C1, C2, C3 = st.vx.copy(), st.vy.copy(), st.vz.copy()
+    C7 = st.th.copy() if bouss else None
+    t0 = time.perf_counter()
+    for s in range(nsub):
+        o = w["ord"] - (s % w["ord"])
+        if bouss:
+            O.bouss_substep(g, st, C1, C2, C3, C7, o, w["ord"], w["dt"], w["nu"], w["kappa"],
+                            w["xmom"], w["xtemp"])
+        else:
+            O.hd_substep(g, st, C1, C2, C3, o, w["ord"], w["dt"], w["nu"])
+    dt = (time.perf_counter() - t0) / nsub
+    return dt, workers
 
 
 def _ensemble(nruns, max_slots, chunk):
@@ -273,68 +307,54 @@ def _ensemble(nruns, max_slots, chunk):
     }
 
 
-def _log(msg):
-    print(f"[bench] {time.strftime('%H:%M:%S')} {msg}", file=sys.stderr, flush=True)
-
-
-def _bench_large(args, w):
-    """The 1024^3 HD and 512^3 Boussinesq workloads: the mixed run and the
-    float64 run go one after the other from the same initial state (kept on
-    the host; at 1024^3 the two do not fit together); each runs --err-steps
-    full steps for the error comparison, then is timed."""
-    import numpy as np
+def _bench(args, w):
+    """Both precisions on the same workload, one after the other from the same
+    initial state (its spectral fields are kept on the host, so the largest
+    grids never hold two plans at once): --err-steps full steps for the error
+    comparison, then the timed stages and the per-kernel pass."""
     import torch
-    from specter_amd import ic
+    from specter_amd import Solver
     from specter_amd.lib import SPX_FP32_MIXED, SPX_FP64, SpxError
-    nx, ny, nz, C = w["nx"], w["ny"], w["nzp"] + w["C"], w["C"]
+    nx, ny, nz = w["nx"], w["ny"], w["nzp"] + w["C"]
     npts = nx * ny * w["nzp"]
+    ntot = nx * ny * nz
     bouss = w.get("solver") == "BOUSS"
-    _log("initial fields")
-    if bouss:
-        names = ("th",)
-        fields = [ic.thermal_perturbation(nx, ny, nz, C, 1001)]
-    else:
-        names = ("vx", "vy", "vz", "fx", "fy", "fz")
-        fields = [*ic.spectrum_peaked_velocity_device(nx, ny, nz, C, 1000),
-                  *ic.large_scale_forcing_device(nx, ny, nz, C)]
-        torch.cuda.empty_cache()   # the generator's temporaries go back to the device
     out_names = ("vx", "vy", "vz", "th") if bouss else ("vx", "vy", "vz")
-    res, host_ic, diag = {}, None, {}
-    for mode in ("mixed", "fp64"):
+    modes = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
+    _log(f"{args.workload}: initial fields")
+    real_ic = _initial_fields(w)
+    res, host_ic, diag, zinfo = {}, None, {}, None
+    for mode in modes:
         prec = SPX_FP32_MIXED if mode == "mixed" else SPX_FP64
         try:
             plan = _plan(w, prec)
-            from specter_amd import Solver, SolverParams
-            prm = dict(solver="BOUSS", kappa=w["kappa"], xmom=w["xmom"],
-                       xtemp=w["xtemp"]) if bouss else dict(solver="HD")
-            sol = Solver(plan, SolverParams(ord=w["ord"], dt=w["dt"], nu=w["nu"], **prm))
+            sol = Solver(plan, _params(w))
         except SpxError as e:
             res[mode] = {"error": str(e)}
             _log(f"{mode}: {e}")
             continue
+        if zinfo is None:
+            zinfo = {0: "direct", 1: "bluestein", 2: "rader"}[plan.z_algorithm] + \
+                    f" (in-LDS FFT length {plan.z_fft_len})"
         if host_ic is None:
-            for i, name in enumerate(names):
-                sol.set_real_field(name, fields[i])
-                fields[i] = None
+            for name in list(real_ic):
+                sol.set_real_field(name, real_ic.pop(name))
                 torch.cuda.empty_cache()
-            host_ic = {n: sol.spectral(n) for n in names}
+            host_ic = {n: sol.spectral(n) for n in sol.FIELDS if n != "pr" and
+                       (bouss or n not in ("th", "fs"))}
         else:
-            for n in names:
-                sol.field(n).copy_(torch.from_numpy(host_ic[n]))
+            for n, a in host_ic.items():
+                sol.field(n).copy_(torch.from_numpy(a))
             plan.sync()
-        _log(f"{mode}: {args.err_steps} steps")
-        sol.step(args.err_steps)
-        plan.sync()
-        vx, vy, vz = sol.field("vx"), sol.field("vy"), sol.field("vz")
-        diag[mode] = {"spectrum": np.array(plan.energy_spectrum(vx, vy, vz, nx // 2)),
-                      "dissipation": w["nu"] * plan.energy(vx, vy, vz, 0),
-                      "wall_shear": np.array(plan.wall_shear(vx, vy, w["nu"])),
-                      "fields": {n: sol.spectral(n) for n in out_names}}
-        if bouss:
-            diag[mode]["nusselt"] = np.array(plan.nusselt(sol.field("th")))
+        if args.err_steps > 0 and len(modes) == 2:
+            _log(f"{mode}: {args.err_steps} steps")
+            sol.step(args.err_steps)
+            plan.sync()
+            diag[mode] = _diagnostics(w, plan, sol, bouss, out_names)
         _log(f"{mode}: timing")
-        ms, wall, pos = _time(sol, w, args.steps, args.warmup)
-        r = {"ms_per_step": ms, "value": npts / (ms * 1e-3) / 1e6}
+        ms, wall, pos, wst = _time(sol, w, args.steps, args.warmup)
+        r = {"ms_per_step": ms, "wall_s": wall, "value": npts / (ms * 1e-3) / 1e6,
+             "warmup_stages": wst}
         if not args.no_profile:
             r["rows"], r["kernels"] = _kernels(plan, sol, w, args.steps, pos)
         res[mode] = r
@@ -342,54 +362,62 @@ def _bench_large(args, w):
         plan.close()
         del sol, plan
         torch.cuda.empty_cache()
-    hr = res["mixed"]
+    head = modes[0]
+    hr = res[head]
     if "value" not in hr:
         print(json.dumps({"metric": "RK substep Mpts/s", "value": None, "unit": "Mpts/s",
-                          "workload": w["label"], "error": res}), flush=True)
+                          "config": {"workload": w["label"]}, "error": res}), flush=True)
         return
     line = {
-        "metric": "RK substep Mpts/s (physical points per RK stage per second), FP32-mixed",
-        "value": round(hr["value"], 3), "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(hr["ms_per_step"], 4),
+        "metric": "RK substep Mpts/s (physical points per RK stage per second), "
+                  + ("FP32-mixed" if head == "mixed" else "FP64"),
+        "value": round(hr["value"], 3), "unit": "Mpts/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "warmup_stages": hr["warmup_stages"],
+        "ms_per_step": round(hr["ms_per_step"], 4), "wall_s": round(hr["wall_s"], 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 nonlinear term / f64 state, continuation, pressure, walls",
-        "precision_mode": "mixed",
+        "dtype": "f32 nonlinear term / f64 state, continuation, pressure, walls"
+                 if head == "mixed" else "f64",
+        "precision_mode": head,
         "data": "synthetic (seeded white-noise theta', v = 0)" if bouss else
                 "synthetic (seeded spectrum-peaked solenoidal IC generated on the GPU "
                 "x sin(pi z/Lz))",
-        "config": {"workload": w["label"], "grid": [nx, ny, nz], "physical_points": npts,
+        "config": {"workload": w["label"], "grid": [nx, ny, nz], "z_dft": zinfo,
+                   "physical_points": npts, "total_points": ntot,
+                   "total_Mpts_per_s": round(ntot / (hr["ms_per_step"] * 1e-3) / 1e6, 3),
                    "parallelism": "single GPU"},
     }
     if not args.no_profile:
-        line["roofline"] = _roofline(hr["rows"], {})
+        line["roofline"] = _roofline(hr["rows"], _measured_traffic(args.workload, head))
         line["kernels"] = hr["kernels"]
     f64 = res.get("fp64", {})
-    if "ms_per_step" in f64:
-        line["fp64"] = {"value": round(f64["value"], 3), "ms_per_step": round(f64["ms_per_step"], 4)}
+    if head == "mixed" and "ms_per_step" in f64:
+        line["fp64"] = {"value": round(f64["value"], 3),
+                        "ms_per_step": round(f64["ms_per_step"], 4)}
         if not args.no_profile:
+            line["fp64"]["roofline"] = _roofline(f64["rows"],
+                                                 _measured_traffic(args.workload, "fp64"))
             line["fp64"]["kernels"] = f64["kernels"]
         line["speedup_vs_fp64"] = round(f64["ms_per_step"] / hr["ms_per_step"], 3)
-        a, b = diag["mixed"], diag["fp64"]
-
-        def rel(x, y, norm):
-            d = norm(y)
-            return float(f"{norm(x - y) / d:.3e}") if d > 0 else None
-
-        l2 = np.linalg.norm
-        amax = lambda v: float(np.abs(v).max())
-        err = {"steps": args.err_steps}
-        for n in out_names:
-            err[n] = rel(a["fields"][n], b["fields"][n], l2)
-        err["energy_spectrum"] = rel(a["spectrum"], b["spectrum"], l2)
-        err["dissipation"] = rel(a["dissipation"], b["dissipation"], abs)
-        err["wall_shear"] = rel(a["wall_shear"], b["wall_shear"], amax)
-        if bouss:
-            err["nusselt"] = rel(a["nusselt"], b["nusselt"], amax)
-            line["nusselt_fp64"] = [round(float(x), 10) for x in b["nusselt"]]
-        if args.err_steps > 0:
-            line["error_vs_fp64"] = err
-    else:
+        if diag:
+            line["error_vs_fp64"] = _errors(diag["mixed"], diag["fp64"], out_names, bouss,
+                                            args.err_steps)
+            if bouss:
+                line["nusselt_fp64"] = [round(float(x), 10) for x in diag["fp64"]["nusselt"]]
+    elif head == "mixed" and f64:
         line["fp64"] = f64
+    if args.ensemble_runs > 0:
+        _log("ensemble")
+        line["ensemble"] = _ensemble(args.ensemble_runs, args.ensemble_slots,
+                                     args.ensemble_chunk)
+    if args.cpu_substeps > 0 and host_ic is not None:
+        _log("cpu baseline")
+        t, cores = _cpu_baseline(w, host_ic, args.cpu_substeps)
+        line["cpu_baseline"] = {"value": round(npts / t / 1e6, 4), "unit": "Mpts/s",
+                                "cores": cores, "kind": "port",
+                                "sample": f"{args.cpu_substeps} RK stage(s) of the same "
+                                          f"workload from the same initial state, float64 "
+                                          f"oracle (scipy.fft workers={cores}, numpy "
+                                          f"elementwise single-threaded), {t:.2f} s per stage"}
     print(json.dumps(line), flush=True)
 
 
@@ -397,16 +425,18 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8, help="timed RK stages")
-    ap.add_argument("--warmup", type=int, default=4, help="untimed RK stages")
-    ap.add_argument("--workload", default="hd256", choices=sorted(WORKLOADS))
+    ap.add_argument("--warmup", type=int, default=4,
+                    help="untimed RK stages (rounded up to whole RK steps)")
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="both", choices=("both", "mixed", "fp64"))
     ap.add_argument("--err-steps", type=int, default=100,
                     help="full RK steps of the mixed-vs-float64 field comparison (0 = skip)")
     ap.add_argument("--cpu-substeps", type=int, default=1,
                     help="RK stages of the CPU baseline sample (0 = skip)")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel pass")
-    ap.add_argument("--ensemble-runs", type=int, default=96,
-                    help="requests of the ensemble-serving measurement (0 = skip)")
+    ap.add_argument("--ensemble-runs", type=int, default=None,
+                    help="requests of the ensemble-serving measurement (0 = skip; default "
+                         "96 for the default workload, 0 otherwise)")
     ap.add_argument("--ensemble-slots", type=int, default=16)
     ap.add_argument("--ensemble-chunk", type=int, default=5, help="RK steps of a class's first chunk")
     args = ap.parse_args()
@@ -414,79 +444,9 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the "
               "multi-GPU slab decomposition is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
-    from specter_amd.lib import SPX_FP32_MIXED, SPX_FP64
-    w = WORKLOADS[args.workload]
-    if args.workload in ("hd1024", "bouss512"):
-        _bench_large(args, w)
-        return
-    u, f = _fields(w)
-    npts = w["nx"] * w["ny"] * w["nzp"]
-    ntot = w["nx"] * w["ny"] * (w["nzp"] + w["C"])
-    modes = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
-    plans, res = {}, {}
-    for mode in modes:
-        plan = _plan(w, SPX_FP32_MIXED if mode == "mixed" else SPX_FP64)
-        plans[mode] = plan
-        sol = _solver(plan, w, u, f)
-        ms, wall, pos = _time(sol, w, args.steps, args.warmup)
-        r = {"ms_per_step": ms, "wall_s": wall, "value": npts / (ms * 1e-3) / 1e6}
-        if not args.no_profile:
-            r["rows"], r["kernels"] = _kernels(plan, sol, w, args.steps, pos)
-        sol.close()
-        res[mode] = r
-    head = modes[0]
-    hr = res[head]
-    line = {
-        "metric": "RK substep Mpts/s (physical points per RK stage per second), "
-                  + ("FP32-mixed" if head == "mixed" else "FP64"),
-        "value": round(hr["value"], 3),
-        "unit": "Mpts/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(hr["ms_per_step"], 4),
-        "wall_s": round(hr["wall_s"], 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 nonlinear term / f64 state, continuation, pressure, walls"
-                 if head == "mixed" else "f64",
-        "precision_mode": head,
-        "data": "synthetic (seeded spectrum-peaked solenoidal IC x sin(pi z/Lz))",
-        "config": {"workload": w["label"], "grid": [w["nx"], w["ny"], w["nzp"] + w["C"]],
-                   "z_dft": {0: "direct", 1: "bluestein", 2: "rader"}[
-                       plans[head].z_algorithm] + f" (in-LDS FFT length {plans[head].z_fft_len})",
-                   "physical_points": npts, "total_points": ntot,
-                   "total_Mpts_per_s": round(ntot / (hr["ms_per_step"] * 1e-3) / 1e6, 3),
-                   "parallelism": "single GPU"},
-    }
-    if "fp64" in res and head == "mixed":
-        line["fp64"] = {"value": round(res["fp64"]["value"], 3),
-                        "ms_per_step": round(res["fp64"]["ms_per_step"], 4)}
-        line["speedup_vs_fp64"] = round(res["fp64"]["ms_per_step"] / hr["ms_per_step"], 3)
-        if args.err_steps > 0:
-            err = _field_error(w, u, f, args.err_steps, plans)
-            line["error_vs_fp64"] = {"steps": args.err_steps,
-                                     **{k: float(f"{v:.3e}") for k, v in err.items()}}
-    if not args.no_profile:
-        line["roofline"] = _roofline(hr["rows"], _measured_traffic(args.workload, head))
-        line["kernels"] = hr["kernels"]
-        if "fp64" in res and head == "mixed":
-            line["fp64"]["roofline"] = _roofline(res["fp64"]["rows"],
-                                                 _measured_traffic(args.workload, "fp64"))
-            line["fp64"]["kernels"] = res["fp64"]["kernels"]
-    if args.ensemble_runs > 0:
-        line["ensemble"] = _ensemble(args.ensemble_runs, args.ensemble_slots,
-                                     args.ensemble_chunk)
-    if args.cpu_substeps > 0:
-        t, cores = _cpu_baseline(w, u, f, args.cpu_substeps)
-        line["cpu_baseline"] = {"value": round(npts / t / 1e6, 4), "unit": "Mpts/s",
-                                "cores": cores, "kind": "port",
-                                "sample": f"{args.cpu_substeps} RK stage(s) of the same "
-                                          f"workload, float64 oracle (scipy.fft workers="
-                                          f"{cores}, numpy elementwise single-threaded), "
-                                          f"{t:.2f} s per stage"}
-    print(json.dumps(line), flush=True)
+    if args.ensemble_runs is None:
+        args.ensemble_runs = 96 if args.workload == DEFAULT_WORKLOAD else 0
+    _bench(args, WORKLOADS[args.workload])
 
 
 if __name__ == "__main__":
