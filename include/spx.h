@@ -68,7 +68,15 @@ typedef struct {
   size_t bytes_real;
   int z_fft_len;           /* length of the z FFT run in LDS (Bluestein size for awkward nz) */
   int z_algorithm;         /* z DFT algorithm: 0 direct, 1 Bluestein, 2 Rader               */
-  size_t device_bytes;     /* device memory held by the plan (tables + scratch)              */
+  size_t device_bytes;     /* device memory the plan holds now (tables, basis, scratch)      */
+  size_t work_bytes;       /* more device memory the plan allocates on first use of the
+                              nonlinear term (gradre work arrays; 0 once allocated)          */
+  size_t work_bytes_bouss; /* and on first use of advect (Boussinesq), beyond work_bytes     */
+  size_t state_bytes;      /* device memory of one spx_state (all members): HD              */
+  size_t state_bytes_bouss;/* ... Boussinesq                                                 */
+  int z_threads_per_line;  /* launch shape of the float64 z line kernels                      */
+  int z_lines_per_block;
+  int z_tables_global;     /* 1: Rader / Bluestein tables read from global memory, 0: LDS    */
 } spx_plan_info;
 
 /* Replaces fcgram_create_plan (fftp/fcgram_mod.f90:49-144) including
@@ -152,6 +160,8 @@ int spx_energy(spx_plan* p, const void* a, const void* b, const void* c, int kin
                double* out);                                   /* pseudospec_hd.f90:405-635 */
 int spx_cross(spx_plan* p, const void* a, const void* b, const void* c, const void* d,
               const void* e, const void* f, double* out);      /* :778-940, kin = 1 */
+int spx_helicity(spx_plan* p, const void* a, const void* b, const void* c,
+                 double* out);                                 /* :638-775 */
 int spx_divergence(spx_plan* p, const void* a, const void* b, const void* c,
                    double* out);                               /* :1118-1235 */
 int spx_bouncheck_z(spx_plan* p, const void* a, const void* b, double* bot,

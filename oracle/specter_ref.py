@@ -27,6 +27,7 @@ Pinning of each function (tests/test_oracle_*.py):
                                laplace_z satisfies d2/dz2 - khom^2 = 0 and the
                                Neumann data in closed form
   * energy/cross/divergence    Parseval vs real-space sums (tests/energy.f90:20-32)
+  * helicity                   Beltrami fields: <v.curl v> = -k <|v|^2> exactly
   * hd_substep / bouss_substep composition of the above, order as in
                                include/hd/hd_rkstep2.f90 and include/bouss/bouss_rkstep2.f90
                                (parity unpinned beyond its parts: the reference run that
@@ -115,6 +116,12 @@ class Grid:
     Ly: float = 1.0
     Lz: float = 1.0  # physical wall-to-wall length
     dirm: np.ndarray = field(default=None, repr=False)
+    # arithmetic type of the run: float64 (the reference's) or longdouble (an
+    # extended-precision referee of the same discrete problem: every constant
+    # is formed in float64 as the reference forms it, then held exactly in
+    # longdouble, and the arithmetic -- transforms included -- runs in
+    # longdouble)
+    precision: str = "float64"
 
     def __post_init__(self):
         nx, ny, nz, Cz = self.nx, self.ny, self.nz, self.Cz
@@ -145,6 +152,12 @@ class Grid:
         kxy = self.kx[:, None] ** 2 + self.ky[None, :] ** 2
         self.kk2 = kxy[:, :, None] + (self.kz ** 2)[None, None, :]
         self.khom = np.sqrt(self.kk2[:, :, 0])
+        if self.precision not in ("float64", "longdouble"):
+            raise ValueError(f"unknown oracle precision {self.precision!r}")
+        self.rdtype = np.float64 if self.precision == "float64" else np.longdouble
+        self.cdtype = np.complex128 if self.precision == "float64" else np.clongdouble
+        for name in ("x", "y", "z", "kx", "ky", "kz", "kk2", "khom"):
+            setattr(self, name, getattr(self, name).astype(self.rdtype))
 
     @property
     def spec_shape(self):
@@ -155,11 +168,13 @@ class Grid:
         return (self.nz, self.ny, self.nx)
 
 
-def make_grid(nx, ny, nz, Cz, d, tdir, Lx=1.0, Ly=1.0, Lz=1.0) -> Grid:
-    """fcgram_create_plan (fcgram_mod.f90:49-144) + grid setup."""
-    g = Grid(nx, ny, nz, Cz, d, Lx, Ly, Lz)
+def make_grid(nx, ny, nz, Cz, d, tdir, Lx=1.0, Ly=1.0, Lz=1.0,
+              precision="float64") -> Grid:
+    """fcgram_create_plan (fcgram_mod.f90:49-144) + grid setup.  dir = A*Q^T
+    is formed in float64 in both precisions (fcgram_mod.f90:26)."""
+    g = Grid(nx, ny, nz, Cz, d, Lx, Ly, Lz, precision=precision)
     A, Q = load_tables(tdir, Cz, d)
-    g.dirm = dir_matrix(A, Q)
+    g.dirm = dir_matrix(A, Q).astype(g.rdtype)
     return g
 
 
@@ -266,10 +281,11 @@ FILTER_P = 50.0
 def filter_factors(g: Grid):
     """Per-axis exponential filter factors of fc_filter (pseudospec_hd.f90:1103-1105):
     exp(-alpha*(2*k_i/n_i/Dk_i)**(2p))."""
-    fx = np.exp(-FILTER_ALPHA * (2 * g.kx / g.nx / g.Dkx) ** (2 * FILTER_P))
-    fy = np.exp(-FILTER_ALPHA * (2 * g.ky / g.ny / g.Dky) ** (2 * FILTER_P))
-    fz = np.exp(-FILTER_ALPHA * (2 * g.kz / g.nz / g.Dkz) ** (2 * FILTER_P))
-    return fx, fy, fz
+    f64 = lambda k: np.asarray(k, np.float64)
+    fx = np.exp(-FILTER_ALPHA * (2 * f64(g.kx) / g.nx / g.Dkx) ** (2 * FILTER_P))
+    fy = np.exp(-FILTER_ALPHA * (2 * f64(g.ky) / g.ny / g.Dky) ** (2 * FILTER_P))
+    fz = np.exp(-FILTER_ALPHA * (2 * f64(g.kz) / g.nz / g.Dkz) ** (2 * FILTER_P))
+    return fx.astype(g.rdtype), fy.astype(g.rdtype), fz.astype(g.rdtype)
 
 
 def fc_filter(g: Grid, a: np.ndarray) -> None:
@@ -289,9 +305,9 @@ def gradre(g: Grid, a, b, c):
     4 c2r each, products on physical planes only, scale 1/(nx*ny*nz)^2, 3 r2c."""
     npz = g.np_
     tmp = 1.0 / (float(g.nx) * float(g.ny) * float(g.nz)) ** 2
-    rx = np.zeros(g.real_shape)
-    ry = np.zeros(g.real_shape)
-    rz = np.zeros(g.real_shape)
+    rx = np.zeros(g.real_shape, g.rdtype)
+    ry = np.zeros(g.real_shape, g.rdtype)
+    rz = np.zeros(g.real_shape, g.rdtype)
     for stage, vcomp in enumerate((a, b, c), start=1):
         r1 = fftp3d_complex_to_real(g, vcomp.copy())
         r2 = fftp3d_complex_to_real(g, derivk(g, a, stage))
@@ -318,7 +334,7 @@ def advect(g: Grid, a, b, c, d):
     """(A.grad)B for a scalar B (pseudospec_phd.f90:23-113)."""
     npz = g.np_
     tmp = 1.0 / (float(g.nx) * float(g.ny) * float(g.nz)) ** 2
-    r3 = np.zeros(g.real_shape)
+    r3 = np.zeros(g.real_shape, g.rdtype)
     for stage, vcomp in enumerate((a, b, c), start=1):
         r1 = fftp3d_complex_to_real(g, vcomp.copy())
         r2 = fftp3d_complex_to_real(g, derivk(g, d, stage))
@@ -380,8 +396,8 @@ def laplace_z(g: Grid, bc: np.ndarray, bczsta: int, bczend: int):
     pure-Neumann case (1,1) is on the hot path; Dirichlet (0,0) is also
     restated for the conductor case."""
     khom, Lz = g.khom, g.Lz
-    coef1 = np.empty(khom.shape, np.complex128)
-    coef2 = np.empty(khom.shape, np.complex128)
+    coef1 = np.empty(khom.shape, g.cdtype)
+    coef2 = np.empty(khom.shape, g.cdtype)
     b1, b2 = bc[:, :, 0], bc[:, :, 1]
     with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
         e1 = np.exp(-khom * Lz)
@@ -429,7 +445,7 @@ def sol_project(g: Grid, a, b, c, d_out, bctarget: int, bczsta: int = 0, bczend:
     if bczsta == 2 or bczend == 2:
         raise ValueError("Robin BCs are not on the hot path")
     fftp1d_complex_to_real_z(g, C1)
-    bc = np.empty((g.nkx, g.ny, 2), np.complex128)
+    bc = np.empty((g.nkx, g.ny, 2), g.cdtype)
     sgn = -1.0 if bctarget == 0 else 1.0
     bc[:, :, 0] = C1[:, :, 0] if bctarget == 1 else -C1[:, :, 0]
     bc[:, :, 1] = C1[:, :, g.np_ - 1] if bctarget == 1 else -C1[:, :, g.np_ - 1]
@@ -582,6 +598,18 @@ def cross(g: Grid, a, b, c, d, e, f) -> float:
     return _mean_phys(g, R1)
 
 
+def helicity(g: Grid, a, b, c) -> float:
+    """Kinetic helicity <v . curl v> over physical z (pseudospec_hd.f90:638-775):
+    Re(IFFT(v_i) conj(IFFT(w_i))) with w = (curlk(b,c,1), curlk(a,c,2),
+    curlk(a,b,3)), summed over the components in that order."""
+    R1 = None
+    for v, w in ((a, curlk(g, b, c, 1)), (b, curlk(g, a, c, 2)), (c, curlk(g, a, b, 3))):
+        zv, zw = _zinv(g, v), _zinv(g, w)
+        r = (zv * np.conj(zw)).real
+        R1 = r if R1 is None else R1 + r
+    return _mean_phys(g, R1)
+
+
 def divergence(g: Grid, a, b, c) -> float:
     """<|div v|^2> over the physical domain (pseudospec_hd.f90:1118-1235)."""
     C2 = derivk(g, a, 1)
@@ -610,6 +638,12 @@ def hdcheck(g: Grid, st: HDState):
     """energy, enstrophy and injection rate of balance.txt (pseudospec_hd.f90:943-1005)."""
     return (energy(g, st.vx, st.vy, st.vz, 1), energy(g, st.vx, st.vy, st.vz, 0),
             cross(g, st.vx, st.vy, st.vz, st.fx, st.fy, st.fz))
+
+
+def hdcheck_helicity(g: Grid, st: HDState) -> float:
+    """The helicity.txt value hdcheck writes with hel=1 (pseudospec_hd.f90:980-1003;
+    include/hd/hd_global.f90:2 passes hel=1)."""
+    return helicity(g, st.vx, st.vy, st.vz)
 
 
 def vdiagnostic(g: Grid, st: HDState):

@@ -57,7 +57,10 @@ static int alloc_dev(spx_plan* p, void** ptr, size_t bytes) {
     return fail(SPX_ERR_ALLOC,
                 "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
   }
-  if (p) p->owned.push_back(*ptr);
+  if (p) {
+    p->owned.push_back(*ptr);
+    p->dev_bytes += bytes;
+  }
   return SPX_OK;
 }
 
@@ -67,6 +70,31 @@ static int upload(spx_plan* p, double** dst, const V& host) {
   if (rc) return rc;
   SPX_HIP(hipMemcpy(*dst, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
   return SPX_OK;
+}
+
+// device bytes of the work arrays ensure_work allocates: the gradre set (with
+// the boundary-plane buffers of mixed plans) and the advect additions
+static void work_sizes(const spx_plan* p, size_t* hd, size_t* bouss) {
+  const size_t cb = p->nc * p->nb * p->csize, rb = p->nr * p->nb * p->rsize;
+  size_t h = 3 * (3 * cb + 2 * rb) + p->nr * p->nb * 8;
+  if (p->fp32 && p->g.Cz > 0) {
+    const size_t d2 = 2 * size_t(p->g.oz), nbm = size_t(p->nb);
+    const size_t bc = size_t(p->nkx) * p->g.ny * d2 * nbm * 16;
+    const size_t br = size_t(p->g.nx) * p->g.ny * d2 * nbm * 8;
+    h += 16 * bc + 7 * br;
+  }
+  *hd = h;
+  *bouss = 3 * cb + rb;
+}
+
+// device bytes of one solver state (all members): 10 float64 spectral fields,
+// the 3 nonlinear terms in the pipeline's precision, the diffusivity tables;
+// Boussinesq adds th, fs, the th start copy and its nonlinear term
+static size_t state_size(const spx_plan* p, bool bouss) {
+  const size_t c16 = p->nc * size_t(p->nb) * 16, cs = p->nc * size_t(p->nb) * p->csize;
+  size_t b = 10 * c16 + 3 * cs + 2 * size_t(p->nb) * p->nkx * sizeof(double);
+  if (bouss) b += 3 * c16 + cs;
+  return b;
 }
 
 // gradre / advect work arrays in the nonlinear pipeline's precision
@@ -229,6 +257,10 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
   p->nb = p->nact = nmembers;
   p->mem = 1;
   p->stream = static_cast<hipStream_t>(stream);
+  if (hipGetDevice(&p->device) != hipSuccess) {
+    delete p;
+    return fail(SPX_ERR_HIP, "spx_plan_create: hipGetDevice failed");
+  }
   const int nx = g->nx, ny = g->ny, nz = g->nz, C = g->Cz, d = g->oz;
   p->nkx = nx / 2 + 1;
   p->np = nz - C;
@@ -379,14 +411,19 @@ int spx_plan_get_info(const spx_plan* p, spx_plan_info* info) {
   info->bytes_real = p->nr * 8;
   info->z_fft_len = p->fz.M;
   info->z_algorithm = p->fz.mode;
-  size_t b = p->nc * 16 + size_t(p->g.ny) * p->nkx * 48;
-  if (p->XV[0]) b += 9 * p->nc * p->csize + 6 * p->nr * p->rsize + p->nr * 8;
-  if (p->work_bouss) b += 3 * p->nc * p->csize + p->nr * p->rsize;
-  info->device_bytes = b * size_t(p->nb);
-  // harmonic basis table (shared by the members) and its line index
-  if (p->d_htab)
-    info->device_bytes += 2 * size_t(p->nhtab) * p->g.nz * 16 +
-                          size_t(p->g.ny) * p->nkx * p->nb * sizeof(int);
+  size_t tab = 0;
+  for (const spx::LineFFTHost* f : {&p->fx, &p->fy, &p->fz, &p->fx32, &p->fy32, &p->fz32})
+    tab += f->bytes;
+  info->device_bytes = p->dev_bytes + tab;
+  size_t whd, wb;
+  work_sizes(p, &whd, &wb);
+  info->work_bytes = p->XV[0] ? 0 : whd;
+  info->work_bytes_bouss = p->work_bouss ? 0 : wb;
+  info->state_bytes = state_size(p, false);
+  info->state_bytes_bouss = state_size(p, true);
+  info->z_threads_per_line = p->fz.tpl;
+  info->z_lines_per_block = p->fz.nl;
+  info->z_tables_global = (p->fz.mode != 0 && p->fz.tab == p->fz.M) ? 1 : 0;
   return SPX_OK;
 }
 
@@ -503,7 +540,7 @@ namespace spx {
 // three derivatives and accumulates the products in the reference's order.
 // In mixed plans the whole pipeline runs in float32: the first z pass reads
 // the float64 state and the outputs are float32.
-static int gradre_impl(spx_plan* p, const void* const v[3], void* const out[3]) {
+int gradre_impl(spx_plan* p, const void* const v[3], void* const out[3]) {
   int rc;
   const int np = p->np;
   const bool lo = p->fp32;
@@ -712,6 +749,24 @@ int spx_cross(spx_plan* p, const void* a, const void* b, const void* c, const vo
     for (size_t l = 0; l < nl; ++l) acc[l] += p->h_part[l];
   }
   for (int b = 0; b < p->nact; ++b) out[b] = weighted_sum(p, acc, b) * diag_tmp(p);
+  return SPX_OK;
+}
+
+int spx_helicity(spx_plan* p, const void* a, const void* b, const void* c, double* out) {
+  SPX_REQ(p && a && b && c && out, "spx_helicity: null argument");
+  MemberView mv(p, p->nact);
+  const size_t nl = size_t(p->wlines());
+  std::vector<double> acc(nl, 0.0);
+  // curlk(b,c,1) = d_y c - d_z b ; curlk(a,c,2) = d_z a - d_x c ; curlk(a,b,3) = d_x b - d_y a
+  const LinComb w[3] = {two(c, 2, b, 3, -1), two(a, 3, c, 1, -1), two(b, 1, a, 2, -1)};
+  const void* v[3] = {a, b, c};
+  int rc;
+  for (int t = 0; t < 3; ++t) {
+    if ((rc = zline_dot_lc(p, w[t], v[t], p->part))) return rc;
+    if ((rc = fetch_parts(p, nl))) return rc;
+    for (size_t l = 0; l < nl; ++l) acc[l] += p->h_part[l];
+  }
+  for (int m = 0; m < p->nact; ++m) out[m] = weighted_sum(p, acc, m) * diag_tmp(p);
   return SPX_OK;
 }
 

@@ -54,6 +54,7 @@ struct LineFFTHost {
   void* chirp = nullptr;
   void* bfil = nullptr;
   void* perm = nullptr;
+  size_t bytes = 0;  // device bytes of the tables above
   // LDS bytes of a launch: tables + nbuf line buffers
   size_t lds_bytes(size_t csize, int nbuf = 1) const {
     return (size_t(tab) + size_t(nbuf) * size_t(nl) * size_t(ldl)) * csize;
@@ -117,6 +118,7 @@ struct spx_plan {
   // the planes the pass works on, members one after the other, so the y and
   // x passes see mem times as many planes.
   int nb = 1, nact = 1, mem = 1;
+  int device = 0;           // HIP device current at creation
   int wkx() const { return nkx * mem; }
   long wlines() const { return long(g.ny) * nkx * mem; }
   size_t wnc() const { return nc * size_t(mem); }
@@ -181,6 +183,7 @@ struct spx_plan {
   double* part = nullptr;   // per-line partial sums of the diagnostics
   std::vector<double> h_part;
   std::vector<void*> owned;
+  size_t dev_bytes = 0;     // device bytes of the allocations in owned
   spx::Profiler prof;
 };
 
@@ -299,8 +302,14 @@ struct LinComb {
   int dir[3];    // 0 none, 1 x, 2 y, 3 z (i k_dir), 4 kk2
   int sgn[3];    // +1 / -1 for terms 1,2 (term 0 always +1); 0 = term absent
 };
+// gradre (pseudospec_hd.f90:209-319) through the plan's current work arrays
+// XV, XZ, XY, RV, RO (and the boundary-plane buffers of mixed plans)
+int gradre_impl(spx_plan* p, const void* const v[3], void* const out[3]);
+
 int zline_sumsq(spx_plan* p, const LinComb& lc, double* line_out);
 int zline_dot(spx_plan* p, const void* a, const void* b, double* line_out);
+// sum_k<np Re(IFFT(lincomb) conj IFFT(b)) per line
+int zline_dot_lc(spx_plan* p, const LinComb& lc, const void* b, double* line_out);
 int zline_walls(spx_plan* p, const void* a, double* line_out /*2 per line*/);
 // IFFT_z(lincomb) at z = 0 and z = Lz: (re, im) bottom, (re, im) top per line
 int zline_wall_values(spx_plan* p, const LinComb& lc, double* line_out /*4 per line*/);

@@ -1055,19 +1055,20 @@ static int tpl_for(int M, int n, const std::vector<int>& fac) {
   return t;
 }
 
-static int dev_upload(void** dst, const void* src, size_t bytes) {
+static int dev_upload(void** dst, const void* src, size_t bytes, size_t* acc) {
   if (hipMalloc(dst, bytes) != hipSuccess) return fail(SPX_ERR_ALLOC, "hipMalloc of FFT tables");
+  *acc += bytes;
   if (hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
     return fail(SPX_ERR_HIP, "hipMemcpy of FFT tables");
   return SPX_OK;
 }
 
 // upload a complex table in the plan's precision (csize 16: float64, 8: float32)
-static int upload_table(void** dst, const std::vector<double>& v, int csize) {
-  if (csize == 16) return dev_upload(dst, v.data(), v.size() * sizeof(double));
+static int upload_table(void** dst, const std::vector<double>& v, int csize, size_t* acc) {
+  if (csize == 16) return dev_upload(dst, v.data(), v.size() * sizeof(double), acc);
   std::vector<float> f(v.size());
   for (size_t i = 0; i < v.size(); ++i) f[i] = float(v[i]);
-  return dev_upload(dst, f.data(), f.size() * sizeof(float));
+  return dev_upload(dst, f.data(), f.size() * sizeof(float), acc);
 }
 
 // naive long-double DFT of length M (used once per plan for the filters)
@@ -1113,7 +1114,9 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
     f.magic[i] = Ns > 1 ? unsigned(((1ULL << 32) + Ns - 1) / Ns) : 0u;
     Ns *= fac[i];
   }
-  f.tpl = tpl_for(f.M, n, fac);
+  // at least 2 threads per line: the wall kernels handle z = 0 and z = Lz on
+  // lanes 0 and 1
+  f.tpl = std::max(2, tpl_for(f.M, n, fac));
   if (f.tpl > kMaxBlock)
     return fail(SPX_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " exceeds the LDS engine");
   f.lgtpl = 0;
@@ -1159,7 +1162,7 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
     tw[2 * m] = double(wr[m]);
     tw[2 * m + 1] = double(wi[m]);
   }
-  int rc = upload_table(&f.tw, tw, csize);
+  int rc = upload_table(&f.tw, tw, csize, &f.bytes);
   if (rc) return rc;
   if (f.mode == 1) {
     // chirp w[m] = exp(-pi i (m^2 mod 2n)/n)
@@ -1185,8 +1188,8 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
     }
     std::vector<double> bf;
     dft_ld(br, bi, wr, wi, bf, 1.0L / M);
-    if ((rc = upload_table(&f.chirp, ch, csize))) return rc;
-    if ((rc = upload_table(&f.bfil, bf, csize))) return rc;
+    if ((rc = upload_table(&f.chirp, ch, csize, &f.bytes))) return rc;
+    if ((rc = upload_table(&f.bfil, bf, csize, &f.bytes))) return rc;
   } else if (f.mode == 2) {
     // Rader: a_q = x[g^q], b_m = W_n^(g^-m), X[g^-p] = x0 + (a (*) b)_p
     const int g = primitive_root(n);
@@ -1205,8 +1208,8 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
     }
     std::vector<double> bf;
     dft_ld(br, bi, wr, wi, bf, 1.0L / M);
-    if ((rc = upload_table(&f.bfil, bf, csize))) return rc;
-    if ((rc = dev_upload(&f.perm, perm.data(), perm.size() * sizeof(int)))) return rc;
+    if ((rc = upload_table(&f.bfil, bf, csize, &f.bytes))) return rc;
+    if ((rc = dev_upload(&f.perm, perm.data(), perm.size() * sizeof(int), &f.bytes))) return rc;
   }
   return SPX_OK;
 }
@@ -1217,6 +1220,7 @@ void free_line_fft(LineFFTHost& f) {
   if (f.bfil) (void)hipFree(f.bfil);
   if (f.perm) (void)hipFree(f.perm);
   f.tw = f.chirp = f.bfil = f.perm = nullptr;
+  f.bytes = 0;
 }
 #endif  // SPX_PART 1 (plan tables)
 
@@ -1608,8 +1612,10 @@ int build_harmonic_table(spx_plan* p) {
     return SPX_OK;
   }
   p->owned.push_back(tab);
+  p->dev_bytes += tab_bytes;
   SPX_HIP(hipMalloc(&hid, idx.size() * sizeof(int)));
   p->owned.push_back(hid);
+  p->dev_bytes += idx.size() * sizeof(int);
   SPX_HIP(hipMalloc(&kv, khv.size() * sizeof(double)));
   SPX_HIP(hipMemcpy(hid, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
   SPX_HIP(hipMemcpy(kv, khv.data(), khv.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -1685,6 +1691,10 @@ int zline_dot(spx_plan* p, const void* a, const void* b, double* line_out) {
   lc.dir[0] = lc.dir[1] = lc.dir[2] = 0;
   lc.sgn[0] = 1;
   lc.sgn[1] = lc.sgn[2] = 0;
+  return zline_launch(p, lc, b, 1, line_out);
+}
+
+int zline_dot_lc(spx_plan* p, const LinComb& lc, const void* b, double* line_out) {
   return zline_launch(p, lc, b, 1, line_out);
 }
 

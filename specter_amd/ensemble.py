@@ -82,11 +82,14 @@ class RunResult:
         return self.finished - self.arrival
 
 
-def state_bytes(plan: FCPlan, members: int) -> int:
-    """Device bytes of a batched HD state plus the plan's work arrays."""
+def state_bytes(plan: FCPlan, members: int | None = None) -> int:
+    """Device bytes of a batched plan with its HD solver state: what the plan
+    holds, the work arrays it allocates on first use and the state's fields
+    (libspx counts each allocation; ``members`` must be the plan's own)."""
     info = plan.info()
-    # 13 float64 spectral arrays per member (v, pr, forcing, RK copies, N)
-    return int(info.device_bytes) + 13 * members * plan.nkx * plan.ny * plan.nz * 16
+    if members is not None and members != plan.members:
+        raise ValueError("state_bytes: members must match the plan")
+    return int(info.device_bytes + info.work_bytes + info.state_bytes)
 
 
 def member_bytes(req: RunRequest) -> int:
@@ -115,7 +118,8 @@ class _Class:
         self.runs: list[RunRequest] = []          # active members, slot order
         self.done: list[int] = []                  # steps done per active member
         self.admitted: list[float] = []
-        self.bytes = state_bytes(self.plan, slots)
+        # the member plan for per-run I/O is part of the class's footprint
+        self.bytes = state_bytes(self.plan) + int(self.sol._member_plan().info().device_bytes)
         self.stepping = 0          # steps of the chunk in flight (0: idle)
         self.stepping_members = 0  # members the chunk in flight advances
         self.event = torch.cuda.Event()

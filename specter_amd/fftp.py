@@ -261,6 +261,13 @@ class FCPlan:
                                                 zip((a, b, c, d, e, f), "abcdef")), out))
         return self._out(list(out))
 
+    def helicity(self, a, b, c):
+        """<v . curl v> over physical z (pseudospec_hd.f90:638-775)."""
+        out = self._vals()
+        _l.check(self._lib.spx_helicity(self._p, self._spec(a, "a"), self._spec(b, "b"),
+                                        self._spec(c, "c"), out))
+        return self._out(list(out))
+
     def divergence(self, a, b, c):
         out = self._vals()
         _l.check(self._lib.spx_divergence(self._p, self._spec(a, "a"), self._spec(b, "b"),

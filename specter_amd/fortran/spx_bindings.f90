@@ -107,6 +107,11 @@ MODULE spx
          TYPE(C_PTR), VALUE :: plan, a, b, c, d, e, f
          REAL(C_DOUBLE), INTENT(OUT) :: out
       END FUNCTION
+      INTEGER(C_INT) FUNCTION spx_helicity(plan, a, b, c, out) BIND(C, name='spx_helicity')
+         IMPORT :: C_INT, C_PTR, C_DOUBLE
+         TYPE(C_PTR), VALUE :: plan, a, b, c
+         REAL(C_DOUBLE), INTENT(OUT) :: out
+      END FUNCTION
       INTEGER(C_INT) FUNCTION spx_divergence(plan, a, b, c, out) BIND(C, name='spx_divergence')
          IMPORT :: C_INT, C_PTR, C_DOUBLE
          TYPE(C_PTR), VALUE :: plan, a, b, c
@@ -155,6 +160,27 @@ MODULE spx
          IMPORT :: C_INT, C_PTR
          TYPE(C_PTR), VALUE :: plan, state
          INTEGER(C_INT), VALUE :: nsteps
+      END FUNCTION
+      ! per-kernel timing with HIP events (benchmark.txt)
+      INTEGER(C_INT) FUNCTION spx_prof_enable(plan, on) BIND(C, name='spx_prof_enable')
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: plan
+         INTEGER(C_INT), VALUE :: on
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION spx_prof_resolve(plan, nk) BIND(C, name='spx_prof_resolve')
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: plan
+         INTEGER(C_INT), INTENT(OUT) :: nk
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION spx_prof_get(plan, idx, name, length, launches, total_ms, &
+                                           total_bytes) BIND(C, name='spx_prof_get')
+         IMPORT :: C_INT, C_PTR, C_CHAR, C_SIZE_T, C_LONG, C_DOUBLE
+         TYPE(C_PTR), VALUE :: plan
+         INTEGER(C_INT), VALUE :: idx
+         CHARACTER(KIND=C_CHAR), INTENT(OUT) :: name(*)
+         INTEGER(C_SIZE_T), VALUE :: length
+         INTEGER(C_LONG), INTENT(OUT) :: launches
+         REAL(C_DOUBLE), INTENT(OUT) :: total_ms, total_bytes
       END FUNCTION
       ! memory helpers
       INTEGER(C_INT) FUNCTION spx_dev_alloc(bytes, ptr) BIND(C, name='spx_dev_alloc')

@@ -17,7 +17,19 @@
 ! specter.fpp:1003-1053: vx, vy, vz (and wx, wy, wz when outs >= 1) as
 ! v/(nx ny nz) and pr as p'/(nx ny dt) on the physical planes, to
 ! odir/<name>.MMMM.out.  x.txt, y.txt, z.txt hold the grid
-! (specter.fpp:752-765).
+! (specter.fpp:752-765).  mult divides dt and multiplies step, tstep and
+! cstep (specter.fpp:367-370); stat = 1 writes the initial vx, vy, vz as
+! the 0000 outputs (specter.fpp:920-932); helicity.txt is appended with the
+! balance.txt line (hdcheck with hel = 1, pseudospec_hd.f90:980-1003).
+! bench = 1 (2) skips all outputs and global quantities, times the RK loop
+! and appends one line to benchmark.txt in the reference's column layout
+! (specter.fpp:1182-1228; with bench = 2 also the plan-creation time):
+! TCPU, TOMP and TWTIME per step from the CPU clock and the wall clock
+! around the loop (which runs one captured HIP graph per step), TFFT and
+! TTOT per step from HIP events around every kernel of one extra profiled
+! step (TFFT: the line-transform kernels, which fuse the continuation, the
+! transposes, the products and the wall update; TTOT: all kernels); TTRA,
+! TCOM, TCONT, TNEU and TROB are 0 (fused into those kernels, or no MPI).
 !
 ! solver = 'BOUSS' in &grid (the reference's compile-time SOLVER choice)
 ! runs the Boussinesq solver: &boussinesq (gama, xmom, xtemp; the
@@ -63,9 +75,16 @@ PROGRAM spx_hd
    REAL(8) :: sv, sg, sp, nus(2)
    REAL(8), ALLOCATABLE, TARGET :: R(:, :, :), A(:, :), Q(:, :)
    REAL(8) :: eng, ens, pot, dv, tb, tt, nb, nt
-   INTEGER :: t, timec, timet, tind, ini, np, ios
+   INTEGER :: t, timec, timet, tind, ini, np, ios, nk, ik
+   REAL(8) :: khe, cpu0, cpu1, cpup0, cpup1, tfft, ttot, tms, tby
+   INTEGER(8) :: clk0, clk1, clkp0, clkp1, clkrate
+   INTEGER(C_LONG) :: nlaunch
+   CHARACTER(KIND=C_CHAR) :: kname(64)
+   CHARACTER(len=64) :: kstr
+   LOGICAL :: bbenchexist
    REAL(8), ALLOCATABLE, TARGET :: R2(:, :, :)
    INTEGER(C_SIZE_T) :: rbytes, cbytes
+   INTEGER :: i_
    CHARACTER(len=4) :: ext
    LOGICAL :: have
 
@@ -77,6 +96,10 @@ PROGRAM spx_hd
    READ(1, NML=boxparams)
    REWIND(1)
    READ(1, NML=parameter)
+   dt = dt / REAL(mult, 8)
+   step = step * mult
+   tstep = tstep * mult
+   cstep = cstep * mult
    REWIND(1)
    READ(1, NML=velocity)
    REWIND(1)
@@ -110,7 +133,12 @@ PROGRAM spx_hd
    CALL read_table(TRIM(tdir)//'/Q'//itoa(oz)//'.dat', Q, oz*oz)
 
    g = spx_grid(nx, ny, nz, Cz, oz, Lx, Ly, Lz, prec)
+   CALL CPU_TIME(cpup0)
+   CALL SYSTEM_CLOCK(clkp0, clkrate)
    CALL spx_check(spx_plan_create(g, A, Q, C_NULL_PTR, plan), 'spx_plan_create')
+   CALL spx_check(spx_plan_sync(plan), 'sync')
+   CALL CPU_TIME(cpup1)
+   CALL SYSTEM_CLOCK(clkp1)
    IF (bouss) THEN
       prm = spx_params(SPX_SOLVER_BOUSS, ord, dt, nu, kappa, xmom, xtemp, &
                        (/vxzsta, vyzsta/), (/vxzend, vyzend/))
@@ -160,6 +188,15 @@ PROGRAM spx_hd
 
    CALL write_grid()
 
+   ! initial fields as the 0000 outputs (specter.fpp:920-932)
+   IF (stat .eq. 1) THEN
+      ext = '0000'
+      CALL store_field('vx', vx)
+      CALL store_field('vy', vy)
+      CALL store_field('vz', vz)
+      WRITE(ext, '(i4.4)') stat
+   ENDIF
+
    ! RK loop with restart counters (specter.fpp:885-891, 1002-1165)
    ini = (stat - 1) * tstep + 1
    tind = stat
@@ -167,10 +204,16 @@ PROGRAM spx_hd
    timec = MODULO(ini - 1, cstep)
    OPEN(2, file='balance.txt', position='append')
    OPEN(3, file='noslip_diagnostic.txt', position='append')
+   OPEN(9, file='helicity.txt', position='append')
    IF (bouss) THEN
       OPEN(4, file='scalar.txt', position='append')
       OPEN(7, file='scalar_constant_diagnostic.txt', position='append')
       OPEN(8, file='nusselt.txt', position='append')
+   ENDIF
+   IF (bench .ge. 1) THEN
+      CALL spx_check(spx_plan_sync(plan), 'sync')
+      CALL CPU_TIME(cpu0)
+      CALL SYSTEM_CLOCK(clk0)
    ENDIF
    DO t = ini, step
       IF (timet .eq. tstep .AND. bench .eq. 0) THEN
@@ -190,12 +233,14 @@ PROGRAM spx_hd
          CALL store_pressure()
          IF (bouss) CALL store_field('th', th)
       ENDIF
-      IF (timec .eq. cstep) THEN
+      IF (timec .eq. cstep .AND. bench .eq. 0) THEN
          timec = 0
          CALL spx_check(spx_energy(plan, vx, vy, vz, 1, eng), 'energy')
          CALL spx_check(spx_energy(plan, vx, vy, vz, 0, ens), 'enstrophy')
+         CALL spx_check(spx_helicity(plan, vx, vy, vz, khe), 'helicity')
          CALL spx_check(spx_cross(plan, vx, vy, vz, fx, fy, fz, pot), 'cross')
          WRITE(2, '(1P E13.6,2E23.16,E24.16)') (t-1)*dt, eng, ens, pot
+         WRITE(9, '(1P E13.6,E24.16)') (t-1)*dt, khe
          CALL spx_check(spx_divergence(plan, vx, vy, vz, dv), 'divergence')
          CALL spx_check(spx_bouncheck_z(plan, vx, vy, tb, tt), 'bouncheck')
          CALL spx_check(spx_bouncheck_z(plan, vz, C_NULL_PTR, nb, nt), 'bouncheck')
@@ -216,8 +261,15 @@ PROGRAM spx_hd
       timet = timet + 1
       timec = timec + 1
    ENDDO
+   IF (bench .ge. 1) THEN
+      CALL spx_check(spx_plan_sync(plan), 'sync')
+      CALL CPU_TIME(cpu1)
+      CALL SYSTEM_CLOCK(clk1)
+      CALL write_benchmark()
+   ENDIF
    CLOSE(2)
    CLOSE(3)
+   CLOSE(9)
    IF (bouss) THEN
       CLOSE(4)
       CLOSE(7)
@@ -231,6 +283,46 @@ PROGRAM spx_hd
    CALL spx_check(spx_plan_destroy(plan), 'plan destroy')
 
 CONTAINS
+
+   ! benchmark.txt (specter.fpp:1182-1228, column layout of the FFTW build)
+   SUBROUTINE write_benchmark()
+      REAL(8) :: ns, wall
+      ns = REAL(MAX(step - ini + 1, 1), 8)
+      wall = REAL(clk1 - clk0, 8) / REAL(clkrate, 8)
+      ! one extra step with HIP events around every kernel
+      CALL spx_check(spx_prof_enable(plan, 1), 'prof on')
+      CALL spx_check(spx_step(plan, state, 1), 'spx_step')
+      CALL spx_check(spx_prof_resolve(plan, nk), 'prof resolve')
+      tfft = 0.d0
+      ttot = 0.d0
+      DO ik = 0, nk - 1
+         CALL spx_check(spx_prof_get(plan, ik, kname, INT(64, C_SIZE_T), nlaunch, tms, tby), &
+                        'prof get')
+         kstr = ' '
+         DO i_ = 1, 64
+            IF (kname(i_) .eq. C_NULL_CHAR) EXIT
+            kstr(i_:i_) = kname(i_)
+         ENDDO
+         ttot = ttot + tms * 1.d-3
+         IF (kstr(1:1) .eq. 'z' .OR. kstr(1:1) .eq. 'y' .OR. kstr(1:1) .eq. 'x' .OR. &
+             kstr(1:6) .eq. 'noslip') tfft = tfft + tms * 1.d-3
+      ENDDO
+      CALL spx_check(spx_prof_enable(plan, 0), 'prof off')
+      INQUIRE(file='benchmark.txt', exist=bbenchexist)
+      OPEN(14, file='benchmark.txt', position='append')
+      IF (.NOT. bbenchexist) THEN
+         WRITE(14, *) '# nx ny nz nsteps nprocs nth TCPU TOMP TWTIME TFFT TTRA TCOM', &
+                      'TCONT TNEU TROB TTOT'
+      ENDIF
+      WRITE(14, *) nx, ny, nz, (step - ini + 1), 1, 1, (cpu1 - cpu0) / ns, wall / ns, &
+                   wall / ns, tfft, 0.d0, 0.d0, 0.d0, 0.d0, 0.d0, ttot
+      IF (bench .eq. 2) THEN
+         WRITE(14, *) 'FFTW: Create_plan = ', (cpup1 - cpup0) / ns, &
+                      REAL(clkp1 - clkp0, 8) / REAL(clkrate, 8) / ns, &
+                      REAL(clkp1 - clkp0, 8) / REAL(clkrate, 8) / ns
+      ENDIF
+      CLOSE(14)
+   END SUBROUTINE write_benchmark
 
    FUNCTION itoa(i) RESULT(s)
       INTEGER, INTENT(IN) :: i

@@ -30,7 +30,10 @@ class SpxGrid(ctypes.Structure):
 class SpxPlanInfo(ctypes.Structure):
     _fields_ = [("nkx", c_int), ("np", c_int), ("n_spectral", c_size_t), ("n_real", c_size_t),
                 ("bytes_spectral", c_size_t), ("bytes_real", c_size_t), ("z_fft_len", c_int),
-                ("z_algorithm", c_int), ("device_bytes", c_size_t)]
+                ("z_algorithm", c_int), ("device_bytes", c_size_t), ("work_bytes", c_size_t),
+                ("work_bytes_bouss", c_size_t), ("state_bytes", c_size_t),
+                ("state_bytes_bouss", c_size_t), ("z_threads_per_line", c_int),
+                ("z_lines_per_block", c_int), ("z_tables_global", c_int)]
 
 
 class SpxParams(ctypes.Structure):
@@ -63,6 +66,7 @@ _SIGS = {
     "spx_s_imposebc": [_P, _P],
     "spx_energy": [_P, _P, _P, _P, c_int, POINTER(c_double)],
     "spx_cross": [_P, _P, _P, _P, _P, _P, _P, POINTER(c_double)],
+    "spx_helicity": [_P, _P, _P, _P, POINTER(c_double)],
     "spx_divergence": [_P, _P, _P, _P, POINTER(c_double)],
     "spx_bouncheck_z": [_P, _P, _P, POINTER(c_double), POINTER(c_double)],
     "spx_state_create": [_P, POINTER(SpxParams), POINTER(_P)],

@@ -138,6 +138,10 @@ class Solver:
         return (p.energy(vx, vy, vz, 1), p.energy(vx, vy, vz, 0),
                 p.cross(vx, vy, vz, fx, fy, fz))
 
+    def helicity(self):
+        """The helicity.txt value of hdcheck with hel=1 (pseudospec_hd.f90:980-1003)."""
+        return self.plan.helicity(self.field("vx"), self.field("vy"), self.field("vz"))
+
     def vdiagnostic(self):
         p = self.plan
         vx, vy, vz = self.field("vx"), self.field("vy"), self.field("vz")

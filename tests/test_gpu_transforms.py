@@ -210,6 +210,7 @@ def test_diagnostics(shape):
         assert rel_err(p.energy(ta, tb, tc, kin), O.energy(g, a, b, c, kin)) < 1e-12
     assert rel_err(p.cross(ta, tb, tc, td, te, tf), O.cross(g, a, b, c, d, e, f)) < 1e-12
     assert rel_err(p.divergence(ta, tb, tc), O.divergence(g, a, b, c)) < 1e-12
+    assert rel_err(p.helicity(ta, tb, tc), O.helicity(g, a, b, c)) < 1e-12
     bt = p.bouncheck_z(ta, tb)
     rb = O.bouncheck_z(g, a, b)
     assert rel_err(np.array(bt), np.array(rb)) < 1e-12

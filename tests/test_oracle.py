@@ -329,6 +329,56 @@ def test_enstrophy_reference_quirk_and_cross():
     assert abs(O.cross(g, *S, *S) - O.energy(g, *S, 1)) < 1e-14
 
 
+def _kz0_spectral(g, R2):
+    """Spectral array of the z-independent field R2(y, x): only the kz = 0
+    coefficient of each line, nz times its 2D DFT (no continuation involved,
+    so z derivatives vanish exactly)."""
+    X = np.fft.rfft2(R2)                              # (ny, nkx)
+    S = np.zeros(g.spec_shape, complex)
+    S[:, :, 0] = g.nz * X.T
+    return S
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_helicity_beltrami(k):
+    """v = (sin ky, cos kx, cos ky + sin kx) has curl v = -k v, so
+    <v . curl v> = -k <|v|^2>; a swapped component pair or a lost sign breaks it."""
+    g = grid(32, 24, 41, 25, 5)
+    X, Y = g.x[None, :], g.y[:, None]
+    one = np.ones((g.ny, g.nx))
+    u, v, w = np.sin(k * Y) * one, np.cos(k * X) * one, (np.cos(k * Y) + np.sin(k * X)) * one
+    S = [_kz0_spectral(g, a) for a in (u, v, w)]
+    e = O.energy(g, *S, 1)
+    assert abs(e - 2.0) < 1e-13
+    assert abs(O.helicity(g, *S) - (-k * e)) < 1e-12 * k
+    # planted mistakes: the enstrophy quirk's pairing (w_y twice, no w_z) and a
+    # flipped curl sign both give other values
+    S1 = [_kz0_spectral(g, a) for a in (u, v, 0 * w)]
+    bad = (O.cross(g, *S, O.curlk(g, S[1], S[2], 1), O.curlk(g, S[0], S[2], 2),
+                   O.curlk(g, S[0], S[2], 2)))
+    assert abs(bad - (-k * e)) > 0.1
+    assert abs(O.helicity(g, *S1) - (-k * O.energy(g, *S1, 1))) > 0.1 or \
+        abs(O.helicity(g, S[0], S[1], -S[2]) - (-k * e)) > 0.1
+
+
+def test_helicity_parseval_kat():
+    """Mixed-domain helicity equals the physical-space mean of v . w with w
+    the curl computed from the same spectral fields (tests/energy.f90:20-32
+    style: Parseval over x, y and physical z)."""
+    g = grid(24, 16, 57, 25, 5)
+    rng = np.random.default_rng(3)
+    S = [O.fftp3d_real_to_complex(g, rng.standard_normal(g.real_shape)) for _ in range(3)]
+    for a in S:  # no Nyquist content (the x Nyquist column carries weight 2)
+        a[g.nkx - 1] = 0.0
+        a[:, g.ny // 2] = 0.0
+    W = [O.curlk(g, S[1], S[2], 1), O.curlk(g, S[0], S[2], 2), O.curlk(g, S[0], S[1], 3)]
+    Rv = [O.fftp3d_complex_to_real(g, a.copy()) for a in S]
+    Rw = [O.fftp3d_complex_to_real(g, a.copy()) for a in W]
+    n = float(g.nx * g.ny * g.nz)
+    ref = sum(np.mean(rv[: g.np_] * rw[: g.np_]) for rv, rw in zip(Rv, Rw)) / n ** 2
+    assert abs(O.helicity(g, *S) - ref) < 1e-12 * abs(ref) + 1e-12
+
+
 def test_divergence_kat():
     """tests/poisson.f90:36-55: numeric vs analytic mean-square divergence."""
     g = grid(32, 32, 64 + 25, 25, 5)
