@@ -135,6 +135,17 @@ int spx_fc_filter(spx_plan* p, void* a);                           /* :1082-1115
  * (complex float in SPX_FP32_MIXED plans, complex double otherwise). */
 int spx_gradre(spx_plan* p, const void* a, const void* b, const void* c, void* d, void* e,
                void* f);
+/* No counterpart in the reference: gradre for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  Same
+ * arguments and results as spx_gradre, but the work arrays come from a static
+ * 128 MiB device array that every GPU the library runs on holds until the
+ * process ends, instead of memory the plan allocates on first use.  Grids
+ * whose work set exceeds it (spx_gradre_scratch_bytes: need > limit; e.g.
+ * float64 256^3) are refused with SPX_ERR_UNSUPPORTED.  Calls that overlap on
+ * one GPU share that array and race.  The plan's device must be current. */
+int spx_gradre_noalloc(spx_plan* p, const void* a, const void* b, const void* c, void* d,
+                       void* e, void* f);
+int spx_gradre_scratch_bytes(const spx_plan* p, size_t* need, size_t* limit);
 /* advect (pseudospec_phd.f90:23-113): (A.grad)B for a scalar B = d; result e
  * (complex float in SPX_FP32_MIXED plans). */
 int spx_advect(spx_plan* p, const void* a, const void* b, const void* c, const void* d,

@@ -60,6 +60,8 @@ _SIGS = {
     "spx_fc_filter": [_P, _P],
     "spx_gradre": [_P, _P, _P, _P, _P, _P, _P],
     "spx_advect": [_P, _P, _P, _P, _P, _P],
+    "spx_gradre_noalloc": [_P, _P, _P, _P, _P, _P, _P],
+    "spx_gradre_scratch_bytes": [_P, POINTER(c_size_t), POINTER(c_size_t)],
     "spx_sol_project": [_P, _P, _P, _P, _P, c_int, c_int, c_int],
     "spx_v_imposebc_and_project": [_P, _P, _P, _P, _P, c_int, c_int, POINTER(c_double),
                                    POINTER(c_double)],
