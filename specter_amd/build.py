@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libspx.so")
 # (source, defines): spx_fft.hip is compiled in three parts (see its header)
 SOURCES = [("spx_fft.hip", ["-DSPX_PART=1"]), ("spx_fft.hip", ["-DSPX_PART=2"]),
            ("spx_fft.hip", ["-DSPX_PART=3"]), ("spx_ops.hip", []), ("spx_api.hip", []),
-           ("spx_noalloc.hip", [])]
+           ("spx_noalloc.hip", []), ("spx_wall.hip", [])]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -350,6 +350,7 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
     }
     if ((rc = upload(p, reinterpret_cast<double**>(&p->d_wtop), w))) return bail(rc);
     if ((rc = build_harmonic_table(p))) return bail(rc);
+    if ((rc = build_wall_mfma(p))) return bail(rc);
   }
   if (p->fp32) {
     if ((rc = build_line_fft(p->fx32, nx, 8, 0))) return bail(rc);

@@ -100,6 +100,19 @@ struct Profiler {
   std::vector<double> total_ms, total_bytes;
 };
 
+// the wall update as a rank-(2+C) correction on the matrix cores (spx_wall.hip)
+struct WallMfmaHost {
+  bool on = false;
+  int nbk = 0;   // 16-row blocks of the 2 nz real line values
+  int kbw = 0;   // blocks per wave
+  int nw = 0;    // waves per workgroup
+  int mt = 0;    // 16-row tiles of the partial inverse DFT (2 (2d + C) rows)
+  int kt = 0;    // 4-column steps of the correction (2 (2 + C) columns)
+  int wpe = 0;   // waves per SIMD the kernel's register budget is sized for
+  double* winv = nullptr;
+  double* wfwd = nullptr;
+};
+
 }  // namespace spx
 
 // Opaque plan (FCPLAN + fftp plans + module globals of kes/grid, fcgram_mod.f90:34-38,
@@ -146,6 +159,7 @@ struct spx_plan {
   // nonlinear pipeline.
   spx::LineFFTHost fx, fy, fz, fz2;
   spx::LineFFTHost fx32, fy32, fz32;
+  spx::WallMfmaHost wall;
   bool work_bouss = false;
   // scratch
   void* X = nullptr;        // xy-stage buffer, nc float64 complex
@@ -204,6 +218,15 @@ int fail(int code, const std::string& msg);
 // The RK stage update of one component applied while a kernel loads it
 // (include/hd/hd_rkstep2.f90:11-32, same operation order as op_rk_update):
 //   v <- c0 + ((nu*(-kk2 v) - ((nl*fx)*fy)*fz + f)*dt)*rmp
+// Wall conditions of the wall-update kernels: comp 0 vx (kx), comp 1 vy (ky);
+// scalar mode: zero at both walls (s_constant_z).
+struct WallParams {
+  double tau;          // pressure time-extrapolation factor
+  double vb[2][2];     // [comp][pos] wall velocity
+  double nxny;         // nx*ny
+  int scalar;          // 1: s_constant_z (zero at walls)
+};
+
 struct RkLoadComp {
   const void* c0;       // float64 complex
   const void* nl;       // nonlinear term, float64 or float32 complex (nl32)
@@ -284,6 +307,12 @@ int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, cons
                int ncomp = 1);
 // fused v_imposebc: goto_domain_w_boundaries + noslip_z + goto_3d_fourier on vx,vy
 // rk: the RK update of vx, vy (c[0], c[1]) is applied on load (null: none)
+// wall update on the matrix cores (spx_wall.hip): tables at plan creation
+// (no-op when the (d, C) pair or nz is not covered), the launch returns
+// SPX_ERR_UNSUPPORTED when the plan has no tables
+int build_wall_mfma(spx_plan* p);
+int wall_mfma(spx_plan* p, void* vx, void* vy, const void* pr, const WallParams& wp, int ncomp,
+              const RkLoad* rk);
 int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
                  const double vzsta[2], const double vzend[2], const RkLoad* rk = nullptr);
 // s_imposebc: z IFFT, 1/nz, th = 0 at both walls, continuation + z FFT

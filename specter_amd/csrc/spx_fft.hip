@@ -577,12 +577,6 @@ using LFd = LineFFT<Cd>;
 // goto_domain_w_boundaries + noslip_z + goto_3d_fourier fused per line
 // (boundary_mod.fpp:72-194, vboundary.f90:154-211, sboundary.f90:122-165).
 // comp 0: vx (kx), comp 1: vy (ky); scalar mode: wall values are zero.
-struct WallParams {
-  double tau;          // pressure time-extrapolation factor
-  double vb[2][2];     // [comp][pos] wall velocity
-  double nxny;         // nx*ny
-  int scalar;          // 1: s_constant_z (zero at walls)
-};
 
 template <typename TS, int RS>
 __global__ void __launch_bounds__(kMaxBlock) k_noslip(cpx<TS>* __restrict__ vx, cpx<TS>* __restrict__ vy,
@@ -1508,6 +1502,7 @@ int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, cons
 // ---- float64 boundary / projection / diagnostics kernels ----
 static int noslip_launch(spx_plan* p, void* vx, void* vy, const void* pr, const WallParams& wp,
                          int ncomp, const char* name, const RkLoad* rk) {
+  if (p->wall.on) return wall_mfma(p, vx, vy, pr, wp, ncomp, rk);
   const LineFFTHost& f = p->fz;
   const long nlines = p->wlines();
   dim3 grid(unsigned((nlines + f.nl - 1) / f.nl), unsigned(ncomp));

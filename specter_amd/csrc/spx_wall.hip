@@ -1,0 +1,384 @@
+// Wall update of the velocity (and scalar) z lines as a rank-(2+C) correction
+// on the matrix cores (v_mfma_f64_16x16x4f64).
+//
+// goto_domain_w_boundaries + noslip_z + goto_3d_fourier (boundary_mod.fpp:72-194,
+// vboundary.f90:154-211; s_constant_z, sboundary.f90:122-165) turn a z line V
+// into FFT(x) where x = IFFT(V)/nz on the physical points, with the two wall
+// values replaced and the C continuation points recomputed from the 2d
+// boundary values (fftp.fpp:755-773).  Only 2 + C points of x change, so
+//
+//   V' = V + FFT(delta),  delta supported on z = 0, np-1, np..nz-1,
+//
+// with delta from the unnormalised inverse DFT of V at the 2d + C points
+// z in {0..d-1, np-d..np-1, np..nz-1}.  Both are dense products over a tile
+// of 16 lines:
+//
+//   GEMM A  X[q][l]   = sum_kappa WinvT[q][kappa] Vr[kappa][l]      (partial IDFT)
+//   GEMM C  Vr'[kappa][l] = Vr[kappa][l] + sum_rho Wf[kappa][rho] dr[rho][l]
+//
+// in real form (kappa = 2z + {re, im}; q and rho likewise), float64, with the
+// DFT matrices formed in long double.  Vr stays in registers between the two
+// products.  v_mfma_f64_16x16x4f64 takes A[i][k] from lane i + 16k, B[k][j]
+// from lane j + 16k and keeps D[i][j] in register i / 4 of lane j + 16 (i % 4)
+// (measured).  Each wave holds a range of 16-row kappa blocks of the 16 lines
+// as GEMM C accumulators whose rows are permuted within a block (row
+// 4r + g <-> kappa offset 4g + r), so that lane group g, registers r = 0..3,
+// holds the two consecutive complex values z = 8 block + 2g, +1 of its line
+// (one 32-byte load and store); the same registers are GEMM A's B operand
+// (register s = K step s, kappa offset 4g + s).  A line is read once (with
+// the RK update applied on the load) and written once.  The waves'
+// partial X are summed in LDS in wave order (deterministic), delta is formed
+// there exactly as the reference forms the continuation, and GEMM C
+// accumulates onto Vr.
+//
+// Besides moving the transforms off the LDS FFT engine, the correction keeps
+// the interior of every line exactly as the RK update left it: only delta
+// (a small correction) is rounded through a transform.  The reference's
+// inverse-then-forward round trip rounds the whole line twice per stage.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "spx_engine.h"
+
+namespace spx {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+namespace {
+
+using Cd = cpx<double>;
+
+__device__ __forceinline__ Cd ikmul_d(double k, Cd v) { return mkc<double>(-(k * v.y), k * v.x); }
+
+__device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// GEMM A rows [M0, M1) (16-row tiles) of the wave's kappa blocks, then the
+// waves' partial sums into the ns LDS slots in a fixed order: waves < ns
+// store, waves >= ns add onto slot wave - ns (one barrier between).
+template <int MT, int M0, int M1, int KBW>
+__device__ __forceinline__ void gemm_a_rows(const double* __restrict__ winv, const v4d (&vr)[KBW],
+                                            int b0, int nbk, int lane, int wave, int ns,
+                                            double* red) {
+  v4d acc[M1 - M0];
+#pragma unroll
+  for (int m = 0; m < M1 - M0; ++m) acc[m] = v4d{0, 0, 0, 0};
+#pragma unroll
+  for (int b = 0; b < KBW; ++b) {
+    const int bb = b0 + b;
+    if (bb < nbk) {  // wave-uniform
+      const double* wa = winv + size_t(bb) * 4 * MT * 64 + lane;
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int m = M0; m < M1; ++m)
+          acc[m - M0] = mfma_f64(wa[(s * MT + m) * 64], vr[b][s], acc[m - M0]);
+    }
+  }
+  if (wave < ns) {
+#pragma unroll
+    for (int m = M0; m < M1; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[((wave * MT + m) * 4 + r) * 64 + lane] = acc[m - M0][r];
+  }
+  __syncthreads();
+  if (wave >= ns) {
+    double* slot = red + (wave - ns) * MT * 256;
+#pragma unroll
+    for (int m = M0; m < M1; ++m)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int e = (m * 4 + r) * 64 + lane;
+        slot[e] = slot[e] + acc[m - M0][r];
+      }
+  }
+}
+
+// MT: 16-row tiles of the 2(2d+C) real partial-IDFT outputs; KT: 4-column
+// steps of the 2(2+C) real correction inputs; KBW: 16-row kappa blocks per
+// wave; WPE: waves per SIMD the register budget is sized for.
+template <int MT, int KT, int KBW, int WPE>
+__global__ void __launch_bounds__(512, WPE) k_wall_mfma(
+    double* __restrict__ vx, double* __restrict__ vy, const Cd* __restrict__ pr, long nlines,
+    int ny, int n, int np, int nbk, double invn, WallParams wp, const double* __restrict__ kx,
+    const double* __restrict__ ky, const double* __restrict__ dir, int C, int d,
+    const double* __restrict__ winv, const double* __restrict__ wfwd, RkLoad rk, int rk_on) {
+#pragma clang fp contract(off)
+  extern __shared__ double sm[];
+  const int nw = blockDim.x >> 6;
+  const int ns = nw < 4 ? nw : 4;               // partial-sum slots
+  double* const red = sm;                       // [ns][MT][4][64] partial X
+  double* const xs = red;                       // slot 0 ends up holding X
+  double* const ds = red + ns * MT * 256;       // [4 KT][16] real delta
+  const int comp = blockIdx.y;
+  double* const V = comp ? vy : vx;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int li = lane & 15, g = lane >> 4;
+  const long lbase = long(blockIdx.x) * 16;
+  const long ln = lbase + li;
+  const bool okl = ln < nlines;
+  const long lc = okl ? ln : 0;
+  const long ki = lc / ny;
+  const int kj = int(lc - ki * ny);
+  const int b0 = wave * KBW;
+
+  // ---- load the wave's kappa blocks (RK update applied on the load)
+  v4d vr[KBW];
+#pragma unroll
+  for (int b = 0; b < KBW; ++b) {
+    const int bb = b0 + b;
+    const int z0 = 8 * bb + 2 * g;
+    Cd v0 = mkc<double>(0, 0), v1 = mkc<double>(0, 0);
+    if (okl && bb < nbk) {
+      const Cd* src = reinterpret_cast<const Cd*>(V) + ln * n;
+      if (z0 < n) {
+        v0 = src[z0];
+        if (rk_on) v0 = rk_on_load(rk, comp, v0, ln * n + z0, ki, kj, z0);
+      }
+      if (z0 + 1 < n) {
+        v1 = src[z0 + 1];
+        if (rk_on) v1 = rk_on_load(rk, comp, v1, ln * n + z0 + 1, ki, kj, z0 + 1);
+      }
+    }
+    vr[b] = v4d{v0.x, v0.y, v1.x, v1.y};
+  }
+
+  // ---- GEMM A: partial inverse DFT at the 2d + C points, in two row groups
+  // (fewer live accumulators), summed over the waves in a fixed order; then
+  // slot 0 += slots 1..ns-1
+  constexpr int G = KBW >= 18 ? MT : 3;
+  gemm_a_rows<MT, 0, G, KBW>(winv, vr, b0, nbk, lane, wave, ns, red);
+  if constexpr (G < MT) gemm_a_rows<MT, G, MT, KBW>(winv, vr, b0, nbk, lane, wave, ns, red);
+  __syncthreads();
+  for (int e = threadIdx.x; e < MT * 256; e += blockDim.x) {
+    double s = red[e];
+    for (int w = 1; w < ns; ++w) s = s + red[w * MT * 256 + e];
+    xs[e] = s;
+  }
+  for (int e = threadIdx.x; e < KT * 64; e += blockDim.x) ds[e] = 0.0;
+  __syncthreads();
+
+  // ---- delta at the walls and the continuation points (16 lines x (2 + C))
+  // X of point index qi (0..2d+C-1) and line l: real row 2qi, imaginary 2qi+1
+  auto xget = [&](int qi, int l) -> Cd {
+    auto at = [&](int qr) {
+      const int m = qr >> 4, rem = qr & 15;
+      return xs[(m * 4 + (rem >> 2)) * 64 + (rem & 3) * 16 + l];
+    };
+    return mkc<double>(at(2 * qi), at(2 * qi + 1));
+  };
+  const int nt = 16 * (2 + C);
+  const int top = np - d;
+  for (int e = threadIdx.x; e < nt; e += blockDim.x) {
+    const int l = e / (2 + C), t = e - l * (2 + C);
+    const long lt = lbase + l;
+    if (lt >= nlines) continue;
+    // wall values (vboundary.f90:195-209; zero for the scalar)
+    Cd wb = mkc<double>(0, 0), wt = mkc<double>(0, 0);
+    if (!wp.scalar) {
+      const long ti = lt / ny;
+      const int tj = int(lt - ti * ny);
+      const double kxi = kx[ti], kyj = ky[tj];
+      if (kxi == 0.0 && kyj == 0.0) {
+        wb = mkc<double>(wp.nxny * wp.vb[comp][0], 0.0);
+        wt = mkc<double>(wp.nxny * wp.vb[comp][1], 0.0);
+      } else {
+        wb = cscale(ikmul_d(comp ? kyj : kxi, pr[lt * n]), wp.tau);
+        wt = cscale(ikmul_d(comp ? kyj : kxi, pr[lt * n + np - 1]), wp.tau);
+      }
+    }
+    // physical-domain value of boundary point z (0..d-1 or np-d..np-1)
+    auto xn = [&](int z) -> Cd {
+      if (z == 0) return wb;
+      if (z == np - 1) return wt;
+      const int qi = z < d ? z : d + (z - top);
+      return cscale(xget(qi, l), invn);
+    };
+    Cd delta;
+    if (t == 0) {
+      delta = csub(wb, cscale(xget(0, l), invn));
+    } else if (t == 1) {
+      delta = csub(wt, cscale(xget(2 * d - 1, l), invn));
+    } else {
+      const int ii = t - 2;
+      const double* D0 = dir + ii * d;
+      const double* D1 = dir + (C - 1 - ii) * d;
+      const Cd xt = xn(top), xb = xn(d - 1);
+      double ar = D0[0] * xt.x + D1[0] * xb.x;
+      double ai = D0[0] * xt.y + D1[0] * xb.y;
+      for (int jj = 1; jj < d; ++jj) {
+        const Cd ut = xn(top + jj), ub = xn(d - 1 - jj);
+        ar = ar + D0[jj] * ut.x;
+        ar = ar + D1[jj] * ub.x;
+        ai = ai + D0[jj] * ut.y;
+        ai = ai + D1[jj] * ub.y;
+      }
+      delta = csub(mkc<double>(ar, ai), cscale(xget(2 * d + ii, l), invn));
+    }
+    ds[(2 * t) * 16 + l] = delta.x;
+    ds[(2 * t + 1) * 16 + l] = delta.y;
+  }
+  __syncthreads();
+
+  // ---- GEMM C: Vr += Wf * delta, then store
+#pragma unroll
+  for (int b = 0; b < KBW; ++b) {
+    const int bb = b0 + b;
+    if (bb < nbk) {
+      const double* wa = wfwd + size_t(bb) * KT * 64 + lane;
+#pragma unroll
+      for (int t = 0; t < KT; ++t) vr[b] = mfma_f64(wa[t * 64], ds[(4 * t + g) * 16 + li], vr[b]);
+    }
+  }
+  if (okl) {
+    Cd* dst = reinterpret_cast<Cd*>(V) + ln * n;
+#pragma unroll
+    for (int b = 0; b < KBW; ++b) {
+      const int bb = b0 + b;
+      const int z0 = 8 * bb + 2 * g;
+      if (bb < nbk) {
+        if (z0 < n) dst[z0] = mkc<double>(vr[b][0], vr[b][1]);
+        if (z0 + 1 < n) dst[z0 + 1] = mkc<double>(vr[b][2], vr[b][3]);
+      }
+    }
+  }
+}
+
+constexpr int kMaxWaves = 8;
+
+}  // namespace
+
+// Tables of the plan's wall correction (float64 plans' and mixed plans' state
+// are float64 alike): fragments of WinvT and Wf in the lane order of the MFMA
+// operands, formed in long double.
+int build_wall_mfma(spx_plan* p) {
+  WallMfmaHost& w = p->wall;
+  w = WallMfmaHost();
+  const char* sw = std::getenv("SPX_WALL_MFMA");
+  if (p->g.Cz <= 0 || (sw && std::strcmp(sw, "0") == 0)) return SPX_OK;
+  const int n = p->g.nz, np = p->np, C = p->g.Cz, d = p->g.oz;
+  const int nq = 2 * d + C, nrho = 2 + C;
+  const int mt = (2 * nq + 15) / 16, kt = (2 * nrho + 3) / 4;
+  const int nbk = (2 * n + 15) / 16;
+  if (!((mt == 5 || mt == 6) && kt <= 16 && np >= 2 * d)) return SPX_OK;
+  // waves per workgroup and kappa blocks per wave: the fewest registers that
+  // keep the workgroup within 8 waves
+  int kbw = 9;
+  if ((nbk + kbw - 1) / kbw > kMaxWaves) kbw = 18;
+  if ((nbk + kbw - 1) / kbw > kMaxWaves) return SPX_OK;
+  if (const char* e = std::getenv("SPX_WALL_KBW")) {
+    const int v = std::atoi(e);
+    if ((v == 9 || v == 18) && (nbk + v - 1) / v <= kMaxWaves) kbw = v;
+  }
+  // register budget: 3 or 4 waves per SIMD for 9 blocks per wave, 2 for 18
+  int wpe = 3;
+  if (const char* e = std::getenv("SPX_WALL_WPE")) {
+    const int v = std::atoi(e);
+    if (v == 3 || v == 4) wpe = v;
+  }
+  w.wpe = kbw == 18 ? 2 : wpe;
+  w.nbk = nbk;
+  w.kbw = kbw;
+  w.nw = (nbk + kbw - 1) / kbw;
+  w.mt = mt;
+  w.kt = kt;
+  // points of the partial inverse DFT and of the correction
+  std::vector<int> qz(nq), rz(nrho);
+  for (int i = 0; i < d; ++i) qz[i] = i;
+  for (int i = 0; i < d; ++i) qz[d + i] = np - d + i;
+  for (int i = 0; i < C; ++i) qz[2 * d + i] = np + i;
+  rz[0] = 0;
+  rz[1] = np - 1;
+  for (int i = 0; i < C; ++i) rz[2 + i] = np + i;
+  const long double PI2 = 6.283185307179586476925286766559005768L;
+  std::vector<double> cr(n), ci(n);
+  for (int m = 0; m < n; ++m) {
+    cr[m] = double(cosl(PI2 * (long double)m / (long double)n));
+    ci[m] = double(sinl(PI2 * (long double)m / (long double)n));
+  }
+  // WinvT[qr][kappa]: x_q = sum_z V_z e^{+2 pi i z q / n}
+  auto winvT = [&](int qr, int kap) -> double {
+    const int qi = qr >> 1, qc = qr & 1, z = kap >> 1, zc = kap & 1;
+    if (qi >= nq || z >= n) return 0.0;
+    const int m = int((long long)z * qz[qi] % n);
+    const double wr = cr[m], wi = ci[m];
+    if (qc == 0) return zc == 0 ? wr : -wi;
+    return zc == 0 ? wi : wr;
+  };
+  // Wf[kappa][rho]: V'_z += sum_p delta_p e^{-2 pi i z r_p / n}
+  auto wf = [&](int kap, int rho) -> double {
+    const int z = kap >> 1, zc = kap & 1, pi = rho >> 1, pc = rho & 1;
+    if (pi >= nrho || z >= n) return 0.0;
+    const int m = int((long long)z * rz[pi] % n);
+    const double er = cr[m], ei = -ci[m];
+    if (zc == 0) return pc == 0 ? er : -ei;
+    return pc == 0 ? ei : er;
+  };
+  std::vector<double> ta(size_t(nbk) * 4 * mt * 64), tc(size_t(nbk) * kt * 64);
+  for (int bb = 0; bb < nbk; ++bb)
+    for (int s = 0; s < 4; ++s)
+      for (int m = 0; m < mt; ++m)
+        for (int l = 0; l < 64; ++l)
+          ta[((size_t(bb) * 4 + s) * mt + m) * 64 + l] =
+              winvT(16 * m + (l & 15), 16 * bb + 4 * (l >> 4) + s);
+  for (int bb = 0; bb < nbk; ++bb)
+    for (int t = 0; t < kt; ++t)
+      for (int l = 0; l < 64; ++l)
+        tc[(size_t(bb) * kt + t) * 64 + l] =
+            wf(16 * bb + 4 * ((l & 15) & 3) + ((l & 15) >> 2), 4 * t + (l >> 4));
+  double** dst[2] = {&w.winv, &w.wfwd};
+  const std::vector<double>* src[2] = {&ta, &tc};
+  for (int i = 0; i < 2; ++i) {
+    const size_t bytes = src[i]->size() * sizeof(double);
+    void* dptr = nullptr;
+    if (hipMalloc(&dptr, bytes) != hipSuccess) return fail(SPX_ERR_ALLOC, "wall correction tables");
+    p->owned.push_back(dptr);
+    p->dev_bytes += bytes;
+    SPX_HIP(hipMemcpy(dptr, src[i]->data(), bytes, hipMemcpyHostToDevice));
+    *dst[i] = static_cast<double*>(dptr);
+  }
+  w.on = true;
+  return SPX_OK;
+}
+
+#define SPX_WALL_LAUNCH(MT, KT, KBW, WPE)                                                     \
+  hipLaunchKernelGGL((k_wall_mfma<MT, KT, KBW, WPE>), grid, dim3(64 * w.nw), lds, p->stream,       \
+                     static_cast<double*>(vx), static_cast<double*>(vy),                      \
+                     static_cast<const Cd*>(pr), nlines, p->g.ny, p->g.nz, p->np, w.nbk,      \
+                     1.0 / double(p->g.nz), wp, p->d_kx, p->d_ky, p->d_dir, p->g.Cz, p->g.oz, \
+                     w.winv, w.wfwd, r, rk ? 1 : 0)
+
+int wall_mfma(spx_plan* p, void* vx, void* vy, const void* pr, const WallParams& wp, int ncomp,
+              const RkLoad* rk) {
+  const WallMfmaHost& w = p->wall;
+  if (!w.on) return SPX_ERR_UNSUPPORTED;
+  const long nlines = p->wlines();
+  const dim3 grid(unsigned((nlines + 15) / 16), unsigned(ncomp));
+  const size_t lds =
+      (size_t(std::min(w.nw, 4)) * w.mt * 256 + size_t(w.kt) * 64) * sizeof(double);
+  RkLoad r;
+  std::memset(&r, 0, sizeof(r));
+  if (rk) r = *rk;
+  const double extra = rk ? (rk->nl32 ? 8.0 : 16.0) + 32.0 : 0.0;  // nl, c0, f reads
+  ProfScope ps_(p, rk ? "noslip_rk" : (wp.scalar ? "sconst" : "noslip"),
+                ncomp * double(p->wnc()) * (2.0 * 16.0 + extra));
+  // the partial-IDFT tile count follows (d, C): 5 for d = 5, C = 25 (70 rows),
+  // 6 for d = 8, C = 27 (86 rows); the correction steps 14 / 15
+  const int key = w.mt * 10000 + w.kt * 100 + w.kbw * 10 / 9 + w.wpe;  // kbw 9 -> 10, 18 -> 20
+  switch (key) {
+    case 51413: SPX_WALL_LAUNCH(5, 14, 9, 3); break;
+    case 51414: SPX_WALL_LAUNCH(5, 14, 9, 4); break;
+    case 51422: SPX_WALL_LAUNCH(5, 14, 18, 2); break;
+    case 61513: SPX_WALL_LAUNCH(6, 15, 9, 3); break;
+    case 61514: SPX_WALL_LAUNCH(6, 15, 9, 4); break;
+    case 61522: SPX_WALL_LAUNCH(6, 15, 18, 2); break;
+    default: return SPX_ERR_UNSUPPORTED;
+  }
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+
+}  // namespace spx

@@ -217,3 +217,34 @@ def test_diagnostics(shape):
     bt = p.bouncheck_z(tc)
     rb = O.bouncheck_z(g, c)
     assert rel_err(np.array(bt), np.array(rb)) < 1e-12
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("o,ordr", [(2, 2), (3, 4)])
+def test_wall_update_mfma_matches_transform_path(shape, o, ordr, monkeypatch):
+    """The wall update as a rank-(2+C) correction on the matrix cores (the
+    default) against the same update through the z transforms
+    (SPX_WALL_MFMA=0 at plan creation), and both against the oracle; also the
+    scalar wall condition."""
+    torch, O, g, p = _setup(shape)
+    from specter_amd import FCPlan
+    monkeypatch.setenv("SPX_WALL_MFMA", "0")
+    q = FCPlan(*shape)
+    vx, vy, vz, pr = (_rand_spec(g, s) for s in (40, 41, 42, 43))
+    r = [x.copy() for x in (vx, vy, vz, pr)]
+    O.v_imposebc_and_project(g, *r, o, ordr, (0.0, 0.0), (0.0, 0.0))
+    ta = [_cu(torch, x) for x in (vx, vy, vz, pr)]
+    tb = [_cu(torch, x) for x in (vx, vy, vz, pr)]
+    p.v_imposebc_and_project(*ta, o, ordr)
+    q.v_imposebc_and_project(*tb, o, ordr)
+    for a, b, rr in zip(ta, tb, r):
+        assert rel_err(a.cpu().numpy(), rr) < _proj_tol(shape)
+        assert rel_err(a.cpu().numpy(), b.cpu().numpy()) < _proj_tol(shape)
+    th = _rand_spec(g, 44)
+    rs = th.copy()
+    O.s_imposebc(g, rs)
+    t1, t2 = _cu(torch, th), _cu(torch, th)
+    p.s_imposebc(t1)
+    q.s_imposebc(t2)
+    assert rel_err(t1.cpu().numpy(), rs) < 1e-12
+    assert rel_err(t1.cpu().numpy(), t2.cpu().numpy()) < 1e-12
