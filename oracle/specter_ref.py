@@ -28,10 +28,17 @@ Pinning of each function (tests/test_oracle_*.py):
                                Neumann data in closed form
   * energy/cross/divergence    Parseval vs real-space sums (tests/energy.f90:20-32)
   * helicity                   Beltrami fields: <v.curl v> = -k <|v|^2> exactly
-  * hd_substep / bouss_substep composition of the above, order as in
-                               include/hd/hd_rkstep2.f90 and include/bouss/bouss_rkstep2.f90
-                               (parity unpinned beyond its parts: the reference run that
-                               would pin the trajectory cannot be built here)
+  * hd_substep / bouss_substep composition of the above in the order of include/hd/hd_rkstep2.f90
+    hd_step / bouss_step        and include/bouss/bouss_rkstep2.f90, pinned by
+                               tests/test_oracle_steps.py: a step on an eigenmode of the
+                               wall-projected Laplacian is the Taylor polynomial of its
+                               eigenvalue; tangential wall slip is O(dt^2) only with the
+                               pressure extrapolation factor of vboundary.f90:195-196; the
+                               Boussinesq rest state is hydrostatic; planted composition
+                               mistakes fail
+  * sol_project pressure d      the removed velocity is (i kx d, i ky d, d/dz d)
+  * precision="longdouble"      the same routines in extended precision (constants formed
+                               in float64): the exact-trajectory referee of the float64 runs
 
 Parity notes (deliberate, documented):
   * kz ordering for odd nz: the reference loop specter.fpp:783-786 leaves kz(nz)

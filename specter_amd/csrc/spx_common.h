@@ -109,8 +109,11 @@ struct WallMfmaHost {
   int mt = 0;    // 16-row tiles of the partial inverse DFT (2 (2d + C) rows)
   int kt = 0;    // 4-column steps of the correction (2 (2 + C) columns)
   int wpe = 0;   // waves per SIMD the kernel's register budget is sized for
+  int tab = 1;   // 1: A operands from the roots of unity in LDS, 0: fragment tables
+  unsigned magic = 0;  // floor(2^32 / nz) + 1 (x mod nz by multiplication)
   double* winv = nullptr;
   double* wfwd = nullptr;
+  double* roots = nullptr;  // exp(2 pi i m / nz), m < nz
 };
 
 }  // namespace spx

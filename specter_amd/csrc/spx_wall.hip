@@ -59,23 +59,78 @@ __device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c) {
 // GEMM A rows [M0, M1) (16-row tiles) of the wave's kappa blocks, then the
 // waves' partial sums into the ns LDS slots in a fixed order: waves < ns
 // store, waves >= ns add onto slot wave - ns (one barrier between).
-template <int MT, int M0, int M1, int KBW>
-__device__ __forceinline__ void gemm_a_rows(const double* __restrict__ winv, const v4d (&vr)[KBW],
-                                            int b0, int nbk, int lane, int wave, int ns,
-                                            double* red) {
+// Point z of partial-IDFT index qi (0..d-1, np-d..np-1, np..n-1) and of
+// correction index p (0, np-1, np..n-1).
+__device__ __forceinline__ int qpoint(int qi, int d, int np, int nq) {
+  if (qi >= nq) return 0;
+  return qi < d ? qi : (qi < 2 * d ? np - 2 * d + qi : np - 2 * d + qi);
+}
+__device__ __forceinline__ int rpoint(int p, int np, int nrho) {
+  if (p >= nrho) return 0;
+  return p == 0 ? 0 : (p == 1 ? np - 1 : np + p - 2);
+}
+
+// x mod n for 0 <= x < 2^31 with magic = floor(2^32 / n) + 1
+__device__ __forceinline__ int modn(unsigned x, int n, unsigned magic) {
+  const unsigned q = __umulhi(x, magic);
+  int r = int(x - q * unsigned(n));
+  return r < 0 ? r + n : r;
+}
+
+// TAB 0: A operands read from per-plan fragment tables in global memory;
+// TAB 1: formed from the n roots of unity staged in LDS (index z q mod n).
+template <int MT, int M0, int M1, int KBW, int TAB>
+__device__ __forceinline__ void gemm_a_rows(const double* __restrict__ winv, const Cd* rl,
+                                            const v4d (&vr)[KBW], int b0, int nbk, int lane,
+                                            int wave, int ns, double* red, int n, int np,
+                                            int d, int nq) {
   v4d acc[M1 - M0];
 #pragma unroll
   for (int m = 0; m < M1 - M0; ++m) acc[m] = v4d{0, 0, 0, 0};
+  if constexpr (TAB == 0) {
 #pragma unroll
-  for (int b = 0; b < KBW; ++b) {
-    const int bb = b0 + b;
-    if (bb < nbk) {  // wave-uniform
-      const double* wa = winv + size_t(bb) * 4 * MT * 64 + lane;
+    for (int b = 0; b < KBW; ++b) {
+      const int bb = b0 + b;
+      if (bb < nbk) {  // wave-uniform
+        const double* wa = winv + size_t(bb) * 4 * MT * 64 + lane;
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
+        for (int s = 0; s < 4; ++s)
 #pragma unroll
-        for (int m = M0; m < M1; ++m)
-          acc[m - M0] = mfma_f64(wa[(s * MT + m) * 64], vr[b][s], acc[m - M0]);
+          for (int m = M0; m < M1; ++m)
+            acc[m - M0] = mfma_f64(wa[(s * MT + m) * 64], vr[b][s], acc[m - M0]);
+      }
+    }
+  } else {
+    // lane = i + 16k: row 16m + i = (point qi, re/im qc); K index 4k + s of
+    // block bb = (z = 8 bb + 2k + s/2, re/im s&1); e^{+2 pi i z q / n}
+    const int i = lane & 15, k = lane >> 4;
+    const int qc = i & 1;
+    int q[M1 - M0], inc[M1 - M0], idx[M1 - M0];
+#pragma unroll
+    for (int m = M0; m < M1; ++m) {
+      q[m - M0] = qpoint((16 * m + i) >> 1, d, np, nq);
+      inc[m - M0] = (8 * q[m - M0]) % n;
+      idx[m - M0] = int((long long)(8 * b0 + 2 * k) * q[m - M0] % n);
+    }
+#pragma unroll
+    for (int b = 0; b < KBW; ++b) {
+      const int bb = b0 + b;
+      if (bb < nbk) {
+#pragma unroll
+        for (int m = M0; m < M1; ++m) {
+          int i1 = idx[m - M0] + q[m - M0];
+          i1 = i1 >= n ? i1 - n : i1;
+          const Cd wa = rl[idx[m - M0]], wb = rl[i1];
+          const double a0 = qc ? wa.y : wa.x, a1 = qc ? wa.x : -wa.y;
+          const double a2 = qc ? wb.y : wb.x, a3 = qc ? wb.x : -wb.y;
+          acc[m - M0] = mfma_f64(a0, vr[b][0], acc[m - M0]);
+          acc[m - M0] = mfma_f64(a1, vr[b][1], acc[m - M0]);
+          acc[m - M0] = mfma_f64(a2, vr[b][2], acc[m - M0]);
+          acc[m - M0] = mfma_f64(a3, vr[b][3], acc[m - M0]);
+          const int nx = idx[m - M0] + inc[m - M0];
+          idx[m - M0] = nx >= n ? nx - n : nx;
+        }
+      }
     }
   }
   if (wave < ns) {
@@ -100,17 +155,19 @@ __device__ __forceinline__ void gemm_a_rows(const double* __restrict__ winv, con
 // MT: 16-row tiles of the 2(2d+C) real partial-IDFT outputs; KT: 4-column
 // steps of the 2(2+C) real correction inputs; KBW: 16-row kappa blocks per
 // wave; WPE: waves per SIMD the register budget is sized for.
-template <int MT, int KT, int KBW, int WPE>
+template <int MT, int KT, int KBW, int WPE, int TAB>
 __global__ void __launch_bounds__(512, WPE) k_wall_mfma(
     double* __restrict__ vx, double* __restrict__ vy, const Cd* __restrict__ pr, long nlines,
     int ny, int n, int np, int nbk, double invn, WallParams wp, const double* __restrict__ kx,
     const double* __restrict__ ky, const double* __restrict__ dir, int C, int d,
-    const double* __restrict__ winv, const double* __restrict__ wfwd, RkLoad rk, int rk_on) {
+    const double* __restrict__ winv, const double* __restrict__ wfwd,
+    const Cd* __restrict__ roots, unsigned magic, RkLoad rk, int rk_on) {
 #pragma clang fp contract(off)
   extern __shared__ double sm[];
   const int nw = blockDim.x >> 6;
   const int ns = nw < 4 ? nw : 4;               // partial-sum slots
-  double* const red = sm;                       // [ns][MT][4][64] partial X
+  Cd* const rl = reinterpret_cast<Cd*>(sm);     // [n] roots of unity (TAB 1)
+  double* const red = sm + (TAB ? 2 * n : 0);   // [ns][MT][4][64] partial X
   double* const xs = red;                       // slot 0 ends up holding X
   double* const ds = red + ns * MT * 256;       // [4 KT][16] real delta
   const int comp = blockIdx.y;
@@ -149,9 +206,15 @@ __global__ void __launch_bounds__(512, WPE) k_wall_mfma(
   // ---- GEMM A: partial inverse DFT at the 2d + C points, in two row groups
   // (fewer live accumulators), summed over the waves in a fixed order; then
   // slot 0 += slots 1..ns-1
+  const int nq = 2 * d + C;
+  if constexpr (TAB == 1) {
+    for (int e = threadIdx.x; e < n; e += blockDim.x) rl[e] = roots[e];
+    __syncthreads();
+  }
   constexpr int G = KBW >= 18 ? MT : 3;
-  gemm_a_rows<MT, 0, G, KBW>(winv, vr, b0, nbk, lane, wave, ns, red);
-  if constexpr (G < MT) gemm_a_rows<MT, G, MT, KBW>(winv, vr, b0, nbk, lane, wave, ns, red);
+  gemm_a_rows<MT, 0, G, KBW, TAB>(winv, rl, vr, b0, nbk, lane, wave, ns, red, n, np, d, nq);
+  if constexpr (G < MT)
+    gemm_a_rows<MT, G, MT, KBW, TAB>(winv, rl, vr, b0, nbk, lane, wave, ns, red, n, np, d, nq);
   __syncthreads();
   for (int e = threadIdx.x; e < MT * 256; e += blockDim.x) {
     double s = red[e];
@@ -224,13 +287,39 @@ __global__ void __launch_bounds__(512, WPE) k_wall_mfma(
   __syncthreads();
 
   // ---- GEMM C: Vr += Wf * delta, then store
+  if constexpr (TAB == 0) {
 #pragma unroll
-  for (int b = 0; b < KBW; ++b) {
-    const int bb = b0 + b;
-    if (bb < nbk) {
-      const double* wa = wfwd + size_t(bb) * KT * 64 + lane;
+    for (int b = 0; b < KBW; ++b) {
+      const int bb = b0 + b;
+      if (bb < nbk) {
+        const double* wa = wfwd + size_t(bb) * KT * 64 + lane;
 #pragma unroll
-      for (int t = 0; t < KT; ++t) vr[b] = mfma_f64(wa[t * 64], ds[(4 * t + g) * 16 + li], vr[b]);
+        for (int t = 0; t < KT; ++t)
+          vr[b] = mfma_f64(wa[t * 64], ds[(4 * t + g) * 16 + li], vr[b]);
+      }
+    }
+  } else {
+    // lane = i + 16k: row i = kappa offset 4(i&3) + (i>>2) of block bb
+    // (z = 8 bb + 2(i&3) + (i>>3), re/im (i>>2)&1); column 4t + k = (point
+    // p = 2t + k/2, re/im k&1); e^{-2 pi i z r_p / n}
+    const int zo = 2 * (li & 3) + (li >> 3), zc = (li >> 2) & 1, pc = g & 1;
+    const int nrho = 2 + C;
+    double bv[KT];
+#pragma unroll
+    for (int t = 0; t < KT; ++t) bv[t] = ds[(4 * t + g) * 16 + li];
+#pragma unroll
+    for (int b = 0; b < KBW; ++b) {
+      const int bb = b0 + b;
+      if (bb < nbk) {
+        const unsigned z = unsigned(8 * bb + zo);
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+          const unsigned r = unsigned(rpoint(2 * t + (g >> 1), np, nrho));
+          const Cd w = rl[modn(z * r, n, magic)];
+          const double a = zc == pc ? w.x : (zc ? -w.y : w.y);
+          vr[b] = mfma_f64(a, bv[t], vr[b]);
+        }
+      }
     }
   }
   if (okl) {
@@ -273,8 +362,10 @@ int build_wall_mfma(spx_plan* p) {
     const int v = std::atoi(e);
     if ((v == 9 || v == 18) && (nbk + v - 1) / v <= kMaxWaves) kbw = v;
   }
-  // register budget: 3 or 4 waves per SIMD for 9 blocks per wave, 2 for 18
-  int wpe = 3;
+  // register budget: 3 or 4 waves per SIMD for 9 blocks per wave (4: two
+  // 8-wave workgroups per CU overlap their load and matrix phases; HD 512^3:
+  // 3.64 ms per launch against 5.31 ms with 3), 2 for 18
+  int wpe = 4;
   if (const char* e = std::getenv("SPX_WALL_WPE")) {
     const int v = std::atoi(e);
     if (v == 3 || v == 4) wpe = v;
@@ -329,9 +420,17 @@ int build_wall_mfma(spx_plan* p) {
       for (int l = 0; l < 64; ++l)
         tc[(size_t(bb) * kt + t) * 64 + l] =
             wf(16 * bb + 4 * ((l & 15) & 3) + ((l & 15) >> 2), 4 * t + (l >> 4));
-  double** dst[2] = {&w.winv, &w.wfwd};
-  const std::vector<double>* src[2] = {&ta, &tc};
-  for (int i = 0; i < 2; ++i) {
+  std::vector<double> rt(2 * size_t(n));
+  for (int m = 0; m < n; ++m) {
+    rt[2 * m] = cr[m];
+    rt[2 * m + 1] = ci[m];
+  }
+  w.magic = unsigned((uint64_t(1) << 32) / uint64_t(n) + 1);
+  w.tab = 1;
+  if (const char* e = std::getenv("SPX_WALL_TAB")) w.tab = std::atoi(e) == 0 ? 0 : 1;
+  double** dst[3] = {&w.winv, &w.wfwd, &w.roots};
+  const std::vector<double>* src[3] = {&ta, &tc, &rt};
+  for (int i = 0; i < 3; ++i) {
     const size_t bytes = src[i]->size() * sizeof(double);
     void* dptr = nullptr;
     if (hipMalloc(&dptr, bytes) != hipSuccess) return fail(SPX_ERR_ALLOC, "wall correction tables");
@@ -345,11 +444,14 @@ int build_wall_mfma(spx_plan* p) {
 }
 
 #define SPX_WALL_LAUNCH(MT, KT, KBW, WPE)                                                     \
-  hipLaunchKernelGGL((k_wall_mfma<MT, KT, KBW, WPE>), grid, dim3(64 * w.nw), lds, p->stream,       \
+  if (w.tab) SPX_WALL_LAUNCH_T(MT, KT, KBW, WPE, 1) else SPX_WALL_LAUNCH_T(MT, KT, KBW, WPE, 0)
+#define SPX_WALL_LAUNCH_T(MT, KT, KBW, WPE, TAB)                                              \
+  hipLaunchKernelGGL((k_wall_mfma<MT, KT, KBW, WPE, TAB>), grid, dim3(64 * w.nw), lds, p->stream, \
                      static_cast<double*>(vx), static_cast<double*>(vy),                      \
                      static_cast<const Cd*>(pr), nlines, p->g.ny, p->g.nz, p->np, w.nbk,      \
                      1.0 / double(p->g.nz), wp, p->d_kx, p->d_ky, p->d_dir, p->g.Cz, p->g.oz, \
-                     w.winv, w.wfwd, r, rk ? 1 : 0)
+                     w.winv, w.wfwd, reinterpret_cast<const Cd*>(w.roots), w.magic, r,     \
+                     rk ? 1 : 0);
 
 int wall_mfma(spx_plan* p, void* vx, void* vy, const void* pr, const WallParams& wp, int ncomp,
               const RkLoad* rk) {
@@ -357,8 +459,9 @@ int wall_mfma(spx_plan* p, void* vx, void* vy, const void* pr, const WallParams&
   if (!w.on) return SPX_ERR_UNSUPPORTED;
   const long nlines = p->wlines();
   const dim3 grid(unsigned((nlines + 15) / 16), unsigned(ncomp));
-  const size_t lds =
-      (size_t(std::min(w.nw, 4)) * w.mt * 256 + size_t(w.kt) * 64) * sizeof(double);
+  const size_t lds = (size_t(std::min(w.nw, 4)) * w.mt * 256 + size_t(w.kt) * 64 +
+                      (w.tab ? 2 * size_t(p->g.nz) : 0)) *
+                     sizeof(double);
   RkLoad r;
   std::memset(&r, 0, sizeof(r));
   if (rk) r = *rk;

@@ -79,6 +79,25 @@ def gpu_trajectory(shape, prm, nsteps, seed=1000, diag_every=0, precision=0):
     return sol, np.array(series)
 
 
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def cfg1_longdouble(st):
+    """The longdouble trajectory of cfg1 after 100 steps, rebuilt from the
+    float64 oracle's final state st and the committed difference
+    (tests/golden/make_cfg1_longdouble.py)."""
+    z = np.load(os.path.join(GOLDEN, "cfg1_longdouble_delta.npz"))
+    out = {}
+    for n in ("vx", "vy", "vz"):
+        s = float(z[f"{n}_scale"])
+        dl = (z[f"{n}_re"].astype(np.float64) + 1j * z[f"{n}_im"].astype(np.float64)) * s
+        out[n] = getattr(st, n).astype(np.clongdouble) + dl.astype(np.clongdouble)
+        # the fixture matches this float64 trajectory
+        rel = float(np.linalg.norm(dl.ravel()) / np.linalg.norm(out[n].ravel()))
+        assert abs(rel / float(z[f"{n}_rel"]) - 1) < 1e-2, (n, rel, float(z[f"{n}_rel"]))
+    return out
+
+
 @pytest.fixture(scope="session")
 def cfg1_oracle():
     return oracle_trajectory(CFG1["shape"], CFG1["prm"], CFG1["nsteps"], CFG1["seed"],

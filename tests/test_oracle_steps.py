@@ -29,11 +29,12 @@ the extrapolation factor, the pressure's sign, the buoyancy and heat-current
 signs).
 """
 import math
+import os
 
 import numpy as np
 import pytest
 
-from conftest import TABLES
+from conftest import GOLDEN, TABLES
 from oracle import specter_ref as O
 from specter_amd import ic
 
@@ -413,3 +414,45 @@ def test_bouss_heat_current():
 def test_bouss_planted_heat_current_sign_fails():
     assert _heat_current(-1.0) == _heat_current(None)
     assert _heat_current(+1.0) > 1.0
+
+
+# ---------------------------------------------------------------------------
+# the longdouble oracle (the extended-precision referee of the cfg1 fixture)
+# ---------------------------------------------------------------------------
+def test_longdouble_oracle_is_the_same_algorithm():
+    """One HD RK2 step in longdouble and in float64 from the same fields:
+    they differ only by float64 roundoff (a transposed operand or a dropped
+    term in either would differ at O(1)); the longdouble run is closer to a
+    second longdouble run with a different FFT order of operations than to
+    float64 (it carries more digits)."""
+    nx, ny, nz, C, d = 16, 16, 41, 25, 5
+    g64 = grid(nx, ny, nz, C, d)
+    gld = O.make_grid(nx, ny, nz, C, d, TABLES, precision="longdouble")
+    assert gld.kx.dtype == np.longdouble and gld.dirm.dtype == np.longdouble
+    # constants are the float64 ones, held exactly
+    assert np.array_equal(gld.kk2.astype(np.float64), g64.kk2)
+    assert np.array_equal(gld.dirm.astype(np.float64), g64.dirm)
+    u = ic.spectrum_peaked_velocity(nx, ny, nz, C, 7)
+    out = {}
+    for g in (g64, gld):
+        zero = np.zeros(g.spec_shape, g.cdtype)
+        st = O.HDState(*(O.fftp3d_real_to_complex(g, a.astype(g.rdtype)) for a in u),
+                       zero.copy(), zero.copy(), zero.copy(), zero.copy())
+        O.hd_step(g, st, 2, 1e-3, 2e-3)
+        assert st.vx.dtype == g.cdtype
+        out[g.precision] = st
+    for n in ("vx", "vy", "vz"):
+        a = getattr(out["longdouble"], n)
+        b = getattr(out["float64"], n).astype(np.clongdouble)
+        rel = float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(a.ravel()))
+        assert 1e-17 < rel < 1e-12, (n, rel)
+
+
+def test_cfg1_longdouble_fixture():
+    """The committed difference between the longdouble and float64 cfg1
+    trajectories: finite, of the size the float64 drift has (2-5e-11)."""
+    z = np.load(os.path.join(GOLDEN, "cfg1_longdouble_delta.npz"))
+    for n in ("vx", "vy", "vz"):
+        assert np.all(np.isfinite(z[f"{n}_re"])) and np.all(np.isfinite(z[f"{n}_im"]))
+        assert 1e-11 < float(z[f"{n}_rel"]) < 1e-10
+        assert np.abs(z[f"{n}_re"]).max() <= 1.0 and np.abs(z[f"{n}_im"]).max() <= 1.0
