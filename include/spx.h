@@ -93,6 +93,18 @@ int spx_plan_create(const spx_grid* g, const double* A, const double* Q, void* s
 /* Replaces fcgram_destroy_plan (fftp/fcgram_mod.f90:147-177). */
 int spx_plan_destroy(spx_plan* p);
 int spx_plan_get_info(const spx_plan* p, spx_plan_info* info);
+/* Launch-shape overrides (no counterpart in the reference, whose FFTW plans
+ * choose their own; set by the autotuner specter_amd/autotune.py), applied
+ * to plans created afterwards in this process.  spx_tune_line: for line
+ * FFTs of length n with csize-byte complex elements (16 float64, 8 float32),
+ * lines per workgroup and threads per line (powers of two; 0 = default) and
+ * the Rader / Bluestein tables in global memory (1), LDS (0) or default (-1);
+ * values the engine cannot run (LDS, 512 threads, fewer threads than the
+ * length needs) are ignored.  spx_tune_wall: kappa blocks per wave (9, 18)
+ * and waves per SIMD (3, 4) of the wall kernel for lines of length n. */
+int spx_tune_line(int n, int csize, int lines, int threads, int tables_global);
+int spx_tune_wall(int n, int blocks_per_wave, int waves_per_simd);
+int spx_tune_clear(void);
 /* Block the host until the plan's stream is idle. */
 int spx_plan_sync(spx_plan* p);
 

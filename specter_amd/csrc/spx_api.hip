@@ -428,6 +428,30 @@ int spx_plan_get_info(const spx_plan* p, spx_plan_info* info) {
   return SPX_OK;
 }
 
+int spx_tune_line(int n, int csize, int lines, int threads, int tables_global) {
+  SPX_REQ(n >= 1 && (csize == 8 || csize == 16), "spx_tune_line: bad length or element size");
+  LineTune t;
+  t.lines = lines;
+  t.threads = threads;
+  t.gtab = tables_global;
+  set_line_tune(n, csize, t);
+  return SPX_OK;
+}
+
+int spx_tune_wall(int n, int blocks_per_wave, int waves_per_simd) {
+  SPX_REQ(n >= 1, "spx_tune_wall: bad length");
+  WallTune t;
+  t.kbw = blocks_per_wave;
+  t.wpe = waves_per_simd;
+  set_wall_tune(n, t);
+  return SPX_OK;
+}
+
+int spx_tune_clear(void) {
+  clear_tunes();
+  return SPX_OK;
+}
+
 int spx_plan_get_members(const spx_plan* p, int* nmembers, int* nactive) {
   SPX_REQ(p && nmembers && nactive, "spx_plan_get_members: null argument");
   *nmembers = p->nb;

@@ -310,6 +310,23 @@ int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, cons
                int ncomp = 1);
 // fused v_imposebc: goto_domain_w_boundaries + noslip_z + goto_3d_fourier on vx,vy
 // rk: the RK update of vx, vy (c[0], c[1]) is applied on load (null: none)
+// Launch-shape overrides set by an autotuner (spx_tune_*), consulted when a
+// plan builds its line FFTs and wall tables; 0 / -1 fields keep the default.
+struct LineTune {
+  int lines = 0;     // lines per workgroup (power of two)
+  int threads = 0;   // threads per line (power of two, >= the minimum for n)
+  int gtab = -1;     // Rader / Bluestein tables: 1 global memory, 0 LDS
+};
+struct WallTune {
+  int kbw = 0;       // kappa blocks per wave: 9 or 18
+  int wpe = 0;       // waves per SIMD of the register budget: 3 or 4 (kbw 9)
+};
+void set_line_tune(int n, int csize, const LineTune& t);
+bool get_line_tune(int n, int csize, LineTune* t);
+void set_wall_tune(int n, const WallTune& t);
+bool get_wall_tune(int n, WallTune* t);
+void clear_tunes();
+
 // wall update on the matrix cores (spx_wall.hip): tables at plan creation
 // (no-op when the (d, C) pair or nz is not covered), the launch returns
 // SPX_ERR_UNSUPPORTED when the plan has no tables

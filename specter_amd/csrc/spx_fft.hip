@@ -17,6 +17,8 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <unordered_map>
 
 #include "spx_engine.h"
@@ -1084,6 +1086,39 @@ static void dft_ld(const std::vector<long double>& xr, const std::vector<long do
   }
 }
 
+// autotuner overrides (process-wide, read at plan creation)
+static std::mutex g_tune_mu;
+static std::map<std::pair<int, int>, LineTune> g_line_tune;
+static std::map<int, WallTune> g_wall_tune;
+
+void set_line_tune(int n, int csize, const LineTune& t) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_line_tune[{n, csize}] = t;
+}
+bool get_line_tune(int n, int csize, LineTune* t) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_line_tune.find({n, csize});
+  if (it == g_line_tune.end()) return false;
+  *t = it->second;
+  return true;
+}
+void set_wall_tune(int n, const WallTune& t) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_wall_tune[n] = t;
+}
+bool get_wall_tune(int n, WallTune* t) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_wall_tune.find(n);
+  if (it == g_wall_tune.end()) return false;
+  *t = it->second;
+  return true;
+}
+void clear_tunes() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_line_tune.clear();
+  g_wall_tune.clear();
+}
+
 int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
   std::vector<int> fac;
   f.n = n;
@@ -1111,6 +1146,11 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
   // at least 2 threads per line: the wall kernels handle z = 0 and z = Lz on
   // lanes 0 and 1
   f.tpl = std::max(2, tpl_for(f.M, n, fac));
+  LineTune tune;
+  const bool tuned = get_line_tune(n, csize, &tune);
+  auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+  if (tuned && pow2(tune.threads) && tune.threads >= f.tpl && tune.threads <= kMaxBlock)
+    f.tpl = tune.threads;
   if (f.tpl > kMaxBlock)
     return fail(SPX_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " exceeds the LDS engine");
   f.lgtpl = 0;
@@ -1137,7 +1177,14 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
       return std::min<size_t>(4, lds_cu / ((tab + lines) * size_t(csize)));
     };
     if (wgs(size_t(f.M)) > wgs(size_t(f.tab))) f.tab = f.M;
+    if (tuned && tune.gtab == 1) f.tab = f.M;
+    if (tuned && tune.gtab == 0 && f.tab == f.M)
+      f.tab = f.mode == 1 ? 2 * f.M + n
+                          : 2 * f.M + int((2 * size_t(f.M) * sizeof(int) + csize - 1) / csize);
   }
+  if (tuned && pow2(tune.lines) && tune.lines * f.tpl <= kMaxBlock &&
+      (size_t(f.tab) + size_t(tune.lines) * f.ldl) * csize <= 160 * 1024)
+    nl = tune.lines;
   if ((size_t(f.tab) + size_t(nl) * f.ldl) * csize > 160 * 1024)
     return fail(SPX_ERR_UNSUPPORTED, "FFT line does not fit in LDS");
   f.nl = nl;

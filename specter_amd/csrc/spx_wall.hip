@@ -358,6 +358,10 @@ int build_wall_mfma(spx_plan* p) {
   int kbw = 9;
   if ((nbk + kbw - 1) / kbw > kMaxWaves) kbw = 18;
   if ((nbk + kbw - 1) / kbw > kMaxWaves) return SPX_OK;
+  WallTune tune;
+  const bool tuned = get_wall_tune(n, &tune);
+  if (tuned && (tune.kbw == 9 || tune.kbw == 18) && (nbk + tune.kbw - 1) / tune.kbw <= kMaxWaves)
+    kbw = tune.kbw;
   if (const char* e = std::getenv("SPX_WALL_KBW")) {
     const int v = std::atoi(e);
     if ((v == 9 || v == 18) && (nbk + v - 1) / v <= kMaxWaves) kbw = v;
@@ -366,6 +370,7 @@ int build_wall_mfma(spx_plan* p) {
   // 8-wave workgroups per CU overlap their load and matrix phases; HD 512^3:
   // 3.64 ms per launch against 5.31 ms with 3), 2 for 18
   int wpe = 4;
+  if (tuned && (tune.wpe == 3 || tune.wpe == 4)) wpe = tune.wpe;
   if (const char* e = std::getenv("SPX_WALL_WPE")) {
     const int v = std::atoi(e);
     if (v == 3 || v == 4) wpe = v;

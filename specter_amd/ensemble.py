@@ -24,6 +24,18 @@ A member of a batched plan is computed by exactly the per-line / per-row
 code of a single run, so a result does not depend on what it was batched
 with.  Time to result is measured from a request's arrival to the moment its
 fields are on the host.
+
+Policies:
+* precision: a request without one gets the cheapest precision that meets a
+  1e-5 field error budget against float64 (default_precision: float32-mixed
+  for d = 5 on grids of 128^3 and more, float64 otherwise; the d = 8
+  continuation multiplies float32 noise by |A Q^T| = 2.2e6);
+* slots: a class gets at most max(1, fill_points / points per run) members
+  (a 192^3 run already fills the GPU; batching more of them only slows each
+  one down), and waiting requests join in arrival order;
+* failures: after every chunk the active members' energies are checked; a
+  member whose fields are no longer finite is returned with status
+  "failed" and removed, without touching the other members.
 """
 from __future__ import annotations
 
@@ -51,7 +63,7 @@ class RunRequest:
     arrival: float = 0.0       # seconds after the start of the stream
     ord: int = 2
     dt: float = 1e-3
-    precision: int = _l.SPX_FP32_MIXED
+    precision: int | None = None   # None: chosen by the server's policy
 
     @property
     def grid(self):
@@ -76,6 +88,10 @@ class RunResult:
     finished: float
     fields: dict = field(default_factory=dict)     # spectral vx, vy, vz (float64)
     diagnostics: tuple = ()                          # energy, enstrophy, <div^2>
+    status: str = "ok"                               # "ok" or "failed"
+    error: str = ""
+    precision: int = _l.SPX_FP32_MIXED
+    steps_done: int = 0
 
     @property
     def latency(self) -> float:
@@ -90,6 +106,15 @@ def state_bytes(plan: FCPlan, members: int | None = None) -> int:
     if members is not None and members != plan.members:
         raise ValueError("state_bytes: members must match the plan")
     return int(info.device_bytes + info.work_bytes + info.state_bytes)
+
+
+def default_precision(req: RunRequest) -> int:
+    """The cheapest precision that meets a 1e-5 field error budget against
+    float64 after the ensemble's 300 RK2 steps, from measured errors of the
+    mixed mode: d = 5 on 128^3 and 192^3 grids 8e-7 and 1.6e-6 (mixed);
+    d = 5 on 32^3 and 64^3 grids 3.8e-5 and 1.7e-5, d = 8 5e-4 after only
+    10 steps at 32^3 (float64)."""
+    return _l.SPX_FP32_MIXED if (req.d <= 5 and req.n >= 128) else _l.SPX_FP64
 
 
 def member_bytes(req: RunRequest) -> int:
@@ -125,6 +150,7 @@ class _Class:
         self.event = torch.cuda.Event()
         self.t_launch = 0.0
         self.rate = None           # measured RK steps per second of this class
+        self.sample_rids: set = set()   # runs whose final fields are returned
 
     @property
     def free(self) -> int:
@@ -139,7 +165,7 @@ class _Class:
             # generated on the class's stream: the host only enqueues work
             u = ic.spectrum_peaked_velocity_device(nx, ny, nz, C, req.seed)
             for name in ("pr", "fx", "fy", "fz"):
-                self.sol.field(name)[b].zero_()
+                self.sol.member_field(name, b).zero_()
             for name, arr in zip(("vx", "vy", "vz"), u):
                 self.sol.set_real_field(name, arr, member=b)
             self.sol.set_member(b, req.nu)
@@ -172,26 +198,37 @@ class _Class:
         m = self.stepping_members
         self.done = [s + k if b < m else s for b, s in enumerate(self.done)]
         out = []
+        vx, vy, vz = (self.sol.field(n) for n in ("vx", "vy", "vz"))
+        p = self.plan
+        # per-member health: a run whose fields stopped being finite fails alone
+        eng = np.atleast_1d(p.energy(vx, vy, vz, 1))
         b = 0
         while b < len(self.runs):
             r = self.runs[b]
-            if self.done[b] < r.nsteps:
+            bad = not np.isfinite(eng[b])
+            if self.done[b] < r.nsteps and not bad:
                 b += 1
                 continue
-            res = RunResult(r.rid, r.arrival, self.admitted[b], 0.0)
-            if keep_fields:
-                res.fields = {n: self.sol.spectral(n, member=b) for n in ("vx", "vy", "vz")}
-            vx, vy, vz = (self.sol.field(n) for n in ("vx", "vy", "vz"))
-            p = self.plan
-            res.diagnostics = (p.energy(vx, vy, vz, 1)[b], p.energy(vx, vy, vz, 0)[b],
-                               p.divergence(vx, vy, vz)[b])
+            res = RunResult(r.rid, r.arrival, self.admitted[b], 0.0, precision=r.precision,
+                            steps_done=self.done[b])
+            if bad:
+                res.status = "failed"
+                res.error = f"non-finite fields after {self.done[b]} steps"
+            else:
+                if keep_fields or r.rid in self.sample_rids:
+                    res.fields = {n: self.sol.spectral(n, member=b) for n in ("vx", "vy", "vz")}
+                res.diagnostics = (float(eng[b]),
+                                   float(np.atleast_1d(p.energy(vx, vy, vz, 0))[b]),
+                                   float(np.atleast_1d(p.divergence(vx, vy, vz))[b]))
             res.finished = now_fn()
             out.append(res)
             last = len(self.runs) - 1
             if b != last:
+                eng[b] = eng[last]
                 self.sol.copy_member(b, last)
                 self.runs[b], self.done[b], self.admitted[b] = (self.runs[last], self.done[last],
                                                                 self.admitted[last])
+            eng = eng[:last]
             self.runs.pop()
             self.done.pop()
             self.admitted.pop()
@@ -212,8 +249,15 @@ class EnsembleServer:
     class; chunk: RK steps enqueued per class between scheduling points."""
 
     def __init__(self, hbm_budget: int | None = None, max_slots: int = 16, chunk: int = 5,
-                 keep_fields: bool = False, chunk_seconds: float = 0.05, max_chunk: int = 100):
+                 keep_fields: bool = False, chunk_seconds: float = 0.05, max_chunk: int = 100,
+                 fill_points: int = 1 << 23, precision_policy=default_precision,
+                 sample_per_class: bool = False, tuning: str | None = "default"):
         import torch
+        from . import autotune
+        # launch shapes chosen by the autotuner (specter_amd/tuning.json when
+        # present), registered before any class creates its plan
+        self.tuned = autotune.apply_tuning(autotune.DEFAULT_PATH if tuning == "default"
+                                           else tuning)
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = int(free * 0.85)
@@ -223,16 +267,24 @@ class EnsembleServer:
         self.chunk_seconds = chunk_seconds
         self.max_chunk = max_chunk
         self.keep_fields = keep_fields
+        self.fill_points = fill_points
+        self.precision_policy = precision_policy
+        self.sample_per_class = sample_per_class   # keep the first run's fields of each class
+        self.sampled: dict[tuple, int] = {}
+        self._class_bytes: dict[tuple, int] = {}   # measured footprint per class key
         self.classes: dict[tuple, _Class] = {}
         self.used = 0
         self.peak_used = 0
         self.peak_classes = 0
 
     def _slots_for(self, req: RunRequest) -> int:
-        """Member slots of a new class: max_slots, or what the budget has room for."""
+        """Member slots of a new class: max_slots, at most the runs that fill
+        the GPU (fill_points / points per run, at least 1), and what the budget
+        has room for."""
         per = member_bytes(req)
         room = (self.budget - self.used) // max(per, 1)
-        return int(max(0, min(self.max_slots, room)))
+        cap = max(1, self.fill_points // max(req.points, 1))
+        return int(max(0, min(self.max_slots, cap, room)))
 
     def _evict_idle(self, keep=None):
         """Destroy classes without runs (their graphs and memory go back)."""
@@ -250,8 +302,15 @@ class EnsembleServer:
                 slots = self._slots_for(req)
             if slots < 1:
                 return False
+            need = self._class_bytes.get((req.key, slots))
+            if need is not None and self.used + need > self.budget:
+                self._evict_idle()
+                if self.used + need > self.budget:
+                    return False
             cls = _Class(req, slots)
-            if self.used + cls.bytes > self.budget and self.classes:
+            self._class_bytes[(req.key, slots)] = cls.bytes
+            if self.used + cls.bytes > self.budget:
+                # wait for memory (or, with nothing left to free, fail the stream)
                 cls.close()
                 return False
             self.classes[req.key] = cls
@@ -260,6 +319,9 @@ class EnsembleServer:
             self.peak_classes = max(self.peak_classes, len(self.classes))
         if cls.free < 1:
             return False
+        if self.sample_per_class and req.key not in self.sampled:
+            self.sampled[req.key] = req.rid
+            cls.sample_rids.add(req.rid)
         cls.admit(req, now)
         return True
 
@@ -273,6 +335,9 @@ class EnsembleServer:
         with realtime=False every request is available at once."""
         import torch
         torch.cuda.synchronize()
+        for r in requests:
+            if r.precision is None:
+                r.precision = self.precision_policy(r)
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         waiting: list[RunRequest] = []
         results: list[RunResult] = []
@@ -304,8 +369,9 @@ class EnsembleServer:
                 else:
                     still.append(r)
             waiting = still
-            # idle classes with runs start their next chunk
-            for c in self.classes.values():
+            # idle classes with runs start their next chunk, oldest request first
+            for c in sorted(self.classes.values(),
+                            key=lambda c: min((r.arrival for r in c.runs), default=0.0)):
                 if c.runs and not c.stepping:
                     c.launch(self._chunk_for(c), now())
                     progressed = True
@@ -323,7 +389,7 @@ class EnsembleServer:
 
 def poisson_stream(nruns: int = 96, rate: float = 4.0, seed: int = 2026, nsteps: int = 300,
                    sizes=(32, 64, 128, 192), orders=(5, 8), nu_range=(5e-4, 5e-3),
-                   precision: int = _l.SPX_FP32_MIXED) -> list[RunRequest]:
+                   precision: int | None = None) -> list[RunRequest]:
     """The ensemble workload: Poisson arrivals at `rate` per second, grids
     uniform over `sizes`, nu log-uniform over nu_range, FC order uniform over
     `orders`, seeds 1..nruns, `nsteps` RK2 steps each."""

@@ -87,6 +87,9 @@ _SIGS = {
     "spx_plan_create_batch": [POINTER(SpxGrid), POINTER(c_double), POINTER(c_double), c_int, _P,
                               POINTER(_P)],
     "spx_plan_get_members": [_P, POINTER(c_int), POINTER(c_int)],
+    "spx_tune_line": [c_int, c_int, c_int, c_int, c_int],
+    "spx_tune_wall": [c_int, c_int, c_int],
+    "spx_tune_clear": [],
     "spx_plan_set_active": [_P, c_int],
     "spx_state_set_member": [_P, c_int, c_double, c_double],
     "spx_state_copy_member": [_P, c_int, c_int],

@@ -102,11 +102,25 @@ class Solver:
             r = real.to(device="cuda", dtype=torch.float64).contiguous()
         else:
             r = torch.as_tensor(np.ascontiguousarray(real, dtype=np.float64)).cuda()
+        member = self._single_member(member)
         if member is None:
             self.plan.fftp3d_real_to_complex(r, self.field(name))
         else:
             self._member_plan().fftp3d_real_to_complex(r, self.field(name)[member])
         self.plan.sync()
+
+    def _single_member(self, member):
+        """member of a one-member plan: the whole field (member must be 0)."""
+        if member is not None and self.plan.members == 1:
+            if member != 0:
+                raise ValueError("member out of range for a one-member plan")
+            return None
+        return member
+
+    def member_field(self, name: str, member: int):
+        """Zero-copy view of one member's spectral field."""
+        f = self.field(name)
+        return f if self._single_member(member) is None else f[member]
 
     def set_member(self, member: int, nu: float, kappa: float = 0.0):
         """Diffusivities of one member of a batched state."""
@@ -171,6 +185,7 @@ class Solver:
         """Physical-space field (c2r / (nx ny nz)), physical planes only."""
         p = self.plan
         src = self.field(name)
+        member = self._single_member(member)
         if member is not None:
             src, p = src[member], self._member_plan()
         elif p.members > 1:
@@ -183,6 +198,7 @@ class Solver:
 
     def spectral(self, name: str, member: int | None = None) -> np.ndarray:
         t = self.field(name)
+        member = self._single_member(member)
         if member is not None:
             t = t[member]
         self.plan.sync()

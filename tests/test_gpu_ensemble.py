@@ -42,3 +42,44 @@ def test_results_equal_alone_and_in_the_mix(precision):
     st = latency_stats(results)
     print(st)
     assert 0 < st["p50_s"] <= st["p99_s"]
+
+
+def test_failed_run_is_reported_and_leaves_its_batch_alone():
+    """A run whose fields go non-finite (here nu = NaN) is returned with
+    status "failed" after its first chunk; the runs batched with it finish
+    with the same bits as alone."""
+    from specter_amd.ensemble import EnsembleServer, RunRequest
+    reqs = [RunRequest(rid=0, n=16, d=5, nu=2e-3, seed=11, nsteps=20, precision=1),
+            RunRequest(rid=1, n=16, d=5, nu=float("nan"), seed=12, nsteps=20, precision=1),
+            RunRequest(rid=2, n=16, d=5, nu=3e-3, seed=13, nsteps=20, precision=1)]
+    srv = EnsembleServer(max_slots=3, chunk=5, keep_fields=True)
+    results = {r.rid: r for r in srv.run(reqs, realtime=False)}
+    assert results[1].status == "failed" and "non-finite" in results[1].error
+    assert results[1].steps_done < 20
+    for rid in (0, 2):
+        assert results[rid].status == "ok"
+        ref = _alone(reqs[rid])
+        for name in ("vx", "vy", "vz"):
+            assert np.array_equal(results[rid].fields[name], ref[name]), (rid, name)
+
+
+def test_admission_within_the_hbm_budget():
+    """Requests wait for memory: with room for one class at a time two grids
+    are served one after the other; a budget no class fits is an error."""
+    from specter_amd.ensemble import EnsembleServer, RunRequest, _Class
+    reqs = [RunRequest(rid=0, n=32, d=5, nu=2e-3, seed=1, nsteps=6, precision=0),
+            RunRequest(rid=1, n=16, d=5, nu=2e-3, seed=2, nsteps=6, precision=0)]
+    need = []
+    for r in reqs:
+        c = _Class(r, 1)
+        need.append(c.bytes)
+        c.close()
+    one = max(need) + min(need) // 2      # room for either class, not both
+    srv = EnsembleServer(hbm_budget=one, max_slots=1, chunk=3)
+    res = srv.run(reqs, realtime=False)
+    assert sorted(r.rid for r in res) == [0, 1] and all(r.status == "ok" for r in res)
+    assert srv.peak_classes == 1 and srv.peak_used <= one
+    with pytest.raises(RuntimeError, match="HBM budget"):
+        EnsembleServer(hbm_budget=1 << 20, max_slots=1).run(
+            [RunRequest(rid=0, n=32, d=5, nu=2e-3, seed=1, nsteps=2, precision=0)],
+            realtime=False)
