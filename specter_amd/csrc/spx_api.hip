@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -257,6 +258,7 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
   p->nb = p->nact = nmembers;
   p->mem = 1;
   p->stream = static_cast<hipStream_t>(stream);
+  if (const char* e = std::getenv("SPX_FUSED_PROJECT")) p->fused_project = std::atoi(e) != 0;
   if (hipGetDevice(&p->device) != hipSuccess) {
     delete p;
     return fail(SPX_ERR_HIP, "spx_plan_create: hipGetDevice failed");
@@ -683,7 +685,9 @@ int spx_sol_project(spx_plan* p, void* a, void* b, void* c, void* d, int bctarge
     return fail(SPX_ERR_UNSUPPORTED,
                 "sol_project: only bctarget=1 with homogeneous Dirichlet walls is implemented");
   MemberView mv(p, p->nact);
-  int rc;
+  int rc = project_harmonic(p, a, b, c, d, nullptr, false);
+  if (rc == SPX_OK) return proj_pressure(p, d);
+  if (rc != SPX_ERR_UNSUPPORTED) return rc;
   if ((rc = op_poisson_project_bc(p, a, b, c, d))) return rc;
   if ((rc = proj_pressure(p, d))) return rc;
   return proj_harmonic(p, a, b, c);
@@ -1097,6 +1101,10 @@ static int substep_impl(spx_plan* p, spx_state* s, int o, bool full_pressure) {
     if ((rc = noslip_fused(p, s->vx, s->vy, s->pr, tau, q.v_zsta, q.v_zend, &rk))) return rc;
     RkLoad rz = rk;
     rz.c[0] = RkLoadComp{c0[2], nl[2], f[2], s->d_nu};
+    rc = project_harmonic(p, s->vx, s->vy, s->vz, s->pr, &rz, !full_pressure);
+    if (rc == SPX_OK)
+      return full_pressure ? proj_pressure(p, s->pr) : pressure_walls(p, s->pr);
+    if (rc != SPX_ERR_UNSUPPORTED) return rc;
     if ((rc = op_poisson_project_bc(p, s->vx, s->vy, s->vz, s->pr, &rz, !full_pressure)))
       return rc;
     if ((rc = full_pressure ? proj_pressure(p, s->pr) : pressure_walls(p, s->pr))) return rc;
@@ -1108,10 +1116,16 @@ static int substep_impl(spx_plan* p, spx_state* s, int o, bool full_pressure) {
                          p->fp32)))
     return rc;
   if ((rc = noslip_fused(p, s->vx, s->vy, s->pr, tau, q.v_zsta, q.v_zend, &rk))) return rc;
-  if ((rc = op_poisson_project_bc(p, s->vx, s->vy, s->vz, s->pr, nullptr, !full_pressure)))
-    return rc;
-  if ((rc = full_pressure ? proj_pressure(p, s->pr) : pressure_walls(p, s->pr))) return rc;
-  if ((rc = proj_harmonic(p, s->vx, s->vy, s->vz))) return rc;
+  rc = project_harmonic(p, s->vx, s->vy, s->vz, s->pr, nullptr, !full_pressure);
+  if (rc == SPX_ERR_UNSUPPORTED) {
+    if ((rc = op_poisson_project_bc(p, s->vx, s->vy, s->vz, s->pr, nullptr, !full_pressure)))
+      return rc;
+    if ((rc = full_pressure ? proj_pressure(p, s->pr) : pressure_walls(p, s->pr))) return rc;
+    rc = proj_harmonic(p, s->vx, s->vy, s->vz);
+  } else if (rc == SPX_OK) {
+    rc = full_pressure ? proj_pressure(p, s->pr) : pressure_walls(p, s->pr);
+  }
+  if (rc) return rc;
   if ((rc = spx_s_imposebc(p, s->th))) return rc;
   if ((rc = op_filter(p, s->th))) return rc;
   // round trip of th through real space (include/bouss/bouss_rkstep2.f90:56-59)

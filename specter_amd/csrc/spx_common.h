@@ -135,6 +135,7 @@ struct spx_plan {
   // x passes see mem times as many planes.
   int nb = 1, nact = 1, mem = 1;
   int device = 0;           // HIP device current at creation
+  bool fused_project = true;  // sol_project in one pass per line (SPX_FUSED_PROJECT=0: off)
   int wkx() const { return nkx * mem; }
   long wlines() const { return long(g.ny) * nkx * mem; }
   size_t wnc() const { return nc * size_t(mem); }
@@ -343,6 +344,10 @@ int proj_pressure(spx_plan* p, void* d);             // d <- IFFT_z(d)/nz + harm
 // the potential): all the next stage's wall update reads
 int pressure_walls(spx_plan* p, void* d);
 int proj_harmonic(spx_plan* p, void* a, void* b, void* c);  // v -= grad(harmonic)
+// op_poisson_project_bc + proj_harmonic in one pass per line (plans with the
+// harmonic basis table and nz <= 1088; SPX_ERR_UNSUPPORTED otherwise)
+int project_harmonic(spx_plan* p, void* a, void* b, void* c, void* d, const RkLoad* rk,
+                     bool walls_only);
 int build_harmonic_table(spx_plan* p);  // p->d_htab / d_hidx (kept null on allocation failure)
 // z IFFT of a linear combination of derivatives, reduced per line over physical z
 // mode 0: sum |u|^2 ; the combination is acc = m0(in0) (+/-) m1(in1) (+/-) m2(in2)

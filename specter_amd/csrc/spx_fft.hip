@@ -853,6 +853,119 @@ __global__ void __launch_bounds__(256) k_proj_harmonic_tab(cpx<TS>* __restrict__
   }
 }
 
+// sol_project in one pass per line (boundary_mod.fpp:197-402 with
+// poisson_inhomogeneous :405-448 and laplace_z :451-678), one wavefront per
+// (kx, ky) line holding its KPL points per lane in registers:
+//   d = -i(kx a + ky b + kz c)/kz2 (d(0,0,0) = 0), v -= i k d   (first projection)
+//   bc = IFFT_z(c/nz) at z = 0 and z = Lz as two DFT sums        (wall data)
+//   A = c1 P1 + c2 P2, B = kh (c1 P1 - c2 P2) from bc and the    (harmonic correction
+//   basis table; v -= (i kx A, i ky A, B)                           from the table)
+// The arithmetic of every value is that of k_poisson_project_bc followed by
+// k_proj_harmonic_tab; the projected velocity never goes through HBM.
+// vz takes the RK update on its load when rk_on; pw (when non-null) gets
+// IFFT_z(d)/nz at the two walls and d is then not written there.
+template <int KPL>
+__global__ void __launch_bounds__(256) k_project_harmonic(
+    Cd* __restrict__ a, Cd* __restrict__ b, Cd* __restrict__ c, Cd* __restrict__ d,
+    Cd* __restrict__ bc, long nlines, int nz, int ny, const double* __restrict__ kx,
+    const double* __restrict__ ky, const double* __restrict__ kz, const Cd* __restrict__ wtop,
+    double tmp, RkLoad rk, int rk_on, Cd* __restrict__ pw, int np, const Cd* __restrict__ tab,
+    const int* __restrict__ hidx, double Lz, const double* __restrict__ khom) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const long ln = long(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (ln >= nlines) return;  // uniform per wavefront
+  const long i = ln / ny;
+  const int j = int(ln - i * ny);
+  const double kxi = kx[i], kyj = ky[j];
+  Cd s0 = mkc<double>(0, 0), s1 = mkc<double>(0, 0);
+  Cd q0 = mkc<double>(0, 0), q1 = mkc<double>(0, 0);
+  Cd pa[KPL], pb[KPL], pc[KPL];
+#pragma unroll
+  for (int q = 0; q < KPL; ++q) {
+    const int k = lane + 64 * q;
+    if (k < nz) {
+      const long e = ln * nz + k;
+      const double kzk = kz[k];
+      const double kk2 = (kxi * kxi + kyj * kyj) + kzk * kzk;
+      const Cd A = a[e], B = b[e];
+      Cd Cc = c[e];
+      if (rk_on) Cc = rk_on_load(rk, 0, Cc, e, i, j, k);
+      Cd sm = cadd(cscale(A, kxi), cscale(B, kyj));
+      sm = cadd(sm, cscale(Cc, kzk));
+      Cd dd;
+      if (kk2 == 0.0) {  // the (0,0,0) mode
+        dd = mkc<double>(0, 0);
+      } else {
+        dd = mkc<double>(sm.y / kk2, -sm.x / kk2);
+      }
+      if (pw) {
+        q0 = cadd(q0, dd);
+        q1 = cadd(q1, cmul(dd, wtop[k]));
+        if (k != 0 && k != np - 1) d[e] = dd;
+      } else {
+        d[e] = dd;
+      }
+      pa[q] = csub(A, ikmul(kxi, dd));
+      pb[q] = csub(B, ikmul(kyj, dd));
+      pc[q] = csub(Cc, ikmul(kzk, dd));
+      const Cd c1 = cscale(pc[q], tmp);
+      s0 = cadd(s0, c1);
+      s1 = cadd(s1, cmul(c1, wtop[k]));
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s0.x += __shfl_xor(s0.x, off);
+    s0.y += __shfl_xor(s0.y, off);
+    s1.x += __shfl_xor(s1.x, off);
+    s1.y += __shfl_xor(s1.y, off);
+  }
+  if (lane == 0) {
+    bc[2 * ln] = s0;
+    bc[2 * ln + 1] = s1;
+  }
+  if (pw) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      q0.x += __shfl_xor(q0.x, off);
+      q0.y += __shfl_xor(q0.y, off);
+      q1.x += __shfl_xor(q1.x, off);
+      q1.y += __shfl_xor(q1.y, off);
+    }
+    if (lane == 0) {
+      pw[2 * ln] = cscale(q0, tmp);
+      pw[2 * ln + 1] = cscale(q1, tmp);
+    }
+  }
+  // harmonic correction (the k_proj_harmonic_tab arithmetic)
+  const double kh = khom[ln];
+  const bool zm = kh == 0.0;
+  const Coef cf = neumann_coef(s0, s1, kh, Lz, zm);
+  const Cd* P1 = tab + 2 * long(hidx[ln]) * nz;
+  const Cd* P2 = P1 + nz;
+#pragma unroll
+  for (int q = 0; q < KPL; ++q) {
+    const int k = lane + 64 * q;
+    if (k < nz) {
+      const Cd p1 = P1[k], p2 = P2[k];
+      Cd A, B;
+      if (zm) {
+        A = cadd(cscale(p1, cf.c1.x), cscale(p2, cf.c2.x));
+        B = cscale(p2, cf.c1.x);
+      } else {
+        const Cd u1 = cmul(cf.c1, p1), u2 = cmul(cf.c2, p2);
+        A = cadd(u1, u2);
+        B = cscale(csub(u1, u2), kh);
+      }
+      const long o = ln * nz + k;
+      a[o] = csub(pa[q], ikmul(kxi, A));
+      b[o] = csub(pb[q], ikmul(kyj, A));
+      c[o] = csub(pc[q], B);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // diagnostics: per-line float64 reductions over physical z in the mixed domain
 // ---------------------------------------------------------------------------
@@ -1678,6 +1791,34 @@ int build_harmonic_table(spx_plan* p) {
   p->d_htab = tab;
   p->d_hidx = static_cast<int*>(hid);
   p->nhtab = U;
+  return SPX_OK;
+}
+
+int project_harmonic(spx_plan* p, void* a, void* b, void* c, void* d, const RkLoad* rk,
+                     bool walls_only) {
+  const int nz = p->g.nz;
+  if (!p->fused_project || !p->d_htab || !p->d_wtop || nz > 64 * 17) return SPX_ERR_UNSUPPORTED;
+  const long nlines = p->wlines();
+  RkLoad r;
+  std::memset(&r, 0, sizeof(r));
+  if (rk) r = *rk;
+  const double extra = rk ? (rk->nl32 ? 8.0 : 16.0) + 32.0 : 0.0;
+  ProfScope ps_(p, rk ? "project_harmonic_rk" : "project_harmonic",
+                double(p->wnc()) * (7.0 * 16.0 + extra) + 2.0 * p->nhtab * double(nz) * 16.0);
+  const dim3 grid(unsigned((nlines + 3) / 4)), block(256);
+#define SPX_PH(KPL)                                                                            \
+  hipLaunchKernelGGL(k_project_harmonic<KPL>, grid, block, 0, p->stream, C_<double>(a),        \
+                     C_<double>(b), C_<double>(c), C_<double>(d), C_<double>(p->bc), nlines,   \
+                     nz, p->g.ny, p->d_kx, p->d_ky, p->d_kz,                                   \
+                     static_cast<const Cd*>(p->d_wtop), 1.0 / double(nz), r, rk ? 1 : 0,       \
+                     walls_only ? C_<double>(p->pw) : nullptr, p->np,                          \
+                     static_cast<const Cd*>(p->d_htab), p->d_hidx, p->g.Lz, p->d_khom)
+  if (nz <= 64 * 3) SPX_PH(3);
+  else if (nz <= 64 * 5) SPX_PH(5);
+  else if (nz <= 64 * 9) SPX_PH(9);
+  else SPX_PH(17);
+#undef SPX_PH
+  SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
 

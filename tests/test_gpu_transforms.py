@@ -248,3 +248,29 @@ def test_wall_update_mfma_matches_transform_path(shape, o, ordr, monkeypatch):
     q.s_imposebc(t2)
     assert rel_err(t1.cpu().numpy(), rs) < 1e-12
     assert rel_err(t1.cpu().numpy(), t2.cpu().numpy()) < 1e-12
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_fused_projection_matches_two_kernel_path(precision, monkeypatch):
+    """sol_project in one pass per line (the default) gives the same bits as
+    the Poisson-projection kernel followed by the harmonic-correction kernel
+    (SPX_FUSED_PROJECT=0 at plan creation): two full RK4 steps, HD and
+    Boussinesq."""
+    from specter_amd import FCPlan, Solver, SolverParams, ic
+    nx, ny, nz, C, d = 32, 24, 57, 25, 5
+    u = ic.spectrum_peaked_velocity(nx, ny, nz, C, 21)
+    th = ic.thermal_perturbation(nx, ny, nz, C, 22)
+    for prm in (dict(solver="HD", ord=4, dt=1e-3, nu=2e-3),
+                dict(solver="BOUSS", ord=4, dt=1e-3, nu=2e-3, kappa=2e-3, xmom=1.0, xtemp=1.0)):
+        out = []
+        for fused in ("1", "0"):
+            monkeypatch.setenv("SPX_FUSED_PROJECT", fused)
+            sol = Solver(FCPlan(nx, ny, nz, C, d, precision=precision), SolverParams(**prm))
+            for name, arr in zip(("vx", "vy", "vz"), u):
+                sol.set_real_field(name, arr)
+            if prm["solver"] == "BOUSS":
+                sol.set_real_field("th", th)
+            sol.step(2)
+            out.append({n: sol.spectral(n) for n in ("vx", "vy", "vz", "pr")})
+        for n in ("vx", "vy", "vz", "pr"):
+            assert np.array_equal(out[0][n], out[1][n]), (prm["solver"], n)
