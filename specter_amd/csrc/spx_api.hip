@@ -340,6 +340,7 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
   if ((rc = build_line_fft(p->fy, ny, 16, 16))) return bail(rc);
   if ((rc = build_line_fft(p->fz, nz, 16, 16))) return bail(rc);
   p->fz2 = p->fz.with_lines(p->fz.nl / 2);
+  if ((rc = build_wave_fft(p->fzw, nz, 16))) return bail(rc);
   if (C > 0) {
     // exp(+2 pi i k (np-1) / nz): the top-wall value of a z line as one DFT sum
     std::vector<double> w(size_t(nz) * 2);
@@ -358,6 +359,7 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
     if ((rc = build_line_fft(p->fx32, nx, 8, 0))) return bail(rc);
     if ((rc = build_line_fft(p->fy32, ny, 8, 16))) return bail(rc);
     if ((rc = build_line_fft(p->fz32, nz, 8, 16))) return bail(rc);
+    if ((rc = build_wave_fft(p->fzw32, nz, 8))) return bail(rc);
     if (C > 0) {
       if (d > 8) return bail(fail(SPX_ERR_UNSUPPORTED, "mixed plans: matching order above 8"));
       // exp(+2 pi i k z / nz) for z = 1 (first nz entries) and z = np - d
@@ -398,6 +400,8 @@ int spx_plan_destroy(spx_plan* p) {
   free_line_fft(p->fx32);
   free_line_fft(p->fy32);
   free_line_fft(p->fz32);
+  free_line_fft(p->fzw);
+  free_line_fft(p->fzw32);
   for (hipEvent_t e : p->prof.pool) (void)hipEventDestroy(e);
   for (void* q : p->owned) (void)hipFree(q);
   delete p;
@@ -415,7 +419,8 @@ int spx_plan_get_info(const spx_plan* p, spx_plan_info* info) {
   info->z_fft_len = p->fz.M;
   info->z_algorithm = p->fz.mode;
   size_t tab = 0;
-  for (const spx::LineFFTHost* f : {&p->fx, &p->fy, &p->fz, &p->fx32, &p->fy32, &p->fz32})
+  for (const spx::LineFFTHost* f :
+       {&p->fx, &p->fy, &p->fz, &p->fx32, &p->fy32, &p->fz32, &p->fzw, &p->fzw32})
     tab += f->bytes;
   info->device_bytes = p->dev_bytes + tab;
   size_t whd, wb;
@@ -427,6 +432,7 @@ int spx_plan_get_info(const spx_plan* p, spx_plan_info* info) {
   info->z_threads_per_line = p->fz.tpl;
   info->z_lines_per_block = p->fz.nl;
   info->z_tables_global = (p->fz.mode != 0 && p->fz.tab == p->fz.M) ? 1 : 0;
+  info->z_wave_plan = p->fzw.wave;
   return SPX_OK;
 }
 

@@ -46,6 +46,7 @@ struct LineFFT {
 // Host-side owner of the tables behind a LineFFT.
 struct LineFFTHost {
   int n = 0, M = 0, mode = 0, nstage = 0, tpl = 0, lgtpl = 0, nl = 0, lgnl = 0, ldl = 0;
+  int wave = -1;  // compiled one-wavefront-per-line plan of the line (spx_wave.h), -1: none
   int tab = 0;  // LDS table slots: twiddles M, plus bfil M and chirp n
                 // (Bluestein) or bfil M and the 2M Rader indices (Rader)
   int radix[kMaxStages] = {0};
@@ -163,6 +164,9 @@ struct spx_plan {
   // nonlinear pipeline.
   spx::LineFFTHost fx, fy, fz, fz2;
   spx::LineFFTHost fx32, fy32, fz32;
+  // one-wavefront-per-line variants of fz / fz32 for the z transforms, when
+  // the z length has a compiled wave plan (wave < 0: unused)
+  spx::LineFFTHost fzw, fzw32;
   spx::WallMfmaHost wall;
   bool work_bouss = false;
   // scratch
@@ -274,6 +278,8 @@ struct ProfScope {
 // a float64 S and writes float32 X; the others read and write float32).
 // csize: bytes per complex element (16: float64 tables, 8: float32 tables)
 int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines);
+// the one-wavefront-per-line plan of length n (f.wave < 0 if none is compiled)
+int build_wave_fft(LineFFTHost& f, int n, int csize);
 void free_line_fft(LineFFTHost& f);
 
 // z transforms in place on S (contiguous lines).

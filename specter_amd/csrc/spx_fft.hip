@@ -22,6 +22,7 @@
 #include <unordered_map>
 
 #include "spx_engine.h"
+#include "spx_wave.h"
 
 namespace spx {
 
@@ -48,6 +49,27 @@ constexpr int KEEP = kStage;  // line elements a thread can hold in registers (n
 // Workgroups hold nl lines of tpl threads: 256 threads, or one line of up to
 // 512 threads for the longest lines (e.g. Bluestein length 2100 for nz = 1049).
 constexpr int kMaxBlock = 512;
+
+// Forward DFT of a z line in LDS: the compiled wave plan for radix-set codes
+// RS_WAVE + w (tables: twiddles staged in LDS, filter and chirp global),
+// the general engine otherwise.  Ends with a workgroup barrier either way.
+template <typename T, int RS>
+__device__ __forceinline__ void zline_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw,
+                                          int lane) {
+  if constexpr (RS >= RS_WAVE) {
+    WavePlanOf<RS>::type::template dft<T>(x, lane, tw, f.bfil, f.chirp);
+    __syncthreads();
+  } else {
+    line_dft<T, RS>(x, f, tw, lane);
+  }
+}
+
+// line elements a thread holds in registers in the z kernels (n <= KN * tpl)
+template <int RS>
+constexpr int keep_n() {
+  if constexpr (RS >= RS_WAVE) return (WavePlanOf<RS>::type::n + 63) / 64;
+  else return KEEP;
+}
 
 // XCD-aware tile order of a 2D grid.  Workgroups are handed to the 8 XCDs
 // (each with its own L2) round-robin by linear id; the kx tiles of the
@@ -91,7 +113,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_inplace(cpx<T>* __restrict__ S,
     fc_continue_lds(lds, f.nl, f.ldl, np, C, d, dir);
     __syncthreads();
   }
-  line_dft<T, RS>(xl, f, tws, lane);
+  zline_dft<T, RS>(xl, f, tws, lane);
   if (ok) {
     for (int k = lane; k < n; k += f.tpl) {
       Cp v = xl[k];
@@ -201,7 +223,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_x2s(const cpx<T>* __restrict__ 
       fc_continue_lds(lds, f.nl, f.ldl, np, C, d, dir);
       __syncthreads();
     }
-    line_dft<T, RS>(xl, f, tws, lane);
+    zline_dft<T, RS>(xl, f, tws, lane);
   }
   const int i = i0 + line;
   if (i < nkx)
@@ -317,6 +339,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_s2x(const cpx<TI>* __restrict__
   // kernels keep it in registers, float64 kernels read it again (the second
   // read hits in L2 and keeps the registers free for occupancy)
   constexpr bool KEEPV = DUAL && sizeof(T) == 4;
+  constexpr int KN = keep_n<RS>();
   auto load = [&](int k) {
     Cp v = mkc<T>(0, 0);
     if (ok) {
@@ -327,9 +350,9 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_s2x(const cpx<TI>* __restrict__
     }
     return v;
   };
-  Cp keep[KEEPV ? KEEP : 1];
+  Cp keep[KEEPV ? KN : 1];
 #pragma unroll
-  for (int q = 0; q < KEEP; ++q) {
+  for (int q = 0; q < KN; ++q) {
     const int k = lane + q * f.tpl;
     if (k < n) {
       const Cp v = load(k);
@@ -338,7 +361,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_s2x(const cpx<TI>* __restrict__
     }
   }
   __syncthreads();
-  if (!nofft) line_dft<T, RS>(xl, f, tws, lane);
+  if (!nofft) zline_dft<T, RS>(xl, f, tws, lane);
   const long pl = long(nkx) * ny;
   const int l = tid & (f.nl - 1), ii = i0 + l;
   if (ii < nkx)
@@ -347,7 +370,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_s2x(const cpx<TI>* __restrict__
   if constexpr (DUAL) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < KEEP; ++q) {
+    for (int q = 0; q < KN; ++q) {
       const int k = lane + q * f.tpl;
       if (k < n) {
         if constexpr (KEEPV) xl[k] = cconj(ikmul(T(kz[k]), keep[q]));
@@ -355,7 +378,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_s2x(const cpx<TI>* __restrict__
       }
     }
     __syncthreads();
-    line_dft<T, RS>(xl, f, tws, lane);
+    zline_dft<T, RS>(xl, f, tws, lane);
     if (ii < nkx)
       for (int k = tid >> f.lgnl; k < nzout; k += f.tpl)
         X2[ii + long(nkx) * j + pl * k] = cconj(lds[l * f.ldl + k]);
@@ -1368,6 +1391,37 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
   return SPX_OK;
 }
 
+int build_wave_fft(LineFFTHost& f, int n, int csize) {
+  f = LineFFTHost();
+  if (const char* e = std::getenv("SPX_WAVE"))
+    if (e[0] == '0') return SPX_OK;
+  LineFFTHost h;
+  if (int rc = build_line_fft(h, n, csize, 0)) {
+    free_line_fft(h);
+    return rc;
+  }
+  const int w = wave_plan_for(h.n, h.M, h.mode);
+  if (w < 0) {
+    free_line_fft(h);
+    return SPX_OK;
+  }
+  // one wavefront per line, 8 lines per workgroup; twiddles staged in LDS,
+  // the Bluestein filter and chirp read from global memory
+  h.wave = w;
+  h.tpl = 64;
+  h.lgtpl = 6;
+  h.nl = 8;
+  h.lgnl = 3;
+  h.ldl = h.M + 1;
+  h.tab = h.M;
+  if (h.lds_bytes(size_t(csize)) > 160 * 1024) {
+    free_line_fft(h);
+    return SPX_OK;
+  }
+  f = h;
+  return SPX_OK;
+}
+
 void free_line_fft(LineFFTHost& f) {
   if (f.tw) (void)hipFree(f.tw);
   if (f.chirp) (void)hipFree(f.chirp);
@@ -1395,6 +1449,7 @@ static int lds_limit(K kernel, size_t bytes) {
 // radix set of a plan: the smallest compiled kernel family that covers it
 // (Rader and Bluestein plans never use the power-of-two family)
 [[maybe_unused]] static int rset_of(const LineFFTHost& f) {
+  if (f.wave >= 0) return RS_WAVE + f.wave;
   if (f.mode != 0 && f.tab == f.M) return RS_GTAB;  // radices <= 9 (build_line_fft)
   int rs = f.mode == 0 ? RS_POW2 : RS_SMOOTH;
   for (int i = 0; i < f.nstage; ++i) {
@@ -1422,10 +1477,42 @@ static int lds_limit(K kernel, size_t bytes) {
       if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_ALL TARGS>, (LDS))) return rc_;            \
       hipLaunchKernelGGL((KER<TY SPX_COMMA RS_ALL TARGS>), GRID, BLOCK, LDS, STREAM,         \
                          __VA_ARGS__);                                                       \
-    } else {                                                                                 \
+    } else if (rs_ == RS_GTAB) {                                                             \
       if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_GTAB TARGS>, (LDS))) return rc_;           \
       hipLaunchKernelGGL((KER<TY SPX_COMMA RS_GTAB TARGS>), GRID, BLOCK, LDS, STREAM,        \
                          __VA_ARGS__);                                                       \
+    } else {                                                                                 \
+      return fail(SPX_ERR_STATE, "wave line plan passed to a general kernel");               \
+    }                                                                                        \
+  } while (0)
+
+// z-transform launch: the wave-plan instance when the plan has one, else as
+// SPX_LAUNCH_RS (spelled out: TARGS cannot be forwarded to another macro)
+#define SPX_LAUNCH_Z(F, KER, TY, TARGS, GRID, BLOCK, LDS, STREAM, ...)                       \
+  do {                                                                                       \
+    const int rs_ = rset_of(F);                                                              \
+    if (rs_ == RS_WAVE + 0) {                                                                \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_WAVE + 0 TARGS>, (LDS))) return rc_;       \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_WAVE + 0 TARGS>), GRID, BLOCK, LDS, STREAM,    \
+                         __VA_ARGS__);                                                       \
+    } else if (rs_ == RS_POW2) {                                                             \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_POW2 TARGS>, (LDS))) return rc_;           \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_POW2 TARGS>), GRID, BLOCK, LDS, STREAM,        \
+                         __VA_ARGS__);                                                       \
+    } else if (rs_ == RS_SMOOTH) {                                                           \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_SMOOTH TARGS>, (LDS))) return rc_;         \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_SMOOTH TARGS>), GRID, BLOCK, LDS, STREAM,      \
+                         __VA_ARGS__);                                                       \
+    } else if (rs_ == RS_ALL) {                                                              \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_ALL TARGS>, (LDS))) return rc_;            \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_ALL TARGS>), GRID, BLOCK, LDS, STREAM,         \
+                         __VA_ARGS__);                                                       \
+    } else if (rs_ == RS_GTAB) {                                                             \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_GTAB TARGS>, (LDS))) return rc_;           \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_GTAB TARGS>), GRID, BLOCK, LDS, STREAM,        \
+                         __VA_ARGS__);                                                       \
+    } else {                                                                                 \
+      return fail(SPX_ERR_STATE, "z line plan without a compiled kernel family");            \
     }                                                                                        \
   } while (0)
 
@@ -1439,6 +1526,11 @@ static inline const cpx<T>* C_(const void* p) { return static_cast<const cpx<T>*
 static inline const LineFFTHost& fxp(const spx_plan* p, bool lo) { return lo ? p->fx32 : p->fx; }
 static inline const LineFFTHost& fyp(const spx_plan* p, bool lo) { return lo ? p->fy32 : p->fy; }
 static inline const LineFFTHost& fzp(const spx_plan* p, bool lo) { return lo ? p->fz32 : p->fz; }
+// the z-transform plan: the wave variant where one is compiled
+static inline const LineFFTHost& fzt(const spx_plan* p, bool lo) {
+  if (lo) return p->fzw32.wave >= 0 ? p->fzw32 : p->fz32;
+  return p->fzw.wave >= 0 ? p->fzw : p->fz;
+}
 
 [[maybe_unused]] static int check_lo(const spx_plan* p, bool lo) {
   if (lo && !p->fp32) return fail(SPX_ERR_STATE, "float32 kernels need a mixed-precision plan");
@@ -1447,12 +1539,12 @@ static inline const LineFFTHost& fzp(const spx_plan* p, bool lo) { return lo ? p
 
 #if !defined(SPX_PART) || SPX_PART == 1
 int zfft_inplace(spx_plan* p, void* S, int forward, int cont, double scale) {
-  const LineFFTHost& f = p->fz;
+  const LineFFTHost& f = fzt(p, false);
   const long nlines = p->wlines();
   const unsigned nb = unsigned((nlines + f.nl - 1) / f.nl);
   const int C = p->g.Cz, d = p->g.oz;
   ProfScope ps_(p, "z_inplace", double(p->wnc()) * 2.0 * 16.0);
-  SPX_LAUNCH_RS(f, k_z_inplace, double, , dim3(nb), blk(f), f.lds_bytes(16), p->stream,
+  SPX_LAUNCH_Z(f, k_z_inplace, double, , dim3(nb), blk(f), f.lds_bytes(16), p->stream,
                 C_<double>(S), nlines, f.dev<Cd>(), p->np, forward, (cont && C > 0) ? 1 : 0,
                 scale, p->d_dir, C, d);
   SPX_CHECK_LAUNCH();
@@ -1480,13 +1572,13 @@ static int zfft_x2s_t(spx_plan* p, const void* X, void* S, int nofft, const Line
   ProfScope ps_(p, "z_x2s", double(p->wkx()) * p->g.ny * sizeof(cpx<T>) *
                                 ((nofft ? p->g.nz : p->np) + p->g.nz));
   if (Xb) {
-    SPX_LAUNCH_RS(f, k_z_x2s, T, SPX_COMMA true, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
+    SPX_LAUNCH_Z(f, k_z_x2s, T, SPX_COMMA true, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
                   p->stream, C_<T>(X), C_<T>(S), f.dev<cpx<T>>(), p->nkx, p->g.ny, p->np, nofft,
                   p->d_dir, p->g.Cz, p->g.oz, xstride, C_<double>(Xb));
     SPX_CHECK_LAUNCH();
     return SPX_OK;
   }
-  SPX_LAUNCH_RS(f, k_z_x2s, T, SPX_COMMA false, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
+  SPX_LAUNCH_Z(f, k_z_x2s, T, SPX_COMMA false, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
                 p->stream, C_<T>(X), C_<T>(S), f.dev<cpx<T>>(), p->nkx, p->g.ny, p->np, nofft, p->d_dir,
                 p->g.Cz, p->g.oz, xstride, static_cast<const Cd*>(nullptr));
   SPX_CHECK_LAUNCH();
@@ -1495,8 +1587,8 @@ static int zfft_x2s_t(spx_plan* p, const void* X, void* S, int nofft, const Line
 int zfft_x2s(spx_plan* p, const void* X, void* S, int nofft, bool lo, const void* Xb) {
   if (int rc = check_lo(p, lo)) return rc;
   if (Xb && (nofft || p->g.Cz <= 0)) return fail(SPX_ERR_ARG, "boundary planes need a continuation");
-  return lo ? zfft_x2s_t<float>(p, X, S, nofft, p->fz32, Xb)
-            : zfft_x2s_t<double>(p, X, S, nofft, p->fz, Xb);
+  return lo ? zfft_x2s_t<float>(p, X, S, nofft, fzt(p, true), Xb)
+            : zfft_x2s_t<double>(p, X, S, nofft, fzt(p, false), Xb);
 }
 
 int zfft_s2x_bnd(spx_plan* p, const void* const* S, int ncomp, void* XVb, void* XZb) {
@@ -1544,13 +1636,13 @@ static int zfft_s2x_t(spx_plan* p, const void* S, void* X, int dirn, int nzout, 
   const Cd* in = C_<double>(S);
   if (X2) {
     ProfScope ps_(p, "z_s2x_dual", line * (16.0 * p->g.nz + 2.0 * sizeof(cpx<T>) * nzout));
-    SPX_LAUNCH_RS(f, k_z_s2x, T, SPX_COMMA true SPX_COMMA double, grid, blk(f),
+    SPX_LAUNCH_Z(f, k_z_s2x, T, SPX_COMMA true SPX_COMMA double, grid, blk(f),
                   f.lds_bytes(sizeof(cpx<T>)), p->stream, in, C_<T>(X), C_<T>(X2),
                   f.dev<cpx<T>>(), p->nkx, p->g.ny, nzout, dirn, nofft, p->d_kx, p->d_ky,
                   p->d_kz);
   } else {
     ProfScope ps_(p, "z_s2x", line * (16.0 * p->g.nz + sizeof(cpx<T>) * nzout));
-    SPX_LAUNCH_RS(f, k_z_s2x, T, SPX_COMMA false SPX_COMMA double, grid, blk(f),
+    SPX_LAUNCH_Z(f, k_z_s2x, T, SPX_COMMA false SPX_COMMA double, grid, blk(f),
                   f.lds_bytes(sizeof(cpx<T>)), p->stream, in, C_<T>(X),
                   static_cast<cpx<T>*>(nullptr), f.dev<cpx<T>>(), p->nkx, p->g.ny, nzout, dirn,
                   nofft, p->d_kx, p->d_ky, p->d_kz);
@@ -1561,8 +1653,8 @@ static int zfft_s2x_t(spx_plan* p, const void* S, void* X, int dirn, int nzout, 
 int zfft_s2x(spx_plan* p, const void* S, void* X, int dirn, int nzout, int nofft, void* X2,
              bool lo) {
   if (int rc = check_lo(p, lo)) return rc;
-  return lo ? zfft_s2x_t<float>(p, S, X, dirn, nzout, nofft, X2, p->fz32)
-            : zfft_s2x_t<double>(p, S, X, dirn, nzout, nofft, X2, p->fz);
+  return lo ? zfft_s2x_t<float>(p, S, X, dirn, nzout, nofft, X2, fzt(p, true))
+            : zfft_s2x_t<double>(p, S, X, dirn, nzout, nofft, X2, fzt(p, false));
 }
 #endif  // SPX_PART 1 (z transforms)
 

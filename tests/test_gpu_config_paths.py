@@ -212,3 +212,48 @@ def test_plan_device_bytes_match_allocations():
         assert before == after.device_bytes + after.state_bytes
         sol.close()
         p.close()
+
+
+def test_wave_plan_matches_engine(monkeypatch):
+    """nz = 537 (the 512^3 grids) runs its z transforms on the compiled
+    one-wavefront-per-line Bluestein plan (1080 = 18*6*10); with SPX_WAVE=0
+    at plan creation the general engine runs them.  Both match the oracle,
+    and each other to rounding, in both precisions (the state transforms in
+    float64, gradre's transforms in the plan's precision)."""
+    shape = (8, 8, 537, 25, 5)
+    O, g, _ = _grid_plan(shape)
+    rng = np.random.default_rng(11)
+    R = rng.standard_normal(g.real_shape)
+    S = rng.standard_normal(g.spec_shape) + 1j * rng.standard_normal(g.spec_shape)
+    a, b, c = (rng.standard_normal(g.spec_shape) + 1j * rng.standard_normal(g.spec_shape)
+               for _ in range(3))
+    ref_r2c = O.fftp3d_real_to_complex(g, R.copy())
+    ref_c2r = O.fftp3d_complex_to_real(g, S.copy())
+    ref_grad = O.gradre(g, a, b, c)
+    res = {}
+    for wave in (True, False):
+        if wave:
+            monkeypatch.delenv("SPX_WAVE", raising=False)
+        else:
+            monkeypatch.setenv("SPX_WAVE", "0")
+        for prec in (0, 1):
+            _, _, p = _grid_plan(shape, prec)
+            assert p.info().z_wave_plan == (0 if wave else -1)
+            out = p.empty_spectral()
+            p.fftp3d_real_to_complex(_cu(R), out)
+            outr = p.empty_real()
+            p.fftp3d_complex_to_real(_cu(S), outr)
+            outs = [p.empty_nonlinear() for _ in range(3)]
+            p.gradre(_cu(a), _cu(b), _cu(c), *outs)
+            res[wave, prec] = (out.cpu().numpy(), outr.cpu().numpy(),
+                               [o.cpu().numpy().astype(complex) for o in outs])
+    for prec, tol in ((0, 1e-12), (1, 1e-5)):
+        w, e = res[True, prec], res[False, prec]
+        assert rel_err(w[0], ref_r2c) < 1e-12 and rel_err(w[1], ref_c2r) < 1e-12
+        for o, r in zip(w[2], ref_grad):
+            assert rel_err(o, r) < tol
+        agree = max(rel_err(x, y) for x, y in zip(w[2], e[2]))
+        print(f"precision {prec}: wave vs engine r2c {rel_err(w[0], e[0]):.2e}, "
+              f"c2r {rel_err(w[1], e[1]):.2e}, gradre {agree:.2e}")
+        assert rel_err(w[0], e[0]) < 1e-14 and rel_err(w[1], e[1]) < 1e-14
+        assert agree < (1e-13 if prec == 0 else 2e-6)
