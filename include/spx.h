@@ -42,7 +42,7 @@ enum {
 };
 
 enum { SPX_FP64 = 0, SPX_FP32_MIXED = 1 };
-enum { SPX_SOLVER_HD = 0, SPX_SOLVER_BOUSS = 1 };
+enum { SPX_SOLVER_HD = 0, SPX_SOLVER_BOUSS = 1, SPX_SOLVER_ROTBOUSS = 2 };
 
 typedef struct spx_plan spx_plan;
 typedef struct spx_state spx_state;
@@ -211,13 +211,15 @@ int spx_energy_spectrum(spx_plan* p, const void* a, const void* b, const void* c
 
 /* ---- device-resident solver state (include/hd, include/bouss, specter.fpp RK loop) ---- */
 typedef struct {
-  int solver;        /* SPX_SOLVER_HD or SPX_SOLVER_BOUSS                      */
+  int solver;        /* SPX_SOLVER_HD, SPX_SOLVER_BOUSS or SPX_SOLVER_ROTBOUSS */
   int ord;           /* RK order ORD (specter.fpp:1157)                        */
   double dt, nu;
   double kappa;      /* scalar diffusivity (Boussinesq)                        */
   double xmom, xtemp;/* gama*xmom, gama*xtemp (specter.fpp:480-491)            */
   double v_zsta[2];  /* wall velocities (vxzsta, vyzsta)                        */
   double v_zend[2];  /* (vxzend, vyzend)                                        */
+  double omega[3];   /* ROTBOUSS: (omegax, omegay, omegaz), the &rotation
+                        namelist (specter.fpp:662-677); ignored otherwise      */
 } spx_params;
 
 int spx_state_create(spx_plan* p, const spx_params* prm, spx_state** out);
@@ -227,7 +229,8 @@ int spx_state_destroy(spx_state* s);
 int spx_state_field(spx_state* s, const char* name, void** dev_ptr);
 /* hd_rkstep1 copy (include/hd/hd_rkstep1.f90:4-6). */
 int spx_rkstep1(spx_plan* p, spx_state* s);
-/* one RK stage o (include/hd/hd_rkstep2.f90:3-36 / include/bouss/bouss_rkstep2.f90:3-59) */
+/* one RK stage o (include/hd/hd_rkstep2.f90:3-36 / include/bouss/bouss_rkstep2.f90:3-59 /
+ * include/rotbouss/rotbouss_rkstep2.f90:3-50) */
 int spx_substep(spx_plan* p, spx_state* s, int o);
 /* nsteps full steps: rkstep1 + o = ord..1 (specter.fpp:1142-1161). */
 int spx_step(spx_plan* p, spx_state* s, int nsteps);

@@ -36,6 +36,14 @@ Pinning of each function (tests/test_oracle_*.py):
                                pressure extrapolation factor of vboundary.f90:195-196; the
                                Boussinesq rest state is hydrostatic; planted composition
                                mistakes fail
+  * rotbouss_substep / _step    tests/test_oracle_steps.py: with Omega = Omega_z e_z the
+    coriolis_terms              horizontal mean flow is an inertial oscillation -- the line
+                               follows a matrix model built from the DFT, the continuation
+                               and the filter to 1e-9 and the RK polynomial of the rotation
+                               to FC accuracy (a sign, order or filter mistake fails); the
+                               Coriolis terms are 2 Omega x v (numpy cross product); with
+                               Omega = 0 the stage plus the th filter and round trip is the
+                               Boussinesq stage bit for bit
   * sol_project pressure d      the removed velocity is (i kx d, i ky d, d/dz d)
   * precision="longdouble"      the same routines in extended precision (constants formed
                                in float64): the exact-trajectory referee of the float64 runs
@@ -559,6 +567,49 @@ def bouss_step(g: Grid, st: HDState, ordr, dt, nu, kappa, xmom, xtemp) -> None:
     C1, C2, C3, C7 = st.vx.copy(), st.vy.copy(), st.vz.copy(), st.th.copy()
     for o in range(ordr, 0, -1):
         bouss_substep(g, st, C1, C2, C3, C7, o, ordr, dt, nu, kappa, xmom, xtemp)
+
+
+def coriolis_terms(st: HDState, omega):
+    """The Coriolis contributions 2 Omega x v added to the nonlinear terms
+    C4, C5, C6 (include/rotbouss/rotbouss_rkstep2.f90:13-19), in the
+    reference's operation order."""
+    ox, oy, oz = omega
+    return (2 * (oy * st.vz - oz * st.vy), 2 * (oz * st.vx - ox * st.vz),
+            2 * (ox * st.vy - oy * st.vx))
+
+
+def rotbouss_substep(g: Grid, st: HDState, C1, C2, C3, C7, o: int, ordr: int, dt: float,
+                     nu: float, kappa: float, xmom: float, xtemp: float, omega) -> None:
+    """One RK stage of the rotating Boussinesq solver
+    (include/rotbouss/rotbouss_rkstep2.f90:3-50): the Boussinesq stage with
+    2 Omega x v added to the nonlinear term before the filter (:13-21), and
+    without the Boussinesq stage's filter and real-space round trip of th
+    (the stage ends with s_imposebc, :49-50)."""
+    rmp = 1.0 / float(o)
+    C4, C5, C6 = gradre(g, st.vx, st.vy, st.vz)
+    C8 = advect(g, st.vx, st.vy, st.vz, st.th)
+    r4, r5, r6 = coriolis_terms(st, omega)
+    C4 = C4 + r4
+    C5 = C5 + r5
+    C6 = C6 + r6 - xmom * st.th
+    C8 = C8 - xtemp * st.vz
+    for arr in (C4, C5, C6, C8):
+        fc_filter(g, arr)
+    lx, ly, lz, lt = (laplak(g, st.vx), laplak(g, st.vy), laplak(g, st.vz),
+                      laplak(g, st.th))
+    st.vx[...] = C1 + dt * (nu * lx - C4 + st.fx) * rmp
+    st.vy[...] = C2 + dt * (nu * ly - C5 + st.fy) * rmp
+    st.vz[...] = C3 + dt * (nu * lz - C6 + st.fz) * rmp
+    st.th[...] = C7 + dt * (kappa * lt - C8 + st.fs) * rmp
+    v_imposebc_and_project(g, st.vx, st.vy, st.vz, st.pr, o, ordr)
+    s_imposebc(g, st.th)
+
+
+def rotbouss_step(g: Grid, st: HDState, ordr, dt, nu, kappa, xmom, xtemp, omega) -> None:
+    """rkstep1 copies (include/rotbouss/rotbouss_rkstep1.f90) + stages ORD..1."""
+    C1, C2, C3, C7 = st.vx.copy(), st.vy.copy(), st.vz.copy(), st.th.copy()
+    for o in range(ordr, 0, -1):
+        rotbouss_substep(g, st, C1, C2, C3, C7, o, ordr, dt, nu, kappa, xmom, xtemp, omega)
 
 
 # ---------------------------------------------------------------------------

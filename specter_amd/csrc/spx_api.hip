@@ -962,17 +962,18 @@ int spx_energy_spectrum(spx_plan* p, const void* a, const void* b, const void* c
 // ---- solver state ------------------------------------------------------------------
 int spx_state_create(spx_plan* p, const spx_params* prm, spx_state** out) {
   SPX_REQ(p && prm && out, "spx_state_create: null argument");
-  SPX_REQ(prm->solver == SPX_SOLVER_HD || prm->solver == SPX_SOLVER_BOUSS,
+  SPX_REQ(prm->solver == SPX_SOLVER_HD || prm->solver == SPX_SOLVER_BOUSS ||
+              prm->solver == SPX_SOLVER_ROTBOUSS,
           "spx_state_create: unknown solver");
   SPX_REQ(prm->ord >= 1, "spx_state_create: ord must be >= 1");
   if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "the solvers need walls in z (Cz > 0)");
   *out = nullptr;
-  int rc = ensure_work(p, prm->solver == SPX_SOLVER_BOUSS);
+  const bool bouss = prm->solver != SPX_SOLVER_HD;  // BOUSS and ROTBOUSS carry th
+  int rc = ensure_work(p, bouss);
   if (rc) return rc;
   spx_state* s = new spx_state();
   s->p = p;
   s->prm = *prm;
-  const bool bouss = prm->solver == SPX_SOLVER_BOUSS;
   void** fields[] = {&s->vx, &s->vy, &s->vz, &s->pr, &s->fx, &s->fy, &s->fz,
                      &s->C1, &s->C2, &s->C3, &s->N[0], &s->N[1], &s->N[2]};
   void** bfields[] = {&s->th, &s->fs, &s->C7, &s->N[3]};
@@ -1072,7 +1073,8 @@ static int substep_impl(spx_plan* p, spx_state* s, int o, bool full_pressure) {
   SPX_REQ(p && s && s->p == p, "spx_substep: bad plan/state");
   const spx_params& q = s->prm;
   SPX_REQ(o >= 1 && o <= q.ord, "spx_substep: stage out of range");
-  const bool bouss = q.solver == SPX_SOLVER_BOUSS;
+  const bool rot = q.solver == SPX_SOLVER_ROTBOUSS;
+  const bool bouss = q.solver == SPX_SOLVER_BOUSS || rot;
   const double rmp = 1.0 / double(o);
   int rc;
   MemberView mv(p, p->nact);
@@ -1080,6 +1082,9 @@ static int substep_impl(spx_plan* p, spx_state* s, int o, bool full_pressure) {
   void* nlo[3] = {s->N[0], s->N[1], s->N[2]};
   if ((rc = gradre_impl(p, vel, nlo))) return rc;
   if (bouss && (rc = advect_impl(p, vel, s->th, s->N[3], true))) return rc;
+  // rotating frame: 2 Omega x v joins the nonlinear term before the buoyancy
+  // and the filter (include/rotbouss/rotbouss_rkstep2.f90:13-21)
+  if (rot && (rc = op_coriolis(p, nlo, vel, q.omega, p->fp32))) return rc;
   void* v[4] = {s->vx, s->vy, s->vz, s->th};
   const void* c0[4] = {s->C1, s->C2, s->C3, s->C7};
   void* nl[4] = {s->N[0], s->N[1], s->N[2], s->N[3]};
@@ -1133,6 +1138,8 @@ static int substep_impl(spx_plan* p, spx_state* s, int o, bool full_pressure) {
   }
   if (rc) return rc;
   if ((rc = spx_s_imposebc(p, s->th))) return rc;
+  // the rotating solver ends its stage here (rotbouss_rkstep2.f90:49-50)
+  if (rot) return SPX_OK;
   if ((rc = op_filter(p, s->th))) return rc;
   // round trip of th through real space (include/bouss/bouss_rkstep2.f90:56-59)
   if ((rc = c2r3(p, s->th, p->R0, 0, p->np))) return rc;

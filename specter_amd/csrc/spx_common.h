@@ -393,6 +393,10 @@ int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d,
                           bool walls_only = false);
 int op_axpy_field(spx_plan* p, void* y, const void* x, double alpha);
 int op_copy_fields(spx_plan* p, int n, const void* const* src, void* const* dst);
+// N_c += (2 Omega x v)_c, c = x, y, z (include/rotbouss/rotbouss_rkstep2.f90:13-19);
+// nl_lo: the nonlinear terms are float32
+int op_coriolis(spx_plan* p, void* const* nl, const void* const* v, const double omega[3],
+                bool nl_lo);
 // d <- i kx a + i ky b + i kz c (all active members)
 int op_div_spec(spx_plan* p, const void* a, const void* b, const void* c, void* d);
 // per member: block maxima of |r| over n values at member stride ms -> out[member][block]

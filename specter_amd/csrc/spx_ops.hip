@@ -147,6 +147,24 @@ __global__ void k_rk_update(RkArgs<TN> args, int ncomp, long nc, SIdx ix,
   }
 }
 
+// Coriolis terms 2 Omega x v added to the nonlinear terms in the reference's
+// operation order (include/rotbouss/rotbouss_rkstep2.f90:13-19), before the
+// buoyancy source and the filter that the RK update applies:
+//   Nx += 2 (Oy vz - Oz vy),  Ny += 2 (Oz vx - Ox vz),  Nz += 2 (Ox vy - Oy vx)
+template <typename TN>
+__global__ void k_coriolis(cpx<TN>* __restrict__ nx, cpx<TN>* __restrict__ ny,
+                           cpx<TN>* __restrict__ nz, const Cp* __restrict__ vx,
+                           const Cp* __restrict__ vy, const Cp* __restrict__ vz, long nc,
+                           double ox, double oy, double oz) {
+#pragma clang fp contract(off)
+  const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= nc) return;
+  const Cp a = vx[e], b = vy[e], c = vz[e];
+  st(nx, e, cadd(ld(nx, e), cscale(csub(cscale(c, oy), cscale(b, oz)), 2.0)));
+  st(ny, e, cadd(ld(ny, e), cscale(csub(cscale(a, oz), cscale(c, ox)), 2.0)));
+  st(nz, e, cadd(ld(nz, e), cscale(csub(cscale(b, ox), cscale(a, oy)), 2.0)));
+}
+
 // poisson_inhomogeneous + first projection (boundary_mod.fpp:227-240, 405-448):
 //   d = -i(kx a + ky b + kz c)/kk2, d(0,0,0) = 0 ; a -= i kx d ; b -= i ky d ; c -= i kz d
 // d is float64 for every plan (the pressure potential is kept in float64).
@@ -392,6 +410,23 @@ int op_rk_update(spx_plan* p, int ncomp, void* const* v, const void* const* c0, 
                  const double* const* nu, double dt, double rmp, bool nl_lo) {
   return nl_lo ? rk_launch<float>(p, ncomp, v, c0, nl, f, src, src_coef, nu, dt, rmp)
                : rk_launch<double>(p, ncomp, v, c0, nl, f, src, src_coef, nu, dt, rmp);
+}
+
+int op_coriolis(spx_plan* p, void* const* nl, const void* const* v, const double omega[3],
+                bool nl_lo) {
+  const long nc = long(p->wnc());
+  ProfScope ps_(p, "coriolis", double(nc) * 3.0 * (16.0 + 2.0 * (nl_lo ? 8.0 : 16.0)));
+  if (nl_lo)
+    hipLaunchKernelGGL(k_coriolis<float>, dim3(nblk(nc)), dim3(256), 0, p->stream,
+                       CS<float>(nl[0]), CS<float>(nl[1]), CS<float>(nl[2]), CS<double>(v[0]),
+                       CS<double>(v[1]), CS<double>(v[2]), nc, omega[0], omega[1], omega[2]);
+  else
+    hipLaunchKernelGGL(k_coriolis<double>, dim3(nblk(nc)), dim3(256), 0, p->stream,
+                       CS<double>(nl[0]), CS<double>(nl[1]), CS<double>(nl[2]),
+                       CS<double>(v[0]), CS<double>(v[1]), CS<double>(v[2]), nc, omega[0],
+                       omega[1], omega[2]);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
 }
 
 int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d) {

@@ -13,7 +13,7 @@ MODULE spx
    IMPLICIT NONE
 
    INTEGER(C_INT), PARAMETER :: SPX_FP64 = 0, SPX_FP32_MIXED = 1
-   INTEGER(C_INT), PARAMETER :: SPX_SOLVER_HD = 0, SPX_SOLVER_BOUSS = 1
+   INTEGER(C_INT), PARAMETER :: SPX_SOLVER_HD = 0, SPX_SOLVER_BOUSS = 1, SPX_SOLVER_ROTBOUSS = 2
 
    TYPE, BIND(C) :: spx_grid
       INTEGER(C_INT) :: nx, ny, nz, Cz, oz
@@ -25,6 +25,7 @@ MODULE spx
       INTEGER(C_INT) :: solver, ord
       REAL(C_DOUBLE) :: dt, nu, kappa, xmom, xtemp
       REAL(C_DOUBLE) :: v_zsta(2), v_zend(2)
+      REAL(C_DOUBLE) :: omega(3)
    END TYPE spx_params
 
    INTERFACE

@@ -31,8 +31,9 @@
 ! transposes, the products and the wall update; TTOT: all kernels); TTRA,
 ! TCOM, TCONT, TNEU and TROB are 0 (fused into those kernels, or no MPI).
 !
-! solver = 'BOUSS' in &grid (the reference's compile-time SOLVER choice)
-! runs the Boussinesq solver: &boussinesq (gama, xmom, xtemp; the
+! solver = 'BOUSS' or 'ROTBOUSS' in &grid (the reference's compile-time
+! SOLVER choice) runs the Boussinesq solver, ROTBOUSS with the Coriolis
+! term of &rotation (omegax, omegay, omegaz; specter.fpp:662-677): &boussinesq (gama, xmom, xtemp; the
 ! coefficients used are gama*xmom and gama*xtemp, specter.fpp:476-491) and
 ! &scalar (kappa) are read, th (and optionally fs) start from
 ! th.NNNN.out / fs.NNNN.out, th is written every tstep steps, and every
@@ -49,6 +50,7 @@ PROGRAM spx_hd
    INTEGER :: nx = 64, ny = 64, nz = 89, Cz = 25, oz = 5, ord = 2, prec = 0
    CHARACTER(len=10) :: solver = 'HD'
    REAL(8) :: gama = 1.d0, xmom = 1.d0, xtemp = 1.d0
+   REAL(8) :: omegax = 0.d0, omegay = 0.d0, omegaz = 0.d0
    REAL(8) :: c0 = 0.d0, s0 = 0.d0, skdn = 1.d0, skup = 2.d0, kappa = 1.d-3
    CHARACTER(len=100) :: idir = 'in/', odir = 'out/', tdir = 'tables/'
    INTEGER :: stat = 1, mult = 1, bench = 0, outs = 0
@@ -60,6 +62,7 @@ PROGRAM spx_hd
    REAL(8) :: vxzsta = 0.d0, vxzend = 0.d0, vyzsta = 0.d0, vyzend = 0.d0
    NAMELIST /grid/ nx, ny, nz, Cz, oz, ord, prec, solver
    NAMELIST /boussinesq/ gama, xmom, xtemp
+   NAMELIST /rotation/ omegax, omegay, omegaz
    NAMELIST /scalar/ c0, s0, skdn, skup, kappa
    NAMELIST /status/ idir, odir, tdir, stat, mult, bench, outs
    NAMELIST /boxparams/ Lx, Ly, Lz
@@ -71,7 +74,7 @@ PROGRAM spx_hd
    TYPE(spx_params) :: prm
    TYPE(C_PTR) :: plan, state, dreal, dtmp
    TYPE(C_PTR) :: vx, vy, vz, pr, fx, fy, fz, th, fs
-   LOGICAL :: bouss
+   LOGICAL :: bouss, rot
    REAL(8) :: sv, sg, sp, nus(2)
    REAL(8), ALLOCATABLE, TARGET :: R(:, :, :), A(:, :), Q(:, :)
    REAL(8) :: eng, ens, pot, dv, tb, tt, nb, nt
@@ -104,7 +107,8 @@ PROGRAM spx_hd
    READ(1, NML=velocity)
    REWIND(1)
    READ(1, NML=velbound, IOSTAT=ios)
-   bouss = TRIM(solver) .eq. 'BOUSS'
+   rot = TRIM(solver) .eq. 'ROTBOUSS'
+   bouss = TRIM(solver) .eq. 'BOUSS' .OR. rot
    IF (bouss) THEN
       REWIND(1)
       READ(1, NML=boussinesq)
@@ -112,8 +116,12 @@ PROGRAM spx_hd
       READ(1, NML=scalar)
       xmom = xmom * gama
       xtemp = xtemp * gama
+      IF (rot) THEN
+         REWIND(1)
+         READ(1, NML=rotation)
+      ENDIF
    ELSE IF (TRIM(solver) .ne. 'HD') THEN
-      PRINT *, '[ERROR] solver must be HD or BOUSS. Aborting...'
+      PRINT *, '[ERROR] solver must be HD, BOUSS or ROTBOUSS. Aborting...'
       STOP 1
    ENDIF
    CLOSE(1)
@@ -139,12 +147,15 @@ PROGRAM spx_hd
    CALL spx_check(spx_plan_sync(plan), 'sync')
    CALL CPU_TIME(cpup1)
    CALL SYSTEM_CLOCK(clkp1)
-   IF (bouss) THEN
+   IF (rot) THEN
+      prm = spx_params(SPX_SOLVER_ROTBOUSS, ord, dt, nu, kappa, xmom, xtemp, &
+                       (/vxzsta, vyzsta/), (/vxzend, vyzend/), (/omegax, omegay, omegaz/))
+   ELSE IF (bouss) THEN
       prm = spx_params(SPX_SOLVER_BOUSS, ord, dt, nu, kappa, xmom, xtemp, &
-                       (/vxzsta, vyzsta/), (/vxzend, vyzend/))
+                       (/vxzsta, vyzsta/), (/vxzend, vyzend/), (/0.d0, 0.d0, 0.d0/))
    ELSE
       prm = spx_params(SPX_SOLVER_HD, ord, dt, nu, 0.d0, 0.d0, 0.d0, &
-                       (/vxzsta, vyzsta/), (/vxzend, vyzend/))
+                       (/vxzsta, vyzsta/), (/vxzend, vyzend/), (/0.d0, 0.d0, 0.d0/))
    ENDIF
    CALL spx_check(spx_state_create(plan, prm, state), 'spx_state_create')
    CALL spx_check(spx_state_field(state, 'vx'//C_NULL_CHAR, vx), 'field vx')

@@ -13,7 +13,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "spx.h")
 
 SPX_OK, SPX_ERR_ARG, SPX_ERR_UNSUPPORTED, SPX_ERR_HIP, SPX_ERR_ALLOC, SPX_ERR_STATE = range(6)
 SPX_FP64, SPX_FP32_MIXED = 0, 1
-SPX_SOLVER_HD, SPX_SOLVER_BOUSS = 0, 1
+SPX_SOLVER_HD, SPX_SOLVER_BOUSS, SPX_SOLVER_ROTBOUSS = 0, 1, 2
 
 
 class SpxError(RuntimeError):
@@ -40,7 +40,7 @@ class SpxPlanInfo(ctypes.Structure):
 class SpxParams(ctypes.Structure):
     _fields_ = [("solver", c_int), ("ord", c_int), ("dt", c_double), ("nu", c_double),
                 ("kappa", c_double), ("xmom", c_double), ("xtemp", c_double),
-                ("v_zsta", c_double * 2), ("v_zend", c_double * 2)]
+                ("v_zsta", c_double * 2), ("v_zend", c_double * 2), ("omega", c_double * 3)]
 
 
 _P = c_void_p

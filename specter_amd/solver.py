@@ -1,4 +1,4 @@
-"""Device-resident HD / Boussinesq solver over libspx (the RK loop of
+"""Device-resident HD / Boussinesq / rotating Boussinesq solver over libspx (the RK loop of
 specter.fpp:1002-1165 with include/hd and include/bouss stage bodies).
 
 All state lives in HBM inside libspx; host code only sets it up, steps it and
@@ -27,7 +27,7 @@ class _DevView:
 
 @dataclass
 class SolverParams:
-    solver: str = "HD"        # "HD" or "BOUSS"
+    solver: str = "HD"        # "HD", "BOUSS" or "ROTBOUSS"
     ord: int = 2              # RK order ORD
     dt: float = 1e-3
     nu: float = 2e-3
@@ -36,14 +36,20 @@ class SolverParams:
     xtemp: float = 0.0        # gama*xtemp
     v_zsta: tuple = (0.0, 0.0)
     v_zend: tuple = (0.0, 0.0)
+    omega: tuple = (0.0, 0.0, 0.0)   # ROTBOUSS: (omegax, omegay, omegaz)
 
     def to_c(self) -> _l.SpxParams:
         p = _l.SpxParams()
-        p.solver = _l.SPX_SOLVER_BOUSS if self.solver.upper() == "BOUSS" else _l.SPX_SOLVER_HD
+        kinds = {"HD": _l.SPX_SOLVER_HD, "BOUSS": _l.SPX_SOLVER_BOUSS,
+                 "ROTBOUSS": _l.SPX_SOLVER_ROTBOUSS}
+        if self.solver.upper() not in kinds:
+            raise ValueError(f"solver must be one of {sorted(kinds)}, not {self.solver!r}")
+        p.solver = kinds[self.solver.upper()]
         p.ord, p.dt, p.nu, p.kappa = self.ord, self.dt, self.nu, self.kappa
         p.xmom, p.xtemp = self.xmom, self.xtemp
         p.v_zsta[0], p.v_zsta[1] = self.v_zsta
         p.v_zend[0], p.v_zend[1] = self.v_zend
+        p.omega[0], p.omega[1], p.omega[2] = self.omega
         return p
 
 
@@ -59,7 +65,7 @@ class Solver:
         st = c_void_p()
         _l.check(self._lib.spx_state_create(plan.handle, byref(params.to_c()), byref(st)))
         self._s = st
-        self.bouss = params.solver.upper() == "BOUSS"
+        self.bouss = params.solver.upper() in ("BOUSS", "ROTBOUSS")
 
     def close(self):
         if getattr(self, "_s", None):

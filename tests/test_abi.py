@@ -123,3 +123,23 @@ def test_ensemble_stream_properties():
     sizes = sorted({r.n for r in reqs})
     mb = [member_bytes(next(r for r in reqs if r.n == n)) for n in sizes]
     assert mb == sorted(mb)
+
+
+def test_solver_params_layout():
+    """SolverParams maps the three solver kinds and the rotation vector onto
+    spx_params (the C struct's field order: include/spx.h), and rejects
+    unknown kinds before any device call."""
+    import ctypes
+    from specter_amd import SolverParams
+    from specter_amd import lib as L
+    names = [f[0] for f in L.SpxParams._fields_]
+    assert names == ["solver", "ord", "dt", "nu", "kappa", "xmom", "xtemp", "v_zsta", "v_zend",
+                     "omega"]
+    assert L.SpxParams.omega.offset == L.SpxParams.v_zend.offset + 2 * ctypes.sizeof(
+        ctypes.c_double)
+    for kind, code in (("HD", 0), ("bouss", 1), ("ROTBOUSS", 2)):
+        assert SolverParams(solver=kind).to_c().solver == code
+    c = SolverParams(solver="ROTBOUSS", omega=(0.5, -1.0, 2.0)).to_c()
+    assert list(c.omega) == [0.5, -1.0, 2.0]
+    with pytest.raises(ValueError):
+        SolverParams(solver="MHD").to_c()

@@ -38,6 +38,9 @@ f0 = 0.0, u0 = 1.0, kdn = 2.0, kup = 4.0, nu = {nu}
 &velbound
 vbczsta = "noslip", vbczend = "noslip", vxzsta = 0.0, vxzend = 0.0, vyzsta = 0.0, vyzend = 0.0
 /
+&rotation
+omegax = {ox}, omegay = {oy}, omegaz = {oz}
+/
 """
 
 
@@ -53,7 +56,7 @@ def test_fortran_driver_matches_python_and_oracle(tmp_path):
     (tmp_path / "out").mkdir()
     (tmp_path / "parameter.inp").write_text(PARAM.format(
         nx=nx, ny=ny, nz=nz, C=C, d=d, ord=ordr, tdir=TABLES, dt=dt, step=step,
-        tstep=tstep, cstep=cstep, nu=nu, solver="HD", kappa=1e-3))
+        tstep=tstep, cstep=cstep, nu=nu, solver="HD", kappa=1e-3, ox=0.0, oy=0.0, oz=0.0))
     u = ic.spectrum_peaked_velocity(nx, ny, nz, C, seed=1000)
     fr = ic.large_scale_forcing(nx, ny, nz, C)
     for name, arr in zip(("vx", "vy", "vz", "fx", "fy", "fz"), (*u, *fr)):
@@ -128,22 +131,26 @@ def test_fortran_driver_matches_python_and_oracle(tmp_path):
     assert np.allclose(np.loadtxt(tmp_path / "z.txt"), z[:npz], rtol=1e-14, atol=1e-15)
 
 
-def test_fortran_driver_boussinesq(tmp_path):
-    """solver = BOUSS: the scalar series (scalar.txt, wall <th^2>, Nusselt)
-    and the th output against the Python-driven run and the oracle."""
+@pytest.mark.parametrize("solver", ["BOUSS", "ROTBOUSS"])
+def test_fortran_driver_boussinesq(tmp_path, solver):
+    """solver = BOUSS / ROTBOUSS (&rotation read for the latter): the scalar
+    series (scalar.txt, wall <th^2>, Nusselt) and the th output against the
+    Python-driven run and the oracle."""
     from oracle import specter_ref as O
     from specter_amd import FCPlan, Solver, SolverParams, ic
     if not os.path.exists(BIN):
         pytest.fail("Fortran driver not built (specter_amd.build.build_fortran)")
     nx, ny, nz, C, d, ordr, dt, nu, kappa, step, cstep = 16, 16, 41, 25, 5, 4, 1e-3, 2e-3, 3e-3, 7, 2
     xmom, xtemp = 0.5 * 2.0, 0.5 * 1.0          # gama * xmom, gama * xtemp
+    omega = (0.25, -0.5, 2.0) if solver == "ROTBOUSS" else (0.0, 0.0, 0.0)
     tstep = 5
     npz = nz - C
     (tmp_path / "in").mkdir()
     (tmp_path / "out").mkdir()
     (tmp_path / "parameter.inp").write_text(PARAM.format(
         nx=nx, ny=ny, nz=nz, C=C, d=d, ord=ordr, tdir=TABLES, dt=dt, step=step,
-        tstep=tstep, cstep=cstep, nu=nu, solver="BOUSS", kappa=kappa))
+        tstep=tstep, cstep=cstep, nu=nu, solver=solver, kappa=kappa, ox=omega[0],
+        oy=omega[1], oz=omega[2]))
     u = ic.spectrum_peaked_velocity(nx, ny, nz, C, seed=1000)
     th = 1e-2 * ic.spectrum_peaked_velocity(nx, ny, nz, C, seed=1001)[0]
     fs = 0.1 * ic.spectrum_peaked_velocity(nx, ny, nz, C, seed=1002)[1]
@@ -156,8 +163,8 @@ def test_fortran_driver_boussinesq(tmp_path):
     nus = np.loadtxt(tmp_path / "nusselt.txt", ndmin=2)
     assert sca.shape[0] == scd.shape[0] == nus.shape[0] == (step - 1) // cstep
 
-    prm = SolverParams(solver="BOUSS", ord=ordr, dt=dt, nu=nu, kappa=kappa, xmom=xmom,
-                       xtemp=xtemp)
+    prm = SolverParams(solver=solver, ord=ordr, dt=dt, nu=nu, kappa=kappa, xmom=xmom,
+                       xtemp=xtemp, omega=omega)
     plan = FCPlan(nx, ny, nz, C, d)
     sol = Solver(plan, prm)
     for name, arr in zip(("vx", "vy", "vz", "th", "fs"), (*u, th, fs)):
@@ -180,7 +187,10 @@ def test_fortran_driver_boussinesq(tmp_path):
             rows_or.append(((t - 1) * dt, *O.pscheck(g, st.th, st.fs),
                             *O.bouncheck_z(g, st.th), *O.nusselt(g, st.th)))
         sol.step(1)
-        O.bouss_step(g, st, ordr, dt, nu, kappa, xmom, xtemp)
+        if solver == "ROTBOUSS":
+            O.rotbouss_step(g, st, ordr, dt, nu, kappa, xmom, xtemp, omega)
+        else:
+            O.bouss_step(g, st, ordr, dt, nu, kappa, xmom, xtemp)
         timec += 1
     rows_py = np.array(rows_py)
     rows_or = np.array(rows_or)

@@ -60,7 +60,10 @@ def _run_pair(shape, prm, nsteps, seed=1000, forcing=False, bouss=False, diag_ev
         for s_ in (st, st2):
             if s_ is None:
                 continue
-            if bouss:
+            if sp.solver.upper() == "ROTBOUSS":
+                O.rotbouss_step(g, s_, sp.ord, sp.dt, sp.nu, sp.kappa, sp.xmom, sp.xtemp,
+                                sp.omega)
+            elif bouss:
                 O.bouss_step(g, s_, sp.ord, sp.dt, sp.nu, sp.kappa, sp.xmom, sp.xtemp)
             else:
                 O.hd_step(g, s_, sp.ord, sp.dt, sp.nu)
@@ -97,6 +100,51 @@ def test_bouss_steps():
                              dict(solver="BOUSS", ord=4, dt=1e-3, nu=2e-3, kappa=2e-3,
                                   xmom=1.0, xtemp=1.0), 3, bouss=True, floor=True)
     _check_floor(sol, st, st2, ("vx", "vy", "vz", "th"))
+
+
+ROT = dict(solver="ROTBOUSS", ord=4, dt=1e-3, nu=2e-3, kappa=2e-3, xmom=1.0, xtemp=1.0,
+           omega=(0.7, -1.1, 3.0))
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 41, 25, 5), (16, 16, 43, 27, 8)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_rotbouss_steps(shape):
+    """Rotating Boussinesq (include/rotbouss/rotbouss_rkstep2.f90): the
+    Coriolis term of a rotation with all three components, no th filter or
+    round trip; float64 within 10x the oracle's 1-ulp spread."""
+    sol, st, st2 = _run_pair(shape, ROT, 3, bouss=True, floor=True)
+    _check_floor(sol, st, st2, ("vx", "vy", "vz", "th"))
+
+
+def test_rotbouss_mixed_and_graph_replays():
+    """FP32-mixed rotating Boussinesq (d=5) against the float64 oracle, and
+    graph replays bit-identical to eager launches."""
+    from oracle import specter_ref as O
+    from specter_amd import FCPlan, Solver, SolverParams, ic
+    shape = (16, 16, 41, 25, 5)
+    nx, ny, nz, C, d = shape
+    g = O.make_grid(nx, ny, nz, C, d, TABLES)
+    u = ic.spectrum_peaked_velocity(nx, ny, nz, C, 21)
+    th = ic.thermal_perturbation(nx, ny, nz, C, 22)
+    sp = SolverParams(**ROT)
+    z = np.zeros(g.spec_shape, complex)
+    st = O.HDState(*(O.fftp3d_real_to_complex(g, a.copy()) for a in u), z.copy(), z.copy(),
+                   z.copy(), z.copy(), O.fftp3d_real_to_complex(g, th.copy()), z.copy())
+    for _ in range(3):
+        O.rotbouss_step(g, st, sp.ord, sp.dt, sp.nu, sp.kappa, sp.xmom, sp.xtemp, sp.omega)
+    out = {}
+    for prec, graph in ((1, True), (0, True), (0, False)):
+        sol = Solver(FCPlan(nx, ny, nz, C, d, precision=prec), sp)
+        sol.use_graph(graph)
+        for name, arr in zip(("vx", "vy", "vz", "th"), (*u, th)):
+            sol.set_real_field(name, arr)
+        sol.step(3)
+        out[prec, graph] = {n: sol.spectral(n) for n in ("vx", "vy", "vz", "th")}
+    for n in ("vx", "vy", "vz", "th"):
+        e = rel_l2(out[1, True][n], getattr(st, n))
+        print(f"mixed rotbouss {n}: {e:.2e}")
+        assert e < 1e-5, n
+        assert np.array_equal(out[0, True][n], out[0, False][n]), n
 
 
 def test_cfg1_100_steps_parity(cfg1_oracle):

@@ -417,6 +417,127 @@ def test_bouss_planted_heat_current_sign_fails():
 
 
 # ---------------------------------------------------------------------------
+# rotating Boussinesq (include/rotbouss/rotbouss_rkstep2.f90)
+# ---------------------------------------------------------------------------
+def _inertial_state(g):
+    """Horizontally uniform flow (U(z), 0, 0), U = sin(pi z/Lz): no nonlinear
+    term (kx = ky = 0, vz = 0), no pressure, zero wall values."""
+    R = np.zeros(g.real_shape)
+    R[: g.np_] = np.sin(np.pi * g.z[: g.np_] / g.Lz)[:, None, None]
+    u0 = O.fftp3d_real_to_complex(g, R.copy())
+    z = zero_like(g)
+    return O.HDState(u0.copy(), z.copy(), z.copy(), z.copy(), z.copy(), z.copy(), z.copy(),
+                     z.copy(), z.copy()), u0
+
+
+def _rot_steps(g, st, ordr, dt, omega, nsteps, sign=1.0):
+    """rotbouss steps with nu = kappa = xmom = xtemp = 0; sign = -1 plants a
+    Coriolis term of the wrong sign."""
+    for _ in range(nsteps):
+        if sign == 1.0:
+            O.rotbouss_step(g, st, ordr, dt, 0.0, 0.0, 0.0, 0.0, omega)
+        else:
+            O.rotbouss_step(g, st, ordr, dt, 0.0, 0.0, 0.0, 0.0, tuple(-w for w in omega))
+
+
+def _inertial_line_model(g, u0, ordr, dt, omz, nsteps, filtered=True):
+    """The (kx, ky) = (0, 0) line under rotation about z, from definitions:
+    per stage U <- P (U0 + dt/o 2 Omega_z f V), V <- P (V0 - dt/o 2 Omega_z f U),
+    with f the z filter factors and P the wall map (backward DFT, zero wall
+    values, FC-Gram continuation, forward DFT) of wall_projected_laplacian."""
+    _, _, P = wall_projected_laplacian(g, 0.0)
+    f = O.filter_factors(g)[2] if filtered else np.ones(g.nz)
+    U, V = u0[0, 0].copy(), np.zeros(g.nz, complex)
+    for _ in range(nsteps):
+        C1, C2 = U.copy(), V.copy()
+        for o in range(ordr, 0, -1):
+            U, V = (P @ (C1 + dt / o * 2 * omz * f * V), P @ (C2 - dt / o * 2 * omz * f * U))
+    return U, V
+
+
+@pytest.mark.parametrize("ordr", [2, 3, 4])
+def test_rotbouss_inertial_oscillation(ordr):
+    """With Omega = Omega_z e_z and no dissipation, (U, V) obeys
+    U' = 2 Omega_z V, V' = -2 Omega_z U.  Continuously, one RK step multiplies
+    it by P(h J) = c I + s J, J = [[0, 1], [-1, 0]], c + i s =
+    sum_{j<=ORD} (i h)^j/j!, h = 2 Omega_z dt, so after n steps
+    (U, V) = (C U0, -S U0) with C + i S = (c + i s)^n (here to the accuracy
+    with which the filtered FC line represents sin(pi z/Lz): 1e-5); the
+    discrete line follows the matrix model of _inertial_line_model to 1e-9
+    (the roundoff of the continuation, amplified as in the HD eigenmode
+    test), and a model without the filter of the Coriolis term is 1e-6 off.
+    h = 0.5, so the neighbouring orders' polynomials are far off."""
+    g = grid(8, 8, 89, 25, 5)
+    dt, h, n = 1e-3, 0.5, 4
+    st, u0 = _inertial_state(g)
+    omz = h / (2 * dt)
+    _rot_steps(g, st, ordr, dt, (0.0, 0.0, omz), n)
+    nrm = np.linalg.norm(u0)
+    U, V = _inertial_line_model(g, u0, ordr, dt, omz, n)
+    eu = np.linalg.norm(st.vx[0, 0] - U) / nrm
+    ev = np.linalg.norm(st.vy[0, 0] - V) / nrm
+    assert eu < 1e-9 and ev < 1e-9, (eu, ev)
+    U1, V1 = _inertial_line_model(g, u0, ordr, dt, omz, n, filtered=False)
+    assert np.linalg.norm(st.vx[0, 0] - U1) / nrm > 1e-8
+    assert np.abs(st.vx[1:]).max() == 0 and np.abs(st.vx[0, 1:]).max() == 0
+    assert np.abs(st.vz).max() < 1e-13 * nrm and np.abs(st.th).max() == 0
+    z = taylor(1j * h, ordr) ** n
+    assert np.linalg.norm(st.vx - z.real * u0) / nrm < 1e-5
+    assert np.linalg.norm(st.vy + z.imag * u0) / nrm < 1e-5
+    for other in (ordr - 1, ordr + 1):
+        assert abs(taylor(1j * h, other) ** n - z) > 1e-3
+
+
+def test_rotbouss_planted_coriolis_sign_fails():
+    g = grid(8, 8, 89, 25, 5)
+    dt, h, n = 1e-3, 0.5, 4
+    st, u0 = _inertial_state(g)
+    _rot_steps(g, st, 4, dt, (0.0, 0.0, h / (2 * dt)), n, sign=-1.0)
+    z = taylor(1j * h, 4) ** n
+    assert np.linalg.norm(st.vy + z.imag * u0) / np.linalg.norm(u0) > 0.5
+
+
+def test_rotbouss_coriolis_is_twice_omega_cross_v():
+    """The three Coriolis terms are the components of 2 Omega x v (checked
+    against numpy's cross product on a random state and a generic Omega)."""
+    g = grid(8, 8, 57, 25, 5)
+    rng = np.random.default_rng(4)
+    v = [rng.standard_normal(g.spec_shape) + 1j * rng.standard_normal(g.spec_shape)
+         for _ in range(3)]
+    z = zero_like(g)
+    st = O.HDState(*v, z, z, z, z)
+    omega = (0.3, -1.7, 2.9)
+    got = np.stack(O.coriolis_terms(st, omega), axis=-1)
+    ref = 2 * np.cross(np.array(omega), np.stack(v, axis=-1))
+    assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+def test_rotbouss_without_rotation_is_bouss_without_theta_filter():
+    """With Omega = 0 a rotating stage followed by the Boussinesq stage's
+    filter and real-space round trip of th (bouss_rkstep2.f90:54-59) is the
+    Boussinesq stage, bit for bit: the two solvers share every other step."""
+    g = grid(16, 16, 89, 25, 5)
+    u = ic.spectrum_peaked_velocity(16, 16, 89, 25, 9)
+    th = ic.thermal_perturbation(16, 16, 89, 25, 10)
+    z = zero_like(g)
+    base = O.HDState(*(O.fftp3d_real_to_complex(g, a.copy()) for a in u), z.copy(), z.copy(),
+                     z.copy(), z.copy(), O.fftp3d_real_to_complex(g, th.copy()), z.copy())
+    args = (3, 4, 1e-3, 2e-3, 3e-3, 1.5, 0.7)
+    s1 = O.HDState(*(getattr(base, f).copy() for f in ("vx", "vy", "vz", "pr", "fx", "fy",
+                                                        "fz", "th", "fs")))
+    s2 = O.HDState(*(getattr(base, f).copy() for f in ("vx", "vy", "vz", "pr", "fx", "fy",
+                                                        "fz", "th", "fs")))
+    C = (base.vx.copy(), base.vy.copy(), base.vz.copy(), base.th.copy())
+    O.rotbouss_substep(g, s1, *C, *args, (0.0, 0.0, 0.0))
+    O.fc_filter(g, s1.th)
+    R1 = O.fftp3d_complex_to_real(g, s1.th.copy()) / g.nx / g.ny / g.nz
+    s1.th[...] = O.fftp3d_real_to_complex(g, R1)
+    O.bouss_substep(g, s2, *C, *args)
+    for f in ("vx", "vy", "vz", "pr", "th"):
+        assert np.array_equal(getattr(s1, f), getattr(s2, f)), f
+
+
+# ---------------------------------------------------------------------------
 # the longdouble oracle (the extended-precision referee of the cfg1 fixture)
 # ---------------------------------------------------------------------------
 def test_longdouble_oracle_is_the_same_algorithm():
