@@ -259,6 +259,7 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
   p->mem = 1;
   p->stream = static_cast<hipStream_t>(stream);
   if (const char* e = std::getenv("SPX_FUSED_PROJECT")) p->fused_project = std::atoi(e) != 0;
+  if (const char* e = std::getenv("SPX_XFUSED")) p->xfused = std::atoi(e) != 0;
   if (hipGetDevice(&p->device) != hipSuccess) {
     delete p;
     return fail(SPX_ERR_HIP, "spx_plan_create: hipGetDevice failed");
@@ -573,7 +574,7 @@ namespace spx {
 // three derivatives and accumulates the products in the reference's order.
 // In mixed plans the whole pipeline runs in float32: the first z pass reads
 // the float64 state and the outputs are float32.
-int gradre_impl(spx_plan* p, const void* const v[3], void* const out[3]) {
+int gradre_impl(spx_plan* p, const void* const v[3], void* const out[3], bool keep_rv) {
   int rc;
   const int np = p->np;
   const bool lo = p->fp32;
@@ -583,12 +584,17 @@ int gradre_impl(spx_plan* p, const void* const v[3], void* const out[3]) {
     if ((rc = yfft(p, p->XV[c], 0, np, nullptr, p->XY[c], lo))) return rc;
     if ((rc = yfft(p, p->XZ[c], 0, np, nullptr, nullptr, lo))) return rc;
   }
-  for (int c = 0; c < 3; ++c)
-    if ((rc = xc2r(p, p->XV[c], p->RV[c], np, 0, lo))) return rc;
-  for (int c = 0; c < 3; ++c)
-    if ((rc = xgrad_prod(p, p->XV[c], p->XY[c], p->XZ[c], p->RV[0], p->RV[1], p->RV[2],
-                         p->RO[c], lo)))
-      return rc;
+  rc = p->xfused ? xgrad_fused(p, p->XV, p->XY, p->XZ, keep_rv ? p->RV : nullptr, p->RO, lo)
+                 : SPX_ERR_UNSUPPORTED;
+  if (rc != SPX_OK && rc != SPX_ERR_UNSUPPORTED) return rc;
+  if (rc == SPX_ERR_UNSUPPORTED) {
+    for (int c = 0; c < 3; ++c)
+      if ((rc = xc2r(p, p->XV[c], p->RV[c], np, 0, lo))) return rc;
+    for (int c = 0; c < 3; ++c)
+      if ((rc = xgrad_prod(p, p->XV[c], p->XY[c], p->XZ[c], p->RV[0], p->RV[1], p->RV[2],
+                           p->RO[c], lo)))
+        return rc;
+  }
   const bool bnd = lo && p->g.Cz > 0;
   const int d2 = 2 * p->g.oz;
   if (bnd) {
@@ -669,7 +675,7 @@ int spx_gradre(spx_plan* p, const void* a, const void* b, const void* c, void* d
   MemberView mv(p, p->nact);
   const void* v[3] = {a, b, c};
   void* o[3] = {d, e, f};
-  return gradre_impl(p, v, o);
+  return gradre_impl(p, v, o, false);
 }
 
 int spx_advect(spx_plan* p, const void* a, const void* b, const void* c, const void* dd,
@@ -1080,7 +1086,7 @@ static int substep_impl(spx_plan* p, spx_state* s, int o, bool full_pressure) {
   MemberView mv(p, p->nact);
   const void* vel[3] = {s->vx, s->vy, s->vz};
   void* nlo[3] = {s->N[0], s->N[1], s->N[2]};
-  if ((rc = gradre_impl(p, vel, nlo))) return rc;
+  if ((rc = gradre_impl(p, vel, nlo, bouss))) return rc;
   if (bouss && (rc = advect_impl(p, vel, s->th, s->N[3], true))) return rc;
   // rotating frame: 2 Omega x v joins the nonlinear term before the buoyancy
   // and the filter (include/rotbouss/rotbouss_rkstep2.f90:13-21)

@@ -136,6 +136,7 @@ struct spx_plan {
   // x passes see mem times as many planes.
   int nb = 1, nact = 1, mem = 1;
   int device = 0;           // HIP device current at creation
+  bool xfused = true;       // gradre's x stage in one launch (SPX_XFUSED=0: six)
   bool fused_project = true;  // sol_project in one pass per line (SPX_FUSED_PROJECT=0: off)
   int wkx() const { return nkx * mem; }
   long wlines() const { return long(g.ny) * nkx * mem; }
@@ -315,6 +316,10 @@ int xc2r(spx_plan* p, const void* X, void* R, int nplanes, int dirx = 0, bool lo
 int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, const void* u,
                const void* v, const void* w, void* out, bool lo = false, int nplanes = -1,
                int ncomp = 1);
+// the three components' x stage in one launch on the np planes: RV = c2r(YV)
+// (stored only when RV is given) and out_c as xgrad_prod, bit for bit
+int xgrad_fused(spx_plan* p, void* const* YV, void* const* YY, void* const* XZ, void* const* RV,
+                void* const* out, bool lo);
 // fused v_imposebc: goto_domain_w_boundaries + noslip_z + goto_3d_fourier on vx,vy
 // rk: the RK update of vx, vy (c[0], c[1]) is applied on load (null: none)
 // Launch-shape overrides set by an autotuner (spx_tune_*), consulted when a
@@ -363,8 +368,9 @@ struct LinComb {
   int sgn[3];    // +1 / -1 for terms 1,2 (term 0 always +1); 0 = term absent
 };
 // gradre (pseudospec_hd.f90:209-319) through the plan's current work arrays
-// XV, XZ, XY, RV, RO (and the boundary-plane buffers of mixed plans)
-int gradre_impl(spx_plan* p, const void* const v[3], void* const out[3]);
+// XV, XZ, XY, RV, RO (and the boundary-plane buffers of mixed plans); keep_rv:
+// leave v in real space in RV (advect on the same v reuses it)
+int gradre_impl(spx_plan* p, const void* const v[3], void* const out[3], bool keep_rv = true);
 
 int zline_sumsq(spx_plan* p, const LinComb& lc, double* line_out);
 int zline_dot(spx_plan* p, const void* a, const void* b, double* line_out);

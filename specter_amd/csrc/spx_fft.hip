@@ -593,6 +593,104 @@ __global__ void __launch_bounds__(kMaxBlock) k_xgrad_prod(const cpx<T>* __restri
   }
 }
 
+// gradre's whole x stage on a row pair (pseudospec_hd.f90:254-312): the
+// velocity components u, v, w = c2r(YV_c) are formed first and kept in
+// registers (and stored to rv[c] when given: advect reuses them), then for
+// each component c ((u Dx v_c + v Dy v_c) + w Dz v_c) tmp from the c2r of
+// i kx YV_c, YY_c and XZ_c, in k_xgrad_prod's operation order.  It saves the
+// traffic of the real velocity (written once and read nine times per stage
+// by xc2r x 3 + xgrad_prod x 3).  A line is owned by its tpl threads, so
+// the twelve transforms of a row pair synchronise with line_sync.
+template <typename T>
+struct XGradArgs {
+  const cpx<T>* yv[3];
+  const cpx<T>* yy[3];
+  const cpx<T>* xz[3];
+  T* rv[3];   // may be null
+  T* out[3];
+};
+
+template <typename T, int RS>
+__global__ void __launch_bounds__(kMaxBlock, sizeof(T) == 4 ? 4 : 2) k_xgrad_fused(
+    XGradArgs<T> a, LineFFT<cpx<T>> f, int nkx, long npairs, double tmpd,
+    const double* __restrict__ kx) {
+#pragma clang fp contract(off)
+  using Cp = cpx<T>;
+  SPX_PROLOGUE(Cp, f);
+  const int nx = f.n;
+  const T tmp = T(tmpd);
+  const long p = long(blockIdx.x) * f.nl + line;
+  const bool ok = p < npairs;
+  const long ro = long(nx) * (2 * (ok ? p : 0));
+  const long co = long(nkx) * (2 * (ok ? p : 0));
+  T vel[3][2][KEEP];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    load_c2r_pair(xl, a.yv[c] + co, nkx, nx, lane, f.tpl, ok, 0, kx);
+    // the first barrier also publishes the staged twiddles
+    if (c == 0) __syncthreads();
+    else line_sync(f.tpl);
+    fft_lds<T, RS>(xl, f, tws, lane);
+    line_sync(f.tpl);
+#pragma unroll
+    for (int q = 0; q < KEEP; ++q) {
+      const int x = lane + q * f.tpl;
+      vel[c][0][q] = vel[c][1][q] = T(0);
+      if (x < nx) {
+        const Cp z = xl[x];
+        vel[c][0][q] = z.x;
+        vel[c][1][q] = -z.y;
+        if (a.rv[c] && ok) {
+          a.rv[c][ro + x] = z.x;
+          a.rv[c][ro + x + nx] = -z.y;
+        }
+      }
+    }
+    line_sync(f.tpl);
+  }
+#pragma unroll 1
+  for (int c = 0; c < 3; ++c) {
+    T accA[KEEP], accB[KEEP];
+    const Cp* srcs[3] = {a.yv[c], a.yy[c], a.xz[c]};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      load_c2r_pair(xl, srcs[r] + co, nkx, nx, lane, f.tpl, ok, r == 0 ? 1 : 0, kx);
+      line_sync(f.tpl);
+      fft_lds<T, RS>(xl, f, tws, lane);
+      line_sync(f.tpl);
+#pragma unroll
+      for (int q = 0; q < KEEP; ++q) {
+        const int x = lane + q * f.tpl;
+        if (x < nx) {
+          const Cp z = xl[x];
+          const T va = z.x, vb = -z.y;
+          if (r == 0) {
+            accA[q] = vel[0][0][q] * va;
+            accB[q] = vel[0][1][q] * vb;
+          } else if (r == 1) {
+            accA[q] = accA[q] + vel[1][0][q] * va;
+            accB[q] = accB[q] + vel[1][1][q] * vb;
+          } else {
+            accA[q] = (accA[q] + vel[2][0][q] * va) * tmp;
+            accB[q] = (accB[q] + vel[2][1][q] * vb) * tmp;
+          }
+        }
+      }
+      line_sync(f.tpl);
+    }
+    if (ok) {
+#pragma unroll
+      for (int q = 0; q < KEEP; ++q) {
+        const int x = lane + q * f.tpl;
+        if (x < nx) {
+          a.out[c][ro + x] = accA[q];
+          a.out[c][ro + x + nx] = accB[q];
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // float64 boundary kernels; TS is the storage type of the fields
 // ---------------------------------------------------------------------------
@@ -1741,6 +1839,39 @@ static int xgrad_prod_t(spx_plan* p, const void* YV, const void* YY, const void*
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
+template <typename T>
+static int xgrad_fused_t(spx_plan* p, void* const* YV, void* const* YY, void* const* XZ,
+                         void* const* RV, void* const* out, const LineFFTHost& f) {
+  const int nplanes = p->np;
+  const long npairs = long(p->g.ny) * nplanes * p->mem / 2;
+  const double t = double(p->g.nx) * double(p->g.ny) * double(p->g.nz);
+  const double rows = double(p->g.ny) * nplanes * p->mem;
+  XGradArgs<T> a;
+  for (int c = 0; c < 3; ++c) {
+    a.yv[c] = C_<T>(static_cast<const void*>(YV[c]));
+    a.yy[c] = C_<T>(static_cast<const void*>(YY[c]));
+    a.xz[c] = C_<T>(static_cast<const void*>(XZ[c]));
+    a.rv[c] = RV ? static_cast<T*>(RV[c]) : nullptr;
+    a.out[c] = static_cast<T*>(out[c]);
+  }
+  ProfScope ps_(p, "xgrad_fused", rows * (9.0 * sizeof(cpx<T>) * p->nkx +
+                                          (RV ? 6.0 : 3.0) * sizeof(T) * p->g.nx));
+  hipLaunchKernelGGL((k_xgrad_fused<T, RS_POW2>), dim3(unsigned((npairs + f.nl - 1) / f.nl)),
+                     blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream, a, f.dev<cpx<T>>(), p->nkx,
+                     npairs, 1.0 / (t * t), p->d_kx);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+int xgrad_fused(spx_plan* p, void* const* YV, void* const* YY, void* const* XZ, void* const* RV,
+                void* const* out, bool lo) {
+  if (int rc = check_lo(p, lo)) return rc;
+  // power-of-two x lines only: the other radix families spill the velocity
+  // registers
+  if (rset_of(lo ? p->fx32 : p->fx) != RS_POW2) return SPX_ERR_UNSUPPORTED;
+  return lo ? xgrad_fused_t<float>(p, YV, YY, XZ, RV, out, p->fx32)
+            : xgrad_fused_t<double>(p, YV, YY, XZ, RV, out, p->fx);
+}
+
 int xgrad_prod(spx_plan* p, const void* YV, const void* YY, const void* XZ, const void* u,
                const void* v, const void* w, void* out, bool lo, int nplanes, int ncomp) {
   if (int rc = check_lo(p, lo)) return rc;

@@ -115,7 +115,7 @@ int spx_gradre_noalloc(spx_plan* p, const void* a, const void* b, const void* c,
     MemberView mv(p, p->nact);
     const void* v[3] = {a, b, c};
     void* o[3] = {d, e, f};
-    rc = gradre_impl(p, v, o);
+    rc = gradre_impl(p, v, o, false);
   }
   for (int i = 0; i < 21; ++i) *slots[i] = sv[i];
   return rc;

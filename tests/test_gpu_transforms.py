@@ -274,3 +274,53 @@ def test_fused_projection_matches_two_kernel_path(precision, monkeypatch):
             out.append({n: sol.spectral(n) for n in ("vx", "vy", "vz", "pr")})
         for n in ("vx", "vy", "vz", "pr"):
             assert np.array_equal(out[0][n], out[1][n]), (prm["solver"], n)
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_fused_x_stage_matches_six_launches(precision, monkeypatch):
+    """gradre's x stage in one launch (the default: the real velocity kept in
+    registers, stored only when advect reuses it) agrees with the three xc2r
+    + three xgrad_prod launches (SPX_XFUSED=0 at plan creation) to rounding
+    (the same operations; the compiler contracts the FFT's products into
+    fused multiply-adds differently in the two kernels): gradre alone, and
+    two RK4 steps of HD and Boussinesq (whose advect reads the real velocity
+    the fused stage stores)."""
+    import torch
+    from specter_amd import FCPlan, Solver, SolverParams, ic
+    nx, ny, nz, C, d = 32, 24, 57, 25, 5
+    u = ic.spectrum_peaked_velocity(nx, ny, nz, C, 31)
+    th = ic.thermal_perturbation(nx, ny, nz, C, 32)
+    rng = np.random.default_rng(33)
+    shape = (nx // 2 + 1, ny, nz)
+    abc = [torch.as_tensor(rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).cuda()
+           for _ in range(3)]
+    res = []
+    for fused in ("1", "0"):
+        monkeypatch.setenv("SPX_XFUSED", fused)
+        p = FCPlan(nx, ny, nz, C, d, precision=precision)
+        outs = [p.empty_nonlinear() for _ in range(3)]
+        p.gradre(*abc, *outs)
+        r = {"gradre": [o.cpu().numpy() for o in outs]}
+        for prm in (dict(solver="HD", ord=4, dt=1e-3, nu=2e-3),
+                    dict(solver="BOUSS", ord=4, dt=1e-3, nu=2e-3, kappa=2e-3, xmom=1.0,
+                         xtemp=1.0)):
+            sol = Solver(FCPlan(nx, ny, nz, C, d, precision=precision), SolverParams(**prm))
+            for name, arr in zip(("vx", "vy", "vz"), u):
+                sol.set_real_field(name, arr)
+            if prm["solver"] == "BOUSS":
+                sol.set_real_field("th", th)
+            sol.step(2)
+            names = ("vx", "vy", "vz", "th") if prm["solver"] == "BOUSS" else ("vx", "vy", "vz")
+            r[prm["solver"]] = [sol.spectral(n) for n in names]
+        res.append(r)
+    # gradre itself to rounding; the steps' differences grow through the
+    # pressure assembly (measured 1e-12 after two float64 RK4 steps), far
+    # below what a wrong real velocity for advect would give (O(1) in th)
+    tols = {"gradre": 1e-14 if precision == 0 else 1e-6,
+            "HD": 1e-10 if precision == 0 else 1e-4, "BOUSS": 1e-10 if precision == 0 else 1e-4}
+    for key in res[0]:
+        tol = tols[key]
+        for a, b in zip(res[0][key], res[1][key]):
+            e = rel_err(a, b)
+            print(f"{key}: fused vs six launches {e:.2e}")
+            assert e < tol, key
