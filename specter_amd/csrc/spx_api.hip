@@ -338,7 +338,9 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
     return bail(rc);
   // line FFT plans
   if ((rc = build_line_fft(p->fx, nx, 16, 0))) return bail(rc);
-  if ((rc = build_line_fft(p->fy, ny, 16, 16))) return bail(rc);
+  // y lines (stride nkx in memory): up to 512 threads per workgroup, so a row
+  // of a workgroup's load spans 8 consecutive kx at ny = 512
+  if ((rc = build_line_fft(p->fy, ny, 16, 16, 512, 80 * 1024))) return bail(rc);
   if ((rc = build_line_fft(p->fz, nz, 16, 16))) return bail(rc);
   p->fz2 = p->fz.with_lines(p->fz.nl / 2);
   if ((rc = build_wave_fft(p->fzw, nz, 16))) return bail(rc);
@@ -358,7 +360,7 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
   }
   if (p->fp32) {
     if ((rc = build_line_fft(p->fx32, nx, 8, 0))) return bail(rc);
-    if ((rc = build_line_fft(p->fy32, ny, 8, 16))) return bail(rc);
+    if ((rc = build_line_fft(p->fy32, ny, 8, 16, 512, 80 * 1024))) return bail(rc);
     if ((rc = build_line_fft(p->fz32, nz, 8, 16))) return bail(rc);
     if ((rc = build_wave_fft(p->fzw32, nz, 8))) return bail(rc);
     if (C > 0) {

@@ -278,7 +278,10 @@ struct ProfScope {
 // float32 kernels of a mixed plan instead when lo is true (zfft_s2x then reads
 // a float64 S and writes float32 X; the others read and write float32).
 // csize: bytes per complex element (16: float64 tables, 8: float32 tables)
-int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines);
+// want_lines: lines per workgroup at most (0: as many as max_block threads
+// hold); the tables plus line buffers stay within lds_budget bytes
+int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines, int max_block = 256,
+                   size_t lds_budget = 64 * 1024);
 // the one-wavefront-per-line plan of length n (f.wave < 0 if none is compiled)
 int build_wave_fft(LineFFTHost& f, int n, int csize);
 void free_line_fft(LineFFTHost& f);

@@ -1353,7 +1353,8 @@ void clear_tunes() {
   g_wall_tune.clear();
 }
 
-int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
+int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines, int max_block,
+                   size_t lds_budget) {
   std::vector<int> fac;
   f.n = n;
   if (factor_small(n, fac)) {
@@ -1395,8 +1396,7 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines) {
   if (f.mode == 0) f.tab = f.M;
   else if (f.mode == 1) f.tab = 2 * f.M + n;
   else f.tab = 2 * f.M + int((2 * size_t(f.M) * sizeof(int) + csize - 1) / csize);
-  const size_t lds_budget = 64 * 1024;
-  int nl = std::max(1, 256 / f.tpl);
+  int nl = std::max(1, max_block / f.tpl);
   if (want_lines > 0) nl = std::min(nl, want_lines);
   while (nl > 1 && (size_t(f.tab) + size_t(nl) * f.ldl) * csize > lds_budget) nl >>= 1;
   // Rader / Bluestein: leave the filter / chirp / index tables in global
