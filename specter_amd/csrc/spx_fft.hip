@@ -336,9 +336,10 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_s2x(const cpx<TI>* __restrict__
   const bool ok = i < nkx;
   const long base = long(n) * (j + long(ny) * (ok ? i : 0));
   // the line (times i k_dirn).  The dual transform needs it again: float
-  // kernels keep it in registers, float64 kernels read it again (the second
-  // read hits in L2 and keeps the registers free for occupancy)
-  constexpr bool KEEPV = DUAL && sizeof(T) == 4;
+  // kernels and the wave plans (whose occupancy LDS bounds) keep it in
+  // registers, the other float64 kernels read it again (the second read hits
+  // in L2 and keeps the registers free for occupancy)
+  constexpr bool KEEPV = DUAL && (sizeof(T) == 4 || RS >= RS_WAVE);
   constexpr int KN = keep_n<RS>();
   auto load = [&](int k) {
     Cp v = mkc<T>(0, 0);
@@ -768,7 +769,8 @@ __device__ __forceinline__ Coef neumann_coef(Cd b1, Cd b2, double kh, double Lz,
 }
 
 // d <- IFFT_z(d)*tmp + a_harm(z)  (:354-381 with laplace_z :631-675); d is float64
-template <int RS>
+// (TS = double: the kernels of this family are instantiated per type)
+template <typename TS, int RS>
 __global__ void __launch_bounds__(kMaxBlock) k_proj_pressure(Cd* __restrict__ dd,
                                                        const Cd* __restrict__ bc, LFd f,
                                                        long nlines, int ny, double tmp, double Lz,
@@ -788,7 +790,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_proj_pressure(Cd* __restrict__ dd
   cf.c1 = cf.c2 = mkc<double>(0, 0);
   if (ok) cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
   __syncthreads();
-  line_dft<double, RS>(xl, f, tws, lane);
+  zline_dft<double, RS>(xl, f, tws, lane);
   if (!ok) return;
   for (int k = lane; k < n; k += f.tpl) {
     Cd a;
@@ -1924,28 +1926,14 @@ int sconst_fused(spx_plan* p, void* th) {
 }
 
 int proj_pressure(spx_plan* p, void* d) {
-  const LineFFTHost& f = p->fz;
+  const LineFFTHost& f = fzt(p, false);
   const long nlines = p->wlines();
   ProfScope ps_(p, "proj_pressure", 2.0 * double(p->wnc()) * 16.0);
-  const int rs = rset_of(f);
   const dim3 grid(unsigned((nlines + f.nl - 1) / f.nl));
   const size_t lds = f.lds_bytes(16);
-  if (rs == RS_POW2)
-    hipLaunchKernelGGL(k_proj_pressure<RS_POW2>, grid, blk(f), lds, p->stream, C_<double>(d),
-                       C_<double>(p->bc), f.dev<Cd>(), nlines, p->g.ny, 1.0 / double(p->g.nz),
-                       p->g.Lz, p->d_khom, p->d_z);
-  else if (rs == RS_SMOOTH)
-    hipLaunchKernelGGL(k_proj_pressure<RS_SMOOTH>, grid, blk(f), lds, p->stream, C_<double>(d),
-                       C_<double>(p->bc), f.dev<Cd>(), nlines, p->g.ny, 1.0 / double(p->g.nz),
-                       p->g.Lz, p->d_khom, p->d_z);
-  else if (rs == RS_ALL)
-    hipLaunchKernelGGL(k_proj_pressure<RS_ALL>, grid, blk(f), lds, p->stream, C_<double>(d),
-                       C_<double>(p->bc), f.dev<Cd>(), nlines, p->g.ny, 1.0 / double(p->g.nz),
-                       p->g.Lz, p->d_khom, p->d_z);
-  else
-    hipLaunchKernelGGL(k_proj_pressure<RS_GTAB>, grid, blk(f), lds, p->stream, C_<double>(d),
-                       C_<double>(p->bc), f.dev<Cd>(), nlines, p->g.ny, 1.0 / double(p->g.nz),
-                       p->g.Lz, p->d_khom, p->d_z);
+  SPX_LAUNCH_Z(f, k_proj_pressure, double, , grid, blk(f), lds, p->stream, C_<double>(d),
+               C_<double>(p->bc), f.dev<Cd>(), nlines, p->g.ny, 1.0 / double(p->g.nz), p->g.Lz,
+               p->d_khom, p->d_z);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
