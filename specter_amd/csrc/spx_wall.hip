@@ -304,20 +304,20 @@ __global__ void __launch_bounds__(512, WPE) k_wall_mfma(
     // p = 2t + k/2, re/im k&1); e^{-2 pi i z r_p / n}
     const int zo = 2 * (li & 3) + (li >> 3), zc = (li >> 2) & 1, pc = g & 1;
     const int nrho = 2 + C;
-    double bv[KT];
+    // K step outer, kappa block inner: consecutive MFMAs accumulate into
+    // different blocks, and one B operand (delta) is live at a time
 #pragma unroll
-    for (int t = 0; t < KT; ++t) bv[t] = ds[(4 * t + g) * 16 + li];
+    for (int t = 0; t < KT; ++t) {
+      const double bt = ds[(4 * t + g) * 16 + li];
+      const unsigned r = unsigned(rpoint(2 * t + (g >> 1), np, nrho));
 #pragma unroll
-    for (int b = 0; b < KBW; ++b) {
-      const int bb = b0 + b;
-      if (bb < nbk) {
-        const unsigned z = unsigned(8 * bb + zo);
-#pragma unroll
-        for (int t = 0; t < KT; ++t) {
-          const unsigned r = unsigned(rpoint(2 * t + (g >> 1), np, nrho));
+      for (int b = 0; b < KBW; ++b) {
+        const int bb = b0 + b;
+        if (bb < nbk) {
+          const unsigned z = unsigned(8 * bb + zo);
           const Cd w = rl[modn(z * r, n, magic)];
           const double a = zc == pc ? w.x : (zc ? -w.y : w.y);
-          vr[b] = mfma_f64(a, bv[t], vr[b]);
+          vr[b] = mfma_f64(a, bt, vr[b]);
         }
       }
     }
