@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 
 #include "spx_engine.h"
@@ -1948,6 +1949,77 @@ int pressure_walls(spx_plan* p, void* d) {
   return SPX_OK;
 }
 
+// The two harmonic profiles of each distinct khom value, continued and
+// transformed on the host in long double and rounded once to float64: rows
+// 2u, 2u+1 of tab.  The table is reused by every line with that kh, every
+// stage and every step, so its rounding acts as a systematic error of the
+// projection; a float64 device FFT leaves several ulps in each entry and
+// moves cfg1 (64^3, 100 steps) 4x further from the exact trajectory than
+// per-line transforms do, a correctly rounded table does not.  Real input:
+// P[nz-k] = conj(P[k]).  Threads over the profiles.
+static void harmonic_table_ld(const std::vector<double>& khv, int n, int np, int C, int d,
+                              const std::vector<double>& z, double Lz,
+                              const std::vector<double>& dir, std::vector<double>& out) {
+  const long double PI2 = 6.283185307179586476925286766559005768L;
+  std::vector<long double> wr(n), wi(n);
+  for (int m = 0; m < n; ++m) {
+    wr[m] = cosl(PI2 * (long double)m / (long double)n);
+    wi[m] = -sinl(PI2 * (long double)m / (long double)n);
+  }
+  const int U = int(khv.size());
+  out.assign(size_t(2) * U * n * 2, 0.0);
+  auto work = [&](int u0, int u1) {
+    std::vector<long double> x(n);
+    for (int u = u0; u < u1; ++u) {
+      const long double kh = khv[u];
+      for (int which = 0; which < 2; ++which) {
+        for (int k = 0; k < np; ++k) {
+          const long double zk = z[k];
+          if (kh == 0.0L) x[k] = which == 0 ? zk : 1.0L;
+          else x[k] = which == 0 ? expl(kh * (zk - (long double)Lz)) : expl(-kh * zk);
+        }
+        // FC-Gram continuation (fftp.fpp:755-773 order)
+        const int top = np - d;
+        for (int ii = 0; ii < C; ++ii) {
+          const double* D0 = dir.data() + size_t(ii) * d;
+          const double* D1 = dir.data() + size_t(C - 1 - ii) * d;
+          long double a = (long double)D0[0] * x[top] + (long double)D1[0] * x[d - 1];
+          for (int jj = 1; jj < d; ++jj) {
+            a += (long double)D0[jj] * x[top + jj];
+            a += (long double)D1[jj] * x[d - 1 - jj];
+          }
+          x[np + ii] = a;
+        }
+        double* o = out.data() + (size_t(2) * u + which) * n * 2;
+        for (int k = 0; k <= n / 2; ++k) {
+          long double sr = 0, si = 0;
+          int idx = 0;
+          for (int m = 0; m < n; ++m) {
+            sr += x[m] * wr[idx];
+            si += x[m] * wi[idx];
+            idx += k;
+            if (idx >= n) idx -= n;
+          }
+          o[2 * k] = double(sr);
+          o[2 * k + 1] = double(si);
+          if (k > 0 && n - k != k) {
+            o[2 * (n - k)] = double(sr);
+            o[2 * (n - k) + 1] = double(-si);
+          }
+        }
+      }
+    }
+  };
+  const int nt = std::max(1, std::min(16, int(std::thread::hardware_concurrency())));
+  std::vector<std::thread> th;
+  const int chunk = (U + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    const int u0 = t * chunk, u1 = std::min(U, u0 + chunk);
+    if (u0 < u1) th.emplace_back(work, u0, u1);
+  }
+  for (auto& t : th) t.join();
+}
+
 int build_harmonic_table(spx_plan* p) {
   // SPX_HARMONIC_TABLE=0: no table (the harmonic kernel transforms per line;
   // for plans whose memory is tighter than 2 * (distinct kh) * nz complex values)
@@ -1985,18 +2057,30 @@ int build_harmonic_table(spx_plan* p) {
   SPX_HIP(hipMalloc(&kv, khv.size() * sizeof(double)));
   SPX_HIP(hipMemcpy(hid, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
   SPX_HIP(hipMemcpy(kv, khv.data(), khv.size() * sizeof(double), hipMemcpyHostToDevice));
-  const long tot = long(U) * np;
-  hipLaunchKernelGGL(k_harm_profiles, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, p->stream,
-                     static_cast<Cd*>(tab), static_cast<const double*>(kv), U, n, np, p->d_z,
-                     p->g.Lz);
-  SPX_CHECK_LAUNCH();
-  // continuation + forward z FFT of the 2U profile rows
-  const LineFFTHost& f = p->fz;
-  const long rows = 2L * U;
-  SPX_LAUNCH_RS(f, k_z_inplace, double, , dim3(unsigned((rows + f.nl - 1) / f.nl)), blk(f),
-                f.lds_bytes(16), p->stream, static_cast<Cd*>(tab), rows, f.dev<Cd>(), np, 1, 1,
-                1.0, p->d_dir, p->g.Cz, p->g.oz);
-  SPX_CHECK_LAUNCH();
+  // long double on the host (the default; about 2 U np nz / 2 multiply-adds,
+  // split over up to 16 threads), or the profiles continued and transformed
+  // on the device in float64 for tables whose host cost would be too high
+  // (U nz^2 > 2e11, e.g. 1024^3) or with SPX_HARMONIC_LD=0
+  bool host_ld = double(U) * double(n) * double(n) <= 2e11;
+  if (const char* e = std::getenv("SPX_HARMONIC_LD")) host_ld = e[0] != '0';
+  if (host_ld) {
+    std::vector<double> h;
+    harmonic_table_ld(khv, n, np, p->g.Cz, p->g.oz, p->h_z, p->g.Lz, p->h_dir, h);
+    SPX_HIP(hipMemcpy(tab, h.data(), tab_bytes, hipMemcpyHostToDevice));
+  } else {
+    const long tot = long(U) * np;
+    hipLaunchKernelGGL(k_harm_profiles, dim3(unsigned((tot + 255) / 256)), dim3(256), 0,
+                       p->stream, static_cast<Cd*>(tab), static_cast<const double*>(kv), U, n,
+                       np, p->d_z, p->g.Lz);
+    SPX_CHECK_LAUNCH();
+    // continuation + forward z FFT of the 2U profile rows
+    const LineFFTHost& f = p->fz;
+    const long rows = 2L * U;
+    SPX_LAUNCH_RS(f, k_z_inplace, double, , dim3(unsigned((rows + f.nl - 1) / f.nl)), blk(f),
+                  f.lds_bytes(16), p->stream, static_cast<Cd*>(tab), rows, f.dev<Cd>(), np, 1,
+                  1, 1.0, p->d_dir, p->g.Cz, p->g.oz);
+    SPX_CHECK_LAUNCH();
+  }
   SPX_HIP(hipStreamSynchronize(p->stream));
   (void)hipFree(kv);
   p->d_htab = tab;

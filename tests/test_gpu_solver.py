@@ -153,10 +153,12 @@ def test_cfg1_100_steps_parity(cfg1_oracle):
     longdouble, tests/golden/cfg1_longdouble_delta.npz) and against the
     float64 oracle.  The float64 oracle itself is 2.9-3.1e-11 from the exact
     trajectory (its wall update rounds every line through an inverse and a
-    forward z transform each stage); the GPU path, whose wall update is a
-    rank-(2+C) correction, must be closer than that and within 2.5e-11
-    (measured 1.0e-11, 1.0e-11, 1.9e-11).  GPU and float64 oracle then differ
-    by their two roundoff histories (measured 3.1-3.5e-11)."""
+    forward z transform each stage); the GPU path (wall update as a
+    rank-(2+C) correction, harmonic basis table built in long double) must be
+    within 1e-11 of the exact trajectory (measured 7.2e-13, 5.9e-13,
+    6.8e-13; with the table built by float64 device FFTs 1.0e-11, 1.0e-11,
+    1.9e-11).  GPU and float64 oracle then differ by the oracle's own
+    roundoff history (measured about 3e-11)."""
     _, st, sr = cfg1_oracle
     sol, sg = gpu_trajectory(CFG1["shape"], CFG1["prm"], CFG1["nsteps"], CFG1["seed"],
                              CFG1["diag_every"])
@@ -168,7 +170,7 @@ def test_cfg1_100_steps_parity(cfg1_oracle):
         f_ld = rel_l2(getattr(st, name).astype(np.clongdouble), ld[name])
         print(f"cfg1 {name} after 100 steps: gpu-vs-exact {e_ld:.3e}, float64 oracle-vs-exact "
               f"{f_ld:.3e}, gpu-vs-float64 oracle {err:.3e}")
-        assert e_ld < min(2.5e-11, f_ld), name
+        assert e_ld < 1e-11 and e_ld < f_ld, name
         assert err < 1e-10, name
     # energy, enstrophy and injection series
     for col, label in ((0, "energy"), (1, "enstrophy")):
@@ -207,3 +209,26 @@ def test_graph_replay_matches_eager(precision):
         for name in ("vx", "vy", "vz", "pr"):
             a, b = sols[0].spectral(name), sols[1].spectral(name)
             assert np.array_equal(a, b), (t, name)
+
+
+def test_harmonic_table_long_double_vs_float64(monkeypatch):
+    """The harmonic basis table built on the host in long double (default)
+    and by float64 device transforms (SPX_HARMONIC_LD=0) agree to float64
+    roundoff of the projection after one RK4 step; the long-double table is
+    the one that keeps long trajectories on the exact one (cfg1 test)."""
+    from specter_amd import FCPlan, Solver, SolverParams, ic
+    shape = (32, 32, 57, 25, 5)
+    nx, ny, nz, C, d = shape
+    u = ic.spectrum_peaked_velocity(nx, ny, nz, C, 41)
+    out = []
+    for ld in ("1", "0"):
+        monkeypatch.setenv("SPX_HARMONIC_LD", ld)
+        sol = Solver(FCPlan(nx, ny, nz, C, d), SolverParams(solver="HD", ord=4, dt=1e-3, nu=2e-3))
+        for name, arr in zip(("vx", "vy", "vz"), u):
+            sol.set_real_field(name, arr)
+        sol.step(1)
+        out.append({n: sol.spectral(n) for n in ("vx", "vy", "vz", "pr")})
+    for n in ("vx", "vy", "vz", "pr"):
+        e = rel_l2(out[0][n], out[1][n])
+        print(f"{n}: long-double table vs float64 table {e:.2e}")
+        assert e < 1e-12, n
