@@ -353,7 +353,11 @@ def _bench(args, w):
     out_names = ("vx", "vy", "vz", "th") if bouss else ("vx", "vy", "vz")
     modes = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
     _log(f"{args.workload}: initial fields")
-    real_ic = _initial_fields(w)
+    # generated on the GPU, parked on the host before the first plan exists
+    # (six float64 fields are 53 GB at 1024^3, beside a 230 GB plan + state)
+    real_ic = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v)
+               for k, v in _initial_fields(w).items()}
+    torch.cuda.empty_cache()
     res, host_ic, diag, zinfo = {}, None, {}, None
     for mode in modes:
         prec = SPX_FP32_MIXED if mode == "mixed" else SPX_FP64

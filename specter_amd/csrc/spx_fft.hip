@@ -1596,6 +1596,10 @@ static int lds_limit(K kernel, size_t bytes) {
       if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_WAVE + 0 TARGS>, (LDS))) return rc_;       \
       hipLaunchKernelGGL((KER<TY SPX_COMMA RS_WAVE + 0 TARGS>), GRID, BLOCK, LDS, STREAM,    \
                          __VA_ARGS__);                                                       \
+    } else if (rs_ == RS_WAVE + 1) {                                                         \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_WAVE + 1 TARGS>, (LDS))) return rc_;       \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_WAVE + 1 TARGS>), GRID, BLOCK, LDS, STREAM,    \
+                         __VA_ARGS__);                                                       \
     } else if (rs_ == RS_POW2) {                                                             \
       if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_POW2 TARGS>, (LDS))) return rc_;           \
       hipLaunchKernelGGL((KER<TY SPX_COMMA RS_POW2 TARGS>), GRID, BLOCK, LDS, STREAM,        \

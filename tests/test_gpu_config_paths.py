@@ -214,13 +214,16 @@ def test_plan_device_bytes_match_allocations():
         p.close()
 
 
-def test_wave_plan_matches_engine(monkeypatch):
-    """nz = 537 (the 512^3 grids) runs its z transforms on the compiled
-    one-wavefront-per-line Bluestein plan (1080 = 18*6*10); with SPX_WAVE=0
-    at plan creation the general engine runs them.  Both match the oracle,
-    and each other to rounding, in both precisions (the state transforms in
-    float64, gradre's transforms in the plan's precision)."""
-    shape = (8, 8, 537, 25, 5)
+@pytest.mark.parametrize("shape,plan", [((8, 8, 537, 25, 5), 0), ((8, 8, 539, 27, 8), 1)],
+                         ids=["537-bluestein", "539-direct"])
+def test_wave_plan_matches_engine(monkeypatch, shape, plan):
+    """nz = 537 (the 512^3 grids, d = 5) and 539 (d = 8) run their z
+    transforms on the compiled one-wavefront-per-line plans (Bluestein
+    1080 = 18*6*10; direct 539 = 11*7*7); with SPX_WAVE=0 at plan creation
+    the general engine runs them.  Both match the oracle, and each other to
+    rounding, in both precisions (the state transforms in float64, gradre's
+    transforms in the plan's precision; d = 8 gradre in float32 carries the
+    continuation's amplification of float32 noise)."""
     O, g, _ = _grid_plan(shape)
     rng = np.random.default_rng(11)
     R = rng.standard_normal(g.real_shape)
@@ -238,7 +241,7 @@ def test_wave_plan_matches_engine(monkeypatch):
             monkeypatch.setenv("SPX_WAVE", "0")
         for prec in (0, 1):
             _, _, p = _grid_plan(shape, prec)
-            assert p.info().z_wave_plan == (0 if wave else -1)
+            assert p.info().z_wave_plan == (plan if wave else -1)
             out = p.empty_spectral()
             p.fftp3d_real_to_complex(_cu(R), out)
             outr = p.empty_real()
@@ -247,7 +250,8 @@ def test_wave_plan_matches_engine(monkeypatch):
             p.gradre(_cu(a), _cu(b), _cu(c), *outs)
             res[wave, prec] = (out.cpu().numpy(), outr.cpu().numpy(),
                                [o.cpu().numpy().astype(complex) for o in outs])
-    for prec, tol in ((0, 1e-12), (1, 1e-5)):
+    d = shape[4]
+    for prec, tol in ((0, 1e-12), (1, 1e-5 if d == 5 else 1e-2)):
         w, e = res[True, prec], res[False, prec]
         assert rel_err(w[0], ref_r2c) < 1e-12 and rel_err(w[1], ref_c2r) < 1e-12
         for o, r in zip(w[2], ref_grad):
@@ -256,4 +260,4 @@ def test_wave_plan_matches_engine(monkeypatch):
         print(f"precision {prec}: wave vs engine r2c {rel_err(w[0], e[0]):.2e}, "
               f"c2r {rel_err(w[1], e[1]):.2e}, gradre {agree:.2e}")
         assert rel_err(w[0], e[0]) < 1e-14 and rel_err(w[1], e[1]) < 1e-14
-        assert agree < (1e-13 if prec == 0 else 2e-6)
+        assert agree < (1e-13 if prec == 0 else (2e-6 if d == 5 else 1e-3))
