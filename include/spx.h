@@ -78,6 +78,7 @@ typedef struct {
   int z_lines_per_block;
   int z_tables_global;     /* 1: Rader / Bluestein tables read from global memory, 0: LDS    */
   int z_wave_plan;         /* compiled one-wavefront-per-line plan of the z transforms, -1: none */
+  int z_wave_plan_lo;      /* ... of the float32 z transforms of a mixed plan, -1: none      */
 } spx_plan_info;
 
 /* Replaces fcgram_create_plan (fftp/fcgram_mod.f90:49-144) including

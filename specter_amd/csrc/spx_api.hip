@@ -436,6 +436,7 @@ int spx_plan_get_info(const spx_plan* p, spx_plan_info* info) {
   info->z_lines_per_block = p->fz.nl;
   info->z_tables_global = (p->fz.mode != 0 && p->fz.tab == p->fz.M) ? 1 : 0;
   info->z_wave_plan = p->fzw.wave;
+  info->z_wave_plan_lo = p->fp32 ? p->fzw32.wave : -1;
   return SPX_OK;
 }
 

@@ -58,7 +58,10 @@ template <typename T, int RS>
 __device__ __forceinline__ void zline_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, const cpx<T>* tw,
                                           int lane) {
   if constexpr (RS >= RS_WAVE) {
-    WavePlanOf<RS>::type::template dft<T>(x, lane, tw, f.bfil, f.chirp);
+    using P = typename WavePlanOf<RS>::type;
+    // float32-only plans are never built for float64 lines (wave_plan_for):
+    // their float64 instances are not compiled
+    if constexpr (sizeof(T) == 4 || P::f64) P::template dft<T>(x, lane, tw, f.bfil, f.chirp);
     __syncthreads();
   } else {
     line_dft<T, RS>(x, f, tw, lane);
@@ -1501,7 +1504,7 @@ int build_wave_fft(LineFFTHost& f, int n, int csize) {
     free_line_fft(h);
     return rc;
   }
-  const int w = wave_plan_for(h.n, h.M, h.mode);
+  const int w = wave_plan_for(h.n, h.M, h.mode, csize);
   if (w < 0) {
     free_line_fft(h);
     return SPX_OK;
@@ -1599,6 +1602,10 @@ static int lds_limit(K kernel, size_t bytes) {
     } else if (rs_ == RS_WAVE + 1) {                                                         \
       if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_WAVE + 1 TARGS>, (LDS))) return rc_;       \
       hipLaunchKernelGGL((KER<TY SPX_COMMA RS_WAVE + 1 TARGS>), GRID, BLOCK, LDS, STREAM,    \
+                         __VA_ARGS__);                                                       \
+    } else if (rs_ == RS_WAVE + 2) {                                                         \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_WAVE + 2 TARGS>, (LDS))) return rc_;       \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_WAVE + 2 TARGS>), GRID, BLOCK, LDS, STREAM,    \
                          __VA_ARGS__);                                                       \
     } else if (rs_ == RS_POW2) {                                                             \
       if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_POW2 TARGS>, (LDS))) return rc_;           \

@@ -34,7 +34,7 @@ class SpxPlanInfo(ctypes.Structure):
                 ("work_bytes_bouss", c_size_t), ("state_bytes", c_size_t),
                 ("state_bytes_bouss", c_size_t), ("z_threads_per_line", c_int),
                 ("z_lines_per_block", c_int), ("z_tables_global", c_int),
-                ("z_wave_plan", c_int)]
+                ("z_wave_plan", c_int), ("z_wave_plan_lo", c_int)]
 
 
 class SpxParams(ctypes.Structure):

@@ -214,13 +214,15 @@ def test_plan_device_bytes_match_allocations():
         p.close()
 
 
-@pytest.mark.parametrize("shape,plan", [((8, 8, 537, 25, 5), 0), ((8, 8, 539, 27, 8), 1)],
-                         ids=["537-bluestein", "539-direct"])
+@pytest.mark.parametrize("shape,plan", [((8, 8, 537, 25, 5), (0, 0)), ((8, 8, 539, 27, 8), (1, 1)),
+                                        ((8, 8, 1049, 25, 5), (-1, 2))],
+                         ids=["537-bluestein", "539-direct", "1049-bluestein-float32"])
 def test_wave_plan_matches_engine(monkeypatch, shape, plan):
-    """nz = 537 (the 512^3 grids, d = 5) and 539 (d = 8) run their z
-    transforms on the compiled one-wavefront-per-line plans (Bluestein
-    1080 = 18*6*10; direct 539 = 11*7*7); with SPX_WAVE=0 at plan creation
-    the general engine runs them.  Both match the oracle, and each other to
+    """nz = 537 (the 512^3 grids, d = 5), 539 (d = 8) and 1049 (1024^3; the
+    float32 lines only: eight float64 lines of 2100 points do not fit in
+    LDS) run their z transforms on the compiled one-wavefront-per-line plans
+    (Bluestein 1080 = 18*6*10; direct 539 = 11*7*7; Bluestein 2100 =
+    15*20*7); with SPX_WAVE=0 at plan creation the general engine runs them.  Both match the oracle, and each other to
     rounding, in both precisions (the state transforms in float64, gradre's
     transforms in the plan's precision; d = 8 gradre in float32 carries the
     continuation's amplification of float32 noise)."""
@@ -241,7 +243,9 @@ def test_wave_plan_matches_engine(monkeypatch, shape, plan):
             monkeypatch.setenv("SPX_WAVE", "0")
         for prec in (0, 1):
             _, _, p = _grid_plan(shape, prec)
-            assert p.info().z_wave_plan == (plan if wave else -1)
+            info = p.info()
+            assert info.z_wave_plan == (plan[0] if wave else -1)
+            assert info.z_wave_plan_lo == (plan[1] if wave and prec == 1 else -1)
             out = p.empty_spectral()
             p.fftp3d_real_to_complex(_cu(R), out)
             outr = p.empty_real()
