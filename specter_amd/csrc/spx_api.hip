@@ -341,6 +341,8 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
   // y lines (stride nkx in memory): up to 512 threads per workgroup, so a row
   // of a workgroup's load spans 8 consecutive kx at ny = 512
   if ((rc = build_line_fft(p->fy, ny, 16, 16, 512, 80 * 1024))) return bail(rc);
+  attach_xy_wave(p->fx, 16);
+  attach_xy_wave(p->fy, 16);
   if ((rc = build_line_fft(p->fz, nz, 16, 16))) return bail(rc);
   p->fz2 = p->fz.with_lines(p->fz.nl / 2);
   if ((rc = build_wave_fft(p->fzw, nz, 16))) return bail(rc);
@@ -361,6 +363,8 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
   if (p->fp32) {
     if ((rc = build_line_fft(p->fx32, nx, 8, 0))) return bail(rc);
     if ((rc = build_line_fft(p->fy32, ny, 8, 16, 512, 80 * 1024))) return bail(rc);
+    attach_xy_wave(p->fx32, 8);
+    attach_xy_wave(p->fy32, 8);
     if ((rc = build_line_fft(p->fz32, nz, 8, 16))) return bail(rc);
     if ((rc = build_wave_fft(p->fzw32, nz, 8))) return bail(rc);
     if (C > 0) {

@@ -284,6 +284,9 @@ int build_line_fft(LineFFTHost& f, int n, int csize, int want_lines, int max_blo
                    size_t lds_budget = 64 * 1024);
 // the one-wavefront-per-line plan of length n (f.wave < 0 if none is compiled)
 int build_wave_fft(LineFFTHost& f, int n, int csize);
+// mark an x / y line plan of 512 points (one wavefront per line) for the
+// compiled 512-point wave plan
+void attach_xy_wave(LineFFTHost& f, int csize);
 void free_line_fft(LineFFTHost& f);
 
 // z transforms in place on S (contiguous lines).

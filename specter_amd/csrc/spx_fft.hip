@@ -68,6 +68,21 @@ __device__ __forceinline__ void zline_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, c
   }
 }
 
+// Forward FFT of a direct (mode 0) line without the closing workgroup
+// barrier: the caller synchronises the line (line_sync); the wave plan or
+// the engine's passes.
+template <typename T, int RS>
+__device__ __forceinline__ void line_fft_nosync(cpx<T>* x, const LineFFT<cpx<T>>& f,
+                                                const cpx<T>* tw, int lane) {
+  if constexpr (RS >= RS_WAVE) {
+    using P = typename WavePlanOf<RS>::type;
+    static_assert(P::mode == 0, "direct plans only");
+    P::template dft<T>(x, lane, tw, nullptr, nullptr);
+  } else {
+    fft_lds<T, RS>(x, f, tw, lane);
+  }
+}
+
 // line elements a thread holds in registers in the z kernels (n <= KN * tpl)
 template <int RS>
 constexpr int keep_n() {
@@ -420,7 +435,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_y(const cpx<T>* Xin, cpx<T>* Xout
       if (k < ny) keep[q] = xl[k];
     }
   }
-  line_dft<T, RS>(xl, f, tws, lane);
+  zline_dft<T, RS>(xl, f, tws, lane);
   if (ii < nkx)
     for (int jy = tid >> f.lgnl; jy < ny; jy += f.tpl) {
       Cp v = lds[l * f.ldl + jy];
@@ -435,7 +450,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_y(const cpx<T>* Xin, cpx<T>* Xout
       if (k < ny) xl[k] = cconj(ikmul(T(ky[k]), cconj(keep[q])));
     }
     __syncthreads();
-    line_dft<T, RS>(xl, f, tws, lane);
+    zline_dft<T, RS>(xl, f, tws, lane);
     if (ii < nkx)
       for (int jy = tid >> f.lgnl; jy < ny; jy += f.tpl)
         Y2[base + ii + long(nkx) * jy] = cconj(lds[l * f.ldl + jy]);
@@ -456,7 +471,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_xr2c(const T* __restrict__ R, cpx
   const T* r0 = R + long(nx) * (2 * (ok ? p : 0));
   for (int x = lane; x < nx; x += f.tpl) xl[x] = ok ? mkc<T>(r0[x], r0[x + nx]) : mkc<T>(0, 0);
   __syncthreads();
-  line_dft<T, RS>(xl, f, tws, lane);
+  zline_dft<T, RS>(xl, f, tws, lane);
   if (ok) {
     Cp* o0 = X + long(nkx) * (2 * p);
     const T h = T(0.5);
@@ -510,7 +525,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_xc2r(const cpx<T>* __restrict__ X
   const bool ok = p < npairs;
   load_c2r_pair(xl, X + long(nkx) * (2 * (ok ? p : 0)), nkx, nx, lane, f.tpl, ok, dirx, kx);
   __syncthreads();
-  line_dft<T, RS>(xl, f, tws, lane);
+  zline_dft<T, RS>(xl, f, tws, lane);
   if (ok) {
     T* r0 = R + long(nx) * (2 * p);
     for (int x = lane; x < nx; x += f.tpl) {
@@ -565,7 +580,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_xgrad_prod(const cpx<T>* __restri
       mb[q] = in ? m[x + nx] : T(0);
     }
     __syncthreads();
-    line_dft<T, RS>(xl, f, tws, lane);
+    zline_dft<T, RS>(xl, f, tws, lane);
 #pragma unroll
     for (int q = 0; q < KEEP; ++q) {
       const int x = lane + q * f.tpl;
@@ -635,7 +650,7 @@ __global__ void __launch_bounds__(kMaxBlock, sizeof(T) == 4 ? 4 : 2) k_xgrad_fus
     // the first barrier also publishes the staged twiddles
     if (c == 0) __syncthreads();
     else line_sync(f.tpl);
-    fft_lds<T, RS>(xl, f, tws, lane);
+    line_fft_nosync<T, RS>(xl, f, tws, lane);
     line_sync(f.tpl);
 #pragma unroll
     for (int q = 0; q < KEEP; ++q) {
@@ -661,7 +676,7 @@ __global__ void __launch_bounds__(kMaxBlock, sizeof(T) == 4 ? 4 : 2) k_xgrad_fus
     for (int r = 0; r < 3; ++r) {
       load_c2r_pair(xl, srcs[r] + co, nkx, nx, lane, f.tpl, ok, r == 0 ? 1 : 0, kx);
       line_sync(f.tpl);
-      fft_lds<T, RS>(xl, f, tws, lane);
+      line_fft_nosync<T, RS>(xl, f, tws, lane);
       line_sync(f.tpl);
 #pragma unroll
       for (int q = 0; q < KEEP; ++q) {
@@ -1504,8 +1519,9 @@ int build_wave_fft(LineFFTHost& f, int n, int csize) {
     free_line_fft(h);
     return rc;
   }
+  // the z launch macro lists plans 0..2 (plan 3 is the x / y lines' plan)
   const int w = wave_plan_for(h.n, h.M, h.mode, csize);
-  if (w < 0) {
+  if (w < 0 || w > 2) {
     free_line_fft(h);
     return SPX_OK;
   }
@@ -1524,6 +1540,13 @@ int build_wave_fft(LineFFTHost& f, int n, int csize) {
   }
   f = h;
   return SPX_OK;
+}
+
+void attach_xy_wave(LineFFTHost& f, int csize) {
+  for (const char* v : {"SPX_WAVE", "SPX_WAVE_XY"})
+    if (const char* e = std::getenv(v))
+      if (e[0] == '0') return;
+  if (f.mode == 0 && f.tpl == 64 && wave_plan_for(f.n, f.M, 0, csize) == 3) f.wave = 3;
 }
 
 void free_line_fft(LineFFTHost& f) {
@@ -1590,6 +1613,36 @@ static int lds_limit(K kernel, size_t bytes) {
     }                                                                                        \
   } while (0)
 
+// x / y transform launch: the 512-point wave plan (RS_WAVE + 3) or the
+// general families (spelled out: TARGS cannot be forwarded to another macro)
+#define SPX_LAUNCH_XY(F, KER, TY, TARGS, GRID, BLOCK, LDS, STREAM, ...)                      \
+  do {                                                                                       \
+    const int rs_ = rset_of(F);                                                              \
+    if (rs_ == RS_WAVE + 3) {                                                                \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_WAVE + 3 TARGS>, (LDS))) return rc_;       \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_WAVE + 3 TARGS>), GRID, BLOCK, LDS, STREAM,    \
+                         __VA_ARGS__);                                                       \
+    } else if (rs_ == RS_POW2) {                                                             \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_POW2 TARGS>, (LDS))) return rc_;           \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_POW2 TARGS>), GRID, BLOCK, LDS, STREAM,        \
+                         __VA_ARGS__);                                                       \
+    } else if (rs_ == RS_SMOOTH) {                                                           \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_SMOOTH TARGS>, (LDS))) return rc_;         \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_SMOOTH TARGS>), GRID, BLOCK, LDS, STREAM,      \
+                         __VA_ARGS__);                                                       \
+    } else if (rs_ == RS_ALL) {                                                              \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_ALL TARGS>, (LDS))) return rc_;            \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_ALL TARGS>), GRID, BLOCK, LDS, STREAM,         \
+                         __VA_ARGS__);                                                       \
+    } else if (rs_ == RS_GTAB) {                                                             \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_GTAB TARGS>, (LDS))) return rc_;           \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_GTAB TARGS>), GRID, BLOCK, LDS, STREAM,        \
+                         __VA_ARGS__);                                                       \
+    } else {                                                                                 \
+      return fail(SPX_ERR_STATE, "x/y line plan without a compiled kernel family");          \
+    }                                                                                        \
+  } while (0)
+
 // z-transform launch: the wave-plan instance when the plan has one, else as
 // SPX_LAUNCH_RS (spelled out: TARGS cannot be forwarded to another macro)
 #define SPX_LAUNCH_Z(F, KER, TY, TARGS, GRID, BLOCK, LDS, STREAM, ...)                       \
@@ -1606,6 +1659,10 @@ static int lds_limit(K kernel, size_t bytes) {
     } else if (rs_ == RS_WAVE + 2) {                                                         \
       if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_WAVE + 2 TARGS>, (LDS))) return rc_;       \
       hipLaunchKernelGGL((KER<TY SPX_COMMA RS_WAVE + 2 TARGS>), GRID, BLOCK, LDS, STREAM,    \
+                         __VA_ARGS__);                                                       \
+    } else if (rs_ == RS_WAVE + 3) {                                                         \
+      if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_WAVE + 3 TARGS>, (LDS))) return rc_;       \
+      hipLaunchKernelGGL((KER<TY SPX_COMMA RS_WAVE + 3 TARGS>), GRID, BLOCK, LDS, STREAM,    \
                          __VA_ARGS__);                                                       \
     } else if (rs_ == RS_POW2) {                                                             \
       if (int rc_ = lds_limit(KER<TY SPX_COMMA RS_POW2 TARGS>, (LDS))) return rc_;           \
@@ -1782,12 +1839,12 @@ static int yfft_t(spx_plan* p, void* X, int forward, int nplanes, const void* Xi
   if (Y2) {
     if (forward) return fail(SPX_ERR_ARG, "dual y transform is backward only");
     ProfScope ps_(p, "y_dual", io * 3.0);
-    SPX_LAUNCH_RS(f, k_y, T, SPX_COMMA true, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
+    SPX_LAUNCH_XY(f, k_y, T, SPX_COMMA true, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
                   p->stream, in, C_<T>(X), C_<T>(Y2), f.dev<cpx<T>>(), p->nkx, p->g.ny, forward,
                   p->d_ky);
   } else {
     ProfScope ps_(p, "y", io * 2.0);
-    SPX_LAUNCH_RS(f, k_y, T, SPX_COMMA false, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
+    SPX_LAUNCH_XY(f, k_y, T, SPX_COMMA false, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
                   p->stream, in, C_<T>(X), static_cast<cpx<T>*>(nullptr), f.dev<cpx<T>>(),
                   p->nkx, p->g.ny, forward, p->d_ky);
   }
@@ -1806,7 +1863,7 @@ static int xr2c_t(spx_plan* p, const void* R, void* X, int nplanes, const LineFF
   dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
   ProfScope ps_(p, "xr2c", double(p->g.ny) * nplanes * p->mem *
                                (double(sizeof(T)) * p->g.nx + double(sizeof(cpx<T>)) * p->nkx));
-  SPX_LAUNCH_RS(f, k_xr2c, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
+  SPX_LAUNCH_XY(f, k_xr2c, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
                 static_cast<const T*>(R), C_<T>(X), f.dev<cpx<T>>(), p->nkx, npairs);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
@@ -1823,7 +1880,7 @@ static int xc2r_t(spx_plan* p, const void* X, void* R, int nplanes, int dirx,
   dim3 grid(unsigned((npairs + f.nl - 1) / f.nl));
   ProfScope ps_(p, "xc2r", double(p->g.ny) * nplanes * p->mem *
                                (double(sizeof(T)) * p->g.nx + double(sizeof(cpx<T>)) * p->nkx));
-  SPX_LAUNCH_RS(f, k_xc2r, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
+  SPX_LAUNCH_XY(f, k_xc2r, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
                 C_<T>(X), static_cast<T*>(R), f.dev<cpx<T>>(), p->nkx, npairs, dirx, p->d_kx);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
@@ -1846,7 +1903,7 @@ static int xgrad_prod_t(spx_plan* p, const void* YV, const void* YY, const void*
   const double rows = double(p->g.ny) * nplanes * p->mem * ncomp;
   ProfScope ps_(p, "xgrad_prod", rows * (3.0 * sizeof(cpx<T>) * p->nkx +
                                          4.0 * sizeof(T) * p->g.nx));
-  SPX_LAUNCH_RS(f, k_xgrad_prod, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
+  SPX_LAUNCH_XY(f, k_xgrad_prod, T, , grid, blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream,
                 C_<T>(YV), C_<T>(YY), C_<T>(XZ), static_cast<const T*>(u),
                 static_cast<const T*>(v), static_cast<const T*>(w), static_cast<T*>(out),
                 f.dev<cpx<T>>(), p->nkx, npairs, 1.0 / (t * t), p->d_kx, cstride, rstride);
@@ -1870,9 +1927,14 @@ static int xgrad_fused_t(spx_plan* p, void* const* YV, void* const* YY, void* co
   }
   ProfScope ps_(p, "xgrad_fused", rows * (9.0 * sizeof(cpx<T>) * p->nkx +
                                           (RV ? 6.0 : 3.0) * sizeof(T) * p->g.nx));
-  hipLaunchKernelGGL((k_xgrad_fused<T, RS_POW2>), dim3(unsigned((npairs + f.nl - 1) / f.nl)),
-                     blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream, a, f.dev<cpx<T>>(), p->nkx,
-                     npairs, 1.0 / (t * t), p->d_kx);
+  if (f.wave == 3)
+    hipLaunchKernelGGL((k_xgrad_fused<T, RS_WAVE + 3>), dim3(unsigned((npairs + f.nl - 1) / f.nl)),
+                       blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream, a, f.dev<cpx<T>>(),
+                       p->nkx, npairs, 1.0 / (t * t), p->d_kx);
+  else
+    hipLaunchKernelGGL((k_xgrad_fused<T, RS_POW2>), dim3(unsigned((npairs + f.nl - 1) / f.nl)),
+                       blk(f), f.lds_bytes(sizeof(cpx<T>)), p->stream, a, f.dev<cpx<T>>(),
+                       p->nkx, npairs, 1.0 / (t * t), p->d_kx);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
@@ -1881,7 +1943,8 @@ int xgrad_fused(spx_plan* p, void* const* YV, void* const* YY, void* const* XZ, 
   if (int rc = check_lo(p, lo)) return rc;
   // power-of-two x lines only: the other radix families spill the velocity
   // registers
-  if (rset_of(lo ? p->fx32 : p->fx) != RS_POW2) return SPX_ERR_UNSUPPORTED;
+  const int rs = rset_of(lo ? p->fx32 : p->fx);
+  if (rs != RS_POW2 && rs != RS_WAVE + 3) return SPX_ERR_UNSUPPORTED;
   return lo ? xgrad_fused_t<float>(p, YV, YY, XZ, RV, out, p->fx32)
             : xgrad_fused_t<double>(p, YV, YY, XZ, RV, out, p->fx);
 }
