@@ -137,17 +137,27 @@ class Grid:
     # longdouble, and the arithmetic -- transforms included -- runs in
     # longdouble)
     precision: str = "float64"
+    # non-periodic y (FC-Gram continuation along y in the 3D transforms):
+    # Cy continuation points, matching order oy; Ly is then the physical length
+    Cy: int = 0
+    oy: int = 0
+    diry: np.ndarray = field(default=None, repr=False)
 
     def __post_init__(self):
         nx, ny, nz, Cz = self.nx, self.ny, self.nz, self.Cz
         self.nkx = nx // 2 + 1
         self.np_ = nz - Cz
-        # specter.fpp:683-689 (x periodic), 705-711 (y periodic), 743-749 (z FC)
+        self.npy = ny - self.Cy
+        # specter.fpp:683-689 (x periodic), 705-723 (y periodic or FC), 743-749 (z FC)
         rmp = self.Lx * 2.0 * PI / nx
         self.Dkx = 1.0 / self.Lx
         self.x = rmp * np.arange(nx, dtype=np.float64)
-        rmp = self.Ly * 2.0 * PI / ny
-        self.Dky = 1.0 / self.Ly
+        if self.Cy == 0:
+            rmp = self.Ly * 2.0 * PI / ny
+            self.Dky = 1.0 / self.Ly
+        else:
+            rmp = self.Ly / (ny - self.Cy - 1)
+            self.Dky = 2.0 * PI / (rmp * ny)
         self.y = rmp * np.arange(ny, dtype=np.float64)
         rmp = self.Lz / (nz - Cz - 1)
         self.dz = rmp
@@ -157,7 +167,7 @@ class Grid:
         kx = np.arange(nx, dtype=np.float64)
         kx[nx // 2:] -= nx
         ky = np.arange(ny, dtype=np.float64)
-        ky[ny // 2:] -= ny
+        ky[(ny + 1) // 2:] -= ny  # parity note: standard order for odd ny
         kz = np.arange(nz, dtype=np.float64)
         kz[(nz + 1) // 2:] -= nz  # parity note: standard order for odd nz
         self.kx = (kx * self.Dkx)[: self.nkx]
@@ -184,12 +194,16 @@ class Grid:
 
 
 def make_grid(nx, ny, nz, Cz, d, tdir, Lx=1.0, Ly=1.0, Lz=1.0,
-              precision="float64") -> Grid:
+              precision="float64", Cy=0, oy=0) -> Grid:
     """fcgram_create_plan (fcgram_mod.f90:49-144) + grid setup.  dir = A*Q^T
-    is formed in float64 in both precisions (fcgram_mod.f90:26)."""
-    g = Grid(nx, ny, nz, Cz, d, Lx, Ly, Lz, precision=precision)
+    is formed in float64 in both precisions (fcgram_mod.f90:26); with Cy > 0
+    the y tables A{Cy}-{oy}, Q{oy} as well (:136-151)."""
+    g = Grid(nx, ny, nz, Cz, d, Lx, Ly, Lz, precision=precision, Cy=Cy, oy=oy)
     A, Q = load_tables(tdir, Cz, d)
     g.dirm = dir_matrix(A, Q).astype(g.rdtype)
+    if Cy > 0:
+        Ay, Qy = load_tables(tdir, Cy, oy)
+        g.diry = dir_matrix(Ay, Qy).astype(g.rdtype)
     return g
 
 
@@ -238,17 +252,67 @@ def fftp2d_complex_to_real_xy(g: Grid, S: np.ndarray) -> np.ndarray:
                        workers=_workers())
 
 
+def fc_continue_y(g: Grid, S: np.ndarray) -> None:
+    """FC-Gram Dirichlet continuation along y of the physical z planes, in
+    place on S (nkx, ny, nz) (fftp.fpp:565-581): the Cy slots from the first
+    oy and the last oy physical y points, in the reference's order."""
+    ny, C, d, p = g.ny, g.Cy, g.oy, g.np_
+    D = g.diry
+    top = ny - C - d
+    for ii in range(C):
+        acc = D[ii, 0] * S[:, top, :p] + D[C - 1 - ii, 0] * S[:, d - 1, :p]
+        for jj in range(1, d):
+            acc = acc + D[ii, jj] * S[:, top + jj, :p]
+            acc = acc + D[C - 1 - ii, jj] * S[:, d - 1 - jj, :p]
+        S[:, ny - C + ii, :p] = acc
+
+
+def fftp1d_real_to_complex_x(g: Grid, R: np.ndarray) -> np.ndarray:
+    """x r2c of every row, transposed to (kx, y, z) (fftp.fpp:620-716)."""
+    X = sfft.rfft(R, axis=2, workers=_workers())             # (nz, ny, nkx)
+    return np.ascontiguousarray(X.transpose(2, 1, 0))        # (nkx, ny, nz)
+
+
+def fftp2d_real_to_complex_yz(g: Grid, S: np.ndarray) -> None:
+    """y continuation (physical z planes), z continuation (every y), then the
+    2D (y, z) FFT, in place (fftp.fpp:527-616)."""
+    fc_continue_y(g, S)
+    fc_continue(g, S)
+    S[...] = sfft.fftn(S, axes=(1, 2), workers=_workers())
+
+
+def fftp2d_complex_to_real_yz(g: Grid, S: np.ndarray) -> None:
+    """Unnormalised 2D (y, z) backward DFT in place (fftp.fpp:922-956)."""
+    S[...] = sfft.ifftn(S, axes=(1, 2), norm="forward", workers=_workers())
+
+
+def fftp1d_complex_to_real_x(g: Grid, S: np.ndarray) -> np.ndarray:
+    """Transpose to (z, y, kx) and unnormalised x c2r of every row; like
+    FFTW's c2r the imaginary parts of kx = 0 and nx/2 are ignored
+    (fftp.fpp:961-1056)."""
+    X = np.ascontiguousarray(S.transpose(2, 1, 0))
+    return sfft.irfft(X, n=g.nx, axis=2, norm="forward", workers=_workers())
+
+
 def fftp3d_real_to_complex(g: Grid, R: np.ndarray) -> np.ndarray:
-    """fftp.fpp:388-425 (Cy=0 branch :419-421): 2D xy r2c, continuation, z FFT.
-    Continuation planes of the real input are ignored."""
+    """fftp.fpp:388-425: Cy = 0 (:419-421): 2D xy r2c, continuation, z FFT;
+    Cy > 0 (:416-418): x r2c, y and z continuation, 2D yz FFT.
+    Continuation planes and rows of the real input are ignored."""
+    if g.Cy > 0:
+        S = fftp1d_real_to_complex_x(g, R)
+        fftp2d_real_to_complex_yz(g, S)
+        return S
     S = fftp2d_real_to_complex_xy(g, R)
     fftp1d_real_to_complex_z(g, S)
     return S
 
 
 def fftp3d_complex_to_real(g: Grid, S: np.ndarray) -> np.ndarray:
-    """fftp.fpp:789-821: z IFFT (in place on the input, destroying it) then
-    2D xy c2r."""
+    """fftp.fpp:789-821: z IFFT then 2D xy c2r (Cy = 0), or 2D yz IFFT then
+    x c2r (Cy > 0); in place on the input, destroying it."""
+    if g.Cy > 0:
+        fftp2d_complex_to_real_yz(g, S)
+        return fftp1d_complex_to_real_x(g, S)
     fftp1d_complex_to_real_z(g, S)
     return fftp2d_complex_to_real_xy(g, S)
 
