@@ -121,6 +121,23 @@ int spx_plan_sync(spx_plan* p);
 int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, int nmembers,
                           void* stream, spx_plan** out);
 int spx_plan_get_members(const spx_plan* p, int* nmembers, int* nactive);
+
+/* ---- non-periodic y.  fcgram_create_plan with Cy > 0 (fcgram_mod.f90:49-144,
+ * the y tables :115-126; grid specter.fpp:706-723): as spx_plan_create_batch,
+ * and the forward 3D transforms continue every y line of the physical z
+ * planes by FC-Gram (fftp.fpp:565-581) before the y FFT.  g->ny INCLUDES the Cy
+ * continuation points (ny must stay even: the x kernels pack row pairs), g->Ly
+ * is the physical y length, dy = Ly/(ny-Cy-1), Dky = 2 pi/(dy ny), and ky runs
+ * in standard DFT order.  Ay is the raw A{Cy}-{oy}.dat table (Cy x oy, Fortran
+ * order), Qy the raw Q{oy}.dat table.  Cy = 0 (oy = 0, tables NULL) is
+ * spx_plan_create_batch.  Plans with Cy > 0 are SPX_FP64 only, and the
+ * boundary-condition calls and the solvers (spx_sol_project,
+ * spx_v_imposebc_and_project, spx_s_imposebc, spx_state_create) return
+ * SPX_ERR_UNSUPPORTED, as the reference stops on y boundary conditions
+ * (boundary_mod.fpp:104-105, :179-180). */
+int spx_plan_create_fcy(const spx_grid* g, const double* A, const double* Q, int Cy, int oy,
+                        const double* Ay, const double* Qy, int nmembers, void* stream,
+                        spx_plan** out);
 int spx_plan_set_active(spx_plan* p, int nactive);
 
 /* ---- transforms (fftp/fftp.fpp) ------------------------------------------ */
@@ -140,6 +157,20 @@ int spx_fftp1d_real_to_complex_z(spx_plan* p, void* inout_spec);
 int spx_fftp1d_complex_to_real_z(spx_plan* p, void* inout_spec);
 /* The continuation loop alone (fftp.fpp:755-773), in place. */
 int spx_fc_continue(spx_plan* p, void* inout_spec);
+/* The pieces of the non-periodic-y forward 3D transform (fftp.fpp:416-418) and
+ * their backward counterparts, on the S layout (the reference's backward 3D
+ * transform runs z IFFT + 2D xy c2r for every Cy, :817-818, which equals the
+ * yz IFFT + x c2r up to rounding); with Cy = 0 the yz transforms have no y
+ * continuation.
+ * fftp1d_real_to_complex_x (fftp.fpp:620-716): x r2c of all nz planes, S layout. */
+int spx_fftp1d_real_to_complex_x(spx_plan* p, const void* in_real, void* out_spec);
+/* fftp2d_real_to_complex_yz (fftp.fpp:527-616): y continuation of the physical
+ * z planes, z continuation, 2D (y, z) FFT, in place. */
+int spx_fftp2d_real_to_complex_yz(spx_plan* p, void* inout_spec);
+/* fftp2d_complex_to_real_yz (fftp.fpp:922-956): 2D (y, z) IFFT, in place. */
+int spx_fftp2d_complex_to_real_yz(spx_plan* p, void* inout_spec);
+/* fftp1d_complex_to_real_x (fftp.fpp:961-1056): x c2r of all nz planes. */
+int spx_fftp1d_complex_to_real_x(spx_plan* p, const void* in_spec, void* out_real);
 
 /* ---- spectral operators (pseudo/pseudospec_hd.f90, pseudospec_phd.f90) ---- */
 int spx_derivk(spx_plan* p, const void* a, void* b, int dir);      /* :28-94 */

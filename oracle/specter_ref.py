@@ -308,11 +308,8 @@ def fftp3d_real_to_complex(g: Grid, R: np.ndarray) -> np.ndarray:
 
 
 def fftp3d_complex_to_real(g: Grid, S: np.ndarray) -> np.ndarray:
-    """fftp.fpp:789-821: z IFFT then 2D xy c2r (Cy = 0), or 2D yz IFFT then
-    x c2r (Cy > 0); in place on the input, destroying it."""
-    if g.Cy > 0:
-        fftp2d_complex_to_real_yz(g, S)
-        return fftp1d_complex_to_real_x(g, S)
+    """fftp.fpp:789-821: z IFFT then 2D xy c2r for every Cy (:817-818); in
+    place on the input, destroying it."""
     fftp1d_complex_to_real_z(g, S)
     return fftp2d_complex_to_real_xy(g, S)
 

@@ -146,7 +146,7 @@ static int r2c3(spx_plan* p, const void* in, void* out, bool lo = false,
                 const void* xb = nullptr) {
   int rc;
   if ((rc = xr2c(p, in, p->X, p->np, lo))) return rc;
-  if ((rc = yfft(p, p->X, 1, p->np, nullptr, nullptr, lo))) return rc;
+  if ((rc = yfft(p, p->X, 1, p->np, nullptr, nullptr, lo, p->Cy > 0))) return rc;
   return zfft_x2s(p, p->X, out, 0, lo, xb);
 }
 
@@ -229,18 +229,57 @@ int spx_last_error(char* buf, size_t len) {
   return int(g_err.size());
 }
 
+// dir = A*Q^T (fcgram_mod.f90:250-252), textbook sum in k order, row-major [C][d]
+static std::vector<double> dir_matrix(const double* A, const double* Q, int C, int d) {
+  std::vector<double> dir(size_t(C) * d, 0.0);
+  for (int ii = 0; ii < C; ++ii)
+    for (int jj = 0; jj < d; ++jj) {
+      double s = 0.0;
+      for (int k = 0; k < d; ++k) s = s + A[ii + size_t(C) * k] * Q[jj + size_t(d) * k];
+      dir[size_t(ii) * d + jj] = s;
+    }
+  return dir;
+}
+
+static int plan_create(const spx_grid* g, const double* A, const double* Q, int Cy, int oy,
+                       const double* Ay, const double* Qy, int nmembers, void* stream,
+                       spx_plan** out);
+
 int spx_plan_create(const spx_grid* g, const double* A, const double* Q, void* stream,
                     spx_plan** out) {
-  return spx_plan_create_batch(g, A, Q, 1, stream, out);
+  return plan_create(g, A, Q, 0, 0, nullptr, nullptr, 1, stream, out);
 }
 
 int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, int nmembers,
                           void* stream, spx_plan** out) {
+  return plan_create(g, A, Q, 0, 0, nullptr, nullptr, nmembers, stream, out);
+}
+
+int spx_plan_create_fcy(const spx_grid* g, const double* A, const double* Q, int Cy, int oy,
+                        const double* Ay, const double* Qy, int nmembers, void* stream,
+                        spx_plan** out) {
+  return plan_create(g, A, Q, Cy, oy, Ay, Qy, nmembers, stream, out);
+}
+
+static int plan_create(const spx_grid* g, const double* A, const double* Q, int Cy, int oy,
+                       const double* Ay, const double* Qy, int nmembers, void* stream,
+                       spx_plan** out) {
   SPX_REQ(g && out, "spx_plan_create: null argument");
   *out = nullptr;
   SPX_REQ(nmembers >= 1, "nmembers must be >= 1");
   SPX_REQ(g->nx >= 2 && g->nx % 2 == 0, "nx must be even and >= 2");
   SPX_REQ(g->ny >= 2 && g->ny % 2 == 0, "ny must be even and >= 2");
+  if (Cy > 0) {
+    SPX_REQ(Ay && Qy, "FC-Gram tables Ay and Qy are required when Cy > 0");
+    SPX_REQ(oy >= 1, "matching order oy must be >= 1 when Cy > 0");
+    SPX_REQ(g->ny - Cy >= oy + 1, "too few physical y points for the matching order");
+    if (g->precision != SPX_FP64)
+      return fail(SPX_ERR_UNSUPPORTED,
+                  "non-periodic y needs SPX_FP64: the y continuation amplifies float32 "
+                  "rounding of the boundary rows by its row sums (~1e4)");
+  } else {
+    SPX_REQ(Cy == 0 && oy == 0, "Cy == 0 requires oy == 0 (periodic y)");
+  }
   SPX_REQ(g->nz >= 2, "nz must be >= 2");
   SPX_REQ(g->Cz >= 0 && g->Cz < g->nz, "Cz out of range");
   SPX_REQ(g->Lx > 0 && g->Ly > 0 && g->Lz > 0, "box lengths must be positive");
@@ -277,21 +316,22 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
     spx_plan_destroy(p);
     return code;
   };
-  // dir = A*Q^T (fcgram_mod.f90:250-252), textbook sum in k order
   if (C > 0) {
-    p->h_dir.assign(size_t(C) * d, 0.0);
-    for (int ii = 0; ii < C; ++ii)
-      for (int jj = 0; jj < d; ++jj) {
-        double s = 0.0;
-        for (int k = 0; k < d; ++k) s = s + A[ii + size_t(C) * k] * Q[jj + size_t(d) * k];
-        p->h_dir[size_t(ii) * d + jj] = s;
-      }
+    p->h_dir = dir_matrix(A, Q, C, d);
     if ((rc = upload(p, &p->d_dir, p->h_dir))) return bail(rc);
   }
-  // grids and wavenumbers (specter.fpp:683-800)
+  p->Cy = Cy;
+  p->oy = oy;
+  p->npy = ny - Cy;
+  if (Cy > 0) {
+    p->h_diry = dir_matrix(Ay, Qy, Cy, oy);
+    if ((rc = upload(p, &p->d_diry, p->h_diry))) return bail(rc);
+  }
+  // grids and wavenumbers (specter.fpp:683-800); non-periodic y: Ly is the
+  // physical length, dy = Ly/(ny-Cy-1) and Dky = 2 pi/(dy ny) (:720-723)
   const double PI = 3.14159265358979323846;
   p->Dkx = 1.0 / g->Lx;
-  p->Dky = 1.0 / g->Ly;
+  p->Dky = Cy > 0 ? 2.0 * PI / (g->Ly / double(ny - Cy - 1) * ny) : 1.0 / g->Ly;
   if (C > 0) {
     p->dz = g->Lz / double(nz - C - 1);
     p->Dkz = 2.0 * PI / (p->dz * nz);
@@ -357,8 +397,12 @@ int spx_plan_create_batch(const spx_grid* g, const double* A, const double* Q, i
       w[2 * size_t(k) + 1] = double(sinl(ang));
     }
     if ((rc = upload(p, reinterpret_cast<double**>(&p->d_wtop), w))) return bail(rc);
-    if ((rc = build_harmonic_table(p))) return bail(rc);
-    if ((rc = build_wall_mfma(p))) return bail(rc);
+    // the wall solvers' tables; plans with a y continuation have no boundary
+    // conditions (the reference stops at boundary_mod.fpp:104-105, :179-180)
+    if (Cy == 0) {
+      if ((rc = build_harmonic_table(p))) return bail(rc);
+      if ((rc = build_wall_mfma(p))) return bail(rc);
+    }
   }
   if (p->fp32) {
     if ((rc = build_line_fft(p->fx32, nx, 8, 0))) return bail(rc);
@@ -532,6 +576,59 @@ int spx_fftp2d_complex_to_real_xy(spx_plan* p, void* in, void* out) {
   return SPX_OK;
 }
 
+// The transforms of non-periodic y, on the S layout.  Each works for any
+// plan; with Cy = 0 the yz transforms have no y continuation.
+int spx_fftp1d_real_to_complex_x(spx_plan* p, const void* in, void* out) {
+  SPX_REQ(p && in && out, "spx_fftp1d_real_to_complex_x: null argument");
+  MemberView mv(p, 1);
+  int rc;
+  for (int b = 0; b < p->nact; ++b) {
+    if ((rc = xr2c(p, roff(in, p->nr, b), p->X, p->g.nz))) return rc;
+    if ((rc = zfft_x2s(p, p->X, coff(out, p->nc, b), 1))) return rc;
+  }
+  return SPX_OK;
+}
+
+int spx_fftp2d_real_to_complex_yz(spx_plan* p, void* s) {
+  SPX_REQ(p && s, "spx_fftp2d_real_to_complex_yz: null argument");
+  MemberView mv(p, 1);
+  int rc;
+  // y continuation + y FFT of the physical z planes, then z continuation +
+  // z FFT: the two continuations and the y FFT act on different axes and
+  // commute with each other
+  for (int b = 0; b < p->nact; ++b) {
+    void* sb = coff(s, p->nc, b);
+    if ((rc = zfft_s2x(p, sb, p->X, 0, p->np, 1))) return rc;
+    if ((rc = yfft(p, p->X, 1, p->np, nullptr, nullptr, false, p->Cy > 0))) return rc;
+    if ((rc = zfft_x2s(p, p->X, sb, 0))) return rc;
+  }
+  return SPX_OK;
+}
+
+int spx_fftp2d_complex_to_real_yz(spx_plan* p, void* s) {
+  SPX_REQ(p && s, "spx_fftp2d_complex_to_real_yz: null argument");
+  MemberView mv(p, 1);
+  int rc;
+  for (int b = 0; b < p->nact; ++b) {
+    void* sb = coff(s, p->nc, b);
+    if ((rc = zfft_s2x(p, sb, p->X, 0, p->g.nz, 0))) return rc;
+    if ((rc = yfft(p, p->X, 0, p->g.nz))) return rc;
+    if ((rc = zfft_x2s(p, p->X, sb, 1))) return rc;
+  }
+  return SPX_OK;
+}
+
+int spx_fftp1d_complex_to_real_x(spx_plan* p, const void* in, void* out) {
+  SPX_REQ(p && in && out, "spx_fftp1d_complex_to_real_x: null argument");
+  MemberView mv(p, 1);
+  int rc;
+  for (int b = 0; b < p->nact; ++b) {
+    if ((rc = zfft_s2x(p, coff(in, p->nc, b), p->X, 0, p->g.nz, 1))) return rc;
+    if ((rc = xc2r(p, p->X, roff(out, p->nr, b), p->g.nz))) return rc;
+  }
+  return SPX_OK;
+}
+
 int spx_fftp1d_real_to_complex_z(spx_plan* p, void* s) {
   SPX_REQ(p && s, "spx_fftp1d_real_to_complex_z: null argument");
   MemberView mv(p, p->nact);
@@ -700,6 +797,7 @@ int spx_sol_project(spx_plan* p, void* a, void* b, void* c, void* d, int bctarge
                     int bczend) {
   SPX_REQ(p && a && b && c && d, "spx_sol_project: null argument");
   if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "sol_project needs walls in z (Cz > 0)");
+  if (p->Cy > 0) return fail(SPX_ERR_UNSUPPORTED, "non-periodic y (Cy > 0) has no boundary conditions (boundary_mod.fpp:104-105)");
   if (bctarget != 1 || bczsta != 0 || bczend != 0)
     return fail(SPX_ERR_UNSUPPORTED,
                 "sol_project: only bctarget=1 with homogeneous Dirichlet walls is implemented");
@@ -717,6 +815,7 @@ int spx_v_imposebc_and_project(spx_plan* p, void* vx, void* vy, void* vz, void* 
   SPX_REQ(p && vx && vy && vz && pr, "spx_v_imposebc_and_project: null argument");
   SPX_REQ(rki >= 1 && ord >= 1 && rki <= ord, "spx_v_imposebc_and_project: bad RK stage");
   if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "no-slip walls need Cz > 0");
+  if (p->Cy > 0) return fail(SPX_ERR_UNSUPPORTED, "non-periodic y (Cy > 0) has no boundary conditions (boundary_mod.fpp:104-105)");
   double tau = 1.0 / double(rki);
   if (rki != ord) tau = double(rki + 1) * tau;
   MemberView mv(p, p->nact);
@@ -728,6 +827,7 @@ int spx_v_imposebc_and_project(spx_plan* p, void* vx, void* vy, void* vz, void* 
 int spx_s_imposebc(spx_plan* p, void* th) {
   SPX_REQ(p && th, "spx_s_imposebc: null argument");
   if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "scalar walls need Cz > 0");
+  if (p->Cy > 0) return fail(SPX_ERR_UNSUPPORTED, "non-periodic y (Cy > 0) has no boundary conditions (boundary_mod.fpp:104-105)");
   MemberView mv(p, p->nact);
   return sconst_fused(p, th);
 }
@@ -980,6 +1080,7 @@ int spx_state_create(spx_plan* p, const spx_params* prm, spx_state** out) {
           "spx_state_create: unknown solver");
   SPX_REQ(prm->ord >= 1, "spx_state_create: ord must be >= 1");
   if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "the solvers need walls in z (Cz > 0)");
+  if (p->Cy > 0) return fail(SPX_ERR_UNSUPPORTED, "non-periodic y (Cy > 0) has no boundary conditions (boundary_mod.fpp:104-105)");
   *out = nullptr;
   const bool bouss = prm->solver != SPX_SOLVER_HD;  // BOUSS and ROTBOUSS carry th
   int rc = ensure_work(p, bouss);

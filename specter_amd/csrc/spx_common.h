@@ -153,6 +153,12 @@ struct spx_plan {
   // FC-Gram continuation matrix dir = A*Q^T, row-major [C][d], float64 always.
   double* d_dir = nullptr;
   std::vector<double> h_dir;
+  // Non-periodic y (Cy > 0): the forward 3D transforms continue every y line
+  // of the physical z planes from its first and last oy physical points
+  // before the y FFT; diry = Ay*Qy^T as dir.  Cy = 0: periodic y.
+  int Cy = 0, oy = 0, npy = 0;
+  double* d_diry = nullptr;
+  std::vector<double> h_diry;
   // wavenumbers (float64), filter factors, z grid and khom(kx,ky); the kx,
   // fkx and khom tables are repeated once per member
   double *d_kx = nullptr, *d_ky = nullptr, *d_kz = nullptr;
@@ -310,8 +316,10 @@ int zfft_s2x(spx_plan* p, const void* S, void* X, int dir, int nzout, int nofft,
              void* X2 = nullptr, bool lo = false);
 // y transform of X planes < nplanes: Xout <- y FFT(Xin) (in place when equal);
 // Y2 != nullptr: also Y2 <- y FFT of (i ky) Xin (dual, backward only).
+// cont (forward, plans with Cy > 0): FC-Gram continuation of every y line
+// before the FFT (rows ny-Cy.. from the first and last oy physical rows).
 int yfft(spx_plan* p, void* X, int forward, int nplanes, const void* Xin = nullptr,
-         void* Y2 = nullptr, bool lo = false);
+         void* Y2 = nullptr, bool lo = false, bool cont = false);
 int xr2c(spx_plan* p, const void* R, void* X, int nplanes, bool lo = false);
 // x half complex -> real of planes < nplanes; dirx = 1 multiplies by i kx first.
 int xc2r(spx_plan* p, const void* X, void* R, int nplanes, int dirx = 0, bool lo = false);

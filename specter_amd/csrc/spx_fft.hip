@@ -408,11 +408,15 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_s2x(const cpx<TI>* __restrict__
 // ---------------------------------------------------------------------------
 // y transform of X (lines along y, stride nkx), one plane per blockIdx.y.
 // DUAL (backward): Y2 <- y IFFT of (i ky) Xin from the same load.
+// Cy > 0 (forward): FC-Gram continuation of each line before the FFT
+// (fftp.fpp:565-581): rows npy..ny-1 from rows 0..oy-1 and npy-oy..npy-1.
 // ---------------------------------------------------------------------------
 template <typename T, int RS, bool DUAL>
 __global__ void __launch_bounds__(kMaxBlock) k_y(const cpx<T>* Xin, cpx<T>* Xout,
                                            cpx<T>* __restrict__ Y2, LineFFT<cpx<T>> f, int nkx,
-                                           int ny, int forward, const double* __restrict__ ky) {
+                                           int ny, int forward, const double* __restrict__ ky,
+                                           int npy, int Cy, int oy,
+                                           const double* __restrict__ diry) {
   using Cp = cpx<T>;
   SPX_PROLOGUE(Cp, f);
   int bx, by;
@@ -420,13 +424,19 @@ __global__ void __launch_bounds__(kMaxBlock) k_y(const cpx<T>* Xin, cpx<T>* Xout
   const int i0 = bx * f.nl;
   const long base = long(by) * nkx * ny;
   const int l = tid & (f.nl - 1), ii = i0 + l;
-  for (int jy = tid >> f.lgnl; jy < ny; jy += f.tpl) {
+  const bool cont = forward && Cy > 0;
+  const int nload = cont ? npy : ny;
+  for (int jy = tid >> f.lgnl; jy < nload; jy += f.tpl) {
     Cp v = mkc<T>(0, 0);
     if (ii < nkx) v = Xin[base + ii + long(nkx) * jy];
     if (!forward) v.y = -v.y;
     lds[l * f.ldl + jy] = v;
   }
   __syncthreads();
+  if (cont) {
+    fc_continue_lds(lds, f.nl, f.ldl, npy, Cy, oy, diry);
+    __syncthreads();
+  }
   Cp keep[DUAL ? KEEP : 1];
   if constexpr (DUAL) {
 #pragma unroll
@@ -1830,7 +1840,7 @@ int zfft_s2x(spx_plan* p, const void* S, void* X, int dirn, int nzout, int nofft
 #if !defined(SPX_PART) || SPX_PART == 2
 template <typename T>
 static int yfft_t(spx_plan* p, void* X, int forward, int nplanes, const void* Xin, void* Y2,
-                  const LineFFTHost& f) {
+                  const LineFFTHost& f, bool cont) {
   // members are consecutive blocks of nplanes planes
   const int npl = nplanes * p->mem;
   dim3 grid(unsigned((p->nkx + f.nl - 1) / f.nl), unsigned(npl));
@@ -1841,20 +1851,24 @@ static int yfft_t(spx_plan* p, void* X, int forward, int nplanes, const void* Xi
     ProfScope ps_(p, "y_dual", io * 3.0);
     SPX_LAUNCH_XY(f, k_y, T, SPX_COMMA true, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
                   p->stream, in, C_<T>(X), C_<T>(Y2), f.dev<cpx<T>>(), p->nkx, p->g.ny, forward,
-                  p->d_ky);
+                  p->d_ky, p->npy, 0, 0, static_cast<const double*>(nullptr));
   } else {
     ProfScope ps_(p, "y", io * 2.0);
     SPX_LAUNCH_XY(f, k_y, T, SPX_COMMA false, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
                   p->stream, in, C_<T>(X), static_cast<cpx<T>*>(nullptr), f.dev<cpx<T>>(),
-                  p->nkx, p->g.ny, forward, p->d_ky);
+                  p->nkx, p->g.ny, forward, p->d_ky, p->npy, cont ? p->Cy : 0, p->oy,
+                  p->d_diry);
   }
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
-int yfft(spx_plan* p, void* X, int forward, int nplanes, const void* Xin, void* Y2, bool lo) {
+int yfft(spx_plan* p, void* X, int forward, int nplanes, const void* Xin, void* Y2, bool lo,
+         bool cont) {
   if (int rc = check_lo(p, lo)) return rc;
-  return lo ? yfft_t<float>(p, X, forward, nplanes, Xin, Y2, p->fy32)
-            : yfft_t<double>(p, X, forward, nplanes, Xin, Y2, p->fy);
+  if (cont && (!forward || p->Cy <= 0 || Y2))
+    return fail(SPX_ERR_ARG, "y continuation: forward transforms of plans with Cy > 0 only");
+  return lo ? yfft_t<float>(p, X, forward, nplanes, Xin, Y2, p->fy32, cont)
+            : yfft_t<double>(p, X, forward, nplanes, Xin, Y2, p->fy, cont);
 }
 
 template <typename T>

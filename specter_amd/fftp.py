@@ -41,30 +41,37 @@ class FCPlan:
 
     def __init__(self, nx: int, ny: int, nz: int, Cz: int, oz: int, Lx: float = 1.0,
                  Ly: float = 1.0, Lz: float = 1.0, tdir: str | None = None, stream=None,
-                 precision: int = _l.SPX_FP64, members: int = 1):
+                 precision: int = _l.SPX_FP64, members: int = 1, Cy: int = 0, oy: int = 0):
+        """Cy > 0: non-periodic y (FC-Gram continuation with Cy points and
+        matching order oy, ny including the Cy points, Ly the physical length;
+        fcgram_mod.f90:49-144 with C(2) = Cy)."""
         lib = _l.load_library()
         self._lib = lib
         g = _l.SpxGrid(nx, ny, nz, Cz, oz, Lx, Ly, Lz, precision)
-        A = Q = None
+        A = Q = Ay = Qy = None
         if Cz > 0:
             A, Q = load_dirichlet_tables(Cz, oz, tdir)
-        self.A, self.Q = A, Q
+        if Cy > 0:
+            Ay, Qy = load_dirichlet_tables(Cy, oy, tdir)
+        self.A, self.Q, self.Ay, self.Qy = A, Q, Ay, Qy
         torch = _torch()
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
         self.stream = stream
         ptr = c_void_p()
-        _l.check(lib.spx_plan_create_batch(
-            byref(g),
-            A.ctypes.data_as(ctypes.POINTER(c_double)) if A is not None else None,
-            Q.ctypes.data_as(ctypes.POINTER(c_double)) if Q is not None else None,
-            int(members), c_void_p(stream), byref(ptr)))
+
+        def dp(a):
+            return a.ctypes.data_as(ctypes.POINTER(c_double)) if a is not None else None
+
+        _l.check(lib.spx_plan_create_fcy(byref(g), dp(A), dp(Q), int(Cy), int(oy), dp(Ay),
+                                         dp(Qy), int(members), c_void_p(stream), byref(ptr)))
         self.members = int(members)
         self.active = int(members)
         self._p = ptr
         info = _l.SpxPlanInfo()
         _l.check(lib.spx_plan_get_info(ptr, byref(info)))
         self.nx, self.ny, self.nz, self.Cz, self.oz = nx, ny, nz, Cz, oz
+        self.Cy, self.oy = Cy, oy
         self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
         self.nkx, self.np = info.nkx, info.np
         self.z_fft_len, self.z_algorithm = info.z_fft_len, info.z_algorithm
@@ -194,6 +201,28 @@ class FCPlan:
         """fftp.fpp:1060-1094 (z IFFT, in place)."""
         _l.check(self._lib.spx_fftp1d_complex_to_real_z(self._p, self._spec(inout, "inout")))
         return inout
+
+    def fftp1d_real_to_complex_x(self, inp, out):
+        """fftp.fpp:620-716 (x r2c of every plane, S layout)."""
+        _l.check(self._lib.spx_fftp1d_real_to_complex_x(self._p, self._real(inp, "in"),
+                                                        self._spec(out, "out")))
+        return out
+
+    def fftp2d_real_to_complex_yz(self, inout):
+        """fftp.fpp:527-616 (y and z continuation + 2D yz FFT, in place)."""
+        _l.check(self._lib.spx_fftp2d_real_to_complex_yz(self._p, self._spec(inout, "inout")))
+        return inout
+
+    def fftp2d_complex_to_real_yz(self, inout):
+        """fftp.fpp:922-956 (2D yz IFFT, in place)."""
+        _l.check(self._lib.spx_fftp2d_complex_to_real_yz(self._p, self._spec(inout, "inout")))
+        return inout
+
+    def fftp1d_complex_to_real_x(self, inp, out):
+        """fftp.fpp:961-1056 (x c2r of every plane)."""
+        _l.check(self._lib.spx_fftp1d_complex_to_real_x(self._p, self._spec(inp, "in"),
+                                                        self._real(out, "out")))
+        return out
 
     def fc_continue(self, inout):
         """fftp.fpp:755-773."""

@@ -67,6 +67,37 @@ MODULE spx
          IMPORT :: C_INT, C_PTR
          TYPE(C_PTR), VALUE :: plan, cin, rout
       END FUNCTION
+      ! non-periodic y: fcgram_create_plan with Cy > 0 (fftp/fcgram_mod.f90:49-144)
+      INTEGER(C_INT) FUNCTION spx_plan_create_fcy(g, A, Q, Cy, oy, Ay, Qy, nmembers, stream, &
+            plan) BIND(C, name='spx_plan_create_fcy')
+         IMPORT :: C_INT, C_DOUBLE, C_PTR, spx_grid
+         TYPE(spx_grid), INTENT(IN) :: g
+         REAL(C_DOUBLE), INTENT(IN) :: A(*), Q(*), Ay(*), Qy(*)
+         INTEGER(C_INT), VALUE :: Cy, oy, nmembers
+         TYPE(C_PTR), VALUE :: stream
+         TYPE(C_PTR), INTENT(OUT) :: plan
+      END FUNCTION
+      ! fftp1d_*_x, fftp2d_*_yz (fftp/fftp.fpp:527-716,922-1056)
+      INTEGER(C_INT) FUNCTION spx_fftp1d_real_to_complex_x(plan, rin, cout) &
+            BIND(C, name='spx_fftp1d_real_to_complex_x')
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: plan, rin, cout
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION spx_fftp2d_real_to_complex_yz(plan, c) &
+            BIND(C, name='spx_fftp2d_real_to_complex_yz')
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: plan, c
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION spx_fftp2d_complex_to_real_yz(plan, c) &
+            BIND(C, name='spx_fftp2d_complex_to_real_yz')
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: plan, c
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION spx_fftp1d_complex_to_real_x(plan, cin, rout) &
+            BIND(C, name='spx_fftp1d_complex_to_real_x')
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: plan, cin, rout
+      END FUNCTION
       ! fftp1d_*_z (fftp/fftp.fpp:720-786,1060-1094)
       INTEGER(C_INT) FUNCTION spx_fftp1d_real_to_complex_z(plan, c) &
             BIND(C, name='spx_fftp1d_real_to_complex_z')

@@ -87,6 +87,12 @@ _SIGS = {
     "spx_energy_spectrum": [_P, _P, _P, _P, c_int, POINTER(c_double)],
     "spx_plan_create_batch": [POINTER(SpxGrid), POINTER(c_double), POINTER(c_double), c_int, _P,
                               POINTER(_P)],
+    "spx_plan_create_fcy": [POINTER(SpxGrid), POINTER(c_double), POINTER(c_double), c_int, c_int,
+                            POINTER(c_double), POINTER(c_double), c_int, _P, POINTER(_P)],
+    "spx_fftp1d_real_to_complex_x": [_P, _P, _P],
+    "spx_fftp2d_real_to_complex_yz": [_P, _P],
+    "spx_fftp2d_complex_to_real_yz": [_P, _P],
+    "spx_fftp1d_complex_to_real_x": [_P, _P, _P],
     "spx_plan_get_members": [_P, POINTER(c_int), POINTER(c_int)],
     "spx_tune_line": [c_int, c_int, c_int, c_int, c_int],
     "spx_tune_wall": [c_int, c_int, c_int],

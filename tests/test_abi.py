@@ -61,6 +61,29 @@ def test_plan_create_validates_before_device(lib, grid, code, msg):
     assert not p.value
 
 
+@pytest.mark.parametrize("Cy,oy,prec,with_y,code,msg", [
+    (25, 5, 1, True, 2, "non-periodic y needs SPX_FP64"),
+    (25, 5, 0, False, 1, "tables Ay and Qy"),
+    (25, 0, 0, True, 1, "matching order oy"),
+    (0, 5, 0, False, 1, "Cy == 0 requires oy == 0"),
+    (60, 5, 0, True, 1, "too few physical y"),
+])
+def test_plan_create_fcy_validates_before_device(lib, Cy, oy, prec, with_y, code, msg):
+    """spx_plan_create_fcy checks its y arguments before any device call."""
+    from specter_amd.lib import SpxGrid
+    from specter_amd.tables import load_dirichlet_tables
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+    A, Q = load_dirichlet_tables(25, 5)
+    g = SpxGrid(16, 64, 41, 25, 5, 1.0, 1.0, 1.0, prec)
+    p = ctypes.c_void_p()
+    yA, yQ = (dp(A), dp(Q)) if with_y else (None, None)
+    rc = lib.spx_plan_create_fcy(ctypes.byref(g), dp(A), dp(Q), Cy, oy, yA, yQ, 1, None,
+                                 ctypes.byref(p))
+    assert rc == code
+    assert msg in _err(lib)
+    assert not p.value
+
+
 def test_missing_tables_rejected(lib):
     from specter_amd.lib import SpxGrid
     g = SpxGrid(16, 16, 41, 25, 5, 1.0, 1.0, 1.0, 0)

@@ -132,3 +132,21 @@ def test_planted_mirror_mistake_fails(monkeypatch):
     monkeypatch.setattr(O, "fc_continue_y", bad)
     worse = kat_errors(grid(16, 64, 64))[2]
     assert worse > 100 * good, (good, worse)
+
+
+def test_backward_pieces_compose_to_3d():
+    """fftp2d_complex_to_real_yz + fftp1d_complex_to_real_x (fftp.fpp:922-1056)
+    equal the 3D backward transform (z IFFT + 2D xy c2r, :817-818), and
+    fftp1d_real_to_complex_x + fftp2d_real_to_complex_yz the forward one."""
+    g = grid(8, 20, 24)
+    rng = np.random.default_rng(5)
+    R = rng.standard_normal(g.real_shape)
+    S = O.fftp3d_real_to_complex(g, R.copy())
+    S2 = O.fftp1d_real_to_complex_x(g, R.copy())
+    O.fftp2d_real_to_complex_yz(g, S2)
+    assert np.abs(S2 - S).max() <= 1e-14 * np.abs(S).max()
+    B = O.fftp3d_complex_to_real(g, S.copy())
+    T = S.copy()
+    O.fftp2d_complex_to_real_yz(g, T)
+    B2 = O.fftp1d_complex_to_real_x(g, T)
+    assert np.abs(B2 - B).max() <= 1e-14 * np.abs(B).max()
