@@ -45,6 +45,19 @@ Pinning of each function (tests/test_oracle_*.py):
                                Omega = 0 the stage plus the th filter and round trip is the
                                Boussinesq stage bit for bit
   * sol_project pressure d      the removed velocity is (i kx d, i ky d, d/dz d)
+  * non-periodic y (fc_continue_y and the x / yz transforms)  tests/test_oracle_ycont.py:
+                               the fc_dirichlet KAT with y continued, brute-force DFT of the
+                               continued field, round trip, planted mirror mistake
+  * MHD: neumann_vector,       tests/test_oracle_mhd.py: the Neumann vectors equal the exact
+    neumann/robin_reconstruct, one-sided finite-difference weights (outward first and
+    laplace_z (0,0)/(2,2)/(0,2), second derivative) of the degree d-1 interpolant; polynomial
+    sol_project (potential),   walls reconstruct exactly; grad psi is removed exactly by the
+    a_imposebc_and_project,    matching potential kinds only, with psi returned; laplace_z
+    prodre / vector,           meets its boundary data and a'' = khom^2 a; prodre = gradre -
+    mhd_substep / mhd_step     grad |v|^2/2; vector = numpy cross; wall values after the
+                               magnetic BCs satisfy the discrete conditions; resistive decay of
+                               a line mode; kinetic and magnetic energy exchange in an ideal
+                               step (a flipped Lorentz or electromotive force fails)
   * precision="longdouble"      the same routines in extended precision (constants formed
                                in float64): the exact-trajectory referee of the float64 runs
 
@@ -468,9 +481,9 @@ def poisson_inhomogeneous(g: Grid, a, b, c) -> np.ndarray:
 
 def laplace_z(g: Grid, bc: np.ndarray, bczsta: int, bczend: int):
     """Harmonic solution a (and its z derivative b) in the mixed domain
-    (boundary_mod.fpp:451-678).  bc has shape (nkx, ny, 2).  Only the
-    pure-Neumann case (1,1) is on the hot path; Dirichlet (0,0) is also
-    restated for the conductor case."""
+    (boundary_mod.fpp:451-678).  bc has shape (nkx, ny, 2); kinds 0
+    Dirichlet, 1 Neumann, 2 Robin with constant khom; the combinations (0,0)
+    (:497-526), (1,1) (:528-559), (2,2) (:561-590) and (0,2) (:592-621)."""
     khom, Lz = g.khom, g.Lz
     coef1 = np.empty(khom.shape, g.cdtype)
     coef2 = np.empty(khom.shape, g.cdtype)
@@ -488,6 +501,18 @@ def laplace_z(g: Grid, bc: np.ndarray, bczsta: int, bczend: int):
             coef1[...] = (b2 - b1 * e1) * tmp
             coef2[...] = (b1 - b2 * e1) * tmp
             coef1[0, 0] = (b2[0, 0] - b1[0, 0]) / Lz
+            coef2[0, 0] = b1[0, 0]
+        elif bczsta == 2 and bczend == 2:
+            tmp = 1.0 / (2 * khom)
+            coef1[...] = b2 * tmp
+            coef2[...] = b1 * tmp
+            coef1[0, 0] = b1[0, 0]
+            coef2[0, 0] = 0.0
+        elif bczsta == 0 and bczend == 2:
+            tmp = 1.0 / (2 * khom)
+            coef1[...] = b2 * tmp
+            coef2[...] = (b1 * 2 * khom - b2 * e1) * tmp
+            coef1[0, 0] = b2[0, 0]
             coef2[0, 0] = b1[0, 0]
         else:
             raise ValueError("unsupported BC combination in laplace_z")
@@ -518,14 +543,28 @@ def sol_project(g: Grid, a, b, c, d_out, bctarget: int, bczsta: int = 0, bczend:
         C1 = d * tmp
     else:
         C1 = c * tmp
-    if bczsta == 2 or bczend == 2:
-        raise ValueError("Robin BCs are not on the hot path")
+    robin = bczsta == 2 or bczend == 2
+    if robin:
+        if bctarget != 0:
+            raise ValueError("Robin data only for the potential (bctarget 0)")
+        C2r = derivk(g, C1, 3)                       # (:275-278)
+        fftp1d_complex_to_real_z(g, C2r)
     fftp1d_complex_to_real_z(g, C1)
     bc = np.empty((g.nkx, g.ny, 2), g.cdtype)
-    sgn = -1.0 if bctarget == 0 else 1.0
-    bc[:, :, 0] = C1[:, :, 0] if bctarget == 1 else -C1[:, :, 0]
-    bc[:, :, 1] = C1[:, :, g.np_ - 1] if bctarget == 1 else -C1[:, :, g.np_ - 1]
-    del sgn
+    top = g.np_ - 1
+    # (:280-336): Dirichlet data +-C1 at the wall, Robin data from C1 and C2
+    if bczsta == 0:
+        bc[:, :, 0] = C1[:, :, 0] if bctarget == 1 else -C1[:, :, 0]
+    elif bczsta == 2:
+        bc[:, :, 0] = C2r[:, :, 0] - g.khom * C1[:, :, 0]
+    else:
+        raise ValueError("unsupported BC kind in sol_project")
+    if bczend == 0:
+        bc[:, :, 1] = C1[:, :, top] if bctarget == 1 else -C1[:, :, top]
+    elif bczend == 2:
+        bc[:, :, 1] = -(C2r[:, :, top] + g.khom * C1[:, :, top])
+    else:
+        raise ValueError("unsupported BC kind in sol_project")
     C2, C3 = laplace_z(g, bc, bczsta + bctarget, bczend + bctarget)
     if bctarget == 0:
         d_out[...] = C1 + C2
@@ -555,6 +594,146 @@ def s_imposebc(g: Grid, th) -> None:
     th[:, :, 0] = 0.0
     th[:, :, g.np_ - 1] = 0.0
     goto_3d_fourier(g, th)
+
+
+# ---------------------------------------------------------------------------
+# Neumann / Robin wall reconstruction and the magnetic boundary conditions
+# (fcgram_mod.f90:260-644, boundary/bboundary.f90:100-344)
+# ---------------------------------------------------------------------------
+def load_neumann_table(tdir: str, d: int, ordn: int):
+    """Q{d}.dat and Q{ordn}n{d}.dat: a float64 dxp, then Qn (d x d, Fortran
+    order) as one raw stream (fcgram_mod.f90:319-345)."""
+    Q = np.fromfile(os.path.join(tdir, f"Q{d}.dat"), dtype="<f8")
+    raw = np.fromfile(os.path.join(tdir, f"Q{ordn}n{d}.dat"), dtype="<f8")
+    if Q.size != d * d or raw.size != 1 + d * d:
+        raise ValueError("Neumann table size mismatch")
+    return Q.reshape(d, d).T.copy(), raw[1:].reshape(d, d).T.copy(), float(raw[0])
+
+
+def neumann_vector(g: Grid, tdir: str, ordn: int) -> np.ndarray:
+    """neu (ordn 1) / neu2 (ordn 2) of the z coordinate: Q(d,:) Qn^T, the last
+    entry scaled by (dz/dxp)^ordn for the grid's spacing
+    (fcgram_mod.f90:347-355), float64."""
+    d = g.d
+    Q, Qn, dxp = load_neumann_table(tdir, d, ordn)
+    neu = np.zeros(d)
+    for j in range(d):
+        acc = 0.0
+        for k in range(d):
+            acc = acc + Q[d - 1, k] * Qn[j, k]
+        neu[j] = acc
+    if ordn == 1:
+        neu[d - 1] = neu[d - 1] * (float(g.dz) / dxp)
+    else:
+        neu[d - 1] = neu[d - 1] * (float(g.dz) / dxp) ** 2
+    return neu
+
+
+def _wall_stencil(g: Grid, f, pos: int, vec):
+    """vec(d)*f(wall) + sum_{k<d} vec(k)*f(interior), in the reference's
+    order (fcgram_mod.f90:437-468)."""
+    d = g.d
+    if pos == 0:
+        acc = vec[d - 1] * f[:, :, 0]
+        for k in range(d - 1):
+            acc = acc + vec[k] * f[:, :, d - 1 - k]
+    else:
+        t = g.np_ - 1
+        acc = vec[d - 1] * f[:, :, t]
+        for k in range(d - 1):
+            acc = acc + vec[k] * f[:, :, t - d + 1 + k]
+    return acc
+
+
+def neumann_reconstruct(g: Grid, f, pos: int, vec) -> None:
+    """Wall value from its normal derivative held at the wall point, z walls
+    (boun 5 + pos; fcgram_mod.f90:437-468), in place on the mixed domain."""
+    f[:, :, 0 if pos == 0 else g.np_ - 1] = _wall_stencil(g, f, pos, vec)
+
+
+def robin_reconstruct(g: Grid, f, pos: int, neu) -> None:
+    """Wall value with d_n f + khom f = 0 (fcgram_mod.f90:550-568)."""
+    acc = _wall_stencil(g, f, pos, neu)
+    f[:, :, 0 if pos == 0 else g.np_ - 1] = acc / (g.khom * neu[g.d - 1] + 1)
+
+
+def int_conducting_z(g: Grid, ax, ay, pos: int) -> None:
+    """bboundary.f90:192-236."""
+    ind = 0 if pos == 0 else g.np_ - 1
+    ax[:, :, ind] = 0.0
+    ay[:, :, ind] = 0.0
+
+
+def conducting_z(g: Grid, neu, neu2, ax, ay, az, pos: int) -> None:
+    """bboundary.f90:239-290: zero wall values, then d2/dz2 ax = d2/dz2 ay = 0
+    and d/dz az = 0 at the wall."""
+    ind = 0 if pos == 0 else g.np_ - 1
+    for f in (ax, ay, az):
+        f[:, :, ind] = 0.0
+    neumann_reconstruct(g, ax, pos, neu2)
+    neumann_reconstruct(g, ay, pos, neu2)
+    neumann_reconstruct(g, az, pos, neu)
+
+
+def insulating_z(g: Grid, neu, ax, ay, az, pos: int) -> None:
+    """bboundary.f90:294-344: zero wall values, then the Robin condition with
+    khom for each component."""
+    ind = 0 if pos == 0 else g.np_ - 1
+    for f in (ax, ay, az):
+        f[:, :, ind] = 0.0
+    for f in (ax, ay, az):
+        robin_reconstruct(g, f, pos, neu)
+
+
+def a_imposebc_and_project(g: Grid, neu, neu2, ax, ay, az, ph, bczsta: int,
+                           bczend: int) -> None:
+    """Magnetic vector potential: walls 0 conducting, 1 insulating (vacuum)
+    (bboundary.f90:100-189)."""
+    if bczsta not in (0, 1) or bczend not in (0, 1):
+        raise ValueError("unsupported magnetic boundary kind")
+    az[0, 0, 0] = 0.0
+    if bczsta == 0 or bczend == 0:
+        goto_domain_w_boundaries(g, ax, ay)
+        if bczsta == 0:
+            int_conducting_z(g, ax, ay, 0)
+        if bczend == 0:
+            int_conducting_z(g, ax, ay, 1)
+        goto_3d_fourier(g, ax, ay)
+    sol_project(g, ax, ay, az, ph, 0, 2 * bczsta, 2 * bczend)
+    goto_domain_w_boundaries(g, ax, ay, az)
+    for pos, kind in ((0, bczsta), (1, bczend)):
+        if kind == 0:
+            conducting_z(g, neu, neu2, ax, ay, az, pos)
+        else:
+            insulating_z(g, neu, ax, ay, az, pos)
+    goto_3d_fourier(g, ax, ay, az)
+
+
+def _cross_real(g: Grid, A, B):
+    """A x B through real space on the physical planes, scaled by
+    1/(nx ny nz)^2, in the reference's operation order
+    (pseudospec_hd.f90:383-399, pseudospec_mhd.f90 vector)."""
+    npz = g.np_
+    tmp = 1.0 / (float(g.nx) * float(g.ny) * float(g.nz)) ** 2
+    r1, r2, r3 = (fftp3d_complex_to_real(g, x.copy())[:npz] for x in A)
+    r4, r5, r6 = (fftp3d_complex_to_real(g, x.copy())[:npz] for x in B)
+    outs = [np.zeros(g.real_shape, g.rdtype) for _ in range(3)]
+    outs[0][:npz] = (r2 * r6 - r5 * r3) * tmp
+    outs[1][:npz] = (r3 * r4 - r6 * r1) * tmp
+    outs[2][:npz] = (r1 * r5 - r4 * r2) * tmp
+    return tuple(fftp3d_real_to_complex(g, o) for o in outs)
+
+
+def prodre(g: Grid, a, b, c):
+    """(curl A) x A, the rotational form of the advective term
+    (pseudospec_hd.f90:322-402)."""
+    w = (curlk(g, b, c, 1), curlk(g, a, c, 2), curlk(g, a, b, 3))
+    return _cross_real(g, w, (a, b, c))
+
+
+def vector(g: Grid, a, b, c, d, e, f):
+    """(a,b,c) x (d,e,f) (pseudospec_mhd.f90: SUBROUTINE vector)."""
+    return _cross_real(g, (a, b, c), (d, e, f))
 
 
 # ---------------------------------------------------------------------------
@@ -671,6 +850,82 @@ def rotbouss_step(g: Grid, st: HDState, ordr, dt, nu, kappa, xmom, xtemp, omega)
     C1, C2, C3, C7 = st.vx.copy(), st.vy.copy(), st.vz.copy(), st.th.copy()
     for o in range(ordr, 0, -1):
         rotbouss_substep(g, st, C1, C2, C3, C7, o, ordr, dt, nu, kappa, xmom, xtemp, omega)
+
+
+@dataclass
+class MHDState:
+    """Fields of the MHD solver: velocity, pressure, forcing, the magnetic
+    vector potential a, its projection potential ph and the electromotive
+    forcing m (specter.fpp with MAGFIELD_)."""
+    vx: np.ndarray
+    vy: np.ndarray
+    vz: np.ndarray
+    pr: np.ndarray
+    fx: np.ndarray
+    fy: np.ndarray
+    fz: np.ndarray
+    ax: np.ndarray
+    ay: np.ndarray
+    az: np.ndarray
+    ph: np.ndarray
+    mx: np.ndarray
+    my: np.ndarray
+    mz: np.ndarray
+
+    def copy(self):
+        return MHDState(*(v.copy() for v in (
+            self.vx, self.vy, self.vz, self.pr, self.fx, self.fy, self.fz,
+            self.ax, self.ay, self.az, self.ph, self.mx, self.my, self.mz)))
+
+
+def mhd_substep(g: Grid, st: MHDState, C1, C2, C3, C9, C10, C11, o: int, ordr: int,
+                dt: float, nu: float, mu: float, b0, neu, neu2, bczsta: int = 0,
+                bczend: int = 0) -> None:
+    """One RK stage o of the MHD solver (include/mhd/mhd_rkstep2.f90:3-84)."""
+    rmp = 1.0 / float(o)
+    # b = curl a, plus the uniform field in the (0,0,0) mode (:6-16)
+    C12 = curlk(g, st.ay, st.az, 1)
+    C13 = curlk(g, st.ax, st.az, 2)
+    C14 = curlk(g, st.ax, st.ay, 3)
+    C12[0, 0, 0] = b0[0] * float(g.nx) * float(g.ny) * float(g.nz)
+    C13[0, 0, 0] = b0[1] * float(g.nx) * float(g.ny) * float(g.nz)
+    C14[0, 0, 0] = b0[2] * float(g.nx) * float(g.ny) * float(g.nz)
+    # j = curl b, held in a (:18-21)
+    jx = curlk(g, C13, C14, 1)
+    jy = curlk(g, C12, C14, 2)
+    jz = curlk(g, C12, C13, 3)
+    st.ax[...], st.ay[...], st.az[...] = jx, jy, jz
+    # (curl v) x v - j x b, filtered (:28-43)
+    C4, C5, C6 = prodre(g, st.vx, st.vy, st.vz)
+    C15, C16, C17 = vector(g, st.ax, st.ay, st.az, C12, C13, C14)
+    C4 = C4 - C15
+    C5 = C5 - C16
+    C6 = C6 - C17
+    for arr in (C4, C5, C6):
+        fc_filter(g, arr)
+    # electromotive force v x b, filtered (:45-51)
+    C15, C16, C17 = vector(g, st.vx, st.vy, st.vz, C12, C13, C14)
+    for arr in (C15, C16, C17):
+        fc_filter(g, arr)
+    lx, ly, lz = laplak(g, st.vx), laplak(g, st.vy), laplak(g, st.vz)
+    st.vx[...] = C1 + dt * (nu * lx - C4 + st.fx) * rmp
+    st.vy[...] = C2 + dt * (nu * ly - C5 + st.fy) * rmp
+    st.vz[...] = C3 + dt * (nu * lz - C6 + st.fz) * rmp
+    st.ax[...] = C9 + dt * (-mu * st.ax + C15 + st.mx) * rmp
+    st.ay[...] = C10 + dt * (-mu * st.ay + C16 + st.my) * rmp
+    st.az[...] = C11 + dt * (-mu * st.az + C17 + st.mz) * rmp
+    v_imposebc_and_project(g, st.vx, st.vy, st.vz, st.pr, o, ordr)
+    a_imposebc_and_project(g, neu, neu2, st.ax, st.ay, st.az, st.ph, bczsta, bczend)
+
+
+def mhd_step(g: Grid, st: MHDState, ordr: int, dt: float, nu: float, mu: float, b0, neu,
+             neu2, bczsta: int = 0, bczend: int = 0) -> None:
+    """mhd_rkstep1 copies (include/mhd/mhd_rkstep1.f90) + stages ORD..1."""
+    C1, C2, C3 = st.vx.copy(), st.vy.copy(), st.vz.copy()
+    C9, C10, C11 = st.ax.copy(), st.ay.copy(), st.az.copy()
+    for o in range(ordr, 0, -1):
+        mhd_substep(g, st, C1, C2, C3, C9, C10, C11, o, ordr, dt, nu, mu, b0, neu, neu2,
+                    bczsta, bczend)
 
 
 # ---------------------------------------------------------------------------
