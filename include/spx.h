@@ -42,7 +42,8 @@ enum {
 };
 
 enum { SPX_FP64 = 0, SPX_FP32_MIXED = 1 };
-enum { SPX_SOLVER_HD = 0, SPX_SOLVER_BOUSS = 1, SPX_SOLVER_ROTBOUSS = 2 };
+enum { SPX_SOLVER_HD = 0, SPX_SOLVER_BOUSS = 1, SPX_SOLVER_ROTBOUSS = 2, SPX_SOLVER_MHD = 3 };
+enum { SPX_BWALL_CONDUCTING = 0, SPX_BWALL_INSULATING = 1 };  /* b_parsebc codes (bboundary.f90:27-31) */
 
 typedef struct spx_plan spx_plan;
 typedef struct spx_state spx_state;
@@ -197,8 +198,11 @@ int spx_advect(spx_plan* p, const void* a, const void* b, const void* c, const v
                void* e);
 
 /* ---- boundary conditions and projection (boundary/) ---------------------- */
-/* sol_project (boundary_mod.fpp:197-402) with bctarget = 1 and homogeneous
- * Dirichlet bczsta = bczend = 0 (the configuration used by the velocity BCs). */
+/* sol_project (boundary_mod.fpp:197-402): bctarget = 1 (condition on the
+ * normal component, the velocity BCs) with homogeneous Dirichlet walls
+ * bczsta = bczend = 0; bctarget = 0 (condition on the potential, the magnetic
+ * BCs) with Dirichlet (0) or Robin (2) walls, except Robin bottom with
+ * Dirichlet top (laplace_z stops on it). */
 int spx_sol_project(spx_plan* p, void* a, void* b, void* c, void* d, int bctarget, int bczsta,
                     int bczend);
 /* v_imposebc_and_project (vboundary.f90:67-151) for no-slip walls at z=0,Lz:
@@ -209,6 +213,26 @@ int spx_v_imposebc_and_project(spx_plan* p, void* vx, void* vy, void* vz, void* 
                                int ord, const double* v_zsta, const double* v_zend);
 /* s_imposebc (sboundary.f90:67-120): scalar = 0 at both walls. */
 int spx_s_imposebc(spx_plan* p, void* th);
+
+/* ---- MHD (pseudo/pseudospec_hd.f90, pseudo/pseudospec_mhd.f90, boundary/bboundary.f90) */
+/* load_neumann_tables (fcgram_mod.f90:260-356) for the z walls: Q is the raw
+ * Q{d}.dat table (d x d, Fortran order), Qn the raw Q{ord}n{d}.dat stream (a
+ * float64 dxp, then d x d values); ord 1 (normal derivative) or 2 (second
+ * derivative). */
+int spx_plan_load_neumann(spx_plan* p, int ord, const double* Q, const double* Qn);
+/* curlk (pseudospec_hd.f90:130-206): c = component dir of curl from a, b. */
+int spx_curlk(spx_plan* p, const void* a, const void* b, void* c, int dir);
+/* prodre (pseudospec_hd.f90:322-402): (d,e,f) = (curl A) x A. */
+int spx_prodre(spx_plan* p, const void* a, const void* b, const void* c, void* d, void* e,
+               void* f);
+/* vector (pseudospec_mhd.f90): (g,h,i) = (a,b,c) x (d,e,f). */
+int spx_vector(spx_plan* p, const void* a, const void* b, const void* c, const void* d,
+               const void* e, const void* f, void* g, void* h, void* i);
+/* a_imposebc_and_project (bboundary.f90:100-189): wall kinds 0 conducting, 1
+ * insulating (vacuum); insulating bottom with conducting top is refused, as
+ * laplace_z stops on it.  ph receives the projection potential (mixed domain). */
+int spx_a_imposebc_and_project(spx_plan* p, void* ax, void* ay, void* az, void* ph, int bczsta,
+                               int bczend);
 
 /* ---- global diagnostics (host-side results, synchronous; batched plans write
  * one value per active member to each output) ------------------------------ */
@@ -252,17 +276,25 @@ typedef struct {
   double v_zend[2];  /* (vxzend, vyzend)                                        */
   double omega[3];   /* ROTBOUSS: (omegax, omegay, omegaz), the &rotation
                         namelist (specter.fpp:662-677); ignored otherwise      */
+  double mu;         /* MHD: magnetic diffusivity (&magfield)                  */
+  double b0[3];      /* MHD: uniform field (bx0, by0, bz0), the &uniformb namelist */
+  int bzsta, bzend;  /* MHD: magnetic wall kinds at z = 0 and z = Lz
+                        (SPX_BWALL_CONDUCTING / SPX_BWALL_INSULATING; &magbound) */
 } spx_params;
 
 int spx_state_create(spx_plan* p, const spx_params* prm, spx_state** out);
 int spx_state_destroy(spx_state* s);
-/* Device pointer of a state field: "vx","vy","vz","pr","fx","fy","fz","th","fs".
+/* Device pointer of a state field: "vx","vy","vz","pr","fx","fy","fz","th","fs";
+ * MHD: "ax","ay","az" (vector potential), "ph" (its projection potential),
+ * "mx","my","mz" (electromotive forcing).
  * Fields are S arrays; the caller fills them (e.g. via spx_fftp3d_real_to_complex). */
 int spx_state_field(spx_state* s, const char* name, void** dev_ptr);
 /* hd_rkstep1 copy (include/hd/hd_rkstep1.f90:4-6). */
 int spx_rkstep1(spx_plan* p, spx_state* s);
 /* one RK stage o (include/hd/hd_rkstep2.f90:3-36 / include/bouss/bouss_rkstep2.f90:3-59 /
- * include/rotbouss/rotbouss_rkstep2.f90:3-50) */
+ * include/rotbouss/rotbouss_rkstep2.f90:3-50 / include/mhd/mhd_rkstep2.f90:3-84).
+ * MHD states need SPX_FP64 plans with both Neumann tables loaded
+ * (spx_plan_load_neumann). */
 int spx_substep(spx_plan* p, spx_state* s, int o);
 /* nsteps full steps: rkstep1 + o = ord..1 (specter.fpp:1142-1161). */
 int spx_step(spx_plan* p, spx_state* s, int nsteps);

@@ -157,6 +157,72 @@ static int c2r3(spx_plan* p, const void* in, void* out, int dirn, int nplanes) {
   return xc2r(p, p->X, out, nplanes);
 }
 
+// ---- MHD building blocks -----------------------------------------------------
+static int ensure_mhd_work(spx_plan* p) {
+  if (p->MR[0]) return SPX_OK;
+  const size_t rb = size_t(p->g.nx) * p->g.ny * p->np * p->nb * 8;
+  for (int q = 0; q < 12; ++q)
+    if (int rc = alloc_dev(p, &p->MR[q], rb)) return rc;
+  return SPX_OK;
+}
+
+// real values (physical planes, members of the view packed) of A and B, then
+// out = A x B through real space (pseudospec_hd.f90:383-399, pseudospec_mhd.f90
+// vector).  RA / RB receive the real fields unless have_a / have_b say they
+// already hold them; RO is scratch.
+static int cross_spec(spx_plan* p, const void* const* A, const void* const* B, void* const* out,
+                      void* const* RA, void* const* RB, void* const* RO, bool have_a,
+                      bool have_b) {
+  int rc;
+  for (int c = 0; c < 3 && !have_a; ++c)
+    if ((rc = c2r3(p, A[c], RA[c], 0, p->np))) return rc;
+  for (int c = 0; c < 3 && !have_b; ++c)
+    if ((rc = c2r3(p, B[c], RB[c], 0, p->np))) return rc;
+  const long n = long(p->g.nx) * p->g.ny * p->np * p->mem;
+  if ((rc = op_cross_real(p, RA, RB, RO, n))) return rc;
+  for (int c = 0; c < 3; ++c)
+    if ((rc = r2c3(p, RO[c], out[c]))) return rc;
+  return SPX_OK;
+}
+
+// sol_project with the condition on the potential (bctarget 0; wall kinds
+// k0, k1: 0 Dirichlet, 2 Robin; boundary_mod.fpp:197-402)
+static int sol_project_pot(spx_plan* p, void* a, void* b, void* c, void* d, int k0, int k1) {
+  int rc;
+  if ((rc = op_poisson_project_pot(p, a, b, c, d, k0, k1))) return rc;
+  if ((rc = proj_pressure(p, d, k0, k1))) return rc;
+  return proj_harmonic(p, a, b, c, k0, k1);
+}
+
+// a_imposebc_and_project (bboundary.f90:100-189) for the walls' kinds
+// (0 conducting, 1 insulating): az(0,0,0) = 0; conducting walls zero the
+// tangential components; projection with Dirichlet / Robin conditions on the
+// potential; then the wall values are reconstructed (conducting: zero second
+// derivative of ax, ay and zero first derivative of az; insulating: Robin
+// with khom for each component)
+static int a_imposebc(spx_plan* p, void* ax, void* ay, void* az, void* ph, int bs, int be) {
+  if (!p->neu_loaded[0] || !p->neu_loaded[1])
+    return fail(SPX_ERR_STATE, "magnetic walls need the Neumann tables (spx_plan_load_neumann)");
+  int rc;
+  const double zero[3] = {0.0, 0.0, 0.0};
+  if ((rc = op_set_mode0(p, nullptr, nullptr, az, zero))) return rc;
+  BWallParams bp;
+  std::memset(&bp, 0, sizeof(bp));
+  std::memcpy(bp.neu1, p->neu[0], sizeof(bp.neu1));
+  std::memcpy(bp.neu2, p->neu[1], sizeof(bp.neu2));
+  const int kind[2] = {bs, be};
+  if (bs == 0 || be == 0) {
+    for (int w = 0; w < 2; ++w) bp.mode[0][w] = bp.mode[1][w] = kind[w] == 0 ? 1 : 0;
+    if ((rc = bwall_fused(p, ax, ay, nullptr, 2, bp))) return rc;
+  }
+  if ((rc = sol_project_pot(p, ax, ay, az, ph, 2 * bs, 2 * be))) return rc;
+  for (int w = 0; w < 2; ++w) {
+    bp.mode[0][w] = bp.mode[1][w] = kind[w] == 0 ? 3 : 4;
+    bp.mode[2][w] = kind[w] == 0 ? 2 : 4;
+  }
+  return bwall_fused(p, ax, ay, az, 3, bp);
+}
+
 static double diag_tmp(const spx_plan* p) {
   const double t = double(p->g.nx) * double(p->g.ny) * double(p->g.nz);
   return 1.0 / (t * t) / double(p->g.nz - p->g.Cz);
@@ -203,6 +269,15 @@ struct spx_state {
   void *fx = nullptr, *fy = nullptr, *fz = nullptr, *th = nullptr, *fs = nullptr;
   void *C1 = nullptr, *C2 = nullptr, *C3 = nullptr, *C7 = nullptr;
   void* N[4] = {nullptr, nullptr, nullptr, nullptr};
+  // MHD: vector potential, its projection potential, electromotive forcing,
+  // stage start copies (C9-C11), b = curl a (C12-C14), Lorentz force and
+  // electromotive force (C15-C17 at their two uses)
+  void *ax = nullptr, *ay = nullptr, *az = nullptr, *ph = nullptr;
+  void *mx = nullptr, *my = nullptr, *mz = nullptr;
+  void* C9[3] = {nullptr, nullptr, nullptr};
+  void* B[3] = {nullptr, nullptr, nullptr};
+  void* L[3] = {nullptr, nullptr, nullptr};
+  void* M[3] = {nullptr, nullptr, nullptr};
   std::vector<void*> owned;
   // diffusivities per (member, kx): nu for the velocity, kappa for th
   double* d_nu = nullptr;
@@ -798,9 +873,18 @@ int spx_sol_project(spx_plan* p, void* a, void* b, void* c, void* d, int bctarge
   SPX_REQ(p && a && b && c && d, "spx_sol_project: null argument");
   if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "sol_project needs walls in z (Cz > 0)");
   if (p->Cy > 0) return fail(SPX_ERR_UNSUPPORTED, "non-periodic y (Cy > 0) has no boundary conditions (boundary_mod.fpp:104-105)");
+  if (bctarget == 0) {
+    // conditions on the potential: Dirichlet (0) or Robin (2) walls; laplace_z
+    // has no Robin-bottom / Dirichlet-top case (boundary_mod.fpp:622-627)
+    const bool ok0 = bczsta == 0 || bczsta == 2, ok1 = bczend == 0 || bczend == 2;
+    if (!ok0 || !ok1 || (bczsta == 2 && bczend == 0))
+      return fail(SPX_ERR_UNSUPPORTED, "sol_project: unsupported potential boundary kinds");
+    MemberView mv(p, p->nact);
+    return sol_project_pot(p, a, b, c, d, bczsta, bczend);
+  }
   if (bctarget != 1 || bczsta != 0 || bczend != 0)
     return fail(SPX_ERR_UNSUPPORTED,
-                "sol_project: only bctarget=1 with homogeneous Dirichlet walls is implemented");
+                "sol_project: bctarget=1 supports homogeneous Dirichlet walls only");
   MemberView mv(p, p->nact);
   int rc = project_harmonic(p, a, b, c, d, nullptr, false);
   if (rc == SPX_OK) return proj_pressure(p, d);
@@ -830,6 +914,74 @@ int spx_s_imposebc(spx_plan* p, void* th) {
   if (p->Cy > 0) return fail(SPX_ERR_UNSUPPORTED, "non-periodic y (Cy > 0) has no boundary conditions (boundary_mod.fpp:104-105)");
   MemberView mv(p, p->nact);
   return sconst_fused(p, th);
+}
+
+int spx_plan_load_neumann(spx_plan* p, int ord, const double* Q, const double* Qn) {
+  SPX_REQ(p && Q && Qn, "spx_plan_load_neumann: null argument");
+  SPX_REQ(ord == 1 || ord == 2, "spx_plan_load_neumann: ord must be 1 or 2");
+  if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "Neumann tables need walls in z (Cz > 0)");
+  const int d = p->g.oz;
+  if (d > 16) return fail(SPX_ERR_UNSUPPORTED, "Neumann tables: matching order above 16");
+  // neu = Q(d,:) Qn^T with Qn after the leading dxp (Fortran order), the last
+  // entry scaled by (dz/dxp)^ord (fcgram_mod.f90:347-355)
+  const double dxp = Qn[0];
+  const double* q = Qn + 1;
+  double* neu = p->neu[ord - 1];
+  for (int j = 0; j < d; ++j) {
+    double acc = 0.0;
+    for (int k = 0; k < d; ++k) acc = acc + Q[(d - 1) + size_t(d) * k] * q[j + size_t(d) * k];
+    neu[j] = acc;
+  }
+  const double r = p->dz / dxp;
+  neu[d - 1] = ord == 1 ? neu[d - 1] * r : neu[d - 1] * (r * r);
+  p->neu_loaded[ord - 1] = true;
+  return SPX_OK;
+}
+
+int spx_curlk(spx_plan* p, const void* a, const void* b, void* c, int dir) {
+  SPX_REQ(p && a && b && c, "spx_curlk: null argument");
+  SPX_REQ(dir >= 1 && dir <= 3, "spx_curlk: dir must be 1, 2 or 3");
+  MemberView mv(p, p->nact);
+  return op_curlk(p, a, b, c, dir);
+}
+
+int spx_prodre(spx_plan* p, const void* a, const void* b, const void* c, void* d, void* e,
+               void* f) {
+  SPX_REQ(p && a && b && c && d && e && f, "spx_prodre: null argument");
+  int rc;
+  if ((rc = ensure_mhd_work(p))) return rc;
+  MemberView mv(p, p->nact);
+  const void* v[3] = {a, b, c};
+  void* w[3] = {d, e, f};
+  if ((rc = op_curl3(p, v, w))) return rc;
+  const void* wc[3] = {d, e, f};
+  return cross_spec(p, wc, v, w, p->MR, p->MR + 3, p->MR + 9, false, false);
+}
+
+int spx_vector(spx_plan* p, const void* a, const void* b, const void* c, const void* d,
+               const void* e, const void* f, void* g, void* h, void* i) {
+  SPX_REQ(p && a && b && c && d && e && f && g && h && i, "spx_vector: null argument");
+  int rc;
+  if ((rc = ensure_mhd_work(p))) return rc;
+  MemberView mv(p, p->nact);
+  const void* A[3] = {a, b, c};
+  const void* B[3] = {d, e, f};
+  void* O[3] = {g, h, i};
+  return cross_spec(p, A, B, O, p->MR, p->MR + 3, p->MR + 9, false, false);
+}
+
+int spx_a_imposebc_and_project(spx_plan* p, void* ax, void* ay, void* az, void* ph, int bczsta,
+                               int bczend) {
+  SPX_REQ(p && ax && ay && az && ph, "spx_a_imposebc_and_project: null argument");
+  SPX_REQ((bczsta == 0 || bczsta == 1) && (bczend == 0 || bczend == 1),
+          "spx_a_imposebc_and_project: wall kinds are 0 (conducting) or 1 (insulating)");
+  if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "magnetic walls need Cz > 0");
+  if (p->Cy > 0) return fail(SPX_ERR_UNSUPPORTED, "non-periodic y (Cy > 0) has no boundary conditions (bboundary.f90:132-137)");
+  if (bczsta == 1 && bczend == 0)
+    return fail(SPX_ERR_UNSUPPORTED,
+                "insulating bottom with conducting top: laplace_z has no Robin-Dirichlet case");
+  MemberView mv(p, p->nact);
+  return a_imposebc(p, ax, ay, az, ph, bczsta, bczend);
 }
 
 // ---- diagnostics ------------------------------------------------------------------
@@ -1076,15 +1228,27 @@ int spx_energy_spectrum(spx_plan* p, const void* a, const void* b, const void* c
 int spx_state_create(spx_plan* p, const spx_params* prm, spx_state** out) {
   SPX_REQ(p && prm && out, "spx_state_create: null argument");
   SPX_REQ(prm->solver == SPX_SOLVER_HD || prm->solver == SPX_SOLVER_BOUSS ||
-              prm->solver == SPX_SOLVER_ROTBOUSS,
+              prm->solver == SPX_SOLVER_ROTBOUSS || prm->solver == SPX_SOLVER_MHD,
           "spx_state_create: unknown solver");
   SPX_REQ(prm->ord >= 1, "spx_state_create: ord must be >= 1");
   if (p->g.Cz <= 0) return fail(SPX_ERR_UNSUPPORTED, "the solvers need walls in z (Cz > 0)");
   if (p->Cy > 0) return fail(SPX_ERR_UNSUPPORTED, "non-periodic y (Cy > 0) has no boundary conditions (boundary_mod.fpp:104-105)");
   *out = nullptr;
-  const bool bouss = prm->solver != SPX_SOLVER_HD;  // BOUSS and ROTBOUSS carry th
+  const bool mhd = prm->solver == SPX_SOLVER_MHD;
+  if (mhd) {
+    if (p->fp32) return fail(SPX_ERR_UNSUPPORTED, "the MHD solver runs in SPX_FP64 plans");
+    SPX_REQ((prm->bzsta == 0 || prm->bzsta == 1) && (prm->bzend == 0 || prm->bzend == 1),
+            "spx_state_create: magnetic wall kinds are 0 (conducting) or 1 (insulating)");
+    if (prm->bzsta == 1 && prm->bzend == 0)
+      return fail(SPX_ERR_UNSUPPORTED,
+                  "insulating bottom with conducting top: laplace_z has no Robin-Dirichlet case");
+    if (!p->neu_loaded[0] || !p->neu_loaded[1])
+      return fail(SPX_ERR_STATE, "the MHD solver needs the Neumann tables (spx_plan_load_neumann)");
+  }
+  const bool bouss = prm->solver == SPX_SOLVER_BOUSS || prm->solver == SPX_SOLVER_ROTBOUSS;
   int rc = ensure_work(p, bouss);
   if (rc) return rc;
+  if (mhd && (rc = ensure_mhd_work(p))) return rc;
   spx_state* s = new spx_state();
   s->p = p;
   s->prm = *prm;
@@ -1112,6 +1276,17 @@ int spx_state_create(spx_plan* p, const spx_params* prm, spx_state** out) {
         spx_state_destroy(s);
         return rc;
       }
+  if (mhd) {
+    void** mfields[] = {&s->ax,   &s->ay,   &s->az,   &s->ph,   &s->mx,   &s->my,
+                        &s->mz,   &s->C9[0], &s->C9[1], &s->C9[2], &s->B[0], &s->B[1],
+                        &s->B[2], &s->L[0], &s->L[1], &s->L[2], &s->M[0], &s->M[1],
+                        &s->M[2]};
+    for (void** f : mfields)
+      if ((rc = mk(f))) {
+        spx_state_destroy(s);
+        return rc;
+      }
+  }
   // per-(member, kx) diffusivity tables, every member starting from prm
   s->h_nu.assign(size_t(p->nb) * p->nkx, prm->nu);
   s->h_kappa.assign(size_t(p->nb) * p->nkx, prm->kappa);
@@ -1161,7 +1336,9 @@ int spx_state_field(spx_state* s, const char* name, void** ptr) {
     const char* n;
     void* v;
   } tab[] = {{"vx", s->vx}, {"vy", s->vy}, {"vz", s->vz}, {"pr", s->pr}, {"fx", s->fx},
-             {"fy", s->fy}, {"fz", s->fz}, {"th", s->th}, {"fs", s->fs}};
+             {"fy", s->fy}, {"fz", s->fz}, {"th", s->th}, {"fs", s->fs}, {"ax", s->ax},
+             {"ay", s->ay}, {"az", s->az}, {"ph", s->ph}, {"mx", s->mx}, {"my", s->my},
+             {"mz", s->mz}};
   for (auto& t : tab)
     if (std::strcmp(t.n, name) == 0) {
       if (!t.v) return fail(SPX_ERR_STATE, std::string("field ") + name + " not in this solver");
@@ -1176,7 +1353,47 @@ int spx_rkstep1(spx_plan* p, spx_state* s) {
   MemberView mv(p, p->nact);
   const void* src[4] = {s->vx, s->vy, s->vz, s->th};
   void* dst[4] = {s->C1, s->C2, s->C3, s->C7};
-  return op_copy_fields(p, s->th ? 4 : 3, src, dst);
+  if (int rc = op_copy_fields(p, s->th ? 4 : 3, src, dst)) return rc;
+  if (!s->ax) return SPX_OK;
+  // mhd_rkstep1 also keeps the vector potential (include/mhd/mhd_rkstep1.f90)
+  const void* asrc[3] = {s->ax, s->ay, s->az};
+  return op_copy_fields(p, 3, asrc, s->C9);
+}
+
+// One RK stage of the MHD solver (include/mhd/mhd_rkstep2.f90:3-84).  Real
+// work fields: MR[0..2] curl v, then j; MR[3..5] v; MR[6..8] b; MR[9..11] products.
+static int mhd_substep(spx_plan* p, spx_state* s, int o) {
+  const spx_params& q = s->prm;
+  const double rmp = 1.0 / double(o);
+  int rc;
+  MemberView mv(p, p->nact);
+  void* A[3] = {s->ax, s->ay, s->az};
+  void* V[3] = {s->vx, s->vy, s->vz};
+  void** R = p->MR;
+  // b = curl a with the uniform field in the (0,0,0) mode (:6-16), j = curl b in a (:18-21)
+  if ((rc = op_curl3(p, A, s->B))) return rc;
+  const double b0[3] = {((q.b0[0] * p->g.nx) * p->g.ny) * p->g.nz,
+                        ((q.b0[1] * p->g.nx) * p->g.ny) * p->g.nz,
+                        ((q.b0[2] * p->g.nx) * p->g.ny) * p->g.nz};
+  if ((rc = op_set_mode0(p, s->B[0], s->B[1], s->B[2], b0))) return rc;
+  if ((rc = op_curl3(p, s->B, A))) return rc;
+  // (curl v) x v (:28), j x b (:31), v x b (:45)
+  void* W[3] = {s->N[0], s->N[1], s->N[2]};
+  if ((rc = op_curl3(p, V, W))) return rc;
+  if ((rc = cross_spec(p, W, V, W, R, R + 3, R + 9, false, false))) return rc;
+  if ((rc = cross_spec(p, A, s->B, s->L, R, R + 6, R + 9, false, false))) return rc;
+  if ((rc = cross_spec(p, V, s->B, s->M, R + 3, R + 6, R + 9, true, true))) return rc;
+  // filters, laplak and the stage update (:33-76)
+  const void* c1[3] = {s->C1, s->C2, s->C3};
+  const void* f[3] = {s->fx, s->fy, s->fz};
+  const void* m[3] = {s->mx, s->my, s->mz};
+  if ((rc = op_mhd_update(p, V, A, c1, s->C9, W, s->L, s->M, f, m, s->d_nu, q.mu, q.dt, rmp)))
+    return rc;
+  // boundary conditions and projections (:80-81)
+  if ((rc = spx_v_imposebc_and_project(p, s->vx, s->vy, s->vz, s->pr, o, q.ord, q.v_zsta,
+                                       q.v_zend)))
+    return rc;
+  return a_imposebc(p, s->ax, s->ay, s->az, s->ph, q.bzsta, q.bzend);
 }
 
 // One RK stage.  full_pressure = false (inside a full step, for every stage
@@ -1187,6 +1404,7 @@ static int substep_impl(spx_plan* p, spx_state* s, int o, bool full_pressure) {
   SPX_REQ(p && s && s->p == p, "spx_substep: bad plan/state");
   const spx_params& q = s->prm;
   SPX_REQ(o >= 1 && o <= q.ord, "spx_substep: stage out of range");
+  if (q.solver == SPX_SOLVER_MHD) return mhd_substep(p, s, o);
   const bool rot = q.solver == SPX_SOLVER_ROTBOUSS;
   const bool bouss = q.solver == SPX_SOLVER_BOUSS || rot;
   const double rmp = 1.0 / double(o);

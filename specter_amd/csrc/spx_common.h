@@ -187,6 +187,12 @@ struct spx_plan {
   void* RV[3] = {nullptr};
   void* RO[4] = {nullptr};
   void* R0 = nullptr;       // general real scratch (Boussinesq round trip)
+  // MHD: real work fields (physical planes of every member), allocated on
+  // first use by prodre / vector / the MHD stage; Neumann vectors neu
+  // (order 1) and neu2 (order 2) of the z walls (fcgram_mod.f90:347-355)
+  void* MR[12] = {nullptr};
+  double neu[2][16] = {{0}};
+  bool neu_loaded[2] = {false, false};
   // mixed plans with walls: float64 boundary-plane buffers of the nonlinear
   // pipeline (2d planes per component and member; see zfft_s2x_bnd)
   void* XVb = nullptr;      // z IFFT of v_c, then its y IFFT (4 components)
@@ -240,6 +246,18 @@ struct WallParams {
   double vb[2][2];     // [comp][pos] wall velocity
   double nxny;         // nx*ny
   int scalar;          // 1: s_constant_z (zero at walls)
+};
+
+// Magnetic wall round trip (boundary/bboundary.f90:192-344): per component
+// and wall (0: z = 0, 1: z = Lz) what happens to the wall value in the mixed
+// domain: 0 nothing, 1 set to zero (int_conducting_z), 2 zero + Neumann
+// reconstruction from a zero first normal derivative, 3 the same from a zero
+// second derivative (conducting_z), 4 zero + Robin reconstruction with khom
+// (insulating_z).  neu1 / neu2: the plan's Neumann vectors (float64, d <= 16).
+struct BWallParams {
+  int mode[3][2];
+  double neu1[16];
+  double neu2[16];
 };
 
 struct RkLoadComp {
@@ -364,11 +382,16 @@ int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
 // s_imposebc: z IFFT, 1/nz, th = 0 at both walls, continuation + z FFT
 int sconst_fused(spx_plan* p, void* th);
 // sol_project stages (bctarget = 1, Neumann potential)
-int proj_pressure(spx_plan* p, void* d);             // d <- IFFT_z(d)/nz + harmonic a
+// d <- IFFT_z(d)/nz + harmonic a; (k0, k1): laplace_z kinds of the two walls
+// (0 Dirichlet, 1 Neumann, 2 Robin), (1,1) for the velocity projection
+int proj_pressure(spx_plan* p, void* d, int k0 = 1, int k1 = 1);
+// the magnetic wall round trip of ncomp components (a2 unused for ncomp 2)
+int bwall_fused(spx_plan* p, void* a0, void* a1, void* a2, int ncomp, const BWallParams& bp);
 // proj_pressure at z = 0 and z = np-1 only, from p->pw (the rest of d keeps
 // the potential): all the next stage's wall update reads
 int pressure_walls(spx_plan* p, void* d);
-int proj_harmonic(spx_plan* p, void* a, void* b, void* c);  // v -= grad(harmonic)
+int proj_harmonic(spx_plan* p, void* a, void* b, void* c, int k0 = 1,
+                  int k1 = 1);  // v -= grad(harmonic)
 // op_poisson_project_bc + proj_harmonic in one pass per line (plans with the
 // harmonic basis table and nz <= 1088; SPX_ERR_UNSUPPORTED otherwise)
 int project_harmonic(spx_plan* p, void* a, void* b, void* c, void* d, const RkLoad* rk,
@@ -408,6 +431,19 @@ int op_poisson_project(spx_plan* p, void* a, void* b, void* c, void* d);
 // poisson_project plus the wall data bc of sol_project (boundary_mod.fpp:262-352)
 // walls_only: d is left unwritten at z = 0 and z = np-1 and IFFT_z(d)/nz at
 // those two points goes to p->pw (for pressure_walls)
+// MHD pointwise kernels (spx_ops.hip)
+int op_curlk(spx_plan* p, const void* a, const void* b, void* c, int dir);
+int op_curl3(spx_plan* p, const void* const* in, void* const* out);
+int op_cross_real(spx_plan* p, const void* const* A, const void* const* B, void* const* O,
+                  long n);
+int op_set_mode0(spx_plan* p, void* f0, void* f1, void* f2, const double v[3]);
+int op_mhd_update(spx_plan* p, void* const* v, void* const* a, const void* const* c1,
+                  const void* const* c9, const void* const* N, const void* const* L,
+                  const void* const* M, const void* const* f, const void* const* m,
+                  const double* nu, double mu, double dt, double rmp);
+// sol_project's first pass for a condition on the potential (bctarget 0),
+// wall kinds k0, k1 in {0 Dirichlet, 2 Robin}; bc -> p->bc
+int op_poisson_project_pot(spx_plan* p, void* a, void* b, void* c, void* d, int k0, int k1);
 int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d,
                           const RkLoad* rk = nullptr,  // rk: update of c (c[0]) on load
                           bool walls_only = false);

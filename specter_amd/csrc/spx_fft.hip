@@ -778,6 +778,56 @@ __global__ void __launch_bounds__(kMaxBlock) k_noslip(cpx<TS>* __restrict__ vx, 
     for (int k = lane; k < n; k += f.tpl) V[ln * n + k] = ccast<TS>(xl[k]);
 }
 
+// goto_domain_w_boundaries + the magnetic wall conditions + goto_3d_fourier
+// fused per line (boundary_mod.fpp:72-194, bboundary.f90:192-344,
+// fcgram_mod.f90:437-468, 550-568); blockIdx.y = component (a0, a1, a2).
+template <typename TS, int RS>
+__global__ void __launch_bounds__(kMaxBlock) k_bwall(Cd* __restrict__ a0, Cd* __restrict__ a1,
+                                                     Cd* __restrict__ a2, LFd f, long nlines,
+                                                     int np, double invn, BWallParams bp,
+                                                     const double* __restrict__ khom,
+                                                     const double* __restrict__ dir, int C,
+                                                     int d) {
+#pragma clang fp contract(off)
+  SPX_PROLOGUE(Cd, f);
+  const int n = f.n;
+  const int comp = blockIdx.y;
+  Cd* V = comp == 0 ? a0 : (comp == 1 ? a1 : a2);
+  const long ln = long(blockIdx.x) * f.nl + line;
+  const bool ok = ln < nlines;
+  for (int k = lane; k < n; k += f.tpl) xl[k] = ok ? cconj(V[ln * n + k]) : mkc<double>(0, 0);
+  __syncthreads();
+  line_dft<double, RS>(xl, f, tws, lane);
+  for (int k = lane; k < np; k += f.tpl) xl[k] = cscale(cconj(xl[k]), invn);
+  __syncthreads();
+  // lane 0: the wall at z = 0, lane 1: z = Lz (their stencils do not overlap
+  // for np > d)
+  if (lane < 2 && ok) {
+    const int m = bp.mode[comp][lane];
+    if (m) {
+      const int ind = lane ? np - 1 : 0;
+      xl[ind] = mkc<double>(0, 0);
+      if (m >= 2) {
+        const double* vec = m == 3 ? bp.neu2 : bp.neu1;
+        Cd acc = cscale(xl[ind], vec[d - 1]);
+        for (int k = 0; k < d - 1; ++k)
+          acc = cadd(acc, cscale(xl[lane ? np - d + k : d - 1 - k], vec[k]));
+        if (m == 4) {
+          const double den = khom[ln] * vec[d - 1] + 1;
+          acc = mkc<double>(acc.x / den, acc.y / den);
+        }
+        xl[ind] = acc;
+      }
+    }
+  }
+  __syncthreads();
+  fc_continue_lds(lds, f.nl, f.ldl, np, C, d, dir);
+  __syncthreads();
+  line_dft<double, RS>(xl, f, tws, lane);
+  if (ok)
+    for (int k = lane; k < n; k += f.tpl) V[ln * n + k] = xl[k];
+}
+
 // laplace_z pure-Neumann coefficients (boundary_mod.fpp:531-560)
 struct Coef {
   Cd c1, c2;
@@ -797,6 +847,49 @@ __device__ __forceinline__ Coef neumann_coef(Cd b1, Cd b2, double kh, double Lz,
   return r;
 }
 
+// laplace_z coefficients for the wall kinds (k0, k1) (boundary_mod.fpp:497-621):
+// 0 Dirichlet, 1 Neumann, 2 Robin with constant khom; (1,1), (0,0), (2,2), (0,2)
+__device__ __forceinline__ Coef laplace_coef(int k0, int k1, Cd b1, Cd b2, double kh, double Lz,
+                                             bool zero_mode) {
+#pragma clang fp contract(off)
+  if (k0 == 1) return neumann_coef(b1, b2, kh, Lz, zero_mode);
+  Coef r;
+  if (k0 == 0 && k1 == 0) {
+    if (zero_mode) {
+      const Cd t = csub(b2, b1);
+      r.c1 = mkc<double>(t.x / Lz, t.y / Lz);
+      r.c2 = b1;
+      return r;
+    }
+    const double tmp = 1.0 / (1 - exp(-2 * kh * Lz));
+    const double e1 = exp(-kh * Lz);
+    r.c1 = cscale(csub(b2, cscale(b1, e1)), tmp);
+    r.c2 = cscale(csub(b1, cscale(b2, e1)), tmp);
+    return r;
+  }
+  if (k0 == 2) {  // (2,2)
+    if (zero_mode) {
+      r.c1 = b1;
+      r.c2 = mkc<double>(0, 0);
+      return r;
+    }
+    const double tmp = 1.0 / (2 * kh);
+    r.c1 = cscale(b2, tmp);
+    r.c2 = cscale(b1, tmp);
+    return r;
+  }
+  // (0,2)
+  if (zero_mode) {
+    r.c1 = b2;
+    r.c2 = b1;
+    return r;
+  }
+  const double tmp = 1.0 / (2 * kh);
+  r.c1 = cscale(b2, tmp);
+  r.c2 = cscale(csub(cscale(cscale(b1, 2.0), kh), cscale(b2, exp(-kh * Lz))), tmp);
+  return r;
+}
+
 // d <- IFFT_z(d)*tmp + a_harm(z)  (:354-381 with laplace_z :631-675); d is float64
 // (TS = double: the kernels of this family are instantiated per type)
 template <typename TS, int RS>
@@ -804,7 +897,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_proj_pressure(Cd* __restrict__ dd
                                                        const Cd* __restrict__ bc, LFd f,
                                                        long nlines, int ny, double tmp, double Lz,
                                                        const double* __restrict__ khom,
-                                                       const double* __restrict__ z) {
+                                                       const double* __restrict__ z, int k0,
+                                                       int k1) {
 #pragma clang fp contract(off)
   SPX_PROLOGUE(Cd, f);
   const int n = f.n;
@@ -817,7 +911,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_proj_pressure(Cd* __restrict__ dd
   const bool zm = kh == 0.0;
   Coef cf;
   cf.c1 = cf.c2 = mkc<double>(0, 0);
-  if (ok) cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
+  if (ok) cf = laplace_coef(k0, k1, bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
   __syncthreads();
   zline_dft<double, RS>(xl, f, tws, lane);
   if (!ok) return;
@@ -874,7 +968,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_proj_harmonic(cpx<TS>* __restrict
                                                        const double* __restrict__ kx,
                                                        const double* __restrict__ ky,
                                                        const double* __restrict__ dir, int C,
-                                                       int d) {
+                                                       int d, int k0, int k1) {
 #pragma clang fp contract(off)
   SPX_PROLOGUE(Cd, f);
   const int n = f.n;
@@ -888,7 +982,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_proj_harmonic(cpx<TS>* __restrict
   const bool zm = kh == 0.0;
   Coef cf;
   cf.c1 = cf.c2 = mkc<double>(0, 0);
-  if (ok) cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
+  if (ok) cf = laplace_coef(k0, k1, bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
   // the field values this thread corrects, read before the transforms (the
   // two line buffers bound the occupancy, not the registers)
   Cd pa[KEEP], pb[KEEP], pc[KEEP];
@@ -974,7 +1068,8 @@ __global__ void __launch_bounds__(256) k_proj_harmonic_tab(cpx<TS>* __restrict__
                                                          long nlines, int ny, int n, double Lz,
                                                          const double* __restrict__ khom,
                                                          const double* __restrict__ kx,
-                                                         const double* __restrict__ ky) {
+                                                         const double* __restrict__ ky, int k0,
+                                                         int k1) {
 #pragma clang fp contract(off)
   const long ln = long(blockIdx.x) * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -983,7 +1078,7 @@ __global__ void __launch_bounds__(256) k_proj_harmonic_tab(cpx<TS>* __restrict__
   const int j = int(ln - i * ny);
   const double kh = khom[ln];
   const bool zm = kh == 0.0;
-  const Coef cf = neumann_coef(bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
+  const Coef cf = laplace_coef(k0, k1, bc[2 * ln], bc[2 * ln + 1], kh, Lz, zm);
   const Cd* P1 = tab + 2 * long(hidx[ln]) * n;
   const Cd* P2 = P1 + n;
   const double kxi = kx[i], kyj = ky[j];
@@ -2007,6 +2102,19 @@ int noslip_fused(spx_plan* p, void* vx, void* vy, const void* pr, double tau,
   return noslip_launch(p, vx, vy, pr, wp, 2, "noslip", rk);
 }
 
+int bwall_fused(spx_plan* p, void* a0, void* a1, void* a2, int ncomp, const BWallParams& bp) {
+  if (p->g.oz > 16) return fail(SPX_ERR_UNSUPPORTED, "magnetic walls: matching order above 16");
+  const LineFFTHost& f = p->fz;
+  const long nlines = p->wlines();
+  dim3 grid(unsigned((nlines + f.nl - 1) / f.nl), unsigned(ncomp));
+  ProfScope ps_(p, "bwall", ncomp * double(p->wnc()) * 2.0 * 16.0);
+  SPX_LAUNCH_RS(f, k_bwall, double, , grid, blk(f), f.lds_bytes(16), p->stream, C_<double>(a0),
+                C_<double>(a1), C_<double>(a2), f.dev<Cd>(), nlines, p->np,
+                1.0 / double(p->g.nz), bp, p->d_khom, p->d_dir, p->g.Cz, p->g.oz);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+
 int sconst_fused(spx_plan* p, void* th) {
   WallParams wp;
   std::memset(&wp, 0, sizeof(wp));
@@ -2014,7 +2122,7 @@ int sconst_fused(spx_plan* p, void* th) {
   return noslip_launch(p, th, th, nullptr, wp, 1, "sconst", nullptr);
 }
 
-int proj_pressure(spx_plan* p, void* d) {
+int proj_pressure(spx_plan* p, void* d, int k0, int k1) {
   const LineFFTHost& f = fzt(p, false);
   const long nlines = p->wlines();
   ProfScope ps_(p, "proj_pressure", 2.0 * double(p->wnc()) * 16.0);
@@ -2022,7 +2130,7 @@ int proj_pressure(spx_plan* p, void* d) {
   const size_t lds = f.lds_bytes(16);
   SPX_LAUNCH_Z(f, k_proj_pressure, double, , grid, blk(f), lds, p->stream, C_<double>(d),
                C_<double>(p->bc), f.dev<Cd>(), nlines, p->g.ny, 1.0 / double(p->g.nz), p->g.Lz,
-               p->d_khom, p->d_z);
+               p->d_khom, p->d_z, k0, k1);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
@@ -2205,7 +2313,7 @@ int project_harmonic(spx_plan* p, void* a, void* b, void* c, void* d, const RkLo
   return SPX_OK;
 }
 
-int proj_harmonic(spx_plan* p, void* a, void* b, void* c) {
+int proj_harmonic(spx_plan* p, void* a, void* b, void* c, int k0, int k1) {
   const long nlines = p->wlines();
   if (p->d_htab) {
     ProfScope ps_(p, "proj_harmonic",
@@ -2213,7 +2321,7 @@ int proj_harmonic(spx_plan* p, void* a, void* b, void* c) {
     hipLaunchKernelGGL(k_proj_harmonic_tab<double>, dim3(unsigned((nlines + 3) / 4)), dim3(256), 0,
                        p->stream, C_<double>(a), C_<double>(b), C_<double>(c),
                        C_<double>(p->bc), static_cast<const Cd*>(p->d_htab), p->d_hidx, nlines,
-                       p->g.ny, p->g.nz, p->g.Lz, p->d_khom, p->d_kx, p->d_ky);
+                       p->g.ny, p->g.nz, p->g.Lz, p->d_khom, p->d_kx, p->d_ky, k0, k1);
     SPX_CHECK_LAUNCH();
     return SPX_OK;
   }
@@ -2223,7 +2331,7 @@ int proj_harmonic(spx_plan* p, void* a, void* b, void* c) {
   SPX_LAUNCH_RS(f, k_proj_harmonic, double, , dim3(unsigned((nlines + f.nl - 1) / f.nl)), blk(f),
                 lds, p->stream, C_<double>(a), C_<double>(b), C_<double>(c), C_<double>(p->bc),
                 f.dev<Cd>(), nlines, p->g.ny, p->np, p->g.Lz, p->d_khom, p->d_z, p->d_kx,
-                p->d_ky, p->d_dir, p->g.Cz, p->g.oz);
+                p->d_ky, p->d_dir, p->g.Cz, p->g.oz, k0, k1);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }

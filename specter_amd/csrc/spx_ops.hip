@@ -6,6 +6,7 @@
 // their fields; the state is float64 in every plan, and only the nonlinear
 // terms read by the RK update are float32 in mixed-precision plans.  The
 // kernels are bound by HBM traffic, so float64 arithmetic costs nothing.
+#include <cmath>
 #include <cstring>
 
 #include "spx_engine.h"
@@ -163,6 +164,191 @@ __global__ void k_coriolis(cpx<TN>* __restrict__ nx, cpx<TN>* __restrict__ ny,
   st(nx, e, cadd(ld(nx, e), cscale(csub(cscale(c, oy), cscale(b, oz)), 2.0)));
   st(ny, e, cadd(ld(ny, e), cscale(csub(cscale(a, oz), cscale(c, ox)), 2.0)));
   st(nz, e, cadd(ld(nz, e), cscale(csub(cscale(b, ox), cscale(a, oy)), 2.0)));
+}
+
+// curlk (pseudospec_hd.f90:130-206) for one component: dir 1: i ky b - i kz a,
+// dir 2: i kz a - i kx b, dir 3: i kx b - i ky a
+// (the products and differences are written out here, where the pragma keeps
+// them unfused, so the result is the reference's derivk-then-subtract)
+__device__ __forceinline__ Cp curl_dir(int dir, Cp a, Cp b, double kxi, double kyj, double kzk) {
+#pragma clang fp contract(off)
+  double k1, k2;
+  Cp u, w;  // result = (i k1) u - (i k2) w
+  if (dir == 1) {
+    k1 = kyj, u = b, k2 = kzk, w = a;
+  } else if (dir == 2) {
+    k1 = kzk, u = a, k2 = kxi, w = b;
+  } else {
+    k1 = kxi, u = b, k2 = kyj, w = a;
+  }
+  const double t1x = -(k1 * u.y), t1y = k1 * u.x;
+  const double t2x = -(k2 * w.y), t2y = k2 * w.x;
+  return mkc<double>(t1x - t2x, t1y - t2y);
+}
+
+__global__ void k_curlk(const Cp* __restrict__ a, const Cp* __restrict__ b, Cp* __restrict__ c,
+                        long nc, SIdx ix, int dir, const double* __restrict__ kx,
+                        const double* __restrict__ ky, const double* __restrict__ kz) {
+#pragma clang fp contract(off)
+  const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= nc) return;
+  int k, j;
+  long i;
+  ix.split(e, k, j, i);
+  c[e] = curl_dir(dir, a[e], b[e], kx[i], ky[j], kz[k]);
+}
+
+// the three components of curl (X, Y, Z) in one pass:
+// (curlk(Y,Z,1), curlk(X,Z,2), curlk(X,Y,3)); outputs distinct from inputs
+__global__ void k_curl3(const Cp* __restrict__ X, const Cp* __restrict__ Y,
+                        const Cp* __restrict__ Z, Cp* __restrict__ ox, Cp* __restrict__ oy,
+                        Cp* __restrict__ oz, long nc, SIdx ix, const double* __restrict__ kx,
+                        const double* __restrict__ ky, const double* __restrict__ kz) {
+#pragma clang fp contract(off)
+  const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= nc) return;
+  int k, j;
+  long i;
+  ix.split(e, k, j, i);
+  const double kxi = kx[i], kyj = ky[j], kzk = kz[k];
+  const Cp x = X[e], y = Y[e], z = Z[e];
+  ox[e] = curl_dir(1, y, z, kxi, kyj, kzk);
+  oy[e] = curl_dir(2, x, z, kxi, kyj, kzk);
+  oz[e] = curl_dir(3, x, y, kxi, kyj, kzk);
+}
+
+// A x B on real fields scaled by tmp (pseudospec_hd.f90:386-395): with
+// A = (r1, r2, r3), B = (r4, r5, r6):
+//   o1 = (r2 r6 - r5 r3) tmp, o2 = (r3 r4 - r6 r1) tmp, o3 = (r1 r5 - r4 r2) tmp
+__global__ void k_cross_real(const double* __restrict__ r1, const double* __restrict__ r2,
+                             const double* __restrict__ r3, const double* __restrict__ r4,
+                             const double* __restrict__ r5, const double* __restrict__ r6,
+                             double* __restrict__ o1, double* __restrict__ o2,
+                             double* __restrict__ o3, long n, double tmp) {
+#pragma clang fp contract(off)
+  const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double a1 = r1[e], a2 = r2[e], a3 = r3[e], b1 = r4[e], b2 = r5[e], b3 = r6[e];
+  o1[e] = (a2 * b3 - b2 * a3) * tmp;
+  o2[e] = (a3 * b1 - b3 * a1) * tmp;
+  o3[e] = (a1 * b2 - b1 * a2) * tmp;
+}
+
+// element 0 of each member of up to three fields <- v[c] (a real value; the
+// (kx, ky, kz) = (0, 0, 0) mode), one thread per (member, field)
+__global__ void k_set_mode0(Cp* __restrict__ f0, Cp* __restrict__ f1, Cp* __restrict__ f2,
+                            double v0, double v1, double v2, long nc, int nmem) {
+  const int t = threadIdx.x;
+  if (t >= 3 * nmem) return;
+  const int c = t % 3, m = t / 3;
+  Cp* f = c == 0 ? f0 : (c == 1 ? f1 : f2);
+  if (!f) return;
+  f[long(m) * nc] = mkc<double>(c == 0 ? v0 : (c == 1 ? v1 : v2), 0.0);
+}
+
+// The MHD stage update (include/mhd/mhd_rkstep2.f90:36-76) per element:
+//   v = C1 + dt*(nu*(-kk2 v) - F(N - L) + f)*rmp      (F: fc_filter)
+//   a = C9 + dt*(-mu*j + F(M) + m)*rmp                 (a holds j = curl b)
+struct MhdArgs {
+  Cp* v[3];
+  Cp* a[3];
+  const Cp* c1[3];
+  const Cp* c9[3];
+  const Cp* N[3];
+  const Cp* L[3];
+  const Cp* M[3];
+  const Cp* f[3];
+  const Cp* m[3];
+};
+__global__ void k_mhd_update(MhdArgs g, long nc, SIdx ix, const double* __restrict__ kx,
+                             const double* __restrict__ ky, const double* __restrict__ kz,
+                             const double* __restrict__ fx, const double* __restrict__ fy,
+                             const double* __restrict__ fz, const double* __restrict__ nu,
+                             double mu, double dt, double rmp) {
+#pragma clang fp contract(off)
+  const long e = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= nc) return;
+  int k, j;
+  long i;
+  ix.split(e, k, j, i);
+  const double kk2 = (kx[i] * kx[i] + ky[j] * ky[j]) + kz[k] * kz[k];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    Cp nl = csub(g.N[c][e], g.L[c][e]);
+    nl = cscale(cscale(cscale(nl, fx[i]), fy[j]), fz[k]);
+    const Cp lap = cscale(g.v[c][e], -kk2);
+    const Cp t = cadd(csub(cscale(lap, nu[i]), nl), g.f[c][e]);
+    g.v[c][e] = cadd(g.c1[c][e], cscale(cscale(t, dt), rmp));
+    const Cp em = cscale(cscale(cscale(g.M[c][e], fx[i]), fy[j]), fz[k]);
+    const Cp jm = cscale(g.a[c][e], mu);
+    const Cp u = cadd(cadd(mkc<double>(-jm.x, -jm.y), em), g.m[c][e]);
+    g.a[c][e] = cadd(g.c9[c][e], cscale(cscale(u, dt), rmp));
+  }
+}
+
+// sol_project's first pass with the condition on the potential (bctarget 0,
+// boundary_mod.fpp:227-336): d = -i k.v/kk2, v -= i k d, and the wall data
+// from C1 = d/nz and, for a Robin wall, C2 = i kz C1, at z = 0 and z = Lz as
+// DFT sums: kind 0 (Dirichlet) bc = -C1(wall); kind 2 (Robin) bc = C2 - kh C1
+// at z = 0 and -(C2 + kh C1) at z = Lz.
+__global__ void __launch_bounds__(256) k_poisson_project_pot(
+    Cp* __restrict__ a, Cp* __restrict__ b, Cp* __restrict__ c, Cp* __restrict__ d,
+    Cp* __restrict__ bc, long nlines, int nz, int ny, const double* __restrict__ kx,
+    const double* __restrict__ ky, const double* __restrict__ kz, const Cp* __restrict__ wtop,
+    double tmp, int k0, int k1, const double* __restrict__ khom) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const long ln = long(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (ln >= nlines) return;  // uniform per wavefront
+  const long i = ln / ny;
+  const int j = int(ln - i * ny);
+  const double kxi = kx[i], kyj = ky[j];
+  const bool robin = k0 == 2 || k1 == 2;
+  Cp s0 = mkc<double>(0, 0), s1 = mkc<double>(0, 0);
+  Cp r0 = mkc<double>(0, 0), r1 = mkc<double>(0, 0);
+  for (int k = lane; k < nz; k += 64) {
+    const long e = ln * nz + k;
+    const double kzk = kz[k];
+    const double kk2 = (kxi * kxi + kyj * kyj) + kzk * kzk;
+    const Cp A = a[e], B = b[e], Cc = c[e];
+    Cp sm = cadd(cscale(A, kxi), cscale(B, kyj));
+    sm = cadd(sm, cscale(Cc, kzk));
+    Cp dd;
+    if (kk2 == 0.0) {  // the (0,0,0) mode
+      dd = mkc<double>(0, 0);
+    } else {
+      dd = mkc<double>(sm.y / kk2, -sm.x / kk2);
+    }
+    d[e] = dd;
+    a[e] = csub(A, ik(kxi, dd));
+    b[e] = csub(B, ik(kyj, dd));
+    c[e] = csub(Cc, ik(kzk, dd));
+    const Cp c1 = cscale(dd, tmp);
+    s0 = cadd(s0, c1);
+    s1 = cadd(s1, cmul(c1, wtop[k]));
+    if (robin) {
+      const Cp c2 = ik(kzk, c1);
+      r0 = cadd(r0, c2);
+      r1 = cadd(r1, cmul(c2, wtop[k]));
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s0.x += __shfl_xor(s0.x, off);
+    s0.y += __shfl_xor(s0.y, off);
+    s1.x += __shfl_xor(s1.x, off);
+    s1.y += __shfl_xor(s1.y, off);
+    r0.x += __shfl_xor(r0.x, off);
+    r0.y += __shfl_xor(r0.y, off);
+    r1.x += __shfl_xor(r1.x, off);
+    r1.y += __shfl_xor(r1.y, off);
+  }
+  if (lane == 0) {
+    const double kh = khom[ln];
+    bc[2 * ln] = k0 == 0 ? mkc<double>(-s0.x, -s0.y) : csub(r0, cscale(s0, kh));
+    const Cp t = cadd(r1, cscale(s1, kh));
+    bc[2 * ln + 1] = k1 == 0 ? mkc<double>(-s1.x, -s1.y) : mkc<double>(-t.x, -t.y);
+  }
 }
 
 // poisson_inhomogeneous + first projection (boundary_mod.fpp:227-240, 405-448):
@@ -453,6 +639,85 @@ int op_poisson_project_bc(spx_plan* p, void* a, void* b, void* c, void* d, const
                      static_cast<Cp*>(p->bc), nlines, p->g.nz, p->g.ny, p->d_kx, p->d_ky,
                      p->d_kz, static_cast<const Cp*>(p->d_wtop), 1.0 / double(p->g.nz), r,
                      rk ? 1 : 0, walls_only ? static_cast<Cp*>(p->pw) : nullptr, p->np);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+
+int op_curlk(spx_plan* p, const void* a, const void* b, void* c, int dir) {
+  ProfScope ps_(p, "curlk", 3.0 * double(p->wnc()) * 16.0);
+  hipLaunchKernelGGL(k_curlk, dim3(nblk(long(p->wnc()))), dim3(256), 0, p->stream, CS<double>(a),
+                     CS<double>(b), CS<double>(c), long(p->wnc()), sidx(p), dir, p->d_kx,
+                     p->d_ky, p->d_kz);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+
+int op_curl3(spx_plan* p, const void* const* in, void* const* out) {
+  ProfScope ps_(p, "curl3", 6.0 * double(p->wnc()) * 16.0);
+  hipLaunchKernelGGL(k_curl3, dim3(nblk(long(p->wnc()))), dim3(256), 0, p->stream,
+                     CS<double>(in[0]), CS<double>(in[1]), CS<double>(in[2]), CS<double>(out[0]),
+                     CS<double>(out[1]), CS<double>(out[2]), long(p->wnc()), sidx(p), p->d_kx,
+                     p->d_ky, p->d_kz);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+
+int op_cross_real(spx_plan* p, const void* const* A, const void* const* B, void* const* O,
+                  long n) {
+  const double tmp = 1.0 / std::pow(double(p->g.nx) * double(p->g.ny) * double(p->g.nz), 2);
+  ProfScope ps_(p, "cross_real", 9.0 * double(n) * 8.0);
+  hipLaunchKernelGGL(k_cross_real, dim3(nblk(n)), dim3(256), 0, p->stream,
+                     static_cast<const double*>(A[0]), static_cast<const double*>(A[1]),
+                     static_cast<const double*>(A[2]), static_cast<const double*>(B[0]),
+                     static_cast<const double*>(B[1]), static_cast<const double*>(B[2]),
+                     static_cast<double*>(O[0]), static_cast<double*>(O[1]),
+                     static_cast<double*>(O[2]), n, tmp);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+
+int op_set_mode0(spx_plan* p, void* f0, void* f1, void* f2, const double v[3]) {
+  if (3 * p->mem > 1024) return fail(SPX_ERR_UNSUPPORTED, "set_mode0: too many members");
+  hipLaunchKernelGGL(k_set_mode0, dim3(1), dim3(unsigned(((3 * p->mem + 63) / 64) * 64)), 0,
+                     p->stream, CS<double>(f0), CS<double>(f1), CS<double>(f2), v[0], v[1], v[2],
+                     long(p->nc), p->mem);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+
+int op_mhd_update(spx_plan* p, void* const* v, void* const* a, const void* const* c1,
+                  const void* const* c9, const void* const* N, const void* const* L,
+                  const void* const* M, const void* const* f, const void* const* m,
+                  const double* nu, double mu, double dt, double rmp) {
+  MhdArgs g;
+  for (int c = 0; c < 3; ++c) {
+    g.v[c] = CS<double>(v[c]);
+    g.a[c] = CS<double>(a[c]);
+    g.c1[c] = CS<double>(c1[c]);
+    g.c9[c] = CS<double>(c9[c]);
+    g.N[c] = CS<double>(N[c]);
+    g.L[c] = CS<double>(L[c]);
+    g.M[c] = CS<double>(M[c]);
+    g.f[c] = CS<double>(f[c]);
+    g.m[c] = CS<double>(m[c]);
+  }
+  ProfScope ps_(p, "mhd_update", 3.0 * double(p->wnc()) * 16.0 * 11.0);
+  hipLaunchKernelGGL(k_mhd_update, dim3(nblk(long(p->wnc()))), dim3(256), 0, p->stream, g,
+                     long(p->wnc()), sidx(p), p->d_kx, p->d_ky, p->d_kz, p->d_fkx, p->d_fky,
+                     p->d_fkz, nu, mu, dt, rmp);
+  SPX_CHECK_LAUNCH();
+  return SPX_OK;
+}
+
+int op_poisson_project_pot(spx_plan* p, void* a, void* b, void* c, void* d, int k0, int k1) {
+  if (!p->d_wtop) return fail(SPX_ERR_STATE, "plan has no wall table");
+  const long nlines = p->wlines();
+  ProfScope ps_(p, "poisson_project_pot", double(p->wnc()) * 7.0 * 16.0);
+  hipLaunchKernelGGL(k_poisson_project_pot, dim3(unsigned((nlines + 3) / 4)), dim3(256), 0,
+                     p->stream, CS<double>(a), CS<double>(b), CS<double>(c), CS<double>(d),
+                     static_cast<Cp*>(p->bc), nlines, p->g.nz, p->g.ny, p->d_kx, p->d_ky,
+                     p->d_kz, static_cast<const Cp*>(p->d_wtop), 1.0 / double(p->g.nz), k0, k1,
+                     p->d_khom);
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }

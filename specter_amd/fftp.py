@@ -27,7 +27,7 @@ from ctypes import byref, c_double, c_void_p
 import numpy as np
 
 from . import lib as _l
-from .tables import load_dirichlet_tables
+from .tables import load_dirichlet_tables, load_neumann_tables
 
 
 def _torch():
@@ -72,6 +72,20 @@ class FCPlan:
         _l.check(lib.spx_plan_get_info(ptr, byref(info)))
         self.nx, self.ny, self.nz, self.Cz, self.oz = nx, ny, nz, Cz, oz
         self.Cy, self.oy = Cy, oy
+        # the Neumann reconstructors of the z walls (magnetic boundary
+        # conditions), when the table directory holds them for this order
+        self.neumann = False
+        if Cz > 0:
+            try:
+                tabs = [load_neumann_tables(oz, o, tdir) for o in (1, 2)]
+            except FileNotFoundError:
+                tabs = None
+            if tabs is not None:
+                for o, (Qn, dxp) in zip((1, 2), tabs):
+                    raw = np.concatenate([[dxp], np.asarray(Qn).ravel(order="F")])
+                    _l.check(lib.spx_plan_load_neumann(ptr, o, dp(np.asfortranarray(Q)),
+                                                       dp(np.ascontiguousarray(raw))))
+                self.neumann = True
         self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
         self.nkx, self.np = info.nkx, info.np
         self.z_fft_len, self.z_algorithm = info.z_fft_len, info.z_algorithm
@@ -223,6 +237,32 @@ class FCPlan:
         _l.check(self._lib.spx_fftp1d_complex_to_real_x(self._p, self._spec(inp, "in"),
                                                         self._real(out, "out")))
         return out
+
+    # -- MHD operators ---------------------------------------------------------------
+    def curlk(self, a, b, c, dir: int):
+        """c = component dir of curl from a, b (pseudospec_hd.f90:130-206)."""
+        _l.check(self._lib.spx_curlk(self._p, self._spec(a, "a"), self._spec(b, "b"),
+                                     self._spec(c, "c"), int(dir)))
+        return c
+
+    def prodre(self, a, b, c, d, e, f):
+        """(d,e,f) = (curl A) x A (pseudospec_hd.f90:322-402)."""
+        _l.check(self._lib.spx_prodre(self._p, *(self._spec(x, n) for x, n in
+                                                 zip((a, b, c, d, e, f), "abcdef"))))
+        return d, e, f
+
+    def vector(self, a, b, c, d, e, f, g, h, i):
+        """(g,h,i) = (a,b,c) x (d,e,f) (pseudospec_mhd.f90: vector)."""
+        _l.check(self._lib.spx_vector(self._p, *(self._spec(x, n) for x, n in
+                                                 zip((a, b, c, d, e, f, g, h, i),
+                                                     "abcdefghi"))))
+        return g, h, i
+
+    def a_imposebc_and_project(self, ax, ay, az, ph, bczsta: int, bczend: int):
+        """bboundary.f90:100-189 (wall kinds 0 conducting, 1 insulating)."""
+        _l.check(self._lib.spx_a_imposebc_and_project(
+            self._p, self._spec(ax, "ax"), self._spec(ay, "ay"), self._spec(az, "az"),
+            self._spec(ph, "ph"), int(bczsta), int(bczend)))
 
     def fc_continue(self, inout):
         """fftp.fpp:755-773."""

@@ -14,6 +14,8 @@ MODULE spx
 
    INTEGER(C_INT), PARAMETER :: SPX_FP64 = 0, SPX_FP32_MIXED = 1
    INTEGER(C_INT), PARAMETER :: SPX_SOLVER_HD = 0, SPX_SOLVER_BOUSS = 1, SPX_SOLVER_ROTBOUSS = 2
+   INTEGER(C_INT), PARAMETER :: SPX_SOLVER_MHD = 3
+   INTEGER(C_INT), PARAMETER :: SPX_BWALL_CONDUCTING = 0, SPX_BWALL_INSULATING = 1
 
    TYPE, BIND(C) :: spx_grid
       INTEGER(C_INT) :: nx, ny, nz, Cz, oz
@@ -26,6 +28,8 @@ MODULE spx
       REAL(C_DOUBLE) :: dt, nu, kappa, xmom, xtemp
       REAL(C_DOUBLE) :: v_zsta(2), v_zend(2)
       REAL(C_DOUBLE) :: omega(3)
+      REAL(C_DOUBLE) :: mu, b0(3)
+      INTEGER(C_INT) :: bzsta, bzend
    END TYPE spx_params
 
    INTERFACE
@@ -118,6 +122,29 @@ MODULE spx
       INTEGER(C_INT) FUNCTION spx_gradre(plan, a, b, c, d, e, f) BIND(C, name='spx_gradre')
          IMPORT :: C_INT, C_PTR
          TYPE(C_PTR), VALUE :: plan, a, b, c, d, e, f
+      END FUNCTION
+      ! MHD: Neumann tables, prodre / vector, magnetic BCs (bboundary.f90:100-189)
+      INTEGER(C_INT) FUNCTION spx_plan_load_neumann(plan, ord, Q, Qn) &
+            BIND(C, name='spx_plan_load_neumann')
+         IMPORT :: C_INT, C_PTR, C_DOUBLE
+         TYPE(C_PTR), VALUE :: plan
+         INTEGER(C_INT), VALUE :: ord
+         REAL(C_DOUBLE), INTENT(IN) :: Q(*), Qn(*)
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION spx_prodre(plan, a, b, c, d, e, f) BIND(C, name='spx_prodre')
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: plan, a, b, c, d, e, f
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION spx_vector(plan, a, b, c, d, e, f, g, h, i) &
+            BIND(C, name='spx_vector')
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: plan, a, b, c, d, e, f, g, h, i
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION spx_a_imposebc_and_project(plan, ax, ay, az, ph, bsta, bend) &
+            BIND(C, name='spx_a_imposebc_and_project')
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: plan, ax, ay, az, ph
+         INTEGER(C_INT), VALUE :: bsta, bend
       END FUNCTION
       ! boundary/vboundary.f90:67-151
       INTEGER(C_INT) FUNCTION spx_v_imposebc_and_project(plan, vx, vy, vz, pr, rki, ord, vzsta, vzend) &

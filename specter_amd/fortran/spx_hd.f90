@@ -149,13 +149,16 @@ PROGRAM spx_hd
    CALL SYSTEM_CLOCK(clkp1)
    IF (rot) THEN
       prm = spx_params(SPX_SOLVER_ROTBOUSS, ord, dt, nu, kappa, xmom, xtemp, &
-                       (/vxzsta, vyzsta/), (/vxzend, vyzend/), (/omegax, omegay, omegaz/))
+                       (/vxzsta, vyzsta/), (/vxzend, vyzend/), (/omegax, omegay, omegaz/), &
+                       0.d0, (/0.d0, 0.d0, 0.d0/), 0, 0)
    ELSE IF (bouss) THEN
       prm = spx_params(SPX_SOLVER_BOUSS, ord, dt, nu, kappa, xmom, xtemp, &
-                       (/vxzsta, vyzsta/), (/vxzend, vyzend/), (/0.d0, 0.d0, 0.d0/))
+                       (/vxzsta, vyzsta/), (/vxzend, vyzend/), (/0.d0, 0.d0, 0.d0/), &
+                       0.d0, (/0.d0, 0.d0, 0.d0/), 0, 0)
    ELSE
       prm = spx_params(SPX_SOLVER_HD, ord, dt, nu, 0.d0, 0.d0, 0.d0, &
-                       (/vxzsta, vyzsta/), (/vxzend, vyzend/), (/0.d0, 0.d0, 0.d0/))
+                       (/vxzsta, vyzsta/), (/vxzend, vyzend/), (/0.d0, 0.d0, 0.d0/), &
+                       0.d0, (/0.d0, 0.d0, 0.d0/), 0, 0)
    ENDIF
    CALL spx_check(spx_state_create(plan, prm, state), 'spx_state_create')
    CALL spx_check(spx_state_field(state, 'vx'//C_NULL_CHAR, vx), 'field vx')

@@ -13,7 +13,8 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "spx.h")
 
 SPX_OK, SPX_ERR_ARG, SPX_ERR_UNSUPPORTED, SPX_ERR_HIP, SPX_ERR_ALLOC, SPX_ERR_STATE = range(6)
 SPX_FP64, SPX_FP32_MIXED = 0, 1
-SPX_SOLVER_HD, SPX_SOLVER_BOUSS, SPX_SOLVER_ROTBOUSS = 0, 1, 2
+SPX_SOLVER_HD, SPX_SOLVER_BOUSS, SPX_SOLVER_ROTBOUSS, SPX_SOLVER_MHD = 0, 1, 2, 3
+SPX_BWALL_CONDUCTING, SPX_BWALL_INSULATING = 0, 1
 
 
 class SpxError(RuntimeError):
@@ -40,7 +41,8 @@ class SpxPlanInfo(ctypes.Structure):
 class SpxParams(ctypes.Structure):
     _fields_ = [("solver", c_int), ("ord", c_int), ("dt", c_double), ("nu", c_double),
                 ("kappa", c_double), ("xmom", c_double), ("xtemp", c_double),
-                ("v_zsta", c_double * 2), ("v_zend", c_double * 2), ("omega", c_double * 3)]
+                ("v_zsta", c_double * 2), ("v_zend", c_double * 2), ("omega", c_double * 3),
+                ("mu", c_double), ("b0", c_double * 3), ("bzsta", c_int), ("bzend", c_int)]
 
 
 _P = c_void_p
@@ -93,6 +95,11 @@ _SIGS = {
     "spx_fftp2d_real_to_complex_yz": [_P, _P],
     "spx_fftp2d_complex_to_real_yz": [_P, _P],
     "spx_fftp1d_complex_to_real_x": [_P, _P, _P],
+    "spx_plan_load_neumann": [_P, c_int, POINTER(c_double), POINTER(c_double)],
+    "spx_curlk": [_P, _P, _P, _P, c_int],
+    "spx_prodre": [_P, _P, _P, _P, _P, _P, _P],
+    "spx_vector": [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+    "spx_a_imposebc_and_project": [_P, _P, _P, _P, _P, c_int, c_int],
     "spx_plan_get_members": [_P, POINTER(c_int), POINTER(c_int)],
     "spx_tune_line": [c_int, c_int, c_int, c_int, c_int],
     "spx_tune_wall": [c_int, c_int, c_int],

@@ -1,5 +1,5 @@
-"""Device-resident HD / Boussinesq / rotating Boussinesq solver over libspx (the RK loop of
-specter.fpp:1002-1165 with include/hd and include/bouss stage bodies).
+"""Device-resident HD / Boussinesq / rotating Boussinesq / MHD solver over libspx (the RK
+loop of specter.fpp:1002-1165 with the include/hd, bouss, rotbouss and mhd stage bodies).
 
 All state lives in HBM inside libspx; host code only sets it up, steps it and
 reads diagnostics.
@@ -27,7 +27,7 @@ class _DevView:
 
 @dataclass
 class SolverParams:
-    solver: str = "HD"        # "HD", "BOUSS" or "ROTBOUSS"
+    solver: str = "HD"        # "HD", "BOUSS", "ROTBOUSS" or "MHD"
     ord: int = 2              # RK order ORD
     dt: float = 1e-3
     nu: float = 2e-3
@@ -37,11 +37,15 @@ class SolverParams:
     v_zsta: tuple = (0.0, 0.0)
     v_zend: tuple = (0.0, 0.0)
     omega: tuple = (0.0, 0.0, 0.0)   # ROTBOUSS: (omegax, omegay, omegaz)
+    mu: float = 0.0                  # MHD: magnetic diffusivity
+    b0: tuple = (0.0, 0.0, 0.0)      # MHD: uniform field (bx0, by0, bz0)
+    bzsta: int = 0                   # MHD: magnetic wall at z = 0 (0 conducting, 1 insulating)
+    bzend: int = 0                   # MHD: magnetic wall at z = Lz
 
     def to_c(self) -> _l.SpxParams:
         p = _l.SpxParams()
         kinds = {"HD": _l.SPX_SOLVER_HD, "BOUSS": _l.SPX_SOLVER_BOUSS,
-                 "ROTBOUSS": _l.SPX_SOLVER_ROTBOUSS}
+                 "ROTBOUSS": _l.SPX_SOLVER_ROTBOUSS, "MHD": _l.SPX_SOLVER_MHD}
         if self.solver.upper() not in kinds:
             raise ValueError(f"solver must be one of {sorted(kinds)}, not {self.solver!r}")
         p.solver = kinds[self.solver.upper()]
@@ -50,13 +54,17 @@ class SolverParams:
         p.v_zsta[0], p.v_zsta[1] = self.v_zsta
         p.v_zend[0], p.v_zend[1] = self.v_zend
         p.omega[0], p.omega[1], p.omega[2] = self.omega
+        p.mu = self.mu
+        p.b0[0], p.b0[1], p.b0[2] = self.b0
+        p.bzsta, p.bzend = int(self.bzsta), int(self.bzend)
         return p
 
 
 class Solver:
     """One run: an FCPlan plus a libspx solver state."""
 
-    FIELDS = ("vx", "vy", "vz", "pr", "fx", "fy", "fz", "th", "fs")
+    FIELDS = ("vx", "vy", "vz", "pr", "fx", "fy", "fz", "th", "fs",
+              "ax", "ay", "az", "ph", "mx", "my", "mz")
 
     def __init__(self, plan: FCPlan, params: SolverParams):
         self.plan = plan

@@ -149,7 +149,8 @@ def test_ensemble_stream_properties():
 
 
 def test_solver_params_layout():
-    """SolverParams maps the three solver kinds and the rotation vector onto
+    """SolverParams maps the four solver kinds, the rotation vector and the MHD
+    parameters onto
     spx_params (the C struct's field order: include/spx.h), and rejects
     unknown kinds before any device call."""
     import ctypes
@@ -157,12 +158,16 @@ def test_solver_params_layout():
     from specter_amd import lib as L
     names = [f[0] for f in L.SpxParams._fields_]
     assert names == ["solver", "ord", "dt", "nu", "kappa", "xmom", "xtemp", "v_zsta", "v_zend",
-                     "omega"]
-    assert L.SpxParams.omega.offset == L.SpxParams.v_zend.offset + 2 * ctypes.sizeof(
-        ctypes.c_double)
-    for kind, code in (("HD", 0), ("bouss", 1), ("ROTBOUSS", 2)):
+                     "omega", "mu", "b0", "bzsta", "bzend"]
+    dsz = ctypes.sizeof(ctypes.c_double)
+    assert L.SpxParams.omega.offset == L.SpxParams.v_zend.offset + 2 * dsz
+    assert L.SpxParams.mu.offset == L.SpxParams.omega.offset + 3 * dsz
+    assert L.SpxParams.bzsta.offset == L.SpxParams.b0.offset + 3 * dsz
+    for kind, code in (("HD", 0), ("bouss", 1), ("ROTBOUSS", 2), ("MHD", 3)):
         assert SolverParams(solver=kind).to_c().solver == code
     c = SolverParams(solver="ROTBOUSS", omega=(0.5, -1.0, 2.0)).to_c()
     assert list(c.omega) == [0.5, -1.0, 2.0]
+    c = SolverParams(solver="MHD", mu=0.1, b0=(1.0, 0.0, -2.0), bzsta=0, bzend=1).to_c()
+    assert (c.mu, list(c.b0), c.bzsta, c.bzend) == (0.1, [1.0, 0.0, -2.0], 0, 1)
     with pytest.raises(ValueError):
-        SolverParams(solver="MHD").to_c()
+        SolverParams(solver="MHDB").to_c()
