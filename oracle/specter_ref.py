@@ -946,14 +946,18 @@ def _zinv(g: Grid, a):
 
 
 def energy(g: Grid, a, b, c, kin: int) -> float:
-    """<|v|^2> (kin=1) or the reference's enstrophy (kin=0, which sums
-    wx^2 + wy^2 + wy^2, pseudospec_hd.f90:527,540) (pseudospec_hd.f90:405-635)."""
+    """<|v|^2> (kin=1), the reference's enstrophy (kin=0, which sums
+    wx^2 + wy^2 + wy^2, pseudospec_hd.f90:527,540) or <|curl curl v|^2>
+    (kin=2, the current of mhdcheck, :555-590) (pseudospec_hd.f90:405-635)."""
     if kin == 1:
         comps = (a, b, c)
     elif kin == 0:
         comps = (curlk(g, b, c, 1), curlk(g, a, c, 2), curlk(g, a, c, 2))
+    elif kin == 2:
+        w1, w2, w3 = curlk(g, b, c, 1), curlk(g, a, c, 2), curlk(g, a, b, 3)
+        comps = (curlk(g, w2, w3, 1), curlk(g, w1, w3, 2), curlk(g, w1, w2, 3))
     else:
-        raise ValueError("kin must be 0 or 1")
+        raise ValueError("kin must be 0, 1 or 2")
     R1 = None
     for comp in comps:
         z = _zinv(g, comp)
@@ -1006,6 +1010,22 @@ def bouncheck_z(g: Grid, a, b=None):
     bot = float((R1[0] * tmp).sum() + (2 * R1[1:] * tmp).sum())
     top = float((R2[0] * tmp).sum() + (2 * R2[1:] * tmp).sum())
     return bot, top
+
+
+def mhdcheck(g: Grid, st: "MHDState"):
+    """The values mhdcheck writes with hel = crs = 1 (pseudospec_mhd.f90
+    mhdcheck): balance.txt (eng, ens, cur), energy.txt (engk, engm),
+    helicity.txt (helk, helm), cross.txt (crh, asq)."""
+    engk = energy(g, st.vx, st.vy, st.vz, 1)
+    ens = energy(g, st.vx, st.vy, st.vz, 0)
+    engm = energy(g, st.ax, st.ay, st.az, 0)
+    cur = energy(g, st.ax, st.ay, st.az, 2)
+    helk = helicity(g, st.vx, st.vy, st.vz)
+    helm = helicity(g, st.ax, st.ay, st.az)
+    asq = energy(g, st.ax, st.ay, st.az, 1)
+    c1, c2, c3 = derivk(g, st.ax, 1), derivk(g, st.ay, 2), derivk(g, st.az, 3)
+    crh = cross(g, st.vx, st.vy, st.vz, c1, c2, c3)
+    return (engk + engm, ens, cur, engk, engm, helk, helm, crh, asq)
 
 
 def hdcheck(g: Grid, st: HDState):

@@ -1017,13 +1017,24 @@ static LinComb two(const void* a, int da, const void* b, int db, int sg) {
 
 int spx_energy(spx_plan* p, const void* a, const void* b, const void* c, int kin, double* out) {
   SPX_REQ(p && a && b && c && out, "spx_energy: null argument");
-  SPX_REQ(kin == 0 || kin == 1, "spx_energy: kin must be 0 or 1");
+  SPX_REQ(kin >= 0 && kin <= 2, "spx_energy: kin must be 0, 1 or 2");
+  int rc;
+  if (kin == 2 && !p->MS[0])
+    for (int q = 0; q < 6; ++q)
+      if ((rc = alloc_dev(p, &p->MS[q], p->nc * p->nb * 16))) return rc;
   MemberView mv(p, p->nact);
   std::vector<double> acc(size_t(p->wlines()), 0.0);
-  int rc;
   if (kin == 1) {
     for (const void* q : {a, b, c})
       if ((rc = sumsq_lines(p, one(q, 0), acc, 1.0))) return rc;
+  } else if (kin == 2) {
+    // curl of curl by the reference's curlk calls (pseudospec_hd.f90:555-560)
+    const void* v[3] = {a, b, c};
+    const void* w[3] = {p->MS[0], p->MS[1], p->MS[2]};
+    if ((rc = op_curl3(p, v, p->MS))) return rc;
+    if ((rc = op_curl3(p, w, p->MS + 3))) return rc;
+    for (int q = 3; q < 6; ++q)
+      if ((rc = sumsq_lines(p, one(p->MS[q], 0), acc, 1.0))) return rc;
   } else {
     // curlk(b,c,1) = d_y c - d_z b ; curlk(a,c,2) = d_z a - d_x c, counted twice
     // (pseudospec_hd.f90:527,540)

@@ -191,6 +191,7 @@ struct spx_plan {
   // first use by prodre / vector / the MHD stage; Neumann vectors neu
   // (order 1) and neu2 (order 2) of the z walls (fcgram_mod.f90:347-355)
   void* MR[12] = {nullptr};
+  void* MS[6] = {nullptr};   // spectral scratch of energy(kin = 2), allocated on first use
   double neu[2][16] = {{0}};
   bool neu_loaded[2] = {false, false};
   // mixed plans with walls: float64 boundary-plane buffers of the nonlinear

@@ -166,6 +166,23 @@ class Solver:
         return (p.energy(vx, vy, vz, 1), p.energy(vx, vy, vz, 0),
                 p.cross(vx, vy, vz, fx, fy, fz))
 
+    def mhdcheck(self):
+        """The values mhdcheck writes with hel = crs = 1 (pseudospec_mhd.f90
+        mhdcheck): (eng, ens, cur) of balance.txt, (engk, engm) of energy.txt,
+        (helk, helm) of helicity.txt, (crh, asq) of cross.txt."""
+        p = self.plan
+        v = [self.field(n) for n in ("vx", "vy", "vz")]
+        a = [self.field(n) for n in ("ax", "ay", "az")]
+        engk, ens = p.energy(*v, 1), p.energy(*v, 0)
+        engm, cur = p.energy(*a, 0), p.energy(*a, 2)
+        helk, helm = p.helicity(*v), p.helicity(*a)
+        asq = p.energy(*a, 1)
+        d = [p.empty_spectral() for _ in range(3)]
+        for k, (src, dst) in enumerate(zip(a, d), start=1):
+            p.derivk(src, dst, k)
+        crh = p.cross(*v, *d)
+        return (engk + engm, ens, cur, engk, engm, helk, helm, crh, asq)
+
     def helicity(self):
         """The helicity.txt value of hdcheck with hel=1 (pseudospec_hd.f90:980-1003)."""
         return self.plan.helicity(self.field("vx"), self.field("vy"), self.field("vz"))

@@ -231,3 +231,22 @@ def test_mhd_state_refusals():
                SolverParams(solver="MHD", mu=0.1))
     with pytest.raises(L.SpxError):
         Solver(FCPlan(16, 16, 41, 25, 5), SolverParams(solver="MHD", bzsta=1, bzend=0))
+
+
+@pytest.mark.parametrize("shape", SHAPES[:2], ids=_ids)
+def test_mhdcheck_matches_oracle(shape):
+    """The mhdcheck values (energy kin 0/1/2, helicities, cross helicity) of
+    the same state on the GPU and in the oracle."""
+    from oracle import specter_ref as O
+    from specter_amd import FCPlan, Solver, SolverParams
+    nx, ny, nz, C, d = shape
+    g = O.make_grid(nx, ny, nz, C, d, TABLES)
+    sol = Solver(FCPlan(nx, ny, nz, C, d), SolverParams(solver="MHD", mu=0.1))
+    v, a = _smooth(O, g, 21), _smooth(O, g, 22, amp=0.5)
+    for name, x in zip(("vx", "vy", "vz", "ax", "ay", "az"), (*v, *a)):
+        sol.field(name).copy_(_cu(__import__("torch"), x))
+    zero = np.zeros(g.spec_shape, complex)
+    st = O.MHDState(*v, zero, zero, zero, zero, *a, zero, zero, zero, zero)
+    got, ref = sol.mhdcheck(), O.mhdcheck(g, st)
+    for k, (x, r) in enumerate(zip(got, ref)):
+        assert abs(x - r) <= 1e-12 * max(abs(r), max(map(abs, ref))), (k, x, r)

@@ -303,3 +303,20 @@ def test_ideal_step_exchanges_kinetic_and_magnetic_energy():
     for kw in ({"flip_lorentz": True}, {"flip_emf": True}):
         dk, dm = _ideal_exchange(**kw)
         assert abs(dk + dm) > 0.5 * max(abs(dk), abs(dm)), kw
+
+
+def test_energy_of_curl_curl():
+    """energy(kin=2) is <|curl curl a|^2>: for z-independent a = (cos 2y,
+    sin 3x, 0), curl curl a = (4 cos 2y, 9 sin 3x, 0)."""
+    g = O.make_grid(16, 16, 41, 25, 5, TABLES)
+    z, y, x = np.meshgrid(g.z, g.y, g.x, indexing="ij")
+    ax, ay, zero = np.cos(2 * y), np.sin(3 * x), 0 * x
+    S = [spec(g, w) for w in (ax, ay, zero)]
+    Z = spec(g, zero)
+    ex = O.energy(g, S[0], Z, Z, 1)
+    ey = O.energy(g, Z, S[1], Z, 1)
+    e2 = O.energy(g, *S, 2)
+    assert abs(e2 - (16 * ex + 81 * ey)) < 1e-10 * e2
+    # kin=0 is the reference's enstrophy quirk (wy counted twice, wz never):
+    # here curl a = (0, 0, 3 cos 3x + 2 sin 2y), so kin=0 sees nothing
+    assert abs(O.energy(g, *S, 0)) < 1e-10 * e2
