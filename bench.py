@@ -375,8 +375,7 @@ def _bench(args, w):
             for name in list(real_ic):
                 sol.set_real_field(name, real_ic.pop(name))
                 torch.cuda.empty_cache()
-            host_ic = {n: sol.spectral(n) for n in sol.FIELDS if n != "pr" and
-                       (bouss or n not in ("th", "fs"))}
+            host_ic = {n: sol.spectral(n) for n in sol.fields if n != "pr"}
         else:
             for n, a in host_ic.items():
                 sol.field(n).copy_(torch.from_numpy(a))

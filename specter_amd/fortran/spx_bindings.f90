@@ -119,6 +119,10 @@ MODULE spx
          TYPE(C_PTR), VALUE :: plan, a, b
          INTEGER(C_INT), VALUE :: dir
       END FUNCTION
+      INTEGER(C_INT) FUNCTION spx_laplak(plan, a, b) BIND(C, name='spx_laplak')
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: plan, a, b
+      END FUNCTION
       INTEGER(C_INT) FUNCTION spx_gradre(plan, a, b, c, d, e, f) BIND(C, name='spx_gradre')
          IMPORT :: C_INT, C_PTR
          TYPE(C_PTR), VALUE :: plan, a, b, c, d, e, f

@@ -41,6 +41,18 @@
 ! scalar_constant_diagnostic.txt (<th^2> at the walls, sboundary.f90:167-210,
 ! at full precision) and nusselt.txt (Nu at z = 0 and z = Lz for theta = 1 - z/Lz + th; the
 ! reference computes no Nusselt number) are appended.
+!
+! solver = 'MHD' runs the MHD solver (include/mhd/mhd_rkstep2.f90): &magfield
+! (mu; m0, a0, mkdn, mkup and the mparam/aparam values are read and unused),
+! &magbound (bbczsta, bbczend: 'conducting' or 'vacuum'; x and y periodic) and
+! &uniformb (bx0, by0, bz0) are read, the Neumann tables Q1n{oz}.dat and
+! Q2n{oz}.dat come from tdir, the vector potential ax, ay, az (and
+! optionally mx, my, mz and ph, ph as p'/dt) start from idir like v; every
+! tstep steps ax, ay, az (bx, by, bz when outs >= 1, laplak(a) as jx, jy, jz
+! when outs = 2) and ph/(nx ny dt) are written (specter.fpp:1070-1127); every
+! cstep steps the mhdcheck files are appended with hel = crs = 1
+! (pseudospec_mhd.f90 mhdcheck): balance.txt (eng, ens, cur), energy.txt
+! (engk, engm), helicity.txt (helk, helm), cross.txt (crh, asq).
 !=======================================================================
 PROGRAM spx_hd
    USE, INTRINSIC :: iso_c_binding
@@ -60,6 +72,18 @@ PROGRAM spx_hd
    REAL(8) :: f0 = 0.d0, u0 = 1.d0, kdn = 2.d0, kup = 4.d0, nu = 1.d-3
    CHARACTER(len=15) :: vbczsta = 'noslip', vbczend = 'noslip'
    REAL(8) :: vxzsta = 0.d0, vxzend = 0.d0, vyzsta = 0.d0, vyzend = 0.d0
+   REAL(8) :: m0 = 0.d0, a0 = 1.d0, mkdn = 2.d0, mkup = 4.d0, mu = 1.d-3
+   REAL(8) :: mparam0, mparam1, mparam2, mparam3, mparam4, mparam5, mparam6, mparam7
+   REAL(8) :: mparam8, mparam9, aparam0, aparam1, aparam2, aparam3, aparam4, aparam5
+   REAL(8) :: aparam6, aparam7, aparam8, aparam9
+   REAL(8) :: bx0 = 0.d0, by0 = 0.d0, bz0 = 0.d0
+   CHARACTER(len=15) :: bbcxsta = 'periodic', bbcxend = 'periodic', bbcysta = 'periodic'
+   CHARACTER(len=15) :: bbcyend = 'periodic', bbczsta = 'conducting', bbczend = 'conducting'
+   NAMELIST /magfield/ m0, a0, mkdn, mkup, mu, mparam0, mparam1, mparam2, mparam3, &
+      mparam4, mparam5, mparam6, mparam7, mparam8, mparam9, aparam0, aparam1, aparam2, &
+      aparam3, aparam4, aparam5, aparam6, aparam7, aparam8, aparam9
+   NAMELIST /magbound/ bbcxsta, bbcxend, bbcysta, bbcyend, bbczsta, bbczend
+   NAMELIST /uniformb/ bx0, by0, bz0
    NAMELIST /grid/ nx, ny, nz, Cz, oz, ord, prec, solver
    NAMELIST /boussinesq/ gama, xmom, xtemp
    NAMELIST /rotation/ omegax, omegay, omegaz
@@ -74,7 +98,11 @@ PROGRAM spx_hd
    TYPE(spx_params) :: prm
    TYPE(C_PTR) :: plan, state, dreal, dtmp
    TYPE(C_PTR) :: vx, vy, vz, pr, fx, fy, fz, th, fs
-   LOGICAL :: bouss, rot
+   TYPE(C_PTR) :: ax, ay, az, ph, mx, my, mz, dtmp2, dtmp3
+   LOGICAL :: bouss, rot, mhd
+   INTEGER :: bzs = 0, bze = 0
+   REAL(8) :: engk, engm, cur, helm, crh, asq
+   REAL(8), ALLOCATABLE, TARGET :: Qn(:)
    REAL(8) :: sv, sg, sp, nus(2)
    REAL(8), ALLOCATABLE, TARGET :: R(:, :, :), A(:, :), Q(:, :)
    REAL(8) :: eng, ens, pot, dv, tb, tt, nb, nt
@@ -120,8 +148,19 @@ PROGRAM spx_hd
          REWIND(1)
          READ(1, NML=rotation)
       ENDIF
-   ELSE IF (TRIM(solver) .ne. 'HD') THEN
-      PRINT *, '[ERROR] solver must be HD, BOUSS or ROTBOUSS. Aborting...'
+   ENDIF
+   mhd = TRIM(solver) .eq. 'MHD'
+   IF (mhd) THEN
+      REWIND(1)
+      READ(1, NML=magfield)
+      REWIND(1)
+      READ(1, NML=magbound)
+      REWIND(1)
+      READ(1, NML=uniformb, IOSTAT=ios)
+      bzs = bkind(bbczsta)
+      bze = bkind(bbczend)
+   ELSE IF (.NOT. bouss .AND. TRIM(solver) .ne. 'HD') THEN
+      PRINT *, '[ERROR] solver must be HD, BOUSS, ROTBOUSS or MHD. Aborting...'
       STOP 1
    ENDIF
    CLOSE(1)
@@ -144,10 +183,22 @@ PROGRAM spx_hd
    CALL CPU_TIME(cpup0)
    CALL SYSTEM_CLOCK(clkp0, clkrate)
    CALL spx_check(spx_plan_create(g, A, Q, C_NULL_PTR, plan), 'spx_plan_create')
+   IF (mhd) THEN
+      ! load_neumann_tables (fcgram_mod.f90:260-356) for the magnetic walls
+      ALLOCATE(Qn(1 + oz * oz))
+      CALL read_table(TRIM(tdir)//'/Q1n'//itoa(oz)//'.dat', Qn, 1 + oz * oz)
+      CALL spx_check(spx_plan_load_neumann(plan, 1, Q, Qn), 'Neumann tables 1')
+      CALL read_table(TRIM(tdir)//'/Q2n'//itoa(oz)//'.dat', Qn, 1 + oz * oz)
+      CALL spx_check(spx_plan_load_neumann(plan, 2, Q, Qn), 'Neumann tables 2')
+   ENDIF
    CALL spx_check(spx_plan_sync(plan), 'sync')
    CALL CPU_TIME(cpup1)
    CALL SYSTEM_CLOCK(clkp1)
-   IF (rot) THEN
+   IF (mhd) THEN
+      prm = spx_params(SPX_SOLVER_MHD, ord, dt, nu, 0.d0, 0.d0, 0.d0, &
+                       (/vxzsta, vyzsta/), (/vxzend, vyzend/), (/0.d0, 0.d0, 0.d0/), &
+                       mu, (/bx0, by0, bz0/), bzs, bze)
+   ELSE IF (rot) THEN
       prm = spx_params(SPX_SOLVER_ROTBOUSS, ord, dt, nu, kappa, xmom, xtemp, &
                        (/vxzsta, vyzsta/), (/vxzend, vyzend/), (/omegax, omegay, omegaz/), &
                        0.d0, (/0.d0, 0.d0, 0.d0/), 0, 0)
@@ -172,12 +223,25 @@ PROGRAM spx_hd
       CALL spx_check(spx_state_field(state, 'th'//C_NULL_CHAR, th), 'field th')
       CALL spx_check(spx_state_field(state, 'fs'//C_NULL_CHAR, fs), 'field fs')
    ENDIF
+   IF (mhd) THEN
+      CALL spx_check(spx_state_field(state, 'ax'//C_NULL_CHAR, ax), 'field ax')
+      CALL spx_check(spx_state_field(state, 'ay'//C_NULL_CHAR, ay), 'field ay')
+      CALL spx_check(spx_state_field(state, 'az'//C_NULL_CHAR, az), 'field az')
+      CALL spx_check(spx_state_field(state, 'ph'//C_NULL_CHAR, ph), 'field ph')
+      CALL spx_check(spx_state_field(state, 'mx'//C_NULL_CHAR, mx), 'field mx')
+      CALL spx_check(spx_state_field(state, 'my'//C_NULL_CHAR, my), 'field my')
+      CALL spx_check(spx_state_field(state, 'mz'//C_NULL_CHAR, mz), 'field mz')
+   ENDIF
 
    ALLOCATE(R(nx, ny, nz))
    rbytes = INT(nx, C_SIZE_T) * ny * nz * 8
    cbytes = INT(nz, C_SIZE_T) * ny * (nx / 2 + 1) * 16
    CALL spx_check(spx_dev_alloc(rbytes, dreal), 'spx_dev_alloc')
    CALL spx_check(spx_dev_alloc(cbytes, dtmp), 'spx_dev_alloc')
+   IF (mhd) THEN
+      CALL spx_check(spx_dev_alloc(cbytes, dtmp2), 'spx_dev_alloc')
+      CALL spx_check(spx_dev_alloc(cbytes, dtmp3), 'spx_dev_alloc')
+   ENDIF
 
    ! restart: physical fields -> fftp3d_real_to_complex (specter.fpp:894-899)
    WRITE(ext, '(i4.4)') stat
@@ -190,6 +254,22 @@ PROGRAM spx_hd
    IF (bouss) THEN
       CALL load_field('th', .TRUE., th, have)
       CALL load_field('fs', .FALSE., fs, have)
+   ENDIF
+   IF (mhd) THEN
+      CALL load_field('ax', .TRUE., ax, have)
+      CALL load_field('ay', .TRUE., ay, have)
+      CALL load_field('az', .TRUE., az, have)
+      CALL load_field('mx', .FALSE., mx, have)
+      CALL load_field('my', .FALSE., my, have)
+      CALL load_field('mz', .FALSE., mz, have)
+      ! electric potential phi' = dt*phi in the mixed domain (specter.fpp:949-958)
+      CALL read_raw(TRIM(idir)//'/ph.'//ext//'.out', have)
+      IF (have) THEN
+         R(:, :, np+1:nz) = 0.d0
+         R = R * dt
+         CALL spx_check(spx_memcpy_h2d(plan, dreal, C_LOC(R), rbytes), 'h2d ph')
+         CALL spx_check(spx_fftp2d_real_to_complex_xy(plan, dreal, ph), 'ph r2c')
+      ENDIF
    ENDIF
    ! pressure p' = dt*p in the mixed domain (specter.fpp:901-910)
    CALL read_raw(TRIM(idir)//'/pr.'//ext//'.out', have)
@@ -224,6 +304,10 @@ PROGRAM spx_hd
       OPEN(7, file='scalar_constant_diagnostic.txt', position='append')
       OPEN(8, file='nusselt.txt', position='append')
    ENDIF
+   IF (mhd) THEN
+      OPEN(5, file='energy.txt', position='append')
+      OPEN(6, file='cross.txt', position='append')
+   ENDIF
    IF (bench .ge. 1) THEN
       CALL spx_check(spx_plan_sync(plan), 'sync')
       CALL CPU_TIME(cpu0)
@@ -244,17 +328,37 @@ PROGRAM spx_hd
          CALL store_field('vx', vx)
          CALL store_field('vy', vy)
          CALL store_field('vz', vz)
-         CALL store_pressure()
+         CALL store_pressure('pr', pr)
          IF (bouss) CALL store_field('th', th)
+         IF (mhd) THEN
+            IF (outs .ge. 1) THEN
+               CALL store_curl('bx', az, 2, ay, 3)
+               CALL store_curl('by', ax, 3, az, 1)
+               CALL store_curl('bz', ay, 1, ax, 2)
+            ENDIF
+            IF (outs .eq. 2) THEN
+               CALL store_lap('jx', ax)
+               CALL store_lap('jy', ay)
+               CALL store_lap('jz', az)
+            ENDIF
+            CALL store_field('ax', ax)
+            CALL store_field('ay', ay)
+            CALL store_field('az', az)
+            CALL store_pressure('ph', ph)
+         ENDIF
       ENDIF
       IF (timec .eq. cstep .AND. bench .eq. 0) THEN
          timec = 0
-         CALL spx_check(spx_energy(plan, vx, vy, vz, 1, eng), 'energy')
-         CALL spx_check(spx_energy(plan, vx, vy, vz, 0, ens), 'enstrophy')
-         CALL spx_check(spx_helicity(plan, vx, vy, vz, khe), 'helicity')
-         CALL spx_check(spx_cross(plan, vx, vy, vz, fx, fy, fz, pot), 'cross')
-         WRITE(2, '(1P E13.6,2E23.16,E24.16)') (t-1)*dt, eng, ens, pot
-         WRITE(9, '(1P E13.6,E24.16)') (t-1)*dt, khe
+         IF (mhd) THEN
+            CALL mhd_check(t)
+         ELSE
+            CALL spx_check(spx_energy(plan, vx, vy, vz, 1, eng), 'energy')
+            CALL spx_check(spx_energy(plan, vx, vy, vz, 0, ens), 'enstrophy')
+            CALL spx_check(spx_helicity(plan, vx, vy, vz, khe), 'helicity')
+            CALL spx_check(spx_cross(plan, vx, vy, vz, fx, fy, fz, pot), 'cross')
+            WRITE(2, '(1P E13.6,2E23.16,E24.16)') (t-1)*dt, eng, ens, pot
+            WRITE(9, '(1P E13.6,E24.16)') (t-1)*dt, khe
+         ENDIF
          CALL spx_check(spx_divergence(plan, vx, vy, vz, dv), 'divergence')
          CALL spx_check(spx_bouncheck_z(plan, vx, vy, tb, tt), 'bouncheck')
          CALL spx_check(spx_bouncheck_z(plan, vz, C_NULL_PTR, nb, nt), 'bouncheck')
@@ -288,6 +392,12 @@ PROGRAM spx_hd
       CLOSE(4)
       CLOSE(7)
       CLOSE(8)
+   ENDIF
+   IF (mhd) THEN
+      CLOSE(5)
+      CLOSE(6)
+      CALL spx_check(spx_dev_free(dtmp2), 'free')
+      CALL spx_check(spx_dev_free(dtmp3), 'free')
    ENDIF
 
    CALL spx_check(spx_plan_sync(plan), 'sync')
@@ -424,14 +534,61 @@ CONTAINS
       CALL write_raw(name)
    END SUBROUTINE store_curl
 
-   ! pressure p = p'/dt from the mixed (z, ky, kx) domain (specter.fpp:1041-1053)
-   SUBROUTINE store_pressure()
-      CALL spx_check(spx_memcpy_d2d(plan, dtmp, pr, cbytes), 'd2d pr')
-      CALL spx_check(spx_fftp2d_complex_to_real_xy(plan, dtmp, dreal), 'c2r pr')
-      CALL spx_check(spx_memcpy_d2h(plan, C_LOC(R), dreal, rbytes), 'd2h pr')
+   ! pressure p = p'/dt (or the electric potential phi = phi'/dt) from the
+   ! mixed (z, ky, kx) domain (specter.fpp:1041-1053, 1114-1126)
+   SUBROUTINE store_pressure(name, src)
+      CHARACTER(len=*), INTENT(IN) :: name
+      TYPE(C_PTR), INTENT(IN) :: src
+      CALL spx_check(spx_memcpy_d2d(plan, dtmp, src, cbytes), 'd2d '//name)
+      CALL spx_check(spx_fftp2d_complex_to_real_xy(plan, dtmp, dreal), 'c2r '//name)
+      CALL spx_check(spx_memcpy_d2h(plan, C_LOC(R), dreal, rbytes), 'd2h '//name)
       R = R / (REAL(nx, 8) * REAL(ny, 8) * dt)
-      CALL write_raw('pr')
+      CALL write_raw(name)
    END SUBROUTINE store_pressure
+
+   ! laplak(a) as a/(nx ny nz) in physical space (specter.fpp:1094-1104)
+   SUBROUTINE store_lap(name, src)
+      CHARACTER(len=*), INTENT(IN) :: name
+      TYPE(C_PTR), INTENT(IN) :: src
+      CALL spx_check(spx_laplak(plan, src, dtmp), 'laplak '//name)
+      CALL spx_check(spx_fftp3d_complex_to_real(plan, dtmp, dreal), 'c2r '//name)
+      CALL spx_check(spx_memcpy_d2h(plan, C_LOC(R), dreal, rbytes), 'd2h '//name)
+      R = R / (REAL(nx, 8) * REAL(ny, 8) * REAL(nz, 8))
+      CALL write_raw(name)
+   END SUBROUTINE store_lap
+
+   ! mhdcheck with hel = crs = 1 (pseudospec_mhd.f90 mhdcheck)
+   SUBROUTINE mhd_check(tt_)
+      INTEGER, INTENT(IN) :: tt_
+      CALL spx_check(spx_energy(plan, vx, vy, vz, 1, engk), 'energy')
+      CALL spx_check(spx_energy(plan, vx, vy, vz, 0, ens), 'enstrophy')
+      CALL spx_check(spx_energy(plan, ax, ay, az, 0, engm), 'magnetic energy')
+      CALL spx_check(spx_energy(plan, ax, ay, az, 2, cur), 'current')
+      CALL spx_check(spx_helicity(plan, vx, vy, vz, khe), 'helicity')
+      CALL spx_check(spx_helicity(plan, ax, ay, az, helm), 'magnetic helicity')
+      CALL spx_check(spx_energy(plan, ax, ay, az, 1, asq), 'asq')
+      CALL spx_check(spx_derivk(plan, ax, dtmp, 1), 'derivk ax')
+      CALL spx_check(spx_derivk(plan, ay, dtmp2, 2), 'derivk ay')
+      CALL spx_check(spx_derivk(plan, az, dtmp3, 3), 'derivk az')
+      CALL spx_check(spx_cross(plan, vx, vy, vz, dtmp, dtmp2, dtmp3, crh), 'cross')
+      WRITE(2, '(1P E13.6,3E23.16)') (tt_-1)*dt, engk + engm, ens, cur
+      WRITE(5, '(1P E13.6,2E23.16)') (tt_-1)*dt, engk, engm
+      WRITE(9, '(1P E13.6,2E24.16)') (tt_-1)*dt, khe, helm
+      WRITE(6, '(1P E13.6,E23.16,E24.16)') (tt_-1)*dt, crh, asq
+   END SUBROUTINE mhd_check
+
+   ! b_parsebc (bboundary.f90:20-41) for z walls: conducting 0, vacuum 1
+   INTEGER FUNCTION bkind(str)
+      CHARACTER(len=*), INTENT(IN) :: str
+      IF (TRIM(str) .eq. 'conducting') THEN
+         bkind = SPX_BWALL_CONDUCTING
+      ELSE IF (TRIM(str) .eq. 'vacuum') THEN
+         bkind = SPX_BWALL_INSULATING
+      ELSE
+         PRINT *, '[ERROR] Unknown magnetic boundary condition in z: ', TRIM(str)
+         STOP 1
+      ENDIF
+   END FUNCTION bkind
 
    ! x.txt, y.txt, z.txt (specter.fpp:683-765: periodic x, y; z on the
    ! physical points of the FC-Gram grid)

@@ -75,6 +75,17 @@ class Solver:
         self._s = st
         self.bouss = params.solver.upper() in ("BOUSS", "ROTBOUSS")
 
+    @property
+    def fields(self) -> tuple:
+        """The state fields this solver kind holds (a subset of FIELDS)."""
+        kind = self.params.solver.upper()
+        base = ("vx", "vy", "vz", "pr", "fx", "fy", "fz")
+        if kind in ("BOUSS", "ROTBOUSS"):
+            return base + ("th", "fs")
+        if kind == "MHD":
+            return base + ("ax", "ay", "az", "ph", "mx", "my", "mz")
+        return base
+
     def close(self):
         if getattr(self, "_s", None):
             self._lib.spx_state_destroy(self._s)

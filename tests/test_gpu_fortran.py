@@ -204,3 +204,98 @@ def test_fortran_driver_boussinesq(tmp_path, solver):
     assert np.abs(out - snap).max() <= 4 * np.finfo(float).eps * np.abs(snap).max()
     ref = O.fftp3d_complex_to_real(g, snap_or)[:npz] / (nx * ny * nz)
     assert rel_l2(out, ref) < 1e-9
+
+
+MHD_PARAM = """&magfield
+m0 = 0.0, a0 = 1.0, mkdn = 2.0, mkup = 4.0, mu = {mu}, mparam0 = 0.0, aparam9 = 0.0
+/
+&magbound
+bbcxsta = "periodic", bbcxend = "periodic", bbcysta = "periodic", bbcyend = "periodic",
+bbczsta = "{bsta}", bbczend = "{bend}"
+/
+&uniformb
+bx0 = {bx0}, by0 = {by0}, bz0 = {bz0}
+/
+"""
+
+
+def test_fortran_driver_mhd(tmp_path):
+    """solver = MHD with &magfield / &magbound / &uniformb: the mhdcheck files
+    (balance, energy, helicity, cross) and the ax, bx, ph outputs against the
+    Python-driven run of the same library and the oracle."""
+    from oracle import specter_ref as O
+    from specter_amd import FCPlan, Solver, SolverParams, ic
+    if not os.path.exists(BIN):
+        pytest.fail("Fortran driver not built (specter_amd.build.build_fortran)")
+    nx, ny, nz, C, d, ordr, dt, nu, mu, step, cstep = 16, 16, 41, 25, 5, 2, 1e-3, 5e-2, 4e-2, 7, 2
+    b0 = (0.2, 0.0, 0.3)
+    tstep = 5
+    npz = nz - C
+    (tmp_path / "in").mkdir()
+    (tmp_path / "out").mkdir()
+    (tmp_path / "parameter.inp").write_text(PARAM.format(
+        nx=nx, ny=ny, nz=nz, C=C, d=d, ord=ordr, tdir=TABLES, dt=dt, step=step,
+        tstep=tstep, cstep=cstep, nu=nu, solver="MHD", kappa=1e-3, ox=0.0, oy=0.0, oz=0.0)
+        + MHD_PARAM.format(mu=mu, bsta="conducting", bend="vacuum", bx0=b0[0], by0=b0[1],
+                           bz0=b0[2]))
+    u = ic.spectrum_peaked_velocity(nx, ny, nz, C, seed=1000)
+    x, y, z = ic.physical_grid(nx, ny, nz, C)
+    zz, yy, xx = np.meshgrid(z, y, x, indexing="ij")
+    s = np.sin(np.pi * zz)
+    ar = [0.3 * s * np.cos(yy), 0.3 * s * np.sin(xx + yy), 0.3 * np.cos(np.pi * zz) * np.cos(xx)]
+    for name, arr in zip(("vx", "vy", "vz", "ax", "ay", "az"), (*u, *ar)):
+        np.ascontiguousarray(arr[:npz]).tofile(tmp_path / "in" / f"{name}.0001.out")
+    res = subprocess.run([BIN], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    files = {n: np.loadtxt(tmp_path / f"{n}.txt", ndmin=2)
+             for n in ("balance", "energy", "helicity", "cross")}
+    for f in files.values():
+        assert f.shape[0] == (step - 1) // cstep
+
+    prm = SolverParams(solver="MHD", ord=ordr, dt=dt, nu=nu, mu=mu, b0=b0, bzsta=0, bzend=1)
+    plan = FCPlan(nx, ny, nz, C, d)
+    sol = Solver(plan, prm)
+    for name, arr in zip(("vx", "vy", "vz", "ax", "ay", "az"), (*u, *ar)):
+        sol.set_real_field(name, arr)
+    g = O.make_grid(nx, ny, nz, C, d, TABLES)
+    neu, neu2 = O.neumann_vector(g, TABLES, 1), O.neumann_vector(g, TABLES, 2)
+    zero = np.zeros(g.spec_shape, complex)
+    sp_ = lambda r: O.fftp3d_real_to_complex(g, r.copy())  # noqa: E731
+    st = O.MHDState(*(sp_(r) for r in u), zero.copy(), zero.copy(), zero.copy(), zero.copy(),
+                    *(sp_(r) for r in ar), zero.copy(), zero.copy(), zero.copy(), zero.copy())
+    rows_py, rows_or, snap = [], [], None
+    timec = 0
+    for t in range(1, step + 1):
+        if t == tstep + 1:
+            snap = {n: sol.real_field(n) for n in ("ax", "ay", "az")}
+            snap["ph"] = plan.fftp2d_complex_to_real_xy(sol.field("ph").clone(),
+                                                        plan.empty_real()).cpu().numpy()
+            snap_or = {n: getattr(st, n).copy() for n in ("ax", "ay", "az")}
+        if timec == cstep:
+            timec = 0
+            rows_py.append(((t - 1) * dt, *sol.mhdcheck()))
+            rows_or.append(((t - 1) * dt, *O.mhdcheck(g, st)))
+        sol.step(1)
+        O.mhd_step(g, st, ordr, dt, nu, mu, b0, neu, neu2, 0, 1)
+        timec += 1
+    rows_py, rows_or = np.array(rows_py), np.array(rows_or)
+    # columns of mhdcheck: eng ens cur engk engm helk helm crh asq
+    cols = {"balance": [1, 2, 3], "energy": [4, 5], "helicity": [6, 7], "cross": [8, 9]}
+    for n, c in cols.items():
+        assert np.allclose(files[n][:, 1:], rows_py[:, c], rtol=1e-14, atol=1e-15), n
+        scale = np.abs(rows_or[:, c]).max()
+        assert np.abs(files[n][:, 1:] - rows_or[:, c]).max() < 1e-9 * scale, n
+    for name in ("ax", "ay", "az"):
+        out = np.fromfile(tmp_path / "out" / f"{name}.0002.out").reshape(npz, ny, nx)
+        py = snap[name]
+        assert np.abs(out - py).max() <= 4 * np.finfo(float).eps * np.abs(py).max(), name
+        ref = O.fftp3d_complex_to_real(g, snap_or[name].copy())[:npz] / (nx * ny * nz)
+        assert rel_l2(out, ref) < 1e-10, name
+    out = np.fromfile(tmp_path / "out" / "ph.0002.out").reshape(npz, ny, nx)
+    py = snap["ph"][:npz] / (nx * ny * dt)
+    assert np.abs(out - py).max() <= 4 * np.finfo(float).eps * np.abs(py).max()
+    # b = curl a (outs = 1)
+    out = np.fromfile(tmp_path / "out" / "bx.0002.out").reshape(npz, ny, nx)
+    w = O.derivk(g, snap_or["az"], 2) - O.derivk(g, snap_or["ay"], 3)
+    ref = O.fftp3d_complex_to_real(g, w)[:npz] / (nx * ny * nz)
+    assert rel_l2(out, ref) < 1e-10
