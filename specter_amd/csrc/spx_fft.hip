@@ -90,6 +90,28 @@ constexpr int keep_n() {
   else return KEEP;
 }
 
+// idx = first, first + step, ... < count: wr(idx, rd(idx)).  KN > 0 (the
+// compiled plans, count <= first + KN*step): every LDS read is issued before
+// the first global store instead of one read-wait-store round trip each.
+template <int KN, typename V, typename Rd, typename Wr>
+__device__ __forceinline__ void lds_to_global(int first, int step, int count, Rd rd, Wr wr) {
+  if constexpr (KN > 0) {
+    V v[KN];
+#pragma unroll
+    for (int q = 0; q < KN; ++q) {
+      const int idx = first + q * step;
+      if (idx < count) v[q] = rd(idx);
+    }
+#pragma unroll
+    for (int q = 0; q < KN; ++q) {
+      const int idx = first + q * step;
+      if (idx < count) wr(idx, v[q]);
+    }
+  } else {
+    for (int idx = first; idx < count; idx += step) wr(idx, rd(idx));
+  }
+}
+
 // XCD-aware tile order of a 2D grid.  Workgroups are handed to the 8 XCDs
 // (each with its own L2) round-robin by linear id; the kx tiles of the
 // transposing kernels are narrower than a cache line, so neighbouring tiles
@@ -195,10 +217,27 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_x2s(const cpx<T>* __restrict__ 
   const long pl = long(nkx) * ny;
   {
     const int l = tid & (f.nl - 1), i = i0 + l;
-    for (int k = tid >> f.lgnl; k < nload; k += f.tpl) {
-      Cp v = mkc<T>(0, 0);
-      if (i < nkx) v = X[i + long(nkx) * j + pl * k];
-      lds[l * f.ldl + k] = v;
+    if constexpr (RS >= RS_WAVE) {
+      // compiled plans (tpl 64, n <= KN*tpl): the thread's loads in flight together
+      constexpr int KN = keep_n<RS>();
+      Cp v[KN];
+#pragma unroll
+      for (int q = 0; q < KN; ++q) {
+        const int k = (tid >> f.lgnl) + q * f.tpl;
+        v[q] = mkc<T>(0, 0);
+        if (i < nkx && k < nload) v[q] = X[i + long(nkx) * j + pl * k];
+      }
+#pragma unroll
+      for (int q = 0; q < KN; ++q) {
+        const int k = (tid >> f.lgnl) + q * f.tpl;
+        if (k < nload) lds[l * f.ldl + k] = v[q];
+      }
+    } else {
+      for (int k = tid >> f.lgnl; k < nload; k += f.tpl) {
+        Cp v = mkc<T>(0, 0);
+        if (i < nkx) v = X[i + long(nkx) * j + pl * k];
+        lds[l * f.ldl + k] = v;
+      }
     }
   }
   __syncthreads();
@@ -245,8 +284,11 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_x2s(const cpx<T>* __restrict__ 
     zline_dft<T, RS>(xl, f, tws, lane);
   }
   const int i = i0 + line;
-  if (i < nkx)
-    for (int k = lane; k < n; k += f.tpl) S[k + long(n) * (j + long(ny) * i)] = xl[k];
+  if (i < nkx) {
+    Cp* so = S + long(n) * (j + long(ny) * i);
+    lds_to_global<0, Cp>(lane, f.tpl, n, [&](int k) { return xl[k]; },
+                         [&](int k, Cp v) { so[k] = v; });
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -384,9 +426,14 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_s2x(const cpx<TI>* __restrict__
   if (!nofft) zline_dft<T, RS>(xl, f, tws, lane);
   const long pl = long(nkx) * ny;
   const int l = tid & (f.nl - 1), ii = i0 + l;
-  if (ii < nkx)
-    for (int k = tid >> f.lgnl; k < nzout; k += f.tpl)
-      X[ii + long(nkx) * j + pl * k] = cconj(lds[l * f.ldl + k]);
+  // (batched stores measured slower here: the z line kernels are VALU-bound)
+  constexpr int KB = 0;
+  if (ii < nkx) {
+    Cp* xo = X + ii + long(nkx) * j;
+    lds_to_global<KB, Cp>(
+        tid >> f.lgnl, f.tpl, nzout, [&](int k) { return cconj(lds[l * f.ldl + k]); },
+        [&](int k, Cp v) { xo[pl * k] = v; });
+  }
   if constexpr (DUAL) {
     __syncthreads();
 #pragma unroll
@@ -399,9 +446,12 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_s2x(const cpx<TI>* __restrict__
     }
     __syncthreads();
     zline_dft<T, RS>(xl, f, tws, lane);
-    if (ii < nkx)
-      for (int k = tid >> f.lgnl; k < nzout; k += f.tpl)
-        X2[ii + long(nkx) * j + pl * k] = cconj(lds[l * f.ldl + k]);
+    if (ii < nkx) {
+      Cp* xo = X2 + ii + long(nkx) * j;
+      lds_to_global<KB, Cp>(
+          tid >> f.lgnl, f.tpl, nzout, [&](int k) { return cconj(lds[l * f.ldl + k]); },
+          [&](int k, Cp v) { xo[pl * k] = v; });
+    }
   }
 }
 
@@ -426,11 +476,30 @@ __global__ void __launch_bounds__(kMaxBlock) k_y(const cpx<T>* Xin, cpx<T>* Xout
   const int l = tid & (f.nl - 1), ii = i0 + l;
   const bool cont = forward && Cy > 0;
   const int nload = cont ? npy : ny;
-  for (int jy = tid >> f.lgnl; jy < nload; jy += f.tpl) {
-    Cp v = mkc<T>(0, 0);
-    if (ii < nkx) v = Xin[base + ii + long(nkx) * jy];
-    if (!forward) v.y = -v.y;
-    lds[l * f.ldl + jy] = v;
+  if constexpr (RS >= RS_WAVE) {
+    // compiled plans (tpl 64, ny <= KN*tpl): all of a thread's loads in
+    // flight at once instead of one load-wait-store round trip per element
+    constexpr int KN = keep_n<RS>();
+    Cp v[KN];
+#pragma unroll
+    for (int q = 0; q < KN; ++q) {
+      const int jy = (tid >> f.lgnl) + q * f.tpl;
+      v[q] = mkc<T>(0, 0);
+      if (ii < nkx && jy < nload) v[q] = Xin[base + ii + long(nkx) * jy];
+    }
+#pragma unroll
+    for (int q = 0; q < KN; ++q) {
+      const int jy = (tid >> f.lgnl) + q * f.tpl;
+      if (!forward) v[q].y = -v[q].y;
+      if (jy < nload) lds[l * f.ldl + jy] = v[q];
+    }
+  } else {
+    for (int jy = tid >> f.lgnl; jy < nload; jy += f.tpl) {
+      Cp v = mkc<T>(0, 0);
+      if (ii < nkx) v = Xin[base + ii + long(nkx) * jy];
+      if (!forward) v.y = -v.y;
+      lds[l * f.ldl + jy] = v;
+    }
   }
   __syncthreads();
   if (cont) {
@@ -446,12 +515,18 @@ __global__ void __launch_bounds__(kMaxBlock) k_y(const cpx<T>* Xin, cpx<T>* Xout
     }
   }
   zline_dft<T, RS>(xl, f, tws, lane);
-  if (ii < nkx)
-    for (int jy = tid >> f.lgnl; jy < ny; jy += f.tpl) {
-      Cp v = lds[l * f.ldl + jy];
-      if (!forward) v.y = -v.y;
-      Xout[base + ii + long(nkx) * jy] = v;
-    }
+  constexpr int KB = RS >= RS_WAVE ? keep_n<RS>() : 0;
+  if (ii < nkx) {
+    Cp* xo = Xout + base + ii;
+    lds_to_global<KB, Cp>(
+        tid >> f.lgnl, f.tpl, ny,
+        [&](int jy) {
+          Cp v = lds[l * f.ldl + jy];
+          if (!forward) v.y = -v.y;
+          return v;
+        },
+        [&](int jy, Cp v) { xo[long(nkx) * jy] = v; });
+  }
   if constexpr (DUAL) {
     __syncthreads();
 #pragma unroll
@@ -461,9 +536,12 @@ __global__ void __launch_bounds__(kMaxBlock) k_y(const cpx<T>* Xin, cpx<T>* Xout
     }
     __syncthreads();
     zline_dft<T, RS>(xl, f, tws, lane);
-    if (ii < nkx)
-      for (int jy = tid >> f.lgnl; jy < ny; jy += f.tpl)
-        Y2[base + ii + long(nkx) * jy] = cconj(lds[l * f.ldl + jy]);
+    if (ii < nkx) {
+      Cp* yo = Y2 + base + ii;
+      lds_to_global<KB, Cp>(
+          tid >> f.lgnl, f.tpl, ny, [&](int jy) { return cconj(lds[l * f.ldl + jy]); },
+          [&](int jy, Cp v) { yo[long(nkx) * jy] = v; });
+    }
   }
 }
 
@@ -479,7 +557,28 @@ __global__ void __launch_bounds__(kMaxBlock) k_xr2c(const T* __restrict__ R, cpx
   const long p = long(blockIdx.x) * f.nl + line;
   const bool ok = p < npairs;
   const T* r0 = R + long(nx) * (2 * (ok ? p : 0));
-  for (int x = lane; x < nx; x += f.tpl) xl[x] = ok ? mkc<T>(r0[x], r0[x + nx]) : mkc<T>(0, 0);
+  // float64 rows: batched loads; float32 rows keep the loop (batching them
+  // tripled the register count and halved the occupancy)
+  if constexpr (RS >= RS_WAVE && sizeof(T) == 8) {
+    constexpr int KN = keep_n<RS>();
+    T a[KN], b[KN];
+#pragma unroll
+    for (int q = 0; q < KN; ++q) {
+      const int x = lane + q * f.tpl;
+      a[q] = b[q] = T(0);
+      if (ok && x < nx) {
+        a[q] = r0[x];
+        b[q] = r0[x + nx];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < KN; ++q) {
+      const int x = lane + q * f.tpl;
+      if (x < nx) xl[x] = mkc<T>(a[q], b[q]);
+    }
+  } else {
+    for (int x = lane; x < nx; x += f.tpl) xl[x] = ok ? mkc<T>(r0[x], r0[x + nx]) : mkc<T>(0, 0);
+  }
   __syncthreads();
   zline_dft<T, RS>(xl, f, tws, lane);
   if (ok) {
@@ -497,31 +596,86 @@ __global__ void __launch_bounds__(kMaxBlock) k_xr2c(const T* __restrict__ R, cpx
 // Load of an x c2r row pair into LDS: Z = A + iB from the two half spectra
 // (FFTW c2r semantics: Im of kx = 0 and kx = nx/2 ignored), conjugated for the
 // backward transform; dirx multiplies both by i kx first.
+// KN > 0 (compiled plans, nx <= KN*tpl): every load of the thread is issued
+// before the first is used.
 template <typename T>
+__device__ __forceinline__ cpx<T> c2r_pair_value(cpx<T> A, cpx<T> B, int m, int k, int h,
+                                                 int dirx, double kxk) {
+  if (dirx) {
+    A = ikmul(T(kxk), A);
+    B = ikmul(T(kxk), B);
+  }
+  if (k == 0 || k == h) {
+    A.y = 0;
+    B.y = 0;
+  }
+  cpx<T> v = m > h ? mkc<T>(A.x + B.y, -A.y + B.x) : mkc<T>(A.x - B.y, A.y + B.x);
+  v.y = -v.y;
+  return v;
+}
+
+template <typename T, int KN = 0>
 __device__ __forceinline__ void load_c2r_pair(cpx<T>* xl, const cpx<T>* __restrict__ a0, int nkx,
                                               int nx, int lane, int tpl, bool ok, int dirx,
                                               const double* __restrict__ kx) {
   const int h = nx / 2;
-  for (int m = lane; m < nx; m += tpl) {
-    cpx<T> v = mkc<T>(0, 0);
-    if (ok) {
-      const bool up = m > h;
-      const int k = up ? nx - m : m;
-      cpx<T> A = a0[k];
-      cpx<T> B = a0[k + nkx];
-      if (dirx) {
-        A = ikmul(T(kx[k]), A);
-        B = ikmul(T(kx[k]), B);
+  if constexpr (KN > 0) {
+    cpx<T> A[KN], B[KN];
+    double kk[KN];
+#pragma unroll
+    for (int q = 0; q < KN; ++q) {
+      const int m = lane + q * tpl;
+      const int k = m > h ? nx - m : m;
+      A[q] = B[q] = mkc<T>(0, 0);
+      kk[q] = 0.0;
+      if (ok && m < nx) {
+        A[q] = a0[k];
+        B[q] = a0[k + nkx];
+        if (dirx) kk[q] = kx[k];
       }
-      if (k == 0 || k == h) {
-        A.y = 0;
-        B.y = 0;
-      }
-      v = up ? mkc<T>(A.x + B.y, -A.y + B.x) : mkc<T>(A.x - B.y, A.y + B.x);
-      v.y = -v.y;
     }
-    xl[m] = v;
+#pragma unroll
+    for (int q = 0; q < KN; ++q) {
+      const int m = lane + q * tpl;
+      const int k = m > h ? nx - m : m;
+      if (m < nx) xl[m] = ok ? c2r_pair_value(A[q], B[q], m, k, h, dirx, kk[q]) : mkc<T>(0, 0);
+    }
+  } else {
+    for (int m = lane; m < nx; m += tpl) {
+      cpx<T> v = mkc<T>(0, 0);
+      if (ok) {
+        const bool up = m > h;
+        const int k = up ? nx - m : m;
+        cpx<T> A = a0[k];
+        cpx<T> B = a0[k + nkx];
+        if (dirx) {
+          A = ikmul(T(kx[k]), A);
+          B = ikmul(T(kx[k]), B);
+        }
+        if (k == 0 || k == h) {
+          A.y = 0;
+          B.y = 0;
+        }
+        v = up ? mkc<T>(A.x + B.y, -A.y + B.x) : mkc<T>(A.x - B.y, A.y + B.x);
+        v.y = -v.y;
+      }
+      xl[m] = v;
+    }
   }
+}
+
+template <int RS>
+constexpr int batch_n() {
+  if constexpr (RS >= RS_WAVE) return keep_n<RS>();
+  else return 0;
+}
+// the fused x stage holds the velocity of its row pair in registers: in
+// float32 batched loads cost occupancy (the element-by-element loop is
+// faster); in float64 they win despite the spills they add
+template <typename T, int RS>
+constexpr int xg_batch() {
+  if constexpr (sizeof(T) == 8) return batch_n<RS>();  // float64: measured faster
+  else return 0;
 }
 
 template <typename T, int RS>
@@ -533,7 +687,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_xc2r(const cpx<T>* __restrict__ X
   const int nx = f.n;
   const long p = long(blockIdx.x) * f.nl + line;
   const bool ok = p < npairs;
-  load_c2r_pair(xl, X + long(nkx) * (2 * (ok ? p : 0)), nkx, nx, lane, f.tpl, ok, dirx, kx);
+  load_c2r_pair<T, batch_n<RS>()>(xl, X + long(nkx) * (2 * (ok ? p : 0)), nkx, nx, lane, f.tpl, ok, dirx, kx);
   __syncthreads();
   zline_dft<T, RS>(xl, f, tws, lane);
   if (ok) {
@@ -577,7 +731,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_xgrad_prod(const cpx<T>* __restri
   const Cp* srcs[3] = {YV, YY, XZ};
   const T* mult[3] = {u, v, w};
   for (int r = 0; r < 3; ++r) {
-    load_c2r_pair(xl, srcs[r] + co, nkx, nx, lane, f.tpl, ok, r == 0 ? 1 : 0, kx);
+    load_c2r_pair<T, batch_n<RS>()>(xl, srcs[r] + co, nkx, nx, lane, f.tpl, ok, r == 0 ? 1 : 0, kx);
     // the velocity values this thread multiplies with, loaded before the
     // transform so that their latency overlaps it
     const T* m = mult[r] + ro;
@@ -656,7 +810,7 @@ __global__ void __launch_bounds__(kMaxBlock, sizeof(T) == 4 ? 4 : 2) k_xgrad_fus
   T vel[3][2][KEEP];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    load_c2r_pair(xl, a.yv[c] + co, nkx, nx, lane, f.tpl, ok, 0, kx);
+    load_c2r_pair<T, xg_batch<T, RS>()>(xl, a.yv[c] + co, nkx, nx, lane, f.tpl, ok, 0, kx);
     // the first barrier also publishes the staged twiddles
     if (c == 0) __syncthreads();
     else line_sync(f.tpl);
@@ -684,7 +838,7 @@ __global__ void __launch_bounds__(kMaxBlock, sizeof(T) == 4 ? 4 : 2) k_xgrad_fus
     const Cp* srcs[3] = {a.yv[c], a.yy[c], a.xz[c]};
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      load_c2r_pair(xl, srcs[r] + co, nkx, nx, lane, f.tpl, ok, r == 0 ? 1 : 0, kx);
+      load_c2r_pair<T, xg_batch<T, RS>()>(xl, srcs[r] + co, nkx, nx, lane, f.tpl, ok, r == 0 ? 1 : 0, kx);
       line_sync(f.tpl);
       line_fft_nosync<T, RS>(xl, f, tws, lane);
       line_sync(f.tpl);
@@ -904,7 +1058,23 @@ __global__ void __launch_bounds__(kMaxBlock) k_proj_pressure(Cd* __restrict__ dd
   const int n = f.n;
   const long ln = long(blockIdx.x) * f.nl + line;
   const bool ok = ln < nlines;
-  for (int k = lane; k < n; k += f.tpl) xl[k] = ok ? cconj(dd[ln * n + k]) : mkc<double>(0, 0);
+  if constexpr (RS >= RS_WAVE) {
+    constexpr int KN = keep_n<RS>();
+    Cd v[KN];
+#pragma unroll
+    for (int q = 0; q < KN; ++q) {
+      const int k = lane + q * f.tpl;
+      v[q] = mkc<double>(0, 0);
+      if (ok && k < n) v[q] = cconj(dd[ln * n + k]);
+    }
+#pragma unroll
+    for (int q = 0; q < KN; ++q) {
+      const int k = lane + q * f.tpl;
+      if (k < n) xl[k] = v[q];
+    }
+  } else {
+    for (int k = lane; k < n; k += f.tpl) xl[k] = ok ? cconj(dd[ln * n + k]) : mkc<double>(0, 0);
+  }
   // (kx, ky) = (0, 0) is the only line with khom = 0; coefficients formed
   // before the transform
   const double kh = ok ? khom[ln] : 1.0;
