@@ -413,13 +413,41 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_s2x(const cpx<TI>* __restrict__
     return v;
   };
   Cp keep[KEEPV ? KN : 1];
+  if constexpr (RS >= RS_WAVE) {
+    // compiled plans: the line's raw values (and kz) in flight together, then
+    // the derivative factor and the LDS writes
+    cpx<TI> raw[KN];
+    double kzv[KN];
+    const double kline = (ok && dirn == 1) ? kx[i] : ((ok && dirn == 2) ? ky[j] : 0.0);
 #pragma unroll
-  for (int q = 0; q < KN; ++q) {
-    const int k = lane + q * f.tpl;
-    if (k < n) {
-      const Cp v = load(k);
-      if constexpr (KEEPV) keep[q] = v;
-      xl[k] = cconj(v);
+    for (int q = 0; q < KN; ++q) {
+      const int k = lane + q * f.tpl;
+      raw[q] = mkc<TI>(0, 0);
+      kzv[q] = kline;
+      if (ok && k < n) {
+        raw[q] = S[base + k];
+        if (dirn == 3) kzv[q] = kz[k];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < KN; ++q) {
+      const int k = lane + q * f.tpl;
+      if (k < n) {
+        Cp v = ccast<T>(raw[q]);
+        if (ok && dirn != 0) v = ikmul(T(kzv[q]), v);
+        if constexpr (KEEPV) keep[q] = v;
+        xl[k] = cconj(v);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < KN; ++q) {
+      const int k = lane + q * f.tpl;
+      if (k < n) {
+        const Cp v = load(k);
+        if constexpr (KEEPV) keep[q] = v;
+        xl[k] = cconj(v);
+      }
     }
   }
   __syncthreads();
@@ -614,31 +642,36 @@ __device__ __forceinline__ cpx<T> c2r_pair_value(cpx<T> A, cpx<T> B, int m, int 
   return v;
 }
 
-template <typename T, int KN = 0>
+template <typename T, int KN = 0, int CH = KN>
 __device__ __forceinline__ void load_c2r_pair(cpx<T>* xl, const cpx<T>* __restrict__ a0, int nkx,
                                               int nx, int lane, int tpl, bool ok, int dirx,
                                               const double* __restrict__ kx) {
   const int h = nx / 2;
   if constexpr (KN > 0) {
-    cpx<T> A[KN], B[KN];
-    double kk[KN];
+    // CH elements in flight at a time (CH < KN bounds the registers: the
+    // chunk loop stays a loop, so the compiler cannot hoist every load)
+#pragma unroll(CH == KN ? KN : 1)
+    for (int c0 = 0; c0 < KN; c0 += CH) {
+      cpx<T> A[CH], B[CH];
+      double kk[CH];
 #pragma unroll
-    for (int q = 0; q < KN; ++q) {
-      const int m = lane + q * tpl;
-      const int k = m > h ? nx - m : m;
-      A[q] = B[q] = mkc<T>(0, 0);
-      kk[q] = 0.0;
-      if (ok && m < nx) {
-        A[q] = a0[k];
-        B[q] = a0[k + nkx];
-        if (dirx) kk[q] = kx[k];
+      for (int q = 0; q < CH; ++q) {
+        const int m = lane + (c0 + q) * tpl;
+        const int k = m > h ? nx - m : m;
+        A[q] = B[q] = mkc<T>(0, 0);
+        kk[q] = 0.0;
+        if (ok && m < nx) {
+          A[q] = a0[k];
+          B[q] = a0[k + nkx];
+          if (dirx) kk[q] = kx[k];
+        }
       }
-    }
 #pragma unroll
-    for (int q = 0; q < KN; ++q) {
-      const int m = lane + q * tpl;
-      const int k = m > h ? nx - m : m;
-      if (m < nx) xl[m] = ok ? c2r_pair_value(A[q], B[q], m, k, h, dirx, kk[q]) : mkc<T>(0, 0);
+      for (int q = 0; q < CH; ++q) {
+        const int m = lane + (c0 + q) * tpl;
+        const int k = m > h ? nx - m : m;
+        if (m < nx) xl[m] = ok ? c2r_pair_value(A[q], B[q], m, k, h, dirx, kk[q]) : mkc<T>(0, 0);
+      }
     }
   } else {
     for (int m = lane; m < nx; m += tpl) {
@@ -669,13 +702,18 @@ constexpr int batch_n() {
   if constexpr (RS >= RS_WAVE) return keep_n<RS>();
   else return 0;
 }
-// the fused x stage holds the velocity of its row pair in registers: in
-// float32 batched loads cost occupancy (the element-by-element loop is
-// faster); in float64 they win despite the spills they add
+// the fused x stage holds the velocity of its row pair in registers, so its
+// loads are batched in chunks
 template <typename T, int RS>
 constexpr int xg_batch() {
-  if constexpr (sizeof(T) == 8) return batch_n<RS>();  // float64: measured faster
-  else return 0;
+  return batch_n<RS>();
+}
+template <typename T, int RS>
+constexpr int xg_chunk() {
+  // four elements in flight at a time: at 512^3 the x stage takes 2.2 ms
+  // (float32) / 4.2 ms (float64) per RK stage against 2.9 / 5.5 with one
+  // and 4.3 / 5.3 with all eight (spills)
+  return batch_n<RS>() >= 4 ? 4 : batch_n<RS>();
 }
 
 template <typename T, int RS>
@@ -810,7 +848,7 @@ __global__ void __launch_bounds__(kMaxBlock, sizeof(T) == 4 ? 4 : 2) k_xgrad_fus
   T vel[3][2][KEEP];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    load_c2r_pair<T, xg_batch<T, RS>()>(xl, a.yv[c] + co, nkx, nx, lane, f.tpl, ok, 0, kx);
+    load_c2r_pair<T, xg_batch<T, RS>(), xg_chunk<T, RS>()>(xl, a.yv[c] + co, nkx, nx, lane, f.tpl, ok, 0, kx);
     // the first barrier also publishes the staged twiddles
     if (c == 0) __syncthreads();
     else line_sync(f.tpl);
@@ -838,7 +876,7 @@ __global__ void __launch_bounds__(kMaxBlock, sizeof(T) == 4 ? 4 : 2) k_xgrad_fus
     const Cp* srcs[3] = {a.yv[c], a.yy[c], a.xz[c]};
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      load_c2r_pair<T, xg_batch<T, RS>()>(xl, srcs[r] + co, nkx, nx, lane, f.tpl, ok, r == 0 ? 1 : 0, kx);
+      load_c2r_pair<T, xg_batch<T, RS>(), xg_chunk<T, RS>()>(xl, srcs[r] + co, nkx, nx, lane, f.tpl, ok, r == 0 ? 1 : 0, kx);
       line_sync(f.tpl);
       line_fft_nosync<T, RS>(xl, f, tws, lane);
       line_sync(f.tpl);
