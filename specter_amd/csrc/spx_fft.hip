@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <cstdint>
 #include <thread>
 #include <unordered_map>
 
@@ -247,27 +248,32 @@ __global__ void __launch_bounds__(kMaxBlock) k_z_x2s(const cpx<T>* __restrict__ 
       const int d2 = 2 * d;
       const long pb = long(nkx) * ny;
       const cpx<double>* xb = Xb + blockIdx.z * (pb * d2) + long(nkx) * j;
+      // the tile's float64 boundary values, staged once behind the line
+      // buffers (16-byte aligned; bnd_stage_bytes at launch): bl[l * d2 + q]
+      cpx<double>* const bl = reinterpret_cast<cpx<double>*>(
+          (reinterpret_cast<uintptr_t>(lds + f.nl * f.ldl) + 15) & ~uintptr_t(15));
       for (int e = tid; e < f.nl * d2; e += blockDim.x) {
         const int l = e / d2, q = e - l * d2, i = i0 + l;
         const int z = q < d ? q : np - d2 + q;
         const cpx<double> v = i < nkx ? xb[i + pb * q] : mkc<double>(0, 0);
         lds[l * f.ldl + z] = mkc<T>(T(v.x), T(v.y));
+        bl[e] = v;
       }
+      __syncthreads();
       // continuation (fftp.fpp:755-773 order): x[top+jj] is plane q = d+jj,
       // x[d-1-jj] is plane q = d-1-jj
       for (int e = tid; e < f.nl * C; e += blockDim.x) {
         const int l = e / C, ii = e - l * C, i = i0 + l;
         double ar = 0.0, ai = 0.0;
         if (i < nkx) {
-          const cpx<double>* xt = xb + i + pb * d;
-          const cpx<double>* xo = xb + i;
+          const cpx<double>* bq = bl + l * d2;
           const double* D0 = dir + ii * d;
           const double* D1 = dir + (C - 1 - ii) * d;
-          const cpx<double> t0 = xt[0], b0 = xo[pb * (d - 1)];
+          const cpx<double> t0 = bq[d], b0 = bq[d - 1];
           ar = D0[0] * t0.x + D1[0] * b0.x;
           ai = D0[0] * t0.y + D1[0] * b0.y;
           for (int jj = 1; jj < d; ++jj) {
-            const cpx<double> ut = xt[pb * jj], ub = xo[pb * (d - 1 - jj)];
+            const cpx<double> ut = bq[d + jj], ub = bq[d - 1 - jj];
             ar = ar + D0[jj] * ut.x;
             ar = ar + D1[jj] * ub.x;
             ai = ai + D0[jj] * ut.y;
@@ -2054,9 +2060,12 @@ static int zfft_x2s_t(spx_plan* p, const void* X, void* S, int nofft, const Line
   ProfScope ps_(p, "z_x2s", double(p->wkx()) * p->g.ny * sizeof(cpx<T>) *
                                 ((nofft ? p->g.nz : p->np) + p->g.nz));
   if (Xb) {
-    SPX_LAUNCH_Z(f, k_z_x2s, T, SPX_COMMA true, grid, blk(f), f.lds_bytes(sizeof(cpx<T>)),
-                  p->stream, C_<T>(X), C_<T>(S), f.dev<cpx<T>>(), p->nkx, p->g.ny, p->np, nofft,
-                  p->d_dir, p->g.Cz, p->g.oz, xstride, C_<double>(Xb));
+    // plus the staged float64 boundary values of the tile (nl * 2d, aligned)
+    const size_t bnd_stage_bytes = size_t(f.nl) * 2 * p->g.oz * sizeof(Cd) + 16;
+    SPX_LAUNCH_Z(f, k_z_x2s, T, SPX_COMMA true, grid, blk(f),
+                  f.lds_bytes(sizeof(cpx<T>)) + bnd_stage_bytes, p->stream, C_<T>(X), C_<T>(S),
+                  f.dev<cpx<T>>(), p->nkx, p->g.ny, p->np, nofft, p->d_dir, p->g.Cz, p->g.oz,
+                  xstride, C_<double>(Xb));
     SPX_CHECK_LAUNCH();
     return SPX_OK;
   }
