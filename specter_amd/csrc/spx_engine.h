@@ -539,11 +539,25 @@ __device__ __forceinline__ void line_dft(cpx<T>* x, const LineFFT<cpx<T>>& f, co
 
 // RkLoad applied to component q's value v at element e = line*nz + k,
 // line = i*ny + j.
+// The same update from loaded values: kl2 = kx^2 + ky^2 of the line, kz, the
+// filter factors fx, fy, fz and the diffusivity nu of the line.
+__device__ __forceinline__ cpx<double> rk_apply(const RkLoad& r, cpx<double> v, cpx<double> nl,
+                                                cpx<double> f, cpx<double> c0, double kl2,
+                                                double kz, double fx, double fy, double fz,
+                                                double nu) {
+#pragma clang fp contract(off)
+  const double kk2 = kl2 + kz * kz;
+  nl = cscale(cscale(cscale(nl, fx), fy), fz);
+  const cpx<double> lap = cscale(v, -kk2);
+  const cpx<double> t = cadd(csub(cscale(lap, nu), nl), f);
+  return cadd(c0, cscale(cscale(t, r.dt), r.rmp));
+}
+
 __device__ __forceinline__ cpx<double> rk_on_load(const RkLoad& r, int q, cpx<double> v, long e,
                                                   long i, int j, int k) {
 #pragma clang fp contract(off)
   const RkLoadComp& c = r.c[q];
-  const double kk2 = (r.kx[i] * r.kx[i] + r.ky[j] * r.ky[j]) + r.kz[k] * r.kz[k];
+  const double kl2 = r.kx[i] * r.kx[i] + r.ky[j] * r.ky[j];
   cpx<double> nl;
   if (r.nl32) {
     const cpx<float> t = static_cast<const cpx<float>*>(c.nl)[e];
@@ -551,11 +565,9 @@ __device__ __forceinline__ cpx<double> rk_on_load(const RkLoad& r, int q, cpx<do
   } else {
     nl = static_cast<const cpx<double>*>(c.nl)[e];
   }
-  nl = cscale(cscale(cscale(nl, r.fkx[i]), r.fky[j]), r.fkz[k]);
-  const cpx<double> lap = cscale(v, -kk2);
-  const cpx<double> t =
-      cadd(csub(cscale(lap, c.nu[i]), nl), static_cast<const cpx<double>*>(c.f)[e]);
-  return cadd(static_cast<const cpx<double>*>(c.c0)[e], cscale(cscale(t, r.dt), r.rmp));
+  return rk_apply(r, v, nl, static_cast<const cpx<double>*>(c.f)[e],
+                  static_cast<const cpx<double>*>(c.c0)[e], kl2, r.kz[k], r.fkx[i], r.fky[j],
+                  r.fkz[k], c.nu[i]);
 }
 
 // FC-Gram Dirichlet continuation of nl lines held in LDS (fftp.fpp:755-773):

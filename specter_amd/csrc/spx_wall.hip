@@ -22,10 +22,11 @@
 // from lane j + 16k and keeps D[i][j] in register i / 4 of lane j + 16 (i % 4)
 // (measured).  Each wave holds a range of 16-row kappa blocks of the 16 lines
 // as GEMM C accumulators whose rows are permuted within a block (row
-// 4r + g <-> kappa offset 4g + r), so that lane group g, registers r = 0..3,
-// holds the two consecutive complex values z = 8 block + 2g, +1 of its line
-// (one 32-byte load and store); the same registers are GEMM A's B operand
-// (register s = K step s, kappa offset 4g + s).  A line is read once (with
+// 4r + g <-> kappa offset kap_off(g, r)), so that lane group g, registers
+// r = 0..3, holds the complex values z = 8 block + g and z + 4 of its line (a
+// load instruction reads 64 contiguous bytes of each of its 16 lines); the
+// same registers are GEMM A's B operand (register s = K step s, kappa offset
+// kap_off(g, s)).  A line is read once (with
 // the RK update applied on the load) and written once.  The waves'
 // partial X are summed in LDS in wave order (deterministic), delta is formed
 // there exactly as the reference forms the continuation, and GEMM C
@@ -51,6 +52,10 @@ namespace {
 using Cd = cpx<double>;
 
 __device__ __forceinline__ Cd ikmul_d(double k, Cd v) { return mkc<double>(-(k * v.y), k * v.x); }
+
+// kappa offset (2 z_off + re/im) within a 16-row block held by lane group g in
+// register s: z_off = g + 4 (s / 2), re/im = s % 2
+__host__ __device__ constexpr int kap_off(int g, int s) { return 2 * g + 8 * (s >> 1) + (s & 1); }
 
 __device__ __forceinline__ v4d mfma_f64(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -101,16 +106,17 @@ __device__ __forceinline__ void gemm_a_rows(const double* __restrict__ winv, con
       }
     }
   } else {
-    // lane = i + 16k: row 16m + i = (point qi, re/im qc); K index 4k + s of
-    // block bb = (z = 8 bb + 2k + s/2, re/im s&1); e^{+2 pi i z q / n}
+    // lane = i + 16k: row 16m + i = (point qi, re/im qc); K step s of block
+    // bb, lane group k = (z = 8 bb + k + 4 (s/2), re/im s&1); e^{+2 pi i z q / n}
     const int i = lane & 15, k = lane >> 4;
     const int qc = i & 1;
-    int q[M1 - M0], inc[M1 - M0], idx[M1 - M0];
+    int q4[M1 - M0], inc[M1 - M0], idx[M1 - M0];
 #pragma unroll
     for (int m = M0; m < M1; ++m) {
-      q[m - M0] = qpoint((16 * m + i) >> 1, d, np, nq);
-      inc[m - M0] = (8 * q[m - M0]) % n;
-      idx[m - M0] = int((long long)(8 * b0 + 2 * k) * q[m - M0] % n);
+      const int q = qpoint((16 * m + i) >> 1, d, np, nq);
+      q4[m - M0] = (4 * q) % n;
+      inc[m - M0] = (8 * q) % n;
+      idx[m - M0] = int((long long)(8 * b0 + k) * q % n);
     }
 #pragma unroll
     for (int b = 0; b < KBW; ++b) {
@@ -118,7 +124,7 @@ __device__ __forceinline__ void gemm_a_rows(const double* __restrict__ winv, con
       if (bb < nbk) {
 #pragma unroll
         for (int m = M0; m < M1; ++m) {
-          int i1 = idx[m - M0] + q[m - M0];
+          int i1 = idx[m - M0] + q4[m - M0];
           i1 = i1 >= n ? i1 - n : i1;
           const Cd wa = rl[idx[m - M0]], wb = rl[i1];
           const double a0 = qc ? wa.y : wa.x, a1 = qc ? wa.x : -wa.y;
@@ -150,6 +156,72 @@ __device__ __forceinline__ void gemm_a_rows(const double* __restrict__ winv, con
         slot[e] = slot[e] + acc[m - M0][r];
       }
   }
+}
+
+// Loads of the wave's kappa blocks: lane group g of block bb holds the
+// complex values z = 8 bb + g and z + 4 of its line (row = first element of the
+// line).  Straight-line code: the indices of points past the line end (and of
+// lanes past the last line, which read line 0) are clamped to a valid element
+// and their values replaced by zero, so that every load of the wave can be in
+// flight at once.
+template <int KBW>
+__device__ __forceinline__ void wall_load(v4d (&vr)[KBW], const Cd* __restrict__ src, long row,
+                                          bool okl, int b0, int g, int n) {
+#pragma unroll
+  for (int b = 0; b < KBW; ++b) {
+    const int z0 = 8 * (b0 + b) + g;
+    const bool ok0 = okl && z0 < n, ok1 = okl && z0 + 4 < n;
+    const Cd v0 = src[row + (ok0 ? z0 : 0)], v1 = src[row + (ok1 ? z0 + 4 : 0)];
+    vr[b] = v4d{ok0 ? v0.x : 0.0, ok0 ? v0.y : 0.0, ok1 ? v1.x : 0.0, ok1 ? v1.y : 0.0};
+  }
+}
+
+// The same with the RK update of the stage applied on the load (rk_on_load with
+// the line's factors read once).
+template <int KBW, bool NL32>
+__device__ __forceinline__ void wall_load_rk(v4d (&vr)[KBW], const Cd* __restrict__ src,
+                                             const RkLoad& rk, int comp, long row, bool okl,
+                                             long ki, int kj, int b0, int g, int n) {
+#pragma clang fp contract(off)
+  const RkLoadComp& c = rk.c[comp];
+  const double kxi = rk.kx[ki], kyj = rk.ky[kj];
+  const double kl2 = kxi * kxi + kyj * kyj;
+  const double fx = rk.fkx[ki], fy = rk.fky[kj], nui = c.nu[ki];
+  const Cd* __restrict__ c0p = static_cast<const Cd*>(c.c0);
+  const Cd* __restrict__ fp = static_cast<const Cd*>(c.f);
+#pragma unroll
+  for (int b = 0; b < KBW; ++b) {
+    const int z0 = 8 * (b0 + b) + g;
+    Cd out[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const bool ok = okl && z0 + 4 * h < n;
+      const int z = ok ? z0 + 4 * h : 0;
+      const long e = row + z;
+      Cd nl;
+      if constexpr (NL32) {
+        const cpx<float> t = static_cast<const cpx<float>*>(c.nl)[e];
+        nl = mkc<double>(double(t.x), double(t.y));
+      } else {
+        nl = static_cast<const Cd*>(c.nl)[e];
+      }
+      const Cd u = rk_apply(rk, src[e], nl, fp[e], c0p[e], kl2, rk.kz[z], fx, fy, rk.fkz[z], nui);
+      out[h] = ok ? u : mkc<double>(0, 0);
+    }
+    vr[b] = v4d{out[0].x, out[0].y, out[1].x, out[1].y};
+  }
+}
+
+// GEMM A over all MT row tiles, G tiles at a time (G accumulators live).
+template <int MT, int M0, int G, int KBW, int TAB>
+__device__ __forceinline__ void gemm_a_groups(const double* __restrict__ winv, const Cd* rl,
+                                              const v4d (&vr)[KBW], int b0, int nbk, int lane,
+                                              int wave, int ns, double* red, int n, int np,
+                                              int d, int nq) {
+  constexpr int M1 = M0 + G < MT ? M0 + G : MT;
+  gemm_a_rows<MT, M0, M1, KBW, TAB>(winv, rl, vr, b0, nbk, lane, wave, ns, red, n, np, d, nq);
+  if constexpr (M1 < MT)
+    gemm_a_groups<MT, M1, G, KBW, TAB>(winv, rl, vr, b0, nbk, lane, wave, ns, red, n, np, d, nq);
 }
 
 // MT: 16-row tiles of the 2(2d+C) real partial-IDFT outputs; KT: 4-column
@@ -184,24 +256,13 @@ __global__ void __launch_bounds__(512, WPE) k_wall_mfma(
 
   // ---- load the wave's kappa blocks (RK update applied on the load)
   v4d vr[KBW];
-#pragma unroll
-  for (int b = 0; b < KBW; ++b) {
-    const int bb = b0 + b;
-    const int z0 = 8 * bb + 2 * g;
-    Cd v0 = mkc<double>(0, 0), v1 = mkc<double>(0, 0);
-    if (okl && bb < nbk) {
-      const Cd* src = reinterpret_cast<const Cd*>(V) + ln * n;
-      if (z0 < n) {
-        v0 = src[z0];
-        if (rk_on) v0 = rk_on_load(rk, comp, v0, ln * n + z0, ki, kj, z0);
-      }
-      if (z0 + 1 < n) {
-        v1 = src[z0 + 1];
-        if (rk_on) v1 = rk_on_load(rk, comp, v1, ln * n + z0 + 1, ki, kj, z0 + 1);
-      }
-    }
-    vr[b] = v4d{v0.x, v0.y, v1.x, v1.y};
-  }
+  const Cd* const src = reinterpret_cast<const Cd*>(V);
+  if (!rk_on)
+    wall_load<KBW>(vr, src, lc * n, okl, b0, g, n);
+  else if (rk.nl32)
+    wall_load_rk<KBW, true>(vr, src, rk, comp, lc * n, okl, ki, kj, b0, g, n);
+  else
+    wall_load_rk<KBW, false>(vr, src, rk, comp, lc * n, okl, ki, kj, b0, g, n);
 
   // ---- GEMM A: partial inverse DFT at the 2d + C points, in two row groups
   // (fewer live accumulators), summed over the waves in a fixed order; then
@@ -211,10 +272,11 @@ __global__ void __launch_bounds__(512, WPE) k_wall_mfma(
     for (int e = threadIdx.x; e < n; e += blockDim.x) rl[e] = roots[e];
     __syncthreads();
   }
-  constexpr int G = KBW >= 18 ? MT : 3;
-  gemm_a_rows<MT, 0, G, KBW, TAB>(winv, rl, vr, b0, nbk, lane, wave, ns, red, n, np, d, nq);
-  if constexpr (G < MT)
-    gemm_a_rows<MT, G, MT, KBW, TAB>(winv, rl, vr, b0, nbk, lane, wave, ns, red, n, np, d, nq);
+#ifndef SPX_WALL_G
+#define SPX_WALL_G 1
+#endif
+  constexpr int G = KBW >= 18 ? MT : SPX_WALL_G;
+  gemm_a_groups<MT, 0, G, KBW, TAB>(winv, rl, vr, b0, nbk, lane, wave, ns, red, n, np, d, nq);
   __syncthreads();
   for (int e = threadIdx.x; e < MT * 256; e += blockDim.x) {
     double s = red[e];
@@ -299,10 +361,10 @@ __global__ void __launch_bounds__(512, WPE) k_wall_mfma(
       }
     }
   } else {
-    // lane = i + 16k: row i = kappa offset 4(i&3) + (i>>2) of block bb
-    // (z = 8 bb + 2(i&3) + (i>>3), re/im (i>>2)&1); column 4t + k = (point
+    // lane = i + 16k: row i = kappa offset kap_off(i&3, i>>2) of block bb
+    // (z = 8 bb + (i&3) + 4(i>>3), re/im (i>>2)&1); column 4t + k = (point
     // p = 2t + k/2, re/im k&1); e^{-2 pi i z r_p / n}
-    const int zo = 2 * (li & 3) + (li >> 3), zc = (li >> 2) & 1, pc = g & 1;
+    const int zo = (li & 3) + 4 * (li >> 3), zc = (li >> 2) & 1, pc = g & 1;
     const int nrho = 2 + C;
     // K step outer, kappa block inner: consecutive MFMAs accumulate into
     // different blocks, and one B operand (delta) is live at a time
@@ -327,10 +389,10 @@ __global__ void __launch_bounds__(512, WPE) k_wall_mfma(
 #pragma unroll
     for (int b = 0; b < KBW; ++b) {
       const int bb = b0 + b;
-      const int z0 = 8 * bb + 2 * g;
+      const int z0 = 8 * bb + g;
       if (bb < nbk) {
         if (z0 < n) dst[z0] = mkc<double>(vr[b][0], vr[b][1]);
-        if (z0 + 1 < n) dst[z0 + 1] = mkc<double>(vr[b][2], vr[b][3]);
+        if (z0 + 4 < n) dst[z0 + 4] = mkc<double>(vr[b][2], vr[b][3]);
       }
     }
   }
@@ -419,12 +481,12 @@ int build_wall_mfma(spx_plan* p) {
       for (int m = 0; m < mt; ++m)
         for (int l = 0; l < 64; ++l)
           ta[((size_t(bb) * 4 + s) * mt + m) * 64 + l] =
-              winvT(16 * m + (l & 15), 16 * bb + 4 * (l >> 4) + s);
+              winvT(16 * m + (l & 15), 16 * bb + kap_off(l >> 4, s));
   for (int bb = 0; bb < nbk; ++bb)
     for (int t = 0; t < kt; ++t)
       for (int l = 0; l < 64; ++l)
         tc[(size_t(bb) * kt + t) * 64 + l] =
-            wf(16 * bb + 4 * ((l & 15) & 3) + ((l & 15) >> 2), 4 * t + (l >> 4));
+            wf(16 * bb + kap_off((l & 15) & 3, (l & 15) >> 2), 4 * t + (l >> 4));
   std::vector<double> rt(2 * size_t(n));
   for (int m = 0; m < n; ++m) {
     rt[2 * m] = cr[m];

@@ -250,6 +250,28 @@ def test_wall_update_mfma_matches_transform_path(shape, o, ordr, monkeypatch):
     assert rel_err(t1.cpu().numpy(), t2.cpu().numpy()) < 1e-12
 
 
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_wall_update_fragment_tables_match_roots(shape, monkeypatch):
+    """The two forms of the wall correction's DFT operands -- formed from the
+    roots of unity staged in LDS (the default) and read from per-plan tables
+    in the operands' lane order (SPX_WALL_TAB=0 at plan creation) -- give the
+    same wall update to rounding, and both match the oracle."""
+    torch, O, g, p = _setup(shape)
+    from specter_amd import FCPlan
+    monkeypatch.setenv("SPX_WALL_TAB", "0")
+    q = FCPlan(*shape)
+    vx, vy, vz, pr = (_rand_spec(g, s) for s in (50, 51, 52, 53))
+    r = [x.copy() for x in (vx, vy, vz, pr)]
+    O.v_imposebc_and_project(g, *r, 2, 2, (0.0, 0.0), (0.0, 0.0))
+    ta = [_cu(torch, x) for x in (vx, vy, vz, pr)]
+    tb = [_cu(torch, x) for x in (vx, vy, vz, pr)]
+    p.v_imposebc_and_project(*ta, 2, 2)
+    q.v_imposebc_and_project(*tb, 2, 2)
+    for a, b, rr in zip(ta, tb, r):
+        assert rel_err(b.cpu().numpy(), rr) < _proj_tol(shape)
+        assert rel_err(a.cpu().numpy(), b.cpu().numpy()) < _proj_tol(shape)
+
+
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
 def test_fused_projection_matches_two_kernel_path(precision, monkeypatch):
     """sol_project in one pass per line (the default) gives the same bits as
