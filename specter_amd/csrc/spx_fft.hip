@@ -1325,8 +1325,12 @@ __global__ void __launch_bounds__(256) k_proj_harmonic_tab(cpx<TS>* __restrict__
 // k_proj_harmonic_tab; the projected velocity never goes through HBM.
 // vz takes the RK update on its load when rk_on; pw (when non-null) gets
 // IFFT_z(d)/nz at the two walls and d is then not written there.
-template <int KPL>
-__global__ void __launch_bounds__(256) k_project_harmonic(
+// RKM < 0: any nz, RK update chosen at run time (rk_on); RKM >= 0: nz >=
+// 64 (KPL - 1), so that only the last of a lane's points can lie past the line
+// end and the others are loaded without a branch between them; RKM 0: no RK
+// update, 1: RK update with a float64 nonlinear term, 2: float32.
+template <int KPL, int RKM>
+__global__ void __launch_bounds__(256, RKM >= 0 ? 2 : 1) k_project_harmonic(
     Cd* __restrict__ a, Cd* __restrict__ b, Cd* __restrict__ c, Cd* __restrict__ d,
     Cd* __restrict__ bc, long nlines, int nz, int ny, const double* __restrict__ kx,
     const double* __restrict__ ky, const double* __restrict__ kz, const Cd* __restrict__ wtop,
@@ -1339,19 +1343,41 @@ __global__ void __launch_bounds__(256) k_project_harmonic(
   const long i = ln / ny;
   const int j = int(ln - i * ny);
   const double kxi = kx[i], kyj = ky[j];
+  // the RK update's factors of the line (rk_on_load's, read once)
+  double rkl2 = 0.0, rfx = 0.0, rfy = 0.0, rnu = 0.0;
+  if constexpr (RKM > 0) {
+    const double rx = rk.kx[i], ry = rk.ky[j];
+    rkl2 = rx * rx + ry * ry;
+    rfx = rk.fkx[i];
+    rfy = rk.fky[j];
+    rnu = rk.c[0].nu[i];
+  }
   Cd s0 = mkc<double>(0, 0), s1 = mkc<double>(0, 0);
   Cd q0 = mkc<double>(0, 0), q1 = mkc<double>(0, 0);
   Cd pa[KPL], pb[KPL], pc[KPL];
 #pragma unroll
   for (int q = 0; q < KPL; ++q) {
     const int k = lane + 64 * q;
-    if (k < nz) {
+    if ((RKM >= 0 && q < KPL - 1) || k < nz) {
       const long e = ln * nz + k;
       const double kzk = kz[k];
       const double kk2 = (kxi * kxi + kyj * kyj) + kzk * kzk;
       const Cd A = a[e], B = b[e];
       Cd Cc = c[e];
-      if (rk_on) Cc = rk_on_load(rk, 0, Cc, e, i, j, k);
+      if constexpr (RKM < 0) {
+        if (rk_on) Cc = rk_on_load(rk, 0, Cc, e, i, j, k);
+      } else if constexpr (RKM > 0) {
+        Cd nl;
+        if constexpr (RKM == 2) {
+          const cpx<float> t = static_cast<const cpx<float>*>(rk.c[0].nl)[e];
+          nl = mkc<double>(double(t.x), double(t.y));
+        } else {
+          nl = static_cast<const Cd*>(rk.c[0].nl)[e];
+        }
+        Cc = rk_apply(rk, Cc, nl, static_cast<const Cd*>(rk.c[0].f)[e],
+                      static_cast<const Cd*>(rk.c[0].c0)[e], rkl2, rk.kz[k], rfx, rfy, rk.fkz[k],
+                      rnu);
+      }
       Cd sm = cadd(cscale(A, kxi), cscale(B, kyj));
       sm = cadd(sm, cscale(Cc, kzk));
       Cd dd;
@@ -1408,7 +1434,7 @@ __global__ void __launch_bounds__(256) k_project_harmonic(
 #pragma unroll
   for (int q = 0; q < KPL; ++q) {
     const int k = lane + 64 * q;
-    if (k < nz) {
+    if ((RKM >= 0 && q < KPL - 1) || k < nz) {
       const Cd p1 = P1[k], p2 = P2[k];
       Cd A, B;
       if (zm) {
@@ -2514,18 +2540,36 @@ int project_harmonic(spx_plan* p, void* a, void* b, void* c, void* d, const RkLo
   ProfScope ps_(p, rk ? "project_harmonic_rk" : "project_harmonic",
                 double(p->wnc()) * (7.0 * 16.0 + extra) + 2.0 * p->nhtab * double(nz) * 16.0);
   const dim3 grid(unsigned((nlines + 3) / 4)), block(256);
+  // RK mode of the branch-free variants (lines of at least 64 (KPL - 1) points)
+  const int rkm = rk ? (rk->nl32 ? 2 : 1) : 0;
 #define SPX_PH(KPL)                                                                            \
-  hipLaunchKernelGGL(k_project_harmonic<KPL>, grid, block, 0, p->stream, C_<double>(a),        \
+  if (nz < 64 * (KPL - 1)) {                                                                   \
+    SPX_PH2(KPL, -1);                                                                          \
+  } else if (rkm == 0) {                                                                       \
+    SPX_PH2(KPL, 0);                                                                           \
+  } else if (rkm == 1) {                                                                       \
+    SPX_PH2(KPL, 1);                                                                           \
+  } else {                                                                                     \
+    SPX_PH2(KPL, 2);                                                                           \
+  }
+#define SPX_PH2(KPL, RKM)                                                                      \
+  hipLaunchKernelGGL((k_project_harmonic<KPL, RKM>), grid, block, 0, p->stream, C_<double>(a), \
                      C_<double>(b), C_<double>(c), C_<double>(d), C_<double>(p->bc), nlines,   \
                      nz, p->g.ny, p->d_kx, p->d_ky, p->d_kz,                                   \
                      static_cast<const Cd*>(p->d_wtop), 1.0 / double(nz), r, rk ? 1 : 0,       \
                      walls_only ? C_<double>(p->pw) : nullptr, p->np,                          \
                      static_cast<const Cd*>(p->d_htab), p->d_hidx, p->g.Lz, p->d_khom)
-  if (nz <= 64 * 3) SPX_PH(3);
-  else if (nz <= 64 * 5) SPX_PH(5);
-  else if (nz <= 64 * 9) SPX_PH(9);
-  else SPX_PH(17);
+  if (nz <= 64 * 3) {
+    SPX_PH2(3, -1);
+  } else if (nz <= 64 * 5) {
+    SPX_PH(5)
+  } else if (nz <= 64 * 9) {
+    SPX_PH(9)
+  } else {
+    SPX_PH2(17, -1);  // the branch-free form of 17 points per lane needs 512 registers
+  }
 #undef SPX_PH
+#undef SPX_PH2
   SPX_CHECK_LAUNCH();
   return SPX_OK;
 }
