@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(512, WPE) k_wall_mfma(
 #ifndef SPX_WALL_G
 #define SPX_WALL_G 1
 #endif
-  constexpr int G = KBW >= 18 ? MT : SPX_WALL_G;
+  constexpr int G = SPX_WALL_G;
   gemm_a_groups<MT, 0, G, KBW, TAB>(winv, rl, vr, b0, nbk, lane, wave, ns, red, n, np, d, nq);
   __syncthreads();
   for (int e = threadIdx.x; e < MT * 256; e += blockDim.x) {
