@@ -264,18 +264,18 @@ __global__ void __launch_bounds__(512, WPE) k_wall_mfma(
   else
     wall_load_rk<KBW, false>(vr, src, rk, comp, lc * n, okl, ki, kj, b0, g, n);
 
-  // ---- GEMM A: partial inverse DFT at the 2d + C points, in two row groups
-  // (fewer live accumulators), summed over the waves in a fixed order; then
-  // slot 0 += slots 1..ns-1
+  // ---- GEMM A: partial inverse DFT at the 2d + C points, one row tile at a
+  // time, summed over the waves in a fixed order; then slot 0 += slots
+  // 1..ns-1
   const int nq = 2 * d + C;
   if constexpr (TAB == 1) {
     for (int e = threadIdx.x; e < n; e += blockDim.x) rl[e] = roots[e];
     __syncthreads();
   }
-#ifndef SPX_WALL_G
-#define SPX_WALL_G 1
-#endif
-  constexpr int G = SPX_WALL_G;
+  // one 16-row tile at a time: 8 accumulator registers beside the wave's
+  // line blocks (three at a time spilled 26 registers per lane at 4 waves per
+  // SIMD; HD 512^3: 3.43 -> 3.19 ms per launch with the batched loads)
+  constexpr int G = 1;
   gemm_a_groups<MT, 0, G, KBW, TAB>(winv, rl, vr, b0, nbk, lane, wave, ns, red, n, np, d, nq);
   __syncthreads();
   for (int e = threadIdx.x; e < MT * 256; e += blockDim.x) {
