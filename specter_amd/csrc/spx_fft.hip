@@ -1347,9 +1347,12 @@ __global__ void __launch_bounds__(256) k_proj_harmonic_tab(cpx<TS>* __restrict__
 // RKM < 0: any nz, RK update chosen at run time (rk_on); RKM >= 0: nz >=
 // 64 (KPL - 1), so that only the last of a lane's points can lie past the line
 // end and the others are loaded without a branch between them; RKM 0: no RK
-// update, 1: RK update with a float64 nonlinear term, 2: float32.
+// update, 1: RK update with a float64 nonlinear term, 2: float32.  Register
+// budget of the branch-free forms: 2 waves per SIMD up to 9 points per lane
+// (HD 512^3: 2.43 ms per launch against 2.81 at one wave), one wave for 17
+// (HD 1024^3: 23.1 ms against 25.8 for the general form).
 template <int KPL, int RKM>
-__global__ void __launch_bounds__(256, RKM >= 0 ? 2 : 1) k_project_harmonic(
+__global__ void __launch_bounds__(256, RKM >= 0 && KPL <= 9 ? 2 : 1) k_project_harmonic(
     Cd* __restrict__ a, Cd* __restrict__ b, Cd* __restrict__ c, Cd* __restrict__ d,
     Cd* __restrict__ bc, long nlines, int nz, int ny, const double* __restrict__ kx,
     const double* __restrict__ ky, const double* __restrict__ kz, const Cd* __restrict__ wtop,
@@ -2585,7 +2588,7 @@ int project_harmonic(spx_plan* p, void* a, void* b, void* c, void* d, const RkLo
   } else if (nz <= 64 * 9) {
     SPX_PH(9)
   } else {
-    SPX_PH2(17, -1);  // the branch-free form of 17 points per lane needs 512 registers
+    SPX_PH(17)
   }
 #undef SPX_PH
 #undef SPX_PH2
