@@ -242,6 +242,7 @@ __global__ void __launch_bounds__(512, WPE) k_wall_mfma(
   double* const red = sm + (TAB ? 2 * n : 0);   // [ns][MT][4][64] partial X
   double* const xs = red;                       // slot 0 ends up holding X
   double* const ds = red + ns * MT * 256;       // [4 KT][16] real delta
+  double* const wv = ds + KT * 64;              // [16][4] wall values (re, im bottom, top)
   const int comp = blockIdx.y;
   double* const V = comp ? vy : vx;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -263,6 +264,30 @@ __global__ void __launch_bounds__(512, WPE) k_wall_mfma(
     wall_load_rk<KBW, true>(vr, src, rk, comp, lc * n, okl, ki, kj, b0, g, n);
   else
     wall_load_rk<KBW, false>(vr, src, rk, comp, lc * n, okl, ki, kj, b0, g, n);
+
+  // ---- wall values of the tile's lines (vboundary.f90:195-209; zero for the
+  // scalar), staged in LDS here so that the delta phase reads no global memory
+  if (threadIdx.x < 16) {
+    const long lt = lbase + threadIdx.x;
+    Cd wb = mkc<double>(0, 0), wt = mkc<double>(0, 0);
+    if (!wp.scalar && lt < nlines) {
+      const long ti = lt / ny;
+      const int tj = int(lt - ti * ny);
+      const double kxi = kx[ti], kyj = ky[tj];
+      if (kxi == 0.0 && kyj == 0.0) {
+        wb = mkc<double>(wp.nxny * wp.vb[comp][0], 0.0);
+        wt = mkc<double>(wp.nxny * wp.vb[comp][1], 0.0);
+      } else {
+        wb = cscale(ikmul_d(comp ? kyj : kxi, pr[lt * n]), wp.tau);
+        wt = cscale(ikmul_d(comp ? kyj : kxi, pr[lt * n + np - 1]), wp.tau);
+      }
+    }
+    double* w = wv + 4 * threadIdx.x;
+    w[0] = wb.x;
+    w[1] = wb.y;
+    w[2] = wt.x;
+    w[3] = wt.y;
+  }
 
   // ---- GEMM A: partial inverse DFT at the 2d + C points, one row tile at a
   // time, summed over the waves in a fixed order; then slot 0 += slots
@@ -301,20 +326,8 @@ __global__ void __launch_bounds__(512, WPE) k_wall_mfma(
     const int l = e / (2 + C), t = e - l * (2 + C);
     const long lt = lbase + l;
     if (lt >= nlines) continue;
-    // wall values (vboundary.f90:195-209; zero for the scalar)
-    Cd wb = mkc<double>(0, 0), wt = mkc<double>(0, 0);
-    if (!wp.scalar) {
-      const long ti = lt / ny;
-      const int tj = int(lt - ti * ny);
-      const double kxi = kx[ti], kyj = ky[tj];
-      if (kxi == 0.0 && kyj == 0.0) {
-        wb = mkc<double>(wp.nxny * wp.vb[comp][0], 0.0);
-        wt = mkc<double>(wp.nxny * wp.vb[comp][1], 0.0);
-      } else {
-        wb = cscale(ikmul_d(comp ? kyj : kxi, pr[lt * n]), wp.tau);
-        wt = cscale(ikmul_d(comp ? kyj : kxi, pr[lt * n + np - 1]), wp.tau);
-      }
-    }
+    const Cd wb = mkc<double>(wv[4 * l], wv[4 * l + 1]);
+    const Cd wt = mkc<double>(wv[4 * l + 2], wv[4 * l + 3]);
     // physical-domain value of boundary point z (0..d-1 or np-d..np-1)
     auto xn = [&](int z) -> Cd {
       if (z == 0) return wb;
@@ -526,7 +539,7 @@ int wall_mfma(spx_plan* p, void* vx, void* vy, const void* pr, const WallParams&
   if (!w.on) return SPX_ERR_UNSUPPORTED;
   const long nlines = p->wlines();
   const dim3 grid(unsigned((nlines + 15) / 16), unsigned(ncomp));
-  const size_t lds = (size_t(std::min(w.nw, 4)) * w.mt * 256 + size_t(w.kt) * 64 +
+  const size_t lds = (size_t(std::min(w.nw, 4)) * w.mt * 256 + size_t(w.kt) * 64 + 64 +
                       (w.tab ? 2 * size_t(p->g.nz) : 0)) *
                      sizeof(double);
   RkLoad r;
